@@ -1,0 +1,776 @@
+// HIP kernels of the explicit MHD step for gfx950 (CDNA4).
+//
+// Memory-bound stencils over the padded per-block box of layout.h: a
+// workgroup is 64 x 4 threads, one 64-lane wavefront per k-row, so every
+// slot load and store of a wavefront is one contiguous 512-byte segment.
+// blockIdx.z selects the block (of any case), blockIdx.y packs (i, j-tile),
+// blockIdx.x is the k-tile.
+//
+// FP64 arithmetic is written in the same operation order as the reference
+// (and as the CPU oracle) and the library is compiled with
+// -ffp-contract=off, so the FP64 path reproduces the reference bit for bit
+// wherever division and square root are correctly rounded.
+#include <hip/hip_runtime.h>
+
+#include "../host/setup.h"
+#include "../layout.h"
+#include "kernels.h"
+
+using namespace mpcns;
+
+namespace
+{
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
+
+struct Ctx3
+{
+    const MhdBlockDev *blk;
+    int nb;
+};
+
+// Map a workgroup/thread to (block, i, j, k) over the box [lo, n + hi) per
+// axis.  Returns false for padding threads.
+__device__ __forceinline__ bool thread_cell(const MhdBlockDev *__restrict__ blks, int lo, int hi, MhdBlockDev &b, int &i,
+                                            int &j, int &k)
+{
+    b = blks[blockIdx.z];
+    const int ext_i = b.ni - lo + hi, ext_j = b.nj - lo + hi, ext_k = b.nk - lo + hi;
+    const int jt = (ext_j + 3) >> 2;
+    const int iy = blockIdx.y / jt;
+    const int jy = blockIdx.y - iy * jt;
+    i = lo + iy;
+    j = lo + jy * 4 + (int)threadIdx.y;
+    k = lo + (int)blockIdx.x * 64 + (int)threadIdx.x;
+    return (iy < ext_i) && (j < lo + ext_j) && (k < lo + ext_k);
+}
+
+// ------------------------------------------------------------------ state
+struct Prim
+{
+    double rho_raw, u, v, w, p;
+};
+
+// calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110)
+__device__ __forceinline__ Prim prim_of(const double *__restrict__ U, int64_t sl, int64_t o, double gm)
+{
+    Prim q;
+    const double r0 = U[o];
+    q.rho_raw = r0;
+    const double rho = dmax(r0, 1e-12);
+    q.u = U[sl + o] / rho;
+    q.v = U[2 * sl + o] / rho;
+    q.w = U[3 * sl + o] / rho;
+    const double E = U[4 * sl + o];
+    const double ke = 0.5 * rho * (q.u * q.u + q.v * q.v + q.w * q.w);
+    double eint = E - ke;
+    if (eint < 0.0)
+        eint = 0.0;
+    double p = (gm - 1.0) * eint;
+    q.p = dmax(p, 1e-12);
+    return q;
+}
+
+// -------------------------------------------------------- Rusanov flux
+// Scheme_U_ lambdas (Z4_Lunar/src/physics/LunarSolver_FluidRHS.cpp:5-102)
+template <typename R>
+__device__ __forceinline__ void rusanov(R gm, const R *WL, const R *WR, R k1, R k2, R k3, R *out)
+{
+    const R rhoL = WL[0], uL = WL[1], vL = WL[2], wL = WL[3], pL = WL[4];
+    const R rhoR = WR[0], uR = WR[1], vR = WR[2], wR = WR[3], pR = WR[4];
+    const R one = (R)1.0, half = (R)0.5, zero = (R)0.0;
+    const R eL = pL / (gm - one) + half * rhoL * (uL * uL + vL * vL + wL * wL);
+    const R eR = pR / (gm - one) + half * rhoR * (uR * uR + vR * vR + wR * wR);
+    const R kk = k1 * k1 + k2 * k2 + k3 * k3;
+    const R uvL = k1 * uL + k2 * vL + k3 * wL;
+    const R uvR = k1 * uR + k2 * vR + k3 * wR;
+    const R tL = gm * pL / (rhoL < (R)1e-30 ? (R)1e-30 : rhoL);
+    const R tR = gm * pR / (rhoR < (R)1e-30 ? (R)1e-30 : rhoR);
+    const R cL = sqrt(zero < tL ? tL : zero);
+    const R cR = sqrt(zero < tR ? tR : zero);
+    const R sk = sqrt(kk);
+    const R radL = fabs(uvL) + cL * sk;
+    const R radR = fabs(uvR) + cR * sk;
+    const R rad = (radL < radR) ? radR : radL;
+    const R hr = half * rad;
+    out[0] = half * (rhoL * uvL + rhoR * uvR) - hr * (rhoR - rhoL);
+    out[1] = half * ((rhoL * uvL * uL + k1 * pL) + (rhoR * uvR * uR + k1 * pR)) - hr * (rhoR * uR - rhoL * uL);
+    out[2] = half * ((rhoL * uvL * vL + k2 * pL) + (rhoR * uvR * vR + k2 * pR)) - hr * (rhoR * vR - rhoL * vL);
+    out[3] = half * ((rhoL * uvL * wL + k3 * pL) + (rhoR * uvR * wR + k3 * pR)) - hr * (rhoR * wR - rhoL * wL);
+    out[4] = half * (uvL * (eL + pL) + uvR * (eR + pR)) - hr * (eR - eL);
+}
+
+__device__ __forceinline__ double minmod(double a, double b)
+{
+    if (a * b <= 0.0)
+        return 0.0;
+    return fabs(a) < fabs(b) ? a : b;
+}
+
+} // namespace
+
+// ======================================================================
+// dt: Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140)
+// per-cell CFL limit, wavefront min, one 64-bit atomicMin per workgroup on
+// the bit pattern (positive doubles order like their bits).
+__global__ __launch_bounds__(256) void k_dt(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                            const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+                                            unsigned long long *__restrict__ dtbits, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    const bool ok = thread_cell(blks, 0, 0, b, i, j, k);
+    double local = 1e100;
+    if (ok)
+    {
+        const MhdCaseConst &c = cst[b.case_id];
+        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const double *U = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
+        const double *Bc = st + b.st + (int64_t)ST_BCELL * sl;
+        const double *G = geo + b.geo;
+        const double V = fabs(G[(int64_t)G_JAC * sl + o]);
+        if (V > 0.0)
+        {
+            const double gm = c.gamma;
+            const double rho = dmax(U[o], 1e-2);
+            const double ux = U[sl + o] / rho, uy = U[2 * sl + o] / rho, uz = U[3 * sl + o] / rho;
+            const double E = U[4 * sl + o];
+            const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+            const double eint = E - ke;
+            const double p = (gm - 1.0) * eint;
+            const double cs2 = gm * p / rho;
+            const double Bx = Bc[o], By = Bc[sl + o], Bz = Bc[2 * sl + o];
+            const double B2 = Bx * Bx + By * By + Bz * Bz;
+            const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk;
+            double denom = 0.0;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+            {
+                const int ax = f >> 1;
+                const int64_t oo = o + ((f & 1) ? (ax == 0 ? di : (ax == 1 ? dj : 1)) : 0);
+                const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+                const double axv = A[oo], ayv = A[sl + oo], azv = A[2 * sl + oo];
+                const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
+                double term = 0.0;
+                if (An > 0.0)
+                {
+                    double nx = axv / An, ny = ayv / An, nz = azv / An;
+                    if (!(f & 1))
+                    {
+                        nx = -nx;
+                        ny = -ny;
+                        nz = -nz;
+                    }
+                    const double un = ux * nx + uy * ny + uz * nz;
+                    const double Bn = Bx * nx + By * ny + Bz * nz;
+                    const double vA2 = c.inver_MA2 * B2 / rho;
+                    const double vAn2 = c.inver_MA2 * (Bn * Bn) / rho;
+                    const double tt = cs2 + vA2;
+                    double disc = tt * tt - 4.0 * cs2 * vAn2;
+                    if (disc < 0.0)
+                        disc = 0.0;
+                    const double cf = sqrt(0.5 * (tt + sqrt(disc)));
+                    term = (fabs(un) + cf) * An;
+                }
+                denom += term;
+            }
+            if (denom > 0.0)
+            {
+                const double dtc = c.cfl * V / denom;
+                local = dmin(local, dtc);
+            }
+        }
+    }
+    // wavefront min then workgroup min
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1)
+        local = dmin(local, __shfl_xor(local, s, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0)
+        red[threadIdx.y] = local;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+    {
+        double m = dmin(dmin(red[0], red[1]), dmin(red[2], red[3]));
+        if (m < 1e100)
+            atomicMin(&dtbits[blks[blockIdx.z].case_id], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// finalize: bits -> dt, abort check, reset for the next reduction
+__global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
+                              const int *active)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncase)
+        return;
+    const double v = __longlong_as_double((long long)dtbits[c]);
+    if (active[c])
+    {
+        dt[c] = v;
+        if (!(v == v) || v < cst[c].dt_floor || isinf(v))
+            atomicOr(err, 1);
+    }
+    dtbits[c] = (unsigned long long)__double_as_longlong(1e100);
+}
+
+// ======================================================================
+// contravariant cell velocity per axis, cells [-1, N] (step scratch):
+// cell_u_contra (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:414-428)
+// with U_plus from calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141).
+__global__ __launch_bounds__(256) void k_ucontra(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+                                                 int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, -1, 1, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const double *U = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
+    const double *G = geo + b.geo;
+    const Prim q = prim_of(U, sl, o, cst[b.case_id].gamma);
+    const bool has = q.rho_raw > 1e-20;
+    const double up0 = has ? q.u : 0.0, up1 = has ? q.v : 0.0, up2 = has ? q.w : 0.0;
+    const double inv_jac = 1.0 / (fabs(G[(int64_t)G_JAC * sl + o]) + 1.0e-30);
+    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *UC = st + b.st + (int64_t)ST_UC * sl;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+    {
+        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        const int64_t op = o + step[ax];
+        const double kx = 0.5 * (A[o] + A[op]);
+        const double ky = 0.5 * (A[sl + o] + A[sl + op]);
+        const double kz = 0.5 * (A[2 * sl + o] + A[2 * sl + op]);
+        UC[ax * sl + o] = (kx * up0 + ky * up1 + kz * up2) * inv_jac;
+    }
+}
+
+// ======================================================================
+// Face EMF candidates: AssembleOneDirectionEMF_
+// (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:348-496).
+template <typename R>
+__device__ __forceinline__ double emf_flux(const double *__restrict__ Bf, const double *__restrict__ Ba,
+                                           const double *__restrict__ UC, int64_t sl, const int64_t *step, int bax,
+                                           int fax, int64_t oL, int64_t oR, double bff)
+{
+    const double *Bb = Bf + (int64_t)bax * sl, *Ab = Ba + (int64_t)bax * sl;
+    const int64_t s = step[bax];
+    const double bL0 = Bb[oL], bL1 = Bb[oL + s], bR0 = Bb[oR], bR1 = Bb[oR + s];
+    const double btL = 0.5 * ((bL0 + Ab[oL]) + (bL1 + Ab[oL + s]));
+    const double btR = 0.5 * ((bR0 + Ab[oR]) + (bR1 + Ab[oR + s]));
+    const double biL = 0.5 * (bL0 + bL1);
+    const double biR = 0.5 * (bR0 + bR1);
+    const double ufL = UC[(int64_t)fax * sl + oL], ufR = UC[(int64_t)fax * sl + oR];
+    const double ubL = UC[(int64_t)bax * sl + oL], ubR = UC[(int64_t)bax * sl + oR];
+    if constexpr (sizeof(R) == 8)
+    {
+        const double GL = ufL * btL - ubL * bff;
+        const double GR = ufR * btR - ubR * bff;
+        const double radius = dmax(fabs(ufL), fabs(ufR));
+        return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
+    }
+    else
+    {
+        const float GL = (float)ufL * (float)btL - (float)ubL * (float)bff;
+        const float GR = (float)ufR * (float)btR - (float)ubR * (float)bff;
+        const float radius = fmaxr(fabsf((float)ufL), fabsf((float)ufR));
+        return (double)(0.5f * (GL + GR)) - 0.5 * (double)radius * (biR - biL);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_face_emf(const MhdBlockDev *__restrict__ blks, double *__restrict__ st)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, 0, 1, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *S = st + b.st;
+    const double *Bf = S + (int64_t)ST_B * sl, *Ba = S + (int64_t)ST_BADD * sl, *UC = S + (int64_t)ST_UC * sl;
+    const int n[3] = {b.ni, b.nj, b.nk};
+    const int idx[3] = {i, j, k};
+#pragma unroll
+    for (int fax = 0; fax < 3; ++fax)
+    {
+        // face normal to fax: idx[fax] in [0, n], others in [0, n)
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            in = in && (idx[a] < n[a] + (a == fax ? 1 : 0));
+        if (!in)
+            continue;
+        const int64_t oR = o, oL = o - step[fax];
+        const double bff = Bf[(int64_t)fax * sl + o] + Ba[(int64_t)fax * sl + o];
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        if (fax == 0)
+        {
+            const double Ge = emf_flux<R>(Bf, Ba, UC, sl, step, 1, 0, oL, oR, bff);
+            const double Gz = emf_flux<R>(Bf, Ba, UC, sl, step, 2, 0, oL, oR, bff);
+            e1 = Gz;
+            e2 = -Ge;
+        }
+        else if (fax == 1)
+        {
+            const double Gx = emf_flux<R>(Bf, Ba, UC, sl, step, 0, 1, oL, oR, bff);
+            const double Gz = emf_flux<R>(Bf, Ba, UC, sl, step, 2, 1, oL, oR, bff);
+            e0 = -Gz;
+            e2 = Gx;
+        }
+        else
+        {
+            const double Gx = emf_flux<R>(Bf, Ba, UC, sl, step, 0, 2, oL, oR, bff);
+            const double Ge = emf_flux<R>(Bf, Ba, UC, sl, step, 1, 2, oL, oR, bff);
+            e0 = Ge;
+            e1 = -Gx;
+        }
+        double *Ef = S + (int64_t)(ST_EFACE + 3 * fax) * sl;
+        Ef[o] = e0;
+        Ef[sl + o] = e1;
+        Ef[2 * sl + o] = e2;
+    }
+}
+
+// ======================================================================
+// Edge EMF with UCT upwinding: AssembleEdgeEMF_FromFaceE_Ideal_
+// (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:53-287).
+__device__ __forceinline__ double max_abs4(const double *UCa, int64_t o0, int64_t o1, int64_t o2, int64_t o3)
+{
+    double a = 0.0;
+    a = dmax(a, fabs(UCa[o0]));
+    a = dmax(a, fabs(UCa[o1]));
+    a = dmax(a, fabs(UCa[o2]));
+    a = dmax(a, fabs(UCa[o3]));
+    return a;
+}
+
+__global__ __launch_bounds__(256) void k_edge_emf(const MhdBlockDev *__restrict__ blks, double *__restrict__ st)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, 0, 1, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = st + b.st;
+    const double *Fx = S + (int64_t)(ST_EFACE + 0) * sl, *Fy = S + (int64_t)(ST_EFACE + 3) * sl,
+                 *Fz = S + (int64_t)(ST_EFACE + 6) * sl;
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double *UCx = S + (int64_t)ST_UC * sl, *UCy = UCx + sl, *UCz = UCx + 2 * sl;
+    double *E = S + (int64_t)ST_E * sl;
+    const double Cuct = 0.5;
+    if (i < b.ni && j <= b.nj && k <= b.nk)
+    {
+        const double Ec = 0.25 * (Fy[o] + Fy[o - dk] + Fz[o] + Fz[o - dj]);
+        const double ae = max_abs4(UCy, o, o - dj, o - dk, o - dj - dk);
+        const double az = max_abs4(UCz, o, o - dj, o - dk, o - dj - dk);
+        const double dBz = Bz[o] - Bz[o - dj];
+        const double dBe = By[o] - By[o - dk];
+        E[o] = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
+    }
+    if (i <= b.ni && j < b.nj && k <= b.nk)
+    {
+        const double Ec = 0.25 * (Fx[sl + o] + Fx[sl + o - dk] + Fz[sl + o] + Fz[sl + o - di]);
+        const double az = max_abs4(UCz, o, o - di, o - dk, o - di - dk);
+        const double ax = max_abs4(UCx, o, o - di, o - dk, o - di - dk);
+        const double dBx = Bx[o] - Bx[o - dk];
+        const double dBz = Bz[o] - Bz[o - di];
+        E[sl + o] = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
+    }
+    if (i <= b.ni && j <= b.nj && k < b.nk)
+    {
+        const double Ec = 0.25 * (Fx[2 * sl + o] + Fx[2 * sl + o - dj] + Fy[2 * sl + o] + Fy[2 * sl + o - di]);
+        const double ax = max_abs4(UCx, o, o - di, o - dj, o - di - dj);
+        const double ae = max_abs4(UCy, o, o - di, o - dj, o - di - dj);
+        const double dBe = By[o] - By[o - di];
+        const double dBx = Bx[o] - Bx[o - dj];
+        E[2 * sl + o] = Ec + Cuct * (0.5 * ax * dBe - 0.5 * ae * dBx);
+    }
+}
+
+// ======================================================================
+// B_face += dt * (-curl E): CurlEdgeToFace with multiper -1
+// (Z4_Lunar/src/operators/CTOperators.cpp:6-56) and ApplyUpdate_Euler_
+// (Z4_Lunar/src/physics/LunarSolver_Advance.cpp:61-122).
+__global__ __launch_bounds__(256) void k_curl_update(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                                     const double *__restrict__ dts)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, 0, 1, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = st + b.st;
+    const double *Ex = S + (int64_t)ST_E * sl, *Ey = Ex + sl, *Ez = Ex + 2 * sl;
+    double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double dt = dts[b.case_id];
+    const double mult = -1.0;
+    if (i <= b.ni && j < b.nj && k < b.nk)
+        Bx[o] += dt * (mult * ((Ey[o] - Ey[o + dk]) + (Ez[o + dj] - Ez[o])));
+    if (i < b.ni && j <= b.nj && k < b.nk)
+        By[o] += dt * (mult * ((Ex[o + dk] - Ex[o]) + (Ez[o] - Ez[o + di])));
+    if (i < b.ni && j < b.nj && k <= b.nk)
+        Bz[o] += dt * (mult * ((Ex[o] - Ex[o + dj]) + (Ey[o + di] - Ey[o])));
+}
+
+// ======================================================================
+// Fluid: Rusanov fluxes, divergence, J x B source and Euler update fused:
+// Scheme_U_ + AddSourceToRHS_Fluid + ApplyUpdate_Euler_
+// (Z4_Lunar/src/physics/LunarSolver_FluidRHS.cpp:3-201,528-568;
+//  Z4_Lunar/src/physics/LunarSolver_Advance.cpp:61-122).
+// U is read from buffer `ucur` and written to buffer 1-ucur.
+template <typename R, int RECON>
+__device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t sl, const double *__restrict__ A,
+                                          int64_t oL, int64_t oR, int64_t step, double gm, double *F)
+{
+    double WL[5], WR[5];
+    {
+        const Prim a = prim_of(U, sl, oL, gm), c = prim_of(U, sl, oR, gm);
+        WL[0] = a.rho_raw; WL[1] = a.u; WL[2] = a.v; WL[3] = a.w; WL[4] = a.p;
+        WR[0] = c.rho_raw; WR[1] = c.u; WR[2] = c.v; WR[3] = c.w; WR[4] = c.p;
+    }
+    if (RECON == 1)
+    {
+        const Prim a = prim_of(U, sl, oL - step, gm), c = prim_of(U, sl, oR + step, gm);
+        const double WLL[5] = {a.rho_raw, a.u, a.v, a.w, a.p};
+        const double WRR[5] = {c.rho_raw, c.u, c.v, c.w, c.p};
+        double L2[5], R2[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            L2[m] = WL[m] + 0.5 * minmod(WL[m] - WLL[m], WR[m] - WL[m]);
+            R2[m] = WR[m] - 0.5 * minmod(WR[m] - WL[m], WRR[m] - WR[m]);
+        }
+        if (L2[0] > 0.0 && L2[4] > 0.0 && R2[0] > 0.0 && R2[4] > 0.0)
+        {
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+            {
+                WL[m] = L2[m];
+                WR[m] = R2[m];
+            }
+        }
+    }
+    const int64_t of = oR;
+    if constexpr (sizeof(R) == 8)
+    {
+        rusanov<double>(gm, WL, WR, A[of], A[sl + of], A[2 * sl + of], F);
+    }
+    else
+    {
+        float wl[5], wr[5], f[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            wl[m] = (float)WL[m];
+            wr[m] = (float)WR[m];
+        }
+        rusanov<float>((float)gm, wl, wr, (float)A[of], (float)A[sl + of], (float)A[2 * sl + of], f);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            F[m] = (double)f[m];
+    }
+}
+
+template <typename R, int RECON>
+__global__ __launch_bounds__(256) void k_fluid(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                               const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+                                               const double *__restrict__ dts, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, 0, 0, b, i, j, k))
+        return;
+    const MhdCaseConst &c = cst[b.case_id];
+    const double gm = c.gamma;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *S = st + b.st;
+    const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
+    double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
+    const double *G = geo + b.geo;
+    double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+    {
+        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        double Flo[5], Fhi[5];
+        face_flux<R, RECON>(U, sl, A, o - step[ax], o, step[ax], gm, Flo);
+        face_flux<R, RECON>(U, sl, A, o, o + step[ax], step[ax], gm, Fhi);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
+    }
+    // J x B (Hall_Num_Limiter weighting)
+    const Prim q = prim_of(U, sl, o, gm);
+    const double n_true = dmax(q.rho_raw, 0.0) / c.M_H * c.M_ref;
+    const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
+    const double wgt = n_true / ne_eff;
+    const double fc = c.momentum_hall_coeff * wgt;
+    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+    const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
+    const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
+    const double jx = J1 * B2 - J2 * B1;
+    const double jy = J2 * B0 - J0 * B2;
+    const double jz = J0 * B1 - J1 * B0;
+    rhs[1] += fc * jx;
+    rhs[2] += fc * jy;
+    rhs[3] += fc * jz;
+    rhs[4] += fc * (jx * q.u + jy * q.v + jz * q.w);
+    const double dt = dts[b.case_id];
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        Un[m * sl + o] = U[m * sl + o] + dt * rhs[m];
+}
+
+// ======================================================================
+// Derived magnetic fields on interior cells: calc_Bcell + Calc_J_Edge +
+// calc_Jcell (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:163-236,777-891;
+// CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).
+// J edges are recomputed per cell from beta-weighted face fluxes.
+__global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                                 const double *__restrict__ geo)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, 0, 0, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = st + b.st;
+    const double *G = geo + b.geo;
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+    {
+        const double pt[6] = {-Bx[o] - Ax[o],   Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
+                              By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],   Bz[o + dk] + Az[o + dk]};
+        const double *W = G + (int64_t)G_WB * sl;
+        double xt = 0.0, yt = 0.0, zt = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; ++n)
+        {
+            xt += W[(int64_t)n * sl + o] * pt[n];
+            yt += W[(int64_t)(6 + n) * sl + o] * pt[n];
+            zt += W[(int64_t)(12 + n) * sl + o] * pt[n];
+        }
+        double *Bc = S + (int64_t)ST_BCELL * sl;
+        Bc[o] = xt;
+        Bc[sl + o] = yt;
+        Bc[2 * sl + o] = zt;
+    }
+    {
+        const double *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
+        const double *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
+        auto jxi = [&](int64_t e) {
+            const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
+            return (1.0 * val) * ax[e];
+        };
+        auto jet = [&](int64_t e) {
+            const double val = (bx[e] * Bx[e] - bx[e - dk] * Bx[e - dk]) + (bz[e - di] * Bz[e - di] - bz[e] * Bz[e]);
+            return (1.0 * val) * ay[e];
+        };
+        auto jze = [&](int64_t e) {
+            const double val = (bx[e - dj] * Bx[e - dj] - bx[e] * Bx[e]) + (by[e] * By[e] - by[e - di] * By[e - di]);
+            return (1.0 * val) * az[e];
+        };
+        const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
+                              jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
+        const double *W = G + (int64_t)G_WJ * sl;
+        double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+        for (int n = 0; n < 12; ++n)
+        {
+            x += W[(int64_t)n * sl + o] * s[n];
+            y += W[(int64_t)(12 + n) * sl + o] * s[n];
+            z += W[(int64_t)(24 + n) * sl + o] * s[n];
+        }
+        double *Jc = S + (int64_t)ST_JCELL * sl;
+        Jc[o] = x;
+        Jc[sl + o] = y;
+        Jc[2 * sl + o] = z;
+    }
+}
+
+// ======================================================================
+// Boundary / halo op lists (one phase per launch).
+__device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
+{
+    rho = dmax(rho, 2.5e-3);
+    const double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
+    double eint = E - ke;
+    if (eint < 0.0)
+        eint = 0.0;
+    const double p = (gm - 1.0) * eint;
+    return dmax(p, 1e-12);
+}
+
+__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n, double *__restrict__ st,
+                                             const MhdCaseConst *__restrict__ cst, int ushift)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const MhdOp op = ops[t];
+    const int64_t sh = ushift ? 5 * op.stride : 0;
+    const int64_t d = op.dst + sh, s = op.src + sh;
+    switch (op.kind)
+    {
+    case OP_COPY:
+        for (int c = 0; c < op.ncomp; ++c)
+            st[d + c * op.stride] = st[s + c * op.stride];
+        break;
+    case OP_NEG:
+        for (int c = 0; c < op.ncomp; ++c)
+            st[d + c * op.stride] = -st[s + c * op.stride];
+        break;
+    case OP_ZERO:
+        for (int c = 0; c < op.ncomp; ++c)
+            st[d + c * op.stride] = 0.0;
+        break;
+    case OP_CONST:
+        st[op.dst] = op.val;
+        break;
+    case OP_QINF:
+        for (int c = 0; c < op.ncomp; ++c)
+            st[d + c * op.stride] = cst[op.cas].qinf[c];
+        break;
+    case OP_SOLID_FIRST:
+    case OP_SOLID_GHOST:
+    {
+        // BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184)
+        const double gm = cst[op.cas].gamma;
+        const int64_t L = op.stride;
+        const double rho_r = dmax(st[s], 2.5e-3);
+        const double p_r = solid_p(gm, rho_r, st[s + L], st[s + 2 * L], st[s + 3 * L], st[s + 4 * L]);
+        st[d] = rho_r;
+        st[d + L] = rho_r * 0.0;
+        st[d + 2 * L] = rho_r * 0.0;
+        st[d + 3 * L] = rho_r * 0.0;
+        const double ke_g = 0.5 * rho_r * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+        st[d + 4 * L] = p_r / (gm - 1.0) + ke_g;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
+__global__ __launch_bounds__(256) void k_divb(const MhdBlockDev *__restrict__ blks, const double *__restrict__ st,
+                                              const double *__restrict__ geo, unsigned long long *__restrict__ out)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    const bool ok = thread_cell(blks, 0, 0, b, i, j, k);
+    double v = 0.0;
+    if (ok)
+    {
+        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const double *Bx = st + b.st + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+        const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
+                          geo[b.geo + (int64_t)G_JAC * sl + o];
+        v = fabs(dv);
+        if (!(v == v))
+            v = 1e300;
+    }
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1)
+        v = dmax(v, __shfl_xor(v, s, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0)
+        red[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+    {
+        const double m = dmax(dmax(red[0], red[1]), dmax(red[2], red[3]));
+        atomicMax(&out[blks[blockIdx.z].case_id], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        p[t] = v;
+}
+
+// ======================================================================
+// launch helpers
+namespace mhdk
+{
+static dim3 grid_for(const LaunchShape &s, int lo, int hi)
+{
+    const int ek = s.max_nk - lo + hi, ej = s.max_nj - lo + hi, ei = s.max_ni - lo + hi;
+    return dim3((unsigned)((ek + 63) / 64), (unsigned)(ei * ((ej + 3) / 4)), (unsigned)s.nblocks);
+}
+static const dim3 kBlock(64, 4, 1);
+
+void launch_dt(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo, const MhdCaseConst *cst,
+               unsigned long long *dtbits, int ucur, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_dt, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo, cst, dtbits, ucur);
+}
+void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
+                        const int *active, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_dt_finalize, dim3((ncase + 63) / 64), dim3(64), 0, q, dtbits, dt, err, cst, ncase, active);
+}
+void launch_ucontra(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
+                    const MhdCaseConst *cst, int ucur, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_ucontra, grid_for(s, -1, 1), kBlock, 0, q, blks, st, geo, cst, ucur);
+}
+void launch_face_emf(const LaunchShape &s, const MhdBlockDev *blks, double *st, int mixed, hipStream_t q)
+{
+    if (mixed)
+        hipLaunchKernelGGL(k_face_emf<float>, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+    else
+        hipLaunchKernelGGL(k_face_emf<double>, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+}
+void launch_edge_emf(const LaunchShape &s, const MhdBlockDev *blks, double *st, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_edge_emf, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+}
+void launch_curl_update(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *dts, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_curl_update, grid_for(s, 0, 1), kBlock, 0, q, blks, st, dts);
+}
+void launch_fluid(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
+                  const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon, hipStream_t q)
+{
+    const dim3 g = grid_for(s, 0, 0);
+    if (mixed && recon)
+        hipLaunchKernelGGL((k_fluid<float, 1>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+    else if (mixed)
+        hipLaunchKernelGGL((k_fluid<float, 0>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+    else if (recon)
+        hipLaunchKernelGGL((k_fluid<double, 1>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+    else
+        hipLaunchKernelGGL((k_fluid<double, 0>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+}
+void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_derived, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo);
+}
+void launch_ops(const MhdOp *ops, int n, double *st, const MhdCaseConst *cst, int ushift, hipStream_t q)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, st, cst, ushift);
+}
+void launch_divb(const LaunchShape &s, const MhdBlockDev *blks, const double *st, const double *geo,
+                 unsigned long long *out, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_divb, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo, out);
+}
+void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, q, p, n, v);
+}
+} // namespace mhdk

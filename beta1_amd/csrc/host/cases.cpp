@@ -1,0 +1,273 @@
+// Case builders: Orszag-Tang periodic box and the solar-wind shell cases.
+// Initial states follow the reference's Lunar_Initial
+// (Z4_Lunar/include/2_Initial.h:16-99) and LunarBackgroundState
+// (Z4_Lunar/include/0_BackgroundState.h:16-67); the Orszag-Tang vortex is the
+// standard 3-D extension used as the Cartesian periodic test.
+#include "setup.h"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace mpcns
+{
+
+// Execute an op list on host state arrays (case-relative offsets).
+void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops, const MhdCaseConst &cst)
+{
+    const int64_t bs = (int64_t)ST_NSLOT * blk[0].d.slot;
+    auto at = [&](int64_t rel) -> double & { return blk[rel / bs].st[rel % bs]; };
+    for (const MhdOp &o : ops)
+    {
+        for (int c = 0; c < o.ncomp; ++c)
+        {
+            const int64_t d = o.dst + c * o.stride;
+            switch (o.kind)
+            {
+            case OP_COPY:
+                at(d) = at(o.src + c * o.stride);
+                break;
+            case OP_NEG:
+                at(d) = -at(o.src + c * o.stride);
+                break;
+            case OP_ZERO:
+                at(d) = 0.0;
+                break;
+            case OP_CONST:
+                at(d) = o.val;
+                break;
+            case OP_QINF:
+                at(d) = cst.qinf[c];
+                break;
+            default:
+                throw std::runtime_error("apply_ops_host: op kind not supported on host");
+            }
+        }
+    }
+}
+
+static void alloc_state(std::vector<BlockArrays> &blk)
+{
+    for (auto &A : blk)
+        A.st.assign((size_t)ST_NSLOT * A.d.slot, 0.0);
+}
+
+// Copy-type boundary (DefaultPhysicalCopy, reference
+// src/6_boundary/Boundary.cpp:303-331) plus halo for the three face slots
+// starting at `slot0` -- used for the one-off Badd sync at start-up.
+static std::vector<MhdOp> face_copy_sync(CaseSetup &cs, int slot0)
+{
+    std::vector<MhdOp> ops;
+    const Grid &grd = cs.grid;
+    const int g = grd.g;
+    const int64_t bs = (int64_t)ST_NSLOT * cs.blk[0].d.slot;
+    for (int b = 0; b < (int)grd.blocks.size(); ++b)
+    {
+        const Block &B = grd.blocks[b];
+        const MhdBlockDev &d = cs.blk[b].d;
+        for (int f = 0; f < 6; ++f)
+        {
+            if (B.face[f].kind != 0 || B.face[f].bc == BC_NONE)
+                continue;
+            const int ax = f / 2, sgn = (f % 2) ? 1 : -1;
+            for (int a = 0; a < 3; ++a)
+            {
+                const int loc = LOC_FACE_XI + a;
+                int lo[3], hi[3];
+                for (int q = 0; q < 3; ++q)
+                {
+                    lo[q] = 0;
+                    hi[q] = loc_count(loc, q, B.n(q));
+                }
+                const int ref = sgn < 0 ? 0 : hi[ax] - 1;
+                lo[ax] = ref;
+                hi[ax] = ref + 1;
+                for (int i = lo[0]; i < hi[0]; ++i)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int k = lo[2]; k < hi[2]; ++k)
+                            for (int l = 1; l <= g; ++l)
+                            {
+                                int q[3] = {i, j, k};
+                                q[ax] += sgn * l;
+                                MhdOp o{};
+                                o.dst = b * bs + (int64_t)(slot0 + a) * d.slot + blk_off(d, q[0], q[1], q[2]);
+                                o.src = b * bs + (int64_t)(slot0 + a) * d.slot + blk_off(d, i, j, k);
+                                o.stride = d.slot;
+                                o.kind = OP_COPY;
+                                o.ncomp = 1;
+                                ops.push_back(o);
+                            }
+            }
+        }
+    }
+    // halo: reuse the B halo ops, relocated from ST_B to slot0
+    for (MhdOp o : cs.ops[SY_B_HALO])
+    {
+        const int64_t sl = cs.blk[0].d.slot;
+        auto shift = [&](int64_t rel) {
+            const int64_t b = rel / bs, r = rel % bs;
+            const int64_t s = r / sl, off = r % sl;
+            return b * bs + (s - ST_B + slot0) * sl + off;
+        };
+        o.dst = shift(o.dst);
+        o.src = shift(o.src);
+        ops.push_back(o);
+    }
+    return ops;
+}
+
+static void finalize_grid(CaseSetup &cs)
+{
+    fill_ghost_nodes(cs.grid);
+    build_geometry(cs.grid, cs.blk);
+    for (auto &A : cs.blk)
+        A.d.case_id = 0;
+    alloc_state(cs.blk);
+}
+
+CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t seed, double noise)
+{
+    CaseSetup cs;
+    cs.grid = make_periodic_box(n, n, n, 1.0, 1.0, 1.0, g);
+    finalize_grid(cs);
+    char key[96];
+    std::snprintf(key, sizeof key, "box:%d:%d", n, g);
+    cs.shape_key = key;
+
+    MhdCaseConst &c = cs.cst;
+    c.gamma = gamma;
+    c.cfl = cfl;
+    c.inver_MA2 = 1.0;
+    c.momentum_hall_coeff = 1.0;
+    c.M_H = 1.0;
+    c.M_ref = 1.0;
+    c.ne_floor = 0.01;
+    c.state_coeff = 1.0;
+    c.dt_floor = 1e-7;
+    const double pi = std::acos(-1.0);
+    const double rho0 = 25.0 / (36.0 * pi), p0 = 5.0 / (12.0 * pi), B0 = 1.0 / std::sqrt(4.0 * pi);
+    c.qinf[0] = rho0;
+    c.qinf[1] = c.qinf[2] = c.qinf[3] = 0.0;
+    c.qinf[4] = p0 / (gamma - 1.0);
+    c.B_imf[0] = c.B_imf[1] = c.B_imf[2] = 0.0;
+    c.u_inf[0] = c.u_inf[1] = c.u_inf[2] = 0.0;
+
+    build_ops(cs);
+
+    BlockArrays &A = cs.blk[0];
+    const MhdBlockDev &d = A.d;
+    const double h = 1.0 / n;
+    auto S = [&](int s, int i, int j, int k) -> double & { return A.st[(size_t)s * d.slot + blk_off(d, i, j, k)]; };
+    uint64_t rs = seed;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k)
+            {
+                const double x = (i + 0.5) * h, y = (j + 0.5) * h;
+                const double fr = 1.0 + noise * (2.0 * uniform01(rs) - 1.0);
+                const double fp = 1.0 + noise * (2.0 * uniform01(rs) - 1.0);
+                const double rho = rho0 * fr, p = p0 * fp;
+                const double u = -std::sin(2 * pi * y), v = std::sin(2 * pi * x), w = 0.0;
+                S(ST_U0 + 0, i, j, k) = rho;
+                S(ST_U0 + 1, i, j, k) = rho * u;
+                S(ST_U0 + 2, i, j, k) = rho * v;
+                S(ST_U0 + 3, i, j, k) = rho * w;
+                S(ST_U0 + 4, i, j, k) = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+            }
+    // face fluxes from the zeta-edge circulation a = A_z dz  (curl of edge 1-form)
+    auto az = [&](int i, int j) {
+        const double x = i * h, y = j * h;
+        return B0 * (std::cos(4 * pi * x) / (4 * pi) + std::cos(2 * pi * y) / (2 * pi)) * h;
+    };
+    for (int i = 0; i <= n; ++i)
+        for (int j = 0; j <= n; ++j)
+            for (int k = 0; k <= n; ++k)
+            {
+                if (j < n && k < n)
+                    S(ST_B + 0, i, j, k) = az(i, j + 1) - az(i, j);
+                if (i < n && k < n)
+                    S(ST_B + 1, i, j, k) = az(i, j) - az(i + 1, j);
+                if (i < n && j < n)
+                    S(ST_B + 2, i, j, k) = 0.0;
+            }
+    apply_ops_host(cs.blk, cs.ops[SY_U_HALO], c);
+    apply_ops_host(cs.blk, cs.ops[SY_B_HALO], c);
+    return cs;
+}
+
+// Nondimensionalisation of the Lunar set-up: reference values of
+// Z4_Lunar/9999setup/1_Initial_condition.txt:4-5 and constants of
+// Z4_Lunar/9999setup/0_General_Setting.txt:26-27.
+struct LunarRef
+{
+    double U = 4.61e5, T = 1.2e5, n = 7e6, M = 0.001, B = 1e-9, L = 1.7374e6;
+    double NA = 6.02e23, R = 8.314, mu0 = 1.256637061e-6, qe = 1.602e-19;
+};
+
+CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
+                          const WindParams &w)
+{
+    CaseSetup cs;
+    cs.grid = make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD);
+    finalize_grid(cs);
+    char key[128];
+    std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
+    cs.shape_key = key;
+
+    const LunarRef R;
+    MhdCaseConst &c = cs.cst;
+    const double rho_ref = R.M * R.n / R.NA;
+    const double M_H = 0.001 * w.m_amu;
+    c.gamma = gamma;
+    c.cfl = cfl;
+    // calc_physical_constant (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:28-44)
+    c.inver_MA2 = R.B * R.B / (R.U * R.U * R.mu0 * rho_ref);
+    c.momentum_hall_coeff = (R.B * R.B) / (R.mu0 * rho_ref * R.U * R.U);
+    c.M_H = M_H;
+    c.M_ref = R.M;
+    c.ne_floor = 0.01;
+    c.state_coeff = (R.R * R.T) / (M_H * R.U * R.U);
+    c.dt_floor = 1e-7;
+    // BuildLunarBackgroundState (Z4_Lunar/include/0_BackgroundState.h:16-67)
+    const double k_B = R.R / R.NA;
+    const double n_m3 = w.n_cc * 1e6;
+    const double T_K = w.T_eV * 11604.518;
+    const double rho0 = (n_m3 / R.n) * (M_H / R.M);
+    const double V = w.v_kms * 1e3;
+    const double u0 = w.vdir[0] * V / R.U, v0 = w.vdir[1] * V / R.U, w0 = w.vdir[2] * V / R.U;
+    const double p0 = n_m3 * k_B * T_K / (rho_ref * R.U * R.U);
+    c.qinf[0] = rho0;
+    c.qinf[1] = rho0 * u0;
+    c.qinf[2] = rho0 * v0;
+    c.qinf[3] = rho0 * w0;
+    c.qinf[4] = 0.5 * rho0 * (u0 * u0 + v0 * v0 + w0 * w0) + p0 / (gamma - 1.0);
+    c.u_inf[0] = u0;
+    c.u_inf[1] = v0;
+    c.u_inf[2] = w0;
+    for (int a = 0; a < 3; ++a)
+        c.B_imf[a] = w.B_nT[a] * 1e-9 / R.B;
+
+    build_ops(cs);
+
+    // U = q_inf everywhere, B_face = 0, Badd = B_imf . S_face
+    // (Z4_Lunar/include/2_Initial.h:50-90; Z4_Lunar/include/operators/Dipole.h:334-375)
+    for (auto &A : cs.blk)
+    {
+        const MhdBlockDev &d = A.d;
+        for (int m = 0; m < 5; ++m)
+            for (int64_t o = 0; o < d.slot; ++o)
+                A.st[(size_t)(ST_U0 + m) * d.slot + o] = c.qinf[m];
+        for (int a = 0; a < 3; ++a)
+            for (int64_t o = 0; o < d.slot; ++o)
+            {
+                const double *jd = &A.geo[0];
+                const double sx = jd[(size_t)(G_JD + 3 * a + 0) * d.slot + o];
+                const double sy = jd[(size_t)(G_JD + 3 * a + 1) * d.slot + o];
+                const double sz = jd[(size_t)(G_JD + 3 * a + 2) * d.slot + o];
+                A.st[(size_t)(ST_BADD + a) * d.slot + o] = c.B_imf[0] * sx + c.B_imf[1] * sy + c.B_imf[2] * sz;
+            }
+    }
+    apply_ops_host(cs.blk, face_copy_sync(cs, ST_BADD), c);
+    return cs;
+}
+
+} // namespace mpcns

@@ -1,0 +1,303 @@
+// Host-only C-ABI: build a case on the CPU (grid, topology, geometry,
+// weights, op lists, initial state) and read it back in the reference AoS
+// FieldBlock layout.  Used where no device is present (set-up checks, CPU
+// baselines) and by the device context, which consumes the same builders.
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/beta1_mhd.h"
+#include "setup.h"
+
+using namespace mpcns;
+
+struct mhd_host_case
+{
+    CaseSetup cs;
+    std::string err;
+};
+
+namespace mpcns
+{
+// Field number -> (region, slot, location, components) in the device layout.
+bool field_slot(int f, int ucur, int &region, int &slot, int &loc, int &nc)
+{
+    region = 0;
+    switch (f)
+    {
+    case MHD_F_U_H:
+        slot = ST_U0 + 5 * ucur, loc = LOC_CELL, nc = 5;
+        return true;
+    case MHD_F_B_XI:
+    case MHD_F_B_ETA:
+    case MHD_F_B_ZETA:
+        slot = ST_B + (f - MHD_F_B_XI), loc = LOC_FACE_XI + (f - MHD_F_B_XI), nc = 1;
+        return true;
+    case MHD_F_BADD_XI:
+    case MHD_F_BADD_ETA:
+    case MHD_F_BADD_ZETA:
+        slot = ST_BADD + (f - MHD_F_BADD_XI), loc = LOC_FACE_XI + (f - MHD_F_BADD_XI), nc = 1;
+        return true;
+    case MHD_F_B_CELL:
+        slot = ST_BCELL, loc = LOC_CELL, nc = 3;
+        return true;
+    case MHD_F_J_CELL:
+        slot = ST_JCELL, loc = LOC_CELL, nc = 3;
+        return true;
+    case MHD_F_E_XI:
+    case MHD_F_E_ETA:
+    case MHD_F_E_ZETA:
+        slot = ST_E + (f - MHD_F_E_XI), loc = LOC_EDGE_XI + (f - MHD_F_E_XI), nc = 1;
+        return true;
+    case MHD_F_EFACE_XI:
+    case MHD_F_EFACE_ETA:
+    case MHD_F_EFACE_ZETA:
+        slot = ST_EFACE + 3 * (f - MHD_F_EFACE_XI), loc = LOC_FACE_XI + (f - MHD_F_EFACE_XI), nc = 3;
+        return true;
+    default:
+        break;
+    }
+    region = 1;
+    switch (f)
+    {
+    case MHD_F_JAC:
+        slot = G_JAC, loc = LOC_CELL, nc = 1;
+        return true;
+    case MHD_F_JDXI:
+    case MHD_F_JDET:
+    case MHD_F_JDZE:
+        slot = G_JD + 3 * (f - MHD_F_JDXI), loc = LOC_FACE_XI + (f - MHD_F_JDXI), nc = 3;
+        return true;
+    case MHD_F_BETA_XI:
+    case MHD_F_BETA_ETA:
+    case MHD_F_BETA_ZETA:
+        slot = G_BETA + (f - MHD_F_BETA_XI), loc = LOC_FACE_XI + (f - MHD_F_BETA_XI), nc = 1;
+        return true;
+    case MHD_F_ALPHA_XI:
+    case MHD_F_ALPHA_ETA:
+    case MHD_F_ALPHA_ZETA:
+        slot = G_ALPHA + (f - MHD_F_ALPHA_XI), loc = LOC_EDGE_XI + (f - MHD_F_ALPHA_XI), nc = 1;
+        return true;
+    case MHD_F_BCELL_W:
+        slot = G_WB, loc = LOC_CELL, nc = 18;
+        return true;
+    case MHD_F_JCELL_W:
+        slot = G_WJ, loc = LOC_CELL, nc = 36;
+        return true;
+    default:
+        region = -1;
+        return false;
+    }
+}
+
+size_t aos_count(const MhdBlockDev &d, int loc, int nc)
+{
+    size_t n = (size_t)nc;
+    const int N[3] = {d.ni, d.nj, d.nk};
+    for (int a = 0; a < 3; ++a)
+        n *= (size_t)(loc_count(loc, a, N[a]) + 2 * d.g);
+    return n;
+}
+
+// SoA slots (nc consecutive slots starting at `soa`) <-> AoS FieldBlock.
+void soa_aos(const MhdBlockDev &d, int loc, int nc, double *soa, double *aos, bool to_aos)
+{
+    const int N[3] = {d.ni, d.nj, d.nk};
+    int cnt[3];
+    for (int a = 0; a < 3; ++a)
+        cnt[a] = loc_count(loc, a, N[a]);
+    const int g = d.g;
+    size_t q = 0;
+    for (int i = -g; i < cnt[0] + g; ++i)
+        for (int j = -g; j < cnt[1] + g; ++j)
+            for (int k = -g; k < cnt[2] + g; ++k)
+            {
+                const int64_t o = blk_off(d, i, j, k);
+                for (int m = 0; m < nc; ++m, ++q)
+                {
+                    if (to_aos)
+                        aos[q] = soa[(size_t)m * d.slot + o];
+                    else
+                        soa[(size_t)m * d.slot + o] = aos[q];
+                }
+            }
+}
+} // namespace mpcns
+
+extern "C" int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case **out)
+{
+    if (!d || !out || d->n < 4 || ghost < 1)
+        return MHD_ERR_ARG;
+    try
+    {
+        auto h = std::make_unique<mhd_host_case>();
+        h->cs = make_case_orszag_tang(d->n, ghost, d->gamma, d->cfl, d->seed, d->noise);
+        *out = h.release();
+        return MHD_OK;
+    }
+    catch (const std::exception &)
+    {
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_case **out)
+{
+    if (!d || !out || d->nang < 4 || d->nrad < 4 || ghost < 1)
+        return MHD_ERR_ARG;
+    try
+    {
+        WindParams w;
+        w.n_cc = d->n_cc;
+        w.v_kms = d->v_kms;
+        for (int a = 0; a < 3; ++a)
+        {
+            w.vdir[a] = d->vdir[a];
+            w.B_nT[a] = d->B_nT[a];
+        }
+        w.T_eV = d->T_eV;
+        w.m_amu = d->m_amu > 0 ? d->m_amu : 1.0;
+        auto h = std::make_unique<mhd_host_case>();
+        h->cs = make_case_lunar(d->nang, d->nrad, d->r_in, d->r_out, ghost, d->gamma, d->cfl, w);
+        *out = h.release();
+        return MHD_OK;
+    }
+    catch (const std::exception &)
+    {
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_case_destroy(mhd_host_case *h)
+{
+    delete h;
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost)
+{
+    if (!h)
+        return MHD_ERR_ARG;
+    const MhdBlockDev &d = h->cs.blk[0].d;
+    if (nblocks)
+        *nblocks = (int)h->cs.blk.size();
+    if (ni)
+        *ni = d.ni;
+    if (nj)
+        *nj = d.nj;
+    if (nk)
+        *nk = d.nk;
+    if (ghost)
+        *ghost = d.g;
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_field_count(mhd_host_case *h, int field, size_t *n)
+{
+    int region, slot, loc, nc;
+    if (!h || !n || !field_slot(field, 0, region, slot, loc, nc))
+        return MHD_ERR_ARG;
+    *n = aos_count(h->cs.blk[0].d, loc, nc);
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_download(mhd_host_case *h, int block, int field, double *aos, size_t n)
+{
+    int region, slot, loc, nc;
+    if (!h || !aos || block < 0 || block >= (int)h->cs.blk.size() || !field_slot(field, 0, region, slot, loc, nc))
+        return MHD_ERR_ARG;
+    BlockArrays &A = h->cs.blk[block];
+    if (n != aos_count(A.d, loc, nc))
+        return MHD_ERR_ARG;
+    std::vector<double> &v = region == 0 ? A.st : A.geo;
+    soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, aos, true);
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_upload(mhd_host_case *h, int block, int field, const double *aos, size_t n)
+{
+    int region, slot, loc, nc;
+    if (!h || !aos || block < 0 || block >= (int)h->cs.blk.size() || !field_slot(field, 0, region, slot, loc, nc))
+        return MHD_ERR_ARG;
+    BlockArrays &A = h->cs.blk[block];
+    if (n != aos_count(A.d, loc, nc))
+        return MHD_ERR_ARG;
+    std::vector<double> &v = region == 0 ? A.st : A.geo;
+    soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, const_cast<double *>(aos), false);
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_faces(mhd_host_case *h, int block, int *out)
+{
+    if (!h || !out || block < 0 || block >= (int)h->cs.grid.blocks.size())
+        return MHD_ERR_ARG;
+    const Block &B = h->cs.grid.blocks[block];
+    for (int f = 0; f < 6; ++f)
+    {
+        const FaceLink &L = B.face[f];
+        int *o = out + 14 * f;
+        o[0] = L.kind;
+        o[1] = L.bc;
+        o[2] = L.nb_block;
+        o[3] = L.nb_face;
+        o[4] = L.periodic;
+        for (int a = 0; a < 3; ++a)
+        {
+            o[5 + a] = L.trans.perm[a];
+            o[8 + a] = L.trans.sign[a];
+            o[11 + a] = L.trans.offset[a];
+        }
+    }
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n)
+{
+    if (!h || !out || block < 0 || block >= (int)h->cs.grid.blocks.size() || comp < 0 || comp > 2)
+        return MHD_ERR_ARG;
+    const Block &B = h->cs.grid.blocks[block];
+    const std::vector<double> &v = comp == 0 ? B.x : (comp == 1 ? B.y : B.z);
+    if (n != v.size())
+        return MHD_ERR_ARG;
+    std::memcpy(out, v.data(), sizeof(double) * n);
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_constants(mhd_host_case *h, double *o)
+{
+    if (!h || !o)
+        return MHD_ERR_ARG;
+    const MhdCaseConst &k = h->cs.cst;
+    const double v[22] = {k.gamma,    k.cfl,      k.inver_MA2, k.momentum_hall_coeff, k.M_H,     k.M_ref,
+                          k.ne_floor, k.state_coeff, k.qinf[0], k.qinf[1],            k.qinf[2], k.qinf[3],
+                          k.qinf[4],  k.B_imf[0], k.B_imf[1],  k.B_imf[2],            k.u_inf[0], k.u_inf[1],
+                          k.u_inf[2], k.dt_floor, 0.0,         0.0};
+    std::memcpy(o, v, sizeof(v));
+    return MHD_OK;
+}
+
+// Apply one of the case's halo/boundary op lists to its host arrays
+// (copy / negate / zero / constant ops only).
+extern "C" int mhd_host_apply_sync(mhd_host_case *h, int sync)
+{
+    if (!h || sync < 0 || sync >= SY_COUNT)
+        return MHD_ERR_ARG;
+    try
+    {
+        apply_ops_host(h->cs.blk, h->cs.ops[sync], h->cs.cst);
+    }
+    catch (const std::exception &)
+    {
+        return MHD_ERR_ARG;
+    }
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n)
+{
+    if (!h || !n || sync < 0 || sync >= SY_COUNT)
+        return MHD_ERR_ARG;
+    *n = h->cs.ops[sync].size();
+    return MHD_OK;
+}

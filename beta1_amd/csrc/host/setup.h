@@ -1,0 +1,195 @@
+// Host-side setup for the explicit MHD step: multi-block grids, topology,
+// geometry, mimetic weight tables, halo / boundary operation lists and
+// initial states.  Everything here runs once per grid shape or per case;
+// the per-step work is on the device.
+//
+// The C++ types mirror the reference's layers 1-6 (Grid/Block, TOPO
+// IndexTransform, FieldBlock extents, Halo regions, BoundaryCore slabs) in
+// reduced form: every block face is one patch (interface or physical).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../layout.h"
+
+namespace mpcns
+{
+
+struct Int3
+{
+    int i, j, k;
+};
+
+// Node-space index transform, same convention as the reference
+// (include/2_topology/2_MPCNS_Topology.h:22-29):
+//   nb[perm[a]] = sign[a] * local[a] + offset[a]
+struct IndexTransform
+{
+    int perm[3];
+    int sign[3];
+    int offset[3];
+};
+
+// One face of a block: -i,+i,-j,+j,-k,+k  (face = 2*axis + side).
+struct FaceLink
+{
+    int kind = 0;     // 0 = physical, 1 = interface
+    int bc = BC_NONE; // MhdBc for physical faces
+    int nb_block = -1;
+    int nb_face = -1;
+    int periodic = 0; // periodic interface: ghost nodes are mirrored (reference src/1_grid/1_MPCNS_Grid.cpp:1268)
+    IndexTransform trans{};
+};
+
+struct Block
+{
+    int ni = 0, nj = 0, nk = 0; // cells
+    int gn = 0;                 // ghost node layers
+    std::vector<double> x, y, z;
+    FaceLink face[6];
+
+    int n(int ax) const { return ax == 0 ? ni : (ax == 1 ? nj : nk); }
+    int64_t node_off(int i, int j, int k) const
+    {
+        const int sj = nj + 1 + 2 * gn, sk = nk + 1 + 2 * gn;
+        return ((int64_t)(i + gn) * sj + (j + gn)) * sk + (k + gn);
+    }
+    int64_t node_count() const
+    {
+        return (int64_t)(ni + 1 + 2 * gn) * (nj + 1 + 2 * gn) * (nk + 1 + 2 * gn);
+    }
+};
+
+struct Grid
+{
+    int g = 2; // field ghost depth
+    std::vector<Block> blocks;
+};
+
+// Per-block arrays in the device layout (see layout.h).
+struct BlockArrays
+{
+    MhdBlockDev d{};
+    std::vector<double> geo; // G_NSLOT * slot
+    std::vector<double> st;  // ST_NSLOT * slot
+};
+
+// One element-wise operation of a boundary / halo phase.  Offsets are
+// element offsets relative to the arena the kernel receives.
+struct MhdOp
+{
+    int64_t dst;
+    int64_t src;
+    int64_t stride; // distance between components (slot size)
+    double val;
+    int32_t kind;  // MhdOpKind
+    int32_t ncomp; // components handled by this op
+    int32_t cas;   // case index (constants of OP_QINF / OP_SOLID_*)
+    int32_t pad;
+};
+
+enum MhdOpKind
+{
+    OP_COPY = 0,        // dst[c] = src[c]
+    OP_NEG = 1,         // dst[c] = -src[c]
+    OP_ZERO = 2,        // dst[c] = 0
+    OP_CONST = 3,       // dst[0] = val
+    OP_SOLID_FIRST = 4, // absorbing wall, first interior layer from second
+    OP_SOLID_GHOST = 5, // absorbing wall, ghost from first interior layer
+    OP_QINF = 6         // dst[c] = qinf[c] of the case
+};
+
+// Op lists of one case; offsets are relative to the case's first block
+// state slot 0, in "global state element" units once the case is placed.
+enum MhdSync
+{
+    SY_EFACE_BC = 0,
+    SY_EFACE_HALO,
+    SY_EEDGE_BC,
+    SY_U_BC0,
+    SY_U_BC1,
+    SY_U_HALO,
+    SY_B_BC,
+    SY_B_HALO,
+    SY_JCELL_BC,
+    SY_UC_CORNER, // corner-diagonal cells take the contravariant velocity of their source
+    SY_COUNT
+};
+
+struct CaseSetup
+{
+    Grid grid;
+    MhdCaseConst cst{};
+    std::vector<BlockArrays> blk;          // geometry + initial state
+    // SY_COUNT op lists; offsets are case-relative state offsets
+    // (block * ST_NSLOT * slot + state_slot * slot + blk_off)
+    std::vector<std::vector<MhdOp>> ops;
+    std::string shape_key; // identical keys share geometry
+};
+
+// ---------------------------------------------------------------------------
+// Grid generators.
+Grid make_periodic_box(int nx, int ny, int nz, double lx, double ly, double lz, int g);
+Grid make_cubed_sphere_shell(int nang, int nrad, double r_in, double r_out, int g,
+                             int bc_inner, int bc_outer);
+
+// Link interface faces by matching node coordinates (used for multi-block
+// grids built from separate panels).
+void link_faces_by_coordinates(Grid &grd, double tol);
+
+// Fill ghost nodes: mirror at physical / periodic faces, copy across
+// interfaces.
+void fill_ghost_nodes(Grid &grd);
+
+// Geometry (reference src/3_field/4_MPCNS_Geometry.cpp) and mimetic weights
+// (reference Z4_Lunar/src/LunarSolver.cpp:243-930) into the device layout.
+void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk);
+
+// Locate the source of a ghost DOF through interfaces.
+struct DofSource
+{
+    int kind; // 0 interior, 1 physical ghost of src block, 2 corner clamp, 3 own physical slab (not a halo dof)
+    int block;
+    int loc;
+    int idx[3];
+    int perm[3];
+    int sign[3];
+};
+DofSource resolve_dof(const Grid &grd, int b, int loc, const int idx[3]);
+
+// Build halo + boundary op lists for a case (offsets relative to the case).
+void build_ops(CaseSetup &cs);
+
+// Execute copy/zero/const op lists on host state arrays (start-up syncs).
+void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops, const MhdCaseConst &cst);
+
+// Field number (mhd_field) -> (region 0 state / 1 geometry, first slot,
+// location, components); false when the field is not stored.
+bool field_slot(int field, int ucur, int &region, int &slot, int &loc, int &nc);
+// Elements of a field block in the reference AoS layout incl. ghosts.
+size_t aos_count(const MhdBlockDev &d, int loc, int nc);
+// Convert between nc consecutive SoA slots and the AoS FieldBlock layout.
+void soa_aos(const MhdBlockDev &d, int loc, int nc, double *soa, double *aos, bool to_aos);
+
+// Case builders.
+struct WindParams
+{
+    double n_cc;       // number density [cm^-3]
+    double v_kms;      // speed [km/s]
+    double vdir[3];    // unit flow direction
+    double T_eV;       // temperature [eV]
+    double B_nT[3];    // IMF [nT]
+    double m_amu;      // ion mass [amu] (H+ = 1)
+};
+
+CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t seed, double noise);
+CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma,
+                          double cfl, const WindParams &w);
+
+// Deterministic generator (SplitMix64).
+uint64_t splitmix64(uint64_t &s);
+double uniform01(uint64_t &s);
+
+} // namespace mpcns

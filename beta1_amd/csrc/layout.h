@@ -1,0 +1,130 @@
+// Shared host/device definitions for the explicit MHD step: staggering,
+// per-block device layout, slot numbering and the small POD records that
+// travel between the host setup code and the HIP kernels.
+//
+// Staggering follows the reference's StaggerLocation
+// (include/3_field/Field_Type.h:4-14) and FieldBlock::compute_extent
+// (src/3_field/1_Field_Block.cpp:51-131): a block of Ni x Nj x Nk cells has
+// N+1 node-like and N cell-like DOFs per axis, plus `g` ghost layers.
+//
+// Device layout (ours, not the reference's AoS): every field of a block is
+// stored structure-of-arrays, one "slot" per component, and every slot of
+// a block shares ONE padded index box (Ni+1+2g) x (Nj+1+2g) x (Nk+1+2g)
+// with k fastest.  All locations therefore share one index function and
+// neighbour offsets are the constants 1, SK and SJ*SK.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define MHD_HD __host__ __device__ inline
+#else
+#define MHD_HD inline
+#endif
+
+enum MhdLoc
+{
+    LOC_CELL = 0,
+    LOC_NODE = 1,
+    LOC_FACE_XI = 2,
+    LOC_FACE_ET = 3,
+    LOC_FACE_ZE = 4,
+    LOC_EDGE_XI = 5,
+    LOC_EDGE_ET = 6,
+    LOC_EDGE_ZE = 7
+};
+
+// 1 = cell-like on that axis, 0 = node-like
+// (src/4_halo/0_MPCNS_Halo_Build_BoxMakers.cpp:9-33).
+MHD_HD int loc_delta(int loc, int ax)
+{
+    switch (loc)
+    {
+    case LOC_CELL:
+        return 1;
+    case LOC_NODE:
+        return 0;
+    case LOC_FACE_XI:
+    case LOC_FACE_ET:
+    case LOC_FACE_ZE:
+        return (ax == loc - LOC_FACE_XI) ? 0 : 1;
+    default:
+        return (ax == loc - LOC_EDGE_XI) ? 1 : 0;
+    }
+}
+
+// Number of interior DOFs of `loc` along axis `ax` for N cells.
+MHD_HD int loc_count(int loc, int ax, int n) { return n + 1 - loc_delta(loc, ax); }
+
+// ---------------------------------------------------------------------------
+// State slots (per case, per block).  U is double-buffered: the fluid kernel
+// reads U[cur] and writes U[1-cur].
+enum MhdStateSlot
+{
+    ST_U0 = 0,     // 5 comps: rho, rho u, rho v, rho w, E
+    ST_U1 = 5,     // 5 comps (second buffer)
+    ST_B = 10,     // 3 slots: B_xi (FaceXi), B_eta (FaceEt), B_zeta (FaceZe)
+    ST_BADD = 13,  // 3 slots: Badd_xi/eta/zeta
+    ST_BCELL = 16, // 3 comps: total cell B
+    ST_JCELL = 19, // 3 comps: cell J
+    ST_EFACE = 22, // 9 slots: Eface_xi(3 comps), Eface_eta(3), Eface_zeta(3)
+    ST_E = 31,     // 3 slots: E_xi (EdgeXi), E_eta, E_zeta
+    ST_UC = 34,    // 3 slots: cell contravariant velocity per axis (step scratch)
+    ST_NSLOT = 37
+};
+
+// Geometry slots (shared by every case with the same grid).
+enum MhdGeoSlot
+{
+    G_JAC = 0,    // cell volume
+    G_JD = 1,     // 9 slots: JDxi(3), JDet(3), JDze(3) face area vectors
+    G_BETA = 10,  // 3 slots: face Hodge weight beta = |l*|/|S|
+    G_ALPHA = 13, // 3 slots: edge Hodge weight alpha = |e|/|S*|
+    G_WB = 16,    // 18 comps: Bcell_from_Bface_w
+    G_WJ = 34,    // 36 comps: Jcell_from_Jedge_w
+    G_NSLOT = 70
+};
+
+// Physical boundary kinds on a block face.
+enum MhdBc
+{
+    BC_NONE = 0, // interface (halo) or unused
+    BC_FARFIELD = 1,
+    BC_SOLID = 2
+};
+
+// Per-block record seen by every kernel.
+struct MhdBlockDev
+{
+    int ni, nj, nk;  // cells
+    int g;           // ghost depth
+    int sj, sk;      // strides: off = ((i+g)*sj + (j+g))*sk + (k+g)
+    int case_id;     // which case (dt, constants) this block belongs to
+    int bc[6];       // BC kind per face: -i,+i,-j,+j,-k,+k
+    int64_t slot;    // elements per slot
+    int64_t st;      // element offset of slot 0 of the state region
+    int64_t geo;     // element offset of slot 0 of the geometry region
+    int64_t geo32;   // element offset in the fp32 geometry arena (-1 if none)
+};
+
+// Per-case constants (the reference's calc_physical_constant,
+// Z4_Lunar/src/physics/LunarSolver_Tools.cpp:3-45).
+struct MhdCaseConst
+{
+    double gamma;
+    double cfl;
+    double inver_MA2;           // B_ref^2/(U_ref^2 mu0 rho_ref)
+    double momentum_hall_coeff; // B_ref^2/(mu0 rho_ref U_ref^2)
+    double M_H;                 // ion molar mass [kg/mol]
+    double M_ref;               // reference molar mass [kg/mol]
+    double ne_floor;            // ne_hall_floor (nondimensional)
+    double state_coeff;         // p = rho T coeff
+    double qinf[5];             // far-field conservative state
+    double B_imf[3];            // far-field total B
+    double u_inf[3];            // far-field velocity (primitive)
+    double dt_floor;            // abort threshold (reference: 1e-7)
+};
+
+MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)
+{
+    return ((int64_t)(i + b.g) * b.sj + (j + b.g)) * b.sk + (k + b.g);
+}

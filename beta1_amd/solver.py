@@ -1,0 +1,250 @@
+"""Host-side objects over the C-ABI.
+
+``MhdContext`` owns a device context with one or more cases; ``LunarSolver``
+mirrors the reference driver's solver object (Z4_Lunar/include/LunarSolver.h:
+55-67: construct, ``Advance()``, ``StepOnce()``) on top of it; ``HostCase``
+builds a case on the CPU only (no device needed).
+"""
+import ctypes as C
+from dataclasses import dataclass, field as dfield
+
+import numpy as np
+
+from .capi import MhdError, MhdOptions, MhdOTDesc, MhdWindDesc, lib
+from .fields import FIELD_ID, shape
+
+PREC = {"fp64": 0, "mixed": 1}
+
+
+@dataclass
+class OTDesc:
+    """Orszag-Tang vortex in a periodic box of n^3 cells."""
+    n: int = 64
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.4
+    seed: int = 20261203
+    noise: float = 1e-3
+
+    def c(self):
+        return MhdOTDesc(self.n, self.gamma, self.cfl, self.seed, self.noise)
+
+
+@dataclass
+class WindDesc:
+    """Solar-wind shell case (6-block cubed sphere)."""
+    nang: int = 64
+    nrad: int = 128
+    r_in: float = 1.0
+    r_out: float = 8.0
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.4
+    n_cc: float = 5.0
+    v_kms: float = 400.0
+    vdir: tuple = (-1.0, 0.0, 0.0)
+    T_eV: float = 10.0
+    B_nT: tuple = dfield(default_factory=lambda: (5.0 * 0.7071067811865476, 5.0 * 0.7071067811865476, 0.0))
+    m_amu: float = 1.0
+
+    def c(self):
+        d = MhdWindDesc()
+        d.nang, d.nrad, d.r_in, d.r_out = self.nang, self.nrad, self.r_in, self.r_out
+        d.gamma, d.cfl, d.n_cc, d.v_kms = self.gamma, self.cfl, self.n_cc, self.v_kms
+        for a in range(3):
+            d.vdir[a] = self.vdir[a]
+            d.B_nT[a] = self.B_nT[a]
+        d.T_eV, d.m_amu = self.T_eV, self.m_amu
+        return d
+
+    @property
+    def cells(self):
+        return 6 * self.nang * self.nang * self.nrad
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = ""
+        if ctx is not None:
+            m = lib().mhd_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise MhdError(f"{what} failed ({rc}): {msg}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class HostCase:
+    """A case built on the CPU: grid, topology, geometry, weights, ops, state."""
+
+    def __init__(self, desc, ghost=2):
+        self.h = C.c_void_p()
+        if isinstance(desc, OTDesc):
+            _check(lib().mhd_host_case_ot(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_ot")
+        else:
+            _check(lib().mhd_host_case_lunar(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_lunar")
+        nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
+        lib().mhd_host_case_info(self.h, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g))
+        self.nblocks, self.n, self.g = nb.value, (ni.value, nj.value, nk.value), g.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mhd_host_case_destroy(self.h)
+            self.h = None
+
+    def field(self, name, block):
+        a = np.empty(shape(name, self.n, self.g), dtype=np.float64)
+        _check(lib().mhd_host_download(self.h, block, FIELD_ID[name], _dptr(a), a.size), what=f"download {name}")
+        return a
+
+    def set_field(self, name, block, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().mhd_host_upload(self.h, block, FIELD_ID[name], _dptr(a), a.size), what=f"upload {name}")
+
+    def faces(self, block):
+        out = (C.c_int * 84)()
+        _check(lib().mhd_host_faces(self.h, block, out), what="faces")
+        return np.array(out, dtype=np.int32).reshape(6, 14)
+
+    def nodes(self, block):
+        gn = self.g + 1
+        s = tuple(self.n[a] + 1 + 2 * gn for a in range(3))
+        out = []
+        for comp in range(3):
+            a = np.empty(s, dtype=np.float64)
+            _check(lib().mhd_host_nodes(self.h, block, comp, _dptr(a), a.size), what="nodes")
+            out.append(a)
+        return out
+
+    def constants(self):
+        out = np.zeros(22)
+        lib().mhd_host_constants(self.h, _dptr(out))
+        return out
+
+    def apply_sync(self, sync):
+        _check(lib().mhd_host_apply_sync(self.h, sync), what="apply_sync")
+
+    def op_count(self, sync):
+        n = C.c_size_t()
+        lib().mhd_host_op_count(self.h, sync, C.byref(n))
+        return n.value
+
+
+class MhdContext:
+    """Device context holding one or more cases (all advanced together)."""
+
+    def __init__(self, precision="fp64", recon=0, ghost=2, device=0, hbm_budget_gb=0.0):
+        opt = MhdOptions(device, PREC[precision], recon, ghost, hbm_budget_gb)
+        self.ctx = C.c_void_p()
+        _check(lib().mhd_create(C.byref(opt), C.byref(self.ctx)), what="mhd_create")
+        self.cases = []
+        self.ghost = ghost
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().mhd_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, desc):
+        cid = C.c_int()
+        if isinstance(desc, OTDesc):
+            rc = lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
+        else:
+            rc = lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
+        _check(rc, self.ctx, "add case")
+        self.cases.append(desc)
+        return cid.value
+
+    def finalize(self):
+        _check(lib().mhd_finalize(self.ctx), self.ctx, "mhd_finalize")
+
+    def step(self, n=1, graph=False):
+        f = lib().mhd_step_graph if graph else lib().mhd_step
+        _check(f(self.ctx, n), self.ctx, "mhd_step")
+
+    def sync(self):
+        _check(lib().mhd_synchronize(self.ctx), self.ctx, "sync")
+
+    def info(self, cid):
+        nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
+        _check(lib().mhd_case_info(self.ctx, cid, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g)),
+               self.ctx, "info")
+        return nb.value, (ni.value, nj.value, nk.value), g.value
+
+    def field(self, cid, name, block):
+        _, n, g = self.info(cid)
+        a = np.empty(shape(name, n, g), dtype=np.float64)
+        _check(lib().mhd_download(self.ctx, cid, block, FIELD_ID[name], _dptr(a), a.size), self.ctx,
+               f"download {name}")
+        return a
+
+    def set_field(self, cid, name, block, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().mhd_upload(self.ctx, cid, block, FIELD_ID[name], _dptr(a), a.size), self.ctx, f"upload {name}")
+
+    def compute_dt(self, cid):
+        v = C.c_double()
+        _check(lib().mhd_compute_dt(self.ctx, cid, C.byref(v)), self.ctx, "compute_dt")
+        return v.value
+
+    def last_dt(self, cid):
+        v = C.c_double()
+        _check(lib().mhd_last_dt(self.ctx, cid, C.byref(v)), self.ctx, "last_dt")
+        return v.value
+
+    def divb_max(self, cid):
+        v = C.c_double()
+        _check(lib().mhd_divb_max(self.ctx, cid, C.byref(v)), self.ctx, "divb")
+        return v.value
+
+    def profile(self, on):
+        lib().mhd_profile(self.ctx, 1 if on else 0)
+
+    def profile_read(self):
+        out = {}
+        k = 0
+        while True:
+            ms, n, name = C.c_double(), C.c_int(), C.c_char_p()
+            if lib().mhd_profile_read(self.ctx, k, C.byref(ms), C.byref(n), C.byref(name)) != 0:
+                break
+            out[name.value.decode()] = (ms.value, n.value)
+            k += 1
+        return out
+
+    def memory_bytes(self):
+        v = C.c_double()
+        lib().mhd_memory(self.ctx, C.byref(v))
+        return v.value
+
+
+class LunarSolver:
+    """Solver object with the reference driver's life cycle.
+
+    ``LunarSolver(desc)`` sets up grid, topology, geometry, halo patterns,
+    boundary registry and the initial state (reference constructor,
+    Z4_Lunar/src/LunarSolver.cpp:12-235); ``StepOnce()`` is one explicit step
+    (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92); ``Advance(nsteps)`` loops it
+    (the reference loops until its stop switch, :6-19).
+    """
+
+    def __init__(self, desc, precision="fp64", recon=0, ghost=2, device=0):
+        self.ctx = MhdContext(precision, recon, ghost, device)
+        self.case = self.ctx.add(desc)
+        self.ctx.finalize()
+        self.step = 0
+        self.time = 0.0
+
+    def StepOnce(self):
+        self.ctx.step(1)
+        self.step += 1
+        self.time += self.ctx.last_dt(self.case)
+        return False
+
+    def Advance(self, nsteps, graph=True):
+        self.ctx.step(nsteps, graph=graph)
+        self.step += nsteps
+
+    def field(self, name, block):
+        return self.ctx.field(self.case, name, block)

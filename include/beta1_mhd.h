@@ -1,0 +1,161 @@
+/* beta1_mhd.h -- C-ABI of the MI355X explicit MHD step.
+ *
+ * Drop-in boundary for the explicit step of the reference's solver drivers
+ * (Z4_Lunar, Z5_Mercury).  Each entry point names the reference interface it
+ * replaces.  Conventions: int status (0 = ok, negative = error, never
+ * exit()); caller-owned host buffers; device memory owned by the context;
+ * one HIP stream per context; a context is not thread-safe.
+ *
+ * Field numbering (MHD_F_*) follows the reference's registered field names
+ * (Z4_Lunar/main.cpp:61-121, src/3_field/4_MPCNS_Geometry.cpp:6-73); host
+ * buffers use the reference AoS FieldBlock layout [i][j][k][m] over the
+ * FieldBlock extent with `ghost` layers (src/3_field/1_Field_Block.cpp:51-131).
+ */
+#ifndef BETA1_MHD_H
+#define BETA1_MHD_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhd_ctx mhd_ctx;
+
+enum mhd_status
+{
+    MHD_OK = 0,
+    MHD_ERR_ARG = -1,   /* bad argument / unknown id */
+    MHD_ERR_HIP = -2,   /* HIP runtime error */
+    MHD_ERR_DT = -3,    /* dt NaN or below the abort floor
+                           (LunarSolver_Timestep.cpp:128-139) */
+    MHD_ERR_STATE = -4, /* call out of order (e.g. step before finalize) */
+    MHD_ERR_MEM = -5    /* device memory budget exceeded */
+};
+
+enum mhd_precision
+{
+    MHD_PREC_FP64 = 0, /* plain FP64 step, bit-compatible with the reference */
+    MHD_PREC_MIXED = 1 /* FP32 flux / EMF arithmetic, FP64 state, metric, curl, dt */
+};
+
+enum mhd_field
+{
+    MHD_F_U_H = 0, MHD_F_PV_H, MHD_F_U_PLUS, MHD_F_B_XI, MHD_F_B_ETA, MHD_F_B_ZETA,
+    MHD_F_BADD_XI, MHD_F_BADD_ETA, MHD_F_BADD_ZETA, MHD_F_B_CELL, MHD_F_BIND_CELL,
+    MHD_F_J_XI, MHD_F_J_ETA, MHD_F_J_ZETA, MHD_F_J_CELL, MHD_F_E_XI, MHD_F_E_ETA, MHD_F_E_ZETA,
+    MHD_F_EFACE_XI, MHD_F_EFACE_ETA, MHD_F_EFACE_ZETA, MHD_F_F_XI, MHD_F_F_ETA, MHD_F_F_ZETA,
+    MHD_F_RHS_H, MHD_F_RHS_B_XI, MHD_F_RHS_B_ETA, MHD_F_RHS_B_ZETA, MHD_F_DIVB,
+    MHD_F_JAC, MHD_F_JDXI, MHD_F_JDET, MHD_F_JDZE, MHD_F_BETA_XI, MHD_F_BETA_ETA, MHD_F_BETA_ZETA,
+    MHD_F_ALPHA_XI, MHD_F_ALPHA_ETA, MHD_F_ALPHA_ZETA, MHD_F_BCELL_W, MHD_F_JCELL_W, MHD_F_COUNT
+};
+
+typedef struct
+{
+    int device;    /* HIP device ordinal */
+    int precision; /* mhd_precision */
+    int recon;     /* 0 first order (reference), 1 minmod MUSCL */
+    int ghost;     /* ghost layers (>= 1; >= 2 for minmod) */
+    double hbm_budget_gb; /* admission budget for serving (0 = no limit) */
+} mhd_options;
+
+/* Orszag-Tang periodic box (n^3 cells on [0,1)^3). */
+typedef struct
+{
+    int n;
+    double gamma, cfl;
+    unsigned long long seed;
+    double noise; /* relative uniform noise on rho and p */
+} mhd_ot_desc;
+
+/* Solar-wind shell around an absorbing insulating body: 6-block cubed
+   sphere, nang x nang x nrad cells per block, log-stretched radius. */
+typedef struct
+{
+    int nang, nrad;
+    double r_in, r_out, gamma, cfl;
+    double n_cc, v_kms, vdir[3], T_eV, B_nT[3], m_amu;
+} mhd_wind_desc;
+
+/* Context lifetime.  Replaces the solver constructor's set-up
+   (LunarSolver::LunarSolver, Z4_Lunar/src/LunarSolver.cpp:12-235). */
+int mhd_create(const mhd_options *opt, mhd_ctx **out);
+int mhd_destroy(mhd_ctx *ctx);
+const char *mhd_last_error(mhd_ctx *ctx);
+
+/* Case set-up: grid, topology (Grid_Preprocess + TOPO::build_topology,
+   include/1_grid/1_MPCNS_Grid.h:152, include/2_topology/2_MPCNS_Topology.h:154),
+   geometry (Field::build_geometry, src/3_field/4_MPCNS_Geometry.cpp:4),
+   weight tables (SetupHallFaceScratch_, Z4_Lunar/src/LunarSolver.cpp:243),
+   halo patterns (Halo::build_registered_patterns, include/4_halo/1_MPCNS_Halo.h:9-43),
+   boundary registry (LunarBoundary::Setup, Z4_Lunar/include/1_Boundary.h:29)
+   and the initial state (Lunar_Initial::Initialization, Z4_Lunar/include/2_Initial.h:16). */
+int mhd_add_case_orszag_tang(mhd_ctx *ctx, const mhd_ot_desc *d, int *case_id);
+int mhd_add_case_lunar(mhd_ctx *ctx, const mhd_wind_desc *d, int *case_id);
+/* Upload everything added so far and compute the derived fields
+   (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143). */
+int mhd_finalize(mhd_ctx *ctx);
+
+/* Advance every case by nsteps explicit steps, each case with its own dt
+   (LunarSolver::StepOnce, Z4_Lunar/src/LunarSolver_Driver.cpp:21-92).
+   Returns MHD_ERR_DT if any case hit the dt abort condition. */
+int mhd_step(mhd_ctx *ctx, int nsteps);
+/* Same as mhd_step through a captured HIP graph of two steps. */
+int mhd_step_graph(mhd_ctx *ctx, int nsteps);
+/* Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140)
+   on the current state, without stepping. */
+int mhd_compute_dt(mhd_ctx *ctx, int case_id, double *dt);
+/* dt used by the last step of a case. */
+int mhd_last_dt(mhd_ctx *ctx, int case_id, double *dt);
+/* max |div B| over interior cells (calc_divB, LunarSolver_Tools.cpp:685-775). */
+int mhd_divb_max(mhd_ctx *ctx, int case_id, double *out);
+int mhd_synchronize(mhd_ctx *ctx);
+
+/* Introspection and data exchange (FieldBlock::raw_data round trip,
+   include/3_field/1_Field_Block.h:47). */
+int mhd_case_info(mhd_ctx *ctx, int case_id, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
+int mhd_field_count(mhd_ctx *ctx, int case_id, int field, size_t *n);
+int mhd_download(mhd_ctx *ctx, int case_id, int block, int field, double *aos, size_t n);
+int mhd_upload(mhd_ctx *ctx, int case_id, int block, int field, const double *aos, size_t n);
+/* Topology of a block: 6 faces (-i,+i,-j,+j,-k,+k) x 14 ints
+   {kind (0 physical, 1 interface), bc, nb_block, nb_face, periodic,
+    perm[3], sign[3], offset[3]} with the reference's IndexTransform
+   convention nb[perm[a]] = sign[a]*local[a] + offset[a]
+   (include/2_topology/2_MPCNS_Topology.h:22-29). */
+int mhd_get_faces(mhd_ctx *ctx, int case_id, int block, int *faces14x6);
+/* node coordinates incl. ghost+1 layers, comp 0..2 */
+int mhd_get_nodes(mhd_ctx *ctx, int case_id, int block, int comp, double *out, size_t n);
+/* case constants: gamma, cfl, inver_MA2, momentum_hall_coeff, M_H, M_ref,
+   ne_floor, state_coeff, qinf[5], B_imf[3], u_inf[3], dt_floor (22 doubles) */
+int mhd_get_constants(mhd_ctx *ctx, int case_id, double *out22);
+/* Per-kernel timing of the last mhd_step call (ms, summed per kernel kind)
+   measured with HIP events on the context's stream; names via index. */
+int mhd_profile(mhd_ctx *ctx, int enable);
+int mhd_profile_read(mhd_ctx *ctx, int kernel, double *ms, int *launches, const char **name);
+/* device bytes held by the context */
+int mhd_memory(mhd_ctx *ctx, double *bytes);
+
+/* ---- host-only case builder (no device needed) ------------------------
+   The same grid / topology / geometry / weights / initial-state builders the
+   device context uses, run on the CPU and read back in the AoS layout. */
+typedef struct mhd_host_case mhd_host_case;
+int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case **out);
+int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_case **out);
+int mhd_host_case_destroy(mhd_host_case *h);
+int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
+int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);
+int mhd_host_download(mhd_host_case *h, int block, int field, double *aos, size_t n);
+int mhd_host_upload(mhd_host_case *h, int block, int field, const double *aos, size_t n);
+int mhd_host_faces(mhd_host_case *h, int block, int *faces14x6);
+int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n);
+int mhd_host_constants(mhd_host_case *h, double *out22);
+/* apply one halo / boundary op list (index as in the library's sync table:
+   0 Eface BC, 1 Eface halo, 2 Eedge BC, 3-5 Ucell BC0/BC1/halo, 6-7 Bface
+   BC/halo, 8 J_cell BC, 9 corner velocities) to the host arrays; copy, negate,
+   zero and constant ops only */
+int mhd_host_apply_sync(mhd_host_case *h, int sync);
+int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+/* CPU FP64 oracle of the explicit MHD step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's LunarSolver::StepOnce
+ * (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) on field blocks stored in the
+ * reference's own AoS layout [i][j][k][m] (include/3_field/Field_Array.h:177)
+ * with the extents of FieldBlock::compute_extent
+ * (src/3_field/1_Field_Block.cpp:51-131).  It is used by the tests, by the
+ * smoke check and by the bench's CPU-baseline leg as the checker; nothing in
+ * the product links or calls it.
+ */
+#ifndef MHD_ORACLE_H
+#define MHD_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct
+{
+    int kind; /* 0 physical, 1 interface */
+    int bc;   /* 1 farfield, 2 solid (physical faces) */
+    int nb_block, nb_face, periodic;
+    int perm[3], sign[3], offset[3]; /* nb[perm[a]] = sign[a]*local[a] + offset[a] (node space) */
+} or_face;
+
+typedef struct
+{
+    int nblocks;
+    int g;     /* ghost depth of every field */
+    int recon; /* 0 first order (reference), 1 minmod */
+    double gamma, cfl, inver_MA2, momentum_hall_coeff, M_H, M_ref, ne_floor, state_coeff;
+    double qinf[5], B_imf[3], u_inf[3];
+    double dt_floor;
+} or_desc;
+
+enum
+{
+    OF_UH = 0, OF_PV, OF_UPLUS, OF_BXI, OF_BET, OF_BZE, OF_BADDXI, OF_BADDET, OF_BADDZE,
+    OF_BCELL, OF_BIND, OF_JXI, OF_JET, OF_JZE, OF_JCELL, OF_EXI, OF_EET, OF_EZE,
+    OF_EFXI, OF_EFET, OF_EFZE, OF_FXI, OF_FET, OF_FZE, OF_RHSH, OF_RHSBXI, OF_RHSBET, OF_RHSBZE,
+    OF_DIVB, OF_JAC, OF_JDXI, OF_JDET, OF_JDZE, OF_BETAXI, OF_BETAET, OF_BETAZE,
+    OF_ALXI, OF_ALET, OF_ALZE, OF_WB, OF_WJ, OF_COUNT
+};
+
+typedef struct or_ctx or_ctx;
+
+or_ctx *or_create(const or_desc *d);
+void or_destroy(or_ctx *c);
+/* faces: 6 entries (-i,+i,-j,+j,-k,+k) */
+int or_set_block(or_ctx *c, int b, int ni, int nj, int nk, const or_face *faces);
+/* pointer to the AoS storage of a field (allocated by or_set_block) */
+double *or_field(or_ctx *c, int b, int fid, long *count);
+/* node coordinates, box [-(g+1), n+g+1]^3, k fastest */
+double *or_nodes(or_ctx *c, int b, int comp, long *count);
+/* UpdateDerivedFields_ (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143-161) */
+int or_update_derived(or_ctx *c);
+/* Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140) */
+int or_compute_dt(or_ctx *c, double *dt);
+/* one StepOnce; returns 0 or a negative error (dt underflow / NaN) */
+int or_step(or_ctx *c, double *dt_used);
+/* max |div B| over interior cells (calc_divB, LunarSolver_Tools.cpp:685-775) */
+double or_divb_max(or_ctx *c);
+/* 1: the single-block fully periodic halo uses an index wrap instead of
+   the generic interface resolution */
+void or_use_periodic_wrap(or_ctx *c, int on);
+/* run only the named sync group on the current arrays (tests) */
+int or_sync(or_ctx *c, const char *group);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

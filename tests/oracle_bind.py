@@ -1,0 +1,131 @@
+"""ctypes binding of the CPU oracle (oracle/libmhd_oracle.so) for the tests.
+
+The oracle is the checker: it is loaded here, in ``__graft_entry__.smoke`` and
+in the bench's CPU-baseline leg only.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from beta1_amd.fields import FIELD_ID, shape
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "libmhd_oracle.so")
+
+
+class OrFace(C.Structure):
+    _fields_ = [("kind", C.c_int), ("bc", C.c_int), ("nb_block", C.c_int), ("nb_face", C.c_int),
+                ("periodic", C.c_int), ("perm", C.c_int * 3), ("sign", C.c_int * 3), ("offset", C.c_int * 3)]
+
+
+class OrDesc(C.Structure):
+    _fields_ = [("nblocks", C.c_int), ("g", C.c_int), ("recon", C.c_int),
+                ("gamma", C.c_double), ("cfl", C.c_double), ("inver_MA2", C.c_double),
+                ("momentum_hall_coeff", C.c_double), ("M_H", C.c_double), ("M_ref", C.c_double),
+                ("ne_floor", C.c_double), ("state_coeff", C.c_double), ("qinf", C.c_double * 5),
+                ("B_imf", C.c_double * 3), ("u_inf", C.c_double * 3), ("dt_floor", C.c_double)]
+
+
+_lib = None
+
+
+def olib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        L.or_create.restype = C.c_void_p
+        L.or_create.argtypes = [C.POINTER(OrDesc)]
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_set_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(OrFace)]
+        L.or_field.restype = C.POINTER(C.c_double)
+        L.or_field.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.or_nodes.restype = C.POINTER(C.c_double)
+        L.or_nodes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.or_update_derived.argtypes = [C.c_void_p]
+        L.or_compute_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.or_step.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.or_divb_max.restype = C.c_double
+        L.or_divb_max.argtypes = [C.c_void_p]
+        L.or_use_periodic_wrap.argtypes = [C.c_void_p, C.c_int]
+        L.or_sync.argtypes = [C.c_void_p, C.c_char_p]
+        _lib = L
+    return _lib
+
+
+# oracle field numbering equals the C-ABI field numbering (mhd_field)
+GEOMETRY = ["Jac", "JDxi", "JDet", "JDze", "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta",
+            "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
+STATE = ["U_H", "B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta"]
+
+
+class Oracle:
+    """CPU FP64 restatement of the reference step, fed from a HostCase."""
+
+    def __init__(self, hc, recon=0, periodic_wrap=False):
+        k = hc.constants()
+        d = OrDesc()
+        d.nblocks, d.g, d.recon = hc.nblocks, hc.g, recon
+        (d.gamma, d.cfl, d.inver_MA2, d.momentum_hall_coeff, d.M_H, d.M_ref, d.ne_floor,
+         d.state_coeff) = k[:8]
+        for m in range(5):
+            d.qinf[m] = k[8 + m]
+        for a in range(3):
+            d.B_imf[a] = k[13 + a]
+            d.u_inf[a] = k[16 + a]
+        d.dt_floor = k[19]
+        L = olib()
+        self.c = L.or_create(C.byref(d))
+        self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks
+        for b in range(hc.nblocks):
+            fc = hc.faces(b)
+            faces = (OrFace * 6)()
+            for f in range(6):
+                r = fc[f]
+                faces[f].kind, faces[f].bc, faces[f].nb_block, faces[f].nb_face, faces[f].periodic = map(int, r[:5])
+                for a in range(3):
+                    faces[f].perm[a], faces[f].sign[a], faces[f].offset[a] = int(r[5 + a]), int(r[8 + a]), int(r[11 + a])
+            L.or_set_block(self.c, b, hc.n[0], hc.n[1], hc.n[2], faces)
+            for name in GEOMETRY + STATE:
+                self.view(name, b)[...] = hc.field(name, b)
+            for comp, arr in enumerate(hc.nodes(b)):
+                cnt = C.c_long()
+                p = L.or_nodes(self.c, b, comp, C.byref(cnt))
+                np.ctypeslib.as_array(p, shape=(cnt.value,))[:] = arr.ravel()
+        L.or_use_periodic_wrap(self.c, 1 if periodic_wrap else 0)
+        L.or_update_derived(self.c)
+
+    def __del__(self):
+        if getattr(self, "c", None):
+            olib().or_destroy(self.c)
+            self.c = None
+
+    def view(self, name, b):
+        """Writable numpy view of the oracle's AoS field block."""
+        cnt = C.c_long()
+        p = olib().or_field(self.c, b, FIELD_ID[name], C.byref(cnt))
+        g0 = 0 if name in ("F_xi", "F_eta", "F_zeta", "RHS_H", "RHS_B_xi", "RHS_B_eta", "RHS_B_zeta") else self.g
+        return np.ctypeslib.as_array(p, shape=(cnt.value,)).reshape(shape(name, self.n, g0))
+
+    def step(self, n=1):
+        dts = []
+        for _ in range(n):
+            dt = C.c_double()
+            rc = olib().or_step(self.c, C.byref(dt))
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed ({rc}), dt={dt.value}")
+            dts.append(dt.value)
+        return dts
+
+    def compute_dt(self):
+        dt = C.c_double()
+        olib().or_compute_dt(self.c, C.byref(dt))
+        return dt.value
+
+    def divb_max(self):
+        return olib().or_divb_max(self.c)
+
+    def sync(self, group):
+        return olib().or_sync(self.c, group.encode())
