@@ -1,0 +1,346 @@
+"""Pins for the CPU oracle and the host set-up (CPU only).
+
+The reference ships no tests, fixtures or golden vectors and cannot be built
+here (PETSc is absent), so the oracle is pinned by what the mathematics fixes:
+free-stream preservation, a textbook 1-D Rusanov/LLF update computed
+independently in numpy, div(curl E) = 0 of the CT update, closed-form CFL
+limits, weight tables reproducing uniform fields, box geometry, GCL closure,
+and halo ghosts coinciding with the DOFs they copy.
+"""
+import numpy as np
+import pytest
+
+from beta1_amd import HostCase, OTDesc, WindDesc
+from beta1_amd.fields import interior
+from oracle_bind import Oracle
+
+G = 2
+
+
+def set_uniform(orc, rho, u, p, gamma, nblocks=1):
+    for b in range(nblocks):
+        U = orc.view("U_H", b)
+        U[..., 0] = rho
+        for a in range(3):
+            U[..., 1 + a] = rho * u[a]
+        U[..., 4] = p / (gamma - 1) + 0.5 * rho * sum(x * x for x in u)
+        for name in ("B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta"):
+            orc.view(name, b)[...] = 0.0
+
+
+def refresh(orc):
+    from oracle_bind import olib
+    olib().or_update_derived(orc.c)
+
+
+# ------------------------------------------------------------------ fluxes
+def test_free_stream_is_preserved_exactly():
+    hc = HostCase(OTDesc(n=8), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    set_uniform(orc, 1.3, (0.3, -0.2, 0.1), 0.6, 5 / 3)
+    refresh(orc)
+    U0 = orc.view("U_H", 0).copy()
+    orc.step(3)
+    assert np.array_equal(orc.view("U_H", 0)[G:-G, G:-G, G:-G], U0[G:-G, G:-G, G:-G])
+
+
+def llf_update(U, dt, h, gamma):
+    """Textbook first-order Rusanov (local Lax-Friedrichs) update on a periodic
+    1-D array (Toro, Riemann Solvers, ch. 10), conservative variables
+    (rho, rho u, rho v, rho w, E), flow along x."""
+    rho = U[:, 0]
+    u, v, w = U[:, 1] / rho, U[:, 2] / rho, U[:, 3] / rho
+    E = U[:, 4]
+    p = (gamma - 1) * (E - 0.5 * rho * (u * u + v * v + w * w))
+    F = np.stack([rho * u, rho * u * u + p, rho * u * v, rho * u * w, u * (E + p)], axis=1)
+    a = np.abs(u) + np.sqrt(gamma * p / rho)
+    UL, UR, FL, FR = U, np.roll(U, -1, 0), F, np.roll(F, -1, 0)
+    amax = np.maximum(a, np.roll(a, -1))
+    Fh = 0.5 * (FL + FR) - 0.5 * amax[:, None] * (UR - UL)  # flux at i+1/2
+    return U - dt / h * (Fh - np.roll(Fh, 1, 0))
+
+
+def test_rusanov_matches_textbook_llf_1d():
+    n, gamma = 16, 5 / 3
+    hc = HostCase(OTDesc(n=n), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    x = (np.arange(n) + 0.5) / n
+    rho = 1.0 + 0.4 * np.sin(2 * np.pi * x)
+    u = 0.5 + 0.2 * np.cos(2 * np.pi * x)
+    v = 0.1 * np.sin(4 * np.pi * x)
+    p = 0.8 + 0.3 * np.cos(2 * np.pi * x)
+    U1 = np.stack([rho, rho * u, rho * v, 0 * rho, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], axis=1)
+    set_uniform(orc, 1.0, (0, 0, 0), 1.0, gamma)
+    Uo = orc.view("U_H", 0)
+    Uo[G:-G, G:-G, G:-G, :] = U1[:, None, None, :]
+    orc.sync("Ucell")
+    refresh(orc)
+    dt = orc.step(1)[0]
+    expect = llf_update(U1, dt, 1.0 / n, gamma)
+    got = orc.view("U_H", 0)[G:-G, G + 3, G + 5, :]
+    assert np.abs(got - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+# ---------------------------------------------------------------- CT / EMF
+def test_ct_update_keeps_div_b_change_zero():
+    hc = HostCase(OTDesc(n=12), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    rng = np.random.default_rng(7)
+    n = 12
+    for name in ("B_xi", "B_eta", "B_zeta"):  # deliberately non-solenoidal
+        orc.view(name, 0)[G:-G, G:-G, G:-G] += 1e-3 * rng.standard_normal(orc.view(name, 0)[G:-G, G:-G, G:-G].shape)
+    orc.sync("Bface")
+    refresh(orc)
+
+    def div():
+        bx, by, bz = (orc.view(s, 0)[..., 0] for s in ("B_xi", "B_eta", "B_zeta"))
+        sl = slice(G, G + n)
+        return (bx[G + 1:G + n + 1, sl, sl] - bx[sl, sl, sl] + by[sl, G + 1:G + n + 1, sl] - by[sl, sl, sl]
+                + bz[sl, sl, G + 1:G + n + 1] - bz[sl, sl, sl])
+
+    d0 = div()
+    orc.step(2)
+    d1 = div()
+    assert np.abs(d0).max() > 1e-5
+    assert np.abs(d1 - d0).max() <= 1e-15 * max(1.0, np.abs(d0).max()) * 64
+
+
+def test_orszag_tang_div_b_stays_at_roundoff():
+    orc = Oracle(HostCase(OTDesc(n=16), ghost=G), periodic_wrap=True)
+    assert orc.divb_max() < 1e-12
+    orc.step(20)
+    assert orc.divb_max() < 1e-11
+
+
+def test_uniform_flow_uniform_field_gives_zero_emf_change():
+    """Uniform u and uniform B: face EMF candidates equal -(u x B).dr exactly
+    up to round-off, upwind terms vanish, B faces do not change."""
+    n, gamma = 8, 5 / 3
+    hc = HostCase(OTDesc(n=n), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    set_uniform(orc, 1.0, (0.3, 0.2, -0.1), 0.5, gamma)
+    h = 1.0 / n
+    Bv = (0.7, -0.4, 0.25)
+    for a, name in enumerate(("B_xi", "B_eta", "B_zeta")):
+        orc.view(name, 0)[...] = Bv[a] * h * h
+    refresh(orc)
+    B0 = [orc.view(s, 0).copy() for s in ("B_xi", "B_eta", "B_zeta")]
+    orc.step(2)
+    for a, s in enumerate(("B_xi", "B_eta", "B_zeta")):
+        assert np.abs(orc.view(s, 0) - B0[a]).max() <= 1e-15
+    Ex = orc.view("E_xi", 0)[G:-G, G:-G, G:-G, 0]
+    u = np.array((0.3, 0.2, -0.1))
+    E = -np.cross(u, np.array(Bv))
+    assert np.allclose(Ex, E[0] * h, rtol=1e-12, atol=1e-16)
+
+
+# ------------------------------------------------------------------ CFL dt
+def test_dt_closed_form_hydro():
+    n, gamma, cfl = 8, 5 / 3, 0.4
+    hc = HostCase(OTDesc(n=n, cfl=cfl), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    rho, u, p = 1.2, (0.3, -0.5, 0.2), 0.7
+    set_uniform(orc, rho, u, p, gamma)
+    refresh(orc)
+    cs = np.sqrt(gamma * p / rho)
+    h = 1.0 / n
+    expect = cfl * h / (2 * sum(abs(x) for x in u) + 6 * cs)
+    assert abs(orc.compute_dt() - expect) <= 1e-13 * expect
+
+
+def test_dt_closed_form_magnetosonic():
+    n, gamma, cfl = 8, 5 / 3, 0.4
+    hc = HostCase(OTDesc(n=n, cfl=cfl), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    rho, p, Bx = 1.1, 0.3, 1.7
+    set_uniform(orc, rho, (0, 0, 0), p, gamma)
+    h = 1.0 / n
+    orc.view("B_xi", 0)[...] = Bx * h * h
+    refresh(orc)
+    cs2, vA2 = gamma * p / rho, Bx * Bx / rho  # inver_MA2 = 1 in this unit system
+    expect = cfl * h / (2 * np.sqrt(max(cs2, vA2)) + 4 * np.sqrt(cs2 + vA2))
+    assert abs(orc.compute_dt() - expect) <= 1e-12 * expect
+
+
+# ---------------------------------------------------- geometry and weights
+def test_box_geometry_closed_form():
+    n = 8
+    hc = HostCase(OTDesc(n=n), ghost=G)
+    h = 1.0 / n
+    jac = hc.field("Jac", 0)[..., 0]
+    assert np.allclose(jac, h ** 3, rtol=1e-12)
+    for a, name in enumerate(("JDxi", "JDet", "JDze")):
+        jd = hc.field(name, 0)
+        e = np.zeros(3)
+        e[a] = h * h
+        assert np.allclose(jd, e, rtol=0, atol=1e-15)
+    for name in ("beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta", "alpha_zeta"):
+        sl = interior(name, (n, n, n), G)
+        assert np.allclose(hc.field(name, 0)[sl], 1.0 / h, rtol=1e-12)
+
+
+def test_weights_reproduce_uniform_fields_on_cubed_sphere():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    rng = np.random.default_rng(3)
+    n = hc.n
+    for b in range(6):
+        Wb = hc.field("Bcell_from_Bface_w", b)[G:G + n[0], G:G + n[1], G:G + n[2]]
+        Wj = hc.field("Jcell_from_Jedge_w", b)[G:G + n[0], G:G + n[1], G:G + n[2]]
+        x, y, z = hc.nodes(b)
+        P = np.stack([x, y, z], -1)
+        gn = G + 1
+        jd = [hc.field(s, b) for s in ("JDxi", "JDet", "JDze")]
+        bvec = rng.standard_normal(3)
+        i, j, k = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+        S = [-jd[0][i + G, j + G, k + G], jd[0][i + 1 + G, j + G, k + G], -jd[1][i + G, j + G, k + G],
+             jd[1][i + G, j + 1 + G, k + G], -jd[2][i + G, j + G, k + G], jd[2][i + G, j + G, k + 1 + G]]
+        phi = np.stack([s @ bvec for s in S], -1)
+        Bc = np.stack([(Wb[..., 6 * c:6 * c + 6] * phi).sum(-1) for c in range(3)], -1)
+        assert np.abs(Bc - bvec).max() <= 1e-11 * np.abs(bvec).max()
+
+        def node(di, dj, dk):
+            return P[i + gn + di, j + gn + dj, k + gn + dk]
+        ends = [((0, 0, 0), (1, 0, 0)), ((0, 1, 0), (1, 1, 0)), ((0, 0, 1), (1, 0, 1)), ((0, 1, 1), (1, 1, 1)),
+                ((0, 0, 0), (0, 1, 0)), ((1, 0, 0), (1, 1, 0)), ((0, 0, 1), (0, 1, 1)), ((1, 0, 1), (1, 1, 1)),
+                ((0, 0, 0), (0, 0, 1)), ((1, 0, 0), (1, 0, 1)), ((0, 1, 0), (0, 1, 1)), ((1, 1, 0), (1, 1, 1))]
+        s = np.stack([(node(*e1) - node(*e0)) @ bvec for e0, e1 in ends], -1)
+        Jc = np.stack([(Wj[..., 12 * c:12 * c + 12] * s).sum(-1) for c in range(3)], -1)
+        assert np.abs(Jc - bvec).max() <= 1e-11 * np.abs(bvec).max()
+
+
+def test_gcl_closure_on_cubed_sphere():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    n = hc.n
+    for b in range(6):
+        jx, je, jz = (hc.field(s, b) for s in ("JDxi", "JDet", "JDze"))
+        sl = lambda a, o: slice(G + o, G + n[a] + o)  # noqa: E731
+        tot = (jx[sl(0, 1), sl(1, 0), sl(2, 0)] - jx[sl(0, 0), sl(1, 0), sl(2, 0)]
+               + je[sl(0, 0), sl(1, 1), sl(2, 0)] - je[sl(0, 0), sl(1, 0), sl(2, 0)]
+               + jz[sl(0, 0), sl(1, 0), sl(2, 1)] - jz[sl(0, 0), sl(1, 0), sl(2, 0)])
+        assert np.abs(tot).max() <= 1e-14 * np.abs(jx).max()
+        assert hc.field("Jac", b)[G:-G, G:-G, G:-G].min() > 0
+
+
+# ------------------------------------------------------------------- halos
+SY_U_HALO, SY_B_HALO = 5, 7
+
+
+def cell_centres(hc, b):
+    x, y, z = hc.nodes(b)
+    P = np.stack([x, y, z], -1)
+    c = sum(P[di:P.shape[0] - 1 + di, dj:P.shape[1] - 1 + dj, dk:P.shape[2] - 1 + dk]
+            for di in (0, 1) for dj in (0, 1) for dk in (0, 1)) / 8.0
+    return c[1:-1, 1:-1, 1:-1]  # cells [-G, n+G)
+
+
+def single_hop_mask(hc, b, loc_counts):
+    """Ghost DOFs outside exactly one interface face and inside physically."""
+    f = hc.faces(b)
+    shp = tuple(c + 2 * G for c in loc_counts)
+    m = np.zeros(shp, bool)
+    idx = np.meshgrid(*[np.arange(s) - G for s in shp], indexing="ij")
+    out_if = np.zeros(shp, int)
+    out_any = np.zeros(shp, int)
+    for a in range(3):
+        lo, hi = idx[a] < 0, idx[a] >= loc_counts[a]
+        out_any += lo | hi
+        out_if += (lo & (f[2 * a][0] == 1)) | (hi & (f[2 * a + 1][0] == 1))
+    m = (out_if == 1) & (out_any == 1)
+    return m
+
+
+def test_cell_halo_ghosts_coincide_with_source_cells():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    n = hc.n
+    for b in range(6):
+        U = hc.field("U_H", b)
+        c = cell_centres(hc, b)
+        U[G:-G, G:-G, G:-G, :3] = c[G:-G, G:-G, G:-G]
+        hc.set_field("U_H", b, U)
+    hc.apply_sync(SY_U_HALO)
+    for b in range(6):
+        U = hc.field("U_H", b)
+        c = cell_centres(hc, b)
+        m = single_hop_mask(hc, b, n)
+        assert m.sum() > 0
+        assert np.abs(U[m][:, :3] - c[m]).max() <= 1e-12
+
+
+def test_face_halo_maps_fluxes_with_orientation():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    n = hc.n
+    Bv = np.array([0.3, -1.1, 0.7])
+    names = ("B_xi", "B_eta", "B_zeta")
+    jds = ("JDxi", "JDet", "JDze")
+    for b in range(6):
+        for a in range(3):
+            flux = hc.field(jds[a], b) @ Bv
+            B = np.zeros_like(hc.field(names[a], b))
+            sl = interior(names[a], n, G)
+            B[sl] = flux[sl][..., None]
+            hc.set_field(names[a], b, B)
+    hc.apply_sync(SY_B_HALO)
+    for b in range(6):
+        for a in range(3):
+            cnt = [n[q] + (1 if q == a else 0) for q in range(3)]
+            m = single_hop_mask(hc, b, cnt)
+            flux = hc.field(jds[a], b) @ Bv
+            got = hc.field(names[a], b)[..., 0]
+            assert np.abs(got[m] - flux[m]).max() <= 1e-12 * np.abs(flux).max()
+
+
+def test_oracle_generic_halo_equals_periodic_wrap():
+    hc = HostCase(OTDesc(n=12), ghost=G)
+    a = Oracle(hc, periodic_wrap=True)
+    b = Oracle(hc, periodic_wrap=False)
+    a.step(4)
+    b.step(4)
+    for name in ("U_H", "B_xi", "B_eta", "B_zeta"):
+        assert np.array_equal(a.view(name, 0), b.view(name, 0))
+
+
+# ---------------------------------------------------------- lunar boundaries
+@pytest.fixture(scope="module")
+def lunar_run():
+    hc = HostCase(WindDesc(nang=6, nrad=12), ghost=G)
+    orc = Oracle(hc)
+    orc.step(6)
+    return hc, orc
+
+
+def test_absorbing_wall_zeroes_first_layer_velocity(lunar_run):
+    hc, orc = lunar_run
+    for b in range(6):
+        U = orc.view("U_H", b)
+        wall = U[G:-G, G:-G, G]          # first interior layer at r = 1
+        second = U[G:-G, G:-G, G + 1]
+        assert np.all(wall[..., 1:4] == 0.0)
+        assert np.array_equal(wall[..., 0], np.maximum(second[..., 0], 2.5e-3))
+        assert np.all(U[G:-G, G:-G, :G, 1:4] == 0.0)
+
+
+def test_outer_boundary_flux_unchanged_by_uniform_farfield_emf(lunar_run):
+    """Far-field edges carry -(u x B).dr of a uniform field, whose circulation
+    around each outer face vanishes, so B_zeta on the outer faces stays 0."""
+    hc, orc = lunar_run
+    nk = hc.n[2]
+    for b in range(6):
+        Bz = orc.view("B_zeta", b)[G:-G, G:-G, G + nk, 0]
+        assert np.abs(Bz).max() <= 1e-14
+
+
+def test_lunar_div_b_at_roundoff(lunar_run):
+    _, orc = lunar_run
+    assert orc.divb_max() < 1e-12
+
+
+def test_dt_abort_below_floor():
+    """dt < 1e-7 aborts the step (LunarSolver_Timestep.cpp:128-139); NaN cells
+    are skipped by the reference's min, exactly as here."""
+    hc = HostCase(OTDesc(n=8), ghost=G)
+    orc = Oracle(hc, periodic_wrap=True)
+    U = orc.view("U_H", 0)
+    U[G + 3, G + 3, G + 3, 1] = 1e9 * U[G + 3, G + 3, G + 3, 0]
+    U[G + 3, G + 3, G + 3, 4] = 1e19
+    with pytest.raises(RuntimeError):
+        orc.step(1)
