@@ -44,7 +44,18 @@ def main():
     ap.add_argument("--nrad", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy five times (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 5 x 1073741824 bytes", flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the step shards by case or by block across GPUs, "
               "which is not built here); run with --gpus 1", file=sys.stderr)
