@@ -29,7 +29,8 @@ KERNEL_BYTES = {
     "k_edge_emf": 72 + 24 + 24 + 24,          # Eface, UC, B -> E
     "k_fluid": 40 + 72 + 8 + 24 + 24 + 40,    # U, JD, Jac, B_cell, J_cell -> U'
     "k_curl_update": 24 + 24 + 24,            # E, B -> B
-    "k_derived": 24 + 24 + 24 + 24 + 144 + 288 + 24 + 24,  # B, Badd, beta, alpha, Wb, Wj -> B_cell, J_cell
+    # B, Badd, beta, alpha, Wb, Wj, U, JD, Jac -> B_cell, J_cell, UC (+ next step's dt)
+    "k_derived": 24 + 24 + 24 + 24 + 144 + 288 + 40 + 72 + 8 + 24 + 24 + 24,
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -52,9 +53,9 @@ def main():
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)  # one plain elementwise read + write kernel
         torch.cuda.synchronize()
-        print("copied 5 x 1073741824 bytes", flush=True)
+        print("streamed 5 x (1073741824 read + 1073741824 written) bytes", flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the step shards by case or by block across GPUs, "

@@ -73,6 +73,7 @@ struct mhd_ctx
     MhdCaseConst *d_cst = nullptr;
     double *d_dt = nullptr;
     unsigned long long *d_dtbits = nullptr;
+    unsigned long long *d_dtscratch = nullptr; // mhd_compute_dt (leaves the step's dt untouched)
     unsigned long long *d_divb = nullptr;
     int *d_err = nullptr;
     int *d_active = nullptr;
@@ -135,7 +136,8 @@ static void free_device(mhd_ctx *c)
     if (c->gexec)
         (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
-    void *ptrs[] = {c->d_st, c->d_geo, c->d_blk, c->d_cst, c->d_dt, c->d_dtbits, c->d_divb, c->d_err, c->d_active, c->d_ops};
+    void *ptrs[] = {c->d_st,   c->d_geo, c->d_blk, c->d_cst,    c->d_dt, c->d_dtbits,
+                    c->d_divb, c->d_err, c->d_active, c->d_ops, c->d_dtscratch};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -143,7 +145,7 @@ static void free_device(mhd_ctx *c)
     c->d_blk = nullptr;
     c->d_cst = nullptr;
     c->d_dt = nullptr;
-    c->d_dtbits = c->d_divb = nullptr;
+    c->d_dtbits = c->d_divb = c->d_dtscratch = nullptr;
     c->d_err = c->d_active = nullptr;
     c->d_ops = nullptr;
 }
@@ -258,7 +260,7 @@ static void derived(mhd_ctx *c)
 {
     {
         ProfScope p(c, KP_DERIVED);
-        mhdk::launch_derived(c->shape, c->d_blk, c->d_st, c->d_geo, c->q);
+        mhdk::launch_derived(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtbits, c->ucur, c->q);
     }
     ProfScope p(c, KP_OPS);
     run_ops(c, SY_JCELL_BC, 0);
@@ -269,17 +271,11 @@ static void enqueue_step(mhd_ctx *c)
 {
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
     const int ncase = (int)c->cases.size();
-    {
-        ProfScope p(c, KP_DT);
-        mhdk::launch_dt(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtbits, c->ucur, c->q);
-    }
+    // dt and the contravariant velocities of this step were computed by the
+    // previous step's derived-field kernel (or by mhd_finalize)
     {
         ProfScope p(c, KP_DTFIN);
         mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, ncase, c->d_active, c->q);
-    }
-    {
-        ProfScope p(c, KP_UC);
-        mhdk::launch_ucontra(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->ucur, c->q);
     }
     {
         ProfScope p(c, KP_OPS);
@@ -408,6 +404,7 @@ extern "C" int mhd_finalize(mhd_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_dt, sizeof(double) * ncase));
     HIPCHK(c, hipMalloc(&c->d_dtbits, sizeof(unsigned long long) * ncase));
     HIPCHK(c, hipMalloc(&c->d_divb, sizeof(unsigned long long) * ncase));
+    HIPCHK(c, hipMalloc(&c->d_dtscratch, sizeof(unsigned long long) * ncase));
     HIPCHK(c, hipMalloc(&c->d_err, sizeof(int)));
     HIPCHK(c, hipMemset(c->d_err, 0, sizeof(int)));
     HIPCHK(c, hipMemset(c->d_dt, 0, sizeof(double) * ncase));
@@ -532,16 +529,14 @@ extern "C" int mhd_compute_dt(mhd_ctx *c, int case_id, double *dt)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "mhd_finalize first");
-    mhdk::launch_dt(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtbits, c->ucur, c->q);
-    unsigned long long bits = 0;
-    HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtbits + case_id, 8, hipMemcpyDeviceToHost, c->q));
+    const double big = 1e100;
+    unsigned long long bits;
+    std::memcpy(&bits, &big, 8);
+    mhdk::launch_fill_u64(c->d_dtscratch, (int)c->cases.size(), bits, c->q);
+    mhdk::launch_dt(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtscratch, c->ucur, c->q);
+    HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtscratch + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
     std::memcpy(dt, &bits, 8);
-    const double big = 1e100;
-    std::vector<unsigned long long> init(c->cases.size());
-    std::memcpy(&bits, &big, 8);
-    std::fill(init.begin(), init.end(), bits);
-    HIPCHK(c, hipMemcpy(c->d_dtbits, init.data(), 8 * init.size(), hipMemcpyHostToDevice));
     return MHD_OK;
 }
 

@@ -25,7 +25,8 @@ void launch_edge_emf(const LaunchShape &s, const MhdBlockDev *blks, double *st, 
 void launch_curl_update(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *dts, hipStream_t q);
 void launch_fluid(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
                   const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon, hipStream_t q);
-void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo, hipStream_t q);
+void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
+                    const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q);
 void launch_ops(const mpcns::MhdOp *ops, int n, double *st, const MhdCaseConst *cst, int ushift, hipStream_t q);
 void launch_divb(const LaunchShape &s, const MhdBlockDev *blks, const double *st, const double *geo,
                  unsigned long long *out, hipStream_t q);

@@ -111,9 +111,87 @@ __device__ __forceinline__ double minmod(double a, double b)
 } // namespace
 
 // ======================================================================
-// dt: Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140)
-// per-cell CFL limit, wavefront min, one 64-bit atomicMin per workgroup on
-// the bit pattern (positive doubles order like their bits).
+// dt: Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140).
+// Per-cell CFL limit; the caller reduces with a wavefront min and one 64-bit
+// atomicMin per workgroup on the bit pattern (positive doubles order like
+// their bits).  Cells with V <= 0 or a non-positive / NaN denominator are
+// skipped, as the reference does.
+__device__ __forceinline__ double cell_dt(const MhdBlockDev &b, const MhdCaseConst &c, const double *__restrict__ U,
+                                          const double *__restrict__ G, int64_t o, double Bx, double By, double Bz)
+{
+    const int64_t sl = b.slot;
+    double local = 1e100;
+    const double V = fabs(G[(int64_t)G_JAC * sl + o]);
+    if (V > 0.0)
+    {
+        const double gm = c.gamma;
+        const double rho = dmax(U[o], 1e-2);
+        const double ux = U[sl + o] / rho, uy = U[2 * sl + o] / rho, uz = U[3 * sl + o] / rho;
+        const double E = U[4 * sl + o];
+        const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+        const double eint = E - ke;
+        const double p = (gm - 1.0) * eint;
+        const double cs2 = gm * p / rho;
+        const double B2 = Bx * Bx + By * By + Bz * Bz;
+        const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk;
+        double denom = 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+        {
+            const int ax = f >> 1;
+            const int64_t oo = o + ((f & 1) ? (ax == 0 ? di : (ax == 1 ? dj : 1)) : 0);
+            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+            const double axv = A[oo], ayv = A[sl + oo], azv = A[2 * sl + oo];
+            const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
+            double term = 0.0;
+            if (An > 0.0)
+            {
+                double nx = axv / An, ny = ayv / An, nz = azv / An;
+                if (!(f & 1))
+                {
+                    nx = -nx;
+                    ny = -ny;
+                    nz = -nz;
+                }
+                const double un = ux * nx + uy * ny + uz * nz;
+                const double Bn = Bx * nx + By * ny + Bz * nz;
+                const double vA2 = c.inver_MA2 * B2 / rho;
+                const double vAn2 = c.inver_MA2 * (Bn * Bn) / rho;
+                const double tt = cs2 + vA2;
+                double disc = tt * tt - 4.0 * cs2 * vAn2;
+                if (disc < 0.0)
+                    disc = 0.0;
+                const double cf = sqrt(0.5 * (tt + sqrt(disc)));
+                term = (fabs(un) + cf) * An;
+            }
+            denom += term;
+        }
+        if (denom > 0.0)
+        {
+            const double dtc = c.cfl * V / denom;
+            local = dmin(local, dtc);
+        }
+    }
+    return local;
+}
+
+__device__ __forceinline__ void reduce_dt(double local, unsigned long long *__restrict__ dtbits, int case_id)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1)
+        local = dmin(local, __shfl_xor(local, s, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0)
+        red[threadIdx.y] = local;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+    {
+        const double m = dmin(dmin(red[0], red[1]), dmin(red[2], red[3]));
+        if (m < 1e100)
+            atomicMin(&dtbits[case_id], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_dt(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
                                             const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
                                             unsigned long long *__restrict__ dtbits, int ucur)
@@ -124,78 +202,12 @@ __global__ __launch_bounds__(256) void k_dt(const MhdBlockDev *__restrict__ blks
     double local = 1e100;
     if (ok)
     {
-        const MhdCaseConst &c = cst[b.case_id];
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
         const double *U = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
         const double *Bc = st + b.st + (int64_t)ST_BCELL * sl;
-        const double *G = geo + b.geo;
-        const double V = fabs(G[(int64_t)G_JAC * sl + o]);
-        if (V > 0.0)
-        {
-            const double gm = c.gamma;
-            const double rho = dmax(U[o], 1e-2);
-            const double ux = U[sl + o] / rho, uy = U[2 * sl + o] / rho, uz = U[3 * sl + o] / rho;
-            const double E = U[4 * sl + o];
-            const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
-            const double eint = E - ke;
-            const double p = (gm - 1.0) * eint;
-            const double cs2 = gm * p / rho;
-            const double Bx = Bc[o], By = Bc[sl + o], Bz = Bc[2 * sl + o];
-            const double B2 = Bx * Bx + By * By + Bz * Bz;
-            const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk;
-            double denom = 0.0;
-#pragma unroll
-            for (int f = 0; f < 6; ++f)
-            {
-                const int ax = f >> 1;
-                const int64_t oo = o + ((f & 1) ? (ax == 0 ? di : (ax == 1 ? dj : 1)) : 0);
-                const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
-                const double axv = A[oo], ayv = A[sl + oo], azv = A[2 * sl + oo];
-                const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
-                double term = 0.0;
-                if (An > 0.0)
-                {
-                    double nx = axv / An, ny = ayv / An, nz = azv / An;
-                    if (!(f & 1))
-                    {
-                        nx = -nx;
-                        ny = -ny;
-                        nz = -nz;
-                    }
-                    const double un = ux * nx + uy * ny + uz * nz;
-                    const double Bn = Bx * nx + By * ny + Bz * nz;
-                    const double vA2 = c.inver_MA2 * B2 / rho;
-                    const double vAn2 = c.inver_MA2 * (Bn * Bn) / rho;
-                    const double tt = cs2 + vA2;
-                    double disc = tt * tt - 4.0 * cs2 * vAn2;
-                    if (disc < 0.0)
-                        disc = 0.0;
-                    const double cf = sqrt(0.5 * (tt + sqrt(disc)));
-                    term = (fabs(un) + cf) * An;
-                }
-                denom += term;
-            }
-            if (denom > 0.0)
-            {
-                const double dtc = c.cfl * V / denom;
-                local = dmin(local, dtc);
-            }
-        }
+        local = cell_dt(b, cst[b.case_id], U, geo + b.geo, o, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
     }
-    // wavefront min then workgroup min
-#pragma unroll
-    for (int s = 32; s > 0; s >>= 1)
-        local = dmin(local, __shfl_xor(local, s, 64));
-    __shared__ double red[4];
-    if (threadIdx.x == 0)
-        red[threadIdx.y] = local;
-    __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-    {
-        double m = dmin(dmin(red[0], red[1]), dmin(red[2], red[3]));
-        if (m < 1e100)
-            atomicMin(&dtbits[blks[blockIdx.z].case_id], (unsigned long long)__double_as_longlong(m));
-    }
+    reduce_dt(local, dtbits, blks[blockIdx.z].case_id);
 }
 
 // finalize: bits -> dt, abort check, reset for the next reduction
@@ -219,23 +231,15 @@ __global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, 
 // contravariant cell velocity per axis, cells [-1, N] (step scratch):
 // cell_u_contra (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:414-428)
 // with U_plus from calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141).
-__global__ __launch_bounds__(256) void k_ucontra(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
-                                                 int ucur)
+__device__ __forceinline__ void cell_ucontra(const MhdBlockDev &b, double gm, const double *__restrict__ U,
+                                             const double *__restrict__ G, double *__restrict__ UC, int64_t o)
 {
-    MhdBlockDev b;
-    int i, j, k;
-    if (!thread_cell(blks, -1, 1, b, i, j, k))
-        return;
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const double *U = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
-    const double *G = geo + b.geo;
-    const Prim q = prim_of(U, sl, o, cst[b.case_id].gamma);
+    const int64_t sl = b.slot;
+    const Prim q = prim_of(U, sl, o, gm);
     const bool has = q.rho_raw > 1e-20;
     const double up0 = has ? q.u : 0.0, up1 = has ? q.v : 0.0, up2 = has ? q.w : 0.0;
     const double inv_jac = 1.0 / (fabs(G[(int64_t)G_JAC * sl + o]) + 1.0e-30);
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-    double *UC = st + b.st + (int64_t)ST_UC * sl;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
     {
@@ -246,6 +250,19 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdBlockDev *__restrict__
         const double kz = 0.5 * (A[2 * sl + o] + A[2 * sl + op]);
         UC[ax * sl + o] = (kx * up0 + ky * up1 + kz * up2) * inv_jac;
     }
+}
+
+__global__ __launch_bounds__(256) void k_ucontra(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
+                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+                                                 int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    if (!thread_cell(blks, -1, 1, b, i, j, k))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    cell_ucontra(b, cst[b.case_id].gamma, st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl, geo + b.geo,
+                 st + b.st + (int64_t)ST_UC * sl, o);
 }
 
 // ======================================================================
@@ -534,14 +551,24 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdBlockDev *__restrict__ b
 // calc_Jcell (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:163-236,777-891;
 // CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).
 // J edges are recomputed per cell from beta-weighted face fluxes.
+// Fused with the next step's CFL limit (dt depends on U after the Ucell
+// sync and on the B_cell computed here, nothing changes them before the next
+// step) and with the contravariant velocities of cells [-1, N].
 __global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                                 const double *__restrict__ geo)
+                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+                                                 unsigned long long *__restrict__ dtbits, int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, 0, 0, b, i, j, k))
-        return;
+    const bool ok = thread_cell(blks, -1, 1, b, i, j, k);
+    double local = 1e100;
+    if (ok)
+    {
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const double *Ucur = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
+    cell_ucontra(b, cst[b.case_id].gamma, Ucur, geo + b.geo, st + b.st + (int64_t)ST_UC * sl, o);
+    if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
+    {
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = st + b.st;
     const double *G = geo + b.geo;
@@ -563,6 +590,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__
         Bc[o] = xt;
         Bc[sl + o] = yt;
         Bc[2 * sl + o] = zt;
+        local = cell_dt(b, cst[b.case_id], Ucur, G, o, xt, yt, zt);
     }
     {
         const double *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
@@ -595,6 +623,9 @@ __global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__
         Jc[sl + o] = y;
         Jc[2 * sl + o] = z;
     }
+    }
+    }
+    reduce_dt(local, dtbits, blks[blockIdx.z].case_id);
 }
 
 // ======================================================================
@@ -754,9 +785,10 @@ void launch_fluid(const LaunchShape &s, const MhdBlockDev *blks, double *st, con
     else
         hipLaunchKernelGGL((k_fluid<double, 0>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
 }
-void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo, hipStream_t q)
+void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
+                    const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_derived, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo);
+    hipLaunchKernelGGL(k_derived, grid_for(s, -1, 1), kBlock, 0, q, blks, st, geo, cst, dtbits, ucur);
 }
 void launch_ops(const MhdOp *ops, int n, double *st, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {

@@ -111,3 +111,18 @@ def test_mixed_precision_close_to_fp64():
             a, b = m.field(cm, name, blk)[sl], f.field(cf, name, blk)[sl]
             assert rel_linf(a, b) <= 1e-5, name
     assert m.divb_max(cm) < 1e-9
+
+
+def test_compute_dt_matches_oracle_and_leaves_step_untouched():
+    desc = WindDesc(nang=8, nrad=16)
+    hc = HostCase(desc)
+    orc = Oracle(hc)
+    ctx = MhdContext()
+    cid = ctx.add(desc)
+    ctx.finalize()
+    assert ctx.compute_dt(cid) == orc.compute_dt()
+    ctx.step(3)
+    orc.step(3)
+    assert ctx.compute_dt(cid) == orc.compute_dt()
+    ctx.step(1)
+    assert ctx.last_dt(cid) == orc.step(1)[0]
