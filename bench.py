@@ -35,6 +35,22 @@ KERNEL_BYTES = {
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (profiles/traffic_<workload>.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 5 and cols[0].endswith(kernel) and cols[0].split()[-1].split("<")[0] == kernel:
+            try:
+                return float(cols[4]) * 1e9
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +151,8 @@ def main():
         "config": {"workload": f"lunar_wake_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}",
                    "cells": cells, "blocks": 6, "recon": "first-order", "cfl": desc.cfl},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(f"lunar_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}", dom),
                      "alg_bytes_per_cell": KERNEL_BYTES[dom], "avg_ms": round(avg_ms, 4)},
         "cpu_baseline": cpu,
     }
