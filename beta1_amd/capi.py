@@ -11,7 +11,7 @@ class MhdError(RuntimeError):
 
 class MhdOptions(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("recon", C.c_int), ("ghost", C.c_int),
-                ("hbm_budget_gb", C.c_double)]
+                ("hbm_budget_gb", C.c_double), ("autotune", C.c_int), ("xcd_order", C.c_int)]
 
 
 class MhdOTDesc(C.Structure):
@@ -56,6 +56,9 @@ SIGNATURES = {
     "mhd_profile": (_I, [_P, _I]),
     "mhd_profile_read": (_I, [_P, _I, _PD, _PI, C.POINTER(C.c_char_p)]),
     "mhd_memory": (_I, [_P, _PD]),
+    "mhd_retire": (_I, [_P, _I]),
+    "mhd_case_steps": (_I, [_P, _I, C.POINTER(C.c_longlong)]),
+    "mhd_tuning": (_I, [_P, _I, _PI]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_destroy": (_I, [_P]),

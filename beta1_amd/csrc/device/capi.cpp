@@ -1,5 +1,6 @@
 // C-ABI of the explicit MHD step (include/beta1_mhd.h): context, case
-// placement in device arenas, the step sequence and data exchange.
+// placement on the device, live admission / retirement, launch tiling and
+// tuning, the step sequence and data exchange.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/beta1_mhd.h"
@@ -22,20 +24,43 @@ namespace
 {
 struct CaseRec
 {
-    CaseSetup cs;         // grid, constants, op lists (arrays freed after upload)
-    int geo_set = -1;     // geometry set index
-    int64_t st_base = 0;  // arena offset of block 0 state
-    int first_block = 0;  // index into the global block table
+    CaseSetup cs;                     // grid, topology, constants; arrays freed once placed
+    std::vector<MhdOp> ops[SY_COUNT]; // case-relative element offsets
+    int geo_set = -1;
+    double *d_st = nullptr; // the case's state allocation
+    int64_t slot = 0;
     int nblocks = 0;
-    int active = 1;
+    int state = 0;        // 0 pending, 1 active, 2 retired
+    int first_block = -1; // index into the block table while active
+    long long steps = 0;
+    size_t st_bytes = 0;
 };
 
 struct GeoSet
 {
     std::string key;
-    int64_t base = 0;
+    double *d = nullptr;
     int64_t block_stride = 0;
+    int refs = 0;
+    size_t bytes = 0;
 };
+
+struct TileArr
+{
+    int *d_tile = nullptr, *d_start = nullptr;
+    size_t cap_tile = 0, cap_start = 0;
+    int n = 0;
+};
+
+enum Range
+{
+    R_CELL = 0, // cells [0, N)
+    R_FACE,     // [0, N]   (faces / edges incl. both boundary layers)
+    R_EXT,      // [-1, N]  (cells incl. one ghost ring)
+    R_COUNT
+};
+const int kRangeLo[R_COUNT] = {0, 0, -1}, kRangeHi[R_COUNT] = {0, 1, 1};
+const int kJt[3] = {1, 2, 4};
 
 enum KernelId
 {
@@ -50,8 +75,9 @@ enum KernelId
     KP_OPS,
     KP_COUNT
 };
-const char *kKernelName[KP_COUNT] = {"k_dt", "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
+const char *kKernelName[KP_COUNT] = {"k_dt",    "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid", "k_curl_update", "k_derived", "k_ops"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL};
 } // namespace
 
 struct mhd_ctx
@@ -63,30 +89,31 @@ struct mhd_ctx
     std::vector<GeoSet> geos;
     bool finalized = false;
     int ucur = 0;
+    size_t bytes_used = 0;
 
-    double *d_st = nullptr;
-    int64_t st_n = 0;
-    double *d_geo = nullptr;
-    int64_t geo_n = 0;
-    MhdBlockDev *d_blk = nullptr;
+    // block table and launch tiling (rebuilt whenever the active set changes)
     std::vector<MhdBlockDev> h_blk;
+    MhdBlockDev *d_blk = nullptr;
+    size_t blk_cap = 0;
+    TileArr tiles[R_COUNT][3];
+    int jt_idx[KP_COUNT];
+
+    // case-indexed device arrays
+    int case_cap = 0;
     MhdCaseConst *d_cst = nullptr;
-    double *d_dt = nullptr;
-    unsigned long long *d_dtbits = nullptr;
-    unsigned long long *d_dtscratch = nullptr; // mhd_compute_dt (leaves the step's dt untouched)
-    unsigned long long *d_divb = nullptr;
-    int *d_err = nullptr;
-    int *d_active = nullptr;
+    double *d_dt = nullptr, *d_zero = nullptr;
+    unsigned long long *d_dtbits = nullptr, *d_dtscratch = nullptr, *d_divb = nullptr;
+    int *d_err = nullptr, *d_active = nullptr;
+
+    // concatenated op lists of the active cases
     MhdOp *d_ops = nullptr;
+    size_t ops_cap = 0;
     int op_start[SY_COUNT] = {0};
     int op_count[SY_COUNT] = {0};
-    mhdk::LaunchShape shape{};
 
     hipGraphExec_t gexec = nullptr;
-    int graph_ucur = -1;
 
     bool prof = false;
-    hipEvent_t ev[2 * KP_COUNT * 4] = {};
     double prof_ms[KP_COUNT] = {0};
     int prof_n[KP_COUNT] = {0};
 };
@@ -109,11 +136,26 @@ static int fail(mhd_ctx *c, int code, const std::string &msg)
     return code;
 }
 
+static unsigned long long dbits(double v)
+{
+    unsigned long long b;
+    std::memcpy(&b, &v, 8);
+    return b;
+}
+
+template <class T> static void free_t(T *&p)
+{
+    if (p)
+        (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// ------------------------------------------------------------ life cycle
 extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
 {
     if (!opt || !out)
         return MHD_ERR_ARG;
-    if (opt->ghost < 1 || (opt->recon == 1 && opt->ghost < 2))
+    if (opt->ghost < 1 || (opt->recon == 1 && opt->ghost < 2) || opt->precision < 0 || opt->precision > 1)
         return MHD_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt->device)
@@ -122,6 +164,8 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
         return MHD_ERR_HIP;
     mhd_ctx *c = new mhd_ctx();
     c->opt = *opt;
+    for (int k = 0; k < KP_COUNT; ++k)
+        c->jt_idx[k] = 2; // 64 x 4 until tuned
     if (hipStreamCreateWithFlags(&c->q, hipStreamNonBlocking) != hipSuccess)
     {
         delete c;
@@ -131,35 +175,34 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
     return MHD_OK;
 }
 
-static void free_device(mhd_ctx *c)
-{
-    if (c->gexec)
-        (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-    void *ptrs[] = {c->d_st,   c->d_geo, c->d_blk, c->d_cst,    c->d_dt, c->d_dtbits,
-                    c->d_divb, c->d_err, c->d_active, c->d_ops, c->d_dtscratch};
-    for (void *p : ptrs)
-        if (p)
-            (void)hipFree(p);
-    c->d_st = c->d_geo = nullptr;
-    c->d_blk = nullptr;
-    c->d_cst = nullptr;
-    c->d_dt = nullptr;
-    c->d_dtbits = c->d_divb = c->d_dtscratch = nullptr;
-    c->d_err = c->d_active = nullptr;
-    c->d_ops = nullptr;
-}
-
 extern "C" int mhd_destroy(mhd_ctx *c)
 {
     if (!c)
         return MHD_OK;
     (void)hipSetDevice(c->opt.device);
     (void)hipStreamSynchronize(c->q);
-    free_device(c);
-    for (auto &e : c->ev)
-        if (e)
-            (void)hipEventDestroy(e);
+    if (c->gexec)
+        (void)hipGraphExecDestroy(c->gexec);
+    for (auto &r : c->cases)
+        free_t(r->d_st);
+    for (auto &g : c->geos)
+        free_t(g.d);
+    for (auto &rr : c->tiles)
+        for (auto &t : rr)
+        {
+            free_t(t.d_tile);
+            free_t(t.d_start);
+        }
+    free_t(c->d_blk);
+    free_t(c->d_cst);
+    free_t(c->d_dt);
+    free_t(c->d_zero);
+    free_t(c->d_dtbits);
+    free_t(c->d_dtscratch);
+    free_t(c->d_divb);
+    free_t(c->d_err);
+    free_t(c->d_active);
+    free_t(c->d_ops);
     (void)hipStreamDestroy(c->q);
     delete c;
     return MHD_OK;
@@ -167,16 +210,411 @@ extern "C" int mhd_destroy(mhd_ctx *c)
 
 extern "C" const char *mhd_last_error(mhd_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+// ------------------------------------------------------- device tables
+template <class T> static int grow(mhd_ctx *c, T *&p, size_t &cap, size_t need)
+{
+    if (need <= cap && p)
+        return MHD_OK;
+    const size_t ncap = std::max<size_t>(std::max<size_t>(need, 2 * cap), 1);
+    free_t(p);
+    HIPCHK(c, hipMalloc((void **)&p, sizeof(T) * ncap));
+    cap = ncap;
+    return MHD_OK;
+}
+
+template <class T> static int regrow(mhd_ctx *c, T *&p, int old, int cap, bool keep)
+{
+    T *np = nullptr;
+    HIPCHK(c, hipMalloc((void **)&np, sizeof(T) * cap));
+    HIPCHK(c, hipMemset(np, 0, sizeof(T) * cap));
+    if (keep && p && old > 0)
+        HIPCHK(c, hipMemcpy(np, p, sizeof(T) * old, hipMemcpyDeviceToDevice));
+    free_t(p);
+    p = np;
+    return MHD_OK;
+}
+
+static int ensure_case_capacity(mhd_ctx *c)
+{
+    const int need = (int)c->cases.size();
+    if (need <= c->case_cap && c->case_cap > 0)
+        return MHD_OK;
+    const int old = c->case_cap;
+    const int cap = std::max(8, std::max(need, 2 * old));
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    int rc;
+    if ((rc = regrow(c, c->d_cst, old, cap, true)) || (rc = regrow(c, c->d_dt, old, cap, true)) ||
+        (rc = regrow(c, c->d_zero, old, cap, false)) || (rc = regrow(c, c->d_dtbits, old, cap, true)) ||
+        (rc = regrow(c, c->d_dtscratch, old, cap, false)) || (rc = regrow(c, c->d_divb, old, cap, false)) ||
+        (rc = regrow(c, c->d_err, old, cap, true)) || (rc = regrow(c, c->d_active, old, cap, true)))
+        return rc;
+    std::vector<unsigned long long> big(cap - old, dbits(1e100));
+    HIPCHK(c, hipMemcpy(c->d_dtbits + old, big.data(), 8 * (size_t)(cap - old), hipMemcpyHostToDevice));
+    c->case_cap = cap;
+    return MHD_OK;
+}
+
+static int build_tiles(mhd_ctx *c)
+{
+    const int nblk = (int)c->h_blk.size();
+    for (int r = 0; r < R_COUNT; ++r)
+        for (int ji = 0; ji < 3; ++ji)
+        {
+            TileArr &T = c->tiles[r][ji];
+            const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
+            std::vector<int> start(std::max(nblk, 1), 0), tb;
+            for (int b = 0; b < nblk; ++b)
+            {
+                const MhdBlockDev &d = c->h_blk[b];
+                const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
+                const long long cnt = (long long)ei * ((ej + jt - 1) / jt) * ((ek + 63) / 64);
+                start[b] = (int)tb.size();
+                tb.insert(tb.end(), (size_t)cnt, b);
+            }
+            T.n = (int)tb.size();
+            int rc;
+            if ((rc = grow(c, T.d_tile, T.cap_tile, std::max<size_t>(tb.size(), 1))) ||
+                (rc = grow(c, T.d_start, T.cap_start, start.size())))
+                return rc;
+            if (!tb.empty())
+                HIPCHK(c, hipMemcpy(T.d_tile, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(T.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+        }
+    return MHD_OK;
+}
+
+static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
+{
+    const int r = kKernelRange[kernel];
+    const int ji = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
+    const TileArr &T = c->tiles[r][ji];
+    mhdk::Launch L;
+    L.tm.blks = c->d_blk;
+    L.tm.tile_block = T.d_tile;
+    L.tm.block_start = T.d_start;
+    L.tm.ntiles = T.n;
+    L.tm.lo = kRangeLo[r];
+    L.tm.hi = kRangeHi[r];
+    L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
+    L.jt = kJt[ji];
+    return L;
+}
+
+// Rebuild the block table, tiles, op lists and per-case constants for the
+// current set of active cases.
+static int rebuild(mhd_ctx *c)
+{
+    int rc = ensure_case_capacity(c);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    if (c->gexec)
+    {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->h_blk.clear();
+    std::vector<MhdCaseConst> cst(c->case_cap);
+    std::vector<int> act(c->case_cap, 0);
+    for (int ci = 0; ci < (int)c->cases.size(); ++ci)
+    {
+        CaseRec &r = *c->cases[ci];
+        cst[ci] = r.cs.cst;
+        if (r.state != 1)
+        {
+            r.first_block = -1;
+            continue;
+        }
+        act[ci] = 1;
+        r.first_block = (int)c->h_blk.size();
+        const GeoSet &G = c->geos[r.geo_set];
+        for (int b = 0; b < r.nblocks; ++b)
+        {
+            MhdBlockDev d = r.cs.blk[b].d;
+            d.case_id = ci;
+            d.st = (int64_t)(uintptr_t)(r.d_st + (int64_t)b * ST_NSLOT * r.slot);
+            d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_stride);
+            c->h_blk.push_back(d);
+        }
+    }
+    if ((rc = grow(c, c->d_blk, c->blk_cap, std::max<size_t>(c->h_blk.size(), 1))))
+        return rc;
+    if (!c->h_blk.empty())
+        HIPCHK(c, hipMemcpy(c->d_blk, c->h_blk.data(), sizeof(MhdBlockDev) * c->h_blk.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_cst, cst.data(), sizeof(MhdCaseConst) * c->case_cap, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_active, act.data(), sizeof(int) * c->case_cap, hipMemcpyHostToDevice));
+    if ((rc = build_tiles(c)))
+        return rc;
+    // op lists relocated to absolute addresses
+    std::vector<MhdOp> all;
+    for (int sy = 0; sy < SY_COUNT; ++sy)
+    {
+        c->op_start[sy] = (int)all.size();
+        for (int ci = 0; ci < (int)c->cases.size(); ++ci)
+        {
+            const CaseRec &r = *c->cases[ci];
+            if (r.state != 1)
+                continue;
+            for (MhdOp o : r.ops[sy])
+            {
+                o.dst = (int64_t)(uintptr_t)(r.d_st + o.dst);
+                if (o.kind == OP_COPY || o.kind == OP_NEG || o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
+                    o.src = (int64_t)(uintptr_t)(r.d_st + o.src);
+                else
+                    o.src = o.dst;
+                o.cas = ci;
+                all.push_back(o);
+            }
+        }
+        c->op_count[sy] = (int)all.size() - c->op_start[sy];
+    }
+    if ((rc = grow(c, c->d_ops, c->ops_cap, std::max<size_t>(all.size(), 1))))
+        return rc;
+    if (!all.empty())
+        HIPCHK(c, hipMemcpy(c->d_ops, all.data(), sizeof(MhdOp) * all.size(), hipMemcpyHostToDevice));
+    return MHD_OK;
+}
+
+// ------------------------------------------------------------- placement
+static int place_case(mhd_ctx *c, CaseRec &r)
+{
+    const int64_t slot = r.cs.blk[0].d.slot;
+    r.slot = slot;
+    int gi = -1;
+    for (int g = 0; g < (int)c->geos.size(); ++g)
+        if (c->geos[g].refs > 0 && c->geos[g].key == r.cs.shape_key)
+            gi = g;
+    const size_t st_bytes = sizeof(double) * (size_t)ST_NSLOT * slot * r.nblocks;
+    const size_t geo_bytes = gi < 0 ? sizeof(double) * (size_t)G_NSLOT * slot * r.nblocks : 0;
+    const double budget = c->opt.hbm_budget_gb * 1e9;
+    if (budget > 0 && (double)(c->bytes_used + st_bytes + geo_bytes) > budget)
+        return fail(c, MHD_ERR_MEM, "HBM budget exceeded");
+    if (gi < 0)
+    {
+        GeoSet G;
+        G.key = r.cs.shape_key;
+        G.block_stride = (int64_t)G_NSLOT * slot;
+        G.bytes = geo_bytes;
+        HIPCHK(c, hipMalloc((void **)&G.d, geo_bytes));
+        for (int b = 0; b < r.nblocks; ++b)
+            HIPCHK(c, hipMemcpy(G.d + (int64_t)b * G.block_stride, r.cs.blk[b].geo.data(),
+                                sizeof(double) * G.block_stride, hipMemcpyHostToDevice));
+        c->bytes_used += geo_bytes;
+        gi = 0;
+        while (gi < (int)c->geos.size() && c->geos[gi].refs > 0)
+            ++gi;
+        if (gi == (int)c->geos.size())
+            c->geos.push_back(G);
+        else
+            c->geos[gi] = G;
+    }
+    c->geos[gi].refs += 1;
+    r.geo_set = gi;
+    HIPCHK(c, hipMalloc((void **)&r.d_st, st_bytes));
+    for (int b = 0; b < r.nblocks; ++b)
+    {
+        double *dst = r.d_st + (int64_t)b * ST_NSLOT * slot;
+        HIPCHK(c, hipMemcpy(dst, r.cs.blk[b].st.data(), sizeof(double) * ST_NSLOT * slot, hipMemcpyHostToDevice));
+        if (c->ucur == 1) // admitted mid-run: the current U buffer is the second one
+            HIPCHK(c, hipMemcpy(dst + ST_U1 * slot, dst + ST_U0 * slot, sizeof(double) * 5 * slot,
+                                hipMemcpyDeviceToDevice));
+        std::vector<double>().swap(r.cs.blk[b].geo);
+        std::vector<double>().swap(r.cs.blk[b].st);
+    }
+    r.st_bytes = st_bytes;
+    c->bytes_used += st_bytes;
+    for (int sy = 0; sy < SY_COUNT && sy < (int)r.cs.ops.size(); ++sy)
+        r.ops[sy] = std::move(r.cs.ops[sy]);
+    r.cs.ops.clear();
+    r.state = 1;
+    return MHD_OK;
+}
+
+struct ProfScope
+{
+    mhd_ctx *c;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mhd_ctx *c_, int id_) : c(c_), id(id_)
+    {
+        if (c->prof)
+        {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->q);
+        }
+    }
+    ~ProfScope()
+    {
+        if (c->prof)
+        {
+            (void)hipEventRecord(b, c->q);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            c->prof_ms[id] += ms;
+            c->prof_n[id] += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+
+static void run_ops(mhd_ctx *c, int sy, int ushift)
+{
+    mhdk::launch_ops(c->d_ops + c->op_start[sy], c->op_count[sy], c->d_cst, ushift, c->q);
+}
+
+static void derived(mhd_ctx *c)
+{
+    {
+        ProfScope p(c, KP_DERIVED);
+        mhdk::launch_derived(launch_of(c, KP_DERIVED), c->d_cst, c->d_dtbits, c->ucur, c->q);
+    }
+    ProfScope p(c, KP_OPS);
+    run_ops(c, SY_JCELL_BC, 0);
+}
+
+// One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
+// case.  dt and the contravariant velocities were computed by the previous
+// step's derived-field kernel (or at placement).
+static void enqueue_step(mhd_ctx *c)
+{
+    const int mixed = c->opt.precision == MHD_PREC_MIXED;
+    {
+        ProfScope p(c, KP_DTFIN);
+        mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, c->case_cap, c->d_active, c->q);
+    }
+    {
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_UC_CORNER, 0);
+    }
+    {
+        ProfScope p(c, KP_FACE);
+        mhdk::launch_face_emf(launch_of(c, KP_FACE), mixed, c->q);
+    }
+    {
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_EFACE_BC, 0);
+        run_ops(c, SY_EFACE_HALO, 0);
+    }
+    {
+        ProfScope p(c, KP_EDGE);
+        mhdk::launch_edge_emf(launch_of(c, KP_EDGE), c->q);
+    }
+    {
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_EEDGE_BC, 0);
+    }
+    {
+        ProfScope p(c, KP_FLUID);
+        mhdk::launch_fluid(launch_of(c, KP_FLUID), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
+    }
+    {
+        ProfScope p(c, KP_CURL);
+        mhdk::launch_curl_update(launch_of(c, KP_CURL), c->d_dt, c->q);
+    }
+    c->ucur ^= 1;
+    {
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_U_BC0, c->ucur);
+        run_ops(c, SY_U_BC1, c->ucur);
+        run_ops(c, SY_U_HALO, c->ucur);
+        run_ops(c, SY_B_BC, 0);
+        run_ops(c, SY_B_HALO, 0);
+    }
+    derived(c);
+}
+
+// Autotuner: per kernel, time each workgroup height (64x1, 64x2, 64x4) on the
+// placed cases and keep the fastest.  Extra launches must not change the
+// state: face/edge EMF and the fluid kernel write scratch the next step
+// overwrites, the curl update runs with dt = 0, the derived kernel recomputes
+// the same values.
+static int autotune(mhd_ctx *c)
+{
+    if (c->h_blk.empty())
+        return MHD_OK;
+    const int mixed = c->opt.precision == MHD_PREC_MIXED;
+    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int k : kernels)
+    {
+        double best = 1e30;
+        int best_j = c->jt_idx[k];
+        for (int ji = 0; ji < 3; ++ji)
+        {
+            auto launch = [&]() {
+                const mhdk::Launch L = launch_of(c, k, ji);
+                switch (k)
+                {
+                case KP_FACE:
+                    mhdk::launch_face_emf(L, mixed, c->q);
+                    break;
+                case KP_EDGE:
+                    mhdk::launch_edge_emf(L, c->q);
+                    break;
+                case KP_FLUID:
+                    mhdk::launch_fluid(L, c->d_cst, c->d_zero, c->ucur, mixed, c->opt.recon, c->q);
+                    break;
+                case KP_CURL:
+                    mhdk::launch_curl_update(L, c->d_zero, c->q);
+                    break;
+                default:
+                    mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, c->q);
+                    break;
+                }
+            };
+            launch(); // warm
+            HIPCHK(c, hipEventRecord(e0, c->q));
+            for (int rep = 0; rep < 3; ++rep)
+                launch();
+            HIPCHK(c, hipEventRecord(e1, c->q));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best)
+            {
+                best = ms;
+                best_j = ji;
+            }
+        }
+        c->jt_idx[k] = best_j;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return MHD_OK;
+}
+
 static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
 {
-    if (c->finalized)
-        return fail(c, MHD_ERR_STATE, "cases must be added before mhd_finalize");
     auto r = std::make_unique<CaseRec>();
     r->cs = std::move(cs);
     r->nblocks = (int)r->cs.blk.size();
-    if (case_id)
-        *case_id = (int)c->cases.size();
+    const int id = (int)c->cases.size();
     c->cases.push_back(std::move(r));
+    if (case_id)
+        *case_id = id;
+    if (!c->finalized)
+        return MHD_OK;
+    // live admission: place, rebuild the tables, compute the new case's
+    // derived fields, dt and contravariant velocities (the derived kernel is
+    // idempotent for the cases already running)
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    int rc = ensure_case_capacity(c);
+    if (rc)
+        return rc;
+    if ((rc = place_case(c, *c->cases[id])))
+    {
+        c->cases[id]->state = 2;
+        return rc;
+    }
+    if ((rc = rebuild(c)))
+        return rc;
+    derived(c);
+    HIPCHK(c, hipGetLastError());
     return MHD_OK;
 }
 
@@ -219,251 +657,70 @@ extern "C" int mhd_add_case_lunar(mhd_ctx *c, const mhd_wind_desc *d, int *case_
     }
 }
 
-// ------------------------------------------------------------- step driver
-static int run_ops(mhd_ctx *c, int sy, int ushift)
-{
-    mhdk::launch_ops(c->d_ops + c->op_start[sy], c->op_count[sy], c->d_st, c->d_cst, ushift, c->q);
-    return MHD_OK;
-}
-
-struct ProfScope
-{
-    mhd_ctx *c;
-    int id;
-    hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(mhd_ctx *c_, int id_) : c(c_), id(id_)
-    {
-        if (c->prof)
-        {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, c->q);
-        }
-    }
-    ~ProfScope()
-    {
-        if (c->prof)
-        {
-            (void)hipEventRecord(b, c->q);
-            (void)hipEventSynchronize(b);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            c->prof_ms[id] += ms;
-            c->prof_n[id] += 1;
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        }
-    }
-};
-
-static void derived(mhd_ctx *c)
-{
-    {
-        ProfScope p(c, KP_DERIVED);
-        mhdk::launch_derived(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtbits, c->ucur, c->q);
-    }
-    ProfScope p(c, KP_OPS);
-    run_ops(c, SY_JCELL_BC, 0);
-}
-
-// One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every case.
-static void enqueue_step(mhd_ctx *c)
-{
-    const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int ncase = (int)c->cases.size();
-    // dt and the contravariant velocities of this step were computed by the
-    // previous step's derived-field kernel (or by mhd_finalize)
-    {
-        ProfScope p(c, KP_DTFIN);
-        mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, ncase, c->d_active, c->q);
-    }
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_UC_CORNER, 0);
-    }
-    {
-        ProfScope p(c, KP_FACE);
-        mhdk::launch_face_emf(c->shape, c->d_blk, c->d_st, mixed, c->q);
-    }
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_EFACE_BC, 0);
-        run_ops(c, SY_EFACE_HALO, 0);
-    }
-    {
-        ProfScope p(c, KP_EDGE);
-        mhdk::launch_edge_emf(c->shape, c->d_blk, c->d_st, c->q);
-    }
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_EEDGE_BC, 0);
-    }
-    {
-        ProfScope p(c, KP_FLUID);
-        mhdk::launch_fluid(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
-    }
-    {
-        ProfScope p(c, KP_CURL);
-        mhdk::launch_curl_update(c->shape, c->d_blk, c->d_st, c->d_dt, c->q);
-    }
-    c->ucur ^= 1;
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_U_BC0, c->ucur);
-        run_ops(c, SY_U_BC1, c->ucur);
-        run_ops(c, SY_U_HALO, c->ucur);
-        run_ops(c, SY_B_BC, 0);
-        run_ops(c, SY_B_HALO, 0);
-    }
-    derived(c);
-}
-
 extern "C" int mhd_finalize(mhd_ctx *c)
 {
     if (!c)
         return MHD_ERR_ARG;
     if (c->finalized)
         return fail(c, MHD_ERR_STATE, "already finalized");
-    if (c->cases.empty())
-        return fail(c, MHD_ERR_STATE, "no cases");
     HIPCHK(c, hipSetDevice(c->opt.device));
-    // geometry sets and arena placement
-    int64_t geo_n = 0, st_n = 0;
-    std::map<std::string, int> geo_index;
-    c->h_blk.clear();
-    for (int ci = 0; ci < (int)c->cases.size(); ++ci)
-    {
-        CaseRec &r = *c->cases[ci];
-        const int64_t slot = r.cs.blk[0].d.slot;
-        auto it = geo_index.find(r.cs.shape_key);
-        if (it == geo_index.end())
-        {
-            GeoSet gs;
-            gs.key = r.cs.shape_key;
-            gs.base = geo_n;
-            gs.block_stride = (int64_t)G_NSLOT * slot;
-            geo_n += gs.block_stride * r.nblocks;
-            geo_index[gs.key] = (int)c->geos.size();
-            c->geos.push_back(gs);
-        }
-        r.geo_set = geo_index[r.cs.shape_key];
-        r.st_base = st_n;
-        st_n += (int64_t)ST_NSLOT * slot * r.nblocks;
-        r.first_block = (int)c->h_blk.size();
-        for (int b = 0; b < r.nblocks; ++b)
-        {
-            MhdBlockDev d = r.cs.blk[b].d;
-            d.case_id = ci;
-            d.st = r.st_base + (int64_t)b * ST_NSLOT * slot;
-            d.geo = c->geos[r.geo_set].base + (int64_t)b * c->geos[r.geo_set].block_stride;
-            d.geo32 = -1;
-            c->h_blk.push_back(d);
-        }
-    }
-    double budget = c->opt.hbm_budget_gb * 1e9;
-    if (budget > 0 && (double)(geo_n + st_n) * 8.0 > budget)
-        return fail(c, MHD_ERR_MEM, "device memory budget exceeded");
-    c->geo_n = geo_n;
-    c->st_n = st_n;
-    HIPCHK(c, hipMalloc(&c->d_geo, sizeof(double) * geo_n));
-    HIPCHK(c, hipMalloc(&c->d_st, sizeof(double) * st_n));
-    // upload geometry (once per set) and state
-    std::vector<bool> geo_done(c->geos.size(), false);
-    for (auto &rp : c->cases)
-    {
-        CaseRec &r = *rp;
-        const int64_t slot = r.cs.blk[0].d.slot;
-        for (int b = 0; b < r.nblocks; ++b)
-        {
-            const MhdBlockDev &d = c->h_blk[r.first_block + b];
-            if (!geo_done[r.geo_set])
-                HIPCHK(c, hipMemcpy(c->d_geo + d.geo, r.cs.blk[b].geo.data(), sizeof(double) * G_NSLOT * slot,
-                                    hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->d_st + d.st, r.cs.blk[b].st.data(), sizeof(double) * ST_NSLOT * slot,
-                                hipMemcpyHostToDevice));
-            std::vector<double>().swap(r.cs.blk[b].geo);
-            std::vector<double>().swap(r.cs.blk[b].st);
-        }
-        geo_done[r.geo_set] = true;
-    }
-    // block table, constants, dt
-    const int nblk = (int)c->h_blk.size(), ncase = (int)c->cases.size();
-    HIPCHK(c, hipMalloc(&c->d_blk, sizeof(MhdBlockDev) * nblk));
-    HIPCHK(c, hipMemcpy(c->d_blk, c->h_blk.data(), sizeof(MhdBlockDev) * nblk, hipMemcpyHostToDevice));
-    std::vector<MhdCaseConst> cst(ncase);
-    std::vector<int> act(ncase);
-    for (int ci = 0; ci < ncase; ++ci)
-    {
-        cst[ci] = c->cases[ci]->cs.cst;
-        act[ci] = c->cases[ci]->active;
-    }
-    HIPCHK(c, hipMalloc(&c->d_cst, sizeof(MhdCaseConst) * ncase));
-    HIPCHK(c, hipMemcpy(c->d_cst, cst.data(), sizeof(MhdCaseConst) * ncase, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMalloc(&c->d_active, sizeof(int) * ncase));
-    HIPCHK(c, hipMemcpy(c->d_active, act.data(), sizeof(int) * ncase, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMalloc(&c->d_dt, sizeof(double) * ncase));
-    HIPCHK(c, hipMalloc(&c->d_dtbits, sizeof(unsigned long long) * ncase));
-    HIPCHK(c, hipMalloc(&c->d_divb, sizeof(unsigned long long) * ncase));
-    HIPCHK(c, hipMalloc(&c->d_dtscratch, sizeof(unsigned long long) * ncase));
-    HIPCHK(c, hipMalloc(&c->d_err, sizeof(int)));
-    HIPCHK(c, hipMemset(c->d_err, 0, sizeof(int)));
-    HIPCHK(c, hipMemset(c->d_dt, 0, sizeof(double) * ncase));
-    {
-        std::vector<unsigned long long> init(ncase);
-        const double big = 1e100;
-        unsigned long long bits;
-        std::memcpy(&bits, &big, 8);
-        std::fill(init.begin(), init.end(), bits);
-        HIPCHK(c, hipMemcpy(c->d_dtbits, init.data(), 8 * ncase, hipMemcpyHostToDevice));
-    }
-    // op lists: relocate to global offsets, concatenate per sync group
-    std::vector<MhdOp> all;
-    for (int sy = 0; sy < SY_COUNT; ++sy)
-    {
-        c->op_start[sy] = (int)all.size();
-        for (int ci = 0; ci < ncase; ++ci)
-        {
-            const CaseRec &r = *c->cases[ci];
-            for (MhdOp o : r.cs.ops[sy])
-            {
-                o.dst += r.st_base;
-                if (o.kind == OP_COPY || o.kind == OP_NEG || o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
-                    o.src += r.st_base;
-                o.cas = ci;
-                all.push_back(o);
-            }
-        }
-        c->op_count[sy] = (int)all.size() - c->op_start[sy];
-    }
-    HIPCHK(c, hipMalloc(&c->d_ops, sizeof(MhdOp) * std::max<size_t>(1, all.size())));
-    if (!all.empty())
-        HIPCHK(c, hipMemcpy(c->d_ops, all.data(), sizeof(MhdOp) * all.size(), hipMemcpyHostToDevice));
-    for (auto &rp : c->cases)
-        rp->cs.ops.assign(SY_COUNT, {});
-    c->shape.nblocks = nblk;
-    c->shape.max_ni = c->shape.max_nj = c->shape.max_nk = 0;
-    for (const auto &d : c->h_blk)
-    {
-        c->shape.max_ni = std::max(c->shape.max_ni, d.ni);
-        c->shape.max_nj = std::max(c->shape.max_nj, d.nj);
-        c->shape.max_nk = std::max(c->shape.max_nk, d.nk);
-    }
+    int rc = ensure_case_capacity(c);
+    if (rc)
+        return rc;
+    for (auto &r : c->cases)
+        if (r->state == 0 && (rc = place_case(c, *r)))
+            return rc;
+    if ((rc = rebuild(c)))
+        return rc;
     c->ucur = 0;
     c->finalized = true;
     derived(c);
+    if (c->opt.autotune && (rc = autotune(c)))
+        return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->q));
     return MHD_OK;
 }
 
+extern "C" int mhd_retire(mhd_ctx *c, int case_id)
+{
+    if (!c || case_id < 0 || case_id >= (int)c->cases.size())
+        return MHD_ERR_ARG;
+    CaseRec &r = *c->cases[case_id];
+    if (r.state != 1)
+        return fail(c, MHD_ERR_STATE, "case is not active");
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    free_t(r.d_st);
+    c->bytes_used -= r.st_bytes;
+    GeoSet &G = c->geos[r.geo_set];
+    if (--G.refs == 0)
+    {
+        free_t(G.d);
+        c->bytes_used -= G.bytes;
+    }
+    r.state = 2;
+    for (auto &o : r.ops)
+        std::vector<MhdOp>().swap(o);
+    return rebuild(c);
+}
+
+// ------------------------------------------------------------- stepping
 static int check_err(mhd_ctx *c)
 {
-    int e = 0;
-    HIPCHK(c, hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->q));
+    std::vector<int> e(c->case_cap, 0);
+    HIPCHK(c, hipMemcpyAsync(e.data(), c->d_err, sizeof(int) * c->case_cap, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
-    if (e)
-        return fail(c, MHD_ERR_DT, "dt too small or NaN");
+    for (int ci = 0; ci < (int)c->cases.size(); ++ci)
+        if (e[ci] && c->cases[ci]->state == 1)
+            return fail(c, MHD_ERR_DT, "dt too small or NaN in case " + std::to_string(ci));
     return MHD_OK;
+}
+
+static void count_steps(mhd_ctx *c, int n)
+{
+    for (auto &r : c->cases)
+        if (r->state == 1)
+            r->steps += n;
 }
 
 extern "C" int mhd_step(mhd_ctx *c, int nsteps)
@@ -472,10 +729,13 @@ extern "C" int mhd_step(mhd_ctx *c, int nsteps)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "mhd_finalize first");
+    if (c->h_blk.empty())
+        return MHD_OK;
     HIPCHK(c, hipSetDevice(c->opt.device));
     for (int s = 0; s < nsteps; ++s)
         enqueue_step(c);
     HIPCHK(c, hipGetLastError());
+    count_steps(c, nsteps);
     return check_err(c);
 }
 
@@ -485,9 +745,11 @@ extern "C" int mhd_step_graph(mhd_ctx *c, int nsteps)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "mhd_finalize first");
-    HIPCHK(c, hipSetDevice(c->opt.device));
+    if (c->h_blk.empty())
+        return MHD_OK;
     if (c->prof)
         return mhd_step(c, nsteps);
+    HIPCHK(c, hipSetDevice(c->opt.device));
     int s = 0;
     if (c->ucur != 0 && nsteps > 0)
     {
@@ -512,6 +774,7 @@ extern "C" int mhd_step_graph(mhd_ctx *c, int nsteps)
     for (; s < nsteps; ++s)
         enqueue_step(c);
     HIPCHK(c, hipGetLastError());
+    count_steps(c, nsteps);
     return check_err(c);
 }
 
@@ -523,17 +786,21 @@ extern "C" int mhd_synchronize(mhd_ctx *c)
     return MHD_OK;
 }
 
+static CaseRec *active_case(mhd_ctx *c, int case_id)
+{
+    if (!c || case_id < 0 || case_id >= (int)c->cases.size() || !c->finalized)
+        return nullptr;
+    CaseRec *r = c->cases[case_id].get();
+    return r->state == 1 ? r : nullptr;
+}
+
 extern "C" int mhd_compute_dt(mhd_ctx *c, int case_id, double *dt)
 {
-    if (!c || !dt || case_id < 0 || case_id >= (int)c->cases.size())
+    if (!dt || !active_case(c, case_id))
         return MHD_ERR_ARG;
-    if (!c->finalized)
-        return fail(c, MHD_ERR_STATE, "mhd_finalize first");
-    const double big = 1e100;
-    unsigned long long bits;
-    std::memcpy(&bits, &big, 8);
-    mhdk::launch_fill_u64(c->d_dtscratch, (int)c->cases.size(), bits, c->q);
-    mhdk::launch_dt(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_cst, c->d_dtscratch, c->ucur, c->q);
+    mhdk::launch_fill_u64(c->d_dtscratch, c->case_cap, dbits(1e100), c->q);
+    mhdk::launch_dt(launch_of(c, KP_DT), c->d_cst, c->d_dtscratch, c->ucur, c->q);
+    unsigned long long bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtscratch + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
     std::memcpy(dt, &bits, 8);
@@ -542,19 +809,27 @@ extern "C" int mhd_compute_dt(mhd_ctx *c, int case_id, double *dt)
 
 extern "C" int mhd_last_dt(mhd_ctx *c, int case_id, double *dt)
 {
-    if (!c || !dt || case_id < 0 || case_id >= (int)c->cases.size() || !c->finalized)
+    if (!dt || !active_case(c, case_id))
         return MHD_ERR_ARG;
     HIPCHK(c, hipMemcpyAsync(dt, c->d_dt + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
     return MHD_OK;
 }
 
+extern "C" int mhd_case_steps(mhd_ctx *c, int case_id, long long *steps)
+{
+    if (!c || !steps || case_id < 0 || case_id >= (int)c->cases.size())
+        return MHD_ERR_ARG;
+    *steps = c->cases[case_id]->steps;
+    return MHD_OK;
+}
+
 extern "C" int mhd_divb_max(mhd_ctx *c, int case_id, double *out)
 {
-    if (!c || !out || case_id < 0 || case_id >= (int)c->cases.size() || !c->finalized)
+    if (!out || !active_case(c, case_id))
         return MHD_ERR_ARG;
-    mhdk::launch_fill_u64(c->d_divb, (int)c->cases.size(), 0ull, c->q);
-    mhdk::launch_divb(c->shape, c->d_blk, c->d_st, c->d_geo, c->d_divb, c->q);
+    mhdk::launch_fill_u64(c->d_divb, c->case_cap, 0ull, c->q);
+    mhdk::launch_divb(launch_of(c, KP_DT), c->d_divb, c->q);
     unsigned long long bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_divb + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
@@ -565,21 +840,21 @@ extern "C" int mhd_divb_max(mhd_ctx *c, int case_id, double *out)
 // ----------------------------------------------------------- data exchange
 static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, size_t n, bool down)
 {
-    if (!c || !aos || case_id < 0 || case_id >= (int)c->cases.size())
+    if (!aos || !c)
         return MHD_ERR_ARG;
-    if (!c->finalized)
-        return fail(c, MHD_ERR_STATE, "mhd_finalize first");
-    CaseRec &r = *c->cases[case_id];
-    if (block < 0 || block >= r.nblocks)
+    CaseRec *r = active_case(c, case_id);
+    if (!r)
+        return fail(c, MHD_ERR_STATE, "case not active (or context not finalized)");
+    if (block < 0 || block >= r->nblocks)
         return MHD_ERR_ARG;
     int region, slot, loc, nc;
     if (!field_slot(field, c->ucur, region, slot, loc, nc))
         return fail(c, MHD_ERR_ARG, "field not stored on the device");
-    const MhdBlockDev &d = c->h_blk[r.first_block + block];
+    const MhdBlockDev &d = c->h_blk[r->first_block + block];
     if (n != aos_count(d, loc, nc))
         return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
     HIPCHK(c, hipStreamSynchronize(c->q));
-    double *dev = (region == 0 ? c->d_st + d.st : c->d_geo + d.geo) + (int64_t)slot * d.slot;
+    double *dev = (region == 0 ? st_ptr(d) : const_cast<double *>(geo_ptr(d))) + (int64_t)slot * d.slot;
     std::vector<double> soa((size_t)nc * d.slot);
     HIPCHK(c, hipMemcpy(soa.data(), dev, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
     soa_aos(d, loc, nc, soa.data(), aos, down);
@@ -675,9 +950,10 @@ extern "C" int mhd_get_constants(mhd_ctx *c, int case_id, double *o)
     if (!c || !o || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
     const MhdCaseConst &k = c->cases[case_id]->cs.cst;
-    const double v[22] = {k.gamma,   k.cfl,     k.inver_MA2, k.momentum_hall_coeff, k.M_H,     k.M_ref,   k.ne_floor, k.state_coeff,
-                          k.qinf[0], k.qinf[1], k.qinf[2],   k.qinf[3],             k.qinf[4], k.B_imf[0], k.B_imf[1], k.B_imf[2],
-                          k.u_inf[0], k.u_inf[1], k.u_inf[2], k.dt_floor,           0.0,       0.0};
+    const double v[22] = {k.gamma,    k.cfl,         k.inver_MA2, k.momentum_hall_coeff, k.M_H,      k.M_ref,
+                          k.ne_floor, k.state_coeff, k.qinf[0],   k.qinf[1],             k.qinf[2],  k.qinf[3],
+                          k.qinf[4],  k.B_imf[0],    k.B_imf[1],  k.B_imf[2],            k.u_inf[0], k.u_inf[1],
+                          k.u_inf[2], k.dt_floor,    0.0,         0.0};
     std::memcpy(o, v, sizeof(v));
     return MHD_OK;
 }
@@ -708,10 +984,18 @@ extern "C" int mhd_profile_read(mhd_ctx *c, int kernel, double *ms, int *launche
     return MHD_OK;
 }
 
+extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows)
+{
+    if (!c || !rows || kernel < 0 || kernel >= KP_COUNT)
+        return MHD_ERR_ARG;
+    *rows = kJt[c->jt_idx[kernel]];
+    return MHD_OK;
+}
+
 extern "C" int mhd_memory(mhd_ctx *c, double *bytes)
 {
     if (!c || !bytes)
         return MHD_ERR_ARG;
-    *bytes = 8.0 * (double)(c->st_n + c->geo_n);
+    *bytes = (double)c->bytes_used;
     return MHD_OK;
 }

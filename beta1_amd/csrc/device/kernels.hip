@@ -3,14 +3,16 @@
 // Memory-bound stencils over the padded per-block box of layout.h: a
 // workgroup is 64 x 4 threads, one 64-lane wavefront per k-row, so every
 // slot load and store of a wavefront is one contiguous 512-byte segment.
-// blockIdx.z selects the block (of any case), blockIdx.y packs (i, j-tile),
-// blockIdx.x is the k-tile.
+// A launch is a 1-D grid over the tiles (block, i, j-tile, k-tile) of every
+// placed block, see MhdTileMap.
 //
 // FP64 arithmetic is written in the same operation order as the reference
 // (and as the CPU oracle) and the library is compiled with
 // -ffp-contract=off, so the FP64 path reproduces the reference bit for bit
 // wherever division and square root are correctly rounded.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "../host/setup.h"
 #include "../layout.h"
@@ -24,26 +26,31 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
-struct Ctx3
+// Map a workgroup/thread to (block, i, j, k) of its tile.  `wg` is false for
+// padding workgroups of the XCD-ordered grid (no block at all); the return
+// value is false for padding threads.
+__device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, int &i, int &j, int &k, bool &wg)
 {
-    const MhdBlockDev *blk;
-    int nb;
-};
-
-// Map a workgroup/thread to (block, i, j, k) over the box [lo, n + hi) per
-// axis.  Returns false for padding threads.
-__device__ __forceinline__ bool thread_cell(const MhdBlockDev *__restrict__ blks, int lo, int hi, MhdBlockDev &b, int &i,
-                                            int &j, int &k)
-{
-    b = blks[blockIdx.z];
-    const int ext_i = b.ni - lo + hi, ext_j = b.nj - lo + hi, ext_k = b.nk - lo + hi;
-    const int jt = (ext_j + 3) >> 2;
-    const int iy = blockIdx.y / jt;
-    const int jy = blockIdx.y - iy * jt;
-    i = lo + iy;
-    j = lo + jy * 4 + (int)threadIdx.y;
-    k = lo + (int)blockIdx.x * 64 + (int)threadIdx.x;
-    return (iy < ext_i) && (j < lo + ext_j) && (k < lo + ext_k);
+    int t = (int)blockIdx.x;
+    if (tm.xcd_chunk > 0)
+        t = (t & 7) * tm.xcd_chunk + (t >> 3);
+    wg = t < tm.ntiles;
+    if (!wg)
+        return false;
+    const int bi = tm.tile_block[t];
+    b = tm.blks[bi];
+    const int lt = t - tm.block_start[bi];
+    const int jt = (int)blockDim.y;
+    const int ext_j = b.nj - tm.lo + tm.hi, ext_k = b.nk - tm.lo + tm.hi;
+    const int ntk = (ext_k + 63) >> 6, ntj = (ext_j + jt - 1) / jt;
+    const int kt = lt % ntk;
+    const int r = lt / ntk;
+    const int jy = r % ntj;
+    const int iy = r / ntj;
+    i = tm.lo + iy;
+    j = tm.lo + jy * jt + (int)threadIdx.y;
+    k = tm.lo + kt * 64 + (int)threadIdx.x;
+    return (j < tm.lo + ext_j) && (k < tm.lo + ext_k);
 }
 
 // ------------------------------------------------------------------ state
@@ -186,31 +193,34 @@ __device__ __forceinline__ void reduce_dt(double local, unsigned long long *__re
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0)
     {
-        const double m = dmin(dmin(red[0], red[1]), dmin(red[2], red[3]));
+        double m = red[0];
+        for (int q = 1; q < (int)blockDim.y; ++q)
+            m = dmin(m, red[q]);
         if (m < 1e100)
             atomicMin(&dtbits[case_id], (unsigned long long)__double_as_longlong(m));
     }
 }
 
-__global__ __launch_bounds__(256) void k_dt(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                            const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+__global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                             unsigned long long *__restrict__ dtbits, int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
-    const bool ok = thread_cell(blks, 0, 0, b, i, j, k);
+    bool wg;
+    const bool ok = tile_cell(tm, b, i, j, k, wg);
     double local = 1e100;
     if (ok)
     {
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-        const double *U = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
-        const double *Bc = st + b.st + (int64_t)ST_BCELL * sl;
-        local = cell_dt(b, cst[b.case_id], U, geo + b.geo, o, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
+        const double *U = st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
+        const double *Bc = st_ptr(b) + (int64_t)ST_BCELL * sl;
+        local = cell_dt(b, cst[b.case_id], U, geo_ptr(b), o, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
     }
-    reduce_dt(local, dtbits, blks[blockIdx.z].case_id);
+    if (wg)
+        reduce_dt(local, dtbits, b.case_id);
 }
 
-// finalize: bits -> dt, abort check, reset for the next reduction
+// finalize: bits -> dt, abort check per case, reset for the next reduction
 __global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
                               const int *active)
 {
@@ -222,7 +232,7 @@ __global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, 
     {
         dt[c] = v;
         if (!(v == v) || v < cst[c].dt_floor || isinf(v))
-            atomicOr(err, 1);
+            err[c] = 1;
     }
     dtbits[c] = (unsigned long long)__double_as_longlong(1e100);
 }
@@ -252,17 +262,17 @@ __device__ __forceinline__ void cell_ucontra(const MhdBlockDev &b, double gm, co
     }
 }
 
-__global__ __launch_bounds__(256) void k_ucontra(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+__global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, -1, 1, b, i, j, k))
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    cell_ucontra(b, cst[b.case_id].gamma, st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl, geo + b.geo,
-                 st + b.st + (int64_t)ST_UC * sl, o);
+    cell_ucontra(b, cst[b.case_id].gamma, st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, geo_ptr(b),
+                 st_ptr(b) + (int64_t)ST_UC * sl, o);
 }
 
 // ======================================================================
@@ -299,15 +309,16 @@ __device__ __forceinline__ double emf_flux(const double *__restrict__ Bf, const 
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_face_emf(const MhdBlockDev *__restrict__ blks, double *__restrict__ st)
+__global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, 0, 1, b, i, j, k))
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-    double *S = st + b.st;
+    double *S = st_ptr(b);
     const double *Bf = S + (int64_t)ST_B * sl, *Ba = S + (int64_t)ST_BADD * sl, *UC = S + (int64_t)ST_UC * sl;
     const int n[3] = {b.ni, b.nj, b.nk};
     const int idx[3] = {i, j, k};
@@ -365,15 +376,16 @@ __device__ __forceinline__ double max_abs4(const double *UCa, int64_t o0, int64_
     return a;
 }
 
-__global__ __launch_bounds__(256) void k_edge_emf(const MhdBlockDev *__restrict__ blks, double *__restrict__ st)
+__global__ __launch_bounds__(256) void k_edge_emf(const MhdTileMap tm)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, 0, 1, b, i, j, k))
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st + b.st;
+    double *S = st_ptr(b);
     const double *Fx = S + (int64_t)(ST_EFACE + 0) * sl, *Fy = S + (int64_t)(ST_EFACE + 3) * sl,
                  *Fz = S + (int64_t)(ST_EFACE + 6) * sl;
     const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
@@ -413,16 +425,16 @@ __global__ __launch_bounds__(256) void k_edge_emf(const MhdBlockDev *__restrict_
 // B_face += dt * (-curl E): CurlEdgeToFace with multiper -1
 // (Z4_Lunar/src/operators/CTOperators.cpp:6-56) and ApplyUpdate_Euler_
 // (Z4_Lunar/src/physics/LunarSolver_Advance.cpp:61-122).
-__global__ __launch_bounds__(256) void k_curl_update(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                                     const double *__restrict__ dts)
+__global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const double *__restrict__ dts)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, 0, 1, b, i, j, k))
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st + b.st;
+    double *S = st_ptr(b);
     const double *Ex = S + (int64_t)ST_E * sl, *Ey = Ex + sl, *Ez = Ex + 2 * sl;
     double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
     const double dt = dts[b.case_id];
@@ -495,22 +507,22 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t 
 }
 
 template <typename R, int RECON>
-__global__ __launch_bounds__(256) void k_fluid(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                               const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+__global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                const double *__restrict__ dts, int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
-    if (!thread_cell(blks, 0, 0, b, i, j, k))
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const MhdCaseConst &c = cst[b.case_id];
     const double gm = c.gamma;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-    double *S = st + b.st;
+    double *S = st_ptr(b);
     const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
     double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
-    const double *G = geo + b.geo;
+    const double *G = geo_ptr(b);
     double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
 #pragma unroll
@@ -554,24 +566,24 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdBlockDev *__restrict__ b
 // Fused with the next step's CFL limit (dt depends on U after the Ucell
 // sync and on the B_cell computed here, nothing changes them before the next
 // step) and with the contravariant velocities of cells [-1, N].
-__global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__ blks, double *__restrict__ st,
-                                                 const double *__restrict__ geo, const MhdCaseConst *__restrict__ cst,
+__global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  unsigned long long *__restrict__ dtbits, int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
-    const bool ok = thread_cell(blks, -1, 1, b, i, j, k);
+    bool wg;
+    const bool ok = tile_cell(tm, b, i, j, k, wg);
     double local = 1e100;
     if (ok)
     {
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const double *Ucur = st + b.st + (int64_t)(ST_U0 + 5 * ucur) * sl;
-    cell_ucontra(b, cst[b.case_id].gamma, Ucur, geo + b.geo, st + b.st + (int64_t)ST_UC * sl, o);
+    const double *Ucur = st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
+    cell_ucontra(b, cst[b.case_id].gamma, Ucur, geo_ptr(b), st_ptr(b) + (int64_t)ST_UC * sl, o);
     if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
     {
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st + b.st;
-    const double *G = geo + b.geo;
+    double *S = st_ptr(b);
+    const double *G = geo_ptr(b);
     const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
     const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
     {
@@ -625,7 +637,8 @@ __global__ __launch_bounds__(256) void k_derived(const MhdBlockDev *__restrict__
     }
     }
     }
-    reduce_dt(local, dtbits, blks[blockIdx.z].case_id);
+    if (wg)
+        reduce_dt(local, dtbits, b.case_id);
 }
 
 // ======================================================================
@@ -641,7 +654,9 @@ __device__ __forceinline__ double solid_p(double gm, double rho, double mx, doub
     return dmax(p, 1e-12);
 }
 
-__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n, double *__restrict__ st,
+// Op dst/src hold absolute device addresses; components are `stride`
+// elements apart.  `ushift` moves U ops to the second U buffer.
+__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n,
                                              const MhdCaseConst *__restrict__ cst, int ushift)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -649,27 +664,28 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
         return;
     const MhdOp op = ops[t];
     const int64_t sh = ushift ? 5 * op.stride : 0;
-    const int64_t d = op.dst + sh, s = op.src + sh;
+    double *d = reinterpret_cast<double *>((uintptr_t)op.dst) + sh;
+    const double *s = reinterpret_cast<const double *>((uintptr_t)op.src) + sh;
     switch (op.kind)
     {
     case OP_COPY:
         for (int c = 0; c < op.ncomp; ++c)
-            st[d + c * op.stride] = st[s + c * op.stride];
+            d[c * op.stride] = s[c * op.stride];
         break;
     case OP_NEG:
         for (int c = 0; c < op.ncomp; ++c)
-            st[d + c * op.stride] = -st[s + c * op.stride];
+            d[c * op.stride] = -s[c * op.stride];
         break;
     case OP_ZERO:
         for (int c = 0; c < op.ncomp; ++c)
-            st[d + c * op.stride] = 0.0;
+            d[c * op.stride] = 0.0;
         break;
     case OP_CONST:
-        st[op.dst] = op.val;
+        d[0] = op.val;
         break;
     case OP_QINF:
         for (int c = 0; c < op.ncomp; ++c)
-            st[d + c * op.stride] = cst[op.cas].qinf[c];
+            d[c * op.stride] = cst[op.cas].qinf[c];
         break;
     case OP_SOLID_FIRST:
     case OP_SOLID_GHOST:
@@ -677,14 +693,14 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
         // BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184)
         const double gm = cst[op.cas].gamma;
         const int64_t L = op.stride;
-        const double rho_r = dmax(st[s], 2.5e-3);
-        const double p_r = solid_p(gm, rho_r, st[s + L], st[s + 2 * L], st[s + 3 * L], st[s + 4 * L]);
-        st[d] = rho_r;
-        st[d + L] = rho_r * 0.0;
-        st[d + 2 * L] = rho_r * 0.0;
-        st[d + 3 * L] = rho_r * 0.0;
+        const double rho_r = dmax(s[0], 2.5e-3);
+        const double p_r = solid_p(gm, rho_r, s[L], s[2 * L], s[3 * L], s[4 * L]);
+        d[0] = rho_r;
+        d[L] = rho_r * 0.0;
+        d[2 * L] = rho_r * 0.0;
+        d[3 * L] = rho_r * 0.0;
         const double ke_g = 0.5 * rho_r * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
-        st[d + 4 * L] = p_r / (gm - 1.0) + ke_g;
+        d[4 * L] = p_r / (gm - 1.0) + ke_g;
         break;
     }
     default:
@@ -693,19 +709,19 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
-__global__ __launch_bounds__(256) void k_divb(const MhdBlockDev *__restrict__ blks, const double *__restrict__ st,
-                                              const double *__restrict__ geo, unsigned long long *__restrict__ out)
+__global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long long *__restrict__ out)
 {
     MhdBlockDev b;
     int i, j, k;
-    const bool ok = thread_cell(blks, 0, 0, b, i, j, k);
+    bool wg;
+    const bool ok = tile_cell(tm, b, i, j, k, wg);
     double v = 0.0;
     if (ok)
     {
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-        const double *Bx = st + b.st + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+        const double *Bx = st_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
         const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
-                          geo[b.geo + (int64_t)G_JAC * sl + o];
+                          geo_ptr(b)[(int64_t)G_JAC * sl + o];
         v = fabs(dv);
         if (!(v == v))
             v = 1e300;
@@ -717,10 +733,12 @@ __global__ __launch_bounds__(256) void k_divb(const MhdBlockDev *__restrict__ bl
     if (threadIdx.x == 0)
         red[threadIdx.y] = v;
     __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0)
+    if (threadIdx.x == 0 && threadIdx.y == 0 && wg)
     {
-        const double m = dmax(dmax(red[0], red[1]), dmax(red[2], red[3]));
-        atomicMax(&out[blks[blockIdx.z].case_id], (unsigned long long)__double_as_longlong(m));
+        double m = red[0];
+        for (int q = 1; q < (int)blockDim.y; ++q)
+            m = dmax(m, red[q]);
+        atomicMax(&out[b.case_id], (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -735,71 +753,67 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // launch helpers
 namespace mhdk
 {
-static dim3 grid_for(const LaunchShape &s, int lo, int hi)
+static dim3 grid_of(const Launch &L)
 {
-    const int ek = s.max_nk - lo + hi, ej = s.max_nj - lo + hi, ei = s.max_ni - lo + hi;
-    return dim3((unsigned)((ek + 63) / 64), (unsigned)(ei * ((ej + 3) / 4)), (unsigned)s.nblocks);
+    const int n = L.tm.xcd_chunk > 0 ? 8 * L.tm.xcd_chunk : L.tm.ntiles;
+    return dim3((unsigned)std::max(n, 1));
 }
-static const dim3 kBlock(64, 4, 1);
+static dim3 block_of(const Launch &L) { return dim3(64, (unsigned)L.jt, 1); }
 
-void launch_dt(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo, const MhdCaseConst *cst,
-               unsigned long long *dtbits, int ucur, hipStream_t q)
+void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_dt, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo, cst, dtbits, ucur);
+    hipLaunchKernelGGL(k_dt, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
 }
 void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
                         const int *active, hipStream_t q)
 {
     hipLaunchKernelGGL(k_dt_finalize, dim3((ncase + 63) / 64), dim3(64), 0, q, dtbits, dt, err, cst, ncase, active);
 }
-void launch_ucontra(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
-                    const MhdCaseConst *cst, int ucur, hipStream_t q)
+void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_ucontra, grid_for(s, -1, 1), kBlock, 0, q, blks, st, geo, cst, ucur);
+    hipLaunchKernelGGL(k_ucontra, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
 }
-void launch_face_emf(const LaunchShape &s, const MhdBlockDev *blks, double *st, int mixed, hipStream_t q)
+void launch_face_emf(const Launch &L, int mixed, hipStream_t q)
 {
     if (mixed)
-        hipLaunchKernelGGL(k_face_emf<float>, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+        hipLaunchKernelGGL(k_face_emf<float>, grid_of(L), block_of(L), 0, q, L.tm);
     else
-        hipLaunchKernelGGL(k_face_emf<double>, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+        hipLaunchKernelGGL(k_face_emf<double>, grid_of(L), block_of(L), 0, q, L.tm);
 }
-void launch_edge_emf(const LaunchShape &s, const MhdBlockDev *blks, double *st, hipStream_t q)
+void launch_edge_emf(const Launch &L, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_edge_emf, grid_for(s, 0, 1), kBlock, 0, q, blks, st);
+    hipLaunchKernelGGL(k_edge_emf, grid_of(L), block_of(L), 0, q, L.tm);
 }
-void launch_curl_update(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *dts, hipStream_t q)
+void launch_curl_update(const Launch &L, const double *dts, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_curl_update, grid_for(s, 0, 1), kBlock, 0, q, blks, st, dts);
+    hipLaunchKernelGGL(k_curl_update, grid_of(L), block_of(L), 0, q, L.tm, dts);
 }
-void launch_fluid(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
-                  const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon, hipStream_t q)
+void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
+                  hipStream_t q)
 {
-    const dim3 g = grid_for(s, 0, 0);
+    const dim3 g = grid_of(L), bl = block_of(L);
     if (mixed && recon)
-        hipLaunchKernelGGL((k_fluid<float, 1>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+        hipLaunchKernelGGL((k_fluid<float, 1>), g, bl, 0, q, L.tm, cst, dts, ucur);
     else if (mixed)
-        hipLaunchKernelGGL((k_fluid<float, 0>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+        hipLaunchKernelGGL((k_fluid<float, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
     else if (recon)
-        hipLaunchKernelGGL((k_fluid<double, 1>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+        hipLaunchKernelGGL((k_fluid<double, 1>), g, bl, 0, q, L.tm, cst, dts, ucur);
     else
-        hipLaunchKernelGGL((k_fluid<double, 0>), g, kBlock, 0, q, blks, st, geo, cst, dts, ucur);
+        hipLaunchKernelGGL((k_fluid<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
-void launch_derived(const LaunchShape &s, const MhdBlockDev *blks, double *st, const double *geo,
-                    const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
+void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_derived, grid_for(s, -1, 1), kBlock, 0, q, blks, st, geo, cst, dtbits, ucur);
+    hipLaunchKernelGGL(k_derived, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
 }
-void launch_ops(const MhdOp *ops, int n, double *st, const MhdCaseConst *cst, int ushift, hipStream_t q)
+void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {
     if (n <= 0)
         return;
-    hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, st, cst, ushift);
+    hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, cst, ushift);
 }
-void launch_divb(const LaunchShape &s, const MhdBlockDev *blks, const double *st, const double *geo,
-                 unsigned long long *out, hipStream_t q)
+void launch_divb(const Launch &L, unsigned long long *out, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_divb, grid_for(s, 0, 0), kBlock, 0, q, blks, st, geo, out);
+    hipLaunchKernelGGL(k_divb, grid_of(L), block_of(L), 0, q, L.tm, out);
 }
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
 {

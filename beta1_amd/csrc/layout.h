@@ -101,9 +101,9 @@ struct MhdBlockDev
     int case_id;     // which case (dt, constants) this block belongs to
     int bc[6];       // BC kind per face: -i,+i,-j,+j,-k,+k
     int64_t slot;    // elements per slot
-    int64_t st;      // element offset of slot 0 of the state region
-    int64_t geo;     // element offset of slot 0 of the geometry region
-    int64_t geo32;   // element offset in the fp32 geometry arena (-1 if none)
+    int64_t st;      // device address of slot 0 of the block's state (0 before placement)
+    int64_t geo;     // device address of slot 0 of the block's geometry
+    int64_t spare;
 };
 
 // Per-case constants (the reference's calc_physical_constant,
@@ -122,6 +122,24 @@ struct MhdCaseConst
     double B_imf[3];            // far-field total B
     double u_inf[3];            // far-field velocity (primitive)
     double dt_floor;            // abort threshold (reference: 1e-7)
+};
+
+MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }
+MHD_HD const double *geo_ptr(const MhdBlockDev &b) { return reinterpret_cast<const double *>((uintptr_t)b.geo); }
+
+// Launch tiling: every (block, i, j-tile, k-tile) of a launch range is one
+// workgroup of 64 x jt threads; tile_block[t] / block_start[b] map a tile to
+// its block.  With xcd_chunk > 0 workgroup w runs tile (w % 8) * xcd_chunk +
+// w / 8, so each of the 8 XCDs (workgroups are dealt round-robin) sweeps a
+// contiguous range of tiles and neighbour planes stay in its own L2.
+struct MhdTileMap
+{
+    const MhdBlockDev *blks;
+    const int *tile_block;
+    const int *block_start;
+    int ntiles;
+    int lo, hi; // range [lo, n + hi) on every axis
+    int xcd_chunk;
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)

@@ -132,8 +132,10 @@ class HostCase:
 class MhdContext:
     """Device context holding one or more cases (all advanced together)."""
 
-    def __init__(self, precision="fp64", recon=0, ghost=2, device=0, hbm_budget_gb=0.0):
-        opt = MhdOptions(device, PREC[precision], recon, ghost, hbm_budget_gb)
+    def __init__(self, precision="fp64", recon=0, ghost=2, device=0, hbm_budget_gb=0.0, autotune=False,
+                 xcd_order=True):
+        opt = MhdOptions(device, PREC[precision], recon, ghost, hbm_budget_gb, 1 if autotune else 0,
+                         1 if xcd_order else 0)
         self.ctx = C.c_void_p()
         _check(lib().mhd_create(C.byref(opt), C.byref(self.ctx)), what="mhd_create")
         self.cases = []
@@ -159,6 +161,35 @@ class MhdContext:
 
     def finalize(self):
         _check(lib().mhd_finalize(self.ctx), self.ctx, "mhd_finalize")
+
+    def try_add(self, desc):
+        """Admit a case into the running context; None when the HBM budget is full."""
+        cid = C.c_int()
+        if isinstance(desc, OTDesc):
+            rc = lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
+        else:
+            rc = lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
+        if rc == -5:
+            return None
+        _check(rc, self.ctx, "admit case")
+        self.cases.append(desc)
+        return cid.value
+
+    def retire(self, cid):
+        _check(lib().mhd_retire(self.ctx, cid), self.ctx, "mhd_retire")
+
+    def case_steps(self, cid):
+        v = C.c_longlong()
+        _check(lib().mhd_case_steps(self.ctx, cid, C.byref(v)), self.ctx, "case_steps")
+        return v.value
+
+    def tuning(self):
+        out = {}
+        for k, name in enumerate(self.profile_read()):
+            rows = C.c_int()
+            lib().mhd_tuning(self.ctx, k, C.byref(rows))
+            out[name] = rows.value
+        return out
 
     def step(self, n=1, graph=False):
         f = lib().mhd_step_graph if graph else lib().mhd_step

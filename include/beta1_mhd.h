@@ -55,7 +55,9 @@ typedef struct
     int precision; /* mhd_precision */
     int recon;     /* 0 first order (reference), 1 minmod MUSCL */
     int ghost;     /* ghost layers (>= 1; >= 2 for minmod) */
-    double hbm_budget_gb; /* admission budget for serving (0 = no limit) */
+    double hbm_budget_gb; /* admission budget: bytes of state + geometry (0 = no limit) */
+    int autotune;         /* 1: time workgroup heights per kernel at mhd_finalize */
+    int xcd_order;        /* 1: deal tiles to the 8 XCDs in contiguous ranges */
 } mhd_options;
 
 /* Orszag-Tang periodic box (n^3 cells on [0,1)^3). */
@@ -92,8 +94,14 @@ const char *mhd_last_error(mhd_ctx *ctx);
 int mhd_add_case_orszag_tang(mhd_ctx *ctx, const mhd_ot_desc *d, int *case_id);
 int mhd_add_case_lunar(mhd_ctx *ctx, const mhd_wind_desc *d, int *case_id);
 /* Upload everything added so far and compute the derived fields
-   (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143). */
+   (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143).
+   After mhd_finalize, mhd_add_case_* admits a case into the running context
+   (between steps): it fails with MHD_ERR_MEM when the HBM budget is full. */
 int mhd_finalize(mhd_ctx *ctx);
+/* Free a case's device state (its geometry when no other case shares it). */
+int mhd_retire(mhd_ctx *ctx, int case_id);
+/* Steps a case has been advanced by. */
+int mhd_case_steps(mhd_ctx *ctx, int case_id, long long *steps);
 
 /* Advance every case by nsteps explicit steps, each case with its own dt
    (LunarSolver::StepOnce, Z4_Lunar/src/LunarSolver_Driver.cpp:21-92).
@@ -131,8 +139,11 @@ int mhd_get_constants(mhd_ctx *ctx, int case_id, double *out22);
    measured with HIP events on the context's stream; names via index. */
 int mhd_profile(mhd_ctx *ctx, int enable);
 int mhd_profile_read(mhd_ctx *ctx, int kernel, double *ms, int *launches, const char **name);
-/* device bytes held by the context */
+/* device bytes of state + geometry held by the context */
 int mhd_memory(mhd_ctx *ctx, double *bytes);
+/* workgroup height (rows of 64 threads) the tuner chose for a kernel index
+   of mhd_profile_read */
+int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows);
 
 /* ---- host-only case builder (no device needed) ------------------------
    The same grid / topology / geometry / weights / initial-state builders the

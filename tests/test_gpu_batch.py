@@ -1,0 +1,88 @@
+"""Batching and serving on the GPU: every case of a mixed batch, and every
+case admitted into a running context, evolves exactly as it does alone."""
+import numpy as np
+import pytest
+
+from beta1_amd import MhdContext, WindDesc, OTDesc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U_H", "B_xi", "B_eta", "B_zeta")
+
+
+def alone(desc, steps, **kw):
+    ctx = MhdContext(**kw)
+    cid = ctx.add(desc)
+    ctx.finalize()
+    ctx.step(steps)
+    nb = ctx.info(cid)[0]
+    out = {(f, b): ctx.field(cid, f, b) for f in FIELDS for b in range(nb)}
+    ctx.close()
+    return out
+
+
+def same(ctx, cid, ref):
+    for (f, b), a in ref.items():
+        assert np.array_equal(ctx.field(cid, f, b), a), (cid, f, b)
+
+
+DESCS = [WindDesc(nang=8, nrad=16, v_kms=350.0), WindDesc(nang=12, nrad=16, n_cc=12.0),
+         WindDesc(nang=8, nrad=24, B_nT=(1.0, -4.0, 3.0)), OTDesc(n=12)]
+
+
+def test_mixed_batch_each_case_as_alone():
+    ctx = MhdContext(autotune=True)
+    ids = [ctx.add(d) for d in DESCS]
+    ctx.finalize()
+    ctx.step(5, graph=True)
+    for d, cid in zip(DESCS, ids):
+        same(ctx, cid, alone(d, 5))
+
+
+def test_admission_and_retirement_mid_run():
+    a, b, c = DESCS[0], DESCS[1], DESCS[3]
+    ctx = MhdContext()
+    ia = ctx.add(a)
+    ctx.finalize()
+    ctx.step(3)                       # odd: the running case sits in the second U buffer
+    ib = ctx.try_add(b)
+    ctx.step(4, graph=True)
+    ic = ctx.try_add(c)
+    ctx.step(2)
+    assert ctx.case_steps(ia) == 9 and ctx.case_steps(ib) == 6 and ctx.case_steps(ic) == 2
+    same(ctx, ia, alone(a, 9))
+    same(ctx, ib, alone(b, 6))
+    ctx.retire(ia)
+    ctx.step(2)
+    same(ctx, ib, alone(b, 8))
+    same(ctx, ic, alone(c, 4))
+
+
+def test_admission_respects_hbm_budget():
+    probe = MhdContext()
+    probe.add(DESCS[0])
+    probe.finalize()
+    one = probe.memory_bytes()
+    probe.close()
+    ctx = MhdContext(hbm_budget_gb=1.5 * one / 1e9)
+    ctx.finalize()
+    first = ctx.try_add(DESCS[0])
+    assert first is not None
+    assert ctx.try_add(DESCS[2]) is None      # a second grid shape does not fit
+    second = ctx.try_add(DESCS[0])            # same shape shares geometry: fits
+    assert second is not None
+    ctx.retire(first)
+    assert ctx.try_add(DESCS[2]) is None
+    ctx.retire(second)
+    assert ctx.try_add(DESCS[2]) is not None
+
+
+def test_xcd_order_and_tuning_do_not_change_results():
+    d = DESCS[1]
+    ref = alone(d, 4, xcd_order=False)
+    ctx = MhdContext(autotune=True, xcd_order=True)
+    cid = ctx.add(d)
+    ctx.finalize()
+    ctx.step(4)
+    same(ctx, cid, ref)
+    assert set(ctx.tuning().values()) <= {1, 2, 4}
