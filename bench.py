@@ -41,9 +41,11 @@ def measured_traffic(workload, kernel):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     if not os.path.exists(path):
         return None
+    import re
     for line in open(path):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) >= 5 and cols[0].endswith(kernel) and cols[0].split()[-1].split("<")[0] == kernel:
+        m = re.search(r"\b(k_\w+)", cols[0]) if cols else None
+        if len(cols) >= 5 and m and m.group(1) == kernel:
             try:
                 return float(cols[4]) * 1e9
             except ValueError:
@@ -83,7 +85,7 @@ def main():
     desc = WindDesc(nang=args.nang, nrad=args.nrad)
     cells = desc.cells
     t0 = time.time()
-    ctx = MhdContext(precision=args.precision, recon=0, ghost=2)
+    ctx = MhdContext(precision=args.precision, recon=0, ghost=2, autotune=True, xcd_order=True)
     cid = ctx.add(desc)
     ctx.finalize()
     setup_s = time.time() - t0
@@ -115,6 +117,7 @@ def main():
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     kernels_txt = {k: round(v[0], 4) for k, v in per_kernel.items()}
     print(f"[bench] per-kernel avg ms: {kernels_txt}", file=sys.stderr, flush=True)
+    print(f"[bench] tuned workgroup rows: {ctx.tuning()}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
