@@ -33,6 +33,170 @@ KERNEL_BYTES = {
     "k_derived": 24 + 24 + 24 + 24 + 144 + 288 + 40 + 72 + 8 + 24 + 24 + 24,
 }
 HBM_PEAK_GBS = 8000.0
+# Whole-step algorithmic bytes per cell-update, each array once per step:
+# U R/W 80 + B_face R/W 48 + Badd R 24 (per case) + geometry Jac 8, JD 72,
+# B_cell weights 144, J_cell weights 288, beta 24, alpha 24 (read once per
+# case: no kernel shares a geometry tile between cases).
+STEP_BYTES = 152 + 560
+SEED = 20261203
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _record(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(ms, 4) if ms is not None else None, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def batch_descs(n, seed=SEED):
+    """Seeded shells: v~U[300,700] km/s, n~U[1,20] cm^-3, |B|~U[2,10] nT with
+    the IMF direction uniform on the sphere; angular size from {32,48,64},
+    radial from {64,96,128}, independently."""
+    import numpy as np
+    from beta1_amd import WindDesc
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nang = int(rng.choice([32, 48, 64]))
+        nrad = int(rng.choice([64, 96, 128]))
+        v = float(rng.uniform(300, 700))
+        ncc = float(rng.uniform(1, 20))
+        bmag = float(rng.uniform(2, 10))
+        z = float(rng.uniform(-1, 1))
+        phi = float(rng.uniform(0, 2 * np.pi))
+        s = (1 - z * z) ** 0.5
+        out.append(WindDesc(nang=nang, nrad=nrad, v_kms=v, n_cc=ncc,
+                            B_nT=(bmag * s * np.cos(phi), bmag * s * np.sin(phi), bmag * z)))
+    return out
+
+
+def bench_batch(args):
+    """64 seeded shells of mixed sizes advanced together in one context
+    (mixed precision, graph-captured steps, tuned workgroups)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from beta1_amd import HostCase, MhdContext
+
+    descs = batch_descs(args.cases)
+    cells = sum(d.cells for d in descs)
+    prec = args.precision
+    t0 = time.time()
+    ctx = MhdContext(precision=prec, autotune=True, xcd_order=True)
+    # the largest case first: the context is finalized (and its workgroup
+    # shapes tuned) on it, the others are admitted live as their host set-up
+    # completes, so host arrays are released case by case
+    order = sorted(range(len(descs)), key=lambda i: -descs[i].cells)
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        futs = [pool.submit(HostCase, descs[i]) for i in order]
+        for n, f in enumerate(futs):
+            hc = f.result()
+            if ctx.adopt(hc) is None:
+                raise RuntimeError("batch does not fit the device")
+            futs[n] = None
+            del hc
+            if n == 0:
+                ctx.finalize()
+    log(f"batch set-up {time.time() - t0:.1f}s: {len(descs)} cases, {cells} cells, "
+        f"{ctx.memory_bytes() / 1e9:.1f} GB on device")
+    ctx.step(args.warmup, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(args.steps, graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    value = cells * args.steps / el / 1e6
+    log(f"batch: {args.steps} steps in {el:.2f}s, {value:.1f} Mcell-updates/s")
+    hbm = value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9)
+    return _record("Mcell-updates/s (64-case batch)", value, "Mcell-updates/s", args.steps, args.warmup,
+                   1e3 * el / args.steps, "f32/f64" if prec == "mixed" else "f64",
+                   {"workload": f"batch_{len(descs)}_shells_{prec}", "cases": len(descs), "cells": cells,
+                    "seed": SEED, "graph": True, "autotune": ctx.tuning()},
+                   step_hbm={"bytes_per_cell_update": STEP_BYTES, "frac": round(hbm, 4)})
+
+
+def bench_serve(args):
+    """Poisson request stream (70% Lunar, 30% Mercury shells of 6x32^2x64 ..
+    6x128^2x256 cells, `--req-steps` steps each) served by continuous batching
+    under an HBM budget; latency from arrival to result on the host."""
+    import numpy as np
+    from beta1_amd.serve import ServingEngine, poisson_requests
+
+    prec = args.precision
+    # capacity estimate from the request mix and a short calibration batch
+    probe = poisson_requests(64, 1.0, seed=SEED, steps=args.req_steps)
+    mean_work = float(np.mean([r.cells for r in probe])) * args.req_steps
+    eng = ServingEngine(precision=prec, hbm_budget_gb=args.budget_gb, chunk=20, workers=2)
+    from beta1_amd import HostCase, WindDesc
+    cal = eng.ctx.adopt(HostCase(WindDesc(nang=64, nrad=128)))
+    eng.ctx.step(4, graph=True)
+    eng.ctx.sync()
+    t0 = time.perf_counter()
+    eng.ctx.step(20, graph=True)
+    eng.ctx.sync()
+    cap = WindDesc(nang=64, nrad=128).cells * 20 / (time.perf_counter() - t0)  # cell-updates/s
+    eng.ctx.retire(cal)
+    rate = args.load * cap / mean_work
+    reqs = poisson_requests(args.requests, rate, seed=SEED, steps=args.req_steps)
+    log(f"serve: capacity ~{cap / 1e6:.0f} Mcell-updates/s, mean request {mean_work / 1e9:.2f} G cell-updates, "
+        f"rate {rate:.3f}/s (offered load {args.load})")
+    res = eng.run(reqs, log=log)
+    eng.close()
+    offered = rate * float(np.mean([r.cells for r in reqs])) * args.req_steps / cap
+    return _record("p50/p99 solve latency + Mcell-updates/s (serving)", res["mcell_updates_per_s"],
+                   "Mcell-updates/s", args.req_steps, 0, None, "f32/f64" if prec == "mixed" else "f64",
+                   {"workload": f"serve_poisson_{args.requests}req_{prec}", "requests": args.requests,
+                    "rate_hz": round(rate, 4), "offered_load": round(offered, 3), "mix": "70% Lunar / 30% Mercury",
+                    "steps_per_request": args.req_steps, "hbm_budget_gb": args.budget_gb, "seed": SEED},
+                   latency={"p50_s": round(res["p50_s"], 3), "p99_s": round(res["p99_s"], 3),
+                            "wall_s": round(res["wall_s"], 2)})
+
+
+def bench_precision(args):
+    """FP64 and mixed precision on the same shell: throughput of each and the
+    per-variable relative L-inf / L2 of mixed against FP64 (the FP64 path is
+    bit-identical to the oracle), plus max |div B| of both."""
+    import numpy as np
+    from beta1_amd import MhdContext, WindDesc
+
+    nang, nrad, steps = args.nang, args.nrad, args.steps
+    desc = WindDesc(nang=nang, nrad=nrad)
+    res, rate, divb = {}, {}, {}
+    for prec in ("fp64", "mixed"):
+        ctx = MhdContext(precision=prec, autotune=True)
+        cid = ctx.add(desc)
+        ctx.finalize()
+        ctx.step(2, graph=True)  # graph capture outside the timed region; 'steps' in total
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps - 2, graph=True)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        rate[prec] = desc.cells * (steps - 2) / el / 1e6
+        divb[prec] = ctx.divb_max(cid)
+        res[prec] = {(f, b): ctx.field(cid, f, b) for f in ("U_H", "B_xi", "B_eta", "B_zeta") for b in range(6)}
+        log(f"precision {prec}: {steps} steps in {el:.2f}s, {rate[prec]:.1f} Mcell-updates/s, div B {divb[prec]:.2e}")
+        ctx.close()
+    from beta1_amd.fields import interior
+    names = {"U_H": ["rho", "rho_u", "rho_v", "rho_w", "E"], "B_xi": ["B_xi"], "B_eta": ["B_eta"],
+             "B_zeta": ["B_zeta"]}
+    err = {}
+    n = (nang, nang, nrad)
+    for f, labels in names.items():
+        for m, lab in enumerate(labels):
+            a = np.concatenate([res["mixed"][(f, b)][interior(f, n, 2)][..., m].ravel() for b in range(6)])
+            r = np.concatenate([res["fp64"][(f, b)][interior(f, n, 2)][..., m].ravel() for b in range(6)])
+            d = a - r
+            err[lab] = {"rel_linf": float(np.abs(d).max() / max(np.abs(r).max(), 1e-300)),
+                        "rel_l2": float(np.sqrt(np.sum(d * d) / max(np.sum(r * r), 1e-300)))}
+    return _record("Mcell-updates/s FP64 vs mixed", rate["mixed"], "Mcell-updates/s", steps, 0, None, "f32/f64",
+                   {"workload": f"precision_lunar_6x{nang}x{nang}x{nrad}", "cells": desc.cells},
+                   fp64_value=round(rate["fp64"], 3), errors_mixed_vs_fp64=err,
+                   divb_max={k: float(v) for k, v in divb.items()})
 
 
 def measured_traffic(workload, kernel):
@@ -56,16 +220,31 @@ def measured_traffic(workload, kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; batch 100; precision 500)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--nang", type=int, default=64)
-    ap.add_argument("--nrad", type=int, default=128)
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default fp64 for the single shell, mixed for batch and serve")
+    ap.add_argument("--nang", type=int, default=None, help="default 64 (precision: 128)")
+    ap.add_argument("--nrad", type=int, default=None, help="default 128 (precision: 256)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy five times (counter calibration)")
+    ap.add_argument("--workload", default="lunar", choices=["lunar", "batch", "serve", "precision"],
+                    help="lunar: one 6-block shell (default); batch: 64 seeded shells of mixed sizes in one "
+                         "context; serve: Poisson request stream with continuous batching; precision: FP64 vs "
+                         "mixed on one shell")
+    ap.add_argument("--cases", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--req-steps", type=int, default=1000)
+    ap.add_argument("--load", type=float, default=0.8, help="offered load of the request stream")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
+    big = args.workload == "precision"
+    args.nang = args.nang or (128 if big else 64)
+    args.nrad = args.nrad or (256 if big else 128)
+    args.steps = args.steps or {"lunar": 200, "batch": 100, "serve": 0, "precision": 500}[args.workload]
+    args.precision = args.precision or ("fp64" if args.workload == "lunar" else "mixed")
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -79,6 +258,11 @@ def main():
         print("bench.py: this build measures one GPU only (the step shards by case or by block across GPUs, "
               "which is not built here); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "lunar":
+        out = {"batch": bench_batch, "serve": bench_serve, "precision": bench_precision}[args.workload](args)
+        print(json.dumps(out), flush=True)
+        return
 
     from beta1_amd import MhdContext, WindDesc
 
@@ -158,6 +342,8 @@ def main():
                      "traffic": measured_traffic(f"lunar_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}", dom),
                      "alg_bytes_per_cell": KERNEL_BYTES[dom], "avg_ms": round(avg_ms, 4)},
         "cpu_baseline": cpu,
+        "step_hbm": {"bytes_per_cell_update": STEP_BYTES,
+                     "frac": round(value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4)},
     }
     print(json.dumps(out), flush=True)
 

@@ -25,6 +25,10 @@ class MhdWindDesc(C.Structure):
                 ("vdir", C.c_double * 3), ("T_eV", C.c_double), ("B_nT", C.c_double * 3), ("m_amu", C.c_double)]
 
 
+class MhdDipoleDesc(C.Structure):
+    _fields_ = [("B_eq_nT", C.c_double), ("offset_z", C.c_double), ("softening", C.c_double)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -39,6 +43,8 @@ SIGNATURES = {
     "mhd_last_error": (C.c_char_p, [_P]),
     "mhd_add_case_orszag_tang": (_I, [_P, C.POINTER(MhdOTDesc), _PI]),
     "mhd_add_case_lunar": (_I, [_P, C.POINTER(MhdWindDesc), _PI]),
+    "mhd_add_case_mercury": (_I, [_P, C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _PI]),
+    "mhd_adopt_host_case": (_I, [_P, _P, _PI]),
     "mhd_finalize": (_I, [_P]),
     "mhd_step": (_I, [_P, _I]),
     "mhd_step_graph": (_I, [_P, _I]),
@@ -61,6 +67,7 @@ SIGNATURES = {
     "mhd_tuning": (_I, [_P, _I, _PI]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
+    "mhd_host_case_mercury": (_I, [C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_destroy": (_I, [_P]),
     "mhd_host_case_info": (_I, [_P, _PI, _PI, _PI, _PI, _PI]),
     "mhd_host_field_count": (_I, [_P, _I, C.POINTER(_S)]),

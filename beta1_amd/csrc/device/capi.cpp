@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../../include/beta1_mhd.h"
+#include "../host/host_case.h"
 #include "../host/setup.h"
 #include "../layout.h"
 #include "kernels.h"
@@ -608,7 +609,10 @@ static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
         return rc;
     if ((rc = place_case(c, *c->cases[id])))
     {
-        c->cases[id]->state = 2;
+        if (rc == MHD_ERR_MEM) // nothing was placed: the id is free again
+            c->cases.pop_back();
+        else
+            c->cases[id]->state = 2;
         return rc;
     }
     if ((rc = rebuild(c)))
@@ -634,27 +638,62 @@ extern "C" int mhd_add_case_orszag_tang(mhd_ctx *c, const mhd_ot_desc *d, int *c
 
 extern "C" int mhd_add_case_lunar(mhd_ctx *c, const mhd_wind_desc *d, int *case_id)
 {
-    if (!c || !d || d->nang < 4 || d->nrad < 4)
+    if (!c || !wind_desc_ok(d))
         return MHD_ERR_ARG;
     try
     {
-        WindParams w;
-        w.n_cc = d->n_cc;
-        w.v_kms = d->v_kms;
-        for (int a = 0; a < 3; ++a)
-        {
-            w.vdir[a] = d->vdir[a];
-            w.B_nT[a] = d->B_nT[a];
-        }
-        w.T_eV = d->T_eV;
-        w.m_amu = d->m_amu > 0 ? d->m_amu : 1.0;
-        return add_case(c, make_case_lunar(d->nang, d->nrad, d->r_in, d->r_out, c->opt.ghost, d->gamma, d->cfl, w),
+        return add_case(c,
+                        make_case_lunar(d->nang, d->nrad, d->r_in, d->r_out, c->opt.ghost, d->gamma, d->cfl,
+                                        wind_params(*d)),
                         case_id);
     }
     catch (const std::exception &e)
     {
         return fail(c, MHD_ERR_ARG, e.what());
     }
+}
+
+extern "C" int mhd_add_case_mercury(mhd_ctx *c, const mhd_wind_desc *d, const mhd_dipole_desc *dp, int *case_id)
+{
+    if (!c || !wind_desc_ok(d) || !dp || dp->softening < 0)
+        return MHD_ERR_ARG;
+    try
+    {
+        return add_case(c,
+                        make_case_mercury(d->nang, d->nrad, d->r_in, d->r_out, c->opt.ghost, d->gamma, d->cfl,
+                                          wind_params(*d), dipole_params(*dp)),
+                        case_id);
+    }
+    catch (const std::exception &e)
+    {
+        return fail(c, MHD_ERR_ARG, e.what());
+    }
+}
+
+// Bytes a case would add to the context (its geometry only when no resident
+// case shares the grid shape).
+static size_t case_bytes(const mhd_ctx *c, const CaseSetup &cs)
+{
+    const size_t slot = (size_t)cs.blk[0].d.slot, nb = cs.blk.size();
+    size_t need = sizeof(double) * ST_NSLOT * slot * nb;
+    bool shared = false;
+    for (const GeoSet &G : c->geos)
+        shared = shared || (G.refs > 0 && G.key == cs.shape_key);
+    if (!shared)
+        need += sizeof(double) * G_NSLOT * slot * nb;
+    return need;
+}
+
+extern "C" int mhd_adopt_host_case(mhd_ctx *c, mhd_host_case *h, int *case_id)
+{
+    if (!c || !h || h->cs.blk.empty())
+        return MHD_ERR_ARG;
+    if (h->cs.blk[0].d.g != c->opt.ghost)
+        return fail(c, MHD_ERR_ARG, "host case built with another ghost width");
+    const double budget = c->opt.hbm_budget_gb * 1e9;
+    if (c->finalized && budget > 0 && (double)(c->bytes_used + case_bytes(c, h->cs)) > budget)
+        return fail(c, MHD_ERR_MEM, "HBM budget exceeded"); // the host case stays intact
+    return add_case(c, std::move(h->cs), case_id);
 }
 
 extern "C" int mhd_finalize(mhd_ctx *c)

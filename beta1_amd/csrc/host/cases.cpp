@@ -270,4 +270,131 @@ CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, 
     return cs;
 }
 
+// Mercury-configured shell: reference values of Z5_Mercury/9999setup
+// (REF U=4.031e5 m/s, T=1.7407e5 K, n=4e7 m^-3, M=1e-3 kg/mol, B=1 nT,
+// L=2.44e6 m), an offset dipole added to Badd through the curl of its edge
+// circulation (Z5_Mercury/include/operators/Dipole.h:89-333, vector potential
+// A = m x dr / (|dr|^2 + a^2)^(3/2)), plus the IMF flux.  The body is the
+// absorbing wall of the Lunar set-up; the second ion species, Hall,
+// ambipolar and resistive terms of the reference's Mercury step are not part
+// of this single-fluid explicit step.
+CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
+                            const WindParams &w, const DipoleParams &dp)
+{
+    CaseSetup cs;
+    cs.grid = make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD);
+    finalize_grid(cs);
+    char key[128];
+    std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
+    cs.shape_key = key;
+
+    struct
+    {
+        double U = 4.031e5, T = 1.7407e5, n = 4e7, M = 0.001, B = 1e-9, L = 2.44e6;
+        double NA = 6.02e23, R = 8.314, mu0 = 1.256637061e-6;
+    } R;
+    MhdCaseConst &c = cs.cst;
+    const double rho_ref = R.M * R.n / R.NA;
+    const double M_H = 0.001 * w.m_amu;
+    c.gamma = gamma;
+    c.cfl = cfl;
+    c.inver_MA2 = R.B * R.B / (R.U * R.U * R.mu0 * rho_ref);
+    c.momentum_hall_coeff = (R.B * R.B) / (R.mu0 * rho_ref * R.U * R.U);
+    c.M_H = M_H;
+    c.M_ref = R.M;
+    c.ne_floor = 0.01;
+    c.state_coeff = (R.R * R.T) / (M_H * R.U * R.U);
+    c.dt_floor = 1e-7;
+    const double k_B = R.R / R.NA;
+    const double n_m3 = w.n_cc * 1e6;
+    const double T_K = w.T_eV * 11604.518;
+    const double rho0 = (n_m3 / R.n) * (M_H / R.M);
+    const double V = w.v_kms * 1e3;
+    const double u0 = w.vdir[0] * V / R.U, v0 = w.vdir[1] * V / R.U, w0 = w.vdir[2] * V / R.U;
+    const double p0 = n_m3 * k_B * T_K / (rho_ref * R.U * R.U);
+    c.qinf[0] = rho0;
+    c.qinf[1] = rho0 * u0;
+    c.qinf[2] = rho0 * v0;
+    c.qinf[3] = rho0 * w0;
+    c.qinf[4] = 0.5 * rho0 * (u0 * u0 + v0 * v0 + w0 * w0) + p0 / (gamma - 1.0);
+    c.u_inf[0] = u0;
+    c.u_inf[1] = v0;
+    c.u_inf[2] = w0;
+    for (int a = 0; a < 3; ++a)
+        c.B_imf[a] = w.B_nT[a] * 1e-9 / R.B;
+
+    build_ops(cs);
+
+    // nondimensional dipole: |B| = |m| / r^3 at the equator of the body
+    const double m[3] = {0.0, 0.0, -dp.B_eq_nT * 1e-9 / R.B};
+    const double r0[3] = {0.0, 0.0, dp.offset_z};
+    const double a2 = dp.softening * dp.softening;
+    auto A_of = [&](const double *x, double *A) {
+        const double d[3] = {x[0] - r0[0], x[1] - r0[1], x[2] - r0[2]};
+        double f = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + a2;
+        f = std::sqrt(f) * f;
+        A[0] = (m[1] * d[2] - m[2] * d[1]) / f;
+        A[1] = (m[2] * d[0] - m[0] * d[2]) / f;
+        A[2] = (m[0] * d[1] - m[1] * d[0]) / f;
+    };
+    for (size_t b = 0; b < cs.blk.size(); ++b)
+    {
+        BlockArrays &A = cs.blk[b];
+        const Block &B = cs.grid.blocks[b];
+        const MhdBlockDev &d = A.d;
+        for (int q = 0; q < 5; ++q)
+            for (int64_t o = 0; o < d.slot; ++o)
+                A.st[(size_t)(ST_U0 + q) * d.slot + o] = c.qinf[q];
+        // IMF flux everywhere (the outermost ghost faces keep only this part)
+        for (int a = 0; a < 3; ++a)
+            for (int64_t o = 0; o < d.slot; ++o)
+            {
+                const double sx = A.geo[(size_t)(G_JD + 3 * a + 0) * d.slot + o];
+                const double sy = A.geo[(size_t)(G_JD + 3 * a + 1) * d.slot + o];
+                const double sz = A.geo[(size_t)(G_JD + 3 * a + 2) * d.slot + o];
+                A.st[(size_t)(ST_BADD + a) * d.slot + o] = c.B_imf[0] * sx + c.B_imf[1] * sy + c.B_imf[2] * sz;
+            }
+        // edge circulation of the dipole potential (midpoint rule), boxes [-g, n+g]
+        std::vector<double> Ed[3];
+        for (int e = 0; e < 3; ++e)
+            Ed[e].assign((size_t)d.slot, 0.0);
+        for (int i = -g; i <= B.ni + g; ++i)
+            for (int j = -g; j <= B.nj + g; ++j)
+                for (int k = -g; k <= B.nk + g; ++k)
+                    for (int e = 0; e < 3; ++e)
+                    {
+                        const int64_t o0 = B.node_off(i, j, k);
+                        const int64_t o1 = B.node_off(i + (e == 0), j + (e == 1), k + (e == 2));
+                        const double x1[3] = {B.x[o0], B.y[o0], B.z[o0]}, x2[3] = {B.x[o1], B.y[o1], B.z[o1]};
+                        const double xm[3] = {0.5 * (x1[0] + x2[0]), 0.5 * (x1[1] + x2[1]), 0.5 * (x1[2] + x2[2])};
+                        double Av[3];
+                        A_of(xm, Av);
+                        Ed[e][blk_off(d, i, j, k)] =
+                            Av[0] * (x2[0] - x1[0]) + Av[1] * (x2[1] - x1[1]) + Av[2] * (x2[2] - x1[2]);
+                    }
+        const int64_t di = (int64_t)d.sj * d.sk, dj = d.sk, dk = 1;
+        for (int i = -g; i < B.ni + g; ++i)
+            for (int j = -g; j < B.nj + g; ++j)
+                for (int k = -g; k < B.nk + g; ++k)
+                {
+                    const int64_t o = blk_off(d, i, j, k);
+                    // curl of the edge 1-form (CurlEdgeToFace with multiper +1)
+                    const double fx = (Ed[1][o] - Ed[1][o + dk]) + (Ed[2][o + dj] - Ed[2][o]);
+                    const double fy = (Ed[0][o + dk] - Ed[0][o]) + (Ed[2][o] - Ed[2][o + di]);
+                    const double fz = (Ed[0][o] - Ed[0][o + dj]) + (Ed[1][o + di] - Ed[1][o]);
+                    const double f[3] = {fx, fy, fz};
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        const double sx = A.geo[(size_t)(G_JD + 3 * a + 0) * d.slot + o];
+                        const double sy = A.geo[(size_t)(G_JD + 3 * a + 1) * d.slot + o];
+                        const double sz = A.geo[(size_t)(G_JD + 3 * a + 2) * d.slot + o];
+                        A.st[(size_t)(ST_BADD + a) * d.slot + o] =
+                            f[a] + (c.B_imf[0] * sx + c.B_imf[1] * sy + c.B_imf[2] * sz);
+                    }
+                }
+    }
+    apply_ops_host(cs.blk, face_copy_sync(cs, ST_BADD), c);
+    return cs;
+}
+
 } // namespace mpcns

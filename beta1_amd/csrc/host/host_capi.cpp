@@ -7,19 +7,41 @@
 #include <stdexcept>
 #include <string>
 
-#include "../../../include/beta1_mhd.h"
-#include "setup.h"
+#include "host_case.h"
 
 using namespace mpcns;
 
-struct mhd_host_case
-{
-    CaseSetup cs;
-    std::string err;
-};
-
 namespace mpcns
 {
+WindParams wind_params(const mhd_wind_desc &d)
+{
+    WindParams w;
+    w.n_cc = d.n_cc;
+    w.v_kms = d.v_kms;
+    for (int a = 0; a < 3; ++a)
+    {
+        w.vdir[a] = d.vdir[a];
+        w.B_nT[a] = d.B_nT[a];
+    }
+    w.T_eV = d.T_eV;
+    w.m_amu = d.m_amu > 0 ? d.m_amu : 1.0;
+    return w;
+}
+
+DipoleParams dipole_params(const mhd_dipole_desc &d)
+{
+    DipoleParams p;
+    p.B_eq_nT = d.B_eq_nT;
+    p.offset_z = d.offset_z;
+    p.softening = d.softening;
+    return p;
+}
+
+bool wind_desc_ok(const mhd_wind_desc *d)
+{
+    return d && d->nang >= 4 && d->nrad >= 4 && d->r_in > 0 && d->r_out > d->r_in;
+}
+
 // Field number -> (region, slot, location, components) in the device layout.
 bool field_slot(int f, int ucur, int &region, int &slot, int &loc, int &nc)
 {
@@ -144,22 +166,31 @@ extern "C" int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case *
 
 extern "C" int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_case **out)
 {
-    if (!d || !out || d->nang < 4 || d->nrad < 4 || ghost < 1)
+    if (!wind_desc_ok(d) || !out || ghost < 1)
         return MHD_ERR_ARG;
     try
     {
-        WindParams w;
-        w.n_cc = d->n_cc;
-        w.v_kms = d->v_kms;
-        for (int a = 0; a < 3; ++a)
-        {
-            w.vdir[a] = d->vdir[a];
-            w.B_nT[a] = d->B_nT[a];
-        }
-        w.T_eV = d->T_eV;
-        w.m_amu = d->m_amu > 0 ? d->m_amu : 1.0;
         auto h = std::make_unique<mhd_host_case>();
-        h->cs = make_case_lunar(d->nang, d->nrad, d->r_in, d->r_out, ghost, d->gamma, d->cfl, w);
+        h->cs = make_case_lunar(d->nang, d->nrad, d->r_in, d->r_out, ghost, d->gamma, d->cfl, wind_params(*d));
+        *out = h.release();
+        return MHD_OK;
+    }
+    catch (const std::exception &)
+    {
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_case_mercury(const mhd_wind_desc *d, const mhd_dipole_desc *dp, int ghost,
+                                     mhd_host_case **out)
+{
+    if (!wind_desc_ok(d) || !dp || !out || ghost < 1 || dp->softening < 0)
+        return MHD_ERR_ARG;
+    try
+    {
+        auto h = std::make_unique<mhd_host_case>();
+        h->cs = make_case_mercury(d->nang, d->nrad, d->r_in, d->r_out, ghost, d->gamma, d->cfl, wind_params(*d),
+                                  dipole_params(*dp));
         *out = h.release();
         return MHD_OK;
     }

@@ -184,6 +184,16 @@ struct WindParams
     double m_amu;      // ion mass [amu] (H+ = 1)
 };
 
+// Offset dipole of the Mercury set-up (nondimensional lengths).
+struct DipoleParams
+{
+    double B_eq_nT;   // equatorial surface field of the dipole [nT]
+    double offset_z;  // dipole centre along +z [body radii]
+    double softening; // core softening length [body radii]
+};
+
+CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
+                            const WindParams &w, const DipoleParams &dp);
 CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t seed, double noise);
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma,
                           double cfl, const WindParams &w);

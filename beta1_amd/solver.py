@@ -10,7 +10,7 @@ from dataclasses import dataclass, field as dfield
 
 import numpy as np
 
-from .capi import MhdError, MhdOptions, MhdOTDesc, MhdWindDesc, lib
+from .capi import MhdDipoleDesc, MhdError, MhdOptions, MhdOTDesc, MhdWindDesc, lib
 from .fields import FIELD_ID, shape
 
 PREC = {"fp64": 0, "mixed": 1}
@@ -60,6 +60,22 @@ class WindDesc:
         return 6 * self.nang * self.nang * self.nrad
 
 
+@dataclass
+class MercuryDesc(WindDesc):
+    """Mercury-configured shell: solar wind at 0.3-0.47 AU plus an offset
+    dipole (190 nT equatorial surface field, centre +0.2 R_M along z)."""
+    r_out: float = 10.0
+    n_cc: float = 40.0
+    T_eV: float = 15.0
+    B_nT: tuple = (-15.2, 8.4, -8.5)
+    B_eq_nT: float = 190.0
+    offset_z: float = 0.2
+    softening: float = 0.05
+
+    def dipole(self):
+        return MhdDipoleDesc(self.B_eq_nT, self.offset_z, self.softening)
+
+
 def _check(rc, ctx=None, what=""):
     if rc != 0:
         msg = ""
@@ -80,8 +96,12 @@ class HostCase:
         self.h = C.c_void_p()
         if isinstance(desc, OTDesc):
             _check(lib().mhd_host_case_ot(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_ot")
+        elif isinstance(desc, MercuryDesc):
+            _check(lib().mhd_host_case_mercury(C.byref(desc.c()), C.byref(desc.dipole()), ghost, C.byref(self.h)),
+                   what="mhd_host_case_mercury")
         else:
             _check(lib().mhd_host_case_lunar(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_lunar")
+        self.desc = desc
         nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
         lib().mhd_host_case_info(self.h, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g))
         self.nblocks, self.n, self.g = nb.value, (ni.value, nj.value, nk.value), g.value
@@ -149,13 +169,16 @@ class MhdContext:
     def __del__(self):
         self.close()
 
+    def _add(self, desc, cid):
+        if isinstance(desc, OTDesc):
+            return lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
+        if isinstance(desc, MercuryDesc):
+            return lib().mhd_add_case_mercury(self.ctx, C.byref(desc.c()), C.byref(desc.dipole()), C.byref(cid))
+        return lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
+
     def add(self, desc):
         cid = C.c_int()
-        if isinstance(desc, OTDesc):
-            rc = lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
-        else:
-            rc = lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
-        _check(rc, self.ctx, "add case")
+        _check(self._add(desc, cid), self.ctx, "add case")
         self.cases.append(desc)
         return cid.value
 
@@ -165,14 +188,22 @@ class MhdContext:
     def try_add(self, desc):
         """Admit a case into the running context; None when the HBM budget is full."""
         cid = C.c_int()
-        if isinstance(desc, OTDesc):
-            rc = lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
-        else:
-            rc = lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
+        rc = self._add(desc, cid)
         if rc == -5:
             return None
         _check(rc, self.ctx, "admit case")
         self.cases.append(desc)
+        return cid.value
+
+    def adopt(self, host_case):
+        """Move a ``HostCase`` into the context; None (host case intact) when
+        the HBM budget is full."""
+        cid = C.c_int()
+        rc = lib().mhd_adopt_host_case(self.ctx, host_case.h, C.byref(cid))
+        if rc == -5:
+            return None
+        _check(rc, self.ctx, "adopt case")
+        self.cases.append(host_case.desc)
         return cid.value
 
     def retire(self, cid):

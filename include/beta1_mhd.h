@@ -78,6 +78,16 @@ typedef struct
     double n_cc, v_kms, vdir[3], T_eV, B_nT[3], m_amu;
 } mhd_wind_desc;
 
+/* Planetary dipole of the Mercury set-up: moment along -z with equatorial
+   surface field B_eq_nT, centre offset_z body radii along +z, core softening
+   length (body radii) in the vector potential. */
+typedef struct
+{
+    double B_eq_nT, offset_z, softening;
+} mhd_dipole_desc;
+
+typedef struct mhd_host_case mhd_host_case;
+
 /* Context lifetime.  Replaces the solver constructor's set-up
    (LunarSolver::LunarSolver, Z4_Lunar/src/LunarSolver.cpp:12-235). */
 int mhd_create(const mhd_options *opt, mhd_ctx **out);
@@ -93,6 +103,21 @@ const char *mhd_last_error(mhd_ctx *ctx);
    and the initial state (Lunar_Initial::Initialization, Z4_Lunar/include/2_Initial.h:16). */
 int mhd_add_case_orszag_tang(mhd_ctx *ctx, const mhd_ot_desc *d, int *case_id);
 int mhd_add_case_lunar(mhd_ctx *ctx, const mhd_wind_desc *d, int *case_id);
+/* Mercury-configured shell (Z5_Mercury/main.cpp set-up with the reference
+   values of Z5_Mercury/9999setup): solar wind as for the Lunar case plus an
+   offset dipole in the background field Badd (DipoleField,
+   Z5_Mercury/include/operators/Dipole.h:89-333).  Single fluid: the step is
+   the shared explicit step (MercurySolver::StepOnce,
+   Z5_Mercury/src/MercurySolver_Driver.cpp:21-92, without the Na+ species,
+   Hall, ambipolar and resistive terms). */
+int mhd_add_case_mercury(mhd_ctx *ctx, const mhd_wind_desc *d, const mhd_dipole_desc *dp, int *case_id);
+/* Move a case built by mhd_host_case_* into the context (same effect as
+   mhd_add_case_*; the host set-up can run on another thread meanwhile).
+   On MHD_ERR_MEM the host case is left intact for a later attempt; on
+   success its arrays are consumed and only mhd_host_case_destroy remains
+   valid on it.  No counterpart in the reference (its set-up runs once, in
+   the solver constructor). */
+int mhd_adopt_host_case(mhd_ctx *ctx, mhd_host_case *h, int *case_id);
 /* Upload everything added so far and compute the derived fields
    (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143).
    After mhd_finalize, mhd_add_case_* admits a case into the running context
@@ -148,9 +173,9 @@ int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows);
 /* ---- host-only case builder (no device needed) ------------------------
    The same grid / topology / geometry / weights / initial-state builders the
    device context uses, run on the CPU and read back in the AoS layout. */
-typedef struct mhd_host_case mhd_host_case;
 int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case **out);
 int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_case **out);
+int mhd_host_case_mercury(const mhd_wind_desc *d, const mhd_dipole_desc *dp, int ghost, mhd_host_case **out);
 int mhd_host_case_destroy(mhd_host_case *h);
 int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
 int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);

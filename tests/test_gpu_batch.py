@@ -3,7 +3,7 @@ case admitted into a running context, evolves exactly as it does alone."""
 import numpy as np
 import pytest
 
-from beta1_amd import MhdContext, WindDesc, OTDesc
+from beta1_amd import MercuryDesc, MhdContext, OTDesc, WindDesc
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +75,32 @@ def test_admission_respects_hbm_budget():
     assert ctx.try_add(DESCS[2]) is None
     ctx.retire(second)
     assert ctx.try_add(DESCS[2]) is not None
+
+
+def test_serving_results_equal_alone():
+    """Continuous batching with a budget that holds about two cases: queued
+    requests wait, are admitted as others retire, and each result equals the
+    case run alone for the same number of steps."""
+    from beta1_amd.serve import Request, ServingEngine
+
+    descs = [WindDesc(nang=8, nrad=16, v_kms=420.0), MercuryDesc(nang=8, nrad=16),
+             WindDesc(nang=12, nrad=16, n_cc=9.0), MercuryDesc(nang=8, nrad=24, n_cc=30.0),
+             WindDesc(nang=8, nrad=16, B_nT=(2.0, -3.0, 1.0))]
+    probe = MhdContext()
+    probe.add(descs[2])
+    probe.finalize()
+    big = probe.memory_bytes()
+    probe.close()
+    reqs = [Request(i, d, steps=5 + 2 * i, arrival=0.02 * i) for i, d in enumerate(descs)]
+    eng = ServingEngine(precision="fp64", hbm_budget_gb=1.8 * big / 1e9, chunk=3, workers=2, keep_results=True)
+    out = eng.run(reqs)
+    eng.close()
+    assert out["requests"] == len(reqs)
+    for r in out["records"]:
+        assert r.admitted >= r.arrival and r.done >= r.admitted
+        ref = alone(r.desc, r.steps)
+        for key, a in ref.items():
+            assert np.array_equal(r.result[key], a), (r.rid, key)
 
 
 def test_xcd_order_and_tuning_do_not_change_results():

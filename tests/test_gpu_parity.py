@@ -8,7 +8,7 @@ round-off.
 import numpy as np
 import pytest
 
-from beta1_amd import HostCase, MhdContext, OTDesc, WindDesc
+from beta1_amd import HostCase, MercuryDesc, MhdContext, OTDesc, WindDesc
 from beta1_amd.fields import interior
 from oracle_bind import Oracle
 
@@ -73,6 +73,14 @@ def test_orszag_tang_minmod_matches_oracle():
 def test_lunar_shell_fp64_matches_oracle(nang, nrad, steps):
     desc = WindDesc(nang=nang, nrad=nrad)
     hc, orc, ctx, cid, dts = run_pair(desc, steps)
+    assert ctx.last_dt(cid) == dts[-1]
+    compare(ctx, cid, orc, 6, hc.n, hc.g)
+    assert ctx.divb_max(cid) < 1e-9
+
+
+def test_mercury_shell_fp64_matches_oracle():
+    desc = MercuryDesc(nang=8, nrad=16)
+    hc, orc, ctx, cid, dts = run_pair(desc, 8)
     assert ctx.last_dt(cid) == dts[-1]
     compare(ctx, cid, orc, 6, hc.n, hc.g)
     assert ctx.divb_max(cid) < 1e-9
