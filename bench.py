@@ -29,8 +29,10 @@ KERNEL_BYTES = {
     "k_edge_emf": 72 + 24 + 24 + 24,          # Eface, UC, B -> E
     "k_fluid": 40 + 72 + 8 + 24 + 24 + 40,    # U, JD, Jac, B_cell, J_cell -> U'
     "k_curl_update": 24 + 24 + 24,            # E, B -> B
-    # B, Badd, beta, alpha, Wb, Wj, U, JD, Jac -> B_cell, J_cell, UC (+ next step's dt)
-    "k_derived": 24 + 24 + 24 + 24 + 144 + 288 + 40 + 72 + 8 + 24 + 24 + 24,
+    # U, JD, Jac, B, Badd, W_B -> UC, B_cell (+ next step's dt)
+    "k_derived": 40 + 72 + 8 + 24 + 24 + 144 + 24 + 24,
+    # B, beta, alpha, W_J -> J_cell
+    "k_jcell": 24 + 24 + 24 + 288 + 24,
 }
 HBM_PEAK_GBS = 8000.0
 # Whole-step algorithmic bytes per cell-update, each array once per step:
@@ -101,6 +103,8 @@ def bench_batch(args):
             del hc
             if n == 0:
                 ctx.finalize()
+            if n % 8 == 7:
+                log(f"batch set-up: {n + 1} cases placed after {time.time() - t0:.1f}s")
     log(f"batch set-up {time.time() - t0:.1f}s: {len(descs)} cases, {cells} cells, "
         f"{ctx.memory_bytes() / 1e9:.1f} GB on device")
     ctx.step(args.warmup, graph=True)

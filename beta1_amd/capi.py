@@ -64,7 +64,7 @@ SIGNATURES = {
     "mhd_memory": (_I, [_P, _PD]),
     "mhd_retire": (_I, [_P, _I]),
     "mhd_case_steps": (_I, [_P, _I, C.POINTER(C.c_longlong)]),
-    "mhd_tuning": (_I, [_P, _I, _PI]),
+    "mhd_tuning": (_I, [_P, _I, _PI, _PI]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_mercury": (_I, [C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _I, C.POINTER(_P)]),

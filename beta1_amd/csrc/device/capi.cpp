@@ -25,14 +25,15 @@ namespace
 {
 struct CaseRec
 {
-    CaseSetup cs;                     // grid, topology, constants; arrays freed once placed
-    std::vector<MhdOp> ops[SY_COUNT]; // case-relative element offsets
+    CaseSetup cs; // grid, topology, constants; arrays and op lists freed once placed
+    int id = -1;
     int geo_set = -1;
     double *d_st = nullptr; // the case's state allocation
     int64_t slot = 0;
     int nblocks = 0;
     int state = 0;        // 0 pending, 1 active, 2 retired
     int first_block = -1; // index into the block table while active
+    size_t op_off[SY_COUNT] = {0}, op_n[SY_COUNT] = {0}; // segments in the op groups
     long long steps = 0;
     size_t st_bytes = 0;
 };
@@ -48,9 +49,17 @@ struct GeoSet
 
 struct TileArr
 {
-    int *d_tile = nullptr, *d_start = nullptr;
-    size_t cap_tile = 0, cap_start = 0;
+    int *d_start = nullptr;
+    size_t cap_start = 0;
     int n = 0;
+};
+
+// Op list of one sync group on the device: the cases' ops back to back,
+// each case's segment appended when it is placed and cut out when it retires.
+struct OpGroup
+{
+    MhdOp *d = nullptr;
+    size_t cap = 0, n = 0;
 };
 
 enum Range
@@ -62,6 +71,8 @@ enum Range
 };
 const int kRangeLo[R_COUNT] = {0, 0, -1}, kRangeHi[R_COUNT] = {0, 1, 1};
 const int kJt[3] = {1, 2, 4};
+// launch variants: workgroup rows kJt[v % 3], tile order i-inner if v >= 3
+const int kVariants = 6;
 
 enum KernelId
 {
@@ -74,11 +85,12 @@ enum KernelId
     KP_CURL,
     KP_DERIVED,
     KP_OPS,
+    KP_JCELL,
     KP_COUNT
 };
 const char *kKernelName[KP_COUNT] = {"k_dt",    "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
-                                     "k_fluid", "k_curl_update", "k_derived", "k_ops"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL};
+                                     "k_fluid", "k_curl_update", "k_derived", "k_ops", "k_jcell"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL, R_CELL};
 } // namespace
 
 struct mhd_ctx
@@ -106,11 +118,8 @@ struct mhd_ctx
     unsigned long long *d_dtbits = nullptr, *d_dtscratch = nullptr, *d_divb = nullptr;
     int *d_err = nullptr, *d_active = nullptr;
 
-    // concatenated op lists of the active cases
-    MhdOp *d_ops = nullptr;
-    size_t ops_cap = 0;
-    int op_start[SY_COUNT] = {0};
-    int op_count[SY_COUNT] = {0};
+    // op lists of the active cases, one device array per sync group
+    OpGroup og[SY_COUNT];
 
     hipGraphExec_t gexec = nullptr;
 
@@ -166,7 +175,7 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
     mhd_ctx *c = new mhd_ctx();
     c->opt = *opt;
     for (int k = 0; k < KP_COUNT; ++k)
-        c->jt_idx[k] = 2; // 64 x 4 until tuned
+        c->jt_idx[k] = 2; // 64 x 4, j-tile before i, until tuned
     if (hipStreamCreateWithFlags(&c->q, hipStreamNonBlocking) != hipSuccess)
     {
         delete c;
@@ -190,10 +199,9 @@ extern "C" int mhd_destroy(mhd_ctx *c)
         free_t(g.d);
     for (auto &rr : c->tiles)
         for (auto &t : rr)
-        {
-            free_t(t.d_tile);
             free_t(t.d_start);
-        }
+    for (auto &g : c->og)
+        free_t(g.d);
     free_t(c->d_blk);
     free_t(c->d_cst);
     free_t(c->d_dt);
@@ -203,7 +211,6 @@ extern "C" int mhd_destroy(mhd_ctx *c)
     free_t(c->d_divb);
     free_t(c->d_err);
     free_t(c->d_active);
-    free_t(c->d_ops);
     (void)hipStreamDestroy(c->q);
     delete c;
     return MHD_OK;
@@ -263,22 +270,21 @@ static int build_tiles(mhd_ctx *c)
         {
             TileArr &T = c->tiles[r][ji];
             const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
-            std::vector<int> start(std::max(nblk, 1), 0), tb;
+            std::vector<int> start(std::max(nblk, 1), 0);
+            long long total = 0;
             for (int b = 0; b < nblk; ++b)
             {
                 const MhdBlockDev &d = c->h_blk[b];
                 const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
-                const long long cnt = (long long)ei * ((ej + jt - 1) / jt) * ((ek + 63) / 64);
-                start[b] = (int)tb.size();
-                tb.insert(tb.end(), (size_t)cnt, b);
+                start[b] = (int)total;
+                total += (long long)ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
             }
-            T.n = (int)tb.size();
+            if (total > (1LL << 31) - 64)
+                return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+            T.n = (int)total;
             int rc;
-            if ((rc = grow(c, T.d_tile, T.cap_tile, std::max<size_t>(tb.size(), 1))) ||
-                (rc = grow(c, T.d_start, T.cap_start, start.size())))
+            if ((rc = grow(c, T.d_start, T.cap_start, start.size())))
                 return rc;
-            if (!tb.empty())
-                HIPCHK(c, hipMemcpy(T.d_tile, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy(T.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
         }
     return MHD_OK;
@@ -287,16 +293,18 @@ static int build_tiles(mhd_ctx *c)
 static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
 {
     const int r = kKernelRange[kernel];
-    const int ji = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
+    const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
+    const int ji = v % 3;
     const TileArr &T = c->tiles[r][ji];
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
-    L.tm.tile_block = T.d_tile;
     L.tm.block_start = T.d_start;
+    L.tm.nblk = (int)c->h_blk.size();
     L.tm.ntiles = T.n;
     L.tm.lo = kRangeLo[r];
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
+    L.tm.i_inner = v >= 3;
     L.jt = kJt[ji];
     return L;
 }
@@ -344,39 +352,51 @@ static int rebuild(mhd_ctx *c)
         HIPCHK(c, hipMemcpy(c->d_blk, c->h_blk.data(), sizeof(MhdBlockDev) * c->h_blk.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_cst, cst.data(), sizeof(MhdCaseConst) * c->case_cap, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_active, act.data(), sizeof(int) * c->case_cap, hipMemcpyHostToDevice));
-    if ((rc = build_tiles(c)))
-        return rc;
-    // op lists relocated to absolute addresses
-    std::vector<MhdOp> all;
-    for (int sy = 0; sy < SY_COUNT; ++sy)
-    {
-        c->op_start[sy] = (int)all.size();
-        for (int ci = 0; ci < (int)c->cases.size(); ++ci)
-        {
-            const CaseRec &r = *c->cases[ci];
-            if (r.state != 1)
-                continue;
-            for (MhdOp o : r.ops[sy])
-            {
-                o.dst = (int64_t)(uintptr_t)(r.d_st + o.dst);
-                if (o.kind == OP_COPY || o.kind == OP_NEG || o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
-                    o.src = (int64_t)(uintptr_t)(r.d_st + o.src);
-                else
-                    o.src = o.dst;
-                o.cas = ci;
-                all.push_back(o);
-            }
-        }
-        c->op_count[sy] = (int)all.size() - c->op_start[sy];
-    }
-    if ((rc = grow(c, c->d_ops, c->ops_cap, std::max<size_t>(all.size(), 1))))
-        return rc;
-    if (!all.empty())
-        HIPCHK(c, hipMemcpy(c->d_ops, all.data(), sizeof(MhdOp) * all.size(), hipMemcpyHostToDevice));
-    return MHD_OK;
+    return build_tiles(c);
 }
 
 // ------------------------------------------------------------- placement
+static int og_append(mhd_ctx *c, int sy, const std::vector<MhdOp> &ops, size_t &off)
+{
+    OpGroup &G = c->og[sy];
+    off = G.n;
+    if (ops.empty())
+        return MHD_OK;
+    if (G.n + ops.size() > G.cap)
+    {
+        const size_t ncap = std::max(G.n + ops.size(), 2 * G.cap);
+        MhdOp *np = nullptr;
+        HIPCHK(c, hipMalloc((void **)&np, sizeof(MhdOp) * ncap));
+        if (G.n)
+            HIPCHK(c, hipMemcpy(np, G.d, sizeof(MhdOp) * G.n, hipMemcpyDeviceToDevice));
+        free_t(G.d);
+        G.d = np;
+        G.cap = ncap;
+    }
+    HIPCHK(c, hipMemcpy(G.d + G.n, ops.data(), sizeof(MhdOp) * ops.size(), hipMemcpyHostToDevice));
+    G.n += ops.size();
+    return MHD_OK;
+}
+
+// Remove [off, off + cnt) from a group: later ops move down in chunks of at
+// most cnt, so no chunk overlaps its destination.
+static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt)
+{
+    OpGroup &G = c->og[sy];
+    if (cnt == 0)
+        return MHD_OK;
+    for (size_t s = off + cnt; s < G.n; s += cnt)
+    {
+        const size_t m = std::min(cnt, G.n - s);
+        HIPCHK(c, hipMemcpy(G.d + s - cnt, G.d + s, sizeof(MhdOp) * m, hipMemcpyDeviceToDevice));
+    }
+    G.n -= cnt;
+    for (auto &r : c->cases)
+        if (r->state == 1 && r->op_off[sy] > off)
+            r->op_off[sy] -= cnt;
+    return MHD_OK;
+}
+
 static int place_case(mhd_ctx *c, CaseRec &r)
 {
     const int64_t slot = r.cs.blk[0].d.slot;
@@ -424,8 +444,25 @@ static int place_case(mhd_ctx *c, CaseRec &r)
     }
     r.st_bytes = st_bytes;
     c->bytes_used += st_bytes;
-    for (int sy = 0; sy < SY_COUNT && sy < (int)r.cs.ops.size(); ++sy)
-        r.ops[sy] = std::move(r.cs.ops[sy]);
+    // op lists: case-relative offsets -> absolute addresses, appended to the
+    // device op groups
+    for (int sy = 0; sy < SY_COUNT; ++sy)
+    {
+        std::vector<MhdOp> ops = sy < (int)r.cs.ops.size() ? std::move(r.cs.ops[sy]) : std::vector<MhdOp>();
+        for (MhdOp &o : ops)
+        {
+            o.dst = (int64_t)(uintptr_t)(r.d_st + o.dst);
+            if (o.kind == OP_COPY || o.kind == OP_NEG || o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
+                o.src = (int64_t)(uintptr_t)(r.d_st + o.src);
+            else
+                o.src = o.dst;
+            o.cas = r.id;
+        }
+        int rc = og_append(c, sy, ops, r.op_off[sy]);
+        if (rc)
+            return rc;
+        r.op_n[sy] = ops.size();
+    }
     r.cs.ops.clear();
     r.state = 1;
     return MHD_OK;
@@ -463,17 +500,20 @@ struct ProfScope
 
 static void run_ops(mhd_ctx *c, int sy, int ushift)
 {
-    mhdk::launch_ops(c->d_ops + c->op_start[sy], c->op_count[sy], c->d_cst, ushift, c->q);
+    mhdk::launch_ops(c->og[sy].d, (int)c->og[sy].n, c->d_cst, ushift, c->q);
 }
 
+// Derived fields (contravariant velocities, B_cell and the next dt, then
+// J_cell); the J_cell wall condition (the SY_JCELL_BC op list of the host
+// set-up) is applied inside the J_cell kernel.
 static void derived(mhd_ctx *c)
 {
     {
         ProfScope p(c, KP_DERIVED);
         mhdk::launch_derived(launch_of(c, KP_DERIVED), c->d_cst, c->d_dtbits, c->ucur, c->q);
     }
-    ProfScope p(c, KP_OPS);
-    run_ops(c, SY_JCELL_BC, 0);
+    ProfScope p(c, KP_JCELL);
+    mhdk::launch_jcell(launch_of(c, KP_JCELL), c->q);
 }
 
 // One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
@@ -537,7 +577,7 @@ static int autotune(mhd_ctx *c)
     if (c->h_blk.empty())
         return MHD_OK;
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED};
+    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED, KP_JCELL};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -545,7 +585,7 @@ static int autotune(mhd_ctx *c)
     {
         double best = 1e30;
         int best_j = c->jt_idx[k];
-        for (int ji = 0; ji < 3; ++ji)
+        for (int ji = 0; ji < kVariants; ++ji)
         {
             auto launch = [&]() {
                 const mhdk::Launch L = launch_of(c, k, ji);
@@ -562,6 +602,9 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_CURL:
                     mhdk::launch_curl_update(L, c->d_zero, c->q);
+                    break;
+                case KP_JCELL:
+                    mhdk::launch_jcell(L, c->q);
                     break;
                 default:
                     mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, c->q);
@@ -595,6 +638,7 @@ static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
     r->cs = std::move(cs);
     r->nblocks = (int)r->cs.blk.size();
     const int id = (int)c->cases.size();
+    r->id = id;
     c->cases.push_back(std::move(r));
     if (case_id)
         *case_id = id;
@@ -738,8 +782,13 @@ extern "C" int mhd_retire(mhd_ctx *c, int case_id)
         c->bytes_used -= G.bytes;
     }
     r.state = 2;
-    for (auto &o : r.ops)
-        std::vector<MhdOp>().swap(o);
+    for (int sy = 0; sy < SY_COUNT; ++sy)
+    {
+        int rc = og_cut(c, sy, r.op_off[sy], r.op_n[sy]);
+        if (rc)
+            return rc;
+        r.op_n[sy] = 0;
+    }
     return rebuild(c);
 }
 
@@ -1023,11 +1072,12 @@ extern "C" int mhd_profile_read(mhd_ctx *c, int kernel, double *ms, int *launche
     return MHD_OK;
 }
 
-extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows)
+extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows, int *i_inner)
 {
-    if (!c || !rows || kernel < 0 || kernel >= KP_COUNT)
+    if (!c || !rows || !i_inner || kernel < 0 || kernel >= KP_COUNT)
         return MHD_ERR_ARG;
-    *rows = kJt[c->jt_idx[kernel]];
+    *rows = kJt[c->jt_idx[kernel] % 3];
+    *i_inner = c->jt_idx[kernel] >= 3;
     return MHD_OK;
 }
 

@@ -24,6 +24,7 @@ void launch_edge_emf(const Launch &L, hipStream_t q);
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q);
 void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                   hipStream_t q);
+void launch_jcell(const Launch &L, hipStream_t q);
 void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q);
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, hipStream_t q);

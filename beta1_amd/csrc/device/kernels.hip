@@ -22,6 +22,15 @@ using namespace mpcns;
 
 namespace
 {
+// Device addresses travel as integers (block table, op lists).  Converting
+// them through a global-address-space pointer lets the compiler emit global
+// (not flat) loads and stores, whose completion it can count per access
+// instead of draining every outstanding access at each wait.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ double *gptr(int64_t a) { return (double *)(gdouble *)(uintptr_t)a; }
+__device__ __forceinline__ double *dst_ptr(const MhdBlockDev &b) { return gptr(b.st); }
+__device__ __forceinline__ const double *dgeo_ptr(const MhdBlockDev &b) { return gptr(b.geo); }
+
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
@@ -37,20 +46,32 @@ __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, 
     wg = t < tm.ntiles;
     if (!wg)
         return false;
-    const int bi = tm.tile_block[t];
+    int bi = 0, hi = tm.nblk - 1;
+    while (bi < hi)
+    {
+        const int mid = (bi + hi + 1) >> 1;
+        if (tm.block_start[mid] <= t)
+            bi = mid;
+        else
+            hi = mid - 1;
+    }
     b = tm.blks[bi];
     const int lt = t - tm.block_start[bi];
-    const int jt = (int)blockDim.y;
-    const int ext_j = b.nj - tm.lo + tm.hi, ext_k = b.nk - tm.lo + tm.hi;
-    const int ntk = (ext_k + 63) >> 6, ntj = (ext_j + jt - 1) / jt;
-    const int kt = lt % ntk;
-    const int r = lt / ntk;
-    const int jy = r % ntj;
-    const int iy = r / ntj;
+    // a tile is 64 * rows consecutive points of one (j, k) plane of the range,
+    // k fastest, so every wavefront is full however the extents divide by 64
+    const int per_tile = 64 * (int)blockDim.y;
+    const int ext_i = b.ni - tm.lo + tm.hi, ext_j = b.nj - tm.lo + tm.hi, ext_k = b.nk - tm.lo + tm.hi;
+    const int plane = ext_j * ext_k;
+    const int ntp = (plane + per_tile - 1) / per_tile;
+    // i-inner order: a tile's i-neighbours are the next tiles in launch order
+    // (still in L2 when they are read again)
+    const int pt = tm.i_inner ? lt / ext_i : lt % ntp;
+    const int iy = tm.i_inner ? lt % ext_i : lt / ntp;
+    const int f = pt * per_tile + (int)threadIdx.y * 64 + (int)threadIdx.x;
     i = tm.lo + iy;
-    j = tm.lo + jy * jt + (int)threadIdx.y;
-    k = tm.lo + kt * 64 + (int)threadIdx.x;
-    return (j < tm.lo + ext_j) && (k < tm.lo + ext_k);
+    j = tm.lo + f / ext_k;
+    k = tm.lo + f % ext_k;
+    return f < plane;
 }
 
 // ------------------------------------------------------------------ state
@@ -60,16 +81,14 @@ struct Prim
 };
 
 // calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110)
-__device__ __forceinline__ Prim prim_of(const double *__restrict__ U, int64_t sl, int64_t o, double gm)
+__device__ __forceinline__ Prim prim_from(double r0, double m1, double m2, double m3, double E, double gm)
 {
     Prim q;
-    const double r0 = U[o];
     q.rho_raw = r0;
     const double rho = dmax(r0, 1e-12);
-    q.u = U[sl + o] / rho;
-    q.v = U[2 * sl + o] / rho;
-    q.w = U[3 * sl + o] / rho;
-    const double E = U[4 * sl + o];
+    q.u = m1 / rho;
+    q.v = m2 / rho;
+    q.w = m3 / rho;
     const double ke = 0.5 * rho * (q.u * q.u + q.v * q.v + q.w * q.w);
     double eint = E - ke;
     if (eint < 0.0)
@@ -77,6 +96,11 @@ __device__ __forceinline__ Prim prim_of(const double *__restrict__ U, int64_t sl
     double p = (gm - 1.0) * eint;
     q.p = dmax(p, 1e-12);
     return q;
+}
+
+__device__ __forceinline__ Prim prim_of(const double *__restrict__ U, int64_t sl, int64_t o, double gm)
+{
+    return prim_from(U[o], U[sl + o], U[2 * sl + o], U[3 * sl + o], U[4 * sl + o], gm);
 }
 
 // -------------------------------------------------------- Rusanov flux
@@ -123,32 +147,57 @@ __device__ __forceinline__ double minmod(double a, double b)
 // atomicMin per workgroup on the bit pattern (positive doubles order like
 // their bits).  Cells with V <= 0 or a non-positive / NaN denominator are
 // skipped, as the reference does.
-__device__ __forceinline__ double cell_dt(const MhdBlockDev &b, const MhdCaseConst &c, const double *__restrict__ U,
-                                          const double *__restrict__ G, int64_t o, double Bx, double By, double Bz)
+// The values a cell's CFL limit and contravariant velocities read: U (5),
+// Jac and the area vectors JD of its 6 faces (f = 2 * axis + side).  Loaded
+// up front, unconditionally, so no load sits behind a data-dependent branch.
+struct CellGeo
+{
+    double u[5];
+    double jac;
+    double a[6][3];
+};
+
+__device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__restrict__ U,
+                                          const double *__restrict__ G, int64_t o, CellGeo &q)
 {
     const int64_t sl = b.slot;
+    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        q.u[m] = U[m * sl + o];
+    q.jac = G[(int64_t)G_JAC * sl + o];
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        const int ax = f >> 1;
+        const int64_t oo = o + ((f & 1) ? step[ax] : 0);
+        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        q.a[f][0] = A[oo];
+        q.a[f][1] = A[sl + oo];
+        q.a[f][2] = A[2 * sl + oo];
+    }
+}
+
+__device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &q, double Bx, double By, double Bz)
+{
     double local = 1e100;
-    const double V = fabs(G[(int64_t)G_JAC * sl + o]);
+    const double V = fabs(q.jac);
     if (V > 0.0)
     {
         const double gm = c.gamma;
-        const double rho = dmax(U[o], 1e-2);
-        const double ux = U[sl + o] / rho, uy = U[2 * sl + o] / rho, uz = U[3 * sl + o] / rho;
-        const double E = U[4 * sl + o];
+        const double rho = dmax(q.u[0], 1e-2);
+        const double ux = q.u[1] / rho, uy = q.u[2] / rho, uz = q.u[3] / rho;
+        const double E = q.u[4];
         const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
         const double eint = E - ke;
         const double p = (gm - 1.0) * eint;
         const double cs2 = gm * p / rho;
         const double B2 = Bx * Bx + By * By + Bz * Bz;
-        const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk;
         double denom = 0.0;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
         {
-            const int ax = f >> 1;
-            const int64_t oo = o + ((f & 1) ? (ax == 0 ? di : (ax == 1 ? dj : 1)) : 0);
-            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
-            const double axv = A[oo], ayv = A[sl + oo], azv = A[2 * sl + oo];
+            const double axv = q.a[f][0], ayv = q.a[f][1], azv = q.a[f][2];
             const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
             double term = 0.0;
             if (An > 0.0)
@@ -212,9 +261,11 @@ __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseCo
     if (ok)
     {
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-        const double *U = st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
-        const double *Bc = st_ptr(b) + (int64_t)ST_BCELL * sl;
-        local = cell_dt(b, cst[b.case_id], U, geo_ptr(b), o, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
+        const double *U = dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
+        const double *Bc = dst_ptr(b) + (int64_t)ST_BCELL * sl;
+        CellGeo q;
+        load_cell(b, U, dgeo_ptr(b), o, q);
+        local = cell_dt(cst[b.case_id], q, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
     }
     if (wg)
         reduce_dt(local, dtbits, b.case_id);
@@ -241,24 +292,19 @@ __global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, 
 // contravariant cell velocity per axis, cells [-1, N] (step scratch):
 // cell_u_contra (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:414-428)
 // with U_plus from calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141).
-__device__ __forceinline__ void cell_ucontra(const MhdBlockDev &b, double gm, const double *__restrict__ U,
-                                             const double *__restrict__ G, double *__restrict__ UC, int64_t o)
+__device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double *uc)
 {
-    const int64_t sl = b.slot;
-    const Prim q = prim_of(U, sl, o, gm);
+    const Prim q = prim_from(c.u[0], c.u[1], c.u[2], c.u[3], c.u[4], gm);
     const bool has = q.rho_raw > 1e-20;
     const double up0 = has ? q.u : 0.0, up1 = has ? q.v : 0.0, up2 = has ? q.w : 0.0;
-    const double inv_jac = 1.0 / (fabs(G[(int64_t)G_JAC * sl + o]) + 1.0e-30);
-    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    const double inv_jac = 1.0 / (fabs(c.jac) + 1.0e-30);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
     {
-        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
-        const int64_t op = o + step[ax];
-        const double kx = 0.5 * (A[o] + A[op]);
-        const double ky = 0.5 * (A[sl + o] + A[sl + op]);
-        const double kz = 0.5 * (A[2 * sl + o] + A[2 * sl + op]);
-        UC[ax * sl + o] = (kx * up0 + ky * up1 + kz * up2) * inv_jac;
+        const double kx = 0.5 * (c.a[2 * ax][0] + c.a[2 * ax + 1][0]);
+        const double ky = 0.5 * (c.a[2 * ax][1] + c.a[2 * ax + 1][1]);
+        const double kz = 0.5 * (c.a[2 * ax][2] + c.a[2 * ax + 1][2]);
+        uc[ax] = (kx * up0 + ky * up1 + kz * up2) * inv_jac;
     }
 }
 
@@ -271,8 +317,14 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdC
     if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    cell_ucontra(b, cst[b.case_id].gamma, st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, geo_ptr(b),
-                 st_ptr(b) + (int64_t)ST_UC * sl, o);
+    CellGeo q;
+    load_cell(b, dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, dgeo_ptr(b), o, q);
+    double uc[3];
+    cell_ucontra(cst[b.case_id].gamma, q, uc);
+    double *UC = dst_ptr(b) + (int64_t)ST_UC * sl;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+        UC[ax * sl + o] = uc[ax];
 }
 
 // ======================================================================
@@ -318,7 +370,7 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-    double *S = st_ptr(b);
+    double *S = dst_ptr(b);
     const double *Bf = S + (int64_t)ST_B * sl, *Ba = S + (int64_t)ST_BADD * sl, *UC = S + (int64_t)ST_UC * sl;
     const int n[3] = {b.ni, b.nj, b.nk};
     const int idx[3] = {i, j, k};
@@ -385,7 +437,7 @@ __global__ __launch_bounds__(256) void k_edge_emf(const MhdTileMap tm)
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st_ptr(b);
+    double *S = dst_ptr(b);
     const double *Fx = S + (int64_t)(ST_EFACE + 0) * sl, *Fy = S + (int64_t)(ST_EFACE + 3) * sl,
                  *Fz = S + (int64_t)(ST_EFACE + 6) * sl;
     const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
@@ -434,7 +486,7 @@ __global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const 
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st_ptr(b);
+    double *S = dst_ptr(b);
     const double *Ex = S + (int64_t)ST_E * sl, *Ey = Ex + sl, *Ez = Ex + 2 * sl;
     double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
     const double dt = dts[b.case_id];
@@ -519,10 +571,10 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
     const double gm = c.gamma;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-    double *S = st_ptr(b);
+    double *S = dst_ptr(b);
     const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
     double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
-    const double *G = geo_ptr(b);
+    const double *G = dgeo_ptr(b);
     double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
 #pragma unroll
@@ -559,13 +611,11 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
 }
 
 // ======================================================================
-// Derived magnetic fields on interior cells: calc_Bcell + Calc_J_Edge +
-// calc_Jcell (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:163-236,777-891;
-// CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).
-// J edges are recomputed per cell from beta-weighted face fluxes.
-// Fused with the next step's CFL limit (dt depends on U after the Ucell
-// sync and on the B_cell computed here, nothing changes them before the next
-// step) and with the contravariant velocities of cells [-1, N].
+// Derived fields, part 1: contravariant velocities of cells [-1, N] and, on
+// interior cells, B_cell (calc_Bcell, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
+// 163-236) fused with the next step's CFL limit (dt depends on U after the
+// Ucell sync and on this B_cell; nothing changes them before the next step).
+// Each result is stored as soon as it is known, which keeps few values live.
 __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  unsigned long long *__restrict__ dtbits, int ucur)
 {
@@ -576,69 +626,98 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
     double local = 1e100;
     if (ok)
     {
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const double *Ucur = st_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
-    cell_ucontra(b, cst[b.case_id].gamma, Ucur, geo_ptr(b), st_ptr(b) + (int64_t)ST_UC * sl, o);
-    if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
-    {
-    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = st_ptr(b);
-    const double *G = geo_ptr(b);
-    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
-    {
-        const double pt[6] = {-Bx[o] - Ax[o],   Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
-                              By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],   Bz[o + dk] + Az[o + dk]};
-        const double *W = G + (int64_t)G_WB * sl;
-        double xt = 0.0, yt = 0.0, zt = 0.0;
-#pragma unroll
-        for (int n = 0; n < 6; ++n)
+        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+        double *S = dst_ptr(b);
+        const double *G = dgeo_ptr(b);
+        CellGeo q;
+        load_cell(b, S + (int64_t)(ST_U0 + 5 * ucur) * sl, G, o, q);
         {
-            xt += W[(int64_t)n * sl + o] * pt[n];
-            yt += W[(int64_t)(6 + n) * sl + o] * pt[n];
-            zt += W[(int64_t)(12 + n) * sl + o] * pt[n];
-        }
-        double *Bc = S + (int64_t)ST_BCELL * sl;
-        Bc[o] = xt;
-        Bc[sl + o] = yt;
-        Bc[2 * sl + o] = zt;
-        local = cell_dt(b, cst[b.case_id], Ucur, G, o, xt, yt, zt);
-    }
-    {
-        const double *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
-        const double *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
-        auto jxi = [&](int64_t e) {
-            const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
-            return (1.0 * val) * ax[e];
-        };
-        auto jet = [&](int64_t e) {
-            const double val = (bx[e] * Bx[e] - bx[e - dk] * Bx[e - dk]) + (bz[e - di] * Bz[e - di] - bz[e] * Bz[e]);
-            return (1.0 * val) * ay[e];
-        };
-        auto jze = [&](int64_t e) {
-            const double val = (bx[e - dj] * Bx[e - dj] - bx[e] * Bx[e]) + (by[e] * By[e] - by[e - di] * By[e - di]);
-            return (1.0 * val) * az[e];
-        };
-        const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
-                              jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
-        const double *W = G + (int64_t)G_WJ * sl;
-        double x = 0.0, y = 0.0, z = 0.0;
+            double uc[3];
+            cell_ucontra(cst[b.case_id].gamma, q, uc);
+            double *UC = S + (int64_t)ST_UC * sl;
 #pragma unroll
-        for (int n = 0; n < 12; ++n)
-        {
-            x += W[(int64_t)n * sl + o] * s[n];
-            y += W[(int64_t)(12 + n) * sl + o] * s[n];
-            z += W[(int64_t)(24 + n) * sl + o] * s[n];
+            for (int a = 0; a < 3; ++a)
+                UC[a * sl + o] = uc[a];
         }
-        double *Jc = S + (int64_t)ST_JCELL * sl;
-        Jc[o] = x;
-        Jc[sl + o] = y;
-        Jc[2 * sl + o] = z;
-    }
-    }
+        if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
+        {
+            const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+            const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+            const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
+                                  By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
+            const double *W = G + (int64_t)G_WB * sl;
+            double bc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 6; ++n)
+            {
+                bc[0] += W[(int64_t)n * sl + o] * pt[n];
+                bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
+                bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
+            }
+            double *Bc = S + (int64_t)ST_BCELL * sl;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                Bc[a * sl + o] = bc[a];
+            local = cell_dt(cst[b.case_id], q, bc[0], bc[1], bc[2]);
+        }
     }
     if (wg)
         reduce_dt(local, dtbits, b.case_id);
+}
+
+// Derived fields, part 2, interior cells: Calc_J_Edge + calc_Jcell
+// (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891; CTOperators::
+// CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).  The 12
+// J edges of a cell are recomputed from beta-weighted face fluxes.
+__global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = dst_ptr(b);
+    const double *G = dgeo_ptr(b);
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
+    const double *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
+    auto jxi = [&](int64_t e) {
+        const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
+        return (1.0 * val) * ax[e];
+    };
+    auto jet = [&](int64_t e) {
+        const double val = (bx[e] * Bx[e] - bx[e - dk] * Bx[e - dk]) + (bz[e - di] * Bz[e - di] - bz[e] * Bz[e]);
+        return (1.0 * val) * ay[e];
+    };
+    auto jze = [&](int64_t e) {
+        const double val = (bx[e - dj] * Bx[e - dj] - bx[e] * Bx[e]) + (by[e] * By[e] - by[e - di] * By[e - di]);
+        return (1.0 * val) * az[e];
+    };
+    const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
+                          jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
+    const double *W = G + (int64_t)G_WJ * sl;
+    double jc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int n = 0; n < 12; ++n)
+    {
+        jc[0] += W[(int64_t)n * sl + o] * s[n];
+        jc[1] += W[(int64_t)(12 + n) * sl + o] * s[n];
+        jc[2] += W[(int64_t)(24 + n) * sl + o] * s[n];
+    }
+    // solid wall: J_cell = 0 on the two cell layers next to it
+    // (Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:382-427)
+    const int idx[3] = {i, j, k}, nn[3] = {b.ni, b.nj, b.nk};
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (b.bc[f] == BC_SOLID && ((f & 1) ? idx[f >> 1] >= nn[f >> 1] - 2 : idx[f >> 1] < 2))
+            jc[0] = jc[1] = jc[2] = 0.0;
+    double *Jc = S + (int64_t)ST_JCELL * sl;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        Jc[a * sl + o] = jc[a];
 }
 
 // ======================================================================
@@ -664,8 +743,8 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
         return;
     const MhdOp op = ops[t];
     const int64_t sh = ushift ? 5 * op.stride : 0;
-    double *d = reinterpret_cast<double *>((uintptr_t)op.dst) + sh;
-    const double *s = reinterpret_cast<const double *>((uintptr_t)op.src) + sh;
+    double *d = gptr(op.dst) + sh;
+    const double *s = gptr(op.src) + sh;
     switch (op.kind)
     {
     case OP_COPY:
@@ -719,9 +798,9 @@ __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long
     if (ok)
     {
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-        const double *Bx = st_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+        const double *Bx = dst_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
         const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
-                          geo_ptr(b)[(int64_t)G_JAC * sl + o];
+                          dgeo_ptr(b)[(int64_t)G_JAC * sl + o];
         v = fabs(dv);
         if (!(v == v))
             v = 1e300;
@@ -801,6 +880,11 @@ void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, i
     else
         hipLaunchKernelGGL((k_fluid<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
+void launch_jcell(const Launch &L, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_jcell, grid_of(L), block_of(L), 0, q, L.tm);
+}
+
 void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
 {
     hipLaunchKernelGGL(k_derived, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);

@@ -812,9 +812,20 @@ void halo_ops(const OpCtx &C, int b, int loc, int ncomp, bool eface, SlotOf slot
                 }
                 else
                 {
-                    // Eface: component c is the candidate of the edge along axis c
+                    // Eface: component c is the candidate of the edge along axis c.
+                    // The edge EMF reads component c != a of a face with normal a
+                    // only one layer outside the block along the third axis t,
+                    // every other index interior; no other ghost value of this
+                    // step scratch is read, so only those are exchanged.
+                    const int a = loc - LOC_FACE_XI;
                     for (int c = 0; c < 3; ++c)
                     {
+                        if (c == a)
+                            continue;
+                        const int t = 3 - a - c;
+                        if (idx[a] < 0 || idx[a] >= cnt[a] || idx[c] < 0 || idx[c] >= cnt[c] ||
+                            (idx[t] != -1 && idx[t] != cnt[t]))
+                            continue;
                         const int64_t d = dst + (int64_t)c * stride;
                         const int64_t sr = C.ref(s.block, slot_of(s.loc) + s.perm[c], s.idx[0], s.idx[1], s.idx[2]);
                         out.push_back(mk(d, sr, s.sign[c] > 0 ? OP_COPY : OP_NEG, 1, stride));

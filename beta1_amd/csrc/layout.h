@@ -127,19 +127,22 @@ struct MhdCaseConst
 MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }
 MHD_HD const double *geo_ptr(const MhdBlockDev &b) { return reinterpret_cast<const double *>((uintptr_t)b.geo); }
 
-// Launch tiling: every (block, i, j-tile, k-tile) of a launch range is one
-// workgroup of 64 x jt threads; tile_block[t] / block_start[b] map a tile to
-// its block.  With xcd_chunk > 0 workgroup w runs tile (w % 8) * xcd_chunk +
-// w / 8, so each of the 8 XCDs (workgroups are dealt round-robin) sweeps a
-// contiguous range of tiles and neighbour planes stay in its own L2.
+// Launch tiling: a workgroup of 64 x jt threads covers 64 * jt consecutive
+// points (k fastest) of one (block, i) plane of a launch range;
+// block_start[b] is the first tile of block b
+// (ascending, nblk entries) and a tile finds its block by binary search.
+// With xcd_chunk > 0 workgroup w runs tile (w % 8) * xcd_chunk + w / 8, so
+// each of the 8 XCDs (workgroups are dealt round-robin) sweeps a contiguous
+// range of tiles and neighbour planes stay in its own L2.
 struct MhdTileMap
 {
     const MhdBlockDev *blks;
-    const int *tile_block;
     const int *block_start;
+    int nblk;
     int ntiles;
     int lo, hi; // range [lo, n + hi) on every axis
     int xcd_chunk;
+    int i_inner; // 1: consecutive tiles step along i, 0: along the (j, k) plane
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)

@@ -217,9 +217,9 @@ class MhdContext:
     def tuning(self):
         out = {}
         for k, name in enumerate(self.profile_read()):
-            rows = C.c_int()
-            lib().mhd_tuning(self.ctx, k, C.byref(rows))
-            out[name] = rows.value
+            rows, inner = C.c_int(), C.c_int()
+            lib().mhd_tuning(self.ctx, k, C.byref(rows), C.byref(inner))
+            out[name] = (rows.value, "i-inner" if inner.value else "j-inner")
         return out
 
     def step(self, n=1, graph=False):

@@ -166,9 +166,10 @@ int mhd_profile(mhd_ctx *ctx, int enable);
 int mhd_profile_read(mhd_ctx *ctx, int kernel, double *ms, int *launches, const char **name);
 /* device bytes of state + geometry held by the context */
 int mhd_memory(mhd_ctx *ctx, double *bytes);
-/* workgroup height (rows of 64 threads) the tuner chose for a kernel index
-   of mhd_profile_read */
-int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows);
+/* launch shape the tuner chose for a kernel index of mhd_profile_read:
+   workgroup height (rows of 64 threads along j) and tile order (i_inner 1:
+   consecutive tiles step along i before the next row group of j) */
+int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner);
 
 /* ---- host-only case builder (no device needed) ------------------------
    The same grid / topology / geometry / weights / initial-state builders the

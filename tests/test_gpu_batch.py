@@ -111,4 +111,4 @@ def test_xcd_order_and_tuning_do_not_change_results():
     ctx.finalize()
     ctx.step(4)
     same(ctx, cid, ref)
-    assert set(ctx.tuning().values()) <= {1, 2, 4}
+    assert {r for r, _ in ctx.tuning().values()} <= {1, 2, 4}

@@ -289,6 +289,47 @@ def test_face_halo_maps_fluxes_with_orientation():
             assert np.abs(got[m] - flux[m]).max() <= 1e-12 * np.abs(flux).max()
 
 
+def test_eface_exchange_covers_every_value_the_edge_emf_reads():
+    """The device exchanges only the face-EMF ghosts the edge EMF reads
+    (component c of a face with normal a, one layer outside along the third
+    axis).  On that read set the host op lists (wall BC, then halo) give the
+    oracle's full Eface sync exactly."""
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    orc = Oracle(hc)
+    n = hc.n
+    rng = np.random.default_rng(7)
+    names = ("Eface_xi", "Eface_eta", "Eface_zeta")
+    for b in range(6):
+        for a in range(3):
+            E = np.full(hc.field(names[a], b).shape, np.nan)
+            sl = interior(names[a], n, G)
+            E[sl] = rng.standard_normal(E[sl].shape)
+            hc.set_field(names[a], b, E)
+            orc.view(names[a], b)[...] = E
+    hc.apply_sync(0)  # Eface wall BC
+    hc.apply_sync(1)  # Eface halo
+    orc.sync("Eface")
+    checked = 0
+    for b in range(6):
+        for a in range(3):
+            got, ref = hc.field(names[a], b), orc.view(names[a], b)
+            for c in range(3):
+                if c == a:
+                    continue
+                t = 3 - a - c
+                rng_ax = [None] * 3
+                rng_ax[a] = range(G, G + n[a] + 1)
+                rng_ax[c] = range(G, G + n[c])
+                rng_ax[t] = range(G - 1, G + n[t] + 1)
+                idx = np.ix_(*rng_ax)
+                g, r = got[idx][..., c], ref[idx][..., c]
+                # ghosts past the far-field side are written by no sync (their
+                # edges are overwritten by the far-field E): NaN in both
+                assert np.array_equal(g, r, equal_nan=True), (b, a, c)
+                checked += np.isfinite(r).sum()
+    assert checked > 0
+
+
 def test_oracle_generic_halo_equals_periodic_wrap():
     hc = HostCase(OTDesc(n=12), ghost=G)
     a = Oracle(hc, periodic_wrap=True)
