@@ -1,10 +1,9 @@
 // HIP kernels of the explicit MHD step for gfx950 (CDNA4).
 //
-// Memory-bound stencils over the padded per-block box of layout.h: a
-// workgroup is 64 x 4 threads, one 64-lane wavefront per k-row, so every
-// slot load and store of a wavefront is one contiguous 512-byte segment.
-// A launch is a 1-D grid over the tiles (block, i, j-tile, k-tile) of every
-// placed block, see MhdTileMap.
+// Memory-bound stencils over the padded per-block box of layout.h.  A
+// launch is a 1-D grid over tiles of 64 x rows consecutive points (k
+// fastest) of one (block, i) plane of the launch range, so a wavefront's
+// slot loads and stores are contiguous runs along k; see MhdTileMap.
 //
 // FP64 arithmetic is written in the same operation order as the reference
 // (and as the CPU oracle) and the library is compiled with
@@ -147,6 +146,7 @@ __device__ __forceinline__ double minmod(double a, double b)
 // atomicMin per workgroup on the bit pattern (positive doubles order like
 // their bits).  Cells with V <= 0 or a non-positive / NaN denominator are
 // skipped, as the reference does.
+//
 // The values a cell's CFL limit and contravariant velocities read: U (5),
 // Jac and the area vectors JD of its 6 faces (f = 2 * axis + side).  Loaded
 // up front, unconditionally, so no load sits behind a data-dependent branch.
