@@ -131,9 +131,9 @@ def bench_serve(args):
     from beta1_amd.serve import ServingEngine, poisson_requests
 
     prec = args.precision
-    # capacity estimate from the request mix and a short calibration batch
-    probe = poisson_requests(64, 1.0, seed=SEED, steps=args.req_steps)
-    mean_work = float(np.mean([r.cells for r in probe])) * args.req_steps
+    # the request list at unit rate, rescaled below to the offered load
+    reqs = poisson_requests(args.requests, 1.0, seed=SEED, steps=args.req_steps)
+    mean_work = float(np.mean([r.cells for r in reqs])) * args.req_steps
     eng = ServingEngine(precision=prec, hbm_budget_gb=args.budget_gb, chunk=20, workers=2)
     from beta1_amd import HostCase, WindDesc
     cal = eng.ctx.adopt(HostCase(WindDesc(nang=64, nrad=128)))
@@ -145,12 +145,13 @@ def bench_serve(args):
     cap = WindDesc(nang=64, nrad=128).cells * 20 / (time.perf_counter() - t0)  # cell-updates/s
     eng.ctx.retire(cal)
     rate = args.load * cap / mean_work
-    reqs = poisson_requests(args.requests, rate, seed=SEED, steps=args.req_steps)
+    for r in reqs:  # exponential gaps of mean 1 -> mean 1 / rate
+        r.arrival /= rate
     log(f"serve: capacity ~{cap / 1e6:.0f} Mcell-updates/s, mean request {mean_work / 1e9:.2f} G cell-updates, "
         f"rate {rate:.3f}/s (offered load {args.load})")
     res = eng.run(reqs, log=log)
     eng.close()
-    offered = rate * float(np.mean([r.cells for r in reqs])) * args.req_steps / cap
+    offered = rate * mean_work / cap
     return _record("p50/p99 solve latency + Mcell-updates/s (serving)", res["mcell_updates_per_s"],
                    "Mcell-updates/s", args.req_steps, 0, None, "f32/f64" if prec == "mixed" else "f64",
                    {"workload": f"serve_poisson_{args.requests}req_{prec}", "requests": args.requests,

@@ -41,7 +41,7 @@ struct CaseRec
 struct GeoSet
 {
     std::string key;
-    double *d = nullptr;
+    char *d = nullptr; // G_NSLOT slots per block, elements of geo_esize bytes
     int64_t block_stride = 0;
     int refs = 0;
     size_t bytes = 0;
@@ -138,6 +138,11 @@ struct mhd_ctx
             return MHD_ERR_HIP;                                                                                        \
         }                                                                                                              \
     } while (0)
+
+// Mixed-precision contexts store the geometry (metric and weight tables) in
+// float; the FP64 step keeps it in double.
+static int mixed_of(const mhd_ctx *c) { return c->opt.precision == MHD_PREC_MIXED; }
+static size_t geo_esize(const mhd_ctx *c) { return mixed_of(c) ? sizeof(float) : sizeof(double); }
 
 static int fail(mhd_ctx *c, int code, const std::string &msg)
 {
@@ -342,7 +347,7 @@ static int rebuild(mhd_ctx *c)
             MhdBlockDev d = r.cs.blk[b].d;
             d.case_id = ci;
             d.st = (int64_t)(uintptr_t)(r.d_st + (int64_t)b * ST_NSLOT * r.slot);
-            d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_stride);
+            d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_stride * (int64_t)geo_esize(c));
             c->h_blk.push_back(d);
         }
     }
@@ -406,7 +411,7 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         if (c->geos[g].refs > 0 && c->geos[g].key == r.cs.shape_key)
             gi = g;
     const size_t st_bytes = sizeof(double) * (size_t)ST_NSLOT * slot * r.nblocks;
-    const size_t geo_bytes = gi < 0 ? sizeof(double) * (size_t)G_NSLOT * slot * r.nblocks : 0;
+    const size_t geo_bytes = gi < 0 ? geo_esize(c) * (size_t)G_NSLOT * slot * r.nblocks : 0;
     const double budget = c->opt.hbm_budget_gb * 1e9;
     if (budget > 0 && (double)(c->bytes_used + st_bytes + geo_bytes) > budget)
         return fail(c, MHD_ERR_MEM, "HBM budget exceeded");
@@ -417,9 +422,18 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         G.block_stride = (int64_t)G_NSLOT * slot;
         G.bytes = geo_bytes;
         HIPCHK(c, hipMalloc((void **)&G.d, geo_bytes));
+        std::vector<float> f32;
         for (int b = 0; b < r.nblocks; ++b)
-            HIPCHK(c, hipMemcpy(G.d + (int64_t)b * G.block_stride, r.cs.blk[b].geo.data(),
-                                sizeof(double) * G.block_stride, hipMemcpyHostToDevice));
+        {
+            const size_t nbytes = geo_esize(c) * (size_t)G.block_stride;
+            const void *src = r.cs.blk[b].geo.data();
+            if (mixed_of(c))
+            {
+                f32.assign(r.cs.blk[b].geo.begin(), r.cs.blk[b].geo.end());
+                src = f32.data();
+            }
+            HIPCHK(c, hipMemcpy(G.d + (int64_t)b * nbytes, src, nbytes, hipMemcpyHostToDevice));
+        }
         c->bytes_used += geo_bytes;
         gi = 0;
         while (gi < (int)c->geos.size() && c->geos[gi].refs > 0)
@@ -510,10 +524,10 @@ static void derived(mhd_ctx *c)
 {
     {
         ProfScope p(c, KP_DERIVED);
-        mhdk::launch_derived(launch_of(c, KP_DERIVED), c->d_cst, c->d_dtbits, c->ucur, c->q);
+        mhdk::launch_derived(launch_of(c, KP_DERIVED), c->d_cst, c->d_dtbits, c->ucur, mixed_of(c), c->q);
     }
     ProfScope p(c, KP_JCELL);
-    mhdk::launch_jcell(launch_of(c, KP_JCELL), c->q);
+    mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), c->q);
 }
 
 // One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
@@ -604,10 +618,10 @@ static int autotune(mhd_ctx *c)
                     mhdk::launch_curl_update(L, c->d_zero, c->q);
                     break;
                 case KP_JCELL:
-                    mhdk::launch_jcell(L, c->q);
+                    mhdk::launch_jcell(L, mixed, c->q);
                     break;
                 default:
-                    mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, c->q);
+                    mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, mixed, c->q);
                     break;
                 }
             };
@@ -724,7 +738,7 @@ static size_t case_bytes(const mhd_ctx *c, const CaseSetup &cs)
     for (const GeoSet &G : c->geos)
         shared = shared || (G.refs > 0 && G.key == cs.shape_key);
     if (!shared)
-        need += sizeof(double) * G_NSLOT * slot * nb;
+        need += geo_esize(c) * G_NSLOT * slot * nb;
     return need;
 }
 
@@ -887,7 +901,7 @@ extern "C" int mhd_compute_dt(mhd_ctx *c, int case_id, double *dt)
     if (!dt || !active_case(c, case_id))
         return MHD_ERR_ARG;
     mhdk::launch_fill_u64(c->d_dtscratch, c->case_cap, dbits(1e100), c->q);
-    mhdk::launch_dt(launch_of(c, KP_DT), c->d_cst, c->d_dtscratch, c->ucur, c->q);
+    mhdk::launch_dt(launch_of(c, KP_DT), c->d_cst, c->d_dtscratch, c->ucur, mixed_of(c), c->q);
     unsigned long long bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtscratch + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
@@ -917,7 +931,7 @@ extern "C" int mhd_divb_max(mhd_ctx *c, int case_id, double *out)
     if (!out || !active_case(c, case_id))
         return MHD_ERR_ARG;
     mhdk::launch_fill_u64(c->d_divb, c->case_cap, 0ull, c->q);
-    mhdk::launch_divb(launch_of(c, KP_DT), c->d_divb, c->q);
+    mhdk::launch_divb(launch_of(c, KP_DT), c->d_divb, mixed_of(c), c->q);
     unsigned long long bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_divb + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
@@ -942,8 +956,23 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
     if (n != aos_count(d, loc, nc))
         return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
     HIPCHK(c, hipStreamSynchronize(c->q));
-    double *dev = (region == 0 ? st_ptr(d) : const_cast<double *>(geo_ptr(d))) + (int64_t)slot * d.slot;
     std::vector<double> soa((size_t)nc * d.slot);
+    if (region == 1 && mixed_of(c))
+    {
+        // geometry of a mixed-precision context is stored as float
+        float *dev = reinterpret_cast<float *>((uintptr_t)d.geo) + (int64_t)slot * d.slot;
+        std::vector<float> f32(soa.size());
+        HIPCHK(c, hipMemcpy(f32.data(), dev, sizeof(float) * f32.size(), hipMemcpyDeviceToHost));
+        soa.assign(f32.begin(), f32.end());
+        soa_aos(d, loc, nc, soa.data(), aos, down);
+        if (!down)
+        {
+            f32.assign(soa.begin(), soa.end());
+            HIPCHK(c, hipMemcpy(dev, f32.data(), sizeof(float) * f32.size(), hipMemcpyHostToDevice));
+        }
+        return MHD_OK;
+    }
+    double *dev = (region == 0 ? st_ptr(d) : const_cast<double *>(geo_ptr(d))) + (int64_t)slot * d.slot;
     HIPCHK(c, hipMemcpy(soa.data(), dev, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
     soa_aos(d, loc, nc, soa.data(), aos, down);
     if (!down)

@@ -15,18 +15,21 @@ struct Launch
     int jt;
 };
 
-void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q);
+// `mixed`: geometry stored as float (mixed-precision contexts), else double.
+void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
+               hipStream_t q);
 void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
                         const int *active, hipStream_t q);
-void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q);
+void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q);
 void launch_face_emf(const Launch &L, int mixed, hipStream_t q);
 void launch_edge_emf(const Launch &L, hipStream_t q);
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q);
 void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                   hipStream_t q);
-void launch_jcell(const Launch &L, hipStream_t q);
-void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q);
+void launch_jcell(const Launch &L, int mixed, hipStream_t q);
+void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
+                    hipStream_t q);
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q);
-void launch_divb(const Launch &L, unsigned long long *out, hipStream_t q);
+void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);
 } // namespace mhdk

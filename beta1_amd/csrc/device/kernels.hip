@@ -28,7 +28,11 @@ namespace
 typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ double *gptr(int64_t a) { return (double *)(gdouble *)(uintptr_t)a; }
 __device__ __forceinline__ double *dst_ptr(const MhdBlockDev &b) { return gptr(b.st); }
-__device__ __forceinline__ const double *dgeo_ptr(const MhdBlockDev &b) { return gptr(b.geo); }
+// Geometry is stored as GT: double for the FP64 step, float for the mixed one.
+template <typename GT> __device__ __forceinline__ const GT *dgeo_ptr(const MhdBlockDev &b)
+{
+    return (const GT *)(__attribute__((address_space(1))) GT *)(uintptr_t)b.geo;
+}
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
@@ -157,8 +161,9 @@ struct CellGeo
     double a[6][3];
 };
 
+template <typename GT>
 __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__restrict__ U,
-                                          const double *__restrict__ G, int64_t o, CellGeo &q)
+                                          const GT *__restrict__ G, int64_t o, CellGeo &q)
 {
     const int64_t sl = b.slot;
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
@@ -171,7 +176,7 @@ __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__
     {
         const int ax = f >> 1;
         const int64_t oo = o + ((f & 1) ? step[ax] : 0);
-        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        const GT *A = G + (int64_t)(G_JD + 3 * ax) * sl;
         q.a[f][0] = A[oo];
         q.a[f][1] = A[sl + oo];
         q.a[f][2] = A[2 * sl + oo];
@@ -250,6 +255,7 @@ __device__ __forceinline__ void reduce_dt(double local, unsigned long long *__re
     }
 }
 
+template <typename GT>
 __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                             unsigned long long *__restrict__ dtbits, int ucur)
 {
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseCo
         const double *U = dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
         const double *Bc = dst_ptr(b) + (int64_t)ST_BCELL * sl;
         CellGeo q;
-        load_cell(b, U, dgeo_ptr(b), o, q);
+        load_cell(b, U, dgeo_ptr<GT>(b), o, q);
         local = cell_dt(cst[b.case_id], q, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
     }
     if (wg)
@@ -308,6 +314,7 @@ __device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double
     }
 }
 
+template <typename GT>
 __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  int ucur)
 {
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdC
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     CellGeo q;
-    load_cell(b, dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, dgeo_ptr(b), o, q);
+    load_cell(b, dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, dgeo_ptr<GT>(b), o, q);
     double uc[3];
     cell_ucontra(cst[b.case_id].gamma, q, uc);
     double *UC = dst_ptr(b) + (int64_t)ST_UC * sl;
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const 
 //  Z4_Lunar/src/physics/LunarSolver_Advance.cpp:61-122).
 // U is read from buffer `ucur` and written to buffer 1-ucur.
 template <typename R, int RECON>
-__device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t sl, const double *__restrict__ A,
+__device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t sl, const R *__restrict__ A,
                                           int64_t oL, int64_t oR, int64_t step, double gm, double *F)
 {
     double WL[5], WR[5];
@@ -574,13 +581,13 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
     double *S = dst_ptr(b);
     const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
     double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
-    const double *G = dgeo_ptr(b);
+    const R *G = dgeo_ptr<R>(b);
     double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+    const double invJ = 1.0 / (double)G[(int64_t)G_JAC * sl + o];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
     {
-        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        const R *A = G + (int64_t)(G_JD + 3 * ax) * sl;
         double Flo[5], Fhi[5];
         face_flux<R, RECON>(U, sl, A, o - step[ax], o, step[ax], gm, Flo);
         face_flux<R, RECON>(U, sl, A, o, o + step[ax], step[ax], gm, Fhi);
@@ -616,6 +623,7 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
 // 163-236) fused with the next step's CFL limit (dt depends on U after the
 // Ucell sync and on this B_cell; nothing changes them before the next step).
 // Each result is stored as soon as it is known, which keeps few values live.
+template <typename GT>
 __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  unsigned long long *__restrict__ dtbits, int ucur)
 {
@@ -629,7 +637,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
         const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
         double *S = dst_ptr(b);
-        const double *G = dgeo_ptr(b);
+        const GT *G = dgeo_ptr<GT>(b);
         CellGeo q;
         load_cell(b, S + (int64_t)(ST_U0 + 5 * ucur) * sl, G, o, q);
         {
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
             const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
                                   By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
-            const double *W = G + (int64_t)G_WB * sl;
+            const GT *W = G + (int64_t)G_WB * sl;
             double bc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int n = 0; n < 6; ++n)
@@ -670,6 +678,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
 // (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891; CTOperators::
 // CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).  The 12
 // J edges of a cell are recomputed from beta-weighted face fluxes.
+template <typename GT>
 __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
     MhdBlockDev b;
@@ -680,10 +689,10 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = dst_ptr(b);
-    const double *G = dgeo_ptr(b);
+    const GT *G = dgeo_ptr<GT>(b);
     const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const double *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
-    const double *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
+    const GT *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
+    const GT *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
     auto jxi = [&](int64_t e) {
         const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
         return (1.0 * val) * ax[e];
@@ -698,7 +707,7 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     };
     const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
                           jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
-    const double *W = G + (int64_t)G_WJ * sl;
+    const GT *W = G + (int64_t)G_WJ * sl;
     double jc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 0; n < 12; ++n)
@@ -788,6 +797,7 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
+template <typename GT>
 __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long long *__restrict__ out)
 {
     MhdBlockDev b;
@@ -800,7 +810,7 @@ __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
         const double *Bx = dst_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
         const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
-                          dgeo_ptr(b)[(int64_t)G_JAC * sl + o];
+                          (double)dgeo_ptr<GT>(b)[(int64_t)G_JAC * sl + o];
         v = fabs(dv);
         if (!(v == v))
             v = 1e300;
@@ -839,18 +849,25 @@ static dim3 grid_of(const Launch &L)
 }
 static dim3 block_of(const Launch &L) { return dim3(64, (unsigned)L.jt, 1); }
 
-void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
+void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
+               hipStream_t q)
 {
-    hipLaunchKernelGGL(k_dt, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    if (mixed)
+        hipLaunchKernelGGL(k_dt<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    else
+        hipLaunchKernelGGL(k_dt<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
 }
 void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
                         const int *active, hipStream_t q)
 {
     hipLaunchKernelGGL(k_dt_finalize, dim3((ncase + 63) / 64), dim3(64), 0, q, dtbits, dt, err, cst, ncase, active);
 }
-void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q)
+void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_ucontra, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+    if (mixed)
+        hipLaunchKernelGGL(k_ucontra<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+    else
+        hipLaunchKernelGGL(k_ucontra<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
 }
 void launch_face_emf(const Launch &L, int mixed, hipStream_t q)
 {
@@ -880,14 +897,21 @@ void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, i
     else
         hipLaunchKernelGGL((k_fluid<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
-void launch_jcell(const Launch &L, hipStream_t q)
+void launch_jcell(const Launch &L, int mixed, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_jcell, grid_of(L), block_of(L), 0, q, L.tm);
+    if (mixed)
+        hipLaunchKernelGGL(k_jcell<float>, grid_of(L), block_of(L), 0, q, L.tm);
+    else
+        hipLaunchKernelGGL(k_jcell<double>, grid_of(L), block_of(L), 0, q, L.tm);
 }
 
-void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, hipStream_t q)
+void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
+                    hipStream_t q)
 {
-    hipLaunchKernelGGL(k_derived, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    if (mixed)
+        hipLaunchKernelGGL(k_derived<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    else
+        hipLaunchKernelGGL(k_derived<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
 }
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {
@@ -895,9 +919,12 @@ void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ush
         return;
     hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, cst, ushift);
 }
-void launch_divb(const Launch &L, unsigned long long *out, hipStream_t q)
+void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_divb, grid_of(L), block_of(L), 0, q, L.tm, out);
+    if (mixed)
+        hipLaunchKernelGGL(k_divb<float>, grid_of(L), block_of(L), 0, q, L.tm, out);
+    else
+        hipLaunchKernelGGL(k_divb<double>, grid_of(L), block_of(L), 0, q, L.tm, out);
 }
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
 {
