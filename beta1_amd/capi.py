@@ -45,6 +45,14 @@ SIGNATURES = {
     "mhd_add_case_lunar": (_I, [_P, C.POINTER(MhdWindDesc), _PI]),
     "mhd_add_case_mercury": (_I, [_P, C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _PI]),
     "mhd_adopt_host_case": (_I, [_P, _P, _PI]),
+    "mhd_add_case_from_dir": (_I, [_P, C.c_char_p, C.POINTER(MhdWindDesc), _PI]),
+    "mhd_refresh": (_I, [_P]),
+    "mhd_write_restart": (_I, [_P, _I, C.c_char_p, C.c_longlong, _D]),
+    "mhd_read_restart": (_I, [_P, _I, C.c_char_p, C.POINTER(C.c_longlong), _PD]),
+    "mhd_host_case_write_dir": (_I, [_P, C.c_char_p]),
+    "mhd_host_case_from_dir": (_I, [C.c_char_p, C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
+    "mhd_host_write_restart": (_I, [_P, C.c_char_p, C.c_longlong, _D]),
+    "mhd_host_read_restart": (_I, [_P, C.c_char_p, C.POINTER(C.c_longlong), _PD]),
     "mhd_finalize": (_I, [_P]),
     "mhd_step": (_I, [_P, _I]),
     "mhd_step_graph": (_I, [_P, _I]),

@@ -1117,3 +1117,105 @@ extern "C" int mhd_memory(mhd_ctx *c, double *bytes)
     *bytes = (double)c->bytes_used;
     return MHD_OK;
 }
+
+// ---------------------------------------------------- refresh and restart
+// Ghosts from the current interior (U and B syncs: physical, then halo) and
+// the derived fields, for every resident case: what the reference runs after
+// a restart read (Sync of U_H and B faces, UpdateDerivedFields_,
+// Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143-161).  Idempotent for cases
+// whose ghosts are already current.
+extern "C" int mhd_refresh(mhd_ctx *c)
+{
+    if (!c)
+        return MHD_ERR_ARG;
+    if (!c->finalized)
+        return fail(c, MHD_ERR_STATE, "context not finalized");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    run_ops(c, SY_U_BC0, c->ucur);
+    run_ops(c, SY_U_BC1, c->ucur);
+    run_ops(c, SY_U_HALO, c->ucur);
+    run_ops(c, SY_B_BC, 0);
+    run_ops(c, SY_B_HALO, 0);
+    // the next step's dt is a min accumulated by the derived kernel: restart it
+    mhdk::launch_fill_u64(c->d_dtbits, c->case_cap, dbits(1e100), c->q);
+    derived(c);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    return MHD_OK;
+}
+
+namespace
+{
+struct DeviceIO : FieldIO
+{
+    mhd_ctx *c;
+    int cid;
+    CaseRec *r;
+    DeviceIO(mhd_ctx *c_, int cid_, CaseRec *r_) : c(c_), cid(cid_), r(r_) {}
+    int nblocks() override { return r->nblocks; }
+    const MhdBlockDev &block(int b) override { return c->h_blk[r->first_block + b]; }
+    int get(int b, int field, std::vector<double> &aos) override
+    {
+        int region, slot, loc, nc;
+        field_slot(field, 0, region, slot, loc, nc);
+        aos.assign(aos_count(block(b), loc, nc), 0.0);
+        return exchange(c, cid, b, field, aos.data(), aos.size(), true);
+    }
+    int put(int b, int field, const std::vector<double> &aos) override
+    {
+        return exchange(c, cid, b, field, const_cast<double *>(aos.data()), aos.size(), false);
+    }
+};
+} // namespace
+
+extern "C" int mhd_write_restart(mhd_ctx *c, int case_id, const char *path, long long step, double time)
+{
+    if (!c || !path)
+        return MHD_ERR_ARG;
+    CaseRec *r = active_case(c, case_id);
+    if (!r)
+        return fail(c, MHD_ERR_STATE, "case not active (or context not finalized)");
+    try
+    {
+        DeviceIO io(c, case_id, r);
+        return restart_write(io, path, step, time);
+    }
+    catch (const std::exception &e)
+    {
+        return fail(c, MHD_ERR_ARG, e.what());
+    }
+}
+
+extern "C" int mhd_read_restart(mhd_ctx *c, int case_id, const char *path, long long *step, double *time)
+{
+    if (!c || !path)
+        return MHD_ERR_ARG;
+    CaseRec *r = active_case(c, case_id);
+    if (!r)
+        return fail(c, MHD_ERR_STATE, "case not active (or context not finalized)");
+    try
+    {
+        DeviceIO io(c, case_id, r);
+        const int rc = restart_read(io, path, step, time);
+        if (rc)
+            return rc;
+    }
+    catch (const std::exception &e)
+    {
+        return fail(c, MHD_ERR_ARG, e.what());
+    }
+    return mhd_refresh(c);
+}
+
+extern "C" int mhd_add_case_from_dir(mhd_ctx *c, const char *dir, const mhd_wind_desc *d, int *case_id)
+{
+    if (!c || !dir || !d)
+        return MHD_ERR_ARG;
+    mhd_host_case *h = nullptr;
+    const int rc = mhd_host_case_from_dir(dir, d, c->opt.ghost, &h);
+    if (rc)
+        return fail(c, rc, std::string("cannot build a case from ") + dir);
+    const int ra = mhd_adopt_host_case(c, h, case_id);
+    mhd_host_case_destroy(h);
+    return ra;
+}

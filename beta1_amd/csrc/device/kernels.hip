@@ -415,10 +415,15 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
             e0 = Ge;
             e1 = -Gx;
         }
+        // the component along the face normal is 0 and never read (the edge
+        // EMF and the exchange use the other two): not stored
         double *Ef = S + (int64_t)(ST_EFACE + 3 * fax) * sl;
-        Ef[o] = e0;
-        Ef[sl + o] = e1;
-        Ef[2 * sl + o] = e2;
+        if (fax != 0)
+            Ef[o] = e0;
+        if (fax != 1)
+            Ef[sl + o] = e1;
+        if (fax != 2)
+            Ef[2 * sl + o] = e2;
     }
 }
 

@@ -5,6 +5,7 @@
 // standard 3-D extension used as the Cartesian periodic test.
 #include "setup.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -18,6 +19,26 @@ void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops
     auto at = [&](int64_t rel) -> double & { return blk[rel / bs].st[rel % bs]; };
     for (const MhdOp &o : ops)
     {
+        if (o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
+        {
+            // BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184),
+            // the same arithmetic as the device op
+            const double gm = cst.gamma;
+            const int64_t L = o.stride;
+            const double rho_r = std::max(at(o.src), 2.5e-3);
+            const double r2 = std::max(rho_r, 2.5e-3);
+            const double mx = at(o.src + L), my = at(o.src + 2 * L), mz = at(o.src + 3 * L), E = at(o.src + 4 * L);
+            double eint = E - 0.5 * (mx * mx + my * my + mz * mz) / r2;
+            if (eint < 0.0)
+                eint = 0.0;
+            const double p_r = std::max((gm - 1.0) * eint, 1e-12);
+            at(o.dst) = rho_r;
+            at(o.dst + L) = rho_r * 0.0;
+            at(o.dst + 2 * L) = rho_r * 0.0;
+            at(o.dst + 3 * L) = rho_r * 0.0;
+            at(o.dst + 4 * L) = p_r / (gm - 1.0) + 0.5 * rho_r * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+            continue;
+        }
         for (int c = 0; c < o.ncomp; ++c)
         {
             const int64_t d = o.dst + c * o.stride;
@@ -206,12 +227,18 @@ struct LunarRef
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
                           const WindParams &w)
 {
-    CaseSetup cs;
-    cs.grid = make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD);
-    finalize_grid(cs);
     char key[128];
     std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
-    cs.shape_key = key;
+    return make_case_lunar_on(make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD), key, gamma,
+                              cfl, w);
+}
+
+CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gamma, double cfl, const WindParams &w)
+{
+    CaseSetup cs;
+    cs.grid = std::move(grid);
+    finalize_grid(cs);
+    cs.shape_key = shape_key;
 
     const LunarRef R;
     MhdCaseConst &c = cs.cst;

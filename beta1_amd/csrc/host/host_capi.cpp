@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "formats.h"
 #include "host_case.h"
 
 using namespace mpcns;
@@ -331,4 +332,200 @@ extern "C" int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n)
         return MHD_ERR_ARG;
     *n = h->cs.ops[sync].size();
     return MHD_OK;
+}
+
+// ------------------------------------------------- CASE directory, restart
+namespace
+{
+// The restart whitelist of the Lunar solver (Z4_Lunar/src/LunarSolver.cpp:38-39).
+struct RestartSpec
+{
+    const char *name;
+    int field;
+};
+const RestartSpec kRestart[4] = {
+    {"U_H", MHD_F_U_H}, {"B_xi", MHD_F_B_XI}, {"B_eta", MHD_F_B_ETA}, {"B_zeta", MHD_F_B_ZETA}};
+
+void box_of(const MhdBlockDev &d, int loc, int *lo, int *hi)
+{
+    const int N[3] = {d.ni, d.nj, d.nk};
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = -d.g;
+        hi[a] = loc_count(loc, a, N[a]) + d.g;
+    }
+}
+} // namespace
+
+namespace mpcns
+{
+int restart_write(FieldIO &io, const char *path, long long step, double time)
+{
+    std::vector<RestartField> fields;
+    for (const RestartSpec &S : kRestart)
+    {
+        int region, slot, loc, nc;
+        field_slot(S.field, 0, region, slot, loc, nc);
+        RestartField F;
+        F.name = S.name;
+        F.loc_code = restart_loc_code(loc);
+        F.ncomp = nc;
+        F.nghost = io.block(0).g;
+        for (int b = 0; b < io.nblocks(); ++b)
+        {
+            RestartBlock B;
+            box_of(io.block(b), loc, B.lo, B.hi);
+            const int rc = io.get(b, S.field, B.v);
+            if (rc)
+                return rc;
+            F.blocks.push_back(std::move(B));
+        }
+        fields.push_back(std::move(F));
+    }
+    write_restart_file(path, step, time, fields);
+    return MHD_OK;
+}
+
+int restart_read(FieldIO &io, const char *path, long long *step, double *time)
+{
+    long long st = 0;
+    double t = 0.0;
+    const std::vector<RestartField> fields = read_restart_file(path, st, t);
+    for (const RestartField &F : fields)
+    {
+        const RestartSpec *spec = nullptr;
+        for (const RestartSpec &S : kRestart)
+            if (F.name == S.name)
+                spec = &S;
+        if (!spec)
+            continue; // not part of this step's state: skipped, as the reference does
+        int region, slot, loc, nc;
+        field_slot(spec->field, 0, region, slot, loc, nc);
+        if (F.loc_code != restart_loc_code(loc) || F.ncomp != nc)
+            throw std::runtime_error("restart: location or component mismatch for " + F.name);
+        if ((int)F.blocks.size() != io.nblocks())
+            throw std::runtime_error("restart: block count mismatch");
+        for (int b = 0; b < io.nblocks(); ++b)
+        {
+            const RestartBlock &B = F.blocks[b];
+            int lo[3], hi[3];
+            box_of(io.block(b), loc, lo, hi);
+            for (int a = 0; a < 3; ++a)
+                if (B.hi[a] - B.lo[a] - 2 * F.nghost != hi[a] - lo[a] - 2 * io.block(b).g)
+                    throw std::runtime_error("restart: block extent mismatch for " + F.name);
+            std::vector<double> aos;
+            int rc = io.get(b, spec->field, aos);
+            if (rc)
+                return rc;
+            copy_overlap(B, nc, lo, hi, aos.data());
+            if ((rc = io.put(b, spec->field, aos)))
+                return rc;
+        }
+    }
+    if (step)
+        *step = st;
+    if (time)
+        *time = t;
+    return MHD_OK;
+}
+} // namespace mpcns
+
+namespace
+{
+struct HostIO : FieldIO
+{
+    mhd_host_case *h;
+    explicit HostIO(mhd_host_case *h_) : h(h_) {}
+    int nblocks() override { return (int)h->cs.blk.size(); }
+    const MhdBlockDev &block(int b) override { return h->cs.blk[b].d; }
+    int get(int b, int field, std::vector<double> &aos) override
+    {
+        int region, slot, loc, nc;
+        field_slot(field, 0, region, slot, loc, nc);
+        BlockArrays &A = h->cs.blk[b];
+        aos.assign(aos_count(A.d, loc, nc), 0.0);
+        soa_aos(A.d, loc, nc, &A.st[(size_t)slot * A.d.slot], aos.data(), true);
+        return MHD_OK;
+    }
+    int put(int b, int field, const std::vector<double> &aos) override
+    {
+        int region, slot, loc, nc;
+        field_slot(field, 0, region, slot, loc, nc);
+        BlockArrays &A = h->cs.blk[b];
+        soa_aos(A.d, loc, nc, &A.st[(size_t)slot * A.d.slot], const_cast<double *>(aos.data()), false);
+        return MHD_OK;
+    }
+};
+} // namespace
+
+extern "C" int mhd_host_case_write_dir(mhd_host_case *h, const char *dir)
+{
+    if (!h || !dir || h->cs.blk.empty() || h->cs.blk[0].st.empty())
+        return MHD_ERR_ARG;
+    try
+    {
+        write_case_dir(h->cs.grid, dir);
+        return MHD_OK;
+    }
+    catch (const std::exception &e)
+    {
+        h->err = e.what();
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, int ghost, mhd_host_case **out)
+{
+    if (!dir || !d || !out || ghost < 1)
+        return MHD_ERR_ARG;
+    try
+    {
+        auto h = std::make_unique<mhd_host_case>();
+        h->cs = make_case_lunar_on(read_case_dir(dir, ghost), std::string("dir:") + dir + ":" + std::to_string(ghost),
+                                   d->gamma, d->cfl, wind_params(*d));
+        *out = h.release();
+        return MHD_OK;
+    }
+    catch (const std::exception &)
+    {
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_write_restart(mhd_host_case *h, const char *path, long long step, double time)
+{
+    if (!h || !path || h->cs.blk.empty() || h->cs.blk[0].st.empty())
+        return MHD_ERR_ARG;
+    try
+    {
+        HostIO io(h);
+        return restart_write(io, path, step, time);
+    }
+    catch (const std::exception &e)
+    {
+        h->err = e.what();
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_read_restart(mhd_host_case *h, const char *path, long long *step, double *time)
+{
+    if (!h || !path || h->cs.blk.empty() || h->cs.blk[0].st.empty())
+        return MHD_ERR_ARG;
+    try
+    {
+        HostIO io(h);
+        const int rc = restart_read(io, path, step, time);
+        if (rc)
+            return rc;
+        // ghosts from the new interior: U and B syncs (physical, then halo)
+        for (int sy : {SY_U_BC0, SY_U_BC1, SY_U_HALO, SY_B_BC, SY_B_HALO})
+            apply_ops_host(h->cs.blk, h->cs.ops[sy], h->cs.cst);
+        return MHD_OK;
+    }
+    catch (const std::exception &e)
+    {
+        h->err = e.what();
+        return MHD_ERR_ARG;
+    }
 }

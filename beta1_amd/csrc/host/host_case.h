@@ -2,6 +2,7 @@
 // the C-ABI; the device context adopts its contents without rebuilding them.
 #pragma once
 #include <string>
+#include <vector>
 
 #include "../../../include/beta1_mhd.h"
 #include "setup.h"
@@ -18,4 +19,17 @@ namespace mpcns
 WindParams wind_params(const mhd_wind_desc &d);
 DipoleParams dipole_params(const mhd_dipole_desc &d);
 bool wind_desc_ok(const mhd_wind_desc *d);
+
+// Whitelisted restart fields of one case through an AoS get/put interface
+// (host arrays or device memory); get/put return an mhd_status.
+struct FieldIO
+{
+    virtual ~FieldIO() = default;
+    virtual int nblocks() = 0;
+    virtual const MhdBlockDev &block(int b) = 0;
+    virtual int get(int b, int field, std::vector<double> &aos) = 0;
+    virtual int put(int b, int field, const std::vector<double> &aos) = 0;
+};
+int restart_write(FieldIO &io, const char *path, long long step, double time);
+int restart_read(FieldIO &io, const char *path, long long *step, double *time);
 } // namespace mpcns

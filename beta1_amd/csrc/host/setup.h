@@ -197,6 +197,9 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
 CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t seed, double noise);
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma,
                           double cfl, const WindParams &w);
+// Lunar physics and initial state on a given grid (ghost nodes are filled
+// here); shape_key groups cases that may share geometry.
+CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gamma, double cfl, const WindParams &w);
 
 // Deterministic generator (SplitMix64).
 uint64_t splitmix64(uint64_t &s);

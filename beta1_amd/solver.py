@@ -76,6 +76,15 @@ class MercuryDesc(WindDesc):
         return MhdDipoleDesc(self.B_eq_nT, self.offset_z, self.softening)
 
 
+@dataclass
+class CaseDirDesc:
+    """A Lunar case on the grid of a reference CASE directory (Plot3D grid,
+    boundary and inner files); the wind comes from `wind` (its grid fields
+    are ignored)."""
+    path: str
+    wind: WindDesc = dfield(default_factory=WindDesc)
+
+
 def _check(rc, ctx=None, what=""):
     if rc != 0:
         msg = ""
@@ -99,6 +108,9 @@ class HostCase:
         elif isinstance(desc, MercuryDesc):
             _check(lib().mhd_host_case_mercury(C.byref(desc.c()), C.byref(desc.dipole()), ghost, C.byref(self.h)),
                    what="mhd_host_case_mercury")
+        elif isinstance(desc, CaseDirDesc):
+            _check(lib().mhd_host_case_from_dir(desc.path.encode(), C.byref(desc.wind.c()), ghost, C.byref(self.h)),
+                   what=f"mhd_host_case_from_dir({desc.path})")
         else:
             _check(lib().mhd_host_case_lunar(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_lunar")
         self.desc = desc
@@ -140,6 +152,18 @@ class HostCase:
         lib().mhd_host_constants(self.h, _dptr(out))
         return out
 
+    def write_dir(self, path):
+        """Write the case's grid as a reference CASE directory."""
+        _check(lib().mhd_host_case_write_dir(self.h, path.encode()), what="write_dir")
+
+    def write_restart(self, path, step=0, time=0.0):
+        _check(lib().mhd_host_write_restart(self.h, path.encode(), step, time), what="write_restart")
+
+    def read_restart(self, path):
+        st, t = C.c_longlong(), C.c_double()
+        _check(lib().mhd_host_read_restart(self.h, path.encode(), C.byref(st), C.byref(t)), what="read_restart")
+        return st.value, t.value
+
     def apply_sync(self, sync):
         _check(lib().mhd_host_apply_sync(self.h, sync), what="apply_sync")
 
@@ -174,6 +198,8 @@ class MhdContext:
             return lib().mhd_add_case_orszag_tang(self.ctx, C.byref(desc.c()), C.byref(cid))
         if isinstance(desc, MercuryDesc):
             return lib().mhd_add_case_mercury(self.ctx, C.byref(desc.c()), C.byref(desc.dipole()), C.byref(cid))
+        if isinstance(desc, CaseDirDesc):
+            return lib().mhd_add_case_from_dir(self.ctx, desc.path.encode(), C.byref(desc.wind.c()), C.byref(cid))
         return lib().mhd_add_case_lunar(self.ctx, C.byref(desc.c()), C.byref(cid))
 
     def add(self, desc):
@@ -228,6 +254,19 @@ class MhdContext:
 
     def sync(self):
         _check(lib().mhd_synchronize(self.ctx), self.ctx, "sync")
+
+    def refresh(self):
+        """Re-sync U/B ghosts and recompute derived fields after uploads."""
+        _check(lib().mhd_refresh(self.ctx), self.ctx, "refresh")
+
+    def write_restart(self, cid, path, step=0, time=0.0):
+        _check(lib().mhd_write_restart(self.ctx, cid, path.encode(), step, time), self.ctx, "write_restart")
+
+    def read_restart(self, cid, path):
+        st, t = C.c_longlong(), C.c_double()
+        _check(lib().mhd_read_restart(self.ctx, cid, path.encode(), C.byref(st), C.byref(t)), self.ctx,
+               "read_restart")
+        return st.value, t.value
 
     def info(self, cid):
         nb, ni, nj, nk, g = (C.c_int() for _ in range(5))

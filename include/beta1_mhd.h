@@ -118,6 +118,16 @@ int mhd_add_case_mercury(mhd_ctx *ctx, const mhd_wind_desc *d, const mhd_dipole_
    valid on it.  No counterpart in the reference (its set-up runs once, in
    the solver constructor). */
 int mhd_adopt_host_case(mhd_ctx *ctx, mhd_host_case *h, int *case_id);
+/* Lunar physics and initial state on a grid read from a reference CASE
+   directory: <dir>/geometry/grid0000.grd (Plot3D binary,
+   Grid::Read_Grid_Binary, src/1_grid/1_MPCNS_Grid.cpp:123-221) and
+   <dir>/geometry/boundary_condition/{boundary0000.txt, boundary_name.fvbnd,
+   inner0000.txt} (Grid::Read_Phy_Boundary / Read_Phy_Name /
+   Read_Inner_Boundary, :224-505), interface transforms derived as
+   TOPO::build_topology does (src/2_topology/2_MPCNS_Topology.cpp:77-165).
+   Whole-face patches, Farfield and Solid_Surface conditions.  The grid
+   fields of d (nang, nrad, r_in, r_out) are ignored. */
+int mhd_add_case_from_dir(mhd_ctx *ctx, const char *dir, const mhd_wind_desc *d, int *case_id);
 /* Upload everything added so far and compute the derived fields
    (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143).
    After mhd_finalize, mhd_add_case_* admits a case into the running context
@@ -142,6 +152,17 @@ int mhd_last_dt(mhd_ctx *ctx, int case_id, double *dt);
 /* max |div B| over interior cells (calc_divB, LunarSolver_Tools.cpp:685-775). */
 int mhd_divb_max(mhd_ctx *ctx, int case_id, double *out);
 int mhd_synchronize(mhd_ctx *ctx);
+/* Re-sync U and B ghosts from the current interior and recompute the derived
+   fields of every case (after mhd_upload of state, as the reference does
+   after a restart read). */
+int mhd_refresh(mhd_ctx *ctx);
+/* MPCNSRST restart of a case's U_H, B_xi, B_eta, B_zeta
+   (IOModule::WriteRestartBinFile / ReadRestartBinFile,
+   src/5_io/IOModule_Bin.cpp:168-330; whitelist of
+   Z4_Lunar/src/LunarSolver.cpp:38-39).  Reading accepts any ghost depth in
+   the file, copies the overlap and calls mhd_refresh. */
+int mhd_write_restart(mhd_ctx *ctx, int case_id, const char *path, long long step, double time);
+int mhd_read_restart(mhd_ctx *ctx, int case_id, const char *path, long long *step, double *time);
 
 /* Introspection and data exchange (FieldBlock::raw_data round trip,
    include/3_field/1_Field_Block.h:47). */
@@ -177,6 +198,13 @@ int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner);
 int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case **out);
 int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_case **out);
 int mhd_host_case_mercury(const mhd_wind_desc *d, const mhd_dipole_desc *dp, int ghost, mhd_host_case **out);
+/* CASE directory of the case's grid, readable by the reference
+   (Grid::Read_* above) and by mhd_host_case_from_dir / mhd_add_case_from_dir. */
+int mhd_host_case_write_dir(mhd_host_case *h, const char *dir);
+int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, int ghost, mhd_host_case **out);
+/* MPCNSRST restart of the host arrays (read also re-syncs U and B ghosts). */
+int mhd_host_write_restart(mhd_host_case *h, const char *path, long long step, double time);
+int mhd_host_read_restart(mhd_host_case *h, const char *path, long long *step, double *time);
 int mhd_host_case_destroy(mhd_host_case *h);
 int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
 int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);
@@ -187,8 +215,7 @@ int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n)
 int mhd_host_constants(mhd_host_case *h, double *out22);
 /* apply one halo / boundary op list (index as in the library's sync table:
    0 Eface BC, 1 Eface halo, 2 Eedge BC, 3-5 Ucell BC0/BC1/halo, 6-7 Bface
-   BC/halo, 8 J_cell BC, 9 corner velocities) to the host arrays; copy, negate,
-   zero and constant ops only */
+   BC/halo, 8 J_cell BC, 9 corner velocities) to the host arrays */
 int mhd_host_apply_sync(mhd_host_case *h, int sync);
 int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n);
 

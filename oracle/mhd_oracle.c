@@ -11,6 +11,12 @@
 #include <string.h>
 
 /* std::max / std::min semantics (first argument wins ties) */
+/* Per-cell loops over the i planes of a block run on c->nthreads OpenMP
+ * threads; every iteration writes its own cells only, so the result does not
+ * depend on the thread count. */
+#define OMP_PRAGMA(x) _Pragma(#x)
+#define OMP_FOR OMP_PRAGMA(omp parallel for schedule(static) num_threads(c->nthreads) if (c->nthreads > 1))
+
 static inline double dmax(double a, double b) { return (a < b) ? b : a; }
 static inline double dmin(double a, double b) { return (b < a) ? b : a; }
 
@@ -68,7 +74,8 @@ struct or_ctx
     oblock *b;
     double dt;
     int wrap;
-    hmap hm[8]; /* per location */
+    int nthreads; /* OpenMP threads of the per-cell loops (1: serial) */
+    hmap hm[8];   /* per location */
 };
 
 static const struct
@@ -90,6 +97,7 @@ or_ctx *or_create(const or_desc *d)
 {
     or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
     c->d = *d;
+    c->nthreads = 1;
     c->b = (oblock *)calloc((size_t)d->nblocks, sizeof(oblock));
     return c;
 }
@@ -155,6 +163,7 @@ double *or_nodes(or_ctx *c, int b, int comp, long *count)
 }
 
 void or_use_periodic_wrap(or_ctx *c, int on) { c->wrap = on; }
+void or_set_threads(or_ctx *c, int n) { c->nthreads = n > 1 ? n : 1; }
 
 static inline double NODE(const oblock *B, int q, int i, int j, int k)
 {
@@ -298,6 +307,9 @@ static void halo_apply(or_ctx *c, int loc, int fid_base, int is_triple, int efac
     }
     build_hmap(c, loc);
     const hmap *H = &c->hm[loc];
+    /* each entry writes its own ghost; sources are interior DOFs or ghosts
+       written by the physical phase, none written in this pass */
+    OMP_FOR
     for (long q = 0; q < H->n; ++q)
     {
         const hentry *E = &H->e[q];
@@ -628,6 +640,7 @@ static void calc_pv(or_ctx *c)
     {
         fblk *U = &c->b[b].f[OF_UH], *PV = &c->b[b].f[OF_PV];
         const int g = U->g;
+        OMP_FOR
         for (int i = -g; i < U->cnt[0] + g; ++i)
             for (int j = -g; j < U->cnt[1] + g; ++j)
                 for (int k = -g; k < U->cnt[2] + g; ++k)
@@ -659,6 +672,7 @@ static void calc_uplus(or_ctx *c)
     {
         fblk *U = &c->b[b].f[OF_UH], *PV = &c->b[b].f[OF_PV], *Up = &c->b[b].f[OF_UPLUS];
         const int g = U->g;
+        OMP_FOR
         for (int i = -g; i < U->cnt[0] + g; ++i)
             for (int j = -g; j < U->cnt[1] + g; ++j)
                 for (int k = -g; k < U->cnt[2] + g; ++k)
@@ -679,6 +693,7 @@ static void calc_bcell(or_ctx *c)
         fblk *Bc = &B->f[OF_BCELL], *Bi = &B->f[OF_BIND], *W = &B->f[OF_WB];
         fblk *Bx = &B->f[OF_BXI], *By = &B->f[OF_BET], *Bz = &B->f[OF_BZE];
         fblk *Ax = &B->f[OF_BADDXI], *Ay = &B->f[OF_BADDET], *Az = &B->f[OF_BADDZE];
+        OMP_FOR
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -718,6 +733,7 @@ static void calc_jedge(or_ctx *c)
         fblk *bx = &B->f[OF_BETAXI], *by = &B->f[OF_BETAET], *bz = &B->f[OF_BETAZE];
         fblk *Jx = &B->f[OF_JXI], *Jy = &B->f[OF_JET], *Jz = &B->f[OF_JZE];
         fblk *ax = &B->f[OF_ALXI], *ay = &B->f[OF_ALET], *az = &B->f[OF_ALZE];
+        OMP_FOR
         for (int i = 0; i < Jx->cnt[0]; ++i)
             for (int j = 0; j < Jx->cnt[1]; ++j)
                 for (int k = 0; k < Jx->cnt[2]; ++k)
@@ -726,6 +742,7 @@ static void calc_jedge(or_ctx *c)
                                        (AT(bz, i, j, k, 0) * AT(Fz, i, j, k, 0) - AT(bz, i, j - 1, k, 0) * AT(Fz, i, j - 1, k, 0));
                     AT(Jx, i, j, k, 0) = 1.0 * val;
                 }
+        OMP_FOR
         for (int i = 0; i < Jy->cnt[0]; ++i)
             for (int j = 0; j < Jy->cnt[1]; ++j)
                 for (int k = 0; k < Jy->cnt[2]; ++k)
@@ -734,6 +751,7 @@ static void calc_jedge(or_ctx *c)
                                        (AT(bz, i - 1, j, k, 0) * AT(Fz, i - 1, j, k, 0) - AT(bz, i, j, k, 0) * AT(Fz, i, j, k, 0));
                     AT(Jy, i, j, k, 0) = 1.0 * val;
                 }
+        OMP_FOR
         for (int i = 0; i < Jz->cnt[0]; ++i)
             for (int j = 0; j < Jz->cnt[1]; ++j)
                 for (int k = 0; k < Jz->cnt[2]; ++k)
@@ -742,12 +760,15 @@ static void calc_jedge(or_ctx *c)
                                        (AT(by, i, j, k, 0) * AT(Fy, i, j, k, 0) - AT(by, i - 1, j, k, 0) * AT(Fy, i - 1, j, k, 0));
                     AT(Jz, i, j, k, 0) = 1.0 * val;
                 }
+        OMP_FOR
         for (int i = 0; i < Jx->cnt[0]; ++i)
             for (int j = 0; j < Jx->cnt[1]; ++j)
                 for (int k = 0; k < Jx->cnt[2]; ++k) AT(Jx, i, j, k, 0) *= AT(ax, i, j, k, 0);
+        OMP_FOR
         for (int i = 0; i < Jy->cnt[0]; ++i)
             for (int j = 0; j < Jy->cnt[1]; ++j)
                 for (int k = 0; k < Jy->cnt[2]; ++k) AT(Jy, i, j, k, 0) *= AT(ay, i, j, k, 0);
+        OMP_FOR
         for (int i = 0; i < Jz->cnt[0]; ++i)
             for (int j = 0; j < Jz->cnt[1]; ++j)
                 for (int k = 0; k < Jz->cnt[2]; ++k) AT(Jz, i, j, k, 0) *= AT(az, i, j, k, 0);
@@ -763,6 +784,7 @@ static void calc_jcell(or_ctx *c)
         oblock *B = &c->b[b];
         fblk *Jc = &B->f[OF_JCELL], *W = &B->f[OF_WJ];
         fblk *Jx = &B->f[OF_JXI], *Jy = &B->f[OF_JET], *Jz = &B->f[OF_JZE];
+        OMP_FOR
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -803,6 +825,12 @@ int or_compute_dt(or_ctx *c, double *dt_out)
         oblock *B = &c->b[b];
         fblk *U = &B->f[OF_UH], *Bc = &B->f[OF_BCELL], *Jac = &B->f[OF_JAC];
         fblk *Ax = &B->f[OF_JDXI], *Ay = &B->f[OF_JDET], *Az = &B->f[OF_JDZE];
+        /* per-thread minima combined with the same min: the minimum of finite
+           values does not depend on the order they are taken in */
+        OMP_PRAGMA(omp parallel num_threads(c->nthreads) if (c->nthreads > 1))
+        {
+        double dt_mine = 1e100;
+        OMP_PRAGMA(omp for schedule(static))
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -846,9 +874,12 @@ int or_compute_dt(or_ctx *c, double *dt_out)
                     if (denom > 0.0)
                     {
                         const double dtc = CFL * V / denom;
-                        dt_local = dmin(dt_local, dtc);
+                        dt_mine = dmin(dt_mine, dtc);
                     }
                 }
+        OMP_PRAGMA(omp critical(or_dt_min))
+        dt_local = dmin(dt_local, dt_mine);
+        }
     }
     *dt_out = dt_local;
     if (!isfinite(dt_local) || dt_local < c->d.dt_floor) return -2;
@@ -911,6 +942,7 @@ static void scheme_u(or_ctx *c)
         for (int d = 0; d < 3; ++d)
         {
             fblk *Fd = F[d];
+            OMP_FOR
             for (int i = 0; i < Fd->cnt[0]; ++i)
                 for (int j = 0; j < Fd->cnt[1]; ++j)
                     for (int k = 0; k < Fd->cnt[2]; ++k)
@@ -938,6 +970,7 @@ static void scheme_u(or_ctx *c)
                         for (int m = 0; m < 5; ++m) AT(Fd, i, j, k, m) = flux[m];
                     }
         }
+        OMP_FOR
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -961,6 +994,7 @@ static void add_source(or_ctx *c)
     {
         oblock *B = &c->b[b];
         fblk *U = &B->f[OF_UH], *PV = &B->f[OF_PV], *Bc = &B->f[OF_BCELL], *J = &B->f[OF_JCELL], *RHS = &B->f[OF_RHSH];
+        OMP_FOR
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -1088,6 +1122,7 @@ static void face_emf(or_ctx *c)
         for (int fax = 0; fax < 3; ++fax)
         {
             fblk *Ef = &B->f[OF_EFXI + fax];
+            OMP_FOR
             for (int i = 0; i < Ef->cnt[0]; ++i)
                 for (int j = 0; j < Ef->cnt[1]; ++j)
                     for (int k = 0; k < Ef->cnt[2]; ++k)
@@ -1142,6 +1177,7 @@ static void edge_emf(or_ctx *c)
         fblk *Fx = &B->f[OF_EFXI], *Fy = &B->f[OF_EFET], *Fz = &B->f[OF_EFZE];
         fblk *Bx = v.B[0], *By = v.B[1], *Bz = v.B[2];
         fblk *Ex = &B->f[OF_EXI], *Ey = &B->f[OF_EET], *Ez = &B->f[OF_EZE];
+        OMP_FOR
         for (int i = 0; i < Ex->cnt[0]; ++i)
             for (int j = 0; j < Ex->cnt[1]; ++j)
                 for (int k = 0; k < Ex->cnt[2]; ++k)
@@ -1153,6 +1189,7 @@ static void edge_emf(or_ctx *c)
                     const double dBe = AT(By, i, j, k, 0) - AT(By, i, j, k - 1, 0);
                     AT(Ex, i, j, k, 0) = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
                 }
+        OMP_FOR
         for (int i = 0; i < Ey->cnt[0]; ++i)
             for (int j = 0; j < Ey->cnt[1]; ++j)
                 for (int k = 0; k < Ey->cnt[2]; ++k)
@@ -1164,6 +1201,7 @@ static void edge_emf(or_ctx *c)
                     const double dBz = AT(Bz, i, j, k, 0) - AT(Bz, i - 1, j, k, 0);
                     AT(Ey, i, j, k, 0) = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
                 }
+        OMP_FOR
         for (int i = 0; i < Ez->cnt[0]; ++i)
             for (int j = 0; j < Ez->cnt[1]; ++j)
                 for (int k = 0; k < Ez->cnt[2]; ++k)
@@ -1186,14 +1224,17 @@ static void curl_edge_to_face(or_ctx *c, double mult)
         oblock *B = &c->b[b];
         fblk *Ex = &B->f[OF_EXI], *Ey = &B->f[OF_EET], *Ez = &B->f[OF_EZE];
         fblk *Rx = &B->f[OF_RHSBXI], *Ry = &B->f[OF_RHSBET], *Rz = &B->f[OF_RHSBZE];
+        OMP_FOR
         for (int i = 0; i < Rx->cnt[0]; ++i)
             for (int j = 0; j < Rx->cnt[1]; ++j)
                 for (int k = 0; k < Rx->cnt[2]; ++k)
                     AT(Rx, i, j, k, 0) = mult * ((AT(Ey, i, j, k, 0) - AT(Ey, i, j, k + 1, 0)) + (AT(Ez, i, j + 1, k, 0) - AT(Ez, i, j, k, 0)));
+        OMP_FOR
         for (int i = 0; i < Ry->cnt[0]; ++i)
             for (int j = 0; j < Ry->cnt[1]; ++j)
                 for (int k = 0; k < Ry->cnt[2]; ++k)
                     AT(Ry, i, j, k, 0) = mult * ((AT(Ex, i, j, k + 1, 0) - AT(Ex, i, j, k, 0)) + (AT(Ez, i, j, k, 0) - AT(Ez, i + 1, j, k, 0)));
+        OMP_FOR
         for (int i = 0; i < Rz->cnt[0]; ++i)
             for (int j = 0; j < Rz->cnt[1]; ++j)
                 for (int k = 0; k < Rz->cnt[2]; ++k)
@@ -1225,6 +1266,7 @@ static void apply_update(or_ctx *c, double dt)
     {
         oblock *B = &c->b[b];
         fblk *U = &B->f[OF_UH], *R = &B->f[OF_RHSH];
+        OMP_FOR
         for (int i = 0; i < B->n[0]; ++i)
             for (int j = 0; j < B->n[1]; ++j)
                 for (int k = 0; k < B->n[2]; ++k)
@@ -1232,6 +1274,7 @@ static void apply_update(or_ctx *c, double dt)
         for (int a = 0; a < 3; ++a)
         {
             fblk *Bf = &B->f[OF_BXI + a], *RB = &B->f[OF_RHSBXI + a];
+            OMP_FOR
             for (int i = 0; i < Bf->cnt[0]; ++i)
                 for (int j = 0; j < Bf->cnt[1]; ++j)
                     for (int k = 0; k < Bf->cnt[2]; ++k) AT(Bf, i, j, k, 0) += dt * AT(RB, i, j, k, 0);

@@ -64,6 +64,9 @@ double or_divb_max(or_ctx *c);
 /* 1: the single-block fully periodic halo uses an index wrap instead of
    the generic interface resolution */
 void or_use_periodic_wrap(or_ctx *c, int on);
+/* OpenMP threads for the per-cell loops (default 1); results are identical
+   for every thread count */
+void or_set_threads(or_ctx *c, int n);
 /* run only the named sync group on the current arrays (tests) */
 int or_sync(or_ctx *c, const char *group);
 

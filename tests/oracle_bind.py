@@ -50,6 +50,7 @@ def olib():
         L.or_divb_max.restype = C.c_double
         L.or_divb_max.argtypes = [C.c_void_p]
         L.or_use_periodic_wrap.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.or_sync.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
@@ -64,7 +65,7 @@ STATE = ["U_H", "B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta"]
 class Oracle:
     """CPU FP64 restatement of the reference step, fed from a HostCase."""
 
-    def __init__(self, hc, recon=0, periodic_wrap=False):
+    def __init__(self, hc, recon=0, periodic_wrap=False, threads=1):
         k = hc.constants()
         d = OrDesc()
         d.nblocks, d.g, d.recon = hc.nblocks, hc.g, recon
@@ -95,6 +96,7 @@ class Oracle:
                 p = L.or_nodes(self.c, b, comp, C.byref(cnt))
                 np.ctypeslib.as_array(p, shape=(cnt.value,))[:] = arr.ravel()
         L.or_use_periodic_wrap(self.c, 1 if periodic_wrap else 0)
+        L.or_set_threads(self.c, threads)
         L.or_update_derived(self.c)
 
     def __del__(self):

@@ -375,6 +375,17 @@ def test_lunar_div_b_at_roundoff(lunar_run):
     assert orc.divb_max() < 1e-12
 
 
+def test_threaded_oracle_equals_serial():
+    """The OpenMP loops of the oracle (used for the multi-core CPU baseline)
+    give the serial result bit for bit."""
+    hc = HostCase(WindDesc(nang=6, nrad=12), ghost=G)
+    a, b = Oracle(hc), Oracle(hc, threads=4)
+    assert a.step(4) == b.step(4)
+    for blk in range(6):
+        for name in ("U_H", "B_xi", "B_eta", "B_zeta", "J_cell"):
+            assert np.array_equal(a.view(name, blk), b.view(name, blk)), (blk, name)
+
+
 def test_dt_abort_below_floor():
     """dt < 1e-7 aborts the step (LunarSolver_Timestep.cpp:128-139); NaN cells
     are skipped by the reference's min, exactly as here."""
