@@ -21,25 +21,38 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic HBM bytes per interior cell and launch for each step kernel:
-# every array the kernel must touch counted once at its stored width.
-KERNEL_BYTES = {
-    "k_dt": 40 + 24 + 8 + 72,                 # U, B_cell, Jac, JD
-    "k_ucontra": 40 + 72 + 8 + 24,            # U, JD, Jac -> UC
-    "k_face_emf": 24 + 24 + 24 + 72,          # B, Badd, UC -> Eface
-    "k_edge_emf": 72 + 24 + 24 + 24,          # Eface, UC, B -> E
-    "k_fluid": 40 + 72 + 8 + 24 + 24 + 40,    # U, JD, Jac, B_cell, J_cell -> U'
-    "k_curl_update": 24 + 24 + 24,            # E, B -> B
-    # U, JD, Jac, B, Badd, W_B -> UC, B_cell (+ next step's dt)
-    "k_derived": 40 + 72 + 8 + 24 + 24 + 144 + 24 + 24,
-    # B, beta, alpha, W_J -> J_cell
-    "k_jcell": 24 + 24 + 24 + 288 + 24,
+# every array the kernel must touch counted once at its stored width, as
+# (per-case state, geometry in FP64); mixed precision stores geometry as
+# float (half the geometry bytes).
+KERNEL_PARTS = {
+    "k_dt": (40 + 24, 8 + 72),                     # U, B_cell | Jac, JD
+    "k_ucontra": (40 + 24, 72 + 8),                # U -> UC | JD, Jac
+    "k_face_emf": (24 + 24 + 24 + 72, 0),          # B, Badd, UC -> Eface
+    "k_edge_emf": (72 + 24 + 24 + 24, 0),          # Eface, UC, B -> E
+    "k_fluid": (40 + 24 + 24 + 40, 72 + 8),        # U, B_cell, J_cell -> U' | JD, Jac
+    "k_curl_update": (24 + 24 + 24, 0),            # E, B -> B
+    # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac, W_B
+    "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8 + 144),
+    # B -> J_cell | beta, alpha, W_J
+    "k_jcell": (24 + 24, 24 + 24 + 288),
 }
+
+
+def kernel_bytes(prec="fp64"):
+    return {k: s + (g if prec == "fp64" else g // 2) for k, (s, g) in KERNEL_PARTS.items()}
+
+
+KERNEL_BYTES = kernel_bytes("fp64")
 HBM_PEAK_GBS = 8000.0
 # Whole-step algorithmic bytes per cell-update, each array once per step:
 # U R/W 80 + B_face R/W 48 + Badd R 24 (per case) + geometry Jac 8, JD 72,
-# B_cell weights 144, J_cell weights 288, beta 24, alpha 24 (read once per
-# case: no kernel shares a geometry tile between cases).
-STEP_BYTES = 152 + 560
+# B_cell weights 144, J_cell weights 288, beta 24, alpha 24, counted once per
+# case (K = 1: the geometry-shared tile order lets cases of one grid meet
+# their geometry in L2, which is a cache effect, not a guaranteed single
+# read).  Mixed precision stores the geometry as float: 280 B.
+STATE_BYTES = 152
+GEO_BYTES = {"fp64": 560, "mixed": 280}
+STEP_BYTES = STATE_BYTES + GEO_BYTES["fp64"]
 SEED = 20261203
 
 
@@ -107,6 +120,10 @@ def bench_batch(args):
                 log(f"batch set-up: {n + 1} cases placed after {time.time() - t0:.1f}s")
     log(f"batch set-up {time.time() - t0:.1f}s: {len(descs)} cases, {cells} cells, "
         f"{ctx.memory_bytes() / 1e9:.1f} GB on device")
+    # the tuner ran on the first case alone: tune again on the whole batch
+    # (this is where the geometry-shared tile order can pay)
+    ctx.autotune()
+    log(f"batch launch shapes: {ctx.tuning()}")
     ctx.step(args.warmup, graph=True)
     ctx.sync()
     t0 = time.perf_counter()
@@ -115,12 +132,22 @@ def bench_batch(args):
     el = time.perf_counter() - t0
     value = cells * args.steps / el / 1e6
     log(f"batch: {args.steps} steps in {el:.2f}s, {value:.1f} Mcell-updates/s")
-    hbm = value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9)
+    step_bytes = STATE_BYTES + GEO_BYTES[prec]
+    hbm = value * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9)
+    ctx.profile(True)
+    ctx.step(4)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    kb = kernel_bytes(prec)
+    per_kernel = {k: {"ms": round(ms / n, 4), "alg_GBs": round(kb[k] * cells / (ms / n * 1e-3) / 1e9, 1)
+                      if k in kb else None} for k, (ms, n) in prof.items() if n}
+    log(f"batch per-kernel avg ms and algorithmic GB/s: {per_kernel}")
     return _record("Mcell-updates/s (64-case batch)", value, "Mcell-updates/s", args.steps, args.warmup,
                    1e3 * el / args.steps, "f32/f64" if prec == "mixed" else "f64",
                    {"workload": f"batch_{len(descs)}_shells_{prec}", "cases": len(descs), "cells": cells,
                     "seed": SEED, "graph": True, "autotune": ctx.tuning()},
-                   step_hbm={"bytes_per_cell_update": STEP_BYTES, "frac": round(hbm, 4)})
+                   step_hbm={"bytes_per_cell_update": step_bytes, "frac": round(hbm, 4),
+                             "geometry_sharing_K": 1}, per_kernel=per_kernel)
 
 
 def bench_serve(args):

@@ -72,7 +72,9 @@ SIGNATURES = {
     "mhd_memory": (_I, [_P, _PD]),
     "mhd_retire": (_I, [_P, _I]),
     "mhd_case_steps": (_I, [_P, _I, C.POINTER(C.c_longlong)]),
-    "mhd_tuning": (_I, [_P, _I, _PI, _PI]),
+    "mhd_tuning": (_I, [_P, _I, _PI, _PI, _PI]),
+    "mhd_set_tuning": (_I, [_P, _I, _I, _I, _I]),
+    "mhd_autotune": (_I, [_P]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_mercury": (_I, [C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _I, C.POINTER(_P)]),

@@ -71,8 +71,10 @@ enum Range
 };
 const int kRangeLo[R_COUNT] = {0, 0, -1}, kRangeHi[R_COUNT] = {0, 1, 1};
 const int kJt[3] = {1, 2, 4};
-// launch variants: workgroup rows kJt[v % 3], tile order i-inner if v >= 3
-const int kVariants = 6;
+// launch variants: workgroup rows kJt[v % 3], tile order i-inner if
+// v % 6 >= 3, geometry-shared tile map (cases of one geometry set
+// interleaved per tile) if v >= 6
+const int kVariants = 12;
 
 enum KernelId
 {
@@ -108,8 +110,13 @@ struct mhd_ctx
     std::vector<MhdBlockDev> h_blk;
     MhdBlockDev *d_blk = nullptr;
     size_t blk_cap = 0;
-    TileArr tiles[R_COUNT][3];
+    TileArr tiles[2][R_COUNT][3]; // [geometry-shared][range][rows]
     int jt_idx[KP_COUNT];
+    // geometry-shared map: one entry per (geometry set, block) with the
+    // block-table indices of the active cases sharing it
+    std::vector<int> h_lead_k, h_lead_moff, h_members;
+    int *d_lead_k = nullptr, *d_lead_moff = nullptr, *d_members = nullptr;
+    size_t cap_lead_k = 0, cap_lead_moff = 0, cap_members = 0;
 
     // case-indexed device arrays
     int case_cap = 0;
@@ -202,9 +209,13 @@ extern "C" int mhd_destroy(mhd_ctx *c)
         free_t(r->d_st);
     for (auto &g : c->geos)
         free_t(g.d);
-    for (auto &rr : c->tiles)
-        for (auto &t : rr)
-            free_t(t.d_start);
+    for (auto &sh : c->tiles)
+        for (auto &rr : sh)
+            for (auto &t : rr)
+                free_t(t.d_start);
+    free_t(c->d_lead_k);
+    free_t(c->d_lead_moff);
+    free_t(c->d_members);
     for (auto &g : c->og)
         free_t(g.d);
     free_t(c->d_blk);
@@ -267,31 +278,36 @@ static int ensure_case_capacity(mhd_ctx *c)
     return MHD_OK;
 }
 
+// Tile starts of the plain map (one entry per block) and of the
+// geometry-shared map (one entry per (geometry set, block), K cases each).
 static int build_tiles(mhd_ctx *c)
 {
     const int nblk = (int)c->h_blk.size();
-    for (int r = 0; r < R_COUNT; ++r)
-        for (int ji = 0; ji < 3; ++ji)
-        {
-            TileArr &T = c->tiles[r][ji];
-            const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
-            std::vector<int> start(std::max(nblk, 1), 0);
-            long long total = 0;
-            for (int b = 0; b < nblk; ++b)
+    const int nlead = (int)c->h_lead_k.size();
+    for (int sh = 0; sh < 2; ++sh)
+        for (int r = 0; r < R_COUNT; ++r)
+            for (int ji = 0; ji < 3; ++ji)
             {
-                const MhdBlockDev &d = c->h_blk[b];
-                const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
-                start[b] = (int)total;
-                total += (long long)ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
+                TileArr &T = c->tiles[sh][r][ji];
+                const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
+                const int nent = sh ? nlead : nblk;
+                std::vector<int> start(std::max(nent, 1), 0);
+                long long total = 0;
+                for (int e = 0; e < nent; ++e)
+                {
+                    const MhdBlockDev &d = c->h_blk[sh ? c->h_members[c->h_lead_moff[e]] : e];
+                    const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
+                    start[e] = (int)total;
+                    total += (long long)(sh ? c->h_lead_k[e] : 1) * ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
+                }
+                if (total > (1LL << 31) - 64)
+                    return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                T.n = (int)total;
+                int rc;
+                if ((rc = grow(c, T.d_start, T.cap_start, start.size())))
+                    return rc;
+                HIPCHK(c, hipMemcpy(T.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
             }
-            if (total > (1LL << 31) - 64)
-                return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
-            T.n = (int)total;
-            int rc;
-            if ((rc = grow(c, T.d_start, T.cap_start, start.size())))
-                return rc;
-            HIPCHK(c, hipMemcpy(T.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
-        }
     return MHD_OK;
 }
 
@@ -299,17 +315,20 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
 {
     const int r = kKernelRange[kernel];
     const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
-    const int ji = v % 3;
-    const TileArr &T = c->tiles[r][ji];
+    const int ji = v % 3, sh = v >= 6 ? 1 : 0;
+    const TileArr &T = c->tiles[sh][r][ji];
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
     L.tm.block_start = T.d_start;
-    L.tm.nblk = (int)c->h_blk.size();
+    L.tm.lead_k = sh ? c->d_lead_k : nullptr;
+    L.tm.lead_moff = sh ? c->d_lead_moff : nullptr;
+    L.tm.members = sh ? c->d_members : nullptr;
+    L.tm.nblk = sh ? (int)c->h_lead_k.size() : (int)c->h_blk.size();
     L.tm.ntiles = T.n;
     L.tm.lo = kRangeLo[r];
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
-    L.tm.i_inner = v >= 3;
+    L.tm.i_inner = (v % 6) >= 3;
     L.jt = kJt[ji];
     return L;
 }
@@ -355,6 +374,45 @@ static int rebuild(mhd_ctx *c)
         return rc;
     if (!c->h_blk.empty())
         HIPCHK(c, hipMemcpy(c->d_blk, c->h_blk.data(), sizeof(MhdBlockDev) * c->h_blk.size(), hipMemcpyHostToDevice));
+    // geometry-shared entries: per geometry set (in order of first use), per
+    // block, the active cases holding it, in case order
+    c->h_lead_k.clear();
+    c->h_lead_moff.clear();
+    c->h_members.clear();
+    {
+        std::vector<std::vector<int>> by_geo(c->geos.size());
+        std::vector<int> geo_order;
+        for (auto &rp : c->cases)
+            if (rp->state == 1)
+            {
+                if (by_geo[rp->geo_set].empty())
+                    geo_order.push_back(rp->geo_set);
+                by_geo[rp->geo_set].push_back(rp->id);
+            }
+        for (int g : geo_order)
+        {
+            const std::vector<int> &cs = by_geo[g];
+            for (int b = 0; b < c->cases[cs[0]]->nblocks; ++b)
+            {
+                c->h_lead_k.push_back((int)cs.size());
+                c->h_lead_moff.push_back((int)c->h_members.size());
+                for (int ci : cs)
+                    c->h_members.push_back(c->cases[ci]->first_block + b);
+            }
+        }
+    }
+    if ((rc = grow(c, c->d_lead_k, c->cap_lead_k, std::max<size_t>(c->h_lead_k.size(), 1))) ||
+        (rc = grow(c, c->d_lead_moff, c->cap_lead_moff, std::max<size_t>(c->h_lead_moff.size(), 1))) ||
+        (rc = grow(c, c->d_members, c->cap_members, std::max<size_t>(c->h_members.size(), 1))))
+        return rc;
+    if (!c->h_lead_k.empty())
+    {
+        HIPCHK(c, hipMemcpy(c->d_lead_k, c->h_lead_k.data(), sizeof(int) * c->h_lead_k.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_lead_moff, c->h_lead_moff.data(), sizeof(int) * c->h_lead_moff.size(),
+                            hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_members, c->h_members.data(), sizeof(int) * c->h_members.size(),
+                            hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMemcpy(c->d_cst, cst.data(), sizeof(MhdCaseConst) * c->case_cap, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_active, act.data(), sizeof(int) * c->case_cap, hipMemcpyHostToDevice));
     return build_tiles(c);
@@ -1101,13 +1159,54 @@ extern "C" int mhd_profile_read(mhd_ctx *c, int kernel, double *ms, int *launche
     return MHD_OK;
 }
 
-extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows, int *i_inner)
+extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows, int *i_inner, int *geo_shared)
 {
-    if (!c || !rows || !i_inner || kernel < 0 || kernel >= KP_COUNT)
+    if (!c || !rows || !i_inner || !geo_shared || kernel < 0 || kernel >= KP_COUNT)
         return MHD_ERR_ARG;
     *rows = kJt[c->jt_idx[kernel] % 3];
-    *i_inner = c->jt_idx[kernel] >= 3;
+    *i_inner = (c->jt_idx[kernel] % 6) >= 3;
+    *geo_shared = c->jt_idx[kernel] >= 6;
     return MHD_OK;
+}
+
+extern "C" int mhd_set_tuning(mhd_ctx *c, int kernel, int rows, int i_inner, int geo_shared)
+{
+    if (!c || kernel < 0 || kernel >= KP_COUNT)
+        return MHD_ERR_ARG;
+    int ji = -1;
+    for (int q = 0; q < 3; ++q)
+        if (kJt[q] == rows)
+            ji = q;
+    if (ji < 0 || (i_inner != 0 && i_inner != 1) || (geo_shared != 0 && geo_shared != 1))
+        return fail(c, MHD_ERR_ARG, "launch shape: rows 1, 2 or 4; i_inner and geo_shared 0 or 1");
+    c->jt_idx[kernel] = ji + 3 * i_inner + 6 * geo_shared;
+    if (c->gexec)
+    {
+        (void)hipStreamSynchronize(c->q);
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return MHD_OK;
+}
+
+extern "C" int mhd_autotune(mhd_ctx *c)
+{
+    if (!c)
+        return MHD_ERR_ARG;
+    if (!c->finalized)
+        return fail(c, MHD_ERR_STATE, "context not finalized");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    int rc = autotune(c);
+    if (rc)
+        return rc;
+    if (c->gexec)
+    {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    return check_err(c);
 }
 
 extern "C" int mhd_memory(mhd_ctx *c, double *bytes)

@@ -58,8 +58,16 @@ __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, 
         else
             hi = mid - 1;
     }
-    b = tm.blks[bi];
-    const int lt = t - tm.block_start[bi];
+    int lt = t - tm.block_start[bi];
+    if (tm.members)
+    {
+        const int kk = tm.lead_k[bi];
+        const int m = lt % kk;
+        lt /= kk;
+        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
+    }
+    else
+        b = tm.blks[bi];
     // a tile is 64 * rows consecutive points of one (j, k) plane of the range,
     // k fastest, so every wavefront is full however the extents divide by 64
     const int per_tile = 64 * (int)blockDim.y;

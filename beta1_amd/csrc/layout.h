@@ -129,8 +129,13 @@ MHD_HD const double *geo_ptr(const MhdBlockDev &b) { return reinterpret_cast<con
 
 // Launch tiling: a workgroup of 64 x jt threads covers 64 * jt consecutive
 // points (k fastest) of one (block, i) plane of a launch range;
-// block_start[b] is the first tile of block b
-// (ascending, nblk entries) and a tile finds its block by binary search.
+// block_start[e] is the first tile of entry e (ascending, nblk entries) and
+// a tile finds its entry by binary search.  An entry is one block, or, in a
+// geometry-shared map (members != 0), the same block of the K cases that
+// share one geometry set: entry e then covers lead_k[e] blocks,
+// blks[members[lead_moff[e] + m]] for m < lead_k[e], and its tiles run the K
+// cases of one geometry tile back to back, so the K - 1 later reads of that
+// geometry tile find it in L2.
 // With xcd_chunk > 0 workgroup w runs tile (w % 8) * xcd_chunk + w / 8, so
 // each of the 8 XCDs (workgroups are dealt round-robin) sweeps a contiguous
 // range of tiles and neighbour planes stay in its own L2.
@@ -138,6 +143,9 @@ struct MhdTileMap
 {
     const MhdBlockDev *blks;
     const int *block_start;
+    const int *lead_k;    // geometry-shared map only (else null)
+    const int *lead_moff; //
+    const int *members;   //
     int nblk;
     int ntiles;
     int lo, hi; // range [lo, n + hi) on every axis

@@ -241,12 +241,22 @@ class MhdContext:
         return v.value
 
     def tuning(self):
+        """kernel -> (workgroup rows, tile order, "geo-shared" or "per-case")"""
         out = {}
         for k, name in enumerate(self.profile_read()):
-            rows, inner = C.c_int(), C.c_int()
-            lib().mhd_tuning(self.ctx, k, C.byref(rows), C.byref(inner))
-            out[name] = (rows.value, "i-inner" if inner.value else "j-inner")
+            rows, inner, sh = C.c_int(), C.c_int(), C.c_int()
+            _check(lib().mhd_tuning(self.ctx, k, C.byref(rows), C.byref(inner), C.byref(sh)), self.ctx, "tuning")
+            out[name] = (rows.value, "i-inner" if inner.value else "j-inner", "geo-shared" if sh.value else "per-case")
         return out
+
+    def set_tuning(self, kernel, rows, i_inner=False, geo_shared=False):
+        """Pin a kernel's launch shape (kernel by name, as in tuning())."""
+        k = list(self.profile_read()).index(kernel)
+        _check(lib().mhd_set_tuning(self.ctx, k, rows, int(i_inner), int(geo_shared)), self.ctx, "set_tuning")
+
+    def autotune(self):
+        """Re-tune launch shapes on the cases resident now."""
+        _check(lib().mhd_autotune(self.ctx), self.ctx, "autotune")
 
     def step(self, n=1, graph=False):
         f = lib().mhd_step_graph if graph else lib().mhd_step

@@ -188,9 +188,17 @@ int mhd_profile_read(mhd_ctx *ctx, int kernel, double *ms, int *launches, const 
 /* device bytes of state + geometry held by the context */
 int mhd_memory(mhd_ctx *ctx, double *bytes);
 /* launch shape the tuner chose for a kernel index of mhd_profile_read:
-   workgroup height (rows of 64 threads along j) and tile order (i_inner 1:
-   consecutive tiles step along i before the next row group of j) */
-int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner);
+   workgroup height (rows of 64 threads along j), tile order (i_inner 1:
+   consecutive tiles step along i before the next row group of j) and
+   geo_shared (1: the cases sharing one geometry set run each geometry tile
+   back to back, so it is fetched from HBM once for all of them) */
+int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner, int *geo_shared);
+/* pin a kernel's launch shape (rows 1, 2 or 4; i_inner, geo_shared 0/1) */
+int mhd_set_tuning(mhd_ctx *ctx, int kernel, int rows, int i_inner, int geo_shared);
+/* Re-run the launch-shape tuner on the cases resident now (the tuner of
+   mhd_options.autotune runs once, at mhd_finalize).  No counterpart in the
+   reference (it has no tuner); state is left as it was. */
+int mhd_autotune(mhd_ctx *ctx);
 
 /* ---- host-only case builder (no device needed) ------------------------
    The same grid / topology / geometry / weights / initial-state builders the

@@ -111,4 +111,42 @@ def test_xcd_order_and_tuning_do_not_change_results():
     ctx.finalize()
     ctx.step(4)
     same(ctx, cid, ref)
-    assert {r for r, _ in ctx.tuning().values()} <= {1, 2, 4}
+    assert {t[0] for t in ctx.tuning().values()} <= {1, 2, 4}
+
+
+SHARED = [WindDesc(nang=8, nrad=16, v_kms=350.0), WindDesc(nang=8, nrad=16, n_cc=12.0),
+          WindDesc(nang=8, nrad=16, B_nT=(1.0, -4.0, 3.0)), OTDesc(n=12), WindDesc(nang=12, nrad=16)]
+KERNELS = ("k_face_emf", "k_edge_emf", "k_fluid", "k_curl_update", "k_derived", "k_jcell")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_geometry_shared_tiles_leave_each_case_as_alone(precision):
+    """Cases of one geometry set run each geometry tile back to back (three
+    shells share one set here); every case still evolves as it does alone,
+    also after a member of the set retires."""
+    ctx = MhdContext(precision=precision)
+    ids = [ctx.add(d) for d in SHARED]
+    ctx.finalize()
+    for n, k in enumerate(KERNELS):
+        ctx.set_tuning(k, rows=(1, 2, 4)[n % 3], i_inner=n % 2 == 1, geo_shared=True)
+    assert all(ctx.tuning()[k][2] == "geo-shared" for k in KERNELS)
+    ctx.step(3, graph=True)
+    ctx.step(2)
+    for d, cid in zip(SHARED, ids):
+        same(ctx, cid, alone(d, 5, precision=precision))
+    ctx.retire(ids[1])
+    ctx.step(3, graph=True)
+    for d, cid in zip(SHARED, ids):
+        if cid != ids[1]:
+            same(ctx, cid, alone(d, 8, precision=precision))
+
+
+def test_autotune_after_admission_keeps_results():
+    ctx = MhdContext(autotune=True)
+    ids = [ctx.add(d) for d in SHARED[:3]]
+    ctx.finalize()
+    ctx.step(2)
+    ctx.autotune()
+    ctx.step(2, graph=True)
+    for d, cid in zip(SHARED[:3], ids):
+        same(ctx, cid, alone(d, 4))
