@@ -41,8 +41,8 @@ struct CaseRec
 struct GeoSet
 {
     std::string key;
-    char *d = nullptr; // G_NSLOT slots per block, elements of geo_esize bytes
-    int64_t block_stride = 0;
+    char *d = nullptr; // per block: metric slots in double, weight slots of weight_esize bytes
+    int64_t block_bytes = 0;
     int refs = 0;
     size_t bytes = 0;
 };
@@ -146,10 +146,10 @@ struct mhd_ctx
         }                                                                                                              \
     } while (0)
 
-// Mixed-precision contexts store the geometry (metric and weight tables) in
-// float; the FP64 step keeps it in double.
+// The metric (Jac, JD) is double in both precisions; mixed-precision contexts
+// store the Hodge weights and the B_cell / J_cell weight tables in float.
 static int mixed_of(const mhd_ctx *c) { return c->opt.precision == MHD_PREC_MIXED; }
-static size_t geo_esize(const mhd_ctx *c) { return mixed_of(c) ? sizeof(float) : sizeof(double); }
+static int weight_esize(const mhd_ctx *c) { return mixed_of(c) ? (int)sizeof(float) : (int)sizeof(double); }
 
 static int fail(mhd_ctx *c, int code, const std::string &msg)
 {
@@ -366,7 +366,7 @@ static int rebuild(mhd_ctx *c)
             MhdBlockDev d = r.cs.blk[b].d;
             d.case_id = ci;
             d.st = (int64_t)(uintptr_t)(r.d_st + (int64_t)b * ST_NSLOT * r.slot);
-            d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_stride * (int64_t)geo_esize(c));
+            d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_bytes);
             c->h_blk.push_back(d);
         }
     }
@@ -469,7 +469,7 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         if (c->geos[g].refs > 0 && c->geos[g].key == r.cs.shape_key)
             gi = g;
     const size_t st_bytes = sizeof(double) * (size_t)ST_NSLOT * slot * r.nblocks;
-    const size_t geo_bytes = gi < 0 ? geo_esize(c) * (size_t)G_NSLOT * slot * r.nblocks : 0;
+    const size_t geo_bytes = gi < 0 ? (size_t)geo_block_bytes(slot, weight_esize(c)) * r.nblocks : 0;
     const double budget = c->opt.hbm_budget_gb * 1e9;
     if (budget > 0 && (double)(c->bytes_used + st_bytes + geo_bytes) > budget)
         return fail(c, MHD_ERR_MEM, "HBM budget exceeded");
@@ -477,20 +477,24 @@ static int place_case(mhd_ctx *c, CaseRec &r)
     {
         GeoSet G;
         G.key = r.cs.shape_key;
-        G.block_stride = (int64_t)G_NSLOT * slot;
+        G.block_bytes = geo_block_bytes(slot, weight_esize(c));
         G.bytes = geo_bytes;
         HIPCHK(c, hipMalloc((void **)&G.d, geo_bytes));
         std::vector<float> f32;
+        const size_t metric = (size_t)G_NMETRIC * slot;
         for (int b = 0; b < r.nblocks; ++b)
         {
-            const size_t nbytes = geo_esize(c) * (size_t)G.block_stride;
-            const void *src = r.cs.blk[b].geo.data();
+            const std::vector<double> &geo = r.cs.blk[b].geo;
+            char *dst = G.d + (int64_t)b * G.block_bytes;
             if (mixed_of(c))
             {
-                f32.assign(r.cs.blk[b].geo.begin(), r.cs.blk[b].geo.end());
-                src = f32.data();
+                HIPCHK(c, hipMemcpy(dst, geo.data(), sizeof(double) * metric, hipMemcpyHostToDevice));
+                f32.assign(geo.begin() + metric, geo.end());
+                HIPCHK(c, hipMemcpy(dst + sizeof(double) * metric, f32.data(), sizeof(float) * f32.size(),
+                                    hipMemcpyHostToDevice));
             }
-            HIPCHK(c, hipMemcpy(G.d + (int64_t)b * nbytes, src, nbytes, hipMemcpyHostToDevice));
+            else
+                HIPCHK(c, hipMemcpy(dst, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice));
         }
         c->bytes_used += geo_bytes;
         gi = 0;
@@ -796,7 +800,7 @@ static size_t case_bytes(const mhd_ctx *c, const CaseSetup &cs)
     for (const GeoSet &G : c->geos)
         shared = shared || (G.refs > 0 && G.key == cs.shape_key);
     if (!shared)
-        need += geo_esize(c) * G_NSLOT * slot * nb;
+        need += (size_t)geo_block_bytes((int64_t)slot, weight_esize(c)) * nb;
     return need;
 }
 
@@ -1015,10 +1019,10 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
         return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
     HIPCHK(c, hipStreamSynchronize(c->q));
     std::vector<double> soa((size_t)nc * d.slot);
-    if (region == 1 && mixed_of(c))
+    if (region == 1 && mixed_of(c) && slot >= G_NMETRIC)
     {
-        // geometry of a mixed-precision context is stored as float
-        float *dev = reinterpret_cast<float *>((uintptr_t)d.geo) + (int64_t)slot * d.slot;
+        // weight tables of a mixed-precision context are stored as float
+        float *dev = reinterpret_cast<float *>((uintptr_t)(d.geo + geo_slot_byte(slot, d.slot, sizeof(float))));
         std::vector<float> f32(soa.size());
         HIPCHK(c, hipMemcpy(f32.data(), dev, sizeof(float) * f32.size(), hipMemcpyDeviceToHost));
         soa.assign(f32.begin(), f32.end());
@@ -1030,7 +1034,8 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
         }
         return MHD_OK;
     }
-    double *dev = (region == 0 ? st_ptr(d) : const_cast<double *>(geo_ptr(d))) + (int64_t)slot * d.slot;
+    double *dev = region == 0 ? st_ptr(d) + (int64_t)slot * d.slot
+                              : reinterpret_cast<double *>((uintptr_t)(d.geo + geo_slot_byte(slot, d.slot, 8)));
     HIPCHK(c, hipMemcpy(soa.data(), dev, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
     soa_aos(d, loc, nc, soa.data(), aos, down);
     if (!down)

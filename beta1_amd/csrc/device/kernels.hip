@@ -28,15 +28,24 @@ namespace
 typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ double *gptr(int64_t a) { return (double *)(gdouble *)(uintptr_t)a; }
 __device__ __forceinline__ double *dst_ptr(const MhdBlockDev &b) { return gptr(b.st); }
-// Geometry is stored as GT: double for the FP64 step, float for the mixed one.
-template <typename GT> __device__ __forceinline__ const GT *dgeo_ptr(const MhdBlockDev &b)
+// The metric (Jac, JD) is stored in double; the Hodge weights and weight
+// tables after it as WT: double for the FP64 step, float for the mixed one.
+// dweights<WT>(b) + (s - G_NMETRIC) * slot is weight slot s.
+__device__ __forceinline__ const double *dmetric(const MhdBlockDev &b) { return gptr(b.geo); }
+template <typename WT> __device__ __forceinline__ const WT *dweights(const MhdBlockDev &b)
 {
-    return (const GT *)(__attribute__((address_space(1))) GT *)(uintptr_t)b.geo;
+    return (const WT *)(__attribute__((address_space(1))) WT *)(uintptr_t)(b.geo + geo_slot_byte(G_NMETRIC, b.slot, 8));
 }
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
+// Division and square root of the flux arithmetic: correctly rounded in
+// FP64 (the reference's operations); in the FP32 flux of the mixed step the
+// hardware reciprocal and square root (1 ulp), well inside FP32 rounding.
+template <typename R> __device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R> __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
+template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 // Map a workgroup/thread to (block, i, j, k) of its tile.  `wg` is false for
 // padding workgroups of the XCD-ordered grid (no block at all); the return
@@ -91,15 +100,29 @@ struct Prim
     double rho_raw, u, v, w, p;
 };
 
-// calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110)
+// calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110).  EXACT: the
+// velocities are quotients as in the reference (FP64 step); otherwise
+// products with one reciprocal of rho (within 2 ulp of FP64, for the mixed
+// step's FP32 flux).
+template <bool EXACT = true>
 __device__ __forceinline__ Prim prim_from(double r0, double m1, double m2, double m3, double E, double gm)
 {
     Prim q;
     q.rho_raw = r0;
     const double rho = dmax(r0, 1e-12);
-    q.u = m1 / rho;
-    q.v = m2 / rho;
-    q.w = m3 / rho;
+    if (EXACT)
+    {
+        q.u = m1 / rho;
+        q.v = m2 / rho;
+        q.w = m3 / rho;
+    }
+    else
+    {
+        const double ir = 1.0 / rho;
+        q.u = m1 * ir;
+        q.v = m2 * ir;
+        q.w = m3 * ir;
+    }
     const double ke = 0.5 * rho * (q.u * q.u + q.v * q.v + q.w * q.w);
     double eint = E - ke;
     if (eint < 0.0)
@@ -109,9 +132,10 @@ __device__ __forceinline__ Prim prim_from(double r0, double m1, double m2, doubl
     return q;
 }
 
+template <bool EXACT = true>
 __device__ __forceinline__ Prim prim_of(const double *__restrict__ U, int64_t sl, int64_t o, double gm)
 {
-    return prim_from(U[o], U[sl + o], U[2 * sl + o], U[3 * sl + o], U[4 * sl + o], gm);
+    return prim_from<EXACT>(U[o], U[sl + o], U[2 * sl + o], U[3 * sl + o], U[4 * sl + o], gm);
 }
 
 // -------------------------------------------------------- Rusanov flux
@@ -122,16 +146,16 @@ __device__ __forceinline__ void rusanov(R gm, const R *WL, const R *WR, R k1, R 
     const R rhoL = WL[0], uL = WL[1], vL = WL[2], wL = WL[3], pL = WL[4];
     const R rhoR = WR[0], uR = WR[1], vR = WR[2], wR = WR[3], pR = WR[4];
     const R one = (R)1.0, half = (R)0.5, zero = (R)0.0;
-    const R eL = pL / (gm - one) + half * rhoL * (uL * uL + vL * vL + wL * wL);
-    const R eR = pR / (gm - one) + half * rhoR * (uR * uR + vR * vR + wR * wR);
+    const R eL = fdiv(pL, gm - one) + half * rhoL * (uL * uL + vL * vL + wL * wL);
+    const R eR = fdiv(pR, gm - one) + half * rhoR * (uR * uR + vR * vR + wR * wR);
     const R kk = k1 * k1 + k2 * k2 + k3 * k3;
     const R uvL = k1 * uL + k2 * vL + k3 * wL;
     const R uvR = k1 * uR + k2 * vR + k3 * wR;
-    const R tL = gm * pL / (rhoL < (R)1e-30 ? (R)1e-30 : rhoL);
-    const R tR = gm * pR / (rhoR < (R)1e-30 ? (R)1e-30 : rhoR);
-    const R cL = sqrt(zero < tL ? tL : zero);
-    const R cR = sqrt(zero < tR ? tR : zero);
-    const R sk = sqrt(kk);
+    const R tL = fdiv(gm * pL, rhoL < (R)1e-30 ? (R)1e-30 : rhoL);
+    const R tR = fdiv(gm * pR, rhoR < (R)1e-30 ? (R)1e-30 : rhoR);
+    const R cL = fsqrt(zero < tL ? tL : zero);
+    const R cR = fsqrt(zero < tR ? tR : zero);
+    const R sk = fsqrt(kk);
     const R radL = fabs(uvL) + cL * sk;
     const R radR = fabs(uvR) + cR * sk;
     const R rad = (radL < radR) ? radR : radL;
@@ -143,11 +167,45 @@ __device__ __forceinline__ void rusanov(R gm, const R *WL, const R *WR, R k1, R 
     out[4] = half * (uvL * (eL + pL) + uvR * (eR + pR)) - hr * (eR - eL);
 }
 
-__device__ __forceinline__ double minmod(double a, double b)
+template <typename T> __device__ __forceinline__ T minmod(T a, T b)
 {
-    if (a * b <= 0.0)
-        return 0.0;
+    if (a * b <= (T)0)
+        return (T)0;
     return fabs(a) < fabs(b) ? a : b;
+}
+
+// Rusanov flux of the mixed-precision step: the central part 1/2 (F(W_L) +
+// F(W_R)) in FP64, the wave speeds and the dissipation 1/2 rad (U_R - U_L) in
+// FP32 on the FP64 differences of the conserved states.  A uniform state
+// keeps its flux to FP64 rounding and the FP32 error is relative to the
+// dissipation, which is small where the flow is smooth.
+__device__ __forceinline__ void rusanov_mixed(double gm, const double *WL, const double *WR, double k1, double k2,
+                                              double k3, double *out)
+{
+    const double rhoL = WL[0], uL = WL[1], vL = WL[2], wL = WL[3], pL = WL[4];
+    const double rhoR = WR[0], uR = WR[1], vR = WR[2], wR = WR[3], pR = WR[4];
+    const double igm = 1.0 / (gm - 1.0);
+    const double eL = pL * igm + 0.5 * rhoL * (uL * uL + vL * vL + wL * wL);
+    const double eR = pR * igm + 0.5 * rhoR * (uR * uR + vR * vR + wR * wR);
+    const double uvL = k1 * uL + k2 * vL + k3 * wL;
+    const double uvR = k1 * uR + k2 * vR + k3 * wR;
+    const float fk1 = (float)k1, fk2 = (float)k2, fk3 = (float)k3;
+    const float sk = fsqrt(fk1 * fk1 + fk2 * fk2 + fk3 * fk3);
+    const float tL = fdiv((float)(gm * pL), (float)(rhoL < 1e-30 ? 1e-30 : rhoL));
+    const float tR = fdiv((float)(gm * pR), (float)(rhoR < 1e-30 ? 1e-30 : rhoR));
+    const float cL = fsqrt(0.0f < tL ? tL : 0.0f);
+    const float cR = fsqrt(0.0f < tR ? tR : 0.0f);
+    const float radL = fabsf((float)uvL) + cL * sk;
+    const float radR = fabsf((float)uvR) + cR * sk;
+    const float hr = 0.5f * ((radL < radR) ? radR : radL);
+    out[0] = 0.5 * (rhoL * uvL + rhoR * uvR) - (double)(hr * (float)(rhoR - rhoL));
+    out[1] = 0.5 * ((rhoL * uvL * uL + k1 * pL) + (rhoR * uvR * uR + k1 * pR)) -
+             (double)(hr * (float)(rhoR * uR - rhoL * uL));
+    out[2] = 0.5 * ((rhoL * uvL * vL + k2 * pL) + (rhoR * uvR * vR + k2 * pR)) -
+             (double)(hr * (float)(rhoR * vR - rhoL * vL));
+    out[3] = 0.5 * ((rhoL * uvL * wL + k3 * pL) + (rhoR * uvR * wR + k3 * pR)) -
+             (double)(hr * (float)(rhoR * wR - rhoL * wL));
+    out[4] = 0.5 * (uvL * (eL + pL) + uvR * (eR + pR)) - (double)(hr * (float)(eR - eL));
 }
 
 } // namespace
@@ -169,9 +227,8 @@ struct CellGeo
     double a[6][3];
 };
 
-template <typename GT>
 __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__restrict__ U,
-                                          const GT *__restrict__ G, int64_t o, CellGeo &q)
+                                          const double *__restrict__ G, int64_t o, CellGeo &q)
 {
     const int64_t sl = b.slot;
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
@@ -184,7 +241,7 @@ __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__
     {
         const int ax = f >> 1;
         const int64_t oo = o + ((f & 1) ? step[ax] : 0);
-        const GT *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
         q.a[f][0] = A[oo];
         q.a[f][1] = A[sl + oo];
         q.a[f][2] = A[2 * sl + oo];
@@ -263,7 +320,6 @@ __device__ __forceinline__ void reduce_dt(double local, unsigned long long *__re
     }
 }
 
-template <typename GT>
 __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                             unsigned long long *__restrict__ dtbits, int ucur)
 {
@@ -278,7 +334,7 @@ __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseCo
         const double *U = dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl;
         const double *Bc = dst_ptr(b) + (int64_t)ST_BCELL * sl;
         CellGeo q;
-        load_cell(b, U, dgeo_ptr<GT>(b), o, q);
+        load_cell(b, U, dmetric(b), o, q);
         local = cell_dt(cst[b.case_id], q, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
     }
     if (wg)
@@ -322,7 +378,6 @@ __device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double
     }
 }
 
-template <typename GT>
 __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  int ucur)
 {
@@ -333,7 +388,7 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdC
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     CellGeo q;
-    load_cell(b, dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, dgeo_ptr<GT>(b), o, q);
+    load_cell(b, dst_ptr(b) + (int64_t)(ST_U0 + 5 * ucur) * sl, dmetric(b), o, q);
     double uc[3];
     cell_ucontra(cst[b.case_id].gamma, q, uc);
     double *UC = dst_ptr(b) + (int64_t)ST_UC * sl;
@@ -344,8 +399,8 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdC
 
 // ======================================================================
 // Face EMF candidates: AssembleOneDirectionEMF_
-// (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:348-496).
-template <typename R>
+// (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:348-496).  FP64 in
+// both precisions: the EMFs feed the constrained-transport curl.
 __device__ __forceinline__ double emf_flux(const double *__restrict__ Bf, const double *__restrict__ Ba,
                                            const double *__restrict__ UC, int64_t sl, const int64_t *step, int bax,
                                            int fax, int64_t oL, int64_t oR, double bff)
@@ -359,23 +414,12 @@ __device__ __forceinline__ double emf_flux(const double *__restrict__ Bf, const 
     const double biR = 0.5 * (bR0 + bR1);
     const double ufL = UC[(int64_t)fax * sl + oL], ufR = UC[(int64_t)fax * sl + oR];
     const double ubL = UC[(int64_t)bax * sl + oL], ubR = UC[(int64_t)bax * sl + oR];
-    if constexpr (sizeof(R) == 8)
-    {
-        const double GL = ufL * btL - ubL * bff;
-        const double GR = ufR * btR - ubR * bff;
-        const double radius = dmax(fabs(ufL), fabs(ufR));
-        return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
-    }
-    else
-    {
-        const float GL = (float)ufL * (float)btL - (float)ubL * (float)bff;
-        const float GR = (float)ufR * (float)btR - (float)ubR * (float)bff;
-        const float radius = fmaxr(fabsf((float)ufL), fabsf((float)ufR));
-        return (double)(0.5f * (GL + GR)) - 0.5 * (double)radius * (biR - biL);
-    }
+    const double GL = ufL * btL - ubL * bff;
+    const double GR = ufR * btR - ubR * bff;
+    const double radius = dmax(fabs(ufL), fabs(ufR));
+    return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
 }
 
-template <typename R>
 __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
 {
     MhdBlockDev b;
@@ -404,22 +448,22 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
         double e0 = 0.0, e1 = 0.0, e2 = 0.0;
         if (fax == 0)
         {
-            const double Ge = emf_flux<R>(Bf, Ba, UC, sl, step, 1, 0, oL, oR, bff);
-            const double Gz = emf_flux<R>(Bf, Ba, UC, sl, step, 2, 0, oL, oR, bff);
+            const double Ge = emf_flux(Bf, Ba, UC, sl, step, 1, 0, oL, oR, bff);
+            const double Gz = emf_flux(Bf, Ba, UC, sl, step, 2, 0, oL, oR, bff);
             e1 = Gz;
             e2 = -Ge;
         }
         else if (fax == 1)
         {
-            const double Gx = emf_flux<R>(Bf, Ba, UC, sl, step, 0, 1, oL, oR, bff);
-            const double Gz = emf_flux<R>(Bf, Ba, UC, sl, step, 2, 1, oL, oR, bff);
+            const double Gx = emf_flux(Bf, Ba, UC, sl, step, 0, 1, oL, oR, bff);
+            const double Gz = emf_flux(Bf, Ba, UC, sl, step, 2, 1, oL, oR, bff);
             e0 = -Gz;
             e2 = Gx;
         }
         else
         {
-            const double Gx = emf_flux<R>(Bf, Ba, UC, sl, step, 0, 2, oL, oR, bff);
-            const double Ge = emf_flux<R>(Bf, Ba, UC, sl, step, 1, 2, oL, oR, bff);
+            const double Gx = emf_flux(Bf, Ba, UC, sl, step, 0, 2, oL, oR, bff);
+            const double Ge = emf_flux(Bf, Ba, UC, sl, step, 1, 2, oL, oR, bff);
             e0 = Ge;
             e1 = -Gx;
         }
@@ -526,26 +570,35 @@ __global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const 
 //  Z4_Lunar/src/physics/LunarSolver_Advance.cpp:61-122).
 // U is read from buffer `ucur` and written to buffer 1-ucur.
 template <typename R, int RECON>
-__device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t sl, const R *__restrict__ A,
+__device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t sl, const double *__restrict__ A,
                                           int64_t oL, int64_t oR, int64_t step, double gm, double *F)
 {
     double WL[5], WR[5];
     {
-        const Prim a = prim_of(U, sl, oL, gm), c = prim_of(U, sl, oR, gm);
+        const Prim a = prim_of<sizeof(R) == 8>(U, sl, oL, gm), c = prim_of<sizeof(R) == 8>(U, sl, oR, gm);
         WL[0] = a.rho_raw; WL[1] = a.u; WL[2] = a.v; WL[3] = a.w; WL[4] = a.p;
         WR[0] = c.rho_raw; WR[1] = c.u; WR[2] = c.v; WR[3] = c.w; WR[4] = c.p;
     }
     if (RECON == 1)
     {
-        const Prim a = prim_of(U, sl, oL - step, gm), c = prim_of(U, sl, oR + step, gm);
+        const Prim a = prim_of<sizeof(R) == 8>(U, sl, oL - step, gm),
+                   c = prim_of<sizeof(R) == 8>(U, sl, oR + step, gm);
         const double WLL[5] = {a.rho_raw, a.u, a.v, a.w, a.p};
         const double WRR[5] = {c.rho_raw, c.u, c.v, c.w, c.p};
         double L2[5], R2[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m)
         {
-            L2[m] = WL[m] + 0.5 * minmod(WL[m] - WLL[m], WR[m] - WL[m]);
-            R2[m] = WR[m] - 0.5 * minmod(WR[m] - WL[m], WRR[m] - WR[m]);
+            if constexpr (sizeof(R) == 8)
+            {
+                L2[m] = WL[m] + 0.5 * minmod(WL[m] - WLL[m], WR[m] - WL[m]);
+                R2[m] = WR[m] - 0.5 * minmod(WR[m] - WL[m], WRR[m] - WR[m]);
+            }
+            else // FP32 slopes, added to the FP64 states
+            {
+                L2[m] = WL[m] + (double)(0.5f * minmod((float)(WL[m] - WLL[m]), (float)(WR[m] - WL[m])));
+                R2[m] = WR[m] - (double)(0.5f * minmod((float)(WR[m] - WL[m]), (float)(WRR[m] - WR[m])));
+            }
         }
         if (L2[0] > 0.0 && L2[4] > 0.0 && R2[0] > 0.0 && R2[4] > 0.0)
         {
@@ -564,17 +617,7 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ U, int64_t 
     }
     else
     {
-        float wl[5], wr[5], f[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m)
-        {
-            wl[m] = (float)WL[m];
-            wr[m] = (float)WR[m];
-        }
-        rusanov<float>((float)gm, wl, wr, (float)A[of], (float)A[sl + of], (float)A[2 * sl + of], f);
-#pragma unroll
-        for (int m = 0; m < 5; ++m)
-            F[m] = (double)f[m];
+        rusanov_mixed(gm, WL, WR, A[of], A[sl + of], A[2 * sl + of], F);
     }
 }
 
@@ -594,13 +637,13 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
     double *S = dst_ptr(b);
     const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
     double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
-    const R *G = dgeo_ptr<R>(b);
+    const double *G = dmetric(b);
     double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const double invJ = 1.0 / (double)G[(int64_t)G_JAC * sl + o];
+    const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
     {
-        const R *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+        const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
         double Flo[5], Fhi[5];
         face_flux<R, RECON>(U, sl, A, o - step[ax], o, step[ax], gm, Flo);
         face_flux<R, RECON>(U, sl, A, o, o + step[ax], step[ax], gm, Fhi);
@@ -609,7 +652,7 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
             rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
     }
     // J x B (Hall_Num_Limiter weighting)
-    const Prim q = prim_of(U, sl, o, gm);
+    const Prim q = prim_of<sizeof(R) == 8>(U, sl, o, gm);
     const double n_true = dmax(q.rho_raw, 0.0) / c.M_H * c.M_ref;
     const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
     const double wgt = n_true / ne_eff;
@@ -636,7 +679,7 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
 // 163-236) fused with the next step's CFL limit (dt depends on U after the
 // Ucell sync and on this B_cell; nothing changes them before the next step).
 // Each result is stored as soon as it is known, which keeps few values live.
-template <typename GT>
+template <typename WT>
 __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                  unsigned long long *__restrict__ dtbits, int ucur)
 {
@@ -650,9 +693,8 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
         const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
         double *S = dst_ptr(b);
-        const GT *G = dgeo_ptr<GT>(b);
         CellGeo q;
-        load_cell(b, S + (int64_t)(ST_U0 + 5 * ucur) * sl, G, o, q);
+        load_cell(b, S + (int64_t)(ST_U0 + 5 * ucur) * sl, dmetric(b), o, q);
         {
             double uc[3];
             cell_ucontra(cst[b.case_id].gamma, q, uc);
@@ -667,7 +709,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
             const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
                                   By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
-            const GT *W = G + (int64_t)G_WB * sl;
+            const WT *W = dweights<WT>(b) + (int64_t)(G_WB - G_NMETRIC) * sl;
             double bc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int n = 0; n < 6; ++n)
@@ -691,7 +733,7 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
 // (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891; CTOperators::
 // CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).  The 12
 // J edges of a cell are recomputed from beta-weighted face fluxes.
-template <typename GT>
+template <typename WT>
 __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
     MhdBlockDev b;
@@ -702,10 +744,10 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = dst_ptr(b);
-    const GT *G = dgeo_ptr<GT>(b);
+    const WT *G = dweights<WT>(b);
     const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const GT *bx = G + (int64_t)G_BETA * sl, *by = bx + sl, *bz = bx + 2 * sl;
-    const GT *ax = G + (int64_t)G_ALPHA * sl, *ay = ax + sl, *az = ax + 2 * sl;
+    const WT *bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *by = bx + sl, *bz = bx + 2 * sl;
+    const WT *ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl, *ay = ax + sl, *az = ax + 2 * sl;
     auto jxi = [&](int64_t e) {
         const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
         return (1.0 * val) * ax[e];
@@ -720,7 +762,7 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     };
     const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
                           jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
-    const GT *W = G + (int64_t)G_WJ * sl;
+    const WT *W = G + (int64_t)(G_WJ - G_NMETRIC) * sl;
     double jc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 0; n < 12; ++n)
@@ -810,7 +852,6 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
-template <typename GT>
 __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long long *__restrict__ out)
 {
     MhdBlockDev b;
@@ -823,7 +864,7 @@ __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long
         const int64_t sl = b.slot, o = blk_off(b, i, j, k);
         const double *Bx = dst_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
         const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
-                          (double)dgeo_ptr<GT>(b)[(int64_t)G_JAC * sl + o];
+                          dmetric(b)[(int64_t)G_JAC * sl + o];
         v = fabs(dv);
         if (!(v == v))
             v = 1e300;
@@ -865,10 +906,8 @@ static dim3 block_of(const Launch &L) { return dim3(64, (unsigned)L.jt, 1); }
 void launch_dt(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
                hipStream_t q)
 {
-    if (mixed)
-        hipLaunchKernelGGL(k_dt<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
-    else
-        hipLaunchKernelGGL(k_dt<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    (void)mixed; // the metric is double in both precisions
+    hipLaunchKernelGGL(k_dt, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
 }
 void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const MhdCaseConst *cst, int ncase,
                         const int *active, hipStream_t q)
@@ -877,17 +916,13 @@ void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const 
 }
 void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q)
 {
-    if (mixed)
-        hipLaunchKernelGGL(k_ucontra<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
-    else
-        hipLaunchKernelGGL(k_ucontra<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+    (void)mixed;
+    hipLaunchKernelGGL(k_ucontra, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
 }
 void launch_face_emf(const Launch &L, int mixed, hipStream_t q)
 {
-    if (mixed)
-        hipLaunchKernelGGL(k_face_emf<float>, grid_of(L), block_of(L), 0, q, L.tm);
-    else
-        hipLaunchKernelGGL(k_face_emf<double>, grid_of(L), block_of(L), 0, q, L.tm);
+    (void)mixed;
+    hipLaunchKernelGGL(k_face_emf, grid_of(L), block_of(L), 0, q, L.tm);
 }
 void launch_edge_emf(const Launch &L, hipStream_t q)
 {
@@ -934,10 +969,8 @@ void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ush
 }
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q)
 {
-    if (mixed)
-        hipLaunchKernelGGL(k_divb<float>, grid_of(L), block_of(L), 0, q, L.tm, out);
-    else
-        hipLaunchKernelGGL(k_divb<double>, grid_of(L), block_of(L), 0, q, L.tm, out);
+    (void)mixed;
+    hipLaunchKernelGGL(k_divb, grid_of(L), block_of(L), 0, q, L.tm, out);
 }
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
 {

@@ -81,8 +81,19 @@ enum MhdGeoSlot
     G_ALPHA = 13, // 3 slots: edge Hodge weight alpha = |e|/|S*|
     G_WB = 16,    // 18 comps: Bcell_from_Bface_w
     G_WJ = 34,    // 36 comps: Jcell_from_Jedge_w
-    G_NSLOT = 70
+    G_NSLOT = 70,
+    G_NMETRIC = 10 // slots [0, G_NMETRIC): the metric (Jac, JD), always double
 };
+
+// Geometry storage of a block: the metric slots in double, then the Hodge
+// weights and the weight tables (slots G_NMETRIC..G_NSLOT) in elements of
+// `wsize` bytes (8, or 4 where the tables are stored as float).
+MHD_HD int64_t geo_slot_byte(int s, int64_t slot, int wsize)
+{
+    return s < G_NMETRIC ? (int64_t)s * slot * 8
+                         : (int64_t)G_NMETRIC * slot * 8 + (int64_t)(s - G_NMETRIC) * slot * wsize;
+}
+MHD_HD int64_t geo_block_bytes(int64_t slot, int wsize) { return geo_slot_byte(G_NSLOT, slot, wsize); }
 
 // Physical boundary kinds on a block face.
 enum MhdBc
@@ -125,7 +136,6 @@ struct MhdCaseConst
 };
 
 MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }
-MHD_HD const double *geo_ptr(const MhdBlockDev &b) { return reinterpret_cast<const double *>((uintptr_t)b.geo); }
 
 // Launch tiling: a workgroup of 64 x jt threads covers 64 * jt consecutive
 // points (k fastest) of one (block, i) plane of a launch range;
