@@ -26,7 +26,8 @@ class MhdWindDesc(C.Structure):
 
 
 class MhdDipoleDesc(C.Structure):
-    _fields_ = [("B_eq_nT", C.c_double), ("offset_z", C.c_double), ("softening", C.c_double)]
+    _fields_ = [("B_eq_nT", C.c_double), ("offset_z", C.c_double), ("softening", C.c_double),
+                ("single_fluid", C.c_int), ("no_ambipolar", C.c_int)]
 
 
 _P = C.c_void_p
@@ -86,6 +87,7 @@ SIGNATURES = {
     "mhd_host_faces": (_I, [_P, _I, _PI]),
     "mhd_host_nodes": (_I, [_P, _I, _I, _PD, _S]),
     "mhd_host_constants": (_I, [_P, _PD]),
+    "mhd_host_constants_two_fluid": (_I, [_P, _PD]),
     "mhd_host_apply_sync": (_I, [_P, _I]),
     "mhd_host_op_count": (_I, [_P, _I, C.POINTER(_S)]),
 }

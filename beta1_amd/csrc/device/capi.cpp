@@ -54,6 +54,7 @@ struct TileArr
     int n = 0;
 };
 
+
 // Op list of one sync group on the device: the cases' ops back to back,
 // each case's segment appended when it is placed and cut out when it retires.
 struct OpGroup
@@ -76,6 +77,27 @@ const int kJt[3] = {1, 2, 4};
 // interleaved per tile) if v >= 6
 const int kVariants = 12;
 
+// Launch tiling over a set of blocks.  An entry is one block or, in a
+// geometry-shared set, the same block of every case sharing a geometry set
+// (lead_k cases, block-table indices members[lead_moff ..]).  Empty members:
+// entry e is block e of the whole table.
+struct TileSet
+{
+    std::vector<int> lead_k, lead_moff, members;
+    int *d_lead_k = nullptr, *d_lead_moff = nullptr, *d_members = nullptr;
+    size_t cap_k = 0, cap_moff = 0, cap_mem = 0;
+    int nent = 0;
+    TileArr tiles[R_COUNT][3]; // [range][rows]
+};
+// block sets: every block, the single-fluid blocks, the two-fluid blocks
+enum BlockSet
+{
+    BS_ALL = 0,
+    BS_ONE,
+    BS_TWO,
+    BS_COUNT
+};
+
 enum KernelId
 {
     KP_DT = 0,
@@ -88,11 +110,16 @@ enum KernelId
     KP_DERIVED,
     KP_OPS,
     KP_JCELL,
+    KP_FLUID2,
+    KP_DERIVED2,
+    KP_AMBI,
     KP_COUNT
 };
-const char *kKernelName[KP_COUNT] = {"k_dt",    "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
-                                     "k_fluid", "k_curl_update", "k_derived", "k_ops", "k_jcell"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL, R_CELL};
+const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
+                                     "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
+                                     "k_fluid2",  "k_derived2",    "k_ambipolar_emf"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE,
+                                    R_EXT,  R_CELL, R_CELL, R_CELL, R_EXT, R_FACE};
 } // namespace
 
 struct mhd_ctx
@@ -110,13 +137,9 @@ struct mhd_ctx
     std::vector<MhdBlockDev> h_blk;
     MhdBlockDev *d_blk = nullptr;
     size_t blk_cap = 0;
-    TileArr tiles[2][R_COUNT][3]; // [geometry-shared][range][rows]
+    TileSet ts[BS_COUNT][2]; // [block set][geometry-shared]
     int jt_idx[KP_COUNT];
-    // geometry-shared map: one entry per (geometry set, block) with the
-    // block-table indices of the active cases sharing it
-    std::vector<int> h_lead_k, h_lead_moff, h_members;
-    int *d_lead_k = nullptr, *d_lead_moff = nullptr, *d_members = nullptr;
-    size_t cap_lead_k = 0, cap_lead_moff = 0, cap_members = 0;
+    int n_two = 0; // two-fluid blocks in the table
 
     // case-indexed device arrays
     int case_cap = 0;
@@ -209,13 +232,16 @@ extern "C" int mhd_destroy(mhd_ctx *c)
         free_t(r->d_st);
     for (auto &g : c->geos)
         free_t(g.d);
-    for (auto &sh : c->tiles)
-        for (auto &rr : sh)
-            for (auto &t : rr)
-                free_t(t.d_start);
-    free_t(c->d_lead_k);
-    free_t(c->d_lead_moff);
-    free_t(c->d_members);
+    for (auto &set : c->ts)
+        for (TileSet &T : set)
+        {
+            for (auto &rr : T.tiles)
+                for (auto &t : rr)
+                    free_t(t.d_start);
+            free_t(T.d_lead_k);
+            free_t(T.d_lead_moff);
+            free_t(T.d_members);
+        }
     for (auto &g : c->og)
         free_t(g.d);
     free_t(c->d_blk);
@@ -278,37 +304,112 @@ static int ensure_case_capacity(mhd_ctx *c)
     return MHD_OK;
 }
 
-// Tile starts of the plain map (one entry per block) and of the
-// geometry-shared map (one entry per (geometry set, block), K cases each).
+// Entries of every block set (plain: one block each; geometry-shared: per
+// geometry set in order of first use, per block, the set's cases in case
+// order) and their tile starts per launch range and workgroup height.
 static int build_tiles(mhd_ctx *c)
 {
     const int nblk = (int)c->h_blk.size();
-    const int nlead = (int)c->h_lead_k.size();
-    for (int sh = 0; sh < 2; ++sh)
-        for (int r = 0; r < R_COUNT; ++r)
-            for (int ji = 0; ji < 3; ++ji)
+    c->n_two = 0;
+    for (const MhdBlockDev &d : c->h_blk)
+        c->n_two += d.model == 1;
+    for (int s = 0; s < BS_COUNT; ++s)
+    {
+        auto in_set = [&](int model) { return s == BS_ALL || (s == BS_ONE) == (model == 0); };
+        for (int sh = 0; sh < 2; ++sh)
+        {
+            TileSet &T = c->ts[s][sh];
+            T.lead_k.clear();
+            T.lead_moff.clear();
+            T.members.clear();
+            if (sh == 0)
             {
-                TileArr &T = c->tiles[sh][r][ji];
-                const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
-                const int nent = sh ? nlead : nblk;
-                std::vector<int> start(std::max(nent, 1), 0);
-                long long total = 0;
-                for (int e = 0; e < nent; ++e)
-                {
-                    const MhdBlockDev &d = c->h_blk[sh ? c->h_members[c->h_lead_moff[e]] : e];
-                    const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
-                    start[e] = (int)total;
-                    total += (long long)(sh ? c->h_lead_k[e] : 1) * ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
-                }
-                if (total > (1LL << 31) - 64)
-                    return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
-                T.n = (int)total;
-                int rc;
-                if ((rc = grow(c, T.d_start, T.cap_start, start.size())))
-                    return rc;
-                HIPCHK(c, hipMemcpy(T.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+                if (s == BS_ALL)
+                    T.nent = nblk; // identity: no member list
+                else
+                    for (int b = 0; b < nblk; ++b)
+                        if (in_set(c->h_blk[b].model))
+                        {
+                            T.lead_k.push_back(1);
+                            T.lead_moff.push_back((int)T.members.size());
+                            T.members.push_back(b);
+                        }
             }
+            else
+            {
+                std::vector<std::vector<int>> by_geo(c->geos.size());
+                std::vector<int> geo_order;
+                for (auto &rp : c->cases)
+                    if (rp->state == 1 && in_set(rp->cs.blk[0].d.model))
+                    {
+                        if (by_geo[rp->geo_set].empty())
+                            geo_order.push_back(rp->geo_set);
+                        by_geo[rp->geo_set].push_back(rp->id);
+                    }
+                for (int g : geo_order)
+                {
+                    const std::vector<int> &cs = by_geo[g];
+                    for (int b = 0; b < c->cases[cs[0]]->nblocks; ++b)
+                    {
+                        T.lead_k.push_back((int)cs.size());
+                        T.lead_moff.push_back((int)T.members.size());
+                        for (int ci : cs)
+                            T.members.push_back(c->cases[ci]->first_block + b);
+                    }
+                }
+            }
+            if (!(s == BS_ALL && sh == 0))
+                T.nent = (int)T.lead_k.size();
+            int rc;
+            if ((rc = grow(c, T.d_lead_k, T.cap_k, std::max<size_t>(T.lead_k.size(), 1))) ||
+                (rc = grow(c, T.d_lead_moff, T.cap_moff, std::max<size_t>(T.lead_moff.size(), 1))) ||
+                (rc = grow(c, T.d_members, T.cap_mem, std::max<size_t>(T.members.size(), 1))))
+                return rc;
+            if (!T.lead_k.empty())
+            {
+                HIPCHK(c, hipMemcpy(T.d_lead_k, T.lead_k.data(), sizeof(int) * T.lead_k.size(), hipMemcpyHostToDevice));
+                HIPCHK(c, hipMemcpy(T.d_lead_moff, T.lead_moff.data(), sizeof(int) * T.lead_moff.size(),
+                                    hipMemcpyHostToDevice));
+                HIPCHK(c, hipMemcpy(T.d_members, T.members.data(), sizeof(int) * T.members.size(),
+                                    hipMemcpyHostToDevice));
+            }
+            const bool ident = T.members.empty();
+            for (int r = 0; r < R_COUNT; ++r)
+                for (int ji = 0; ji < 3; ++ji)
+                {
+                    TileArr &A = T.tiles[r][ji];
+                    const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
+                    std::vector<int> start(std::max(T.nent, 1), 0);
+                    long long total = 0;
+                    for (int e = 0; e < T.nent; ++e)
+                    {
+                        const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
+                        const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
+                        start[e] = (int)total;
+                        total += (long long)(ident ? 1 : T.lead_k[e]) * ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
+                    }
+                    if (total > (1LL << 31) - 64)
+                        return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                    A.n = (int)total;
+                    if ((rc = grow(c, A.d_start, A.cap_start, start.size())))
+                        return rc;
+                    HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+                }
+        }
+    }
     return MHD_OK;
+}
+
+// The block set a kernel kind runs over: the model-specific kernels over
+// their model's blocks (the single-fluid ones over every block when no
+// two-fluid case is resident), the shared kernels over every block.
+static int set_of(const mhd_ctx *c, int kernel)
+{
+    if (kernel == KP_FLUID2 || kernel == KP_DERIVED2 || kernel == KP_AMBI)
+        return BS_TWO;
+    if ((kernel == KP_FLUID || kernel == KP_DERIVED) && c->n_two > 0)
+        return BS_ONE;
+    return BS_ALL;
 }
 
 static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
@@ -316,14 +417,16 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     const int r = kKernelRange[kernel];
     const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
     const int ji = v % 3, sh = v >= 6 ? 1 : 0;
-    const TileArr &T = c->tiles[sh][r][ji];
+    const TileSet &S = c->ts[set_of(c, kernel)][sh];
+    const TileArr &T = S.tiles[r][ji];
+    const bool ident = S.members.empty();
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
     L.tm.block_start = T.d_start;
-    L.tm.lead_k = sh ? c->d_lead_k : nullptr;
-    L.tm.lead_moff = sh ? c->d_lead_moff : nullptr;
-    L.tm.members = sh ? c->d_members : nullptr;
-    L.tm.nblk = sh ? (int)c->h_lead_k.size() : (int)c->h_blk.size();
+    L.tm.lead_k = ident ? nullptr : S.d_lead_k;
+    L.tm.lead_moff = ident ? nullptr : S.d_lead_moff;
+    L.tm.members = ident ? nullptr : S.d_members;
+    L.tm.nblk = S.nent;
     L.tm.ntiles = T.n;
     L.tm.lo = kRangeLo[r];
     L.tm.hi = kRangeHi[r];
@@ -365,7 +468,7 @@ static int rebuild(mhd_ctx *c)
         {
             MhdBlockDev d = r.cs.blk[b].d;
             d.case_id = ci;
-            d.st = (int64_t)(uintptr_t)(r.d_st + (int64_t)b * ST_NSLOT * r.slot);
+            d.st = (int64_t)(uintptr_t)(r.d_st + (int64_t)b * st_nslot(d.model) * r.slot);
             d.geo = (int64_t)(uintptr_t)(G.d + (int64_t)b * G.block_bytes);
             c->h_blk.push_back(d);
         }
@@ -374,45 +477,6 @@ static int rebuild(mhd_ctx *c)
         return rc;
     if (!c->h_blk.empty())
         HIPCHK(c, hipMemcpy(c->d_blk, c->h_blk.data(), sizeof(MhdBlockDev) * c->h_blk.size(), hipMemcpyHostToDevice));
-    // geometry-shared entries: per geometry set (in order of first use), per
-    // block, the active cases holding it, in case order
-    c->h_lead_k.clear();
-    c->h_lead_moff.clear();
-    c->h_members.clear();
-    {
-        std::vector<std::vector<int>> by_geo(c->geos.size());
-        std::vector<int> geo_order;
-        for (auto &rp : c->cases)
-            if (rp->state == 1)
-            {
-                if (by_geo[rp->geo_set].empty())
-                    geo_order.push_back(rp->geo_set);
-                by_geo[rp->geo_set].push_back(rp->id);
-            }
-        for (int g : geo_order)
-        {
-            const std::vector<int> &cs = by_geo[g];
-            for (int b = 0; b < c->cases[cs[0]]->nblocks; ++b)
-            {
-                c->h_lead_k.push_back((int)cs.size());
-                c->h_lead_moff.push_back((int)c->h_members.size());
-                for (int ci : cs)
-                    c->h_members.push_back(c->cases[ci]->first_block + b);
-            }
-        }
-    }
-    if ((rc = grow(c, c->d_lead_k, c->cap_lead_k, std::max<size_t>(c->h_lead_k.size(), 1))) ||
-        (rc = grow(c, c->d_lead_moff, c->cap_lead_moff, std::max<size_t>(c->h_lead_moff.size(), 1))) ||
-        (rc = grow(c, c->d_members, c->cap_members, std::max<size_t>(c->h_members.size(), 1))))
-        return rc;
-    if (!c->h_lead_k.empty())
-    {
-        HIPCHK(c, hipMemcpy(c->d_lead_k, c->h_lead_k.data(), sizeof(int) * c->h_lead_k.size(), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_lead_moff, c->h_lead_moff.data(), sizeof(int) * c->h_lead_moff.size(),
-                            hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->d_members, c->h_members.data(), sizeof(int) * c->h_members.size(),
-                            hipMemcpyHostToDevice));
-    }
     HIPCHK(c, hipMemcpy(c->d_cst, cst.data(), sizeof(MhdCaseConst) * c->case_cap, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_active, act.data(), sizeof(int) * c->case_cap, hipMemcpyHostToDevice));
     return build_tiles(c);
@@ -468,7 +532,8 @@ static int place_case(mhd_ctx *c, CaseRec &r)
     for (int g = 0; g < (int)c->geos.size(); ++g)
         if (c->geos[g].refs > 0 && c->geos[g].key == r.cs.shape_key)
             gi = g;
-    const size_t st_bytes = sizeof(double) * (size_t)ST_NSLOT * slot * r.nblocks;
+    const int nslot = st_nslot(r.cs.blk[0].d.model);
+    const size_t st_bytes = sizeof(double) * (size_t)nslot * slot * r.nblocks;
     const size_t geo_bytes = gi < 0 ? (size_t)geo_block_bytes(slot, weight_esize(c)) * r.nblocks : 0;
     const double budget = c->opt.hbm_budget_gb * 1e9;
     if (budget > 0 && (double)(c->bytes_used + st_bytes + geo_bytes) > budget)
@@ -510,11 +575,16 @@ static int place_case(mhd_ctx *c, CaseRec &r)
     HIPCHK(c, hipMalloc((void **)&r.d_st, st_bytes));
     for (int b = 0; b < r.nblocks; ++b)
     {
-        double *dst = r.d_st + (int64_t)b * ST_NSLOT * slot;
-        HIPCHK(c, hipMemcpy(dst, r.cs.blk[b].st.data(), sizeof(double) * ST_NSLOT * slot, hipMemcpyHostToDevice));
+        double *dst = r.d_st + (int64_t)b * nslot * slot;
+        HIPCHK(c, hipMemcpy(dst, r.cs.blk[b].st.data(), sizeof(double) * nslot * slot, hipMemcpyHostToDevice));
         if (c->ucur == 1) // admitted mid-run: the current U buffer is the second one
+        {
             HIPCHK(c, hipMemcpy(dst + ST_U1 * slot, dst + ST_U0 * slot, sizeof(double) * 5 * slot,
                                 hipMemcpyDeviceToDevice));
+            if (nslot == ST_NSLOT2)
+                HIPCHK(c, hipMemcpy(dst + ST_UNA1 * slot, dst + ST_UNA0 * slot, sizeof(double) * 5 * slot,
+                                    hipMemcpyDeviceToDevice));
+        }
         std::vector<double>().swap(r.cs.blk[b].geo);
         std::vector<double>().swap(r.cs.blk[b].st);
     }
@@ -582,11 +652,19 @@ static void run_ops(mhd_ctx *c, int sy, int ushift)
 // Derived fields (contravariant velocities, B_cell and the next dt, then
 // J_cell); the J_cell wall condition (the SY_JCELL_BC op list of the host
 // set-up) is applied inside the J_cell kernel.
+static bool runs(const mhd_ctx *c, int kernel) { return c->ts[set_of(c, kernel)][0].nent > 0; }
+
 static void derived(mhd_ctx *c)
 {
+    if (runs(c, KP_DERIVED))
     {
         ProfScope p(c, KP_DERIVED);
         mhdk::launch_derived(launch_of(c, KP_DERIVED), c->d_cst, c->d_dtbits, c->ucur, mixed_of(c), c->q);
+    }
+    if (runs(c, KP_DERIVED2))
+    {
+        ProfScope p(c, KP_DERIVED2);
+        mhdk::launch_derived2(launch_of(c, KP_DERIVED2), c->d_cst, c->d_dtbits, c->ucur, mixed_of(c), c->q);
     }
     ProfScope p(c, KP_JCELL);
     mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), c->q);
@@ -619,13 +697,26 @@ static void enqueue_step(mhd_ctx *c)
         ProfScope p(c, KP_EDGE);
         mhdk::launch_edge_emf(launch_of(c, KP_EDGE), c->q);
     }
+    if (runs(c, KP_AMBI))
+    {
+        // two-fluid EMF terms added before the Eedge sync
+        // (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:30-33)
+        ProfScope p(c, KP_AMBI);
+        mhdk::launch_ambipolar_emf(launch_of(c, KP_AMBI), c->d_cst, c->ucur, c->q);
+    }
     {
         ProfScope p(c, KP_OPS);
         run_ops(c, SY_EEDGE_BC, 0);
     }
+    if (runs(c, KP_FLUID))
     {
         ProfScope p(c, KP_FLUID);
         mhdk::launch_fluid(launch_of(c, KP_FLUID), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
+    }
+    if (runs(c, KP_FLUID2))
+    {
+        ProfScope p(c, KP_FLUID2);
+        mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
     }
     {
         ProfScope p(c, KP_CURL);
@@ -653,12 +744,14 @@ static int autotune(mhd_ctx *c)
     if (c->h_blk.empty())
         return MHD_OK;
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED, KP_JCELL};
+    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED, KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     for (int k : kernels)
     {
+        if (!runs(c, k))
+            continue;
         double best = 1e30;
         int best_j = c->jt_idx[k];
         for (int ji = 0; ji < kVariants; ++ji)
@@ -681,6 +774,16 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_JCELL:
                     mhdk::launch_jcell(L, mixed, c->q);
+                    break;
+                case KP_FLUID2:
+                    mhdk::launch_fluid2(L, c->d_cst, c->d_zero, c->ucur, mixed, c->opt.recon, c->q);
+                    break;
+                case KP_DERIVED2:
+                    mhdk::launch_derived2(L, c->d_cst, c->d_dtbits, c->ucur, mixed, c->q);
+                    break;
+                case KP_AMBI:
+                    // E is step scratch: the next step's edge EMF rewrites it
+                    mhdk::launch_ambipolar_emf(L, c->d_cst, c->ucur, c->q);
                     break;
                 default:
                     mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, mixed, c->q);
@@ -795,7 +898,7 @@ extern "C" int mhd_add_case_mercury(mhd_ctx *c, const mhd_wind_desc *d, const mh
 static size_t case_bytes(const mhd_ctx *c, const CaseSetup &cs)
 {
     const size_t slot = (size_t)cs.blk[0].d.slot, nb = cs.blk.size();
-    size_t need = sizeof(double) * ST_NSLOT * slot * nb;
+    size_t need = sizeof(double) * st_nslot(cs.blk[0].d.model) * slot * nb;
     bool shared = false;
     for (const GeoSet &G : c->geos)
         shared = shared || (G.refs > 0 && G.key == cs.shape_key);
@@ -1015,6 +1118,8 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
     if (!field_slot(field, c->ucur, region, slot, loc, nc))
         return fail(c, MHD_ERR_ARG, "field not stored on the device");
     const MhdBlockDev &d = c->h_blk[r->first_block + block];
+    if (region == 0 && slot + nc > st_nslot(d.model))
+        return fail(c, MHD_ERR_ARG, "field not held by this case (two-fluid cases only)");
     if (n != aos_count(d, loc, nc))
         return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
     HIPCHK(c, hipStreamSynchronize(c->q));

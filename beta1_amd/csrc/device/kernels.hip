@@ -336,6 +336,14 @@ __global__ __launch_bounds__(256) void k_dt(const MhdTileMap tm, const MhdCaseCo
         CellGeo q;
         load_cell(b, U, dmetric(b), o, q);
         local = cell_dt(cst[b.case_id], q, Bc[o], Bc[sl + o], Bc[2 * sl + o]);
+        if (b.model == 1) // the two-fluid step also scans U_Na (MercurySolver_Timestep.cpp:118-119)
+        {
+            const double *UN = dst_ptr(b) + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                q.u[m] = UN[m * sl + o];
+            local = dmin(local, cell_dt(cst[b.case_id], q, Bc[o], Bc[sl + o], Bc[2 * sl + o]));
+        }
     }
     if (wg)
         reduce_dt(local, dtbits, b.case_id);
@@ -362,11 +370,8 @@ __global__ void k_dt_finalize(unsigned long long *dtbits, double *dt, int *err, 
 // contravariant cell velocity per axis, cells [-1, N] (step scratch):
 // cell_u_contra (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:414-428)
 // with U_plus from calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141).
-__device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double *uc)
+__device__ __forceinline__ void cell_ucontra_of(const CellGeo &c, double up0, double up1, double up2, double *uc)
 {
-    const Prim q = prim_from(c.u[0], c.u[1], c.u[2], c.u[3], c.u[4], gm);
-    const bool has = q.rho_raw > 1e-20;
-    const double up0 = has ? q.u : 0.0, up1 = has ? q.v : 0.0, up2 = has ? q.w : 0.0;
     const double inv_jac = 1.0 / (fabs(c.jac) + 1.0e-30);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
@@ -376,6 +381,13 @@ __device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double
         const double kz = 0.5 * (c.a[2 * ax][2] + c.a[2 * ax + 1][2]);
         uc[ax] = (kx * up0 + ky * up1 + kz * up2) * inv_jac;
     }
+}
+
+__device__ __forceinline__ void cell_ucontra(double gm, const CellGeo &c, double *uc)
+{
+    const Prim q = prim_from(c.u[0], c.u[1], c.u[2], c.u[3], c.u[4], gm);
+    const bool has = q.rho_raw > 1e-20;
+    cell_ucontra_of(c, has ? q.u : 0.0, has ? q.v : 0.0, has ? q.w : 0.0, uc);
 }
 
 __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
@@ -785,7 +797,312 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 }
 
 // ======================================================================
-// Boundary / halo op lists (one phase per launch).
+// Two-fluid (Mercury) step: H+ and Na+ share the Rusanov fluxes, the CT
+// induction and the derived magnetic fields of the single-fluid step; these
+// kernels add the second species, the composition-weighted ion velocity,
+// the Mercury sources and the ambipolar EMF.
+
+// Hall_Num_Limiter (Z5_Mercury/src/physics/MercurySolver_Tools.cpp:90-136)
+struct NumInfo
+{
+    double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa;
+};
+__device__ __forceinline__ NumInfo num_limiter(const MhdCaseConst &c, double rhoH, double rhoNa)
+{
+    NumInfo n;
+    n.nH = dmax(rhoH, 0.0) / c.M_H * c.M_ref;
+    n.nNa = dmax(rhoNa, 0.0) / c.M_Na * c.M_ref;
+    n.ne = n.nH + n.nNa;
+    n.ne_eff = sqrt(n.ne * n.ne + c.ne_floor * c.ne_floor);
+    if (n.ne > 1.0e-300)
+    {
+        n.chiH = n.nH / n.ne;
+        n.chiNa = n.nNa / n.ne;
+    }
+    else
+    {
+        n.chiH = 1.0;
+        n.chiNa = 0.0;
+    }
+    n.wH = n.nH / n.ne_eff;
+    n.wNa = n.nNa / n.ne_eff;
+    return n;
+}
+
+// Derived fields of the two-fluid step, cells [-1, N]: contravariant
+// velocity of U_plus = chi_H u_H + chi_Na u_Na (calc_Uplus,
+// Z5_Mercury/src/physics/MercurySolver_Tools.cpp:194-277), the electron
+// pressure p_H + p_Na, and on interior cells B_cell and the next dt over both
+// species (Z5_Mercury/src/physics/MercurySolver_Timestep.cpp:8-140).
+template <typename WT>
+__global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                  unsigned long long *__restrict__ dtbits, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    const bool ok = tile_cell(tm, b, i, j, k, wg);
+    double local = 1e100;
+    if (ok)
+    {
+        const MhdCaseConst &c = cst[b.case_id];
+        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+        double *S = dst_ptr(b);
+        CellGeo q;
+        load_cell(b, S + (int64_t)(ST_U0 + 5 * ucur) * sl, dmetric(b), o, q);
+        const double *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+        double un[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            un[m] = UN[m * sl + o];
+        const Prim ph = prim_from(q.u[0], q.u[1], q.u[2], q.u[3], q.u[4], c.gamma);
+        const Prim pn = prim_from(un[0], un[1], un[2], un[3], un[4], c.gamma);
+        {
+            const double rH = dmax(q.u[0], 0.0), rN = dmax(un[0], 0.0);
+            const NumInfo n = num_limiter(c, rH, rN);
+            const bool hH = rH > 1e-20, hN = rN > 1e-20;
+            const double up0 = n.chiH * (hH ? ph.u : 0.0) + n.chiNa * (hN ? pn.u : 0.0);
+            const double up1 = n.chiH * (hH ? ph.v : 0.0) + n.chiNa * (hN ? pn.v : 0.0);
+            const double up2 = n.chiH * (hH ? ph.w : 0.0) + n.chiNa * (hN ? pn.w : 0.0);
+            double uc[3];
+            cell_ucontra_of(q, up0, up1, up2, uc);
+            double *UC = S + (int64_t)ST_UC * sl;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                UC[a * sl + o] = uc[a];
+            S[(int64_t)ST_PE * sl + o] = ph.p + pn.p;
+        }
+        if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
+        {
+            const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+            const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+            const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
+                                  By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
+            const WT *W = dweights<WT>(b) + (int64_t)(G_WB - G_NMETRIC) * sl;
+            double bc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < 6; ++n)
+            {
+                bc[0] += W[(int64_t)n * sl + o] * pt[n];
+                bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
+                bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
+            }
+            double *Bc = S + (int64_t)ST_BCELL * sl;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                Bc[a * sl + o] = bc[a];
+            local = cell_dt(c, q, bc[0], bc[1], bc[2]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                q.u[m] = un[m];
+            local = dmin(local, cell_dt(c, q, bc[0], bc[1], bc[2]));
+        }
+    }
+    if (wg)
+        reduce_dt(local, dtbits, b.case_id);
+}
+
+// Fluid update of the two-fluid step: Rusanov fluxes of each species
+// (Z5_Mercury/src/physics/MercurySolver_FluidRHS.cpp:3-202), the Mercury
+// sources (:206-527: induction force n_s (u_s - u_+) x B, composition-
+// weighted J x B, electron-pressure force, Na+ photoionization mass and
+// energy, drag on the neutrals and thermal exchange) and the Euler update of
+// both species (MercurySolver_Advance.cpp:73-112).  The pressure-gradient
+// metric averages faces i-1 and i (j-1 and j, k-1 and k) as the reference
+// does.
+template <typename R, int RECON>
+__global__ __launch_bounds__(256) void k_fluid2(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                const double *__restrict__ dts, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
+        return;
+    const MhdCaseConst &c = cst[b.case_id];
+    const double gm = c.gamma;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *S = dst_ptr(b);
+    const double *G = dmetric(b);
+    const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+    double rhs[2][5];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+    {
+        const double *U = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * ucur) * sl;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            rhs[sp][m] = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+        {
+            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+            double Flo[5], Fhi[5];
+            face_flux<R, RECON>(U, sl, A, o - step[ax], o, step[ax], gm, Flo);
+            face_flux<R, RECON>(U, sl, A, o, o + step[ax], step[ax], gm, Fhi);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                rhs[sp][m] -= (Fhi[m] - Flo[m]) * invJ;
+        }
+    }
+    const double *UH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+    const Prim ph = prim_of<sizeof(R) == 8>(UH, sl, o, gm), pn = prim_of<sizeof(R) == 8>(UN, sl, o, gm);
+    const NumInfo num = num_limiter(c, UH[o], UN[o]);
+    double upx, upy, upz;
+    {
+        // U_plus of this cell (calc_Uplus), from the same U the step started from
+        const double rH = dmax(UH[o], 0.0), rN = dmax(UN[o], 0.0);
+        const NumInfo n = num_limiter(c, rH, rN);
+        const bool hH = rH > 1e-20, hN = rN > 1e-20;
+        upx = n.chiH * (hH ? ph.u : 0.0) + n.chiNa * (hN ? pn.u : 0.0);
+        upy = n.chiH * (hH ? ph.v : 0.0) + n.chiNa * (hN ? pn.v : 0.0);
+        upz = n.chiH * (hH ? ph.w : 0.0) + n.chiNa * (hN ? pn.w : 0.0);
+    }
+    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+    const double Bx = Bc[o], By = Bc[sl + o], Bz = Bc[2 * sl + o];
+    const double Jx = Jc[o], Jy = Jc[sl + o], Jz = Jc[2 * sl + o];
+    const double sjbx = Jy * Bz - Jz * By;
+    const double sjby = Jz * Bx - Jx * Bz;
+    const double sjbz = Jx * By - Jy * Bx;
+    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
+    if (c.ambipolar != 0.0)
+    {
+        double m9[9];
+        const double V = fabs(G[(int64_t)G_JAC * sl + o]);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+        {
+            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                m9[3 * ax + q] = V <= 0.0 ? 0.0 : 0.5 * (A[q * sl + o - step[ax]] + A[q * sl + o]) / V;
+        }
+        const double *PE = S + (int64_t)ST_PE * sl;
+        const double gi = PE[o + step[0]] - PE[o - step[0]];
+        const double gj = PE[o + step[1]] - PE[o - step[1]];
+        const double gk = PE[o + step[2]] - PE[o - step[2]];
+        dpex = 0.5 * (m9[0] * gi + m9[3] * gj + m9[6] * gk);
+        dpey = 0.5 * (m9[1] * gi + m9[4] * gj + m9[7] * gk);
+        dpez = 0.5 * (m9[2] * gi + m9[5] * gj + m9[8] * gk);
+    }
+    const double sss = S[(int64_t)ST_PHOTO * sl + o];
+    const double vst = S[(int64_t)ST_DRAG * sl + o];
+    const double dt = dts[b.case_id];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+    {
+        const Prim &p = sp ? pn : ph;
+        const double *U = sp ? UN : UH;
+        double *R_ = rhs[sp];
+        const double n_s = sp ? num.nNa : num.nH;
+        const double chi = sp ? num.wNa : num.wH;
+        const double rho = dmax(U[o], 0.0);
+        const double u = p.u, v = p.v, w = p.w;
+        const double u2 = u * u + v * v + w * w;
+        const double subx = (v - upy) * Bz - (w - upz) * By;
+        const double suby = (w - upz) * Bx - (u - upx) * Bz;
+        const double subz = (u - upx) * By - (v - upy) * Bx;
+        const double subu = subx * u + suby * v + subz * w;
+        const double sjbu = sjbx * u + sjby * v + sjbz * w;
+        const double dpeu = dpex * u + dpey * v + dpez * w;
+        const double T = p.p / (dmax(p.rho_raw, 1e-12) * (sp ? c.state_coeff_Na : c.state_coeff));
+        const double Ts0 = T * c.T_ref;
+        const double b1 = sp ? vst * (c.Tn0 - Ts0) * (c.M_Na / (c.M_Na + c.M_Na))
+                             : vst * (c.Tn0 - Ts0) * (c.M_H / (c.M_H + c.M_Na));
+        const double b2 = sp ? (c.M_Na / (c.M_Na + c.M_Na)) * vst : (c.M_Na / (c.M_H + c.M_Na)) * vst;
+        R_[1] += c.induce * n_s * subx + c.momentum_hall_coeff * chi * sjbx;
+        R_[2] += c.induce * n_s * suby + c.momentum_hall_coeff * chi * sjby;
+        R_[3] += c.induce * n_s * subz + c.momentum_hall_coeff * chi * sjbz;
+        R_[4] += c.induce * n_s * subu + c.momentum_hall_coeff * chi * sjbu;
+        if (c.ambipolar != 0.0)
+        {
+            R_[1] -= chi * dpex;
+            R_[2] -= chi * dpey;
+            R_[3] -= chi * dpez;
+            R_[4] -= chi * dpeu;
+        }
+        if (sp)
+        {
+            R_[0] += c.a1_Na * sss;
+            R_[4] += c.a6 * sss * c.Tn0;
+        }
+        R_[1] -= c.a4 * rho * u * vst;
+        R_[2] -= c.a4 * rho * v * vst;
+        R_[3] -= c.a4 * rho * w * vst;
+        R_[4] -= c.a4 * rho * u2 * vst;
+        R_[4] += c.a5 * n_s * b1;
+        R_[4] += c.a4 * rho * u2 * b2;
+        double *Un = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * (1 - ucur)) * sl;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            Un[m * sl + o] = U[m * sl + o] + dt * R_[m];
+    }
+}
+
+// AddAmbipolarEdgeEMF_ (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Ambipo.cpp:3-144):
+// E_edge -= pe_coef (pe(node 1) - pe(node 0)) / ne_eff, pe averaged over the
+// 8 cells around a node, ne_eff of the mean density of the 4 cells around
+// the edge.  Runs after the ideal edge EMF and before the Eedge sync.
+__device__ __forceinline__ double pe_node(const double *__restrict__ PE, int64_t o, int64_t di, int64_t dj, int64_t dk)
+{
+    double sum = 0.0;
+#pragma unroll
+    for (int a = -1; a <= 0; ++a)
+#pragma unroll
+        for (int bb = -1; bb <= 0; ++bb)
+#pragma unroll
+            for (int cc = -1; cc <= 0; ++cc)
+                sum += PE[o + a * di + bb * dj + cc * dk];
+    return 0.125 * sum;
+}
+
+__global__ __launch_bounds__(256) void k_ambipolar_emf(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                       int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
+        return;
+    const MhdCaseConst &c = cst[b.case_id];
+    if (c.ambipolar == 0.0)
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *S = dst_ptr(b);
+    const double *PE = S + (int64_t)ST_PE * sl;
+    const double *RH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *RN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+    double *E = S + (int64_t)ST_E * sl;
+    const int idx[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
+    const double pe0 = pe_node(PE, o, st[0], st[1], st[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        bool in = true;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            in = in && (idx[q] < n[q] + (q == a ? 0 : 1));
+        if (!in)
+            continue;
+        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+        const double pe1 = pe_node(PE, o + st[a], st[0], st[1], st[2]);
+        double rhoH = 0.0, rhoNa = 0.0;
+#pragma unroll
+        for (int s1 = -1; s1 <= 0; ++s1)
+#pragma unroll
+            for (int s2 = -1; s2 <= 0; ++s2)
+            {
+                const int64_t oc = o + s1 * st[t1] + s2 * st[t2];
+                rhoH += RH[oc];
+                rhoNa += RN[oc];
+            }
+        const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
+        E[a * sl + o] -= c.pe_coef * (pe1 - pe0) / nm.ne_eff;
+    }
+}
+
 __device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
 {
     rho = dmax(rho, 2.5e-3);
@@ -829,6 +1146,18 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
     case OP_QINF:
         for (int c = 0; c < op.ncomp; ++c)
             d[c * op.stride] = cst[op.cas].qinf[c];
+        break;
+    case OP_QINF_NA:
+        for (int c = 0; c < op.ncomp; ++c)
+            d[c * op.stride] = cst[op.cas].qinfs[c];
+        break;
+    case OP_NA_FLOOR:
+        // Na+ absorbing wall (Z5_Mercury/src/boundary/Boundary_Condition_Fluid.cpp:144-160)
+        d[0] = 1.0e-12;
+        d[op.stride] = 0.0;
+        d[2 * op.stride] = 0.0;
+        d[3 * op.stride] = 0.0;
+        d[4 * op.stride] = 1.0e-12 / (cst[op.cas].gamma - 1.0);
         break;
     case OP_SOLID_FIRST:
     case OP_SOLID_GHOST:
@@ -960,6 +1289,31 @@ void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long
         hipLaunchKernelGGL(k_derived<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
     else
         hipLaunchKernelGGL(k_derived<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+}
+void launch_derived2(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
+                     hipStream_t q)
+{
+    if (mixed)
+        hipLaunchKernelGGL(k_derived2<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+    else
+        hipLaunchKernelGGL(k_derived2<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+}
+void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
+                   hipStream_t q)
+{
+    const dim3 g = grid_of(L), bl = block_of(L);
+    if (mixed && recon)
+        hipLaunchKernelGGL((k_fluid2<float, 1>), g, bl, 0, q, L.tm, cst, dts, ucur);
+    else if (mixed)
+        hipLaunchKernelGGL((k_fluid2<float, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
+    else if (recon)
+        hipLaunchKernelGGL((k_fluid2<double, 1>), g, bl, 0, q, L.tm, cst, dts, ucur);
+    else
+        hipLaunchKernelGGL((k_fluid2<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
+}
+void launch_ambipolar_emf(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_ambipolar_emf, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
 }
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {

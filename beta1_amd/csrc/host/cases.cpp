@@ -15,7 +15,7 @@ namespace mpcns
 // Execute an op list on host state arrays (case-relative offsets).
 void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops, const MhdCaseConst &cst)
 {
-    const int64_t bs = (int64_t)ST_NSLOT * blk[0].d.slot;
+    const int64_t bs = (int64_t)st_nslot(blk[0].d.model) * blk[0].d.slot;
     auto at = [&](int64_t rel) -> double & { return blk[rel / bs].st[rel % bs]; };
     for (const MhdOp &o : ops)
     {
@@ -59,6 +59,12 @@ void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops
             case OP_QINF:
                 at(d) = cst.qinf[c];
                 break;
+            case OP_QINF_NA:
+                at(d) = cst.qinfs[c];
+                break;
+            case OP_NA_FLOOR:
+                at(d) = c == 0 ? 1.0e-12 : (c == 4 ? 1.0e-12 / (cst.gamma - 1.0) : 0.0);
+                break;
             default:
                 throw std::runtime_error("apply_ops_host: op kind not supported on host");
             }
@@ -69,7 +75,7 @@ void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops
 static void alloc_state(std::vector<BlockArrays> &blk)
 {
     for (auto &A : blk)
-        A.st.assign((size_t)ST_NSLOT * A.d.slot, 0.0);
+        A.st.assign((size_t)st_nslot(A.d.model) * A.d.slot, 0.0);
 }
 
 // Copy-type boundary (DefaultPhysicalCopy, reference
@@ -80,7 +86,7 @@ static std::vector<MhdOp> face_copy_sync(CaseSetup &cs, int slot0)
     std::vector<MhdOp> ops;
     const Grid &grd = cs.grid;
     const int g = grd.g;
-    const int64_t bs = (int64_t)ST_NSLOT * cs.blk[0].d.slot;
+    const int64_t bs = (int64_t)st_nslot(cs.blk[0].d.model) * cs.blk[0].d.slot;
     for (int b = 0; b < (int)grd.blocks.size(); ++b)
     {
         const Block &B = grd.blocks[b];
@@ -136,12 +142,15 @@ static std::vector<MhdOp> face_copy_sync(CaseSetup &cs, int slot0)
     return ops;
 }
 
-static void finalize_grid(CaseSetup &cs)
+static void finalize_grid(CaseSetup &cs, int model = 0)
 {
     fill_ghost_nodes(cs.grid);
     build_geometry(cs.grid, cs.blk);
     for (auto &A : cs.blk)
+    {
         A.d.case_id = 0;
+        A.d.model = model;
+    }
     alloc_state(cs.blk);
 }
 
@@ -297,20 +306,79 @@ CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gam
     return cs;
 }
 
+// Cell centre as the reference's dual grid sums it (Block::calc_Dual_Grids,
+// src/1_grid/1_MPCNS_Grid_GCL.cpp:6-23): dual(i+1, j+1, k+1) of cell (i, j, k).
+static void dual_centre(const Block &B, int i, int j, int k, double out[3])
+{
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const int64_t o[8] = {B.node_off(I, J, K),         B.node_off(I - 1, J, K),     B.node_off(I, J - 1, K),
+                          B.node_off(I - 1, J - 1, K), B.node_off(I, J, K - 1),     B.node_off(I - 1, J, K - 1),
+                          B.node_off(I, J - 1, K - 1), B.node_off(I - 1, J - 1, K - 1)};
+    const std::vector<double> *X[3] = {&B.x, &B.y, &B.z};
+    for (int a = 0; a < 3; ++a)
+    {
+        const std::vector<double> &v = *X[a];
+        out[a] = 0.125 * (v[o[0]] + v[o[1]] + v[o[2]] + v[o[3]] + v[o[4]] + v[o[5]] + v[o[6]] + v[o[7]]);
+    }
+}
+
+// Na+ photoionization production rate [cm^-3 s^-1] at a cell centre: the
+// neutral Na exosphere of Exner et al. (2020, Table 2) -- a radially
+// symmetric part plus Gaussian bells for thermal desorption, photon-stimulated
+// desorption and sputtering -- times the photoionization frequency, lower in
+// the geometric shadow (Mercury_Initial::Initial_Na_And_PhotoRate_,
+// Z5_Mercury/include/2_Initial.h:162-392).
+double mercury_photo_rate(const double x[3], double L_ref)
+{
+    const double r_nd = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (!(r_nd > 1.0) || !std::isfinite(r_nd))
+        return 0.0;
+    const double pi = std::acos(-1.0), deg = pi / 180.0;
+    const double rx = x[0] * (1.0 / r_nd), ry = x[1] * (1.0 / r_nd), rz = x[2] * (1.0 / r_nd);
+    const double dr = L_ref * r_nd - L_ref;
+    auto gauss2 = [](double dth, double H) { return H <= 0.0 ? 0.0 : std::exp(-((dth / H) * (dth / H))); };
+    auto angle_to = [&](double lat, double lon) {
+        const double cl = std::cos(lat);
+        double d = rx * (cl * std::cos(lon)) + ry * (cl * std::sin(lon)) + rz * std::sin(lat);
+        d = std::max(-1.0, std::min(1.0, d));
+        return std::acos(d);
+    };
+    const double day = 0.0, night = pi;
+    const double n_miv = 7.84e6 * std::exp(-dr / 431.0e3);
+    const double n_td = 8.86e9 * std::exp(-dr / 100.0e3) * gauss2(angle_to(0.0, day), 15.0 * deg);
+    const double n_psd = 4.06e10 * std::exp(-dr / 232.0e3) *
+                         (gauss2(angle_to(50.0 * deg, day), 20.0 * deg) + gauss2(angle_to(-50.0 * deg, day), 20.0 * deg));
+    const double sp = 5.67e6 * std::exp(-dr / 748.0e3);
+    const double n_sp_day = sp * (gauss2(angle_to(80.0 * deg, day), 15.0 * deg) + gauss2(angle_to(-80.0 * deg, day), 15.0 * deg));
+    const double n_sp_night =
+        sp * (gauss2(angle_to(15.0 * deg, night), 10.0 * deg) + gauss2(angle_to(-15.0 * deg, night), 10.0 * deg));
+    double n_m3 = (1.0 / (r_nd * r_nd)) * (n_miv + n_td + n_psd + n_sp_day + n_sp_night);
+    if (!std::isfinite(n_m3) || n_m3 < 0.0)
+        n_m3 = 0.0;
+    const double n_cm3 = n_m3 * 1.0e-6;
+    const bool shadow = x[0] < 0.0 && std::sqrt(x[1] * x[1] + x[2] * x[2]) < 1.0;
+    double q = (shadow ? 1.0e-5 : 5.0e-5) * n_cm3;
+    if (!std::isfinite(q) || q < 0.0)
+        q = 0.0;
+    return q;
+}
+
 // Mercury-configured shell: reference values of Z5_Mercury/9999setup
 // (REF U=4.031e5 m/s, T=1.7407e5 K, n=4e7 m^-3, M=1e-3 kg/mol, B=1 nT,
 // L=2.44e6 m), an offset dipole added to Badd through the curl of its edge
 // circulation (Z5_Mercury/include/operators/Dipole.h:89-333, vector potential
-// A = m x dr / (|dr|^2 + a^2)^(3/2)), plus the IMF flux.  The body is the
-// absorbing wall of the Lunar set-up; the second ion species, Hall,
-// ambipolar and resistive terms of the reference's Mercury step are not part
-// of this single-fluid explicit step.
+// A = m x dr / (|dr|^2 + a^2)^(3/2)), plus the IMF flux.  Two-fluid (the
+// reference's physics): H+ solar wind and a Na+ seed state, the Na+
+// photoionization source and the ion-neutral drag; single fluid: the shared
+// Lunar step with the dipole background.  The body is the absorbing wall of
+// the Lunar set-up in both (the reference's Mercury grid continues into a
+// coupled resistive interior, which this shell does not have).
 CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
                             const WindParams &w, const DipoleParams &dp)
 {
     CaseSetup cs;
     cs.grid = make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD);
-    finalize_grid(cs);
+    finalize_grid(cs, dp.two_fluid ? 1 : 0);
     char key[128];
     std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
     cs.shape_key = key;
@@ -349,8 +417,57 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
     c.u_inf[2] = w0;
     for (int a = 0; a < 3; ++a)
         c.B_imf[a] = w.B_nT[a] * 1e-9 / R.B;
+    if (dp.two_fluid)
+    {
+        // calc_physical_constant (Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46),
+        // source coefficients (MercurySolver_FluidRHS.cpp:210-225), seed
+        // state (Z5_Mercury/include/0_BackgroundState.h:65-81, rho_small 1e-6)
+        const double qe = 1.602e-19, M_Na = 0.023, m_Na = M_Na / R.NA, U3 = R.U * R.U * R.U;
+        c.model = 1.0;
+        c.ambipolar = dp.ambipolar ? 1.0 : 0.0;
+        c.M_Na = M_Na;
+        c.state_coeff_Na = (R.R * R.T) / (M_Na * R.U * R.U);
+        c.induce = (qe * R.L * R.B * R.n) / (rho_ref * R.U);
+        c.a1_Na = (R.L / R.U) * (m_Na / rho_ref) * 1e6;
+        c.a4 = R.L / R.U;
+        c.a5 = (3.0 * R.L * k_B * R.n) / (rho_ref * U3);
+        c.a6 = (1.0e6 * R.L * k_B) / (rho_ref * U3 * (gamma - 1.0));
+        c.T_ref = R.T;
+        c.Tn0 = 1000.0;
+        c.pe_coef = (rho_ref * R.U) / (qe * R.L * R.B * R.n);
+        const double rho_small = 1e-6;
+        const double ps = p0 * rho_small / 23.0;
+        c.qinfs[0] = rho_small * rho0;
+        c.qinfs[1] = c.qinfs[2] = c.qinfs[3] = 0.0;
+        c.qinfs[4] = ps / (gamma - 1.0) + 0.5 * c.qinfs[0] * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+    }
 
     build_ops(cs);
+
+    // Na+ seed state, photoionization rate and drag frequency of every cell
+    if (dp.two_fluid)
+        for (size_t b = 0; b < cs.blk.size(); ++b)
+        {
+            BlockArrays &A = cs.blk[b];
+            const Block &B = cs.grid.blocks[b];
+            const MhdBlockDev &d = A.d;
+            for (int q = 0; q < 5; ++q)
+                for (int64_t o = 0; o < d.slot; ++o)
+                    A.st[(size_t)(ST_UNA0 + q) * d.slot + o] = c.qinfs[q];
+            for (int i = -g; i < B.ni + g; ++i)
+                for (int j = -g; j < B.nj + g; ++j)
+                    for (int k = -g; k < B.nk + g; ++k)
+                    {
+                        double xc[3];
+                        dual_centre(B, i, j, k, xc);
+                        const int64_t o = blk_off(d, i, j, k);
+                        A.st[(size_t)ST_PHOTO * d.slot + o] = mercury_photo_rate(xc, R.L);
+                        // drag frequency: photoionization frequency of the
+                        // geometric shadow test (MercurySolver_FluidRHS.cpp:360-367)
+                        const bool shadow = xc[0] < 0.0 && std::sqrt(xc[1] * xc[1] + xc[2] * xc[2]) < 1.0;
+                        A.st[(size_t)ST_DRAG * d.slot + o] = shadow ? 1.0e-5 : 5.0e-5;
+                    }
+        }
 
     // nondimensional dipole: |B| = |m| / r^3 at the equator of the body
     const double m[3] = {0.0, 0.0, -dp.B_eq_nT * 1e-9 / R.B};

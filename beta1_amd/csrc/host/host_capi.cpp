@@ -2,6 +2,7 @@
 // weights, op lists, initial state) and read it back in the reference AoS
 // FieldBlock layout.  Used where no device is present (set-up checks, CPU
 // baselines) and by the device context, which consumes the same builders.
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -35,6 +36,8 @@ DipoleParams dipole_params(const mhd_dipole_desc &d)
     p.B_eq_nT = d.B_eq_nT;
     p.offset_z = d.offset_z;
     p.softening = d.softening;
+    p.two_fluid = d.single_fluid == 0;
+    p.ambipolar = d.no_ambipolar == 0;
     return p;
 }
 
@@ -77,6 +80,16 @@ bool field_slot(int f, int ucur, int &region, int &slot, int &loc, int &nc)
     case MHD_F_EFACE_ETA:
     case MHD_F_EFACE_ZETA:
         slot = ST_EFACE + 3 * (f - MHD_F_EFACE_XI), loc = LOC_FACE_XI + (f - MHD_F_EFACE_XI), nc = 3;
+        return true;
+    // two-fluid cases only (the caller checks the block's slot count)
+    case MHD_F_U_NA:
+        slot = ST_UNA0 + 5 * ucur, loc = LOC_CELL, nc = 5;
+        return true;
+    case MHD_F_PHOTO_RATE:
+        slot = ST_PHOTO, loc = LOC_CELL, nc = 1;
+        return true;
+    case MHD_F_DRAG_RATE:
+        slot = ST_DRAG, loc = LOC_CELL, nc = 1;
         return true;
     default:
         break;
@@ -230,6 +243,8 @@ extern "C" int mhd_host_field_count(mhd_host_case *h, int field, size_t *n)
     int region, slot, loc, nc;
     if (!h || !n || !field_slot(field, 0, region, slot, loc, nc))
         return MHD_ERR_ARG;
+    if (region == 0 && slot + nc > st_nslot(h->cs.blk[0].d.model))
+        return MHD_ERR_ARG;
     *n = aos_count(h->cs.blk[0].d, loc, nc);
     return MHD_OK;
 }
@@ -240,7 +255,7 @@ extern "C" int mhd_host_download(mhd_host_case *h, int block, int field, double 
     if (!h || !aos || block < 0 || block >= (int)h->cs.blk.size() || !field_slot(field, 0, region, slot, loc, nc))
         return MHD_ERR_ARG;
     BlockArrays &A = h->cs.blk[block];
-    if (n != aos_count(A.d, loc, nc))
+    if (n != aos_count(A.d, loc, nc) || (region == 0 && slot + nc > st_nslot(A.d.model)))
         return MHD_ERR_ARG;
     std::vector<double> &v = region == 0 ? A.st : A.geo;
     soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, aos, true);
@@ -253,7 +268,7 @@ extern "C" int mhd_host_upload(mhd_host_case *h, int block, int field, const dou
     if (!h || !aos || block < 0 || block >= (int)h->cs.blk.size() || !field_slot(field, 0, region, slot, loc, nc))
         return MHD_ERR_ARG;
     BlockArrays &A = h->cs.blk[block];
-    if (n != aos_count(A.d, loc, nc))
+    if (n != aos_count(A.d, loc, nc) || (region == 0 && slot + nc > st_nslot(A.d.model)))
         return MHD_ERR_ARG;
     std::vector<double> &v = region == 0 ? A.st : A.geo;
     soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, const_cast<double *>(aos), false);
@@ -293,6 +308,20 @@ extern "C" int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out
     if (n != v.size())
         return MHD_ERR_ARG;
     std::memcpy(out, v.data(), sizeof(double) * n);
+    return MHD_OK;
+}
+
+// Constants of the two-fluid step, in MhdCaseConst order from `model`:
+// model, ambipolar, M_Na, state_coeff_Na, qinfs[5], induce, a1_Na, a4, a5, a6,
+// T_ref, Tn0, pe_coef (17 doubles).
+extern "C" int mhd_host_constants_two_fluid(mhd_host_case *h, double *o)
+{
+    if (!h || !o)
+        return MHD_ERR_ARG;
+    const MhdCaseConst &k = h->cs.cst;
+    static_assert(offsetof(MhdCaseConst, pe_coef) - offsetof(MhdCaseConst, model) == 16 * sizeof(double),
+                  "two-fluid constants are 17 consecutive doubles");
+    std::memcpy(o, &k.model, 17 * sizeof(double));
     return MHD_OK;
 }
 

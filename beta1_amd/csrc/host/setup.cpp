@@ -753,7 +753,7 @@ struct OpCtx
 {
     const Grid &grd;
     const std::vector<BlockArrays> &blk;
-    int64_t blk_stride() const { return (int64_t)ST_NSLOT * blk[0].d.slot; }
+    int64_t blk_stride() const { return (int64_t)st_nslot(blk[0].d.model) * blk[0].d.slot; }
     int64_t ref(int b, int slot, int i, int j, int k) const
     {
         return (int64_t)b * blk_stride() + (int64_t)slot * blk[b].d.slot + blk_off(blk[b].d, i, j, k);
@@ -1006,8 +1006,31 @@ void build_ops(CaseSetup &cs)
                                                           C.ref(s.block, ST_UC + s.perm[a], s.idx[0], s.idx[1], s.idx[2]),
                                                           s.sign[a] > 0 ? OP_COPY : OP_NEG, 1, stride));
                 }
+        // ---- Na+ of the two-fluid step: far-field seed state, absorbing
+        //      wall ghosts at the floor state (reference
+        //      Z5_Mercury/src/boundary/Boundary_Condition_Fluid.cpp:19-165)
+        if (cs.blk[b].d.model == 1)
+            for (int f = 0; f < 6; ++f)
+            {
+                if (B.face[f].kind != 0 || B.face[f].bc == BC_NONE)
+                    continue;
+                Slab s = inner_slab(B, LOC_CELL, f);
+                for (int i = s.lo[0]; i < s.hi[0]; ++i)
+                    for (int j = s.lo[1]; j < s.hi[1]; ++j)
+                        for (int k = s.lo[2]; k < s.hi[2]; ++k)
+                            for (int l = 1; l <= g; ++l)
+                            {
+                                int q[3] = {i, j, k};
+                                q[s.ax] += s.sgn * l;
+                                cs.ops[SY_U_BC0].push_back(mk(C.ref(b, ST_UNA0, q[0], q[1], q[2]), 0,
+                                                              B.face[f].bc == BC_FARFIELD ? OP_QINF_NA : OP_NA_FLOOR, 5,
+                                                              stride));
+                            }
+            }
         // ---- halos
         halo_ops(C, b, LOC_CELL, 5, false, [](int) { return (int)ST_U0; }, cs.ops[SY_U_HALO]);
+        if (cs.blk[b].d.model == 1)
+            halo_ops(C, b, LOC_CELL, 5, false, [](int) { return (int)ST_UNA0; }, cs.ops[SY_U_HALO]);
         for (int a = 0; a < 3; ++a)
         {
             halo_ops(C, b, LOC_FACE_XI + a, 1, false, [](int l) { return (int)ST_B + (l - LOC_FACE_XI); }, cs.ops[SY_B_HALO]);

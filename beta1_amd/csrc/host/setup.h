@@ -73,7 +73,7 @@ struct BlockArrays
 {
     MhdBlockDev d{};
     std::vector<double> geo; // G_NSLOT * slot
-    std::vector<double> st;  // ST_NSLOT * slot
+    std::vector<double> st;  // st_nslot(d.model) * slot
 };
 
 // One element-wise operation of a boundary / halo phase.  Offsets are
@@ -98,7 +98,9 @@ enum MhdOpKind
     OP_CONST = 3,       // dst[0] = val
     OP_SOLID_FIRST = 4, // absorbing wall, first interior layer from second
     OP_SOLID_GHOST = 5, // absorbing wall, ghost from first interior layer
-    OP_QINF = 6         // dst[c] = qinf[c] of the case
+    OP_QINF = 6,        // dst[c] = qinf[c] of the case
+    OP_QINF_NA = 7,     // dst[c] = qinfs[c] (Na+ far field) of the case
+    OP_NA_FLOOR = 8     // Na+ absorbing wall ghost: rho = p = 1e-12, u = 0
 };
 
 // Op lists of one case; offsets are relative to the case's first block
@@ -190,6 +192,8 @@ struct DipoleParams
     double B_eq_nT;   // equatorial surface field of the dipole [nT]
     double offset_z;  // dipole centre along +z [body radii]
     double softening; // core softening length [body radii]
+    bool two_fluid = true;  // H+ and Na+ (the reference's Mercury step)
+    bool ambipolar = true;  // electron-pressure force and ambipolar EMF (two-fluid)
 };
 
 CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,

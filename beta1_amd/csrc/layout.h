@@ -69,8 +69,18 @@ enum MhdStateSlot
     ST_EFACE = 22, // 9 slots: Eface_xi(3 comps), Eface_eta(3), Eface_zeta(3)
     ST_E = 31,     // 3 slots: E_xi (EdgeXi), E_eta, E_zeta
     ST_UC = 34,    // 3 slots: cell contravariant velocity per axis (step scratch)
-    ST_NSLOT = 37
+    ST_NSLOT = 37,
+    // two-fluid (Mercury, H+ and Na+) cases add:
+    ST_UNA0 = 37,  // 5 comps: Na+ conservative state (U_Na)
+    ST_UNA1 = 42,  // 5 comps (second buffer)
+    ST_PHOTO = 47, // Na+ photoionization production rate [cm^-3 s^-1] (static)
+    ST_DRAG = 48,  // ion-neutral drag frequency [1/s] (static)
+    ST_PE = 49,    // electron pressure p_H + p_Na of cells [-1, N] (step scratch)
+    ST_NSLOT2 = 50
 };
+
+// State slots per block for a physics model (0 single fluid, 1 two-fluid).
+MHD_HD int st_nslot(int model) { return model == 1 ? (int)ST_NSLOT2 : (int)ST_NSLOT; }
 
 // Geometry slots (shared by every case with the same grid).
 enum MhdGeoSlot
@@ -114,7 +124,8 @@ struct MhdBlockDev
     int64_t slot;    // elements per slot
     int64_t st;      // device address of slot 0 of the block's state (0 before placement)
     int64_t geo;     // device address of slot 0 of the block's geometry
-    int64_t spare;
+    int model;       // 0 single fluid (Lunar step), 1 two-fluid H+ / Na+ (Mercury step)
+    int pad_;
 };
 
 // Per-case constants (the reference's calc_physical_constant,
@@ -133,6 +144,18 @@ struct MhdCaseConst
     double B_imf[3];            // far-field total B
     double u_inf[3];            // far-field velocity (primitive)
     double dt_floor;            // abort threshold (reference: 1e-7)
+    // two-fluid step (Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46,
+    // MercurySolver_FluidRHS.cpp:210-225, MercurySolver_MagExplicit_Ambipo.cpp:8-9)
+    double model;          // 0 single fluid, 1 two-fluid
+    double ambipolar;      // 1: electron-pressure force and ambipolar edge EMF
+    double M_Na;           // Na+ molar mass [kg/mol]
+    double state_coeff_Na; // p = rho T coeff of Na+
+    double qinfs[5];       // Na+ far-field (seed) state
+    double induce;         // momentum_induce_coeff
+    double a1_Na, a4, a5, a6; // photoionization mass / drag / thermal coefficients
+    double T_ref;          // reference temperature [K]
+    double Tn0;            // neutral temperature [K]
+    double pe_coef;        // ambipolar pressure coefficient
 };
 
 MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }

@@ -21,6 +21,8 @@ FIELDS = [
     ("beta_xi", FACE_XI, 1), ("beta_eta", FACE_ET, 1), ("beta_zeta", FACE_ZE, 1),
     ("alpha_xi", EDGE_XI, 1), ("alpha_eta", EDGE_ET, 1), ("alpha_zeta", EDGE_ZE, 1),
     ("Bcell_from_Bface_w", CELL, 18), ("Jcell_from_Jedge_w", CELL, 36),
+    # two-fluid (Mercury) cases
+    ("U_Na", CELL, 5), ("PV_Na", CELL, 5), ("RHS_Na", CELL, 5), ("Photo_rate", CELL, 1), ("drag_rate", CELL, 1),
 ]
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 LOCATION = {name: loc for name, loc, _ in FIELDS}
@@ -31,7 +33,7 @@ NCOMP = {name: nc for name, _, nc in FIELDS}
 STORED = ["U_H", "B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta", "B_cell", "J_cell",
           "E_xi", "E_eta", "E_zeta", "Eface_xi", "Eface_eta", "Eface_zeta", "Jac", "JDxi", "JDet", "JDze",
           "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta", "alpha_zeta",
-          "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
+          "Bcell_from_Bface_w", "Jcell_from_Jedge_w", "U_Na", "Photo_rate", "drag_rate"]
 
 
 def delta(loc, ax):

@@ -63,7 +63,10 @@ class WindDesc:
 @dataclass
 class MercuryDesc(WindDesc):
     """Mercury-configured shell: solar wind at 0.3-0.47 AU plus an offset
-    dipole (190 nT equatorial surface field, centre +0.2 R_M along z)."""
+    dipole (190 nT equatorial surface field, centre +0.2 R_M along z).
+    two_fluid: H+ and Na+ with the reference's Mercury sources (photo-
+    ionization, drag, induction and ambipolar forces); otherwise the shared
+    single-fluid step with the dipole background."""
     r_out: float = 10.0
     n_cc: float = 40.0
     T_eV: float = 15.0
@@ -71,9 +74,12 @@ class MercuryDesc(WindDesc):
     B_eq_nT: float = 190.0
     offset_z: float = 0.2
     softening: float = 0.05
+    two_fluid: bool = True
+    ambipolar: bool = True
 
     def dipole(self):
-        return MhdDipoleDesc(self.B_eq_nT, self.offset_z, self.softening)
+        return MhdDipoleDesc(self.B_eq_nT, self.offset_z, self.softening, 0 if self.two_fluid else 1,
+                             0 if self.ambipolar else 1)
 
 
 @dataclass
@@ -145,6 +151,11 @@ class HostCase:
             a = np.empty(s, dtype=np.float64)
             _check(lib().mhd_host_nodes(self.h, block, comp, _dptr(a), a.size), what="nodes")
             out.append(a)
+        return out
+
+    def constants_two_fluid(self):
+        out = np.zeros(17)
+        lib().mhd_host_constants_two_fluid(self.h, _dptr(out))
         return out
 
     def constants(self):

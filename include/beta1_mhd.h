@@ -46,7 +46,11 @@ enum mhd_field
     MHD_F_EFACE_XI, MHD_F_EFACE_ETA, MHD_F_EFACE_ZETA, MHD_F_F_XI, MHD_F_F_ETA, MHD_F_F_ZETA,
     MHD_F_RHS_H, MHD_F_RHS_B_XI, MHD_F_RHS_B_ETA, MHD_F_RHS_B_ZETA, MHD_F_DIVB,
     MHD_F_JAC, MHD_F_JDXI, MHD_F_JDET, MHD_F_JDZE, MHD_F_BETA_XI, MHD_F_BETA_ETA, MHD_F_BETA_ZETA,
-    MHD_F_ALPHA_XI, MHD_F_ALPHA_ETA, MHD_F_ALPHA_ZETA, MHD_F_BCELL_W, MHD_F_JCELL_W, MHD_F_COUNT
+    MHD_F_ALPHA_XI, MHD_F_ALPHA_ETA, MHD_F_ALPHA_ZETA, MHD_F_BCELL_W, MHD_F_JCELL_W,
+    /* two-fluid (Mercury) cases: Z5_Mercury/main.cpp:54-55 and the Na+
+       source fields of Z5_Mercury/include/2_Initial.h:162-392; drag_rate is
+       the ion-neutral drag frequency of each cell (a field of this build) */
+    MHD_F_U_NA, MHD_F_PV_NA, MHD_F_RHS_NA, MHD_F_PHOTO_RATE, MHD_F_DRAG_RATE, MHD_F_COUNT
 };
 
 typedef struct
@@ -84,6 +88,11 @@ typedef struct
 typedef struct
 {
     double B_eq_nT, offset_z, softening;
+    int single_fluid; /* 1: the shared single-fluid step with the dipole
+                         background; 0: H+ and Na+ (MercurySolver's step) */
+    int no_ambipolar; /* two-fluid only: 1 turns the electron-pressure force
+                         and ambipolar EMF off (the reference's
+                         is_Ambipolar_Efield 0) */
 } mhd_dipole_desc;
 
 typedef struct mhd_host_case mhd_host_case;
@@ -221,6 +230,10 @@ int mhd_host_upload(mhd_host_case *h, int block, int field, const double *aos, s
 int mhd_host_faces(mhd_host_case *h, int block, int *faces14x6);
 int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n);
 int mhd_host_constants(mhd_host_case *h, double *out22);
+/* two-fluid constants: model, ambipolar, M_Na, state_coeff_Na, qinfs[5],
+   momentum_induce_coeff, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef (17 doubles;
+   Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46) */
+int mhd_host_constants_two_fluid(mhd_host_case *h, double *out17);
 /* apply one halo / boundary op list (index as in the library's sync table:
    0 Eface BC, 1 Eface halo, 2 Eedge BC, 3-5 Ucell BC0/BC1/halo, 6-7 Bface
    BC/halo, 8 J_cell BC, 9 corner velocities) to the host arrays */

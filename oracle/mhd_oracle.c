@@ -89,6 +89,7 @@ static const struct
     {L_CELL, 5, 1}, {L_FXI, 1, 1}, {L_FET, 1, 1}, {L_FZE, 1, 1}, {L_CELL, 1, 0}, {L_CELL, 1, 0},
     {L_FXI, 3, 0}, {L_FET, 3, 0}, {L_FZE, 3, 0}, {L_FXI, 1, 0}, {L_FET, 1, 0}, {L_FZE, 1, 0},
     {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0}, {L_CELL, 18, 0}, {L_CELL, 36, 0},
+    {L_CELL, 5, 0}, {L_CELL, 5, 0}, {L_CELL, 5, 1}, {L_CELL, 1, 0}, {L_CELL, 1, 0},
 };
 /* g0 = 1 marks fields the reference registers with zero ghosts
    (Z4_Lunar/main.cpp:107-121); their interior range is what matters. */
@@ -392,6 +393,43 @@ static void bc_u_farfield(const or_ctx *c, fblk *F, int face, int ngh)
                 }
 }
 
+/* MercuryBoundary::BC_UH_Farfield_Na_ (Z5_Mercury/src/boundary/Boundary_Condition_Fluid.cpp:19-34) */
+static void bc_una_farfield(const or_ctx *c, fblk *F, int face, int ngh)
+{
+    slab s = inner_slab(F, face);
+    for (int i = s.lo[0]; i < s.hi[0]; ++i)
+        for (int j = s.lo[1]; j < s.hi[1]; ++j)
+            for (int k = s.lo[2]; k < s.hi[2]; ++k)
+                for (int l = 1; l <= ngh; ++l)
+                {
+                    int q[3] = {i, j, k};
+                    q[s.ax] += s.sgn * l;
+                    for (int m = 0; m < 5; ++m) AT(F, q[0], q[1], q[2], m) = c->d.qinfs[m];
+                }
+}
+
+/* MercuryBoundary::BC_Solid_Surface_ for Na+ (Z5_Mercury/src/boundary/
+ * Boundary_Condition_Fluid.cpp:36-165, the is_Na branch): absorbing wall, the
+ * first layer is left alone and the ghosts hold the floor state. */
+static void bc_una_solid(const or_ctx *c, fblk *U, int face, int ngh)
+{
+    const double rho_floor = 1.0e-12, p_floor = 1.0e-12;
+    slab s = inner_slab(U, face);
+    for (int i = s.lo[0]; i < s.hi[0]; ++i)
+        for (int j = s.lo[1]; j < s.hi[1]; ++j)
+            for (int k = s.lo[2]; k < s.hi[2]; ++k)
+                for (int l = 1; l <= ngh; ++l)
+                {
+                    int q[3] = {i, j, k};
+                    q[s.ax] += s.sgn * l;
+                    AT(U, q[0], q[1], q[2], 0) = rho_floor;
+                    AT(U, q[0], q[1], q[2], 1) = 0.0;
+                    AT(U, q[0], q[1], q[2], 2) = 0.0;
+                    AT(U, q[0], q[1], q[2], 3) = 0.0;
+                    AT(U, q[0], q[1], q[2], 4) = p_floor / (c->d.gamma - 1.0);
+                }
+}
+
 /* LunarBoundary::BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184):
  * absorbing wall -- first interior layer takes rho and p of the second
  * layer with zero velocity, ghosts copy that layer (again with v = 0). */
@@ -556,6 +594,10 @@ static void physical_phase(or_ctx *c, int fid)
                 if (bc == 1) bc_u_farfield(c, F, f, g);
                 else if (bc == 2) bc_u_solid(c, F, f, g);
                 break;
+            case OF_UNA:
+                if (bc == 1) bc_una_farfield(c, F, f, g);
+                else if (bc == 2) bc_una_solid(c, F, f, g);
+                break;
             case OF_BXI: case OF_BET: case OF_BZE:
                 if (bc == 2) bc_copy(F, f, g); /* far field: nop */
                 break;
@@ -614,11 +656,12 @@ static void sync_fields(or_ctx *c, const int *fids, int n, int do_physical)
 
 int or_sync(or_ctx *c, const char *group)
 {
-    static const int fU[] = {OF_UH}, fB[] = {OF_BXI, OF_BET, OF_BZE}, fBadd[] = {OF_BADDXI, OF_BADDET, OF_BADDZE};
+    static const int fU[] = {OF_UH, OF_UNA}, fB[] = {OF_BXI, OF_BET, OF_BZE}, fBadd[] = {OF_BADDXI, OF_BADDET, OF_BADDZE};
     static const int fEf[] = {OF_EFXI, OF_EFET, OF_EFZE}, fE[] = {OF_EXI, OF_EET, OF_EZE};
     static const int fBc[] = {OF_BCELL, OF_BIND}, fJ[] = {OF_JXI, OF_JET, OF_JZE}, fJc[] = {OF_JCELL};
     static const int fUp[] = {OF_UPLUS};
-    if (!strcmp(group, "Ucell")) sync_fields(c, fU, 1, 1);
+    /* the Mercury Ucell group holds U_H and U_Na (Z5_Mercury/src/boundary/Boundary_Build.cpp:571-578) */
+    if (!strcmp(group, "Ucell")) sync_fields(c, fU, c->d.model == 1 ? 2 : 1, 1);
     else if (!strcmp(group, "Bface")) sync_fields(c, fB, 3, 1);
     else if (!strcmp(group, "Badd")) sync_fields(c, fBadd, 3, 1);
     else if (!strcmp(group, "Eface")) sync_fields(c, fEf, 3, 1);
@@ -632,13 +675,14 @@ int or_sync(or_ctx *c, const char *group)
 }
 
 /* ------------------------------------------------------- derived fields */
-/* calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110): full extent incl. ghosts */
-static void calc_pv(or_ctx *c)
+/* calc_PV (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:58-110; per species in
+ * Z5_Mercury/src/physics/MercurySolver_Tools.cpp:138-192): full extent incl. ghosts */
+static void calc_pv_one(or_ctx *c, int fu, int fpv, double coeff)
 {
-    const double gm = c->d.gamma, coeff = c->d.state_coeff;
+    const double gm = c->d.gamma;
     for (int b = 0; b < c->d.nblocks; ++b)
     {
-        fblk *U = &c->b[b].f[OF_UH], *PV = &c->b[b].f[OF_PV];
+        fblk *U = &c->b[b].f[fu], *PV = &c->b[b].f[fpv];
         const int g = U->g;
         OMP_FOR
         for (int i = -g; i < U->cnt[0] + g; ++i)
@@ -665,10 +709,55 @@ static void calc_pv(or_ctx *c)
     }
 }
 
-/* calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141) */
+static void calc_pv(or_ctx *c)
+{
+    calc_pv_one(c, OF_UH, OF_PV, c->d.state_coeff);
+    if (c->d.model == 1) calc_pv_one(c, OF_UNA, OF_PVNA, c->d.state_coeff_Na);
+}
+
+/* Hall_Num_Limiter (Z5_Mercury/src/physics/MercurySolver_Tools.cpp:90-136) */
+typedef struct { double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa; } numinfo;
+static numinfo num_limiter(const or_ctx *c, double rhoH, double rhoNa)
+{
+    numinfo n;
+    const double rhoH_pos = dmax(rhoH, 0.0), rhoNa_pos = dmax(rhoNa, 0.0);
+    n.nH = rhoH_pos / c->d.M_H * c->d.M_ref;
+    n.nNa = rhoNa_pos / c->d.M_Na * c->d.M_ref;
+    n.ne = n.nH + n.nNa;
+    n.ne_eff = sqrt(n.ne * n.ne + c->d.ne_floor * c->d.ne_floor);
+    if (n.ne > 1.0e-300) { n.chiH = n.nH / n.ne; n.chiNa = n.nNa / n.ne; }
+    else { n.chiH = 1.0; n.chiNa = 0.0; }
+    n.wH = n.nH / n.ne_eff;
+    n.wNa = n.nNa / n.ne_eff;
+    return n;
+}
+
+/* calc_Uplus (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:112-141); two-fluid:
+ * composition-weighted ion velocity (Z5_Mercury/src/physics/MercurySolver_Tools.cpp:194-277) */
 static void calc_uplus(or_ctx *c)
 {
-    for (int b = 0; b < c->d.nblocks; ++b)
+    for (int b = 0; b < c->d.nblocks && c->d.model == 1; ++b)
+    {
+        fblk *UH = &c->b[b].f[OF_UH], *UN = &c->b[b].f[OF_UNA], *PH = &c->b[b].f[OF_PV], *PN = &c->b[b].f[OF_PVNA];
+        fblk *Up = &c->b[b].f[OF_UPLUS];
+        const int g = UH->g;
+        OMP_FOR
+        for (int i = -g; i < UH->cnt[0] + g; ++i)
+            for (int j = -g; j < UH->cnt[1] + g; ++j)
+                for (int k = -g; k < UH->cnt[2] + g; ++k)
+                {
+                    const double rhoH0 = dmax(AT(UH, i, j, k, 0), 0.0), rhoNa0 = dmax(AT(UN, i, j, k, 0), 0.0);
+                    double uh[3], un[3];
+                    for (int m = 0; m < 3; ++m)
+                    {
+                        uh[m] = rhoH0 > 1e-20 ? AT(PH, i, j, k, m) : 0.0;
+                        un[m] = rhoNa0 > 1e-20 ? AT(PN, i, j, k, m) : 0.0;
+                    }
+                    const numinfo n = num_limiter(c, rhoH0, rhoNa0);
+                    for (int m = 0; m < 3; ++m) AT(Up, i, j, k, m) = n.chiH * uh[m] + n.chiNa * un[m];
+                }
+    }
+    for (int b = 0; b < c->d.nblocks && c->d.model == 0; ++b)
     {
         fblk *U = &c->b[b].f[OF_UH], *PV = &c->b[b].f[OF_PV], *Up = &c->b[b].f[OF_UPLUS];
         const int g = U->g;
@@ -816,14 +905,18 @@ int or_update_derived(or_ctx *c)
 }
 
 /* ------------------------------------------------------------ time step */
+/* Compute_Timestep scans U_H, and for the two-fluid step also U_Na with the
+ * same B_cell (Z5_Mercury/src/physics/MercurySolver_Timestep.cpp:8-140) */
 int or_compute_dt(or_ctx *c, double *dt_out)
 {
     const double gm = c->d.gamma, iMA2 = c->d.inver_MA2, CFL = c->d.cfl;
     double dt_local = 1e100;
+    const int nsp = c->d.model == 1 ? 2 : 1;
+    for (int sp = 0; sp < nsp; ++sp)
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         oblock *B = &c->b[b];
-        fblk *U = &B->f[OF_UH], *Bc = &B->f[OF_BCELL], *Jac = &B->f[OF_JAC];
+        fblk *U = &B->f[sp ? OF_UNA : OF_UH], *Bc = &B->f[OF_BCELL], *Jac = &B->f[OF_JAC];
         fblk *Ax = &B->f[OF_JDXI], *Ay = &B->f[OF_JDET], *Az = &B->f[OF_JDZE];
         /* per-thread minima combined with the same min: the minimum of finite
            values does not depend on the order they are taken in */
@@ -930,13 +1023,15 @@ static void cell_w(const fblk *U, const fblk *PV, int i, int j, int k, double *W
     W[1] = AT(PV, i, j, k, 0); W[2] = AT(PV, i, j, k, 1); W[3] = AT(PV, i, j, k, 2); W[4] = AT(PV, i, j, k, 3);
 }
 
-static void scheme_u(or_ctx *c)
+/* Scheme_U_; the two-fluid step runs it for H+ then Na+
+ * (Z5_Mercury/src/physics/MercurySolver_FluidRHS.cpp:115-202) */
+static void scheme_u_one(or_ctx *c, int fu, int fpv, int frhs)
 {
     const double gm = c->d.gamma;
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         oblock *B = &c->b[b];
-        fblk *U = &B->f[OF_UH], *PV = &B->f[OF_PV], *RHS = &B->f[OF_RHSH], *Jac = &B->f[OF_JAC];
+        fblk *U = &B->f[fu], *PV = &B->f[fpv], *RHS = &B->f[frhs], *Jac = &B->f[OF_JAC];
         fblk *F[3] = {&B->f[OF_FXI], &B->f[OF_FET], &B->f[OF_FZE]};
         fblk *M[3] = {&B->f[OF_JDXI], &B->f[OF_JDET], &B->f[OF_JDZE]};
         for (int d = 0; d < 3; ++d)
@@ -986,10 +1081,22 @@ static void scheme_u(or_ctx *c)
     }
 }
 
+static void scheme_u(or_ctx *c)
+{
+    scheme_u_one(c, OF_UH, OF_PV, OF_RHSH);
+    if (c->d.model == 1) scheme_u_one(c, OF_UNA, OF_PVNA, OF_RHSNA);
+}
+
 /* AddSourceToRHS_Fluid (Z4_Lunar/src/physics/LunarSolver_FluidRHS.cpp:528-568)
  * with Hall_Num_Limiter (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:47-56) */
+static void add_source_mercury(or_ctx *c);
 static void add_source(or_ctx *c)
 {
+    if (c->d.model == 1)
+    {
+        add_source_mercury(c);
+        return;
+    }
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         oblock *B = &c->b[b];
@@ -1012,6 +1119,161 @@ static void add_source(or_ctx *c)
                     AT(RHS, i, j, k, 3) += fc * jz;
                     AT(RHS, i, j, k, 4) += fc * (jx * AT(PV, i, j, k, 0) + jy * AT(PV, i, j, k, 1) + jz * AT(PV, i, j, k, 2));
                 }
+    }
+}
+
+/* AddSourceToRHS_Fluid of the two-fluid step (Z5_Mercury/src/physics/
+ * MercurySolver_FluidRHS.cpp:206-527): per species the induction force
+ * n_s (u_s - u_+) x B, the composition-weighted J x B, the ambipolar
+ * electron-pressure force, photoionization (Na+ only), drag on the neutrals
+ * and thermal exchange.  The metric of the pressure gradient averages faces
+ * i-1 and i (j-1 and j, k-1 and k) as the reference does. */
+static void add_source_mercury(or_ctx *c)
+{
+    const double sk1 = 5.0e-5, sk2 = 1.0e-5; /* drag frequency, illuminated / shadow [1/s] (:211-212) */
+    (void)sk1; (void)sk2;
+    const double induce = c->d.induce, hall = c->d.momentum_hall_coeff;
+    const double a1_Na = c->d.a1_Na, a4 = c->d.a4, a5 = c->d.a5, a6 = c->d.a6, Tn0 = c->d.Tn0;
+    const double M_H = c->d.M_H, M_Na = c->d.M_Na, T_ref = c->d.T_ref;
+    for (int b = 0; b < c->d.nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        fblk *UH = &B->f[OF_UH], *UN = &B->f[OF_UNA], *PH = &B->f[OF_PV], *PN = &B->f[OF_PVNA];
+        fblk *Up = &B->f[OF_UPLUS], *Bt = &B->f[OF_BCELL], *Jc = &B->f[OF_JCELL];
+        fblk *Ph = &B->f[OF_PHOTO], *Dr = &B->f[OF_DRAG], *RH = &B->f[OF_RHSH], *RN = &B->f[OF_RHSNA];
+        fblk *Jac = &B->f[OF_JAC], *Ax = &B->f[OF_JDXI], *Ay = &B->f[OF_JDET], *Az = &B->f[OF_JDZE];
+        OMP_FOR
+        for (int i = 0; i < B->n[0]; ++i)
+            for (int j = 0; j < B->n[1]; ++j)
+                for (int k = 0; k < B->n[2]; ++k)
+                {
+                    const double upx = AT(Up, i, j, k, 0), upy = AT(Up, i, j, k, 1), upz = AT(Up, i, j, k, 2);
+                    const double Bx = AT(Bt, i, j, k, 0), By = AT(Bt, i, j, k, 1), Bz = AT(Bt, i, j, k, 2);
+                    const double Jx = AT(Jc, i, j, k, 0), Jy = AT(Jc, i, j, k, 1), Jz = AT(Jc, i, j, k, 2);
+                    const double sjbx = Jy * Bz - Jz * By;
+                    const double sjby = Jz * Bx - Jx * Bz;
+                    const double sjbz = Jx * By - Jy * Bx;
+                    const numinfo num = num_limiter(c, AT(UH, i, j, k, 0), AT(UN, i, j, k, 0));
+                    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
+                    if (c->d.ambipolar)
+                    {
+                        /* derv_at (:238-261) and dd (:228-235) */
+                        double ax, ay, az, bx, by, bz, cx, cy, cz;
+                        const double V = fabs(AT(Jac, i, j, k, 0));
+                        if (V <= 0.0) ax = ay = az = bx = by = bz = cx = cy = cz = 0.0;
+                        else
+                        {
+                            ax = 0.5 * (AT(Ax, i - 1, j, k, 0) + AT(Ax, i, j, k, 0)) / V;
+                            ay = 0.5 * (AT(Ax, i - 1, j, k, 1) + AT(Ax, i, j, k, 1)) / V;
+                            az = 0.5 * (AT(Ax, i - 1, j, k, 2) + AT(Ax, i, j, k, 2)) / V;
+                            bx = 0.5 * (AT(Ay, i, j - 1, k, 0) + AT(Ay, i, j, k, 0)) / V;
+                            by = 0.5 * (AT(Ay, i, j - 1, k, 1) + AT(Ay, i, j, k, 1)) / V;
+                            bz = 0.5 * (AT(Ay, i, j - 1, k, 2) + AT(Ay, i, j, k, 2)) / V;
+                            cx = 0.5 * (AT(Az, i, j, k - 1, 0) + AT(Az, i, j, k, 0)) / V;
+                            cy = 0.5 * (AT(Az, i, j, k - 1, 1) + AT(Az, i, j, k, 1)) / V;
+                            cz = 0.5 * (AT(Az, i, j, k - 1, 2) + AT(Az, i, j, k, 2)) / V;
+                        }
+                        const double pip = AT(PH, i + 1, j, k, 3) + AT(PN, i + 1, j, k, 3);
+                        const double pim = AT(PH, i - 1, j, k, 3) + AT(PN, i - 1, j, k, 3);
+                        const double pjp = AT(PH, i, j + 1, k, 3) + AT(PN, i, j + 1, k, 3);
+                        const double pjm = AT(PH, i, j - 1, k, 3) + AT(PN, i, j - 1, k, 3);
+                        const double pkp = AT(PH, i, j, k + 1, 3) + AT(PN, i, j, k + 1, 3);
+                        const double pkm = AT(PH, i, j, k - 1, 3) + AT(PN, i, j, k - 1, 3);
+                        dpex = 0.5 * (ax * (pip - pim) + bx * (pjp - pjm) + cx * (pkp - pkm));
+                        dpey = 0.5 * (ay * (pip - pim) + by * (pjp - pjm) + cy * (pkp - pkm));
+                        dpez = 0.5 * (az * (pip - pim) + bz * (pjp - pjm) + cz * (pkp - pkm));
+                    }
+                    const double sss = AT(Ph, i, j, k, 0);
+                    const double vst = AT(Dr, i, j, k, 0);
+                    for (int sp = 0; sp < 2; ++sp)
+                    {
+                        fblk *U = sp ? UN : UH, *PV = sp ? PN : PH, *R = sp ? RN : RH;
+                        const double n_s = sp ? num.nNa : num.nH;
+                        const double chi = sp ? num.wNa : num.wH;
+                        const double rho = dmax(AT(U, i, j, k, 0), 0.0);
+                        const double u = AT(PV, i, j, k, 0), v = AT(PV, i, j, k, 1), w = AT(PV, i, j, k, 2);
+                        const double u2 = u * u + v * v + w * w;
+                        const double subx = (v - upy) * Bz - (w - upz) * By;
+                        const double suby = (w - upz) * Bx - (u - upx) * Bz;
+                        const double subz = (u - upx) * By - (v - upy) * Bx;
+                        const double subu = subx * u + suby * v + subz * w;
+                        const double sjbu = sjbx * u + sjby * v + sjbz * w;
+                        const double dpeu = dpex * u + dpey * v + dpez * w;
+                        const double Ts0 = AT(PV, i, j, k, 4) * T_ref;
+                        const double b1 = sp ? vst * (Tn0 - Ts0) * (M_Na / (M_Na + M_Na)) : vst * (Tn0 - Ts0) * (M_H / (M_H + M_Na));
+                        const double b2 = sp ? (M_Na / (M_Na + M_Na)) * vst : (M_Na / (M_H + M_Na)) * vst;
+                        AT(R, i, j, k, 1) += induce * n_s * subx + hall * chi * sjbx;
+                        AT(R, i, j, k, 2) += induce * n_s * suby + hall * chi * sjby;
+                        AT(R, i, j, k, 3) += induce * n_s * subz + hall * chi * sjbz;
+                        AT(R, i, j, k, 4) += induce * n_s * subu + hall * chi * sjbu;
+                        if (c->d.ambipolar)
+                        {
+                            AT(R, i, j, k, 1) -= chi * dpex;
+                            AT(R, i, j, k, 2) -= chi * dpey;
+                            AT(R, i, j, k, 3) -= chi * dpez;
+                            AT(R, i, j, k, 4) -= chi * dpeu;
+                        }
+                        if (sp)
+                        {
+                            AT(R, i, j, k, 0) += a1_Na * sss;
+                            AT(R, i, j, k, 4) += a6 * sss * Tn0;
+                        }
+                        AT(R, i, j, k, 1) -= a4 * rho * u * vst;
+                        AT(R, i, j, k, 2) -= a4 * rho * v * vst;
+                        AT(R, i, j, k, 3) -= a4 * rho * w * vst;
+                        AT(R, i, j, k, 4) -= a4 * rho * u2 * vst;
+                        AT(R, i, j, k, 4) += a5 * n_s * b1;
+                        AT(R, i, j, k, 4) += a4 * rho * u2 * b2;
+                    }
+                }
+    }
+}
+
+/* AddAmbipolarEdgeEMF_ (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Ambipo.cpp:3-144):
+ * E_edge -= pe_coef (pe(node 1) - pe(node 0)) / ne_eff(edge), pe = p_H + p_Na
+ * averaged over the 8 cells around a node, ne from the 4 cells around the edge. */
+static double pe_node(const fblk *PH, const fblk *PN, int i, int j, int k)
+{
+    double sum = 0.0;
+    for (int di = -1; di <= 0; ++di)
+        for (int dj = -1; dj <= 0; ++dj)
+            for (int dk = -1; dk <= 0; ++dk)
+                sum += AT(PH, i + di, j + dj, k + dk, 3) + AT(PN, i + di, j + dj, k + dk, 3);
+    return 0.125 * sum;
+}
+
+static void add_ambipolar_emf(or_ctx *c)
+{
+    for (int b = 0; b < c->d.nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        fblk *UH = &B->f[OF_UH], *UN = &B->f[OF_UNA], *PH = &B->f[OF_PV], *PN = &B->f[OF_PVNA];
+        for (int a = 0; a < 3; ++a)
+        {
+            fblk *E = &B->f[OF_EXI + a];
+            const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2; /* the two transverse axes, ascending */
+            OMP_FOR
+            for (int i = 0; i < E->cnt[0]; ++i)
+                for (int j = 0; j < E->cnt[1]; ++j)
+                    for (int k = 0; k < E->cnt[2]; ++k)
+                    {
+                        const int d1[3] = {a == 0, a == 1, a == 2};
+                        const double pe0 = pe_node(PH, PN, i, j, k);
+                        const double pe1 = pe_node(PH, PN, i + d1[0], j + d1[1], k + d1[2]);
+                        double rhoH = 0.0, rhoNa = 0.0;
+                        for (int s1 = -1; s1 <= 0; ++s1)
+                            for (int s2 = -1; s2 <= 0; ++s2)
+                            {
+                                int q[3] = {i, j, k};
+                                q[t1] += s1;
+                                q[t2] += s2;
+                                rhoH += AT(UH, q[0], q[1], q[2], 0);
+                                rhoNa += AT(UN, q[0], q[1], q[2], 0);
+                            }
+                        const numinfo num = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
+                        AT(E, i, j, k, 0) -= c->d.pe_coef * (pe1 - pe0) / num.ne_eff;
+                    }
+        }
     }
 }
 
@@ -1255,6 +1517,7 @@ static void assemble_ct(or_ctx *c)
     face_emf(c);
     or_sync(c, "Eface");
     edge_emf(c);
+    if (c->d.model == 1 && c->d.ambipolar) add_ambipolar_emf(c);
     or_sync(c, "Eedge");
     curl_edge_to_face(c, -1.0);
 }
@@ -1265,12 +1528,15 @@ static void apply_update(or_ctx *c, double dt)
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         oblock *B = &c->b[b];
-        fblk *U = &B->f[OF_UH], *R = &B->f[OF_RHSH];
-        OMP_FOR
-        for (int i = 0; i < B->n[0]; ++i)
-            for (int j = 0; j < B->n[1]; ++j)
-                for (int k = 0; k < B->n[2]; ++k)
-                    for (int m = 0; m < 5; ++m) AT(U, i, j, k, m) += dt * AT(R, i, j, k, m);
+        for (int sp = 0; sp < (c->d.model == 1 ? 2 : 1); ++sp)
+        {
+            fblk *U = &B->f[sp ? OF_UNA : OF_UH], *R = &B->f[sp ? OF_RHSNA : OF_RHSH];
+            OMP_FOR
+            for (int i = 0; i < B->n[0]; ++i)
+                for (int j = 0; j < B->n[1]; ++j)
+                    for (int k = 0; k < B->n[2]; ++k)
+                        for (int m = 0; m < 5; ++m) AT(U, i, j, k, m) += dt * AT(R, i, j, k, m);
+        }
         for (int a = 0; a < 3; ++a)
         {
             fblk *Bf = &B->f[OF_BXI + a], *RB = &B->f[OF_RHSBXI + a];
@@ -1293,6 +1559,7 @@ int or_step(or_ctx *c, double *dt_used)
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         zero_fblk(&c->b[b].f[OF_RHSH]);
+        zero_fblk(&c->b[b].f[OF_RHSNA]);
         for (int a = 0; a < 3; ++a) zero_fblk(&c->b[b].f[OF_RHSBXI + a]);
     }
     scheme_u(c);

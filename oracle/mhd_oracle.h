@@ -32,6 +32,13 @@ typedef struct
     double gamma, cfl, inver_MA2, momentum_hall_coeff, M_H, M_ref, ne_floor, state_coeff;
     double qinf[5], B_imf[3], u_inf[3];
     double dt_floor;
+    /* two-fluid H+ / Na+ step of MercurySolver (model 1); constants of
+       Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46 and
+       MercurySolver_FluidRHS.cpp:210-225 */
+    int model;     /* 0 single fluid (Lunar), 1 two-fluid (Mercury) */
+    int ambipolar; /* model 1: electron-pressure terms in the momentum source and the edge EMF */
+    double M_Na, state_coeff_Na, qinfs[5];
+    double induce, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef;
 } or_desc;
 
 enum
@@ -40,7 +47,8 @@ enum
     OF_BCELL, OF_BIND, OF_JXI, OF_JET, OF_JZE, OF_JCELL, OF_EXI, OF_EET, OF_EZE,
     OF_EFXI, OF_EFET, OF_EFZE, OF_FXI, OF_FET, OF_FZE, OF_RHSH, OF_RHSBXI, OF_RHSBET, OF_RHSBZE,
     OF_DIVB, OF_JAC, OF_JDXI, OF_JDET, OF_JDZE, OF_BETAXI, OF_BETAET, OF_BETAZE,
-    OF_ALXI, OF_ALET, OF_ALZE, OF_WB, OF_WJ, OF_COUNT
+    OF_ALXI, OF_ALET, OF_ALZE, OF_WB, OF_WJ,
+    OF_UNA, OF_PVNA, OF_RHSNA, OF_PHOTO, OF_DRAG, OF_COUNT
 };
 
 typedef struct or_ctx or_ctx;

@@ -24,7 +24,11 @@ class OrDesc(C.Structure):
                 ("gamma", C.c_double), ("cfl", C.c_double), ("inver_MA2", C.c_double),
                 ("momentum_hall_coeff", C.c_double), ("M_H", C.c_double), ("M_ref", C.c_double),
                 ("ne_floor", C.c_double), ("state_coeff", C.c_double), ("qinf", C.c_double * 5),
-                ("B_imf", C.c_double * 3), ("u_inf", C.c_double * 3), ("dt_floor", C.c_double)]
+                ("B_imf", C.c_double * 3), ("u_inf", C.c_double * 3), ("dt_floor", C.c_double),
+                ("model", C.c_int), ("ambipolar", C.c_int), ("M_Na", C.c_double), ("state_coeff_Na", C.c_double),
+                ("qinfs", C.c_double * 5), ("induce", C.c_double), ("a1_Na", C.c_double), ("a4", C.c_double),
+                ("a5", C.c_double), ("a6", C.c_double), ("T_ref", C.c_double), ("Tn0", C.c_double),
+                ("pe_coef", C.c_double)]
 
 
 _lib = None
@@ -60,6 +64,7 @@ def olib():
 GEOMETRY = ["Jac", "JDxi", "JDet", "JDze", "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta",
             "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
 STATE = ["U_H", "B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta"]
+STATE_TWO_FLUID = ["U_Na", "Photo_rate", "drag_rate"]
 
 
 class Oracle:
@@ -77,6 +82,13 @@ class Oracle:
             d.B_imf[a] = k[13 + a]
             d.u_inf[a] = k[16 + a]
         d.dt_floor = k[19]
+        t = hc.constants_two_fluid()
+        two = t[0] == 1.0
+        d.model, d.ambipolar = int(t[0]), int(t[1])
+        d.M_Na, d.state_coeff_Na = t[2], t[3]
+        for m in range(5):
+            d.qinfs[m] = t[4 + m]
+        d.induce, d.a1_Na, d.a4, d.a5, d.a6, d.T_ref, d.Tn0, d.pe_coef = t[9:17]
         L = olib()
         self.c = L.or_create(C.byref(d))
         self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks
@@ -89,7 +101,7 @@ class Oracle:
                 for a in range(3):
                     faces[f].perm[a], faces[f].sign[a], faces[f].offset[a] = int(r[5 + a]), int(r[8 + a]), int(r[11 + a])
             L.or_set_block(self.c, b, hc.n[0], hc.n[1], hc.n[2], faces)
-            for name in GEOMETRY + STATE:
+            for name in GEOMETRY + STATE + (STATE_TWO_FLUID if two else []):
                 self.view(name, b)[...] = hc.field(name, b)
             for comp, arr in enumerate(hc.nodes(b)):
                 cnt = C.c_long()
@@ -108,7 +120,7 @@ class Oracle:
         """Writable numpy view of the oracle's AoS field block."""
         cnt = C.c_long()
         p = olib().or_field(self.c, b, FIELD_ID[name], C.byref(cnt))
-        g0 = 0 if name in ("F_xi", "F_eta", "F_zeta", "RHS_H", "RHS_B_xi", "RHS_B_eta", "RHS_B_zeta") else self.g
+        g0 = 0 if name in ("F_xi", "F_eta", "F_zeta", "RHS_H", "RHS_Na", "RHS_B_xi", "RHS_B_eta", "RHS_B_zeta") else self.g
         return np.ctypeslib.as_array(p, shape=(cnt.value,)).reshape(shape(name, self.n, g0))
 
     def step(self, n=1):
