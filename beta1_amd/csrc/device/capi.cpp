@@ -117,7 +117,7 @@ enum KernelId
 };
 const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
-                                     "k_fluid2",  "k_derived2",    "k_ambipolar_emf"};
+                                     "k_fluid2",  "k_derived2",    "k_emf_extra"};
 const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE,
                                     R_EXT,  R_CELL, R_CELL, R_CELL, R_EXT, R_FACE};
 } // namespace
@@ -666,8 +666,17 @@ static void derived(mhd_ctx *c)
         ProfScope p(c, KP_DERIVED2);
         mhdk::launch_derived2(launch_of(c, KP_DERIVED2), c->d_cst, c->d_dtbits, c->ucur, mixed_of(c), c->q);
     }
-    ProfScope p(c, KP_JCELL);
-    mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), c->q);
+    {
+        ProfScope p(c, KP_JCELL);
+        mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), c->q);
+    }
+    // B_cell and J_cell ghosts of the two-fluid cases with the Hall EMF
+    // (the reference's B_cell and J_cell syncs; empty for the other cases)
+    ProfScope p(c, KP_OPS);
+    run_ops(c, SY_BCELL_BC, 0);
+    run_ops(c, SY_BCELL_HALO, 0);
+    run_ops(c, SY_JCELL_GH, 0);
+    run_ops(c, SY_JCELL_HALO, 0);
 }
 
 // One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
@@ -702,7 +711,7 @@ static void enqueue_step(mhd_ctx *c)
         // two-fluid EMF terms added before the Eedge sync
         // (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:30-33)
         ProfScope p(c, KP_AMBI);
-        mhdk::launch_ambipolar_emf(launch_of(c, KP_AMBI), c->d_cst, c->ucur, c->q);
+        mhdk::launch_emf_extra(launch_of(c, KP_AMBI), c->d_cst, c->ucur, mixed, c->q);
     }
     {
         ProfScope p(c, KP_OPS);
@@ -783,7 +792,7 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_AMBI:
                     // E is step scratch: the next step's edge EMF rewrites it
-                    mhdk::launch_ambipolar_emf(L, c->d_cst, c->ucur, c->q);
+                    mhdk::launch_emf_extra(L, c->d_cst, c->ucur, mixed, c->q);
                     break;
                 default:
                     mhdk::launch_derived(L, c->d_cst, c->d_dtbits, c->ucur, mixed, c->q);

@@ -34,7 +34,7 @@ void launch_derived2(const Launch &L, const MhdCaseConst *cst, unsigned long lon
                      hipStream_t q);
 void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                    hipStream_t q);
-void launch_ambipolar_emf(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q);
+void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q);
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);

@@ -1041,10 +1041,18 @@ __global__ __launch_bounds__(256) void k_fluid2(const MhdTileMap tm, const MhdCa
     }
 }
 
-// AddAmbipolarEdgeEMF_ (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Ambipo.cpp:3-144):
-// E_edge -= pe_coef (pe(node 1) - pe(node 0)) / ne_eff, pe averaged over the
-// 8 cells around a node, ne_eff of the mean density of the 4 cells around
-// the edge.  Runs after the ideal edge EMF and before the Eedge sync.
+// Two-fluid edge EMF terms added to the ideal EMF before the Eedge sync, in
+// the reference's order (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:30-56):
+//  * AddAmbipolarEdgeEMF_ (MercurySolver_MagExplicit_Ambipo.cpp:3-144):
+//    E -= pe_coef (pe(node 1) - pe(node 0)) / ne_eff, pe averaged over the 8
+//    cells around a node, ne_eff of the mean density of the 4 cells around
+//    the edge;
+//  * AddLocalArtificialResistivityToEdgeEMF_ (MercurySolver_MagExplicit_Resis.cpp:67-186):
+//    E += eta(edge) J_edge, J_edge = alpha CurlAdj(beta B) as in J_cell;
+//  * AddHallEdgeEMF_ (MercurySolver_MagExplicit_Hall.cpp:39-440): E +=
+//    taper (<alpha J x B>_4 cells . dr + whistler LLF diffusion of the
+//    transverse face-flux jumps), alpha = hall_coef / ne_eff.  The Ehall sync
+//    of the reference only fills ghosts, which E never reads.
 __device__ __forceinline__ double pe_node(const double *__restrict__ PE, int64_t o, int64_t di, int64_t dj, int64_t dk)
 {
     double sum = 0.0;
@@ -1058,8 +1066,9 @@ __device__ __forceinline__ double pe_node(const double *__restrict__ PE, int64_t
     return 0.125 * sum;
 }
 
-__global__ __launch_bounds__(256) void k_ambipolar_emf(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
-                                                       int ucur)
+template <typename WT>
+__global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                   int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
@@ -1067,16 +1076,22 @@ __global__ __launch_bounds__(256) void k_ambipolar_emf(const MhdTileMap tm, cons
     if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const MhdCaseConst &c = cst[b.case_id];
-    if (c.ambipolar == 0.0)
+    const bool amb = c.ambipolar != 0.0, loc = c.local_arti != 0.0, hall = c.hall != 0.0;
+    if (!amb && !loc && !hall)
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
     const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
     double *S = dst_ptr(b);
     const double *PE = S + (int64_t)ST_PE * sl;
     const double *RH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *RN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+    const double *Bf = S + (int64_t)ST_B * sl;
+    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+    const double *G = dmetric(b);
+    const WT *Wt = dweights<WT>(b);
+    const WT *beta = Wt + (int64_t)(G_BETA - G_NMETRIC) * sl, *alph = Wt + (int64_t)(G_ALPHA - G_NMETRIC) * sl;
     double *E = S + (int64_t)ST_E * sl;
     const int idx[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
-    const double pe0 = pe_node(PE, o, st[0], st[1], st[2]);
+    const double pe0 = amb ? pe_node(PE, o, st[0], st[1], st[2]) : 0.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
@@ -1087,22 +1102,97 @@ __global__ __launch_bounds__(256) void k_ambipolar_emf(const MhdTileMap tm, cons
         if (!in)
             continue;
         const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-        const double pe1 = pe_node(PE, o + st[a], st[0], st[1], st[2]);
-        double rhoH = 0.0, rhoNa = 0.0;
+        int64_t oc[4];
+        {
+            int q = 0;
 #pragma unroll
-        for (int s1 = -1; s1 <= 0; ++s1)
+            for (int s1 = -1; s1 <= 0; ++s1)
 #pragma unroll
-            for (int s2 = -1; s2 <= 0; ++s2)
+                for (int s2 = -1; s2 <= 0; ++s2)
+                    oc[q++] = o + s1 * st[t1] + s2 * st[t2];
+        }
+        double e = E[a * sl + o];
+        if (amb)
+        {
+            const double pe1 = pe_node(PE, o + st[a], st[0], st[1], st[2]);
+            double rhoH = 0.0, rhoNa = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
             {
-                const int64_t oc = o + s1 * st[t1] + s2 * st[t2];
-                rhoH += RH[oc];
-                rhoNa += RN[oc];
+                rhoH += RH[oc[q]];
+                rhoNa += RN[oc[q]];
             }
-        const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
-        E[a * sl + o] -= c.pe_coef * (pe1 - pe0) / nm.ne_eff;
+            const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
+            e -= c.pe_coef * (pe1 - pe0) / nm.ne_eff;
+        }
+        if (loc)
+        {
+            // J edge of family a (CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110)
+            const int64_t u = st[t1], v = st[t2];
+            const double *Bu = Bf + (int64_t)t1 * sl, *Bv = Bf + (int64_t)t2 * sl;
+            const WT *bu = beta + (int64_t)t1 * sl, *bv = beta + (int64_t)t2 * sl;
+            double val;
+            if (a == 1) // eta edge: (bx Bx(o) - bx Bx(o-dk)) + (bz Bz(o-di) - bz Bz(o))
+                val = (bu[o] * Bu[o] - bu[o - v] * Bu[o - v]) + (bv[o - u] * Bv[o - u] - bv[o] * Bv[o]);
+            else // xi: (by By(o-dk) - by By(o)) + (bz Bz(o) - bz Bz(o-dj)); zeta: (bx Bx(o-dj) - bx Bx(o)) + (by By(o) - by By(o-di))
+                val = (bu[o - v] * Bu[o - v] - bu[o] * Bu[o]) + (bv[o] * Bv[o] - bv[o - u] * Bv[o - u]);
+            const double Je = (1.0 * val) * alph[(int64_t)a * sl + o];
+            e += S[(int64_t)(ST_ETALOC + a) * sl + o] * Je;
+        }
+        if (hall)
+        {
+            double eh[4][3], bt[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const int64_t p = oc[q];
+                const NumInfo nm = num_limiter(c, RH[p], RN[p]);
+                const double alpha = c.hall_coef / nm.ne_eff;
+                const double Jx = Jc[p], Jy = Jc[sl + p], Jz = Jc[2 * sl + p];
+                const double Bx = Bc[p], By = Bc[sl + p], Bz = Bc[2 * sl + p];
+                eh[q][0] = alpha * (Jy * Bz - Jz * By);
+                eh[q][1] = alpha * (Jz * Bx - Jx * Bz);
+                eh[q][2] = alpha * (Jx * By - Jy * Bx);
+                bt[q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+            }
+            double Ec[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                Ec[m] = 0.25 * (eh[0][m] + eh[1][m] + eh[2][m] + eh[3][m]);
+            const double bmax = dmax(dmax(bt[0], bt[1]), dmax(bt[2], bt[3]));
+            const double *dr = S + (int64_t)(ST_DR + 3 * a) * sl;
+            const double d0 = dr[o], d1 = dr[sl + o], d2 = dr[2 * sl + o];
+            const double Ecen = Ec[0] * d0 + Ec[1] * d1 + Ec[2] * d2;
+            const double eps = 1e-14, Cwh = 0.1;
+            const double L = dmax(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
+            double mu[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+            {
+                if (f == a)
+                    continue;
+                const int oax = 3 - a - f; // face family f averaged along axis oax
+                const double *A = G + (int64_t)(G_JD + 3 * f) * sl;
+                const int64_t o1 = o - st[oax];
+                const double A0 = sqrt(A[o] * A[o] + A[sl + o] * A[sl + o] + A[2 * sl + o] * A[2 * sl + o]);
+                const double A1 = sqrt(A[o1] * A[o1] + A[sl + o1] * A[sl + o1] + A[2 * sl + o1] * A[2 * sl + o1]);
+                const double Abar = 0.5 * (A0 + A1);
+                const double inv_h = L / dmax(Abar, eps);
+                mu[oax] = Cwh * bmax * inv_h * inv_h;
+            }
+            const double *F1 = Bf + (int64_t)t1 * sl, *F2 = Bf + (int64_t)t2 * sl;
+            const double dB21 = F2[o] - F2[o - st[t1]]; // family t2 across t1
+            const double dB12 = F1[o] - F1[o - st[t2]]; // family t1 across t2
+            const double h = a == 1 ? Ecen + 0.5 * mu[t2] * dB12 - 0.5 * mu[t1] * dB21
+                                    : Ecen + 0.5 * mu[t1] * dB21 - 0.5 * mu[t2] * dB12;
+            e += S[(int64_t)(ST_TAPER + a) * sl + o] * h;
+        }
+        E[a * sl + o] = e;
     }
 }
 
+// ======================================================================
+// Boundary / halo op lists (one phase per launch).
 __device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
 {
     rho = dmax(rho, 2.5e-3);
@@ -1311,9 +1401,12 @@ void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, 
     else
         hipLaunchKernelGGL((k_fluid2<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
-void launch_ambipolar_emf(const Launch &L, const MhdCaseConst *cst, int ucur, hipStream_t q)
+void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_ambipolar_emf, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+    if (mixed)
+        hipLaunchKernelGGL(k_emf_extra<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+    else
+        hipLaunchKernelGGL(k_emf_extra<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
 }
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {

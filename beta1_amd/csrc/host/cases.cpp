@@ -440,6 +440,16 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
         c.qinfs[0] = rho_small * rho0;
         c.qinfs[1] = c.qinfs[2] = c.qinfs[3] = 0.0;
         c.qinfs[4] = ps / (gamma - 1.0) + 0.5 * c.qinfs[0] * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+        c.hall = dp.hall ? 1.0 : 0.0;
+        c.hall_coef = R.B * R.M / (R.U * qe * R.L * R.mu0 * rho_ref * R.NA);
+        c.hall_rmin = dp.hall_rmin;
+        c.hall_rmax = dp.hall_rmax;
+        c.local_arti = (dp.loc_eta_max > 0.0 && dp.loc_r_decay > 0.0) ? 1.0 : 0.0;
+        c.loc_eta_max = dp.loc_eta_max;
+        for (int a = 0; a < 3; ++a)
+            c.loc_center[a] = dp.loc_center[a];
+        c.loc_r_decay = dp.loc_r_decay;
+        c.loc_r_cutoff = dp.loc_r_cutoff;
     }
 
     build_ops(cs);
@@ -467,6 +477,42 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
                         const bool shadow = xc[0] < 0.0 && std::sqrt(xc[1] * xc[1] + xc[2] * xc[2]) < 1.0;
                         A.st[(size_t)ST_DRAG * d.slot + o] = shadow ? 1.0e-5 : 5.0e-5;
                     }
+            // per edge: the edge vector (dr of src/3_field/4_MPCNS_Geometry.cpp:1251-1320),
+            // the radial Hall taper at the midpoint (HallRadialTaperEdge_,
+            // MercurySolver_MagExplicit_Hall.cpp:7-37) and the local
+            // artificial resistivity (MercurySolver_MagExplicit_Resis.cpp:86-121)
+            for (int e = 0; e < 3; ++e)
+                for (int i = -g; i <= B.ni + g; ++i)
+                    for (int j = -g; j <= B.nj + g; ++j)
+                        for (int k = -g; k <= B.nk + g; ++k)
+                        {
+                            const int64_t o = blk_off(d, i, j, k);
+                            const int64_t n0 = B.node_off(i, j, k), n1 = B.node_off(i + (e == 0), j + (e == 1), k + (e == 2));
+                            const double p0[3] = {B.x[n0], B.y[n0], B.z[n0]}, p1[3] = {B.x[n1], B.y[n1], B.z[n1]};
+                            double m[3];
+                            for (int q = 0; q < 3; ++q)
+                            {
+                                A.st[(size_t)(ST_DR + 3 * e + q) * d.slot + o] = p1[q] - p0[q];
+                                m[q] = 0.5 * (p0[q] + p1[q]);
+                            }
+                            const double r = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+                            const double width = c.hall_rmax - c.hall_rmin;
+                            const double taper =
+                                width <= 0.0 ? (r >= c.hall_rmin ? 1.0 : 0.0)
+                                             : 0.5 * (1.0 + std::tanh(2.0 * (r - 0.5 * (c.hall_rmin + c.hall_rmax)) / width));
+                            A.st[(size_t)(ST_TAPER + e) * d.slot + o] = taper;
+                            double eta = 0.0;
+                            if (c.local_arti != 0.0)
+                            {
+                                const double dx = m[0] - c.loc_center[0], dy = m[1] - c.loc_center[1],
+                                             dz = m[2] - c.loc_center[2];
+                                const double rl = std::sqrt(dx * dx + dy * dy + dz * dz);
+                                eta = (c.loc_r_cutoff > 0.0 && rl >= c.loc_r_cutoff)
+                                          ? 0.0
+                                          : c.loc_eta_max * std::exp(-rl / c.loc_r_decay);
+                            }
+                            A.st[(size_t)(ST_ETALOC + e) * d.slot + o] = eta;
+                        }
         }
 
     // nondimensional dipole: |B| = |m| / r^3 at the equator of the body

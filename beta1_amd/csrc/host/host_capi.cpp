@@ -38,6 +38,14 @@ DipoleParams dipole_params(const mhd_dipole_desc &d)
     p.softening = d.softening;
     p.two_fluid = d.single_fluid == 0;
     p.ambipolar = d.no_ambipolar == 0;
+    p.hall = d.no_hall == 0;
+    p.hall_rmin = d.hall_r_min;
+    p.hall_rmax = d.hall_r_max;
+    p.loc_eta_max = d.local_eta_max;
+    for (int a = 0; a < 3; ++a)
+        p.loc_center[a] = d.local_center[a];
+    p.loc_r_decay = d.local_r_decay;
+    p.loc_r_cutoff = d.local_r_cutoff;
     return p;
 }
 
@@ -313,15 +321,16 @@ extern "C" int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out
 
 // Constants of the two-fluid step, in MhdCaseConst order from `model`:
 // model, ambipolar, M_Na, state_coeff_Na, qinfs[5], induce, a1_Na, a4, a5, a6,
-// T_ref, Tn0, pe_coef (17 doubles).
+// T_ref, Tn0, pe_coef, hall, hall_coef, hall_rmin, hall_rmax, local_arti,
+// loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff (28 doubles).
 extern "C" int mhd_host_constants_two_fluid(mhd_host_case *h, double *o)
 {
     if (!h || !o)
         return MHD_ERR_ARG;
     const MhdCaseConst &k = h->cs.cst;
-    static_assert(offsetof(MhdCaseConst, pe_coef) - offsetof(MhdCaseConst, model) == 16 * sizeof(double),
-                  "two-fluid constants are 17 consecutive doubles");
-    std::memcpy(o, &k.model, 17 * sizeof(double));
+    static_assert(offsetof(MhdCaseConst, loc_r_cutoff) - offsetof(MhdCaseConst, model) == 27 * sizeof(double),
+                  "two-fluid constants are 28 consecutive doubles");
+    std::memcpy(o, &k.model, 28 * sizeof(double));
     return MHD_OK;
 }
 
@@ -366,14 +375,25 @@ extern "C" int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n)
 // ------------------------------------------------- CASE directory, restart
 namespace
 {
-// The restart whitelist of the Lunar solver (Z4_Lunar/src/LunarSolver.cpp:38-39).
+// The restart whitelists of the Lunar solver (Z4_Lunar/src/LunarSolver.cpp:38-39)
+// and of the two-fluid Mercury solver (Z5_Mercury/src/MercurySolver.cpp:31-34).
 struct RestartSpec
 {
     const char *name;
     int field;
 };
-const RestartSpec kRestart[4] = {
+const RestartSpec kRestartOne[4] = {
     {"U_H", MHD_F_U_H}, {"B_xi", MHD_F_B_XI}, {"B_eta", MHD_F_B_ETA}, {"B_zeta", MHD_F_B_ZETA}};
+const RestartSpec kRestartTwo[5] = {{"U_H", MHD_F_U_H},
+                                    {"U_Na", MHD_F_U_NA},
+                                    {"B_xi", MHD_F_B_XI},
+                                    {"B_eta", MHD_F_B_ETA},
+                                    {"B_zeta", MHD_F_B_ZETA}};
+std::vector<RestartSpec> restart_fields(int model)
+{
+    return model == 1 ? std::vector<RestartSpec>(kRestartTwo, kRestartTwo + 5)
+                      : std::vector<RestartSpec>(kRestartOne, kRestartOne + 4);
+}
 
 void box_of(const MhdBlockDev &d, int loc, int *lo, int *hi)
 {
@@ -391,7 +411,7 @@ namespace mpcns
 int restart_write(FieldIO &io, const char *path, long long step, double time)
 {
     std::vector<RestartField> fields;
-    for (const RestartSpec &S : kRestart)
+    for (const RestartSpec &S : restart_fields(io.block(0).model))
     {
         int region, slot, loc, nc;
         field_slot(S.field, 0, region, slot, loc, nc);
@@ -423,7 +443,8 @@ int restart_read(FieldIO &io, const char *path, long long *step, double *time)
     for (const RestartField &F : fields)
     {
         const RestartSpec *spec = nullptr;
-        for (const RestartSpec &S : kRestart)
+        const std::vector<RestartSpec> wl = restart_fields(io.block(0).model);
+        for (const RestartSpec &S : wl)
             if (F.name == S.name)
                 spec = &S;
         if (!spec)

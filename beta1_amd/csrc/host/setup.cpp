@@ -1027,6 +1027,40 @@ void build_ops(CaseSetup &cs)
                                                               stride));
                             }
             }
+        // ---- B_cell and J_cell ghosts, read by the Hall EMF of the two-fluid
+        //      step: far-field B_cell = B_imf, wall B_cell and both J_cell
+        //      copies of the first layer (reference Z5_Mercury/src/boundary/
+        //      Boundary_Build.cpp:112-124, Boundary_Condition.cpp:625-657,
+        //      Boundary_Condition_EM.cpp:118-163), then the halos
+        const bool hall_ghosts = cs.blk[b].d.model == 1 && cs.cst.hall != 0.0;
+        if (hall_ghosts)
+        {
+            for (int f = 0; f < 6; ++f)
+            {
+                if (B.face[f].kind != 0 || B.face[f].bc == BC_NONE)
+                    continue;
+                Slab s = inner_slab(B, LOC_CELL, f);
+                for (int i = s.lo[0]; i < s.hi[0]; ++i)
+                    for (int j = s.lo[1]; j < s.hi[1]; ++j)
+                        for (int k = s.lo[2]; k < s.hi[2]; ++k)
+                            for (int l = 1; l <= g; ++l)
+                            {
+                                int q[3] = {i, j, k};
+                                q[s.ax] += s.sgn * l;
+                                if (B.face[f].bc == BC_FARFIELD)
+                                    for (int m = 0; m < 3; ++m)
+                                        cs.ops[SY_BCELL_BC].push_back(mk(C.ref(b, ST_BCELL + m, q[0], q[1], q[2]), 0,
+                                                                         OP_CONST, 1, stride, cs.cst.B_imf[m]));
+                                else
+                                    cs.ops[SY_BCELL_BC].push_back(mk(C.ref(b, ST_BCELL, q[0], q[1], q[2]),
+                                                                     C.ref(b, ST_BCELL, i, j, k), OP_COPY, 3, stride));
+                                cs.ops[SY_JCELL_GH].push_back(mk(C.ref(b, ST_JCELL, q[0], q[1], q[2]),
+                                                                 C.ref(b, ST_JCELL, i, j, k), OP_COPY, 3, stride));
+                            }
+            }
+            halo_ops(C, b, LOC_CELL, 3, false, [](int) { return (int)ST_BCELL; }, cs.ops[SY_BCELL_HALO]);
+            halo_ops(C, b, LOC_CELL, 3, false, [](int) { return (int)ST_JCELL; }, cs.ops[SY_JCELL_HALO]);
+        }
         // ---- halos
         halo_ops(C, b, LOC_CELL, 5, false, [](int) { return (int)ST_U0; }, cs.ops[SY_U_HALO]);
         if (cs.blk[b].d.model == 1)

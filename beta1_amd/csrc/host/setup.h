@@ -117,6 +117,10 @@ enum MhdSync
     SY_B_HALO,
     SY_JCELL_BC,
     SY_UC_CORNER, // corner-diagonal cells take the contravariant velocity of their source
+    SY_BCELL_BC,   // two-fluid with Hall: B_cell physical ghosts
+    SY_BCELL_HALO, //   B_cell halo
+    SY_JCELL_GH,   //   J_cell physical ghosts (copies of the first layer)
+    SY_JCELL_HALO, //   J_cell halo
     SY_COUNT
 };
 
@@ -194,6 +198,10 @@ struct DipoleParams
     double softening; // core softening length [body radii]
     bool two_fluid = true;  // H+ and Na+ (the reference's Mercury step)
     bool ambipolar = true;  // electron-pressure force and ambipolar EMF (two-fluid)
+    bool hall = true;       // explicit Hall EMF (two-fluid)
+    double hall_rmin = 1.1, hall_rmax = 1.2; // radial taper of the Hall EMF
+    // local artificial resistivity (two-fluid; off when eta_max <= 0 or r_decay <= 0)
+    double loc_eta_max = 0.1, loc_center[3] = {1.53, 0.0, 0.25}, loc_r_decay = 0.05, loc_r_cutoff = 0.05;
 };
 
 CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,

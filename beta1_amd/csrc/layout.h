@@ -76,7 +76,10 @@ enum MhdStateSlot
     ST_PHOTO = 47, // Na+ photoionization production rate [cm^-3 s^-1] (static)
     ST_DRAG = 48,  // ion-neutral drag frequency [1/s] (static)
     ST_PE = 49,    // electron pressure p_H + p_Na of cells [-1, N] (step scratch)
-    ST_NSLOT2 = 50
+    ST_DR = 50,    // 9 slots: edge vectors dr_xi(3), dr_eta(3), dr_zeta(3) (static)
+    ST_TAPER = 59, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
+    ST_ETALOC = 62, // 3 slots: local artificial resistivity of the edges (static)
+    ST_NSLOT2 = 65
 };
 
 // State slots per block for a physics model (0 single fluid, 1 two-fluid).
@@ -156,6 +159,13 @@ struct MhdCaseConst
     double T_ref;          // reference temperature [K]
     double Tn0;            // neutral temperature [K]
     double pe_coef;        // ambipolar pressure coefficient
+    // explicit Hall EMF (MercurySolver_MagExplicit_Hall.cpp) and local
+    // artificial resistivity (MercurySolver_MagExplicit_Resis.cpp:67-187)
+    double hall;           // 1: explicit Hall EMF on
+    double hall_coef;      // B_ref M_ref / (U_ref q_e L_ref mu0 rho_ref N_A)
+    double hall_rmin, hall_rmax; // radial taper of the Hall EMF [body radii]
+    double local_arti;     // 1: local artificial resistivity on
+    double loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff;
 };
 
 MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }

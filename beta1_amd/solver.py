@@ -76,10 +76,23 @@ class MercuryDesc(WindDesc):
     softening: float = 0.05
     two_fluid: bool = True
     ambipolar: bool = True
+    hall: bool = True
+    hall_r: tuple = (1.1, 1.2)
+    # local artificial resistivity of Z5_Mercury/9999setup/3_MHD_Control.txt:33-38
+    local_eta_max: float = 0.1
+    local_center: tuple = (1.53, 0.0, 0.25)
+    local_r_decay: float = 0.05
+    local_r_cutoff: float = 0.05
 
     def dipole(self):
-        return MhdDipoleDesc(self.B_eq_nT, self.offset_z, self.softening, 0 if self.two_fluid else 1,
-                             0 if self.ambipolar else 1)
+        d = MhdDipoleDesc()
+        d.B_eq_nT, d.offset_z, d.softening = self.B_eq_nT, self.offset_z, self.softening
+        d.single_fluid, d.no_ambipolar, d.no_hall = int(not self.two_fluid), int(not self.ambipolar), int(not self.hall)
+        d.hall_r_min, d.hall_r_max = self.hall_r
+        d.local_eta_max, d.local_r_decay, d.local_r_cutoff = self.local_eta_max, self.local_r_decay, self.local_r_cutoff
+        for a in range(3):
+            d.local_center[a] = self.local_center[a]
+        return d
 
 
 @dataclass
@@ -154,7 +167,7 @@ class HostCase:
         return out
 
     def constants_two_fluid(self):
-        out = np.zeros(17)
+        out = np.zeros(28)
         lib().mhd_host_constants_two_fluid(self.h, _dptr(out))
         return out
 

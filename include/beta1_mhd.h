@@ -93,6 +93,11 @@ typedef struct
     int no_ambipolar; /* two-fluid only: 1 turns the electron-pressure force
                          and ambipolar EMF off (the reference's
                          is_Ambipolar_Efield 0) */
+    int no_hall;      /* two-fluid only: 1 turns the explicit Hall EMF off */
+    double hall_r_min, hall_r_max; /* radial taper of the Hall EMF [body radii] */
+    /* local artificial resistivity eta_max exp(-r / r_decay) within r_cutoff
+       of a centre (body radii); off when eta_max or r_decay is not > 0 */
+    double local_eta_max, local_center[3], local_r_decay, local_r_cutoff;
 } mhd_dipole_desc;
 
 typedef struct mhd_host_case mhd_host_case;
@@ -231,12 +236,16 @@ int mhd_host_faces(mhd_host_case *h, int block, int *faces14x6);
 int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n);
 int mhd_host_constants(mhd_host_case *h, double *out22);
 /* two-fluid constants: model, ambipolar, M_Na, state_coeff_Na, qinfs[5],
-   momentum_induce_coeff, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef (17 doubles;
-   Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46) */
-int mhd_host_constants_two_fluid(mhd_host_case *h, double *out17);
+   momentum_induce_coeff, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef, hall,
+   hall_coef, hall_rmin, hall_rmax, local_arti, loc_eta_max, loc_center[3],
+   loc_r_decay, loc_r_cutoff (28 doubles;
+   Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46, 9999setup/3_MHD_Control.txt) */
+int mhd_host_constants_two_fluid(mhd_host_case *h, double *out28);
 /* apply one halo / boundary op list (index as in the library's sync table:
    0 Eface BC, 1 Eface halo, 2 Eedge BC, 3-5 Ucell BC0/BC1/halo, 6-7 Bface
-   BC/halo, 8 J_cell BC, 9 corner velocities) to the host arrays */
+   BC/halo, 8 J_cell BC, 9 corner velocities, 10-11 B_cell BC/halo and
+   12-13 J_cell ghosts/halo of two-fluid cases with the Hall EMF) to the host
+   arrays */
 int mhd_host_apply_sync(mhd_host_case *h, int sync);
 int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n);
 

@@ -90,6 +90,7 @@ static const struct
     {L_FXI, 3, 0}, {L_FET, 3, 0}, {L_FZE, 3, 0}, {L_FXI, 1, 0}, {L_FET, 1, 0}, {L_FZE, 1, 0},
     {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0}, {L_CELL, 18, 0}, {L_CELL, 36, 0},
     {L_CELL, 5, 0}, {L_CELL, 5, 0}, {L_CELL, 5, 1}, {L_CELL, 1, 0}, {L_CELL, 1, 0},
+    {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0},
 };
 /* g0 = 1 marks fields the reference registers with zero ghosts
    (Z4_Lunar/main.cpp:107-121); their interior range is what matters. */
@@ -603,6 +604,7 @@ static void physical_phase(or_ctx *c, int fid)
                 break;
             case OF_BADDXI: case OF_BADDET: case OF_BADDZE:
             case OF_JXI: case OF_JET: case OF_JZE:
+            case OF_EHXI: case OF_EHET: case OF_EHZE: /* Mercury default copy (Boundary_Build.cpp:28-36) */
                 bc_copy(F, f, g);
                 break;
             case OF_BCELL:
@@ -650,6 +652,7 @@ static void sync_fields(or_ctx *c, const int *fids, int n, int do_physical)
         if (fid >= OF_JXI && fid <= OF_JZE) { base = OF_JXI; triple = 1; }
         if (fid >= OF_EXI && fid <= OF_EZE) { base = OF_EXI; triple = 1; }
         if (fid >= OF_EFXI && fid <= OF_EFZE) { base = OF_EFXI; triple = 1; eface = 1; }
+        if (fid >= OF_EHXI && fid <= OF_EHZE) { base = OF_EHXI; triple = 1; }
         halo_apply(c, loc, base, triple, eface);
     }
 }
@@ -659,7 +662,7 @@ int or_sync(or_ctx *c, const char *group)
     static const int fU[] = {OF_UH, OF_UNA}, fB[] = {OF_BXI, OF_BET, OF_BZE}, fBadd[] = {OF_BADDXI, OF_BADDET, OF_BADDZE};
     static const int fEf[] = {OF_EFXI, OF_EFET, OF_EFZE}, fE[] = {OF_EXI, OF_EET, OF_EZE};
     static const int fBc[] = {OF_BCELL, OF_BIND}, fJ[] = {OF_JXI, OF_JET, OF_JZE}, fJc[] = {OF_JCELL};
-    static const int fUp[] = {OF_UPLUS};
+    static const int fUp[] = {OF_UPLUS}, fEh[] = {OF_EHXI, OF_EHET, OF_EHZE};
     /* the Mercury Ucell group holds U_H and U_Na (Z5_Mercury/src/boundary/Boundary_Build.cpp:571-578) */
     if (!strcmp(group, "Ucell")) sync_fields(c, fU, c->d.model == 1 ? 2 : 1, 1);
     else if (!strcmp(group, "Bface")) sync_fields(c, fB, 3, 1);
@@ -670,6 +673,7 @@ int or_sync(or_ctx *c, const char *group)
     else if (!strcmp(group, "Jedge")) sync_fields(c, fJ, 3, 1);
     else if (!strcmp(group, "J_cell")) sync_fields(c, fJc, 1, 1);
     else if (!strcmp(group, "Uplus")) sync_fields(c, fUp, 1, 0);
+    else if (!strcmp(group, "Ehall")) sync_fields(c, fEh, 3, 1);
     else return -1;
     return 0;
 }
@@ -1277,6 +1281,166 @@ static void add_ambipolar_emf(or_ctx *c)
     }
 }
 
+static void zero_fblk(fblk *F);
+
+/* edge midpoint of node (i,j,k) -> (i,j,k) + e_a (HallRadialTaperEdge_,
+ * MercurySolver_MagExplicit_Hall.cpp:19-37; edge_midpoint, Resis.cpp:91-106) */
+static void edge_mid(const oblock *B, int a, int i, int j, int k, double *m)
+{
+    const int ip = i + (a == 0), jp = j + (a == 1), kp = k + (a == 2);
+    for (int q = 0; q < 3; ++q) m[q] = 0.5 * (NODE(B, q, i, j, k) + NODE(B, q, ip, jp, kp));
+}
+
+/* AddLocalArtificialResistivityToEdgeEMF_ (Z5_Mercury/src/physics/
+ * MercurySolver_MagExplicit_Resis.cpp:67-186): E += eta(edge) J_edge with
+ * eta = eta_max exp(-r / r_decay) inside r < r_cutoff of the centre. */
+static void add_local_arti_emf(or_ctx *c)
+{
+    const or_desc *d = &c->d;
+    for (int b = 0; b < d->nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        for (int a = 0; a < 3; ++a)
+        {
+            fblk *E = &B->f[OF_EXI + a], *J = &B->f[OF_JXI + a];
+            for (int i = 0; i < E->cnt[0]; ++i)
+                for (int j = 0; j < E->cnt[1]; ++j)
+                    for (int k = 0; k < E->cnt[2]; ++k)
+                    {
+                        double m[3];
+                        edge_mid(B, a, i, j, k, m);
+                        const double dx = m[0] - d->loc_center[0], dy = m[1] - d->loc_center[1], dz = m[2] - d->loc_center[2];
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        double eta;
+                        if (d->loc_r_cutoff > 0.0 && r >= d->loc_r_cutoff) eta = 0.0;
+                        else eta = d->loc_eta_max * exp(-r / d->loc_r_decay);
+                        AT(E, i, j, k, 0) += eta * AT(J, i, j, k, 0);
+                    }
+        }
+    }
+}
+
+/* HallRadialTaper_ (MercurySolver_MagExplicit_Hall.cpp:7-17) */
+static double hall_taper(const or_desc *d, const double *m)
+{
+    const double r = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    const double width = d->hall_rmax - d->hall_rmin;
+    if (width <= 0.0) return (r >= d->hall_rmin) ? 1.0 : 0.0;
+    const double mid = 0.5 * (d->hall_rmin + d->hall_rmax);
+    return 0.5 * (1.0 + tanh(2.0 * (r - mid) / width));
+}
+
+/* AddHallEdgeEMF_ / BuildHallFaceEMF_Rusanov_diff_ (MercurySolver_MagExplicit_Hall.cpp:39-440):
+ * cell Hall field alpha (J x B) averaged over the 4 cells around an edge and
+ * integrated along it, plus a whistler LLF diffusion of the transverse face
+ * flux jumps, tapered in radius; then Sync("Ehall") and E += Ehall on the
+ * edges of the block (MercurySolver_MagneticRHS.cpp:35-56). */
+static void add_hall_emf(or_ctx *c)
+{
+    const or_desc *d = &c->d;
+    const double eps = 1e-14, Cwh = 0.1;
+    for (int b = 0; b < d->nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        fblk *Bc = &B->f[OF_BCELL], *Jc = &B->f[OF_JCELL], *UH = &B->f[OF_UH], *UN = &B->f[OF_UNA];
+        fblk *Bf[3] = {&B->f[OF_BXI], &B->f[OF_BET], &B->f[OF_BZE]};
+        fblk *JD[3] = {&B->f[OF_JDXI], &B->f[OF_JDET], &B->f[OF_JDZE]};
+        for (int a = 0; a < 3; ++a) zero_fblk(&B->f[OF_EHXI + a]);
+        /* Ehc and beta of every cell incl. ghosts */
+        const int g = Bc->g;
+        const int ext[3] = {Bc->ext[0], Bc->ext[1], Bc->ext[2]};
+        const size_t ncell = (size_t)ext[0] * ext[1] * ext[2];
+        double *Ehc = (double *)malloc(sizeof(double) * 3 * ncell), *beta = (double *)malloc(sizeof(double) * ncell);
+#define CIX(i, j, k) ((((size_t)((i) + g)) * ext[1] + (size_t)((j) + g)) * ext[2] + (size_t)((k) + g))
+        for (int i = -g; i < Bc->cnt[0] + g; ++i)
+            for (int j = -g; j < Bc->cnt[1] + g; ++j)
+                for (int k = -g; k < Bc->cnt[2] + g; ++k)
+                {
+                    const numinfo n = num_limiter(c, AT(UH, i, j, k, 0), AT(UN, i, j, k, 0));
+                    const double alpha = d->hall_coef / n.ne_eff;
+                    const double Jx = AT(Jc, i, j, k, 0), Jy = AT(Jc, i, j, k, 1), Jz = AT(Jc, i, j, k, 2);
+                    const double Bx = AT(Bc, i, j, k, 0), By = AT(Bc, i, j, k, 1), Bz = AT(Bc, i, j, k, 2);
+                    const size_t q = CIX(i, j, k);
+                    Ehc[3 * q + 0] = alpha * (Jy * Bz - Jz * By);
+                    Ehc[3 * q + 1] = alpha * (Jz * Bx - Jx * Bz);
+                    Ehc[3 * q + 2] = alpha * (Jx * By - Jy * Bx);
+                    beta[q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+                }
+        for (int a = 0; a < 3; ++a)
+        {
+            fblk *Eh = &B->f[OF_EHXI + a];
+            const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2; /* transverse axes, ascending */
+            for (int i = 0; i < Eh->cnt[0]; ++i)
+                for (int j = 0; j < Eh->cnt[1]; ++j)
+                    for (int k = 0; k < Eh->cnt[2]; ++k)
+                    {
+                        size_t cq[4];
+                        int nq = 0;
+                        for (int s1 = -1; s1 <= 0; ++s1)
+                            for (int s2 = -1; s2 <= 0; ++s2)
+                            {
+                                int q[3] = {i, j, k};
+                                q[t1] += s1;
+                                q[t2] += s2;
+                                cq[nq++] = CIX(q[0], q[1], q[2]);
+                            }
+                        double Ec[3];
+                        for (int m = 0; m < 3; ++m)
+                            Ec[m] = 0.25 * (Ehc[3 * cq[0] + m] + Ehc[3 * cq[1] + m] + Ehc[3 * cq[2] + m] + Ehc[3 * cq[3] + m]);
+                        const int ip = i + (a == 0), jp = j + (a == 1), kp = k + (a == 2);
+                        const double dr[3] = {NODE(B, 0, ip, jp, kp) - NODE(B, 0, i, j, k),
+                                              NODE(B, 1, ip, jp, kp) - NODE(B, 1, i, j, k),
+                                              NODE(B, 2, ip, jp, kp) - NODE(B, 2, i, j, k)};
+                        const double Ecen = Ec[0] * dr[0] + Ec[1] * dr[1] + Ec[2] * dr[2];
+                        const double beta_e = dmax(dmax(beta[cq[0]], beta[cq[1]]), dmax(beta[cq[2]], beta[cq[3]]));
+                        const double L = dmax(sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]), eps);
+                        /* face areas |S| of the two transverse families, averaged
+                           over the two faces along the other transverse axis */
+                        double mu[3] = {0.0, 0.0, 0.0};
+                        for (int f = 0; f < 3; ++f)
+                        {
+                            if (f == a) continue;
+                            const int o_ax = 3 - a - f; /* face family f is averaged along axis o_ax */
+                            int q0[3] = {i, j, k}, q1[3] = {i, j, k};
+                            q1[o_ax] -= 1;
+                            const double *s0 = &AT(JD[f], q0[0], q0[1], q0[2], 0), *s1 = &AT(JD[f], q1[0], q1[1], q1[2], 0);
+                            const double A0 = sqrt(s0[0] * s0[0] + s0[1] * s0[1] + s0[2] * s0[2]);
+                            const double A1 = sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                            const double Abar = 0.5 * (A0 + A1);
+                            const double inv_h = L / dmax(Abar, eps);
+                            mu[o_ax] = Cwh * beta_e * inv_h * inv_h; /* mu along the averaging axis */
+                        }
+                        /* jumps of the face fluxes: family t2 across t1, family t1 across t2 */
+                        int p1[3] = {i, j, k}, p2[3] = {i, j, k};
+                        p1[t1] -= 1;
+                        p2[t2] -= 1;
+                        const double dB_t2_t1 = AT(Bf[t2], i, j, k, 0) - AT(Bf[t2], p1[0], p1[1], p1[2], 0);
+                        const double dB_t1_t2 = AT(Bf[t1], i, j, k, 0) - AT(Bf[t1], p2[0], p2[1], p2[2], 0);
+                        double m[3];
+                        edge_mid(B, a, i, j, k, m);
+                        double e;
+                        if (a == 1) /* eta edge: + mu_zeta dBxi_zeta - mu_xi dBzeta_xi */
+                            e = Ecen + 0.5 * mu[t2] * dB_t1_t2 - 0.5 * mu[t1] * dB_t2_t1;
+                        else /* xi: + mu_eta dBzeta_eta - mu_zeta dBeta_zeta; zeta: + mu_xi dBeta_xi - mu_eta dBxi_eta */
+                            e = Ecen + 0.5 * mu[t1] * dB_t2_t1 - 0.5 * mu[t2] * dB_t1_t2;
+                        AT(Eh, i, j, k, 0) = hall_taper(d, m) * e;
+                    }
+        }
+#undef CIX
+        free(Ehc);
+        free(beta);
+    }
+    or_sync(c, "Ehall");
+    for (int b = 0; b < d->nblocks; ++b)
+        for (int a = 0; a < 3; ++a)
+        {
+            fblk *E = &c->b[b].f[OF_EXI + a], *Eh = &c->b[b].f[OF_EHXI + a];
+            for (int i = 0; i < E->cnt[0]; ++i)
+                for (int j = 0; j < E->cnt[1]; ++j)
+                    for (int k = 0; k < E->cnt[2]; ++k) AT(E, i, j, k, 0) += AT(Eh, i, j, k, 0);
+        }
+}
+
 /* ------------------------------------------------------------ CT / EMF */
 typedef struct
 {
@@ -1518,6 +1682,8 @@ static void assemble_ct(or_ctx *c)
     or_sync(c, "Eface");
     edge_emf(c);
     if (c->d.model == 1 && c->d.ambipolar) add_ambipolar_emf(c);
+    if (c->d.model == 1 && c->d.local_arti) add_local_arti_emf(c);
+    if (c->d.model == 1 && c->d.hall) add_hall_emf(c);
     or_sync(c, "Eedge");
     curl_edge_to_face(c, -1.0);
 }

@@ -39,6 +39,11 @@ typedef struct
     int ambipolar; /* model 1: electron-pressure terms in the momentum source and the edge EMF */
     double M_Na, state_coeff_Na, qinfs[5];
     double induce, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef;
+    /* explicit Hall EMF (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Hall.cpp)
+       and local artificial resistivity (MercurySolver_MagExplicit_Resis.cpp:67-187) */
+    int hall, local_arti;
+    double hall_coef, hall_rmin, hall_rmax;
+    double loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff;
 } or_desc;
 
 enum
@@ -48,7 +53,7 @@ enum
     OF_EFXI, OF_EFET, OF_EFZE, OF_FXI, OF_FET, OF_FZE, OF_RHSH, OF_RHSBXI, OF_RHSBET, OF_RHSBZE,
     OF_DIVB, OF_JAC, OF_JDXI, OF_JDET, OF_JDZE, OF_BETAXI, OF_BETAET, OF_BETAZE,
     OF_ALXI, OF_ALET, OF_ALZE, OF_WB, OF_WJ,
-    OF_UNA, OF_PVNA, OF_RHSNA, OF_PHOTO, OF_DRAG, OF_COUNT
+    OF_UNA, OF_PVNA, OF_RHSNA, OF_PHOTO, OF_DRAG, OF_EHXI, OF_EHET, OF_EHZE, OF_COUNT
 };
 
 typedef struct or_ctx or_ctx;

@@ -28,7 +28,9 @@ class OrDesc(C.Structure):
                 ("model", C.c_int), ("ambipolar", C.c_int), ("M_Na", C.c_double), ("state_coeff_Na", C.c_double),
                 ("qinfs", C.c_double * 5), ("induce", C.c_double), ("a1_Na", C.c_double), ("a4", C.c_double),
                 ("a5", C.c_double), ("a6", C.c_double), ("T_ref", C.c_double), ("Tn0", C.c_double),
-                ("pe_coef", C.c_double)]
+                ("pe_coef", C.c_double), ("hall", C.c_int), ("local_arti", C.c_int), ("hall_coef", C.c_double),
+                ("hall_rmin", C.c_double), ("hall_rmax", C.c_double), ("loc_eta_max", C.c_double),
+                ("loc_center", C.c_double * 3), ("loc_r_decay", C.c_double), ("loc_r_cutoff", C.c_double)]
 
 
 _lib = None
@@ -89,6 +91,11 @@ class Oracle:
         for m in range(5):
             d.qinfs[m] = t[4 + m]
         d.induce, d.a1_Na, d.a4, d.a5, d.a6, d.T_ref, d.Tn0, d.pe_coef = t[9:17]
+        d.hall, d.hall_coef, d.hall_rmin, d.hall_rmax = int(t[17]), t[18], t[19], t[20]
+        d.local_arti, d.loc_eta_max = int(t[21]), t[22]
+        for a in range(3):
+            d.loc_center[a] = t[23 + a]
+        d.loc_r_decay, d.loc_r_cutoff = t[26], t[27]
         L = olib()
         self.c = L.or_create(C.byref(d))
         self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks

@@ -69,7 +69,7 @@ def test_case_dir_round_trip_is_identical(tmp_path, desc):
             assert np.array_equal(a, c)
         for f in FIELDS:
             assert np.array_equal(back.field(f, b), hc.field(f, b)), (b, f)
-    for sy in range(10):
+    for sy in range(14):
         assert back.op_count(sy) == hc.op_count(sy)
 
 
@@ -195,3 +195,16 @@ def test_restart_reads_across_ghost_depths(tmp_path):
     c.read_restart(path)
     for b in range(6):
         assert np.array_equal(c.field("U_H", b)[interior("U_H", c.n, 3)], a.field("U_H", b)[interior("U_H", a.n, 2)])
+
+
+def test_two_fluid_restart_holds_na(tmp_path):
+    from beta1_amd import MercuryDesc
+    hc = HostCase(MercuryDesc(nang=4, nrad=6), ghost=G)
+    for sy in (3, 4, 5, 6, 7):
+        hc.apply_sync(sy)
+    path = str(tmp_path / "m.bin")
+    hc.write_restart(path, step=7, time=0.5)
+    _, _, _, f = read_restart_py(path)
+    assert list(f) == ["U_H", "U_Na", "B_xi", "B_eta", "B_zeta"]  # the Mercury whitelist
+    for b, (lo, hi, v) in enumerate(f["U_Na"][3]):
+        assert np.array_equal(v, hc.field("U_Na", b))

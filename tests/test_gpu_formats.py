@@ -4,16 +4,16 @@ read by another continues bit for bit."""
 import numpy as np
 import pytest
 
-from beta1_amd import CaseDirDesc, HostCase, MhdContext, WindDesc
+from beta1_amd import CaseDirDesc, HostCase, MercuryDesc, MhdContext, WindDesc
 
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("U_H", "B_xi", "B_eta", "B_zeta")
 
 
-def fields(ctx, cid):
+def fields(ctx, cid, names=FIELDS):
     nb = ctx.info(cid)[0]
-    return {(f, b): ctx.field(cid, f, b) for f in FIELDS for b in range(nb)}
+    return {(f, b): ctx.field(cid, f, b) for f in names for b in range(nb)}
 
 
 def test_case_from_dir_steps_like_generated_case(tmp_path):
@@ -31,8 +31,9 @@ def test_case_from_dir_steps_like_generated_case(tmp_path):
     assert ctx.last_dt(a) == ctx.last_dt(b)
 
 
-def test_restart_continues_bitwise(tmp_path):
-    desc = WindDesc(nang=8, nrad=16)
+@pytest.mark.parametrize("desc,names", [(WindDesc(nang=8, nrad=16), FIELDS),
+                                        (MercuryDesc(nang=8, nrad=16), FIELDS + ("U_Na",))])
+def test_restart_continues_bitwise(tmp_path, desc, names):
     path = str(tmp_path / "rst.bin")
     one = MhdContext()
     c1 = one.add(desc)
@@ -44,13 +45,13 @@ def test_restart_continues_bitwise(tmp_path):
     two.finalize()
     two.step(2)  # a different state, overwritten by the restart
     assert two.read_restart(c2, path) == (5, 0.25)
-    ref = fields(one, c1)
-    got = fields(two, c2)
+    ref = fields(one, c1, names)
+    got = fields(two, c2, names)
     for k in ref:
         assert np.array_equal(ref[k], got[k]), k
     one.step(4)
     two.step(4)
-    ref, got = fields(one, c1), fields(two, c2)
+    ref, got = fields(one, c1, names), fields(two, c2, names)
     for k in ref:
         assert np.array_equal(ref[k], got[k]), k
     assert one.last_dt(c1) == two.last_dt(c2)

@@ -26,11 +26,19 @@ def pair(desc, steps, recon=0, precision="fp64"):
     return hc, orc, ctx, cid, dts
 
 
-@pytest.mark.parametrize("ambipolar,recon,steps", [(True, 0, 8), (False, 0, 6), (True, 1, 5)])
-def test_two_fluid_fp64_bitwise_equals_oracle(ambipolar, recon, steps):
-    desc = MercuryDesc(nang=8, nrad=16, ambipolar=ambipolar)
+# the reference's set-up (ambipolar, Hall, local resistivity of a 0.05 R_M
+# sphere) and variants: terms off, minmod, a local resistivity region that
+# covers many edges of these coarse shells
+VARIANTS = [dict(), dict(ambipolar=False, hall=False), dict(hall=False),
+            dict(local_center=(0.0, 1.2, 0.0), local_r_decay=0.5, local_r_cutoff=1.0)]
+
+
+@pytest.mark.parametrize("kw,recon,steps", [(VARIANTS[0], 0, 8), (VARIANTS[1], 0, 6), (VARIANTS[2], 0, 5),
+                                            (VARIANTS[3], 0, 6), (VARIANTS[0], 1, 5)])
+def test_two_fluid_fp64_bitwise_equals_oracle(kw, recon, steps):
+    desc = MercuryDesc(nang=8, nrad=16, **kw)
+    print(f"two-fluid {kw} recon={recon} steps={steps}", flush=True)
     hc, orc, ctx, cid, dts = pair(desc, steps, recon=recon)
-    print(f"two-fluid ambipolar={ambipolar} recon={recon} steps={steps}", flush=True)
     assert ctx.last_dt(cid) == dts[-1]
     for name in FIELDS:
         sl = interior(name, hc.n, G)
@@ -84,7 +92,7 @@ def test_models_batched_each_as_alone():
         ctx = MhdContext()
         ids = [ctx.add(d) for d in descs]
         ctx.finalize()
-        for k in ("k_fluid", "k_derived", "k_fluid2", "k_derived2", "k_ambipolar_emf", "k_face_emf"):
+        for k in ("k_fluid", "k_derived", "k_fluid2", "k_derived2", "k_emf_extra", "k_face_emf"):
             ctx.set_tuning(k, rows=2, i_inner=shared, geo_shared=shared)
         ctx.step(2, graph=True)
         ctx.step(2)

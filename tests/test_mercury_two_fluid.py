@@ -171,3 +171,88 @@ def test_two_fluid_oracle_threads_equal_serial():
     for blk in range(6):
         for name in ("U_H", "U_Na", "B_xi", "B_eta", "B_zeta"):
             assert np.array_equal(a.view(name, blk), b.view(name, blk))
+
+
+def edge_midpoints(hc, b, a):
+    """Midpoints and vectors of the a-edges (i, j, k) -> (i, j, k) + e_a over
+    the interior edge range of block b (node arrays include G+1 ghosts)."""
+    gn = G + 1
+    X = np.stack(hc.nodes(b), -1)
+    n = list(hc.n)
+    cnt = [n[q] + (0 if q == a else 1) for q in range(3)]
+    s0 = tuple(slice(gn, gn + cnt[q]) for q in range(3))
+    s1 = tuple(slice(gn + (q == a), gn + (q == a) + cnt[q]) for q in range(3))
+    return 0.5 * (X[s0] + X[s1]), X[s1] - X[s0]
+
+
+def edge_view(o, name, hc, b):
+    a = "xi eta zeta".split().index(name.split("_")[1])
+    n = list(hc.n)
+    cnt = [n[q] + (0 if q == a else 1) for q in range(3)]
+    return o.view(name, b)[tuple(slice(G, G + cnt[q]) for q in range(3))][..., 0]
+
+
+def test_local_artificial_resistivity_adds_eta_times_j_edge():
+    kw = dict(ambipolar=False, hall=False, local_center=(0.0, 0.0, 2.0), local_r_decay=0.7, local_r_cutoff=1.5)
+    res = {}
+    for eta in (0.0, 0.3):
+        hc = case(local_eta_max=eta, **kw)
+        rng = np.random.default_rng(11)
+        for blk in range(hc.nblocks):
+            for f in ("B_xi", "B_eta", "B_zeta"):
+                hc.set_field(f, blk, 1e-3 * rng.standard_normal(hc.field(f, blk).shape))
+        for sy in (6, 7):
+            hc.apply_sync(sy)
+        o = Oracle(hc)
+        J = {(b, a): edge_view(o, "J_" + ["xi", "eta", "zeta"][a], hc, b).copy() for b in range(6) for a in range(3)}
+        o.step(1)
+        res[eta] = {(b, a): edge_view(o, "E_" + ["xi", "eta", "zeta"][a], hc, b).copy() for b in range(6) for a in range(3)}
+    hit = 0
+    for (b, a), J_e in J.items():
+        mid, _ = edge_midpoints(hc, b, a)
+        r = np.linalg.norm(mid - np.array([0.0, 0.0, 2.0]), axis=-1)
+        eta = np.where(r >= 1.5, 0.0, 0.3 * np.exp(-r / 0.7))
+        dE = (res[0.3][(b, a)] - res[0.0][(b, a)])[..., :-2]  # clear of the far-field edges
+        want = (eta * J_e)[..., :-2]
+        assert np.allclose(dE, want, rtol=1e-9, atol=1e-16 * np.abs(res[0.0][(b, a)]).max()), (b, a)
+        hit += int(np.count_nonzero(want))
+    assert hit > 100
+
+
+def test_hall_emf_central_term_is_taper_times_alpha_j_cross_b_dot_dr():
+    """Uniform J_cell and B_cell, zero face fluxes and uniform densities:
+    the Hall EMF is taper(r) * alpha (J x B) . dr on every edge clear of the
+    wall and the far field (the whistler diffusion acts on flux jumps, zero
+    here)."""
+    Jv, Bv = np.array([0.3, -0.2, 0.5]), np.array([1.0, 2.0, -0.5])
+    res = {}
+    for hall in (False, True):
+        hc = case(ambipolar=False, hall=hall, local_eta_max=0.0)
+        t = hc.constants_two_fluid()
+        o = Oracle(hc)
+        for b in range(6):
+            o.view("J_cell", b)[...] = Jv
+            o.view("B_cell", b)[...] = Bv
+            for f in ("B_xi", "B_eta", "B_zeta"):
+                o.view(f, b)[...] = 0.0
+        o.step(1)
+        res[hall] = o
+    k = hc.constants()
+    rhoH, rhoNa = k[8], t[4]
+    ne = rhoH / k[4] * k[5] + rhoNa / t[2] * k[5]
+    alpha = t[18] / np.sqrt(ne * ne + k[6] ** 2)
+    C = alpha * np.cross(Jv, Bv)
+    lo, hi = t[19], t[20]
+    for b in range(6):
+        for a, nm in enumerate(("E_xi", "E_eta", "E_zeta")):
+            mid, dr = edge_midpoints(hc, b, a)
+            r = np.linalg.norm(mid, axis=-1)
+            taper = 0.5 * (1 + np.tanh(2 * (r - 0.5 * (lo + hi)) / (hi - lo)))
+            want = taper * (dr @ C)
+            E0 = edge_view(res[False], nm, hc, b)
+            dE = edge_view(res[True], nm, hc, b) - E0
+            sl = (slice(1, -1), slice(1, -1), slice(2, -2))  # interior densities on all 4 cells
+            # the difference of two EMFs carries the rounding of the larger ideal part
+            tol = 1e-13 * max(np.abs(E0).max(), np.abs(want).max())
+            assert np.abs(dE[sl] - want[sl]).max() <= tol + 1e-9 * np.abs(want[sl]).max(), (b, nm)
+            assert np.abs(want[sl]).max() > 1e3 * tol
