@@ -29,7 +29,8 @@ class MhdDipoleDesc(C.Structure):
     _fields_ = [("B_eq_nT", C.c_double), ("offset_z", C.c_double), ("softening", C.c_double),
                 ("single_fluid", C.c_int), ("no_ambipolar", C.c_int), ("no_hall", C.c_int),
                 ("hall_r_min", C.c_double), ("hall_r_max", C.c_double), ("local_eta_max", C.c_double),
-                ("local_center", C.c_double * 3), ("local_r_decay", C.c_double), ("local_r_cutoff", C.c_double)]
+                ("local_center", C.c_double * 3), ("local_r_decay", C.c_double), ("local_r_cutoff", C.c_double),
+                ("no_resistance", C.c_int), ("resist_subcycles", C.c_int), ("resist_eta_max", C.c_double)]
 
 
 _P = C.c_void_p

@@ -113,13 +113,17 @@ enum KernelId
     KP_FLUID2,
     KP_DERIVED2,
     KP_AMBI,
+    KP_ERES,
+    KP_JCELL_RES,
+    KP_CURL_RES,
     KP_COUNT
 };
 const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
-                                     "k_fluid2",  "k_derived2",    "k_emf_extra"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE,
-                                    R_EXT,  R_CELL, R_CELL, R_CELL, R_EXT, R_FACE};
+                                     "k_fluid2",  "k_derived2",    "k_emf_extra", "k_eres",     "k_jcell_res",
+                                     "k_curl_res"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT,
+                                    R_CELL, R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE};
 } // namespace
 
 struct mhd_ctx
@@ -146,6 +150,8 @@ struct mhd_ctx
     MhdCaseConst *d_cst = nullptr;
     double *d_dt = nullptr, *d_zero = nullptr;
     unsigned long long *d_dtbits = nullptr, *d_dtscratch = nullptr, *d_divb = nullptr;
+    double *d_dtres = nullptr; // per-case dt of the current resistive subcycle
+    int max_nsub = 0;          // resistive subcycles of the resident cases (0: none)
     int *d_err = nullptr, *d_active = nullptr;
 
     // op lists of the active cases, one device array per sync group
@@ -251,6 +257,7 @@ extern "C" int mhd_destroy(mhd_ctx *c)
     free_t(c->d_dtbits);
     free_t(c->d_dtscratch);
     free_t(c->d_divb);
+    free_t(c->d_dtres);
     free_t(c->d_err);
     free_t(c->d_active);
     (void)hipStreamDestroy(c->q);
@@ -296,6 +303,7 @@ static int ensure_case_capacity(mhd_ctx *c)
     if ((rc = regrow(c, c->d_cst, old, cap, true)) || (rc = regrow(c, c->d_dt, old, cap, true)) ||
         (rc = regrow(c, c->d_zero, old, cap, false)) || (rc = regrow(c, c->d_dtbits, old, cap, true)) ||
         (rc = regrow(c, c->d_dtscratch, old, cap, false)) || (rc = regrow(c, c->d_divb, old, cap, false)) ||
+        (rc = regrow(c, c->d_dtres, old, cap, false)) ||
         (rc = regrow(c, c->d_err, old, cap, true)) || (rc = regrow(c, c->d_active, old, cap, true)))
         return rc;
     std::vector<unsigned long long> big(cap - old, dbits(1e100));
@@ -405,7 +413,8 @@ static int build_tiles(mhd_ctx *c)
 // two-fluid case is resident), the shared kernels over every block.
 static int set_of(const mhd_ctx *c, int kernel)
 {
-    if (kernel == KP_FLUID2 || kernel == KP_DERIVED2 || kernel == KP_AMBI)
+    if (kernel == KP_FLUID2 || kernel == KP_DERIVED2 || kernel == KP_AMBI || kernel == KP_ERES ||
+        kernel == KP_JCELL_RES || kernel == KP_CURL_RES)
         return BS_TWO;
     if ((kernel == KP_FLUID || kernel == KP_DERIVED) && c->n_two > 0)
         return BS_ONE;
@@ -450,6 +459,7 @@ static int rebuild(mhd_ctx *c)
         c->gexec = nullptr;
     }
     c->h_blk.clear();
+    c->max_nsub = 0;
     std::vector<MhdCaseConst> cst(c->case_cap);
     std::vector<int> act(c->case_cap, 0);
     for (int ci = 0; ci < (int)c->cases.size(); ++ci)
@@ -462,6 +472,8 @@ static int rebuild(mhd_ctx *c)
             continue;
         }
         act[ci] = 1;
+        if (r.cs.cst.model == 1.0 && r.cs.cst.resist != 0.0)
+            c->max_nsub = std::max(c->max_nsub, (int)r.cs.cst.resist_nsub);
         r.first_block = (int)c->h_blk.size();
         const GeoSet &G = c->geos[r.geo_set];
         for (int b = 0; b < r.nblocks; ++b)
@@ -730,6 +742,41 @@ static void enqueue_step(mhd_ctx *c)
     {
         ProfScope p(c, KP_CURL);
         mhdk::launch_curl_update(launch_of(c, KP_CURL), c->d_dt, c->q);
+    }
+    // interior resistivity of the two-fluid cases: explicit subcycles on B
+    // (ResistiveDiffusionSubcycles_, Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141)
+    for (int isub = 0; isub < c->max_nsub; ++isub)
+    {
+        mhdk::launch_dt_res(c->d_dtres, c->d_dt, c->d_cst, c->case_cap, isub, c->q);
+        {
+            ProfScope p(c, KP_OPS);
+            run_ops(c, SY_B_BC, 0);
+            run_ops(c, SY_B_HALO, 0);
+        }
+        {
+            ProfScope p(c, KP_JCELL_RES);
+            mhdk::launch_jcell(launch_of(c, KP_JCELL_RES), mixed, c->q);
+        }
+        {
+            ProfScope p(c, KP_OPS);
+            run_ops(c, SY_JCELL_GH, 0);
+            run_ops(c, SY_JCELL_HALO, 0);
+        }
+        {
+            ProfScope p(c, KP_ERES);
+            mhdk::launch_eres(launch_of(c, KP_ERES), c->d_cst, c->q);
+        }
+        {
+            ProfScope p(c, KP_OPS);
+            run_ops(c, SY_EEDGE_BC, 0); // Eres far field: the Eedge handler (Boundary_Build.cpp:200-205)
+        }
+        {
+            ProfScope p(c, KP_CURL_RES);
+            mhdk::launch_curl_update(launch_of(c, KP_CURL_RES), c->d_dtres, c->q);
+        }
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_B_BC, 0);
+        run_ops(c, SY_B_HALO, 0);
     }
     c->ucur ^= 1;
     {

@@ -35,6 +35,8 @@ void launch_derived2(const Launch &L, const MhdCaseConst *cst, unsigned long lon
 void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                    hipStream_t q);
 void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q);
+void launch_eres(const Launch &L, const MhdCaseConst *cst, hipStream_t q);
+void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int ncase, int isub, hipStream_t q);
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);

@@ -1191,6 +1191,70 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
     }
 }
 
+// Interior resistivity, one explicit subcycle's EMF: AddResistiveEdgeEMF_To_
+// (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Resis.cpp:188-330),
+// E = inver_Rem yita0 (J_cell averaged over the 4 cells around the edge) . dr,
+// 0 where yita0 vanishes; E holds Eres (the step's EMF has been consumed).
+__global__ __launch_bounds__(256) void k_eres(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
+        return;
+    if (cst[b.case_id].resist == 0.0)
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    double *S = dst_ptr(b);
+    const double *Jc = S + (int64_t)ST_JCELL * sl;
+    double *E = S + (int64_t)ST_E * sl;
+    const int idx[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        bool in = true;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            in = in && (idx[q] < n[q] + (q == a ? 0 : 1));
+        if (!in)
+            continue;
+        const double coef = S[(int64_t)(ST_ETARES + a) * sl + o];
+        double e = 0.0;
+        if (coef != 0.0)
+        {
+            const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+            // cells (i,j,k), minus t1, minus t2, minus both (get_edge_cells, :232-261)
+            const int64_t q[4] = {o, o - st[t1], o - st[t2], o - st[t1] - st[t2]};
+            double J[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    J[m] += Jc[m * sl + q[p]];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                J[m] *= 0.25;
+            const double *dr = S + (int64_t)(ST_DR + 3 * a) * sl;
+            const double Jedge = J[0] * dr[o] + J[1] * dr[sl + o] + J[2] * dr[2 * sl + o];
+            e = 0.0 + coef * Jedge;
+        }
+        E[a * sl + o] = e;
+    }
+}
+
+// Per-case time step of resistive subcycle `isub`: dt / nsub while the case
+// has subcycles left, else 0 (its B is left unchanged by the update).
+__global__ void k_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int ncase, int isub)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncase)
+        return;
+    const MhdCaseConst &k = cst[c];
+    const bool on = k.model == 1.0 && k.resist != 0.0 && isub < (int)k.resist_nsub;
+    dtres[c] = on ? dt[c] / (double)(int)k.resist_nsub : 0.0;
+}
+
 // ======================================================================
 // Boundary / halo op lists (one phase per launch).
 __device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
@@ -1407,6 +1471,14 @@ void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mi
         hipLaunchKernelGGL(k_emf_extra<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
     else
         hipLaunchKernelGGL(k_emf_extra<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, ucur);
+}
+void launch_eres(const Launch &L, const MhdCaseConst *cst, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_eres, grid_of(L), block_of(L), 0, q, L.tm, cst);
+}
+void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int ncase, int isub, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_dt_res, dim3((ncase + 63) / 64), dim3(64), 0, q, dtres, dt, cst, ncase, isub);
 }
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {

@@ -450,6 +450,9 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
             c.loc_center[a] = dp.loc_center[a];
         c.loc_r_decay = dp.loc_r_decay;
         c.loc_r_cutoff = dp.loc_r_cutoff;
+        c.resist = dp.resist ? 1.0 : 0.0;
+        c.resist_nsub = dp.resist_nsub;
+        c.inver_Rem = dp.resist_eta_max / (R.mu0 * R.U * R.L);
     }
 
     build_ops(cs);
@@ -512,6 +515,12 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
                                           : c.loc_eta_max * std::exp(-rl / c.loc_r_decay);
                             }
                             A.st[(size_t)(ST_ETALOC + e) * d.slot + o] = eta;
+                            // inver_Rem yita0(r) (MercurySolver_MagExplicit_Resis.cpp:196-206, 268-275)
+                            double yita0 = 0.0;
+                            if (!(r <= 0.84 || r >= 1.04))
+                                yita0 = 0.5 * (std::tanh((r - 0.84) / 0.02) - std::tanh((r - 1.04) / 0.02));
+                            A.st[(size_t)(ST_ETARES + e) * d.slot + o] =
+                                c.resist != 0.0 && yita0 != 0.0 ? c.inver_Rem * yita0 : 0.0;
                         }
         }
 

@@ -46,6 +46,9 @@ DipoleParams dipole_params(const mhd_dipole_desc &d)
         p.loc_center[a] = d.local_center[a];
     p.loc_r_decay = d.local_r_decay;
     p.loc_r_cutoff = d.local_r_cutoff;
+    p.resist = d.no_resistance == 0 && d.resist_eta_max > 0.0;
+    p.resist_nsub = d.resist_subcycles > 1 ? d.resist_subcycles : 1;
+    p.resist_eta_max = d.resist_eta_max;
     return p;
 }
 
@@ -322,15 +325,16 @@ extern "C" int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out
 // Constants of the two-fluid step, in MhdCaseConst order from `model`:
 // model, ambipolar, M_Na, state_coeff_Na, qinfs[5], induce, a1_Na, a4, a5, a6,
 // T_ref, Tn0, pe_coef, hall, hall_coef, hall_rmin, hall_rmax, local_arti,
-// loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff (28 doubles).
+// loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff, resist, resist_nsub,
+// inver_Rem (31 doubles).
 extern "C" int mhd_host_constants_two_fluid(mhd_host_case *h, double *o)
 {
     if (!h || !o)
         return MHD_ERR_ARG;
     const MhdCaseConst &k = h->cs.cst;
-    static_assert(offsetof(MhdCaseConst, loc_r_cutoff) - offsetof(MhdCaseConst, model) == 27 * sizeof(double),
-                  "two-fluid constants are 28 consecutive doubles");
-    std::memcpy(o, &k.model, 28 * sizeof(double));
+    static_assert(offsetof(MhdCaseConst, inver_Rem) - offsetof(MhdCaseConst, model) == 30 * sizeof(double),
+                  "two-fluid constants are 31 consecutive doubles");
+    std::memcpy(o, &k.model, 31 * sizeof(double));
     return MHD_OK;
 }
 

@@ -1027,12 +1027,12 @@ void build_ops(CaseSetup &cs)
                                                               stride));
                             }
             }
-        // ---- B_cell and J_cell ghosts, read by the Hall EMF of the two-fluid
-        //      step: far-field B_cell = B_imf, wall B_cell and both J_cell
+        // ---- B_cell and J_cell ghosts, read by the Hall EMF and the interior
+        //      resistivity of the two-fluid step: far-field B_cell = B_imf, wall B_cell and both J_cell
         //      copies of the first layer (reference Z5_Mercury/src/boundary/
         //      Boundary_Build.cpp:112-124, Boundary_Condition.cpp:625-657,
         //      Boundary_Condition_EM.cpp:118-163), then the halos
-        const bool hall_ghosts = cs.blk[b].d.model == 1 && cs.cst.hall != 0.0;
+        const bool hall_ghosts = cs.blk[b].d.model == 1 && (cs.cst.hall != 0.0 || cs.cst.resist != 0.0);
         if (hall_ghosts)
         {
             for (int f = 0; f < 6; ++f)

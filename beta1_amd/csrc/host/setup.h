@@ -202,6 +202,10 @@ struct DipoleParams
     double hall_rmin = 1.1, hall_rmax = 1.2; // radial taper of the Hall EMF
     // local artificial resistivity (two-fluid; off when eta_max <= 0 or r_decay <= 0)
     double loc_eta_max = 0.1, loc_center[3] = {1.53, 0.0, 0.25}, loc_r_decay = 0.05, loc_r_cutoff = 0.05;
+    // interior resistivity (two-fluid), explicit subcycles
+    bool resist = true;
+    int resist_nsub = 1;
+    double resist_eta_max = 1.25e7; // eta_max_mercury [m^2/s]
 };
 
 CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,

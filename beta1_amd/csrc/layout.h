@@ -79,7 +79,8 @@ enum MhdStateSlot
     ST_DR = 50,    // 9 slots: edge vectors dr_xi(3), dr_eta(3), dr_zeta(3) (static)
     ST_TAPER = 59, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
     ST_ETALOC = 62, // 3 slots: local artificial resistivity of the edges (static)
-    ST_NSLOT2 = 65
+    ST_ETARES = 65, // 3 slots: interior resistivity inver_Rem yita0(r) of the edges (static)
+    ST_NSLOT2 = 68
 };
 
 // State slots per block for a physics model (0 single fluid, 1 two-fluid).
@@ -166,6 +167,11 @@ struct MhdCaseConst
     double hall_rmin, hall_rmax; // radial taper of the Hall EMF [body radii]
     double local_arti;     // 1: local artificial resistivity on
     double loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff;
+    // interior resistivity by explicit subcycles (ResistiveDiffusionSubcycles_,
+    // Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141)
+    double resist;         // 1: on
+    double resist_nsub;    // subcycles per step
+    double inver_Rem;      // eta_max / (mu0 U_ref L_ref)
 };
 
 MHD_HD double *st_ptr(const MhdBlockDev &b) { return reinterpret_cast<double *>((uintptr_t)b.st); }

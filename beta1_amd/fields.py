@@ -23,6 +23,10 @@ FIELDS = [
     ("Bcell_from_Bface_w", CELL, 18), ("Jcell_from_Jedge_w", CELL, 36),
     # two-fluid (Mercury) cases
     ("U_Na", CELL, 5), ("PV_Na", CELL, 5), ("RHS_Na", CELL, 5), ("Photo_rate", CELL, 1), ("drag_rate", CELL, 1),
+    # oracle-only arrays of the two-fluid step (Hall and resistive EMFs, resistive RHS)
+    ("Ehall_xi", EDGE_XI, 1), ("Ehall_eta", EDGE_ET, 1), ("Ehall_zeta", EDGE_ZE, 1),
+    ("Eres_xi", EDGE_XI, 1), ("Eres_eta", EDGE_ET, 1), ("Eres_zeta", EDGE_ZE, 1),
+    ("RHS_b_res_xi", FACE_XI, 1), ("RHS_b_res_eta", FACE_ET, 1), ("RHS_b_res_zeta", FACE_ZE, 1),
 ]
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 LOCATION = {name: loc for name, loc, _ in FIELDS}

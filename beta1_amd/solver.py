@@ -83,6 +83,10 @@ class MercuryDesc(WindDesc):
     local_center: tuple = (1.53, 0.0, 0.25)
     local_r_decay: float = 0.05
     local_r_cutoff: float = 0.05
+    # interior resistivity (explicit subcycles): eta_max_mercury of 0_General_Setting.txt
+    resist: bool = True
+    resist_subcycles: int = 1
+    resist_eta_max: float = 1.25e7
 
     def dipole(self):
         d = MhdDipoleDesc()
@@ -92,6 +96,8 @@ class MercuryDesc(WindDesc):
         d.local_eta_max, d.local_r_decay, d.local_r_cutoff = self.local_eta_max, self.local_r_decay, self.local_r_cutoff
         for a in range(3):
             d.local_center[a] = self.local_center[a]
+        d.no_resistance, d.resist_subcycles, d.resist_eta_max = int(not self.resist), self.resist_subcycles, \
+            self.resist_eta_max
         return d
 
 
@@ -167,7 +173,7 @@ class HostCase:
         return out
 
     def constants_two_fluid(self):
-        out = np.zeros(28)
+        out = np.zeros(31)
         lib().mhd_host_constants_two_fluid(self.h, _dptr(out))
         return out
 

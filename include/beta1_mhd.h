@@ -98,6 +98,12 @@ typedef struct
     /* local artificial resistivity eta_max exp(-r / r_decay) within r_cutoff
        of a centre (body radii); off when eta_max or r_decay is not > 0 */
     double local_eta_max, local_center[3], local_r_decay, local_r_cutoff;
+    /* interior resistivity eta_max_mercury [m^2/s] in 0.84 < r < 1.04,
+       applied by resist_subcycles (>= 1) explicit subcycles per step (the
+       reference's ResistiveDiffusionSubcycles_; its PETSc solve is not
+       built); no_resistance = 1 or eta 0 turns it off */
+    int no_resistance, resist_subcycles;
+    double resist_eta_max;
 } mhd_dipole_desc;
 
 typedef struct mhd_host_case mhd_host_case;
@@ -238,9 +244,9 @@ int mhd_host_constants(mhd_host_case *h, double *out22);
 /* two-fluid constants: model, ambipolar, M_Na, state_coeff_Na, qinfs[5],
    momentum_induce_coeff, a1_Na, a4, a5, a6, T_ref, Tn0, pe_coef, hall,
    hall_coef, hall_rmin, hall_rmax, local_arti, loc_eta_max, loc_center[3],
-   loc_r_decay, loc_r_cutoff (28 doubles;
+   loc_r_decay, loc_r_cutoff, resist, resist_nsub, inver_Rem (31 doubles;
    Z5_Mercury/src/physics/MercurySolver_Tools.cpp:3-46, 9999setup/3_MHD_Control.txt) */
-int mhd_host_constants_two_fluid(mhd_host_case *h, double *out28);
+int mhd_host_constants_two_fluid(mhd_host_case *h, double *out31);
 /* apply one halo / boundary op list (index as in the library's sync table:
    0 Eface BC, 1 Eface halo, 2 Eedge BC, 3-5 Ucell BC0/BC1/halo, 6-7 Bface
    BC/halo, 8 J_cell BC, 9 corner velocities, 10-11 B_cell BC/halo and

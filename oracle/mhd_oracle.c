@@ -91,6 +91,7 @@ static const struct
     {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0}, {L_CELL, 18, 0}, {L_CELL, 36, 0},
     {L_CELL, 5, 0}, {L_CELL, 5, 0}, {L_CELL, 5, 1}, {L_CELL, 1, 0}, {L_CELL, 1, 0},
     {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0},
+    {L_EXI, 1, 0}, {L_EET, 1, 0}, {L_EZE, 1, 0}, {L_FXI, 1, 1}, {L_FET, 1, 1}, {L_FZE, 1, 1},
 };
 /* g0 = 1 marks fields the reference registers with zero ghosts
    (Z4_Lunar/main.cpp:107-121); their interior range is what matters. */
@@ -625,6 +626,9 @@ static void physical_phase(or_ctx *c, int fid)
             case OF_EXI: case OF_EET: case OF_EZE:
                 if (bc == 1) bc_eedge_farfield(c, B, F, f, g, fid - OF_EXI); /* solid: nop */
                 break;
+            case OF_ERXI: case OF_ERET: case OF_ERZE: /* Z5_Mercury/src/boundary/Boundary_Build.cpp:200-205 */
+                if (bc == 1) bc_eedge_farfield(c, B, F, f, g, fid - OF_ERXI);
+                break;
             case OF_EFXI: case OF_EFET: case OF_EFZE:
                 if (bc == 2) bc_eface_solid(F, f, g); /* far field: nop */
                 break;
@@ -653,6 +657,7 @@ static void sync_fields(or_ctx *c, const int *fids, int n, int do_physical)
         if (fid >= OF_EXI && fid <= OF_EZE) { base = OF_EXI; triple = 1; }
         if (fid >= OF_EFXI && fid <= OF_EFZE) { base = OF_EFXI; triple = 1; eface = 1; }
         if (fid >= OF_EHXI && fid <= OF_EHZE) { base = OF_EHXI; triple = 1; }
+        if (fid >= OF_ERXI && fid <= OF_ERZE) { base = OF_ERXI; triple = 1; }
         halo_apply(c, loc, base, triple, eface);
     }
 }
@@ -662,7 +667,7 @@ int or_sync(or_ctx *c, const char *group)
     static const int fU[] = {OF_UH, OF_UNA}, fB[] = {OF_BXI, OF_BET, OF_BZE}, fBadd[] = {OF_BADDXI, OF_BADDET, OF_BADDZE};
     static const int fEf[] = {OF_EFXI, OF_EFET, OF_EFZE}, fE[] = {OF_EXI, OF_EET, OF_EZE};
     static const int fBc[] = {OF_BCELL, OF_BIND}, fJ[] = {OF_JXI, OF_JET, OF_JZE}, fJc[] = {OF_JCELL};
-    static const int fUp[] = {OF_UPLUS}, fEh[] = {OF_EHXI, OF_EHET, OF_EHZE};
+    static const int fUp[] = {OF_UPLUS}, fEh[] = {OF_EHXI, OF_EHET, OF_EHZE}, fEr[] = {OF_ERXI, OF_ERET, OF_ERZE};
     /* the Mercury Ucell group holds U_H and U_Na (Z5_Mercury/src/boundary/Boundary_Build.cpp:571-578) */
     if (!strcmp(group, "Ucell")) sync_fields(c, fU, c->d.model == 1 ? 2 : 1, 1);
     else if (!strcmp(group, "Bface")) sync_fields(c, fB, 3, 1);
@@ -674,6 +679,7 @@ int or_sync(or_ctx *c, const char *group)
     else if (!strcmp(group, "J_cell")) sync_fields(c, fJc, 1, 1);
     else if (!strcmp(group, "Uplus")) sync_fields(c, fUp, 1, 0);
     else if (!strcmp(group, "Ehall")) sync_fields(c, fEh, 3, 1);
+    else if (!strcmp(group, "Eres")) sync_fields(c, fEr, 3, 1);
     else return -1;
     return 0;
 }
@@ -1643,13 +1649,13 @@ static void edge_emf(or_ctx *c)
 }
 
 /* CTOperators::CurlEdgeToFace (Z4_Lunar/src/operators/CTOperators.cpp:6-56) */
-static void curl_edge_to_face(or_ctx *c, double mult)
+static void curl_edges(or_ctx *c, double mult, int fe, int fr)
 {
     for (int b = 0; b < c->d.nblocks; ++b)
     {
         oblock *B = &c->b[b];
-        fblk *Ex = &B->f[OF_EXI], *Ey = &B->f[OF_EET], *Ez = &B->f[OF_EZE];
-        fblk *Rx = &B->f[OF_RHSBXI], *Ry = &B->f[OF_RHSBET], *Rz = &B->f[OF_RHSBZE];
+        fblk *Ex = &B->f[fe], *Ey = &B->f[fe + 1], *Ez = &B->f[fe + 2];
+        fblk *Rx = &B->f[fr], *Ry = &B->f[fr + 1], *Rz = &B->f[fr + 2];
         OMP_FOR
         for (int i = 0; i < Rx->cnt[0]; ++i)
             for (int j = 0; j < Rx->cnt[1]; ++j)
@@ -1667,6 +1673,8 @@ static void curl_edge_to_face(or_ctx *c, double mult)
                     AT(Rz, i, j, k, 0) = mult * ((AT(Ex, i, j, k, 0) - AT(Ex, i, j + 1, k, 0)) + (AT(Ey, i + 1, j, k, 0) - AT(Ey, i, j, k, 0)));
     }
 }
+
+static void curl_edge_to_face(or_ctx *c, double mult) { curl_edges(c, mult, OF_EXI, OF_RHSBXI); }
 
 static void zero_fblk(fblk *F)
 {
@@ -1714,6 +1722,91 @@ static void apply_update(or_ctx *c, double dt)
     }
 }
 
+/* AddResistiveEdgeEMF_To_ (Z5_Mercury/src/physics/MercurySolver_MagExplicit_Resis.cpp:188-330):
+ * E += inver_Rem yita0(r) (J_cell averaged over the 4 cells around the edge) . dr
+ * where 0.84 < r < 1.04 at the edge midpoint. */
+static void add_resistive_emf(or_ctx *c)
+{
+    for (int b = 0; b < c->d.nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        fblk *Jc = &B->f[OF_JCELL];
+        for (int a = 0; a < 3; ++a)
+        {
+            fblk *E = &B->f[OF_ERXI + a];
+            const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+            for (int i = 0; i < E->cnt[0]; ++i)
+                for (int j = 0; j < E->cnt[1]; ++j)
+                    for (int k = 0; k < E->cnt[2]; ++k)
+                    {
+                        double m[3];
+                        edge_mid(B, a, i, j, k, m);
+                        const double r = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+                        double yita0 = 0.0;
+                        if (!(r <= 0.84 || r >= 1.04)) yita0 = 0.5 * (tanh((r - 0.84) / 0.02) - tanh((r - 1.04) / 0.02));
+                        if (yita0 == 0.0) continue;
+                        /* cells (i,j,k), minus t1, minus t2, minus both (get_edge_cells, :232-261) */
+                        int q[4][3] = {{i, j, k}, {i, j, k}, {i, j, k}, {i, j, k}};
+                        q[1][t1] -= 1;
+                        q[2][t2] -= 1;
+                        q[3][t1] -= 1;
+                        q[3][t2] -= 1;
+                        double J[3] = {0.0, 0.0, 0.0};
+                        for (int n = 0; n < 4; ++n)
+                            for (int m2 = 0; m2 < 3; ++m2) J[m2] += AT(Jc, q[n][0], q[n][1], q[n][2], m2);
+                        for (int m2 = 0; m2 < 3; ++m2) J[m2] *= 0.25;
+                        const int ip = i + (a == 0), jp = j + (a == 1), kp = k + (a == 2);
+                        const double dr0 = NODE(B, 0, ip, jp, kp) - NODE(B, 0, i, j, k);
+                        const double dr1 = NODE(B, 1, ip, jp, kp) - NODE(B, 1, i, j, k);
+                        const double dr2 = NODE(B, 2, ip, jp, kp) - NODE(B, 2, i, j, k);
+                        const double Jedge = J[0] * dr0 + J[1] * dr1 + J[2] * dr2;
+                        AT(E, i, j, k, 0) += (c->d.inver_Rem * yita0) * Jedge;
+                    }
+        }
+    }
+}
+
+int or_resistive_emf(or_ctx *c)
+{
+    for (int b = 0; b < c->d.nblocks; ++b)
+        for (int a = 0; a < 3; ++a) zero_fblk(&c->b[b].f[OF_ERXI + a]);
+    add_resistive_emf(c);
+    return 0;
+}
+
+/* ResistiveDiffusionSubcycles_ (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141) */
+static void resistive_subcycles(or_ctx *c, double dt)
+{
+    const int nsub = c->d.resist_nsub > 1 ? c->d.resist_nsub : 1;
+    const double dt_res = dt / (double)nsub;
+    for (int isub = 0; isub < nsub; ++isub)
+    {
+        or_sync(c, "Bface");
+        calc_bcell(c);
+        calc_jedge(c);
+        calc_jcell(c);
+        for (int b = 0; b < c->d.nblocks; ++b)
+            for (int a = 0; a < 3; ++a)
+            {
+                zero_fblk(&c->b[b].f[OF_ERXI + a]);
+                zero_fblk(&c->b[b].f[OF_RBRXI + a]);
+            }
+        add_resistive_emf(c);
+        or_sync(c, "Eres");
+        curl_edges(c, -1.0, OF_ERXI, OF_RBRXI);
+        /* ApplyUpdate_Euler_BfaceOnly_ (MercurySolver_MagExplicit_Hall.cpp:441-479) */
+        for (int b = 0; b < c->d.nblocks; ++b)
+            for (int a = 0; a < 3; ++a)
+            {
+                fblk *Bf = &c->b[b].f[OF_BXI + a], *R = &c->b[b].f[OF_RBRXI + a];
+                for (int i = 0; i < Bf->cnt[0]; ++i)
+                    for (int j = 0; j < Bf->cnt[1]; ++j)
+                        for (int k = 0; k < Bf->cnt[2]; ++k) AT(Bf, i, j, k, 0) += dt_res * AT(R, i, j, k, 0);
+            }
+        or_sync(c, "Bface");
+    }
+}
+
 /* StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) */
 int or_step(or_ctx *c, double *dt_used)
 {
@@ -1732,6 +1825,7 @@ int or_step(or_ctx *c, double *dt_used)
     add_source(c);
     assemble_ct(c);
     apply_update(c, dt);
+    if (c->d.model == 1 && c->d.resist) resistive_subcycles(c, dt);
     or_sync(c, "Ucell");
     or_sync(c, "Bface");
     or_update_derived(c);

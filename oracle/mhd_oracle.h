@@ -44,6 +44,10 @@ typedef struct
     int hall, local_arti;
     double hall_coef, hall_rmin, hall_rmax;
     double loc_eta_max, loc_center[3], loc_r_decay, loc_r_cutoff;
+    /* interior resistivity by explicit subcycles (ResistiveDiffusionSubcycles_,
+       Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141) */
+    int resist, resist_nsub;
+    double inver_Rem;
 } or_desc;
 
 enum
@@ -53,7 +57,8 @@ enum
     OF_EFXI, OF_EFET, OF_EFZE, OF_FXI, OF_FET, OF_FZE, OF_RHSH, OF_RHSBXI, OF_RHSBET, OF_RHSBZE,
     OF_DIVB, OF_JAC, OF_JDXI, OF_JDET, OF_JDZE, OF_BETAXI, OF_BETAET, OF_BETAZE,
     OF_ALXI, OF_ALET, OF_ALZE, OF_WB, OF_WJ,
-    OF_UNA, OF_PVNA, OF_RHSNA, OF_PHOTO, OF_DRAG, OF_EHXI, OF_EHET, OF_EHZE, OF_COUNT
+    OF_UNA, OF_PVNA, OF_RHSNA, OF_PHOTO, OF_DRAG, OF_EHXI, OF_EHET, OF_EHZE,
+    OF_ERXI, OF_ERET, OF_ERZE, OF_RBRXI, OF_RBRET, OF_RBRZE, OF_COUNT
 };
 
 typedef struct or_ctx or_ctx;
@@ -82,6 +87,8 @@ void or_use_periodic_wrap(or_ctx *c, int on);
 void or_set_threads(or_ctx *c, int n);
 /* run only the named sync group on the current arrays (tests) */
 int or_sync(or_ctx *c, const char *group);
+/* the interior-resistivity EMF of the current J_cell into Eres (tests) */
+int or_resistive_emf(or_ctx *c);
 
 #ifdef __cplusplus
 }

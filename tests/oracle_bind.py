@@ -30,7 +30,8 @@ class OrDesc(C.Structure):
                 ("a5", C.c_double), ("a6", C.c_double), ("T_ref", C.c_double), ("Tn0", C.c_double),
                 ("pe_coef", C.c_double), ("hall", C.c_int), ("local_arti", C.c_int), ("hall_coef", C.c_double),
                 ("hall_rmin", C.c_double), ("hall_rmax", C.c_double), ("loc_eta_max", C.c_double),
-                ("loc_center", C.c_double * 3), ("loc_r_decay", C.c_double), ("loc_r_cutoff", C.c_double)]
+                ("loc_center", C.c_double * 3), ("loc_r_decay", C.c_double), ("loc_r_cutoff", C.c_double),
+                ("resist", C.c_int), ("resist_nsub", C.c_int), ("inver_Rem", C.c_double)]
 
 
 _lib = None
@@ -58,6 +59,7 @@ def olib():
         L.or_use_periodic_wrap.argtypes = [C.c_void_p, C.c_int]
         L.or_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.or_sync.argtypes = [C.c_void_p, C.c_char_p]
+        L.or_resistive_emf.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -96,6 +98,7 @@ class Oracle:
         for a in range(3):
             d.loc_center[a] = t[23 + a]
         d.loc_r_decay, d.loc_r_cutoff = t[26], t[27]
+        d.resist, d.resist_nsub, d.inver_Rem = int(t[28]), int(t[29]), t[30]
         L = olib()
         self.c = L.or_create(C.byref(d))
         self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks
@@ -150,3 +153,6 @@ class Oracle:
 
     def sync(self, group):
         return olib().or_sync(self.c, group.encode())
+
+    def resistive_emf(self):
+        return olib().or_resistive_emf(self.c)

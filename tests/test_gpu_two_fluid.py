@@ -48,6 +48,21 @@ def test_two_fluid_fp64_bitwise_equals_oracle(kw, recon, steps):
     assert ctx.divb_max(cid) < 1e-9
 
 
+@pytest.mark.parametrize("nsub", [1, 3])
+def test_two_fluid_interior_resistivity_bitwise_equals_oracle(nsub):
+    """A shell reaching into the resistive layer 0.84 < r < 1.04, explicit
+    subcycles of the interior resistivity (ResistiveDiffusionSubcycles_)."""
+    desc = MercuryDesc(nang=8, nrad=48, r_in=0.85, resist_subcycles=nsub)
+    print(f"two-fluid resistive nsub={nsub}", flush=True)
+    hc, orc, ctx, cid, dts = pair(desc, 4)
+    assert ctx.last_dt(cid) == dts[-1]
+    for name in FIELDS:
+        sl = interior(name, hc.n, G)
+        for b in range(6):
+            gpu, ref = ctx.field(cid, name, b)[sl], orc.view(name, b)[sl]
+            assert np.array_equal(gpu, ref), (name, b, float(np.abs(gpu - ref).max()))
+
+
 def test_single_fluid_mercury_matches_oracle():
     desc = MercuryDesc(nang=8, nrad=16, two_fluid=False)
     hc, orc, ctx, cid, dts = pair(desc, 6)

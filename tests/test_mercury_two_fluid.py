@@ -256,3 +256,42 @@ def test_hall_emf_central_term_is_taper_times_alpha_j_cross_b_dot_dr():
             tol = 1e-13 * max(np.abs(E0).max(), np.abs(want).max())
             assert np.abs(dE[sl] - want[sl]).max() <= tol + 1e-9 * np.abs(want[sl]).max(), (b, nm)
             assert np.abs(want[sl]).max() > 1e3 * tol
+
+
+def test_interior_resistivity_emf_follows_the_profile():
+    """Eres = inver_Rem yita0(r) (J_cell averaged over the 4 cells around the
+    edge) . dr, yita0 = (tanh((r-0.84)/0.02) - tanh((r-1.04)/0.02)) / 2 inside
+    0.84 < r < 1.04, on a shell reaching into the resistive layer."""
+    hc = HostCase(MercuryDesc(nang=6, nrad=48, r_in=0.85), ghost=G)
+    rng = np.random.default_rng(5)
+    for b in range(hc.nblocks):
+        for f in ("B_xi", "B_eta", "B_zeta"):
+            hc.set_field(f, b, 1e-3 * rng.standard_normal(hc.field(f, b).shape))
+    for sy in (6, 7):
+        hc.apply_sync(sy)
+    t = hc.constants_two_fluid()
+    assert t[28] == 1.0 and t[30] == pytest.approx(1.25e7 / (1.256637061e-6 * 4.031e5 * 2.44e6), rel=1e-14)
+    o = Oracle(hc)
+    o.resistive_emf()
+    hit = 0
+    for b in range(hc.nblocks):
+        Jc = o.view("J_cell", b)
+        for a, nm in enumerate(("Eres_xi", "Eres_eta", "Eres_zeta")):
+            mid, dr = edge_midpoints(hc, b, a)
+            r = np.linalg.norm(mid, axis=-1)
+            inside = (r > 0.84) & (r < 1.04)
+            y = np.where(inside, 0.5 * (np.tanh((r - 0.84) / 0.02) - np.tanh((r - 1.04) / 0.02)), 0.0)
+            t1, t2 = [q for q in range(3) if q != a]
+            cnt = mid.shape[:3]
+            acc = 0.0
+            for s1 in (0, 1):
+                for s2 in (0, 1):
+                    sl = [slice(G, G + cnt[q]) for q in range(3)]
+                    sl[t1] = slice(G - s1, G - s1 + cnt[t1])
+                    sl[t2] = slice(G - s2, G - s2 + cnt[t2])
+                    acc = acc + Jc[tuple(sl)]
+            want = t[30] * y * np.einsum("...m,...m->...", 0.25 * acc, dr)
+            got = edge_view(o, nm, hc, b)
+            assert np.allclose(got, want, rtol=1e-10, atol=1e-14 * max(np.abs(want).max(), 1e-30)), (b, nm)
+            hit += int(np.count_nonzero(want))
+    assert hit > 100
