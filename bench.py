@@ -35,6 +35,12 @@ KERNEL_PARTS = {
     "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8 + 144),
     # B -> J_cell | beta, alpha, W_J
     "k_jcell": (24 + 24, 24 + 24 + 288),
+    # two-fluid step: U_H, U_Na -> U_H', U_Na', B_cell, J_cell, p_e, photoionization, drag | JD, Jac
+    "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8),
+    # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac, W_B
+    "k_derived2": (80 + 24 + 24 + 24 + 8 + 24, 72 + 8 + 144),
+    # E R/W, p_e, rho_H, rho_Na, B, B_cell, J_cell, dr, Hall taper, eta | beta, alpha, JD
+    "k_emf_extra": (48 + 8 + 16 + 24 + 24 + 24 + 72 + 24 + 24, 24 + 24 + 72),
 }
 
 
@@ -162,23 +168,30 @@ def bench_serve(args):
     reqs = poisson_requests(args.requests, 1.0, seed=SEED, steps=args.req_steps)
     mean_work = float(np.mean([r.cells for r in reqs])) * args.req_steps
     eng = ServingEngine(precision=prec, hbm_budget_gb=args.budget_gb, chunk=20, workers=2)
-    from beta1_amd import HostCase, WindDesc
-    cal = eng.ctx.adopt(HostCase(WindDesc(nang=64, nrad=128)))
-    eng.ctx.step(4, graph=True)
-    eng.ctx.sync()
-    t0 = time.perf_counter()
-    eng.ctx.step(20, graph=True)
-    eng.ctx.sync()
-    cap = WindDesc(nang=64, nrad=128).cells * 20 / (time.perf_counter() - t0)  # cell-updates/s
-    eng.ctx.retire(cal)
-    rate = args.load * cap / mean_work
+    from beta1_amd import HostCase, MercuryDesc, WindDesc
+    # capacity of each model on a 6x64^2x128 shell alone: a request's service
+    # time is its cell-updates at its model's rate
+    cap = {}
+    for name, d in (("lunar", WindDesc(nang=64, nrad=128)), ("mercury", MercuryDesc(nang=64, nrad=128))):
+        cal = eng.ctx.adopt(HostCase(d))
+        eng.ctx.step(4, graph=True)
+        eng.ctx.sync()
+        t0 = time.perf_counter()
+        eng.ctx.step(20, graph=True)
+        eng.ctx.sync()
+        cap[name] = d.cells * 20 / (time.perf_counter() - t0)  # cell-updates/s
+        eng.ctx.retire(cal)
+    service = [r.cells * args.req_steps / cap["mercury" if isinstance(r.desc, MercuryDesc) else "lunar"]
+               for r in reqs]
+    rate = args.load / float(np.mean(service))
     for r in reqs:  # exponential gaps of mean 1 -> mean 1 / rate
         r.arrival /= rate
-    log(f"serve: capacity ~{cap / 1e6:.0f} Mcell-updates/s, mean request {mean_work / 1e9:.2f} G cell-updates, "
-        f"rate {rate:.3f}/s (offered load {args.load})")
+    log(f"serve: capacity ~{cap['lunar'] / 1e6:.0f} (Lunar) / {cap['mercury'] / 1e6:.0f} (Mercury) "
+        f"Mcell-updates/s, mean request {mean_work / 1e9:.2f} G cell-updates, rate {rate:.3f}/s "
+        f"(offered load {args.load})")
     res = eng.run(reqs, log=log)
     eng.close()
-    offered = rate * mean_work / cap
+    offered = rate * float(np.mean(service))
     return _record("p50/p99 solve latency + Mcell-updates/s (serving)", res["mcell_updates_per_s"],
                    "Mcell-updates/s", args.req_steps, 0, None, "f32/f64" if prec == "mixed" else "f64",
                    {"workload": f"serve_poisson_{args.requests}req_{prec}", "requests": args.requests,
@@ -298,8 +311,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy five times (counter calibration)")
-    ap.add_argument("--workload", default="lunar", choices=["lunar", "batch", "serve", "precision", "ot"],
-                    help="lunar: one 6-block shell (default); batch: 64 seeded shells of mixed sizes in one "
+    ap.add_argument("--workload", default="lunar", choices=["lunar", "mercury", "batch", "serve", "precision", "ot"],
+                    help="lunar: one 6-block shell (default); mercury: the two-fluid Mercury shell (H+, Na+, "
+                         "Hall, ambipolar, resistive terms); batch: 64 seeded shells of mixed sizes in one "
                          "context; serve: Poisson request stream with continuous batching; precision: FP64 vs "
                          "mixed on one shell; ot: Orszag-Tang 64^3 periodic box")
     ap.add_argument("--cases", type=int, default=64)
@@ -311,8 +325,9 @@ def main():
     big = args.workload == "precision"
     args.nang = args.nang or (128 if big else 64)
     args.nrad = args.nrad or (256 if big else 128)
-    args.steps = args.steps or {"lunar": 200, "batch": 100, "serve": 0, "precision": 500, "ot": 100}[args.workload]
-    args.precision = args.precision or ("fp64" if args.workload == "lunar" else "mixed")
+    args.steps = args.steps or {"lunar": 200, "mercury": 200, "batch": 100, "serve": 0, "precision": 500,
+                                "ot": 100}[args.workload]
+    args.precision = args.precision or ("fp64" if args.workload in ("lunar", "mercury") else "mixed")
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -327,15 +342,19 @@ def main():
               "which is not built here); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload != "lunar":
+    if args.workload not in ("lunar", "mercury"):
         out = {"batch": bench_batch, "serve": bench_serve, "precision": bench_precision,
                "ot": bench_ot}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 
-    from beta1_amd import MhdContext, WindDesc
+    from beta1_amd import MercuryDesc, MhdContext, WindDesc
 
-    desc = WindDesc(nang=args.nang, nrad=args.nrad)
+    mercury = args.workload == "mercury"
+    desc = (MercuryDesc if mercury else WindDesc)(nang=args.nang, nrad=args.nrad)
+    kbytes = kernel_bytes(args.precision)
+    # whole-step algorithmic bytes: the second species adds U_Na R/W 80
+    step_bytes = STATE_BYTES + GEO_BYTES[args.precision] + (80 if mercury else 0)
     cells = desc.cells
     t0 = time.time()
     ctx = MhdContext(precision=args.precision, recon=0, ghost=2, autotune=True, xcd_order=True)
@@ -364,9 +383,9 @@ def main():
     prof = ctx.profile_read()
     ctx.profile(False)
     per_kernel = {k: (ms / max(n, 1), n) for k, (ms, n) in prof.items() if n}
-    dom = max((k for k in per_kernel if k in KERNEL_BYTES), key=lambda k: per_kernel[k][0] * per_kernel[k][1])
+    dom = max((k for k in per_kernel if k in kbytes), key=lambda k: per_kernel[k][0] * per_kernel[k][1])
     avg_ms = per_kernel[dom][0]
-    alg_bytes = KERNEL_BYTES[dom] * cells
+    alg_bytes = kbytes[dom] * cells
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     kernels_txt = {k: round(v[0], 4) for k, v in per_kernel.items()}
     print(f"[bench] per-kernel avg ms: {kernels_txt}", file=sys.stderr, flush=True)
@@ -411,16 +430,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"lunar_wake_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}",
+        "config": {"workload": f"{'mercury_two_fluid' if mercury else 'lunar_wake'}_{args.precision}_6x{args.nang}x"
+                               f"{args.nang}x{args.nrad}",
                    "cells": cells, "blocks": 6, "recon": "first-order", "cfl": desc.cfl},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(f"lunar_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}", dom),
-                     "alg_bytes_per_cell": KERNEL_BYTES[dom], "avg_ms": round(avg_ms, 4)},
+                     "traffic": measured_traffic(f"{args.workload}_{args.precision}_6x{args.nang}x{args.nang}x"
+                                                 f"{args.nrad}", dom),
+                     "alg_bytes_per_cell": kbytes[dom], "avg_ms": round(avg_ms, 4)},
         "cpu_baseline": cpu,
-        "step_hbm": {"bytes_per_cell_update": STEP_BYTES,
-                     "frac": round(value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4)},
+        "step_hbm": {"bytes_per_cell_update": step_bytes,
+                     "frac": round(value * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9), 4)},
     }
+    if mercury:
+        out["metric"] = "Mcell-updates/s (two-fluid Mercury shell)"
     print(json.dumps(out), flush=True)
 
 
