@@ -34,7 +34,8 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     jobs = []
-    for src in ("host/setup.cpp", "host/cases.cpp", "host/host_capi.cpp", "host/formats.cpp"):
+    for src in ("host/setup.cpp", "host/cases.cpp", "host/host_capi.cpp", "host/formats.cpp",
+                "host/tecplot.cpp"):
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         jobs.append(["g++", *COMMON, "-fopenmp", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

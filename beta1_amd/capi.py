@@ -52,10 +52,12 @@ SIGNATURES = {
     "mhd_add_case_from_dir": (_I, [_P, C.c_char_p, C.POINTER(MhdWindDesc), _PI]),
     "mhd_refresh": (_I, [_P]),
     "mhd_write_restart": (_I, [_P, _I, C.c_char_p, C.c_longlong, _D]),
+    "mhd_write_tecplot": (_I, [_P, _I, C.c_char_p, _D]),
     "mhd_read_restart": (_I, [_P, _I, C.c_char_p, C.POINTER(C.c_longlong), _PD]),
     "mhd_host_case_write_dir": (_I, [_P, C.c_char_p]),
     "mhd_host_case_from_dir": (_I, [C.c_char_p, C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_write_restart": (_I, [_P, C.c_char_p, C.c_longlong, _D]),
+    "mhd_host_write_tecplot": (_I, [_P, C.c_char_p, _D]),
     "mhd_host_read_restart": (_I, [_P, C.c_char_p, C.POINTER(C.c_longlong), _PD]),
     "mhd_finalize": (_I, [_P]),
     "mhd_step": (_I, [_P, _I]),
@@ -79,6 +81,7 @@ SIGNATURES = {
     "mhd_tuning": (_I, [_P, _I, _PI, _PI, _PI]),
     "mhd_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "mhd_autotune": (_I, [_P]),
+    "mhd_diagnostics": (_I, [_P, _I, _PD]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_mercury": (_I, [C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _I, C.POINTER(_P)]),

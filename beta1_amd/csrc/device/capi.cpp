@@ -151,6 +151,8 @@ struct mhd_ctx
     double *d_dt = nullptr, *d_zero = nullptr;
     unsigned long long *d_dtbits = nullptr, *d_dtscratch = nullptr, *d_divb = nullptr;
     double *d_dtres = nullptr; // per-case dt of the current resistive subcycle
+    unsigned long long *d_diag = nullptr; // 17 order keys per case (min/max diagnostics)
+    size_t cap_diag = 0;
     int max_nsub = 0;          // resistive subcycles of the resident cases (0: none)
     int *d_err = nullptr, *d_active = nullptr;
 
@@ -258,6 +260,7 @@ extern "C" int mhd_destroy(mhd_ctx *c)
     free_t(c->d_dtscratch);
     free_t(c->d_divb);
     free_t(c->d_dtres);
+    free_t(c->d_diag);
     free_t(c->d_err);
     free_t(c->d_active);
     (void)hipStreamDestroy(c->q);
@@ -1160,6 +1163,36 @@ extern "C" int mhd_divb_max(mhd_ctx *c, int case_id, double *out)
     return MHD_OK;
 }
 
+static unsigned long long order_key_host(double x)
+{
+    unsigned long long b;
+    std::memcpy(&b, &x, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+extern "C" int mhd_diagnostics(mhd_ctx *c, int case_id, double *out)
+{
+    if (!out || !active_case(c, case_id))
+        return MHD_ERR_ARG;
+    int rc = grow(c, c->d_diag, c->cap_diag, (size_t)17 * c->case_cap);
+    if (rc)
+        return rc;
+    std::vector<unsigned long long> init((size_t)17 * c->case_cap);
+    for (size_t q = 0; q < init.size(); ++q)
+        init[q] = order_key_host((q % 17) < 8 ? HUGE_VAL : -HUGE_VAL);
+    HIPCHK(c, hipMemcpyAsync(c->d_diag, init.data(), 8 * init.size(), hipMemcpyHostToDevice, c->q));
+    mhdk::launch_diag(launch_of(c, KP_DT), c->d_cst, c->d_diag, c->ucur, c->q);
+    unsigned long long keys[17];
+    HIPCHK(c, hipMemcpyAsync(keys, c->d_diag + 17 * (size_t)case_id, sizeof(keys), hipMemcpyDeviceToHost, c->q));
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    for (int q = 0; q < 17; ++q)
+    {
+        const unsigned long long b = (keys[q] >> 63) ? (keys[q] & 0x7fffffffffffffffull) : ~keys[q];
+        std::memcpy(&out[q], &b, 8);
+    }
+    return MHD_OK;
+}
+
 // ----------------------------------------------------------- data exchange
 static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, size_t n, bool down)
 {
@@ -1444,6 +1477,24 @@ extern "C" int mhd_write_restart(mhd_ctx *c, int case_id, const char *path, long
     {
         DeviceIO io(c, case_id, r);
         return restart_write(io, path, step, time);
+    }
+    catch (const std::exception &e)
+    {
+        return fail(c, MHD_ERR_ARG, e.what());
+    }
+}
+
+extern "C" int mhd_write_tecplot(mhd_ctx *c, int case_id, const char *path, double time)
+{
+    if (!c || !path)
+        return MHD_ERR_ARG;
+    CaseRec *r = active_case(c, case_id);
+    if (!r)
+        return fail(c, MHD_ERR_STATE, "case not active (or context not finalized)");
+    try
+    {
+        DeviceIO io(c, case_id, r);
+        return tecplot_write(io, r->cs.grid, r->cs.cst, r->cs.L_ref, path, time);
     }
     catch (const std::exception &e)
     {

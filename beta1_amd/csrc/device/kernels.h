@@ -40,4 +40,6 @@ void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int
 void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);
+// 17 order keys per case (min/max diagnostics), set to the neutral keys first
+void launch_diag(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, hipStream_t q);
 } // namespace mhdk

@@ -928,7 +928,9 @@ __global__ __launch_bounds__(256) void k_fluid2(const MhdTileMap tm, const MhdCa
     const double *G = dmetric(b);
     const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
     double rhs[2][5];
-#pragma unroll
+    // Kept rolled: unrolling the two species doubles the live flux state
+    // (256 VGPRs, occupancy 2) and measured 22% slower on the 6x64x64x128 shell.
+#pragma unroll 1
     for (int sp = 0; sp < 2; ++sp)
     {
         const double *U = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * ucur) * sl;
@@ -1368,6 +1370,89 @@ __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long
     }
 }
 
+// Min/max diagnostics of PrintMinMaxDiagnostics_ (Z4_Lunar/src/tools/LunarSolver_PrintDiag.cpp:7-60,
+// Z5_Mercury/src/tools/MercurySolver_PrintDiag.cpp:7-92) over interior cells:
+// mins of rho_H, p_H, rho_Na, p_Na, Bx, By, Bz, |B|^2/2, then maxes of the
+// same and of |div B| (17 values per case).  Doubles are reduced as
+// order-preserving 64-bit keys with atomicMin / atomicMax.
+__device__ __forceinline__ unsigned long long order_key(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ __launch_bounds__(256) void k_diag(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                              unsigned long long *__restrict__ out, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    const bool ok = tile_cell(tm, b, i, j, k, wg);
+    double v[17];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        v[q] = __longlong_as_double(0x7ff0000000000000ll);      // +inf
+        v[8 + q] = __longlong_as_double((long long)0xfff0000000000000ull); // -inf
+    }
+    v[16] = __longlong_as_double((long long)0xfff0000000000000ull);
+    if (ok)
+    {
+        const MhdCaseConst &c = cst[b.case_id];
+        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const double *S = dst_ptr(b);
+        const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
+        const Prim ph = prim_of(U, sl, o, c.gamma);
+        double x[8] = {U[o], ph.p, v[2], v[3], 0.0, 0.0, 0.0, 0.0};
+        if (b.model == 1)
+        {
+            const double *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+            x[2] = UN[o];
+            x[3] = prim_of(UN, sl, o, c.gamma).p;
+        }
+        const double *Bc = S + (int64_t)ST_BCELL * sl;
+        x[4] = Bc[o];
+        x[5] = Bc[sl + o];
+        x[6] = Bc[2 * sl + o];
+        x[7] = 0.5 * (x[4] * x[4] + x[5] * x[5] + x[6] * x[6]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            if (b.model != 1 && (q == 2 || q == 3))
+                continue;
+            v[q] = dmin(v[q], x[q]);
+            v[8 + q] = dmax(v[8 + q], x[q]);
+        }
+        const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+        const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
+                          dmetric(b)[(int64_t)G_JAC * sl + o];
+        v[16] = dmax(v[16], fabs(dv));
+    }
+#pragma unroll
+    for (int q = 0; q < 17; ++q)
+    {
+        double m = v[q];
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1)
+        {
+            const double w = __shfl_xor(m, s, 64);
+            m = q < 8 ? dmin(m, w) : dmax(m, w);
+        }
+        v[q] = m;
+    }
+    if (threadIdx.x == 0 && wg)
+    {
+#pragma unroll
+        for (int q = 0; q < 17; ++q)
+        {
+            if (q < 8)
+                atomicMin(&out[17 * b.case_id + q], order_key(v[q]));
+            else
+                atomicMax(&out[17 * b.case_id + q], order_key(v[q]));
+        }
+    }
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1490,6 +1575,10 @@ void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_
 {
     (void)mixed;
     hipLaunchKernelGGL(k_divb, grid_of(L), block_of(L), 0, q, L.tm, out);
+}
+void launch_diag(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_diag, grid_of(L), block_of(L), 0, q, L.tm, cst, out, ucur);
 }
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
 {

@@ -308,7 +308,7 @@ CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gam
 
 // Cell centre as the reference's dual grid sums it (Block::calc_Dual_Grids,
 // src/1_grid/1_MPCNS_Grid_GCL.cpp:6-23): dual(i+1, j+1, k+1) of cell (i, j, k).
-static void dual_centre(const Block &B, int i, int j, int k, double out[3])
+void dual_centre(const Block &B, int i, int j, int k, double out[3])
 {
     const int I = i + 1, J = j + 1, K = k + 1;
     const int64_t o[8] = {B.node_off(I, J, K),         B.node_off(I - 1, J, K),     B.node_off(I, J - 1, K),
@@ -322,13 +322,13 @@ static void dual_centre(const Block &B, int i, int j, int k, double out[3])
     }
 }
 
-// Na+ photoionization production rate [cm^-3 s^-1] at a cell centre: the
-// neutral Na exosphere of Exner et al. (2020, Table 2) -- a radially
-// symmetric part plus Gaussian bells for thermal desorption, photon-stimulated
-// desorption and sputtering -- times the photoionization frequency, lower in
-// the geometric shadow (Mercury_Initial::Initial_Na_And_PhotoRate_,
-// Z5_Mercury/include/2_Initial.h:162-392).
-double mercury_photo_rate(const double x[3], double L_ref)
+// Neutral Na density [cm^-3] at a cell centre: the exosphere of Exner et al.
+// (2020, Table 2) -- a radially symmetric part plus Gaussian bells for
+// thermal desorption, photon-stimulated desorption and sputtering; the Na+
+// photoionization production rate [cm^-3 s^-1] is that density times the
+// photoionization frequency, lower in the geometric shadow
+// (Mercury_Initial::Initial_Na_And_PhotoRate_, Z5_Mercury/include/2_Initial.h:162-392).
+double mercury_na_density(const double x[3], double L_ref)
 {
     const double r_nd = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     if (!(r_nd > 1.0) || !std::isfinite(r_nd))
@@ -355,7 +355,15 @@ double mercury_photo_rate(const double x[3], double L_ref)
     double n_m3 = (1.0 / (r_nd * r_nd)) * (n_miv + n_td + n_psd + n_sp_day + n_sp_night);
     if (!std::isfinite(n_m3) || n_m3 < 0.0)
         n_m3 = 0.0;
-    const double n_cm3 = n_m3 * 1.0e-6;
+    return n_m3 * 1.0e-6;
+}
+
+double mercury_photo_rate(const double x[3], double L_ref)
+{
+    const double r_nd = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (!(r_nd > 1.0) || !std::isfinite(r_nd))
+        return 0.0;
+    const double n_cm3 = mercury_na_density(x, L_ref);
     const bool shadow = x[0] < 0.0 && std::sqrt(x[1] * x[1] + x[2] * x[2]) < 1.0;
     double q = (shadow ? 1.0e-5 : 5.0e-5) * n_cm3;
     if (!std::isfinite(q) || q < 0.0)
@@ -388,6 +396,7 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
         double U = 4.031e5, T = 1.7407e5, n = 4e7, M = 0.001, B = 1e-9, L = 2.44e6;
         double NA = 6.02e23, R = 8.314, mu0 = 1.256637061e-6;
     } R;
+    cs.L_ref = R.L;
     MhdCaseConst &c = cs.cst;
     const double rho_ref = R.M * R.n / R.NA;
     const double M_H = 0.001 * w.m_amu;

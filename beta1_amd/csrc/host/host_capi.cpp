@@ -492,21 +492,28 @@ struct HostIO : FieldIO
     explicit HostIO(mhd_host_case *h_) : h(h_) {}
     int nblocks() override { return (int)h->cs.blk.size(); }
     const MhdBlockDev &block(int b) override { return h->cs.blk[b].d; }
-    int get(int b, int field, std::vector<double> &aos) override
+    // state slots, or geometry slots (region 1)
+    double *base(int b, int field, int &loc, int &nc)
     {
-        int region, slot, loc, nc;
+        int region, slot;
         field_slot(field, 0, region, slot, loc, nc);
         BlockArrays &A = h->cs.blk[b];
-        aos.assign(aos_count(A.d, loc, nc), 0.0);
-        soa_aos(A.d, loc, nc, &A.st[(size_t)slot * A.d.slot], aos.data(), true);
+        return (region == 1 ? A.geo.data() : A.st.data()) + (size_t)slot * A.d.slot;
+    }
+    int get(int b, int field, std::vector<double> &aos) override
+    {
+        int loc, nc;
+        double *p = base(b, field, loc, nc);
+        const MhdBlockDev &d = h->cs.blk[b].d;
+        aos.assign(aos_count(d, loc, nc), 0.0);
+        soa_aos(d, loc, nc, p, aos.data(), true);
         return MHD_OK;
     }
     int put(int b, int field, const std::vector<double> &aos) override
     {
-        int region, slot, loc, nc;
-        field_slot(field, 0, region, slot, loc, nc);
-        BlockArrays &A = h->cs.blk[b];
-        soa_aos(A.d, loc, nc, &A.st[(size_t)slot * A.d.slot], const_cast<double *>(aos.data()), false);
+        int loc, nc;
+        double *p = base(b, field, loc, nc);
+        soa_aos(h->cs.blk[b].d, loc, nc, p, const_cast<double *>(aos.data()), false);
         return MHD_OK;
     }
 };
@@ -554,6 +561,22 @@ extern "C" int mhd_host_write_restart(mhd_host_case *h, const char *path, long l
     {
         HostIO io(h);
         return restart_write(io, path, step, time);
+    }
+    catch (const std::exception &e)
+    {
+        h->err = e.what();
+        return MHD_ERR_ARG;
+    }
+}
+
+extern "C" int mhd_host_write_tecplot(mhd_host_case *h, const char *path, double time)
+{
+    if (!h || !path || h->cs.blk.empty() || h->cs.blk[0].st.empty())
+        return MHD_ERR_ARG;
+    try
+    {
+        HostIO io(h);
+        return tecplot_write(io, h->cs.grid, h->cs.cst, h->cs.L_ref, path, time);
     }
     catch (const std::exception &e)
     {

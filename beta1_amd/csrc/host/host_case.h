@@ -32,4 +32,7 @@ struct FieldIO
 };
 int restart_write(FieldIO &io, const char *path, long long step, double time);
 int restart_read(FieldIO &io, const char *path, long long *step, double *time);
+// Tecplot binary output in the reference's Mixed mode; throws on I/O errors.
+int tecplot_write(FieldIO &io, const Grid &grd, const MhdCaseConst &cst, double L_ref, const char *path,
+                  double time);
 } // namespace mpcns

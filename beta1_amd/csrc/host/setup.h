@@ -133,6 +133,7 @@ struct CaseSetup
     // (block * ST_NSLOT * slot + state_slot * slot + blk_off)
     std::vector<std::vector<MhdOp>> ops;
     std::string shape_key; // identical keys share geometry
+    double L_ref = 0.0;    // body radius [m] of a Mercury case (neutral exosphere for output)
 };
 
 // ---------------------------------------------------------------------------
@@ -210,6 +211,11 @@ struct DipoleParams
 
 CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
                             const WindParams &w, const DipoleParams &dp);
+// Mercury exosphere: neutral Na density [cm^-3] and Na+ photoionization rate
+// at a point (body radii), and the reference's dual-grid cell centre.
+double mercury_na_density(const double x[3], double L_ref);
+double mercury_photo_rate(const double x[3], double L_ref);
+void dual_centre(const Block &B, int i, int j, int k, double out[3]);
 CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t seed, double noise);
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma,
                           double cfl, const WindParams &w);

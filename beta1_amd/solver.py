@@ -194,6 +194,10 @@ class HostCase:
         _check(lib().mhd_host_read_restart(self.h, path.encode(), C.byref(st), C.byref(t)), what="read_restart")
         return st.value, t.value
 
+    def write_tecplot(self, path, time=0.0):
+        """Tecplot binary (.plt) of the case in the reference's Mixed mode."""
+        _check(lib().mhd_host_write_tecplot(self.h, path.encode(), time), what="write_tecplot")
+
     def apply_sync(self, sync):
         _check(lib().mhd_host_apply_sync(self.h, sync), what="apply_sync")
 
@@ -308,6 +312,10 @@ class MhdContext:
                "read_restart")
         return st.value, t.value
 
+    def write_tecplot(self, cid, path, time=0.0):
+        """Tecplot binary (.plt) of case ``cid`` in the reference's Mixed mode."""
+        _check(lib().mhd_write_tecplot(self.ctx, cid, path.encode(), time), self.ctx, "write_tecplot")
+
     def info(self, cid):
         nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
         _check(lib().mhd_case_info(self.ctx, cid, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g)),
@@ -334,6 +342,15 @@ class MhdContext:
         v = C.c_double()
         _check(lib().mhd_last_dt(self.ctx, cid, C.byref(v)), self.ctx, "last_dt")
         return v.value
+
+    def diagnostics(self, cid):
+        """Min/max of rho_H, p_H, rho_Na, p_Na, Bx, By, Bz, |B|^2/2 and max |div B|."""
+        out = np.zeros(17)
+        _check(lib().mhd_diagnostics(self.ctx, cid, _dptr(out)), self.ctx, "diagnostics")
+        names = ["rhoH", "pH", "rhoNa", "pNa", "Bx", "By", "Bz", "Pmag"]
+        d = {n: (out[q], out[8 + q]) for q, n in enumerate(names)}
+        d["divB_max"] = out[16]
+        return d
 
     def divb_max(self, cid):
         v = C.c_double()

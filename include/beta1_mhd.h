@@ -157,6 +157,13 @@ int mhd_finalize(mhd_ctx *ctx);
 int mhd_retire(mhd_ctx *ctx, int case_id);
 /* Steps a case has been advanced by. */
 int mhd_case_steps(mhd_ctx *ctx, int case_id, long long *steps);
+/* Min/max diagnostics over the interior cells of a case
+   (PrintMinMaxDiagnostics_, Z4_Lunar/src/tools/LunarSolver_PrintDiag.cpp:7-60,
+   Z5_Mercury/src/tools/MercurySolver_PrintDiag.cpp:7-92): out[0..7] minima
+   and out[8..15] maxima of rho_H, p_H, rho_Na, p_Na, B_cell x, y, z and
+   |B_cell|^2/2, out[16] max |div B| (Na entries +inf / -inf for a single
+   fluid). */
+int mhd_diagnostics(mhd_ctx *ctx, int case_id, double *out17);
 
 /* Advance every case by nsteps explicit steps, each case with its own dt
    (LunarSolver::StepOnce, Z4_Lunar/src/LunarSolver_Driver.cpp:21-92).
@@ -183,6 +190,13 @@ int mhd_refresh(mhd_ctx *ctx);
    the file, copies the overlap and calls mhd_refresh. */
 int mhd_write_restart(mhd_ctx *ctx, int case_id, const char *path, long long step, double time);
 int mhd_read_restart(mhd_ctx *ctx, int case_id, const char *path, long long *step, double *time);
+/* Tecplot binary (#!TDV112) of a case in the reference's Mixed mode
+   (IOModule::WriteTecplotBinFile, src/5_io/IOModule_Tecplot.cpp:320-579;
+   field lists of Z4_Lunar/src/LunarSolver.cpp:43-64 and
+   Z5_Mercury/src/MercurySolver.cpp:36-65): nodal x, y, z, then cell
+   u_H..T_H, [u_Na..T_Na], Bx..Bz, bx_ind..bz_ind, [n_Na], Jx..Jz as float32,
+   one zone "Zone<b>_Fluid" per block, `time` as the zone's solution time. */
+int mhd_write_tecplot(mhd_ctx *ctx, int case_id, const char *path, double time);
 
 /* Introspection and data exchange (FieldBlock::raw_data round trip,
    include/3_field/1_Field_Block.h:47). */
@@ -233,6 +247,8 @@ int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, int ghost, m
 /* MPCNSRST restart of the host arrays (read also re-syncs U and B ghosts). */
 int mhd_host_write_restart(mhd_host_case *h, const char *path, long long step, double time);
 int mhd_host_read_restart(mhd_host_case *h, const char *path, long long *step, double *time);
+/* Tecplot binary of the host arrays, as mhd_write_tecplot. */
+int mhd_host_write_tecplot(mhd_host_case *h, const char *path, double time);
 int mhd_host_case_destroy(mhd_host_case *h);
 int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
 int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);

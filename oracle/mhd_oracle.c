@@ -1832,6 +1832,37 @@ int or_step(or_ctx *c, double *dt_used)
     return 0;
 }
 
+/* PrintMinMaxDiagnostics_ (Z4_Lunar/src/tools/LunarSolver_PrintDiag.cpp:7-60,
+ * Z5_Mercury/src/tools/MercurySolver_PrintDiag.cpp:7-92) */
+void or_diagnostics(or_ctx *c, double *out)
+{
+    for (int q = 0; q < 8; ++q) { out[q] = INFINITY; out[8 + q] = -INFINITY; }
+    out[16] = -INFINITY;
+    for (int b = 0; b < c->d.nblocks; ++b)
+    {
+        oblock *B = &c->b[b];
+        fblk *UH = &B->f[OF_UH], *PH = &B->f[OF_PV], *UN = &B->f[OF_UNA], *PN = &B->f[OF_PVNA], *Bc = &B->f[OF_BCELL];
+        fblk *Bx = &B->f[OF_BXI], *By = &B->f[OF_BET], *Bz = &B->f[OF_BZE], *Jac = &B->f[OF_JAC];
+        for (int k = 0; k < B->n[2]; ++k)
+            for (int j = 0; j < B->n[1]; ++j)
+                for (int i = 0; i < B->n[0]; ++i)
+                {
+                    const double bx = AT(Bc, i, j, k, 0), by = AT(Bc, i, j, k, 1), bz = AT(Bc, i, j, k, 2);
+                    const double x[8] = {AT(UH, i, j, k, 0), AT(PH, i, j, k, 3), AT(UN, i, j, k, 0), AT(PN, i, j, k, 3),
+                                         bx, by, bz, 0.5 * (bx * bx + by * by + bz * bz)};
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        if (c->d.model != 1 && (q == 2 || q == 3)) continue;
+                        out[q] = dmin(out[q], x[q]);
+                        out[8 + q] = dmax(out[8 + q], x[q]);
+                    }
+                    const double dv = (AT(Bx, i + 1, j, k, 0) - AT(Bx, i, j, k, 0) + AT(By, i, j + 1, k, 0) - AT(By, i, j, k, 0) +
+                                       AT(Bz, i, j, k + 1, 0) - AT(Bz, i, j, k, 0)) / AT(Jac, i, j, k, 0);
+                    out[16] = dmax(out[16], fabs(dv));
+                }
+    }
+}
+
 double or_divb_max(or_ctx *c)
 {
     double mx = 0.0;

@@ -79,6 +79,9 @@ int or_compute_dt(or_ctx *c, double *dt);
 int or_step(or_ctx *c, double *dt_used);
 /* max |div B| over interior cells (calc_divB, LunarSolver_Tools.cpp:685-775) */
 double or_divb_max(or_ctx *c);
+/* PrintMinMaxDiagnostics_: minima [0..7] and maxima [8..15] of rho_H, p_H,
+   rho_Na, p_Na, Bx, By, Bz, |B|^2/2 over interior cells, [16] max |div B| */
+void or_diagnostics(or_ctx *c, double *out17);
 /* 1: the single-block fully periodic halo uses an index wrap instead of
    the generic interface resolution */
 void or_use_periodic_wrap(or_ctx *c, int on);

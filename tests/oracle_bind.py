@@ -60,6 +60,7 @@ def olib():
         L.or_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.or_sync.argtypes = [C.c_void_p, C.c_char_p]
         L.or_resistive_emf.argtypes = [C.c_void_p]
+        L.or_diagnostics.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -153,6 +154,11 @@ class Oracle:
 
     def sync(self, group):
         return olib().or_sync(self.c, group.encode())
+
+    def diagnostics(self):
+        out = np.zeros(17)
+        olib().or_diagnostics(self.c, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def resistive_emf(self):
         return olib().or_resistive_emf(self.c)

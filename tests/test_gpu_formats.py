@@ -55,3 +55,22 @@ def test_restart_continues_bitwise(tmp_path, desc, names):
     for k in ref:
         assert np.array_equal(ref[k], got[k]), k
     assert one.last_dt(c1) == two.last_dt(c2)
+
+
+@pytest.mark.parametrize("desc", [WindDesc(nang=8, nrad=16), MercuryDesc(nang=8, nrad=16)])
+def test_tecplot_of_the_device_state_equals_host_writer(tmp_path, desc):
+    """The device context's .plt after steps is byte-identical to the host
+    writer's on the same state (downloaded into a host case)."""
+    ctx = MhdContext()
+    cid = ctx.add(desc)
+    ctx.finalize()
+    ctx.step(4)
+    dev = str(tmp_path / "dev.plt")
+    ctx.write_tecplot(cid, dev, time=0.75)
+    hc = HostCase(desc)
+    names = ("U_H", "B_xi", "B_eta", "B_zeta", "B_cell", "J_cell") + (("U_Na",) if isinstance(desc, MercuryDesc) else ())
+    for (f, b), a in fields(ctx, cid, names).items():
+        hc.set_field(f, b, a)
+    host = str(tmp_path / "host.plt")
+    hc.write_tecplot(host, time=0.75)
+    assert open(dev, "rb").read() == open(host, "rb").read()

@@ -114,3 +114,18 @@ def test_models_batched_each_as_alone():
         for cid, ref in zip(ids, alone):
             for (n, b), a in ref.items():
                 assert np.array_equal(ctx.field(cid, n, b), a), (shared, cid, n, b)
+
+
+@pytest.mark.parametrize("desc", [WindDesc(nang=8, nrad=16), MercuryDesc(nang=8, nrad=16)])
+def test_min_max_diagnostics_equal_oracle(desc):
+    hc, orc, ctx, cid, dts = pair(desc, 5)
+    d = ctx.diagnostics(cid)
+    ref = orc.diagnostics()
+    names = ["rhoH", "pH", "rhoNa", "pNa", "Bx", "By", "Bz", "Pmag"]
+    for q, n in enumerate(names):
+        assert d[n] == (ref[q], ref[8 + q]), n
+    assert d["divB_max"] == ref[16]
+    if isinstance(desc, MercuryDesc):
+        assert np.isfinite(d["rhoNa"][0]) and d["rhoNa"][1] > d["rhoNa"][0]
+    else:
+        assert d["rhoNa"] == (np.inf, -np.inf)
