@@ -22,42 +22,43 @@ sys.path.insert(0, ROOT)
 
 # Algorithmic HBM bytes per interior cell and launch for each step kernel:
 # every array the kernel must touch counted once at its stored width, as
-# (per-case state, geometry in FP64); mixed precision stores geometry as
-# float (half the geometry bytes).
+# (per-case state, metric and node coordinates (always FP64), Hodge weights
+# and weight tables (FP64, or float in mixed precision: half)).
 KERNEL_PARTS = {
-    "k_dt": (40 + 24, 8 + 72),                     # U, B_cell | Jac, JD
-    "k_ucontra": (40 + 24, 72 + 8),                # U -> UC | JD, Jac
-    "k_face_emf": (24 + 24 + 24 + 72, 0),          # B, Badd, UC -> Eface
-    "k_edge_emf": (72 + 24 + 24 + 24, 0),          # Eface, UC, B -> E
-    "k_fluid": (40 + 24 + 24 + 40, 72 + 8),        # U, B_cell, J_cell -> U' | JD, Jac
-    "k_curl_update": (24 + 24 + 24, 0),            # E, B -> B
-    # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac, W_B
-    "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8 + 144),
-    # B -> J_cell | beta, alpha, W_J
-    "k_jcell": (24 + 24, 24 + 24 + 288),
+    "k_dt": (40 + 24, 8 + 72, 0),                  # U, B_cell | Jac, JD
+    "k_ucontra": (40 + 24, 72 + 8, 0),             # U -> UC | JD, Jac
+    "k_face_emf": (24 + 24 + 24 + 72, 0, 0),       # B, Badd, UC -> Eface
+    "k_edge_emf": (72 + 24 + 24 + 24, 0, 0),       # Eface, UC, B -> E
+    "k_fluid": (40 + 24 + 24 + 40, 72 + 8, 0),     # U, B_cell, J_cell -> U' | JD, Jac
+    "k_curl_update": (24 + 24 + 24, 0, 0),         # E, B -> B
+    # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | W_B
+    "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8, 144),
+    # B -> J_cell | nodes | beta, alpha, Mi, active-edge mask
+    "k_jcell": (24 + 24, 24, 24 + 24 + 72 + 8),
     # two-fluid step: U_H, U_Na -> U_H', U_Na', B_cell, J_cell, p_e, photoionization, drag | JD, Jac
-    "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8),
-    # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac, W_B
-    "k_derived2": (80 + 24 + 24 + 24 + 8 + 24, 72 + 8 + 144),
-    # E R/W, p_e, rho_H, rho_Na, B, B_cell, J_cell, dr, Hall taper, eta | beta, alpha, JD
-    "k_emf_extra": (48 + 8 + 16 + 24 + 24 + 24 + 72 + 24 + 24, 24 + 24 + 72),
+    "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8, 0),
+    # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac | W_B
+    "k_derived2": (80 + 24 + 24 + 24 + 8 + 24, 72 + 8, 144),
+    # E R/W, p_e, rho_H, rho_Na, B, B_cell, J_cell, Hall taper, eta | JD, nodes (dr) | beta, alpha
+    "k_emf_extra": (48 + 8 + 16 + 24 + 24 + 24 + 24 + 24, 72 + 24, 24 + 24),
 }
 
 
 def kernel_bytes(prec="fp64"):
-    return {k: s + (g if prec == "fp64" else g // 2) for k, (s, g) in KERNEL_PARTS.items()}
+    return {k: s + m + (w if prec == "fp64" else w // 2) for k, (s, m, w) in KERNEL_PARTS.items()}
 
 
 KERNEL_BYTES = kernel_bytes("fp64")
 HBM_PEAK_GBS = 8000.0
 # Whole-step algorithmic bytes per cell-update, each array once per step:
-# U R/W 80 + B_face R/W 48 + Badd R 24 (per case) + geometry Jac 8, JD 72,
-# B_cell weights 144, J_cell weights 288, beta 24, alpha 24, counted once per
-# case (K = 1: the geometry-shared tile order lets cases of one grid meet
-# their geometry in L2, which is a cache effect, not a guaranteed single
-# read).  Mixed precision stores the geometry as float: 280 B.
+# U R/W 80 + B_face R/W 48 + Badd R 24 (per case) + geometry: metric Jac 8,
+# JD 72, nodes 24 (FP64 always) and the weights beta 24, alpha 24, B_cell
+# table 144, J_cell Mi 72 + mask 8 (FP64, float in mixed precision), counted
+# once per case (K = 1: the geometry-shared tile order lets cases of one grid
+# meet their geometry in L2, which is a cache effect, not a guaranteed single
+# read).
 STATE_BYTES = 152
-GEO_BYTES = {"fp64": 560, "mixed": 280}
+GEO_BYTES = {"fp64": 104 + 272, "mixed": 104 + 136}
 STEP_BYTES = STATE_BYTES + GEO_BYTES["fp64"]
 SEED = 20261203
 

@@ -11,7 +11,7 @@ class MhdError(RuntimeError):
 
 class MhdOptions(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("recon", C.c_int), ("ghost", C.c_int),
-                ("hbm_budget_gb", C.c_double), ("autotune", C.c_int), ("xcd_order", C.c_int)]
+                ("hbm_budget_gb", C.c_double), ("autotune", C.c_int), ("xcd_order", C.c_int), ("overlap", C.c_int)]
 
 
 class MhdOTDesc(C.Structure):

@@ -88,7 +88,10 @@ struct TileSet
     size_t cap_k = 0, cap_moff = 0, cap_mem = 0;
     int nent = 0;
     TileArr tiles[R_COUNT][3]; // [range][rows]
+    TileArr cols;              // column tiles of the interior cells (kColCi planes each)
 };
+// planes along i per column-tile workgroup
+constexpr int kColCi = 8;
 // block sets: every block, the single-fluid blocks, the two-fluid blocks
 enum BlockSet
 {
@@ -131,6 +134,10 @@ struct mhd_ctx
     mhd_options opt{};
     std::string err;
     hipStream_t q = nullptr;
+    // second stream for the independent branches of a step (opt.overlap):
+    // fork[0]/join[0] around the fluid update, fork[1]/join[1] around J_cell
+    hipStream_t q2 = nullptr;
+    hipEvent_t fork[2] = {nullptr, nullptr}, join[2] = {nullptr, nullptr};
     std::vector<std::unique_ptr<CaseRec>> cases;
     std::vector<GeoSet> geos;
     bool finalized = false;
@@ -219,9 +226,14 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
     c->opt = *opt;
     for (int k = 0; k < KP_COUNT; ++k)
         c->jt_idx[k] = 2; // 64 x 4, j-tile before i, until tuned
-    if (hipStreamCreateWithFlags(&c->q, hipStreamNonBlocking) != hipSuccess)
+    bool ok = hipStreamCreateWithFlags(&c->q, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->q2, hipStreamNonBlocking) == hipSuccess;
+    for (int e = 0; e < 2 && ok; ++e)
+        ok = hipEventCreateWithFlags(&c->fork[e], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->join[e], hipEventDisableTiming) == hipSuccess;
+    if (!ok)
     {
-        delete c;
+        mhd_destroy(c);
         return MHD_ERR_HIP;
     }
     *out = c;
@@ -233,7 +245,10 @@ extern "C" int mhd_destroy(mhd_ctx *c)
     if (!c)
         return MHD_OK;
     (void)hipSetDevice(c->opt.device);
-    (void)hipStreamSynchronize(c->q);
+    if (c->q)
+        (void)hipStreamSynchronize(c->q);
+    if (c->q2)
+        (void)hipStreamSynchronize(c->q2);
     if (c->gexec)
         (void)hipGraphExecDestroy(c->gexec);
     for (auto &r : c->cases)
@@ -246,6 +261,7 @@ extern "C" int mhd_destroy(mhd_ctx *c)
             for (auto &rr : T.tiles)
                 for (auto &t : rr)
                     free_t(t.d_start);
+            free_t(T.cols.d_start);
             free_t(T.d_lead_k);
             free_t(T.d_lead_moff);
             free_t(T.d_members);
@@ -263,7 +279,17 @@ extern "C" int mhd_destroy(mhd_ctx *c)
     free_t(c->d_diag);
     free_t(c->d_err);
     free_t(c->d_active);
-    (void)hipStreamDestroy(c->q);
+    for (int e = 0; e < 2; ++e)
+    {
+        if (c->fork[e])
+            (void)hipEventDestroy(c->fork[e]);
+        if (c->join[e])
+            (void)hipEventDestroy(c->join[e]);
+    }
+    if (c->q2)
+        (void)hipStreamDestroy(c->q2);
+    if (c->q)
+        (void)hipStreamDestroy(c->q);
     delete c;
     return MHD_OK;
 }
@@ -406,6 +432,25 @@ static int build_tiles(mhd_ctx *c)
                         return rc;
                     HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
                 }
+            {
+                TileArr &A = T.cols;
+                const int tj = mhdk::col_tile_rows();
+                std::vector<int> start(std::max(T.nent, 1), 0);
+                long long total = 0;
+                for (int e = 0; e < T.nent; ++e)
+                {
+                    const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
+                    start[e] = (int)total;
+                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni + kColCi - 1) / kColCi) *
+                             ((d.nj + tj - 1) / tj) * ((d.nk + 63) / 64);
+                }
+                if (total > (1LL << 31) - 64)
+                    return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                A.n = (int)total;
+                if ((rc = grow(c, A.d_start, A.cap_start, start.size())))
+                    return rc;
+                HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+            }
         }
     }
     return MHD_OK;
@@ -444,8 +489,35 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
     L.tm.i_inner = (v % 6) >= 3;
+    L.tm.ci = 0;
     L.jt = kJt[ji];
     return L;
+}
+
+// Column-tile launch of a kernel's block set (per-case or geometry-shared
+// entries as the kernel's tuned variant says).
+static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -1)
+{
+    const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
+    mhdk::Launch L = launch_of(c, kernel, v);
+    const TileSet &S = c->ts[set_of(c, kernel)][v >= 6 ? 1 : 0];
+    L.tm.block_start = S.cols.d_start;
+    L.tm.ntiles = S.cols.n;
+    L.tm.xcd_chunk = c->opt.xcd_order ? (S.cols.n + 7) / 8 : 0;
+    L.tm.i_inner = 0;
+    L.tm.ci = kColCi;
+    L.jt = mhdk::col_tile_rows();
+    return L;
+}
+
+// The fluid update: column tiles for the first-order scheme, cell tiles for minmod.
+static void launch_fluid_any(mhd_ctx *c, int variant, const double *dts, hipStream_t q)
+{
+    const int mixed = mixed_of(c);
+    if (c->opt.recon == 0)
+        mhdk::launch_fluid_col(launch_of_col(c, KP_FLUID, variant), c->d_cst, dts, c->ucur, mixed, q);
+    else
+        mhdk::launch_fluid(launch_of(c, KP_FLUID, variant), c->d_cst, dts, c->ucur, mixed, c->opt.recon, q);
 }
 
 // Rebuild the block table, tiles, op lists and per-case constants for the
@@ -601,6 +673,7 @@ static int place_case(mhd_ctx *c, CaseRec &r)
                                     hipMemcpyDeviceToDevice));
         }
         std::vector<double>().swap(r.cs.blk[b].geo);
+        std::vector<double>().swap(r.cs.blk[b].wj);
         std::vector<double>().swap(r.cs.blk[b].st);
     }
     r.st_bytes = st_bytes;
@@ -669,8 +742,35 @@ static void run_ops(mhd_ctx *c, int sy, int ushift)
 // set-up) is applied inside the J_cell kernel.
 static bool runs(const mhd_ctx *c, int kernel) { return c->ts[set_of(c, kernel)][0].nent > 0; }
 
+// Branch b of a step on the second stream: it starts after everything
+// enqueued on the context stream so far; join() makes the context stream wait
+// for it.  Without overlap (or while profiling, which times kernels on the
+// context stream) both run on the context stream in order.
+static bool overlapped(const mhd_ctx *c) { return c->opt.overlap && !c->prof; }
+static hipStream_t fork_branch(mhd_ctx *c, int b)
+{
+    if (!overlapped(c))
+        return c->q;
+    (void)hipEventRecord(c->fork[b], c->q);
+    (void)hipStreamWaitEvent(c->q2, c->fork[b], 0);
+    return c->q2;
+}
+static void join_branch(mhd_ctx *c, int b)
+{
+    if (!overlapped(c))
+        return;
+    (void)hipEventRecord(c->join[b], c->q2);
+    (void)hipStreamWaitEvent(c->q, c->join[b], 0);
+}
+
 static void derived(mhd_ctx *c)
 {
+    {
+        // J_cell reads only B (synced) and the geometry: beside B_cell / U_plus / dt
+        const hipStream_t qb = fork_branch(c, 1);
+        ProfScope p(c, KP_JCELL);
+        mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), qb);
+    }
     if (runs(c, KP_DERIVED))
     {
         ProfScope p(c, KP_DERIVED);
@@ -681,10 +781,7 @@ static void derived(mhd_ctx *c)
         ProfScope p(c, KP_DERIVED2);
         mhdk::launch_derived2(launch_of(c, KP_DERIVED2), c->d_cst, c->d_dtbits, c->ucur, mixed_of(c), c->q);
     }
-    {
-        ProfScope p(c, KP_JCELL);
-        mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), c->q);
-    }
+    join_branch(c, 1);
     // B_cell and J_cell ghosts of the two-fluid cases with the Hall EMF
     // (the reference's B_cell and J_cell syncs; empty for the other cases)
     ProfScope p(c, KP_OPS);
@@ -703,6 +800,23 @@ static void enqueue_step(mhd_ctx *c)
     {
         ProfScope p(c, KP_DTFIN);
         mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, c->case_cap, c->d_active, c->q);
+    }
+    // The fluid update reads U[cur], B_cell, J_cell, p_e and dt and writes
+    // U[1-cur]; the EMF / curl / resistive chain reads U[cur] and writes
+    // Eface, E, B faces, J_cell (subcycles) -- J_cell only after the fluid
+    // update has read it, so the subcycles wait for the fluid branch.
+    {
+        const hipStream_t qf = fork_branch(c, 0);
+        if (runs(c, KP_FLUID))
+        {
+            ProfScope p(c, KP_FLUID);
+            launch_fluid_any(c, -1, c->d_dt, qf);
+        }
+        if (runs(c, KP_FLUID2))
+        {
+            ProfScope p(c, KP_FLUID2);
+            mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, qf);
+        }
     }
     {
         ProfScope p(c, KP_OPS);
@@ -732,20 +846,11 @@ static void enqueue_step(mhd_ctx *c)
         ProfScope p(c, KP_OPS);
         run_ops(c, SY_EEDGE_BC, 0);
     }
-    if (runs(c, KP_FLUID))
-    {
-        ProfScope p(c, KP_FLUID);
-        mhdk::launch_fluid(launch_of(c, KP_FLUID), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
-    }
-    if (runs(c, KP_FLUID2))
-    {
-        ProfScope p(c, KP_FLUID2);
-        mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, c->q);
-    }
     {
         ProfScope p(c, KP_CURL);
         mhdk::launch_curl_update(launch_of(c, KP_CURL), c->d_dt, c->q);
     }
+    join_branch(c, 0);
     // interior resistivity of the two-fluid cases: explicit subcycles on B
     // (ResistiveDiffusionSubcycles_, Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141)
     for (int isub = 0; isub < c->max_nsub; ++isub)
@@ -826,7 +931,7 @@ static int autotune(mhd_ctx *c)
                     mhdk::launch_edge_emf(L, c->q);
                     break;
                 case KP_FLUID:
-                    mhdk::launch_fluid(L, c->d_cst, c->d_zero, c->ucur, mixed, c->opt.recon, c->q);
+                    launch_fluid_any(c, ji, c->d_zero, c->q);
                     break;
                 case KP_CURL:
                     mhdk::launch_curl_update(L, c->d_zero, c->q);
@@ -1204,7 +1309,7 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
     if (block < 0 || block >= r->nblocks)
         return MHD_ERR_ARG;
     int region, slot, loc, nc;
-    if (!field_slot(field, c->ucur, region, slot, loc, nc))
+    if (!field_slot(field, c->ucur, region, slot, loc, nc) || region == 2)
         return fail(c, MHD_ERR_ARG, "field not stored on the device");
     const MhdBlockDev &d = c->h_blk[r->first_block + block];
     if (region == 0 && slot + nc > st_nslot(d.model))

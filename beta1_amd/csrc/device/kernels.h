@@ -26,6 +26,10 @@ void launch_edge_emf(const Launch &L, hipStream_t q);
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q);
 void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                   hipStream_t q);
+// First-order fluid update in column tiles (C.tm.ci planes per workgroup of
+// 64 x col_tile_rows() threads); bit-identical to launch_fluid with recon 0.
+void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, hipStream_t q);
+int col_tile_rows();
 void launch_jcell(const Launch &L, int mixed, hipStream_t q);
 void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
                     hipStream_t q);

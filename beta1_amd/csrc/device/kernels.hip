@@ -50,6 +50,54 @@ template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __bu
 // Map a workgroup/thread to (block, i, j, k) of its tile.  `wg` is false for
 // padding workgroups of the XCD-ordered grid (no block at all); the return
 // value is false for padding threads.
+// Workgroup -> (block, tile index within the block's entry); false for
+// padding workgroups of the XCD-ordered grid.
+__device__ __forceinline__ bool tile_block(const MhdTileMap &tm, MhdBlockDev &b, int &lt)
+{
+    int t = (int)blockIdx.x;
+    if (tm.xcd_chunk > 0)
+        t = (t & 7) * tm.xcd_chunk + (t >> 3);
+    if (t >= tm.ntiles)
+        return false;
+    int bi = 0, hi = tm.nblk - 1;
+    while (bi < hi)
+    {
+        const int mid = (bi + hi + 1) >> 1;
+        if (tm.block_start[mid] <= t)
+            bi = mid;
+        else
+            hi = mid - 1;
+    }
+    lt = t - tm.block_start[bi];
+    if (tm.members)
+    {
+        const int kk = tm.lead_k[bi];
+        const int m = lt % kk;
+        lt /= kk;
+        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
+    }
+    else
+        b = tm.blks[bi];
+    return true;
+}
+
+// Column tiles: a workgroup of 64 x COL_TJ threads owns the cells
+// (i0 .. i0 + ci - 1) x (j0 .. j0 + COL_TJ - 1) x (k0 .. k0 + 63) of a block
+// and marches along i; one thread per (j, k) column.
+constexpr int COL_TJ = 4, COL_TK = 64;
+__device__ __forceinline__ bool tile_col(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &j0, int &k0)
+{
+    int lt;
+    if (!tile_block(tm, b, lt))
+        return false;
+    const int ntk = (b.nk + COL_TK - 1) / COL_TK, ntj = (b.nj + COL_TJ - 1) / COL_TJ;
+    k0 = (lt % ntk) * COL_TK;
+    lt /= ntk;
+    j0 = (lt % ntj) * COL_TJ;
+    i0 = (lt / ntj) * tm.ci;
+    return true;
+}
+
 __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, int &i, int &j, int &k, bool &wg)
 {
     int t = (int)blockIdx.x;
@@ -165,6 +213,80 @@ __device__ __forceinline__ void rusanov(R gm, const R *WL, const R *WR, R k1, R 
     out[2] = half * ((rhoL * uvL * vL + k2 * pL) + (rhoR * uvR * vR + k2 * pR)) - hr * (rhoR * vR - rhoL * vL);
     out[3] = half * ((rhoL * uvL * wL + k3 * pL) + (rhoR * uvR * wR + k3 * pR)) - hr * (rhoR * wR - rhoL * wL);
     out[4] = half * (uvL * (eL + pL) + uvR * (eR + pR)) - hr * (eR - eL);
+}
+
+// The per-cell part of the Rusanov flux -- primitive state, total energy and
+// sound speed -- evaluated once per cell and shared by every face the cell
+// touches; rusanov_pre combines two of them with the same arithmetic as
+// rusanov / rusanov_mixed, so a face's flux is bit for bit the same.
+struct FluxCell
+{
+    double rho, u, v, w, p, e, c;
+};
+template <typename R>
+__device__ __forceinline__ FluxCell flux_cell(const double *__restrict__ U, int64_t sl, int64_t o, double gm)
+{
+    const Prim q = prim_of<sizeof(R) == 8>(U, sl, o, gm);
+    FluxCell f;
+    f.rho = q.rho_raw;
+    f.u = q.u;
+    f.v = q.v;
+    f.w = q.w;
+    f.p = q.p;
+    if constexpr (sizeof(R) == 8)
+    {
+        f.e = fdiv(f.p, gm - 1.0) + 0.5 * f.rho * (f.u * f.u + f.v * f.v + f.w * f.w);
+        const double t = fdiv(gm * f.p, f.rho < 1e-30 ? 1e-30 : f.rho);
+        f.c = fsqrt(0.0 < t ? t : 0.0);
+    }
+    else
+    {
+        const double igm = 1.0 / (gm - 1.0);
+        f.e = f.p * igm + 0.5 * f.rho * (f.u * f.u + f.v * f.v + f.w * f.w);
+        const float t = fdiv((float)(gm * f.p), (float)(f.rho < 1e-30 ? 1e-30 : f.rho));
+        f.c = (double)fsqrt(0.0f < t ? t : 0.0f);
+    }
+    return f;
+}
+
+template <typename R>
+__device__ __forceinline__ void rusanov_pre(const FluxCell &L, const FluxCell &Rc, double k1, double k2, double k3,
+                                            double *out)
+{
+    const double rhoL = L.rho, uL = L.u, vL = L.v, wL = L.w, pL = L.p, eL = L.e;
+    const double rhoR = Rc.rho, uR = Rc.u, vR = Rc.v, wR = Rc.w, pR = Rc.p, eR = Rc.e;
+    const double uvL = k1 * uL + k2 * vL + k3 * wL;
+    const double uvR = k1 * uR + k2 * vR + k3 * wR;
+    if constexpr (sizeof(R) == 8)
+    {
+        const double kk = k1 * k1 + k2 * k2 + k3 * k3;
+        const double sk = fsqrt(kk);
+        const double radL = fabs(uvL) + L.c * sk;
+        const double radR = fabs(uvR) + Rc.c * sk;
+        const double rad = (radL < radR) ? radR : radL;
+        const double hr = 0.5 * rad;
+        out[0] = 0.5 * (rhoL * uvL + rhoR * uvR) - hr * (rhoR - rhoL);
+        out[1] = 0.5 * ((rhoL * uvL * uL + k1 * pL) + (rhoR * uvR * uR + k1 * pR)) - hr * (rhoR * uR - rhoL * uL);
+        out[2] = 0.5 * ((rhoL * uvL * vL + k2 * pL) + (rhoR * uvR * vR + k2 * pR)) - hr * (rhoR * vR - rhoL * vL);
+        out[3] = 0.5 * ((rhoL * uvL * wL + k3 * pL) + (rhoR * uvR * wR + k3 * pR)) - hr * (rhoR * wR - rhoL * wL);
+        out[4] = 0.5 * (uvL * (eL + pL) + uvR * (eR + pR)) - hr * (eR - eL);
+    }
+    else
+    {
+        const float fk1 = (float)k1, fk2 = (float)k2, fk3 = (float)k3;
+        const float sk = fsqrt(fk1 * fk1 + fk2 * fk2 + fk3 * fk3);
+        const float radL = fabsf((float)uvL) + (float)L.c * sk;
+        const float radR = fabsf((float)uvR) + (float)Rc.c * sk;
+        const float hr = 0.5f * ((radL < radR) ? radR : radL);
+        out[0] = 0.5 * (rhoL * uvL + rhoR * uvR) - (double)(hr * (float)(rhoR - rhoL));
+        out[1] = 0.5 * ((rhoL * uvL * uL + k1 * pL) + (rhoR * uvR * uR + k1 * pR)) -
+                 (double)(hr * (float)(rhoR * uR - rhoL * uL));
+        out[2] = 0.5 * ((rhoL * uvL * vL + k2 * pL) + (rhoR * uvR * vR + k2 * pR)) -
+                 (double)(hr * (float)(rhoR * vR - rhoL * vL));
+        out[3] = 0.5 * ((rhoL * uvL * wL + k3 * pL) + (rhoR * uvR * wR + k3 * pR)) -
+                 (double)(hr * (float)(rhoR * wR - rhoL * wL));
+        out[4] = 0.5 * (uvL * (eL + pL) + uvR * (eR + pR)) - (double)(hr * (float)(eR - eL));
+    }
 }
 
 template <typename T> __device__ __forceinline__ T minmod(T a, T b)
@@ -685,6 +807,122 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
         Un[m * sl + o] = U[m * sl + o] + dt * rhs[m];
 }
 
+// The first-order fluid update of k_fluid in column tiles: per plane i the
+// workgroup evaluates flux_cell once for its 64 x COL_TJ cells and their one-
+// cell halo in j and k (LDS); a thread keeps its own column's cells i-1, i,
+// i+1 in registers.  Every face flux is rusanov_pre of the same two cells as
+// in k_fluid, so the update is bit for bit k_fluid's with about a quarter of
+// the divisions.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                   const double *__restrict__ dts, int ucur)
+{
+    constexpr int PJ = COL_TJ + 2, PK = COL_TK + 2;
+    __shared__ double P[7][PJ][PK];
+    MhdBlockDev b;
+    int i0, j0, k0;
+    if (!tile_col(tm, b, i0, j0, k0)) // uniform over the workgroup
+        return;
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * 64 + tx;
+    const int j = j0 + ty, k = k0 + tx;
+    const bool act = j < b.nj && k < b.nk;
+    const int i1 = min(i0 + tm.ci, b.ni);
+    const MhdCaseConst &c = cst[b.case_id];
+    const double gm = c.gamma;
+    const int64_t sl = b.slot, di = (int64_t)b.sj * b.sk, dj = b.sk;
+    double *S = dst_ptr(b);
+    const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
+    double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
+    const double *G = dmetric(b);
+    const double *Ax = G + (int64_t)G_JD * sl, *Ay = Ax + 3 * sl, *Az = Ax + 6 * sl;
+    const double dt = dts[b.case_id];
+    int64_t o = blk_off(b, i0, act ? j : j0, act ? k : k0);
+    FluxCell prev{}, cur{};
+    if (act)
+    {
+        prev = flux_cell<R>(U, sl, o - di, gm);
+        cur = flux_cell<R>(U, sl, o, gm);
+    }
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        // cells (i, j0-1 .. j0+COL_TJ, k0-1 .. k0+64), those up to the first
+        // ghost layer (the rest are read by no active thread)
+        for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
+        {
+            const int pj = p / PK, pk = p - pj * PK;
+            const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+            if (jj <= b.nj && kk <= b.nk)
+            {
+                const FluxCell f = flux_cell<R>(U, sl, blk_off(b, i, jj, kk), gm);
+                P[0][pj][pk] = f.rho;
+                P[1][pj][pk] = f.u;
+                P[2][pj][pk] = f.v;
+                P[3][pj][pk] = f.w;
+                P[4][pj][pk] = f.p;
+                P[5][pj][pk] = f.e;
+                P[6][pj][pk] = f.c;
+            }
+        }
+        FluxCell next{};
+        if (act)
+            next = flux_cell<R>(U, sl, o + di, gm);
+        __syncthreads();
+        if (act)
+        {
+            auto lds = [&](int pj, int pk) {
+                FluxCell f;
+                f.rho = P[0][pj][pk];
+                f.u = P[1][pj][pk];
+                f.v = P[2][pj][pk];
+                f.w = P[3][pj][pk];
+                f.p = P[4][pj][pk];
+                f.e = P[5][pj][pk];
+                f.c = P[6][pj][pk];
+                return f;
+            };
+            const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+            double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            double Flo[5], Fhi[5];
+            rusanov_pre<R>(prev, cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], Flo);
+            rusanov_pre<R>(cur, next, Ax[o + di], Ax[sl + o + di], Ax[2 * sl + o + di], Fhi);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
+            rusanov_pre<R>(lds(ty, tx + 1), cur, Ay[o], Ay[sl + o], Ay[2 * sl + o], Flo);
+            rusanov_pre<R>(cur, lds(ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj], Fhi);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
+            rusanov_pre<R>(lds(ty + 1, tx), cur, Az[o], Az[sl + o], Az[2 * sl + o], Flo);
+            rusanov_pre<R>(cur, lds(ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fhi);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
+            // J x B (Hall_Num_Limiter weighting), as k_fluid
+            const double n_true = dmax(cur.rho, 0.0) / c.M_H * c.M_ref;
+            const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
+            const double wgt = n_true / ne_eff;
+            const double fc = c.momentum_hall_coeff * wgt;
+            const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+            const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
+            const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
+            const double jx = J1 * B2 - J2 * B1;
+            const double jy = J2 * B0 - J0 * B2;
+            const double jz = J0 * B1 - J1 * B0;
+            rhs[1] += fc * jx;
+            rhs[2] += fc * jy;
+            rhs[3] += fc * jz;
+            rhs[4] += fc * (jx * cur.u + jy * cur.v + jz * cur.w);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                Un[m * sl + o] = U[m * sl + o] + dt * rhs[m];
+        }
+        __syncthreads();
+        prev = cur;
+        cur = next;
+    }
+}
+
 // ======================================================================
 // Derived fields, part 1: contravariant velocities of cells [-1, N] and, on
 // interior cells, B_cell (calc_Bcell, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
@@ -744,7 +982,12 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
 // Derived fields, part 2, interior cells: Calc_J_Edge + calc_Jcell
 // (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891; CTOperators::
 // CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).  The 12
-// J edges of a cell are recomputed from beta-weighted face fluxes.
+// J edges of a cell are recomputed from beta-weighted face fluxes and
+// Jcell_from_Jedge_w is formed in place: W[c][n] = Mi[c] . e_n for the active
+// edges, e_n = node(end) - node(start), with the host builder's arithmetic,
+// so J_cell accumulates the reference's W[c][n] * J_n in the reference's
+// order n = 0..11.  (Straight-line code: 256 VGPRs, but faster than a rolled
+// loop over the edge families at 106 VGPRs, 0.27 vs 0.33 ms on 3.1 M cells.)
 template <typename WT>
 __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
@@ -754,12 +997,22 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
     double *S = dst_ptr(b);
     const WT *G = dweights<WT>(b);
-    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const WT *bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *by = bx + sl, *bz = bx + 2 * sl;
-    const WT *ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl, *ay = ax + sl, *az = ax + 2 * sl;
+    const double *Bf = S + (int64_t)ST_B * sl;
+    const WT *beta = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *alph = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl;
+    const double *X = dmetric(b) + (int64_t)G_NODE * sl;
+    const WT *Mw = G + (int64_t)(G_WJM - G_NMETRIC) * sl;
+    double mi[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        mi[q] = (double)Mw[(int64_t)q * sl + o];
+    const unsigned act = (unsigned)G[(int64_t)(G_WJA - G_NMETRIC) * sl + o];
+    const int64_t di = st[0], dj = st[1], dk = st[2];
+    const double *Bx = Bf, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const WT *bx = beta, *by = bx + sl, *bz = bx + 2 * sl;
+    const WT *ax = alph, *ay = ax + sl, *az = ax + 2 * sl;
     auto jxi = [&](int64_t e) {
         const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
         return (1.0 * val) * ax[e];
@@ -772,16 +1025,25 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
         const double val = (bx[e - dj] * Bx[e - dj] - bx[e] * Bx[e]) + (by[e] * By[e] - by[e - di] * By[e - di]);
         return (1.0 * val) * az[e];
     };
-    const double s[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
+    const double sv[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
                           jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
-    const WT *W = G + (int64_t)(G_WJ - G_NMETRIC) * sl;
+    const int64_t lo[12] = {0, dj, dk, dj + dk, 0, di, dk, di + dk, 0, di, dj, di + dj};
     double jc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 0; n < 12; ++n)
     {
-        jc[0] += W[(int64_t)n * sl + o] * s[n];
-        jc[1] += W[(int64_t)(12 + n) * sl + o] * s[n];
-        jc[2] += W[(int64_t)(24 + n) * sl + o] * s[n];
+        const int64_t a = o + lo[n], e = a + (n < 4 ? di : (n < 8 ? dj : dk));
+        double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+        if ((act >> n) & 1u)
+        {
+            const double rx = X[e] - X[a], ry = X[sl + e] - X[sl + a], rz = X[2 * sl + e] - X[2 * sl + a];
+            w0 = mi[0] * rx + mi[1] * ry + mi[2] * rz;
+            w1 = mi[3] * rx + mi[4] * ry + mi[5] * rz;
+            w2 = mi[6] * rx + mi[7] * ry + mi[8] * rz;
+        }
+        jc[0] += w0 * sv[n];
+        jc[1] += w1 * sv[n];
+        jc[2] += w2 * sv[n];
     }
     // solid wall: J_cell = 0 on the two cell layers next to it
     // (Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:382-427)
@@ -1162,8 +1424,10 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
             for (int m = 0; m < 3; ++m)
                 Ec[m] = 0.25 * (eh[0][m] + eh[1][m] + eh[2][m] + eh[3][m]);
             const double bmax = dmax(dmax(bt[0], bt[1]), dmax(bt[2], bt[3]));
-            const double *dr = S + (int64_t)(ST_DR + 3 * a) * sl;
-            const double d0 = dr[o], d1 = dr[sl + o], d2 = dr[2 * sl + o];
+            // edge vector dr = node(o + st[a]) - node(o) (src/3_field/4_MPCNS_Geometry.cpp:1251-1320)
+            const double *X = dmetric(b) + (int64_t)G_NODE * sl;
+            const int64_t o2 = o + st[a];
+            const double d0 = X[o2] - X[o], d1 = X[sl + o2] - X[sl + o], d2 = X[2 * sl + o2] - X[2 * sl + o];
             const double Ecen = Ec[0] * d0 + Ec[1] * d1 + Ec[2] * d2;
             const double eps = 1e-14, Cwh = 0.1;
             const double L = dmax(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
@@ -1237,8 +1501,10 @@ __global__ __launch_bounds__(256) void k_eres(const MhdTileMap tm, const MhdCase
 #pragma unroll
             for (int m = 0; m < 3; ++m)
                 J[m] *= 0.25;
-            const double *dr = S + (int64_t)(ST_DR + 3 * a) * sl;
-            const double Jedge = J[0] * dr[o] + J[1] * dr[sl + o] + J[2] * dr[2 * sl + o];
+            const double *X = dmetric(b) + (int64_t)G_NODE * sl;
+            const int64_t o2 = o + st[a];
+            const double Jedge =
+                J[0] * (X[o2] - X[o]) + J[1] * (X[sl + o2] - X[sl + o]) + J[2] * (X[2 * sl + o2] - X[2 * sl + o]);
             e = 0.0 + coef * Jedge;
         }
         E[a * sl + o] = e;
@@ -1513,6 +1779,15 @@ void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, i
     else
         hipLaunchKernelGGL((k_fluid<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
+void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, hipStream_t q)
+{
+    const dim3 g = grid_of(C), bl(64, COL_TJ, 1);
+    if (mixed)
+        hipLaunchKernelGGL((k_fluid_col<float>), g, bl, 0, q, C.tm, cst, dts, ucur);
+    else
+        hipLaunchKernelGGL((k_fluid_col<double>), g, bl, 0, q, C.tm, cst, dts, ucur);
+}
+int col_tile_rows() { return COL_TJ; }
 void launch_jcell(const Launch &L, int mixed, hipStream_t q)
 {
     if (mixed)

@@ -503,10 +503,7 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
                             const double p0[3] = {B.x[n0], B.y[n0], B.z[n0]}, p1[3] = {B.x[n1], B.y[n1], B.z[n1]};
                             double m[3];
                             for (int q = 0; q < 3; ++q)
-                            {
-                                A.st[(size_t)(ST_DR + 3 * e + q) * d.slot + o] = p1[q] - p0[q];
                                 m[q] = 0.5 * (p0[q] + p1[q]);
-                            }
                             const double r = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
                             const double width = c.hall_rmax - c.hall_rmin;
                             const double taper =

@@ -474,11 +474,20 @@ void sym_eig3(double A[3][3], double V[3][3], double d[3])
 
 // Pseudo-inverse least-squares weights: W_n = M^+ (mu_n s_n),
 // M = sum mu_n s_n s_n^T, with relative activity threshold tau_act and
-// eigenvalue truncation tau_eig (reference :489-622 and :794-912).
-void pinv_weights(int n, const double (*s)[3], const double *mu_in, double tau_act, double *W /*3*n*/)
+// eigenvalue truncation tau_eig (reference :489-622 and :794-912).  Mi_out
+// (9, row-major) and active_out, when given, receive M^+ and the mask of
+// active n (all zero where every weight is zero), from which
+// W[c n + q] = Mi[c][0] mu_q s_q[0] + Mi[c][1] mu_q s_q[1] + Mi[c][2] mu_q s_q[2].
+void pinv_weights(int n, const double (*s)[3], const double *mu_in, double tau_act, double *W /*3*n*/,
+                  double *Mi_out = nullptr, unsigned *active_out = nullptr)
 {
     for (int q = 0; q < 3 * n; ++q)
         W[q] = 0.0;
+    if (Mi_out)
+        for (int q = 0; q < 9; ++q)
+            Mi_out[q] = 0.0;
+    if (active_out)
+        *active_out = 0;
     double mag[36], ref = 0.0;
     for (int q = 0; q < n; ++q)
     {
@@ -523,10 +532,16 @@ void pinv_weights(int n, const double (*s)[3], const double *mu_in, double tau_a
         for (int b = 0; b < 3; ++b)
             for (int q = 0; q < 3; ++q)
                 Mi[a][b] += V[a][q] * il[q] * V[b][q];
+    if (Mi_out)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                Mi_out[3 * a + b] = Mi[a][b];
     for (int q = 0; q < n; ++q)
     {
         if (!active[q])
             continue;
+        if (active_out)
+            *active_out |= 1u << q;
         const double rx = mu[q] * s[q][0], ry = mu[q] * s[q][1], rz = mu[q] * s[q][2];
         W[q] = Mi[0][0] * rx + Mi[0][1] * ry + Mi[0][2] * rz;
         W[n + q] = Mi[1][0] * rx + Mi[1][1] * ry + Mi[1][2] * rz;
@@ -574,6 +589,7 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
         for (int f = 0; f < 6; ++f)
             d.bc[f] = B.face[f].kind == 0 ? B.face[f].bc : BC_NONE;
         A.geo.assign((size_t)G_NSLOT * d.slot, 0.0);
+        A.wj.assign((size_t)36 * d.slot, 0.0);
         NodeView nv{B};
         auto G = [&](int s, int i, int j, int k) -> double & { return A.geo[(size_t)s * d.slot + blk_off(d, i, j, k)]; };
 #pragma omp parallel for schedule(dynamic)
@@ -589,6 +605,9 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
                         G(G_JD + 6 + c, i, j, k) = jz[c];
                     }
                     G(G_JAC, i, j, k) = nv.Jac(i, j, k);
+                    const V3 node = nv.P(i, j, k);
+                    for (int c = 0; c < 3; ++c)
+                        G(G_NODE + c, i, j, k) = node[c];
                     // face Hodge beta = |l*|/|S|  (reference :392-478)
                     const V3 cc = nv.C(i, j, k);
                     const V3 jd[3] = {jx, je, jz};
@@ -657,10 +676,15 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
                         Le[q][2] = d0[2];
                         one[q] = 1.0;
                     }
-                    double Wj[36];
-                    pinv_weights(12, Le, one, 1e-10, Wj);
+                    double Wj[36], Mi[9];
+                    unsigned act = 0;
+                    pinv_weights(12, Le, one, 1e-10, Wj, Mi, &act);
+                    const int64_t o = blk_off(d, i, j, k);
                     for (int q = 0; q < 36; ++q)
-                        G(G_WJ + q, i, j, k) = Wj[q];
+                        A.wj[(size_t)q * d.slot + o] = Wj[q];
+                    for (int q = 0; q < 9; ++q)
+                        G(G_WJM + q, i, j, k) = Mi[q];
+                    G(G_WJA, i, j, k) = (double)act;
                 }
     }
 }

@@ -74,6 +74,7 @@ struct BlockArrays
     MhdBlockDev d{};
     std::vector<double> geo; // G_NSLOT * slot
     std::vector<double> st;  // st_nslot(d.model) * slot
+    std::vector<double> wj;  // 36 * slot: Jcell_from_Jedge_w in full (host only)
 };
 
 // One element-wise operation of a boundary / halo phase.  Offsets are
@@ -172,8 +173,9 @@ void build_ops(CaseSetup &cs);
 // Execute copy/zero/const op lists on host state arrays (start-up syncs).
 void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops, const MhdCaseConst &cst);
 
-// Field number (mhd_field) -> (region 0 state / 1 geometry, first slot,
-// location, components); false when the field is not stored.
+// Field number (mhd_field) -> (region 0 state / 1 geometry / 2 the host-only
+// J_cell weight table, first slot, location, components); false when the
+// field is not stored.
 bool field_slot(int field, int ucur, int &region, int &slot, int &loc, int &nc);
 // Elements of a field block in the reference AoS layout incl. ghosts.
 size_t aos_count(const MhdBlockDev &d, int loc, int nc);

@@ -76,27 +76,35 @@ enum MhdStateSlot
     ST_PHOTO = 47, // Na+ photoionization production rate [cm^-3 s^-1] (static)
     ST_DRAG = 48,  // ion-neutral drag frequency [1/s] (static)
     ST_PE = 49,    // electron pressure p_H + p_Na of cells [-1, N] (step scratch)
-    ST_DR = 50,    // 9 slots: edge vectors dr_xi(3), dr_eta(3), dr_zeta(3) (static)
-    ST_TAPER = 59, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
-    ST_ETALOC = 62, // 3 slots: local artificial resistivity of the edges (static)
-    ST_ETARES = 65, // 3 slots: interior resistivity inver_Rem yita0(r) of the edges (static)
-    ST_NSLOT2 = 68
+    // (edge vectors dr are differences of the geometry's node coordinates)
+    ST_TAPER = 50, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
+    ST_ETALOC = 53, // 3 slots: local artificial resistivity of the edges (static)
+    ST_ETARES = 56, // 3 slots: interior resistivity inver_Rem yita0(r) of the edges (static)
+    ST_NSLOT2 = 59
 };
 
 // State slots per block for a physics model (0 single fluid, 1 two-fluid).
 MHD_HD int st_nslot(int model) { return model == 1 ? (int)ST_NSLOT2 : (int)ST_NSLOT; }
 
-// Geometry slots (shared by every case with the same grid).
+// Geometry slots (shared by every case with the same grid).  The J_cell
+// table Jcell_from_Jedge_w (36 weights per cell) is not stored: its weights
+// are W[c][q] = sum_b Mi[c][b] e_q[b] for the active edges q, with e_q the
+// edge vector (difference of two node coordinates) and Mi the inverse of the
+// least-squares normal matrix, so the kernel forms them from G_NODE, G_WJM
+// and G_WJA with the host builder's arithmetic (bit-identical weights from 13
+// stored values instead of 36).
 enum MhdGeoSlot
 {
-    G_JAC = 0,    // cell volume
-    G_JD = 1,     // 9 slots: JDxi(3), JDet(3), JDze(3) face area vectors
-    G_BETA = 10,  // 3 slots: face Hodge weight beta = |l*|/|S|
-    G_ALPHA = 13, // 3 slots: edge Hodge weight alpha = |e|/|S*|
-    G_WB = 16,    // 18 comps: Bcell_from_Bface_w
-    G_WJ = 34,    // 36 comps: Jcell_from_Jedge_w
-    G_NSLOT = 70,
-    G_NMETRIC = 10 // slots [0, G_NMETRIC): the metric (Jac, JD), always double
+    G_JAC = 0,     // cell volume
+    G_JD = 1,      // 9 slots: JDxi(3), JDet(3), JDze(3) face area vectors
+    G_NODE = 10,   // 3 slots: coordinates x, y, z of node (i, j, k)
+    G_BETA = 13,   // 3 slots: face Hodge weight beta = |l*|/|S|
+    G_ALPHA = 16,  // 3 slots: edge Hodge weight alpha = |e|/|S*|
+    G_WB = 19,     // 18 comps: Bcell_from_Bface_w
+    G_WJM = 37,    // 9 comps: Mi of the J_cell least squares, row-major
+    G_WJA = 46,    // 1 comp: active J edges of the cell, bit q for edge q (0..11)
+    G_NSLOT = 47,
+    G_NMETRIC = 13 // slots [0, G_NMETRIC): metric and nodes, always double
 };
 
 // Geometry storage of a block: the metric slots in double, then the Hodge
@@ -200,6 +208,7 @@ struct MhdTileMap
     int lo, hi; // range [lo, n + hi) on every axis
     int xcd_chunk;
     int i_inner; // 1: consecutive tiles step along i, 0: along the (j, k) plane
+    int ci;      // column tiles: planes per workgroup along i
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)

@@ -211,9 +211,9 @@ class MhdContext:
     """Device context holding one or more cases (all advanced together)."""
 
     def __init__(self, precision="fp64", recon=0, ghost=2, device=0, hbm_budget_gb=0.0, autotune=False,
-                 xcd_order=True):
+                 xcd_order=True, overlap=True):
         opt = MhdOptions(device, PREC[precision], recon, ghost, hbm_budget_gb, 1 if autotune else 0,
-                         1 if xcd_order else 0)
+                         1 if xcd_order else 0, 1 if overlap else 0)
         self.ctx = C.c_void_p()
         _check(lib().mhd_create(C.byref(opt), C.byref(self.ctx)), what="mhd_create")
         self.cases = []

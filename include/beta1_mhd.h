@@ -62,6 +62,9 @@ typedef struct
     double hbm_budget_gb; /* admission budget: bytes of state + geometry (0 = no limit) */
     int autotune;         /* 1: time workgroup heights per kernel at mhd_finalize */
     int xcd_order;        /* 1: deal tiles to the 8 XCDs in contiguous ranges */
+    int overlap;          /* 1: run the fluid update beside the EMF / curl chain and J_cell beside
+                             the other derived fields, on a second stream of the context (they
+                             share no array); results are unchanged */
 } mhd_options;
 
 /* Orszag-Tang periodic box (n^3 cells on [0,1)^3). */

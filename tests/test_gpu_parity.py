@@ -102,6 +102,26 @@ def test_graph_step_equals_eager():
             assert np.array_equal(a.field(ca, name, blk), b.field(cb, name, blk))
 
 
+def test_second_stream_overlap_leaves_results_unchanged():
+    """The fluid update beside the EMF chain and J_cell beside the derived
+    fields (two streams, eager and captured) equal the one-stream order."""
+    descs = [WindDesc(nang=8, nrad=16), MercuryDesc(nang=8, nrad=16)]
+    runs = []
+    for overlap in (False, True):
+        ctx = MhdContext(overlap=overlap)
+        ids = [ctx.add(d) for d in descs]
+        ctx.finalize()
+        ctx.step(3)
+        ctx.step(4, graph=True)
+        ctx.step(1)
+        runs.append({(c, f, b): ctx.field(c, f, b) for c in ids for f in CONSERVED + ["J_cell", "B_cell"]
+                     for b in range(6)})
+        runs[-1]["dt"] = [ctx.last_dt(c) for c in ids]
+        ctx.close()
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
 def test_mixed_precision_close_to_fp64():
     desc = WindDesc(nang=8, nrad=16)
     f = MhdContext("fp64")

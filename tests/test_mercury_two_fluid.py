@@ -12,7 +12,8 @@ Pins (each would catch a dropped term, a wrong sign or a wrong species):
 * where the flow is uniform the Na+ mass source equals a1_Na * Photo and the
   drag removes momentum at the rate a4 * nu;
 * the ambipolar edge EMF is a discrete gradient where n_e is uniform, so
-  its curl (the change it makes to B) vanishes to round-off.
+  its curl (the change it makes to B) vanishes to round-off;
+* the min/max diagnostics equal numpy reductions over the interior cells.
 """
 import numpy as np
 import pytest
@@ -295,3 +296,32 @@ def test_interior_resistivity_emf_follows_the_profile():
             assert np.allclose(got, want, rtol=1e-10, atol=1e-14 * max(np.abs(want).max(), 1e-30)), (b, nm)
             hit += int(np.count_nonzero(want))
     assert hit > 100
+
+
+def test_min_max_diagnostics_reduce_the_interior_cells():
+    """or_diagnostics against numpy reductions over the interior cells: the
+    pressure from U (p = (gamma-1)(E - |m|^2/2rho), no floor reached), the
+    magnetic energy from B_cell, and max |div B| equal to the oracle's own."""
+    hc = case()
+    o = Oracle(hc)
+    o.step(2)
+    gm = hc.constants()[0]
+    sl = interior("U_H", hc.n, G)
+    got = o.diagnostics()
+    cols = {q: [] for q in range(8)}
+    for b in range(hc.nblocks):
+        for s, (qr, qp) in (("U_H", (0, 1)), ("U_Na", (2, 3))):
+            U = o.view(s, b)[sl]
+            cols[qr].append(U[..., 0].ravel())
+            cols[qp].append(((gm - 1) * (U[..., 4] - 0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0])).ravel())
+        Bc = o.view("B_cell", b)[sl]
+        for m in range(3):
+            cols[4 + m].append(Bc[..., m].ravel())
+        cols[7].append(0.5 * (Bc ** 2).sum(-1).ravel())
+    for q in range(8):
+        v = np.concatenate(cols[q])
+        exact = q in (0, 2, 4, 5, 6)  # stored values: min/max are exact
+        tol = 0 if exact else 1e-12 * np.abs(v).max()
+        assert abs(got[q] - v.min()) <= tol and abs(got[8 + q] - v.max()) <= tol, q
+    assert got[16] == o.divb_max()
+    assert cols[0][0].size == np.prod(hc.n)
