@@ -84,7 +84,7 @@ __device__ __forceinline__ bool tile_block(const MhdTileMap &tm, MhdBlockDev &b,
 // Column tiles: a workgroup of 64 x COL_TJ threads owns the cells
 // (i0 .. i0 + ci - 1) x (j0 .. j0 + COL_TJ - 1) x (k0 .. k0 + 63) of a block
 // and marches along i; one thread per (j, k) column.
-constexpr int COL_TJ = 4, COL_TK = 64;
+constexpr int COL_TJ = 8, COL_TK = 64;
 __device__ __forceinline__ bool tile_col(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &j0, int &k0)
 {
     int lt;
@@ -807,15 +807,18 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
         Un[m * sl + o] = U[m * sl + o] + dt * rhs[m];
 }
 
-// The first-order fluid update of k_fluid in column tiles: per plane i the
+// The first-order fluid update of k_fluid in column tiles.  Per plane i the
 // workgroup evaluates flux_cell once for its 64 x COL_TJ cells and their one-
-// cell halo in j and k (LDS); a thread keeps its own column's cells i-1, i,
-// i+1 in registers.  Every face flux is rusanov_pre of the same two cells as
-// in k_fluid, so the update is bit for bit k_fluid's with about a quarter of
-// the divisions.
+// cell halo in j and k (LDS); a thread keeps its own column's cell i-1 in
+// registers.  Each i-face flux is computed once: the face between planes i-1
+// and i closes cell i-1 (whose j and k terms, 1/J and source were kept) and
+// opens cell i.  Every face flux is rusanov_pre of the same two cells as in
+// k_fluid and the update adds the same terms in the same order
+// (((0 - t_i) - t_j) - t_k, then the J x B source), so the result is bit for
+// bit k_fluid's, with about a fifth of its divisions.
 template <typename R>
-__global__ __launch_bounds__(256) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
-                                                   const double *__restrict__ dts, int ucur)
+__global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                           const double *__restrict__ dts, int ucur)
 {
     constexpr int PJ = COL_TJ + 2, PK = COL_TK + 2;
     __shared__ double P[7][PJ][PK];
@@ -835,38 +838,40 @@ __global__ __launch_bounds__(256) void k_fluid_col(const MhdTileMap tm, const Mh
     double *Un = S + (int64_t)(ST_U0 + 5 * (1 - ucur)) * sl;
     const double *G = dmetric(b);
     const double *Ax = G + (int64_t)G_JD * sl, *Ay = Ax + 3 * sl, *Az = Ax + 6 * sl;
+    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
     const double dt = dts[b.case_id];
-    int64_t o = blk_off(b, i0, act ? j : j0, act ? k : k0);
-    FluxCell prev{}, cur{};
+    // the J x B weight of k_fluid (Hall_Num_Limiter)
+    const double fcoef = c.momentum_hall_coeff;
+    int64_t o = blk_off(b, i0, act ? j : j0, act ? k : k0); // cell of plane `pl` in the loop
+    FluxCell prev{};
     if (act)
-    {
         prev = flux_cell<R>(U, sl, o - di, gm);
-        cur = flux_cell<R>(U, sl, o, gm);
-    }
-    for (int i = i0; i < i1; ++i, o += di)
+    // the open cell (plane pl - 1): lower i-face flux, j and k terms, 1/J, source
+    double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tj[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tk[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double src[4] = {0.0, 0.0, 0.0, 0.0}, invJ = 0.0;
+    for (int pl = i0; pl <= i1; ++pl, o += di)
     {
-        // cells (i, j0-1 .. j0+COL_TJ, k0-1 .. k0+64), those up to the first
-        // ghost layer (the rest are read by no active thread)
-        for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
+        FluxCell cur{};
+        if (pl < i1)
         {
-            const int pj = p / PK, pk = p - pj * PK;
-            const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
-            if (jj <= b.nj && kk <= b.nk)
+            for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
             {
-                const FluxCell f = flux_cell<R>(U, sl, blk_off(b, i, jj, kk), gm);
-                P[0][pj][pk] = f.rho;
-                P[1][pj][pk] = f.u;
-                P[2][pj][pk] = f.v;
-                P[3][pj][pk] = f.w;
-                P[4][pj][pk] = f.p;
-                P[5][pj][pk] = f.e;
-                P[6][pj][pk] = f.c;
+                const int pj = p / PK, pk = p - pj * PK;
+                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+                if (jj <= b.nj && kk <= b.nk)
+                {
+                    const FluxCell f = flux_cell<R>(U, sl, blk_off(b, pl, jj, kk), gm);
+                    P[0][pj][pk] = f.rho;
+                    P[1][pj][pk] = f.u;
+                    P[2][pj][pk] = f.v;
+                    P[3][pj][pk] = f.w;
+                    P[4][pj][pk] = f.p;
+                    P[5][pj][pk] = f.e;
+                    P[6][pj][pk] = f.c;
+                }
             }
+            __syncthreads();
         }
-        FluxCell next{};
-        if (act)
-            next = flux_cell<R>(U, sl, o + di, gm);
-        __syncthreads();
         if (act)
         {
             auto lds = [&](int pj, int pk) {
@@ -880,46 +885,58 @@ __global__ __launch_bounds__(256) void k_fluid_col(const MhdTileMap tm, const Mh
                 f.c = P[6][pj][pk];
                 return f;
             };
-            const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
-            double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            double Flo[5], Fhi[5];
-            rusanov_pre<R>(prev, cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], Flo);
-            rusanov_pre<R>(cur, next, Ax[o + di], Ax[sl + o + di], Ax[2 * sl + o + di], Fhi);
+            cur = pl < i1 ? lds(ty + 1, tx + 1) : flux_cell<R>(U, sl, o, gm);
+            double F[5];
+            rusanov_pre<R>(prev, cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], F);
+            if (pl > i0) // close cell pl - 1
+            {
+                const int64_t oc = o - di;
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
-            rusanov_pre<R>(lds(ty, tx + 1), cur, Ay[o], Ay[sl + o], Ay[2 * sl + o], Flo);
-            rusanov_pre<R>(cur, lds(ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj], Fhi);
+                for (int m = 0; m < 5; ++m)
+                {
+                    double r = 0.0;
+                    r -= (F[m] - Flo[m]) * invJ;
+                    r -= tj[m];
+                    r -= tk[m];
+                    if (m >= 1)
+                        r += src[m - 1];
+                    Un[m * sl + oc] = U[m * sl + oc] + dt * r;
+                }
+            }
+            if (pl < i1) // open cell pl
+            {
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
-            rusanov_pre<R>(lds(ty + 1, tx), cur, Az[o], Az[sl + o], Az[2 * sl + o], Flo);
-            rusanov_pre<R>(cur, lds(ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fhi);
+                for (int m = 0; m < 5; ++m)
+                    Flo[m] = F[m];
+                invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+                double Fl[5], Fh[5];
+                rusanov_pre<R>(lds(ty, tx + 1), cur, Ay[o], Ay[sl + o], Ay[2 * sl + o], Fl);
+                rusanov_pre<R>(cur, lds(ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj], Fh);
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                rhs[m] -= (Fhi[m] - Flo[m]) * invJ;
-            // J x B (Hall_Num_Limiter weighting), as k_fluid
-            const double n_true = dmax(cur.rho, 0.0) / c.M_H * c.M_ref;
-            const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
-            const double wgt = n_true / ne_eff;
-            const double fc = c.momentum_hall_coeff * wgt;
-            const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
-            const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
-            const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
-            const double jx = J1 * B2 - J2 * B1;
-            const double jy = J2 * B0 - J0 * B2;
-            const double jz = J0 * B1 - J1 * B0;
-            rhs[1] += fc * jx;
-            rhs[2] += fc * jy;
-            rhs[3] += fc * jz;
-            rhs[4] += fc * (jx * cur.u + jy * cur.v + jz * cur.w);
+                for (int m = 0; m < 5; ++m)
+                    tj[m] = (Fh[m] - Fl[m]) * invJ;
+                rusanov_pre<R>(lds(ty + 1, tx), cur, Az[o], Az[sl + o], Az[2 * sl + o], Fl);
+                rusanov_pre<R>(cur, lds(ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fh);
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                Un[m * sl + o] = U[m * sl + o] + dt * rhs[m];
+                for (int m = 0; m < 5; ++m)
+                    tk[m] = (Fh[m] - Fl[m]) * invJ;
+                const double n_true = dmax(cur.rho, 0.0) / c.M_H * c.M_ref;
+                const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
+                const double fc = fcoef * (n_true / ne_eff);
+                const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
+                const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
+                const double jx = J1 * B2 - J2 * B1;
+                const double jy = J2 * B0 - J0 * B2;
+                const double jz = J0 * B1 - J1 * B0;
+                src[0] = fc * jx;
+                src[1] = fc * jy;
+                src[2] = fc * jz;
+                src[3] = fc * (jx * cur.u + jy * cur.v + jz * cur.w);
+            }
+            prev = cur;
         }
-        __syncthreads();
-        prev = cur;
-        cur = next;
+        if (pl < i1)
+            __syncthreads();
     }
 }
 
