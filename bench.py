@@ -33,8 +33,10 @@ KERNEL_PARTS = {
     "k_curl_update": (24 + 24 + 24, 0, 0),         # E, B -> B
     # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | W_B
     "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8, 144),
-    # B -> J_cell | nodes | beta, alpha, Mi, active-edge mask
-    "k_jcell": (24 + 24, 24, 24 + 24 + 72 + 8),
+    # B -> J edges | beta, alpha
+    "k_jedge": (24 + 24, 0, 24 + 24),
+    # J edges -> J_cell | nodes | Mi, active-edge mask
+    "k_jcell": (24 + 24, 24, 72 + 8),
     # two-fluid step: U_H, U_Na -> U_H', U_Na', B_cell, J_cell, p_e, photoionization, drag | JD, Jac
     "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8, 0),
     # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac | W_B

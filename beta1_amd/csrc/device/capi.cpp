@@ -119,14 +119,15 @@ enum KernelId
     KP_ERES,
     KP_JCELL_RES,
     KP_CURL_RES,
+    KP_JEDGE,
     KP_COUNT
 };
 const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
                                      "k_fluid2",  "k_derived2",    "k_emf_extra", "k_eres",     "k_jcell_res",
-                                     "k_curl_res"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT,
-                                    R_CELL, R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE};
+                                     "k_curl_res", "k_jedge"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL,
+                                    R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_FACE};
 } // namespace
 
 struct mhd_ctx
@@ -766,8 +767,12 @@ static void join_branch(mhd_ctx *c, int b)
 static void derived(mhd_ctx *c)
 {
     {
-        // J_cell reads only B (synced) and the geometry: beside B_cell / U_plus / dt
+        // J edges and J_cell read only B (synced) and the geometry: beside B_cell / U_plus / dt
         const hipStream_t qb = fork_branch(c, 1);
+        {
+            ProfScope p(c, KP_JEDGE);
+            mhdk::launch_jedge(launch_of(c, KP_JEDGE), mixed_of(c), qb);
+        }
         ProfScope p(c, KP_JCELL);
         mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), qb);
     }
@@ -863,6 +868,7 @@ static void enqueue_step(mhd_ctx *c)
         }
         {
             ProfScope p(c, KP_JCELL_RES);
+            mhdk::launch_jedge(launch_of(c, KP_CURL_RES), mixed, c->q); // two-fluid blocks, face range
             mhdk::launch_jcell(launch_of(c, KP_JCELL_RES), mixed, c->q);
         }
         {
@@ -908,7 +914,8 @@ static int autotune(mhd_ctx *c)
     if (c->h_blk.empty())
         return MHD_OK;
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int kernels[] = {KP_FACE, KP_EDGE, KP_FLUID, KP_CURL, KP_DERIVED, KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI};
+    const int kernels[] = {KP_FACE,   KP_EDGE,  KP_FLUID,    KP_CURL, KP_DERIVED,
+                           KP_JEDGE, KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -938,6 +945,9 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_JCELL:
                     mhdk::launch_jcell(L, mixed, c->q);
+                    break;
+                case KP_JEDGE:
+                    mhdk::launch_jedge(L, mixed, c->q);
                     break;
                 case KP_FLUID2:
                     mhdk::launch_fluid2(L, c->d_cst, c->d_zero, c->ucur, mixed, c->opt.recon, c->q);

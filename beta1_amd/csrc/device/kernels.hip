@@ -996,15 +996,49 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
         reduce_dt(local, dtbits, b.case_id);
 }
 
-// Derived fields, part 2, interior cells: Calc_J_Edge + calc_Jcell
-// (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891; CTOperators::
-// CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110).  The 12
-// J edges of a cell are recomputed from beta-weighted face fluxes and
-// Jcell_from_Jedge_w is formed in place: W[c][n] = Mi[c] . e_n for the active
-// edges, e_n = node(end) - node(start), with the host builder's arithmetic,
-// so J_cell accumulates the reference's W[c][n] * J_n in the reference's
-// order n = 0..11.  (Straight-line code: 256 VGPRs, but faster than a rolled
-// loop over the edge families at 106 VGPRs, 0.27 vs 0.33 ms on 3.1 M cells.)
+// Derived fields, part 2: Calc_J_Edge (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
+// 777-830; CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:
+// 59-110), J = alpha (CurlAdj of the beta-weighted face fluxes), each edge
+// of [0, N] once (its family's own range, as the reference's EdgeXi/Et/Ze
+// fields), into the J-edge scratch slots.
+template <typename WT>
+__global__ __launch_bounds__(256) void k_jedge(const MhdTileMap tm)
+{
+    MhdBlockDev b;
+    int i, j, k;
+    bool wg;
+    if (!tile_cell(tm, b, i, j, k, wg))
+        return;
+    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = dst_ptr(b);
+    const WT *G = dweights<WT>(b);
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const WT *bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *by = bx + sl, *bz = bx + 2 * sl;
+    const WT *ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl, *ay = ax + sl, *az = ax + 2 * sl;
+    double *JE = S + (int64_t)ST_JE * sl;
+    if (i < b.ni) // xi edges: i in [0, N), j, k in [0, N]
+    {
+        const double val = (by[o - dk] * By[o - dk] - by[o] * By[o]) + (bz[o] * Bz[o] - bz[o - dj] * Bz[o - dj]);
+        JE[o] = (1.0 * val) * ax[o];
+    }
+    if (j < b.nj)
+    {
+        const double val = (bx[o] * Bx[o] - bx[o - dk] * Bx[o - dk]) + (bz[o - di] * Bz[o - di] - bz[o] * Bz[o]);
+        JE[sl + o] = (1.0 * val) * ay[o];
+    }
+    if (k < b.nk)
+    {
+        const double val = (bx[o - dj] * Bx[o - dj] - bx[o] * Bx[o]) + (by[o] * By[o] - by[o - di] * By[o - di]);
+        JE[2 * sl + o] = (1.0 * val) * az[o];
+    }
+}
+
+// Derived fields, part 3, interior cells: calc_Jcell (LunarSolver_Tools.cpp:
+// 833-891) from the cell's 12 J edges.  Jcell_from_Jedge_w is formed in
+// place: W[c][n] = Mi[c] . e_n for the active edges, e_n = node(end) -
+// node(start), with the host builder's arithmetic, so J_cell accumulates the
+// reference's W[c][n] * J_n in the reference's order n = 0..11.
 template <typename WT>
 __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
@@ -1014,11 +1048,10 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     if (!tile_cell(tm, b, i, j, k, wg))
         return;
     const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = dst_ptr(b);
     const WT *G = dweights<WT>(b);
-    const double *Bf = S + (int64_t)ST_B * sl;
-    const WT *beta = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *alph = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl;
+    const double *JE = S + (int64_t)ST_JE * sl;
     const double *X = dmetric(b) + (int64_t)G_NODE * sl;
     const WT *Mw = G + (int64_t)(G_WJM - G_NMETRIC) * sl;
     double mi[9];
@@ -1026,30 +1059,15 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
     for (int q = 0; q < 9; ++q)
         mi[q] = (double)Mw[(int64_t)q * sl + o];
     const unsigned act = (unsigned)G[(int64_t)(G_WJA - G_NMETRIC) * sl + o];
-    const int64_t di = st[0], dj = st[1], dk = st[2];
-    const double *Bx = Bf, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const WT *bx = beta, *by = bx + sl, *bz = bx + 2 * sl;
-    const WT *ax = alph, *ay = ax + sl, *az = ax + 2 * sl;
-    auto jxi = [&](int64_t e) {
-        const double val = (by[e - dk] * By[e - dk] - by[e] * By[e]) + (bz[e] * Bz[e] - bz[e - dj] * Bz[e - dj]);
-        return (1.0 * val) * ax[e];
-    };
-    auto jet = [&](int64_t e) {
-        const double val = (bx[e] * Bx[e] - bx[e - dk] * Bx[e - dk]) + (bz[e - di] * Bz[e - di] - bz[e] * Bz[e]);
-        return (1.0 * val) * ay[e];
-    };
-    auto jze = [&](int64_t e) {
-        const double val = (bx[e - dj] * Bx[e - dj] - bx[e] * Bx[e]) + (by[e] * By[e] - by[e - di] * By[e - di]);
-        return (1.0 * val) * az[e];
-    };
-    const double sv[12] = {jxi(o), jxi(o + dj), jxi(o + dk), jxi(o + dj + dk), jet(o), jet(o + di),
-                          jet(o + dk), jet(o + di + dk), jze(o), jze(o + di), jze(o + dj), jze(o + di + dj)};
+    // edge n: family n / 4, first node o + lo[n]
     const int64_t lo[12] = {0, dj, dk, dj + dk, 0, di, dk, di + dk, 0, di, dj, di + dj};
     double jc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 0; n < 12; ++n)
     {
-        const int64_t a = o + lo[n], e = a + (n < 4 ? di : (n < 8 ? dj : dk));
+        const int fam = n >> 2;
+        const int64_t a = o + lo[n], e = a + (fam == 0 ? di : (fam == 1 ? dj : dk));
+        const double s = JE[(int64_t)fam * sl + a];
         double w0 = 0.0, w1 = 0.0, w2 = 0.0;
         if ((act >> n) & 1u)
         {
@@ -1058,9 +1076,9 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
             w1 = mi[3] * rx + mi[4] * ry + mi[5] * rz;
             w2 = mi[6] * rx + mi[7] * ry + mi[8] * rz;
         }
-        jc[0] += w0 * sv[n];
-        jc[1] += w1 * sv[n];
-        jc[2] += w2 * sv[n];
+        jc[0] += w0 * s;
+        jc[1] += w1 * s;
+        jc[2] += w2 * s;
     }
     // solid wall: J_cell = 0 on the two cell layers next to it
     // (Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:382-427)
@@ -1805,6 +1823,13 @@ void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dt
         hipLaunchKernelGGL((k_fluid_col<double>), g, bl, 0, q, C.tm, cst, dts, ucur);
 }
 int col_tile_rows() { return COL_TJ; }
+void launch_jedge(const Launch &L, int mixed, hipStream_t q)
+{
+    if (mixed)
+        hipLaunchKernelGGL(k_jedge<float>, grid_of(L), block_of(L), 0, q, L.tm);
+    else
+        hipLaunchKernelGGL(k_jedge<double>, grid_of(L), block_of(L), 0, q, L.tm);
+}
 void launch_jcell(const Launch &L, int mixed, hipStream_t q)
 {
     if (mixed)
