@@ -737,6 +737,14 @@ static void run_ops(mhd_ctx *c, int sy, int ushift)
 {
     mhdk::launch_ops(c->og[sy].d, (int)c->og[sy].n, c->d_cst, ushift, c->q);
 }
+// Two op lists in one launch: only for lists with disjoint destinations of
+// which neither reads what the other writes (U-slot and B-slot lists, the
+// B_cell and J_cell ghost lists).
+static void run_ops2(mhd_ctx *c, int sya, int ushift_a, int syb, int ushift_b)
+{
+    mhdk::launch_ops2(c->og[sya].d, (int)c->og[sya].n, ushift_a, c->og[syb].d, (int)c->og[syb].n, ushift_b, c->d_cst,
+                      c->q);
+}
 
 // Derived fields (contravariant velocities, B_cell and the next dt, then
 // J_cell); the J_cell wall condition (the SY_JCELL_BC op list of the host
@@ -790,10 +798,8 @@ static void derived(mhd_ctx *c)
     // B_cell and J_cell ghosts of the two-fluid cases with the Hall EMF
     // (the reference's B_cell and J_cell syncs; empty for the other cases)
     ProfScope p(c, KP_OPS);
-    run_ops(c, SY_BCELL_BC, 0);
-    run_ops(c, SY_BCELL_HALO, 0);
-    run_ops(c, SY_JCELL_GH, 0);
-    run_ops(c, SY_JCELL_HALO, 0);
+    run_ops2(c, SY_BCELL_BC, 0, SY_JCELL_GH, 0);
+    run_ops2(c, SY_BCELL_HALO, 0, SY_JCELL_HALO, 0);
 }
 
 // One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
@@ -894,12 +900,12 @@ static void enqueue_step(mhd_ctx *c)
     }
     c->ucur ^= 1;
     {
+        // U and B syncs: each field physical -> halo (the U wall ghosts after
+        // the wall's first layer); the U-slot and B-slot lists share launches
         ProfScope p(c, KP_OPS);
-        run_ops(c, SY_U_BC0, c->ucur);
-        run_ops(c, SY_U_BC1, c->ucur);
+        run_ops2(c, SY_U_BC0, c->ucur, SY_B_BC, 0);
+        run_ops2(c, SY_U_BC1, c->ucur, SY_B_HALO, 0);
         run_ops(c, SY_U_HALO, c->ucur);
-        run_ops(c, SY_B_BC, 0);
-        run_ops(c, SY_B_HALO, 0);
     }
     derived(c);
 }

@@ -567,6 +567,10 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
     const double *Bf = S + (int64_t)ST_B * sl, *Ba = S + (int64_t)ST_BADD * sl, *UC = S + (int64_t)ST_UC * sl;
     const int n[3] = {b.ni, b.nj, b.nk};
     const int idx[3] = {i, j, k};
+    // The point range is [0, N]^3 and every slot spans [-g, N + g] per axis
+    // with g >= 1, so the stencil of all three face families is in bounds at
+    // every point: the loads of the three families are issued unconditionally
+    // (together) and only the stores are limited to each family's own range.
 #pragma unroll
     for (int fax = 0; fax < 3; ++fax)
     {
@@ -575,8 +579,6 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             in = in && (idx[a] < n[a] + (a == fax ? 1 : 0));
-        if (!in)
-            continue;
         const int64_t oR = o, oL = o - step[fax];
         const double bff = Bf[(int64_t)fax * sl + o] + Ba[(int64_t)fax * sl + o];
         double e0 = 0.0, e1 = 0.0, e2 = 0.0;
@@ -604,6 +606,8 @@ __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
         // the component along the face normal is 0 and never read (the edge
         // EMF and the exchange use the other two): not stored
         double *Ef = S + (int64_t)(ST_EFACE + 3 * fax) * sl;
+        if (!in)
+            continue;
         if (fax != 0)
             Ef[o] = e0;
         if (fax != 1)
@@ -642,32 +646,38 @@ __global__ __launch_bounds__(256) void k_edge_emf(const MhdTileMap tm)
     const double *UCx = S + (int64_t)ST_UC * sl, *UCy = UCx + sl, *UCz = UCx + 2 * sl;
     double *E = S + (int64_t)ST_E * sl;
     const double Cuct = 0.5;
-    if (i < b.ni && j <= b.nj && k <= b.nk)
+    // The point range is [0, N]^3 and the stencil reaches -1 in each axis, so
+    // the three families' loads are in bounds everywhere and issue together;
+    // each family stores on its own range only.
+    const double Ec0 = 0.25 * (Fy[o] + Fy[o - dk] + Fz[o] + Fz[o - dj]);
+    const double Ec1 = 0.25 * (Fx[sl + o] + Fx[sl + o - dk] + Fz[sl + o] + Fz[sl + o - di]);
+    const double Ec2 = 0.25 * (Fx[2 * sl + o] + Fx[2 * sl + o - dj] + Fy[2 * sl + o] + Fy[2 * sl + o - di]);
     {
-        const double Ec = 0.25 * (Fy[o] + Fy[o - dk] + Fz[o] + Fz[o - dj]);
         const double ae = max_abs4(UCy, o, o - dj, o - dk, o - dj - dk);
         const double az = max_abs4(UCz, o, o - dj, o - dk, o - dj - dk);
         const double dBz = Bz[o] - Bz[o - dj];
         const double dBe = By[o] - By[o - dk];
-        E[o] = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
+        const double v = Ec0 + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
+        if (i < b.ni && j <= b.nj && k <= b.nk)
+            E[o] = v;
     }
-    if (i <= b.ni && j < b.nj && k <= b.nk)
     {
-        const double Ec = 0.25 * (Fx[sl + o] + Fx[sl + o - dk] + Fz[sl + o] + Fz[sl + o - di]);
         const double az = max_abs4(UCz, o, o - di, o - dk, o - di - dk);
         const double ax = max_abs4(UCx, o, o - di, o - dk, o - di - dk);
         const double dBx = Bx[o] - Bx[o - dk];
         const double dBz = Bz[o] - Bz[o - di];
-        E[sl + o] = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
+        const double v = Ec1 + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
+        if (i <= b.ni && j < b.nj && k <= b.nk)
+            E[sl + o] = v;
     }
-    if (i <= b.ni && j <= b.nj && k < b.nk)
     {
-        const double Ec = 0.25 * (Fx[2 * sl + o] + Fx[2 * sl + o - dj] + Fy[2 * sl + o] + Fy[2 * sl + o - di]);
         const double ax = max_abs4(UCx, o, o - di, o - dj, o - di - dj);
         const double ae = max_abs4(UCy, o, o - di, o - dj, o - di - dj);
         const double dBe = By[o] - By[o - di];
         const double dBx = Bx[o] - Bx[o - dj];
-        E[2 * sl + o] = Ec + Cuct * (0.5 * ax * dBe - 0.5 * ae * dBx);
+        const double v = Ec2 + Cuct * (0.5 * ax * dBe - 0.5 * ae * dBx);
+        if (i <= b.ni && j <= b.nj && k < b.nk)
+            E[2 * sl + o] = v;
     }
 }
 
@@ -689,12 +699,18 @@ __global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const 
     double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
     const double dt = dts[b.case_id];
     const double mult = -1.0;
+    // stencil o .. o + 1 in each axis over the points [0, N]^3: in bounds
+    // everywhere, so the loads of the three families issue together
+    const double rx = mult * ((Ey[o] - Ey[o + dk]) + (Ez[o + dj] - Ez[o]));
+    const double ry = mult * ((Ex[o + dk] - Ex[o]) + (Ez[o] - Ez[o + di]));
+    const double rz = mult * ((Ex[o] - Ex[o + dj]) + (Ey[o + di] - Ey[o]));
+    const double bx = Bx[o], by = By[o], bz = Bz[o];
     if (i <= b.ni && j < b.nj && k < b.nk)
-        Bx[o] += dt * (mult * ((Ey[o] - Ey[o + dk]) + (Ez[o + dj] - Ez[o])));
+        Bx[o] = bx + dt * rx;
     if (i < b.ni && j <= b.nj && k < b.nk)
-        By[o] += dt * (mult * ((Ex[o + dk] - Ex[o]) + (Ez[o] - Ez[o + di])));
+        By[o] = by + dt * ry;
     if (i < b.ni && j < b.nj && k <= b.nk)
-        Bz[o] += dt * (mult * ((Ex[o] - Ex[o + dj]) + (Ey[o + di] - Ey[o])));
+        Bz[o] = bz + dt * rz;
 }
 
 // ======================================================================
@@ -970,7 +986,10 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
             for (int a = 0; a < 3; ++a)
                 UC[a * sl + o] = uc[a];
         }
-        if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
+        // B_cell on the interior cells; its loads (face fluxes o .. o + 1, the
+        // cell's weights) are in bounds on the ring too, so they issue with
+        // the velocity's and only the stores and dt are limited to the interior
+        const bool inner = i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk;
         {
             const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
@@ -985,11 +1004,14 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
                 bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
                 bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
             }
-            double *Bc = S + (int64_t)ST_BCELL * sl;
+            if (inner)
+            {
+                double *Bc = S + (int64_t)ST_BCELL * sl;
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                Bc[a * sl + o] = bc[a];
-            local = cell_dt(cst[b.case_id], q, bc[0], bc[1], bc[2]);
+                for (int a = 0; a < 3; ++a)
+                    Bc[a * sl + o] = bc[a];
+                local = cell_dt(cst[b.case_id], q, bc[0], bc[1], bc[2]);
+            }
         }
     }
     if (wg)
@@ -1017,21 +1039,19 @@ __global__ __launch_bounds__(256) void k_jedge(const MhdTileMap tm)
     const WT *bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *by = bx + sl, *bz = bx + 2 * sl;
     const WT *ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl, *ay = ax + sl, *az = ax + 2 * sl;
     double *JE = S + (int64_t)ST_JE * sl;
-    if (i < b.ni) // xi edges: i in [0, N), j, k in [0, N]
-    {
-        const double val = (by[o - dk] * By[o - dk] - by[o] * By[o]) + (bz[o] * Bz[o] - bz[o - dj] * Bz[o - dj]);
-        JE[o] = (1.0 * val) * ax[o];
-    }
+    // points [0, N]^3, stencil o - 1 .. o per axis: in bounds everywhere, so
+    // the three families' loads issue together; each stores on its own range
+    // (xi edges: i in [0, N), j, k in [0, N]; and so on)
+    const double vx = (by[o - dk] * By[o - dk] - by[o] * By[o]) + (bz[o] * Bz[o] - bz[o - dj] * Bz[o - dj]);
+    const double vy = (bx[o] * Bx[o] - bx[o - dk] * Bx[o - dk]) + (bz[o - di] * Bz[o - di] - bz[o] * Bz[o]);
+    const double vz = (bx[o - dj] * Bx[o - dj] - bx[o] * Bx[o]) + (by[o] * By[o] - by[o - di] * By[o - di]);
+    const double jx = (1.0 * vx) * ax[o], jy = (1.0 * vy) * ay[o], jz = (1.0 * vz) * az[o];
+    if (i < b.ni)
+        JE[o] = jx;
     if (j < b.nj)
-    {
-        const double val = (bx[o] * Bx[o] - bx[o - dk] * Bx[o - dk]) + (bz[o - di] * Bz[o - di] - bz[o] * Bz[o]);
-        JE[sl + o] = (1.0 * val) * ay[o];
-    }
+        JE[sl + o] = jy;
     if (k < b.nk)
-    {
-        const double val = (bx[o - dj] * Bx[o - dj] - bx[o] * Bx[o]) + (by[o] * By[o] - by[o - di] * By[o - di]);
-        JE[2 * sl + o] = (1.0 * val) * az[o];
-    }
+        JE[2 * sl + o] = jz;
 }
 
 // Derived fields, part 3, interior cells: calc_Jcell (LunarSolver_Tools.cpp:
@@ -1068,14 +1088,13 @@ __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
         const int fam = n >> 2;
         const int64_t a = o + lo[n], e = a + (fam == 0 ? di : (fam == 1 ? dj : dk));
         const double s = JE[(int64_t)fam * sl + a];
-        double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-        if ((act >> n) & 1u)
-        {
-            const double rx = X[e] - X[a], ry = X[sl + e] - X[sl + a], rz = X[2 * sl + e] - X[2 * sl + a];
-            w0 = mi[0] * rx + mi[1] * ry + mi[2] * rz;
-            w1 = mi[3] * rx + mi[4] * ry + mi[5] * rz;
-            w2 = mi[6] * rx + mi[7] * ry + mi[8] * rz;
-        }
+        // the cube's nodes exist for every cell: load unconditionally (all 12
+        // edges' loads issue together) and select the inactive weights to 0
+        const double rx = X[e] - X[a], ry = X[sl + e] - X[sl + a], rz = X[2 * sl + e] - X[2 * sl + a];
+        const bool on = (act >> n) & 1u;
+        const double w0 = on ? mi[0] * rx + mi[1] * ry + mi[2] * rz : 0.0;
+        const double w1 = on ? mi[3] * rx + mi[4] * ry + mi[5] * rz : 0.0;
+        const double w2 = on ? mi[6] * rx + mi[7] * ry + mi[8] * rz : 0.0;
         jc[0] += w0 * s;
         jc[1] += w1 * s;
         jc[2] += w2 * s;
@@ -1170,7 +1189,10 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
                 UC[a * sl + o] = uc[a];
             S[(int64_t)ST_PE * sl + o] = ph.p + pn.p;
         }
-        if (i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk)
+        // B_cell on the interior cells; its loads (face fluxes o .. o + 1, the
+        // cell's weights) are in bounds on the ring too, so they issue with
+        // the velocity's and only the stores and dt are limited to the interior
+        const bool inner = i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk;
         {
             const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
@@ -1185,15 +1207,18 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
                 bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
                 bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
             }
-            double *Bc = S + (int64_t)ST_BCELL * sl;
+            if (inner)
+            {
+                double *Bc = S + (int64_t)ST_BCELL * sl;
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                Bc[a * sl + o] = bc[a];
-            local = cell_dt(c, q, bc[0], bc[1], bc[2]);
+                for (int a = 0; a < 3; ++a)
+                    Bc[a * sl + o] = bc[a];
+                local = cell_dt(c, q, bc[0], bc[1], bc[2]);
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                q.u[m] = un[m];
-            local = dmin(local, cell_dt(c, q, bc[0], bc[1], bc[2]));
+                for (int m = 0; m < 5; ++m)
+                    q.u[m] = un[m];
+                local = dmin(local, cell_dt(c, q, bc[0], bc[1], bc[2]));
+            }
         }
     }
     if (wg)
@@ -1573,13 +1598,8 @@ __device__ __forceinline__ double solid_p(double gm, double rho, double mx, doub
 
 // Op dst/src hold absolute device addresses; components are `stride`
 // elements apart.  `ushift` moves U ops to the second U buffer.
-__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n,
-                                             const MhdCaseConst *__restrict__ cst, int ushift)
+__device__ __forceinline__ void apply_op(const MhdOp &op, const MhdCaseConst *__restrict__ cst, int ushift)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    const MhdOp op = ops[t];
     const int64_t sh = ushift ? 5 * op.stride : 0;
     double *d = gptr(op.dst) + sh;
     const double *s = gptr(op.src) + sh;
@@ -1635,6 +1655,27 @@ __global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int 
     default:
         break;
     }
+}
+
+__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n,
+                                             const MhdCaseConst *__restrict__ cst, int ushift)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        apply_op(ops[t], cst, ushift);
+}
+
+// Two independent op lists (disjoint destinations, neither reads what the
+// other writes) in one launch: list a on threads [0, na), list b after it.
+__global__ __launch_bounds__(256) void k_ops2(const MhdOp *__restrict__ opa, int na, int ushift_a,
+                                              const MhdOp *__restrict__ opb, int nb, int ushift_b,
+                                              const MhdCaseConst *__restrict__ cst)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < na)
+        apply_op(opa[t], cst, ushift_a);
+    else if (t < na + nb)
+        apply_op(opb[t - na], cst, ushift_b);
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
@@ -1887,6 +1928,16 @@ void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ush
     if (n <= 0)
         return;
     hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, cst, ushift);
+}
+void launch_ops2(const mpcns::MhdOp *opa, int na, int ushift_a, const mpcns::MhdOp *opb, int nb, int ushift_b,
+                 const MhdCaseConst *cst, hipStream_t q)
+{
+    if (na <= 0)
+        return launch_ops(opb, nb, cst, ushift_b, q);
+    if (nb <= 0)
+        return launch_ops(opa, na, cst, ushift_a, q);
+    hipLaunchKernelGGL(k_ops2, dim3((na + nb + 255) / 256), dim3(256), 0, q, opa, na, ushift_a, opb, nb, ushift_b,
+                       cst);
 }
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q)
 {
