@@ -31,16 +31,16 @@ KERNEL_PARTS = {
     "k_edge_emf": (72 + 24 + 24 + 24, 0, 0),       # Eface, UC, B -> E
     "k_fluid": (40 + 24 + 24 + 40, 72 + 8, 0),     # U, B_cell, J_cell -> U' | JD, Jac
     "k_curl_update": (24 + 24 + 24, 0, 0),         # E, B -> B
-    # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | W_B
-    "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8, 144),
+    # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | B_cell least squares Mi (6), mu (6)
+    "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8, 96),
     # B -> J edges | beta, alpha
     "k_jedge": (24 + 24, 0, 24 + 24),
-    # J edges -> J_cell | nodes | Mi, active-edge mask
-    "k_jcell": (24 + 24, 24, 72 + 8),
+    # J edges -> J_cell | nodes | Mi (6), active-edge mask
+    "k_jcell": (24 + 24, 24, 48 + 8),
     # two-fluid step: U_H, U_Na -> U_H', U_Na', B_cell, J_cell, p_e, photoionization, drag | JD, Jac
     "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8, 0),
-    # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac | W_B
-    "k_derived2": (80 + 24 + 24 + 24 + 8 + 24, 72 + 8, 144),
+    # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac | Mi, mu
+    "k_derived2": (80 + 24 + 24 + 24 + 8 + 24, 72 + 8, 96),
     # E R/W, p_e, rho_H, rho_Na, B, B_cell, J_cell, Hall taper, eta | JD, nodes (dr) | beta, alpha
     "k_emf_extra": (48 + 8 + 16 + 24 + 24 + 24 + 24 + 24, 72 + 24, 24 + 24),
 }
@@ -54,13 +54,14 @@ KERNEL_BYTES = kernel_bytes("fp64")
 HBM_PEAK_GBS = 8000.0
 # Whole-step algorithmic bytes per cell-update, each array once per step:
 # U R/W 80 + B_face R/W 48 + Badd R 24 (per case) + geometry: metric Jac 8,
-# JD 72, nodes 24 (FP64 always) and the weights beta 24, alpha 24, B_cell
-# table 144, J_cell Mi 72 + mask 8 (FP64, float in mixed precision), counted
+# JD 72, nodes 24 (FP64 always) and the weights beta 24, alpha 24, the B_cell
+# least squares Mi 48 + mu 48, the J_cell Mi 48 + mask 8 (FP64, float in
+# mixed precision), counted
 # once per case (K = 1: the geometry-shared tile order lets cases of one grid
 # meet their geometry in L2, which is a cache effect, not a guaranteed single
 # read).
 STATE_BYTES = 152
-GEO_BYTES = {"fp64": 104 + 272, "mixed": 104 + 136}
+GEO_BYTES = {"fp64": 104 + 200, "mixed": 104 + 100}
 STEP_BYTES = STATE_BYTES + GEO_BYTES["fp64"]
 SEED = 20261203
 

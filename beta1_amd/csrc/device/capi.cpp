@@ -87,11 +87,15 @@ struct TileSet
     int *d_lead_k = nullptr, *d_lead_moff = nullptr, *d_members = nullptr;
     size_t cap_k = 0, cap_moff = 0, cap_mem = 0;
     int nent = 0;
-    TileArr tiles[R_COUNT][3]; // [range][rows]
+    TileArr tiles[R_COUNT][3];  // [range][rows]
+    TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
     TileArr cols;              // column tiles of the interior cells (kColCi planes each)
 };
 // planes along i per column-tile workgroup
 constexpr int kColCi = 8;
+// planes along i per plane-chunk tile (kernels that march along i and carry
+// the values plane i - 1 shares with plane i in registers): two sizes
+constexpr int kChunkCi[2] = {4, 8};
 // block sets: every block, the single-fluid blocks, the two-fluid blocks
 enum BlockSet
 {
@@ -262,6 +266,10 @@ extern "C" int mhd_destroy(mhd_ctx *c)
             for (auto &rr : T.tiles)
                 for (auto &t : rr)
                     free_t(t.d_start);
+            for (auto &cc : T.chunks)
+                for (auto &rr : cc)
+                    for (auto &t : rr)
+                        free_t(t.d_start);
             free_t(T.cols.d_start);
             free_t(T.d_lead_k);
             free_t(T.d_lead_moff);
@@ -413,10 +421,11 @@ static int build_tiles(mhd_ctx *c)
             }
             const bool ident = T.members.empty();
             for (int r = 0; r < R_COUNT; ++r)
-                for (int ji = 0; ji < 3; ++ji)
+                for (int ji = 0; ji < 9; ++ji)
                 {
-                    TileArr &A = T.tiles[r][ji];
-                    const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
+                    const int ch = ji >= 3 ? kChunkCi[ji / 3 - 1] : 1;
+                    TileArr &A = ji >= 3 ? T.chunks[ji / 3 - 1][r][ji % 3] : T.tiles[r][ji];
+                    const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji % 3];
                     std::vector<int> start(std::max(T.nent, 1), 0);
                     long long total = 0;
                     for (int e = 0; e < T.nent; ++e)
@@ -424,7 +433,8 @@ static int build_tiles(mhd_ctx *c)
                         const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
                         const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
                         start[e] = (int)total;
-                        total += (long long)(ident ? 1 : T.lead_k[e]) * ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
+                        total += (long long)(ident ? 1 : T.lead_k[e]) * ((ei + ch - 1) / ch) *
+                                 ((ej * ek + 64 * jt - 1) / (64 * jt));
                     }
                     if (total > (1LL << 31) - 64)
                         return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
@@ -470,13 +480,27 @@ static int set_of(const mhd_ctx *c, int kernel)
     return BS_ALL;
 }
 
+// Plane-chunk size of a kernel (index into kChunkCi), -1 for one plane per
+// tile; measured per kernel on the 6 x 64 x 64 x 128 shell: the face EMF
+// 152 -> 102 us at 4 planes, J_cell 153 -> 128 us at 8; the edge EMF is
+// faster per point (its carried values halve the occupancy).
+static int chunk_of(int kernel)
+{
+    if (kernel == KP_FACE)
+        return 0;
+    if (kernel == KP_JCELL || kernel == KP_JCELL_RES)
+        return 1;
+    return -1;
+}
+
 static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
 {
     const int r = kKernelRange[kernel];
     const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
     const int ji = v % 3, sh = v >= 6 ? 1 : 0;
     const TileSet &S = c->ts[set_of(c, kernel)][sh];
-    const TileArr &T = S.tiles[r][ji];
+    const int ch = chunk_of(kernel);
+    const TileArr &T = ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
     const bool ident = S.members.empty();
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
@@ -490,7 +514,7 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
     L.tm.i_inner = (v % 6) >= 3;
-    L.tm.ci = 0;
+    L.tm.ci = ch >= 0 ? kChunkCi[ch] : 0;
     L.jt = kJt[ji];
     return L;
 }
@@ -675,6 +699,7 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         }
         std::vector<double>().swap(r.cs.blk[b].geo);
         std::vector<double>().swap(r.cs.blk[b].wj);
+        std::vector<double>().swap(r.cs.blk[b].wb);
         std::vector<double>().swap(r.cs.blk[b].st);
     }
     r.st_bytes = st_bytes;
@@ -1315,6 +1340,67 @@ extern "C" int mhd_diagnostics(mhd_ctx *c, int case_id, double *out)
 }
 
 // ----------------------------------------------------------- data exchange
+// Geometry slots [slot, slot + nc) of a block as doubles (weight slots of a
+// mixed-precision context are stored as float).
+static int geo_download(mhd_ctx *c, const MhdBlockDev &d, int slot, int nc, std::vector<double> &out)
+{
+    out.assign((size_t)nc * d.slot, 0.0);
+    if (mixed_of(c) && slot >= G_NMETRIC)
+    {
+        std::vector<float> f32(out.size());
+        HIPCHK(c, hipMemcpy(f32.data(), reinterpret_cast<const void *>((uintptr_t)(d.geo + geo_slot_byte(slot, d.slot, 4))),
+                            sizeof(float) * f32.size(), hipMemcpyDeviceToHost));
+        out.assign(f32.begin(), f32.end());
+        return MHD_OK;
+    }
+    HIPCHK(c, hipMemcpy(out.data(), reinterpret_cast<const void *>((uintptr_t)(d.geo + geo_slot_byte(slot, d.slot, 8))),
+                        sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+    return MHD_OK;
+}
+
+// Bcell_from_Bface_w of a device block, formed from Mi, mu and the face area
+// vectors with the B_cell kernel's (and the host builder's) arithmetic;
+// interior cells only, as the host table.  The device keeps no table to upload.
+static int bcell_weights_down(mhd_ctx *c, const MhdBlockDev &d, bool down, double *aos)
+{
+    if (!down)
+        return fail(c, MHD_ERR_ARG, "Bcell_from_Bface_w is formed on the device from its least-squares data: "
+                                    "it cannot be uploaded");
+    std::vector<double> mi, mu, jd;
+    int rc;
+    if ((rc = geo_download(c, d, G_WBM, 6, mi)) || (rc = geo_download(c, d, G_WBU, 6, mu)) ||
+        (rc = geo_download(c, d, G_JD, 9, jd)))
+        return rc;
+    const int64_t sl = d.slot, step[3] = {(int64_t)d.sj * d.sk, (int64_t)d.sk, 1};
+    std::vector<double> soa((size_t)18 * sl, 0.0);
+    for (int i = 0; i < d.ni; ++i)
+        for (int j = 0; j < d.nj; ++j)
+            for (int k = 0; k < d.nk; ++k)
+            {
+                const int64_t o = blk_off(d, i, j, k);
+                const double m00 = mi[o], m01 = mi[sl + o], m02 = mi[2 * sl + o], m11 = mi[3 * sl + o],
+                             m12 = mi[4 * sl + o], m22 = mi[5 * sl + o];
+                for (int n = 0; n < 6; ++n)
+                {
+                    const int ax = n >> 1;
+                    const int64_t oo = o + ((n & 1) ? step[ax] : 0);
+                    double sv[3];
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        const double v = jd[(size_t)(3 * ax + a) * sl + oo];
+                        sv[a] = (n & 1) ? v : -v;
+                    }
+                    const double m = mu[(size_t)n * sl + o];
+                    const double rx = m * sv[0], ry = m * sv[1], rz = m * sv[2];
+                    soa[(size_t)n * sl + o] = m00 * rx + m01 * ry + m02 * rz;
+                    soa[(size_t)(6 + n) * sl + o] = m01 * rx + m11 * ry + m12 * rz;
+                    soa[(size_t)(12 + n) * sl + o] = m02 * rx + m12 * ry + m22 * rz;
+                }
+            }
+    soa_aos(d, LOC_CELL, 18, soa.data(), aos, true);
+    return MHD_OK;
+}
+
 static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, size_t n, bool down)
 {
     if (!aos || !c)
@@ -1333,6 +1419,8 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
     if (n != aos_count(d, loc, nc))
         return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
     HIPCHK(c, hipStreamSynchronize(c->q));
+    if (region == 3)
+        return bcell_weights_down(c, d, down, aos);
     std::vector<double> soa((size_t)nc * d.slot);
     if (region == 1 && mixed_of(c) && slot >= G_NMETRIC)
     {

@@ -142,6 +142,54 @@ __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, 
     return f < plane;
 }
 
+// Plane-chunk tiles: as tile_cell, but a tile spans the planes
+// [i0, i1) (tm.ci planes, fewer in the block's last chunk) at one (j, k),
+// so a thread marches along i and keeps what plane i - 1 shares with plane
+// i in registers.  With tm.ci <= 1 a tile is one plane (i1 = i0 + 1).
+__device__ __forceinline__ bool tile_planes(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &i1, int &j, int &k,
+                                            bool &wg)
+{
+    const int ci = tm.ci > 1 ? tm.ci : 1;
+    int t = (int)blockIdx.x;
+    if (tm.xcd_chunk > 0)
+        t = (t & 7) * tm.xcd_chunk + (t >> 3);
+    wg = t < tm.ntiles;
+    if (!wg)
+        return false;
+    int bi = 0, hi = tm.nblk - 1;
+    while (bi < hi)
+    {
+        const int mid = (bi + hi + 1) >> 1;
+        if (tm.block_start[mid] <= t)
+            bi = mid;
+        else
+            hi = mid - 1;
+    }
+    int lt = t - tm.block_start[bi];
+    if (tm.members)
+    {
+        const int kk = tm.lead_k[bi];
+        const int m = lt % kk;
+        lt /= kk;
+        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
+    }
+    else
+        b = tm.blks[bi];
+    const int per_tile = 64 * (int)blockDim.y;
+    const int ext_i = b.ni - tm.lo + tm.hi, ext_j = b.nj - tm.lo + tm.hi, ext_k = b.nk - tm.lo + tm.hi;
+    const int nch = (ext_i + ci - 1) / ci;
+    const int plane = ext_j * ext_k;
+    const int ntp = (plane + per_tile - 1) / per_tile;
+    const int pt = tm.i_inner ? lt / nch : lt % ntp;
+    const int iy = tm.i_inner ? lt % nch : lt / ntp;
+    const int f = pt * per_tile + (int)threadIdx.y * 64 + (int)threadIdx.x;
+    i0 = tm.lo + iy * ci;
+    i1 = min(i0 + ci, tm.lo + ext_i);
+    j = tm.lo + f / ext_k;
+    k = tm.lo + f % ext_k;
+    return f < plane;
+}
+
 // ------------------------------------------------------------------ state
 struct Prim
 {
@@ -370,6 +418,37 @@ __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__
     }
 }
 
+// B_cell = sum_n W[c][n] pt[n] (calc_Bcell, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
+// 194-236) with Bcell_from_Bface_w formed in place from the stored
+// least-squares data: W[c][n] = Mi[c] . (mu_n s_n), s_n = -+ the area vector
+// of face n (q.a), in the host builder's order of operations.
+template <typename WT>
+__device__ __forceinline__ void bcell_from_faces(const MhdBlockDev &b, int64_t o, const CellGeo &q, const double *pt,
+                                                 double *bc)
+{
+    const int64_t sl = b.slot;
+    const WT *Wm = dweights<WT>(b) + (int64_t)(G_WBM - G_NMETRIC) * sl;
+    const WT *Wu = dweights<WT>(b) + (int64_t)(G_WBU - G_NMETRIC) * sl;
+    const double m00 = Wm[o], m01 = Wm[sl + o], m02 = Wm[2 * sl + o], m11 = Wm[3 * sl + o], m12 = Wm[4 * sl + o],
+                 m22 = Wm[5 * sl + o];
+    bc[0] = bc[1] = bc[2] = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+    {
+        const double mu = (double)Wu[(int64_t)n * sl + o];
+        const double s0 = (n & 1) ? q.a[n][0] : -q.a[n][0];
+        const double s1 = (n & 1) ? q.a[n][1] : -q.a[n][1];
+        const double s2 = (n & 1) ? q.a[n][2] : -q.a[n][2];
+        const double rx = mu * s0, ry = mu * s1, rz = mu * s2;
+        const double w0 = m00 * rx + m01 * ry + m02 * rz;
+        const double w1 = m01 * rx + m11 * ry + m12 * rz;
+        const double w2 = m02 * rx + m12 * ry + m22 * rz;
+        bc[0] += w0 * pt[n];
+        bc[1] += w1 * pt[n];
+        bc[2] += w2 * pt[n];
+    }
+}
+
 __device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &q, double Bx, double By, double Bz)
 {
     double local = 1e100;
@@ -535,85 +614,128 @@ __global__ __launch_bounds__(256) void k_ucontra(const MhdTileMap tm, const MhdC
 // Face EMF candidates: AssembleOneDirectionEMF_
 // (Z4_Lunar/src/physics/LunarSolver_MagExplicit_Ideal.cpp:348-496).  FP64 in
 // both precisions: the EMFs feed the constrained-transport curl.
-__device__ __forceinline__ double emf_flux(const double *__restrict__ Bf, const double *__restrict__ Ba,
-                                           const double *__restrict__ UC, int64_t sl, const int64_t *step, int bax,
-                                           int fax, int64_t oL, int64_t oR, double bff)
+// One candidate of a face from its left and right cells: bt = face-B average
+// with the background along the transverse axis, bi = the induced face-B
+// average, uf / ub = contravariant velocities along the face normal / the
+// transverse axis (G = u_f bt - u_b B_face,tot; the result is
+// 1/2 (G_L + G_R) - 1/2 max(|u_f|) (bi_R - bi_L), :430-450).
+__device__ __forceinline__ double emf_pair(double btL, double btR, double biL, double biR, double ufL, double ufR,
+                                           double ubL, double ubR, double bff)
 {
-    const double *Bb = Bf + (int64_t)bax * sl, *Ab = Ba + (int64_t)bax * sl;
-    const int64_t s = step[bax];
-    const double bL0 = Bb[oL], bL1 = Bb[oL + s], bR0 = Bb[oR], bR1 = Bb[oR + s];
-    const double btL = 0.5 * ((bL0 + Ab[oL]) + (bL1 + Ab[oL + s]));
-    const double btR = 0.5 * ((bR0 + Ab[oR]) + (bR1 + Ab[oR + s]));
-    const double biL = 0.5 * (bL0 + bL1);
-    const double biR = 0.5 * (bR0 + bR1);
-    const double ufL = UC[(int64_t)fax * sl + oL], ufR = UC[(int64_t)fax * sl + oR];
-    const double ubL = UC[(int64_t)bax * sl + oL], ubR = UC[(int64_t)bax * sl + oR];
     const double GL = ufL * btL - ubL * bff;
     const double GR = ufR * btR - ubR * bff;
     const double radius = dmax(fabs(ufL), fabs(ufR));
     return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
 }
 
+// Face EMF candidates over plane-chunk tiles of the points [0, N]^3.  At
+// point o the xi, eta and zeta faces share the right cell o; their left
+// cells are o - di, o - dj, o - dk.  A cell's averages along axis a are
+//   bt_a = 1/2 ((B_a + Badd_a)(X) + (B_a + Badd_a)(X + s_a)),
+//   bi_a = 1/2 (B_a(X) + B_a(X + s_a)),
+// each formed once per cell and shared by the faces that need it.  The
+// cell o - di of plane i is cell o of plane i - 1, and the xi fluxes at
+// o, o - dj, o - dk of plane i were loaded at + di on plane i - 1: both are
+// carried in registers.  Every load is in bounds at every point of the range
+// (stencil -1 .. +1, slots span [-g, N + g], g >= 1); each family stores on
+// its own range only.
 __global__ __launch_bounds__(256) void k_face_emf(const MhdTileMap tm)
 {
     MhdBlockDev b;
-    int i, j, k;
+    int i0, i1, j, k;
     bool wg;
-    if (!tile_cell(tm, b, i, j, k, wg))
+    if (!tile_planes(tm, b, i0, i1, j, k, wg))
         return;
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    const int64_t sl = b.slot;
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = dst_ptr(b);
-    const double *Bf = S + (int64_t)ST_B * sl, *Ba = S + (int64_t)ST_BADD * sl, *UC = S + (int64_t)ST_UC * sl;
-    const int n[3] = {b.ni, b.nj, b.nk};
-    const int idx[3] = {i, j, k};
-    // The point range is [0, N]^3 and every slot spans [-g, N + g] per axis
-    // with g >= 1, so the stencil of all three face families is in bounds at
-    // every point: the loads of the three families are issued unconditionally
-    // (together) and only the stores are limited to each family's own range.
-#pragma unroll
-    for (int fax = 0; fax < 3; ++fax)
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+    const double *U0 = S + (int64_t)ST_UC * sl, *U1 = U0 + sl, *U2 = U0 + 2 * sl;
+    double *E0 = S + (int64_t)ST_EFACE * sl, *E1 = E0 + 3 * sl, *E2 = E0 + 6 * sl;
+    const bool jin = j < b.nj, kin = k < b.nk, jon = j <= b.nj, kon = k <= b.nk;
+    int64_t o = blk_off(b, i0, j, k);
+    // left cell of the xi face (o - di): eta / zeta averages and velocities
+    double lbt1, lbt2, lbi1, lbi2, lu0, lu1, lu2;
     {
-        // face normal to fax: idx[fax] in [0, n], others in [0, n)
-        bool in = true;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            in = in && (idx[a] < n[a] + (a == fax ? 1 : 0));
-        const int64_t oR = o, oL = o - step[fax];
-        const double bff = Bf[(int64_t)fax * sl + o] + Ba[(int64_t)fax * sl + o];
-        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-        if (fax == 0)
+        const int64_t x = o - di;
+        const double y0 = By[x], y1 = By[x + dj], z0 = Bz[x], z1 = Bz[x + dk];
+        lbt1 = 0.5 * ((y0 + Ay[x]) + (y1 + Ay[x + dj]));
+        lbt2 = 0.5 * ((z0 + Az[x]) + (z1 + Az[x + dk]));
+        lbi1 = 0.5 * (y0 + y1);
+        lbi2 = 0.5 * (z0 + z1);
+        lu0 = U0[x];
+        lu1 = U1[x];
+        lu2 = U2[x];
+    }
+    // xi fluxes (B, Badd) at o, o - dj, o - dk of the current plane
+    double bx = Bx[o], ax = Ax[o], bxj = Bx[o - dj], axj = Ax[o - dj], bxk = Bx[o - dk], axk = Ax[o - dk];
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        // right cell o
+        const double nbx = Bx[o + di], nax = Ax[o + di];
+        const double y0 = By[o], y1 = By[o + dj], z0 = Bz[o], z1 = Bz[o + dk];
+        const double ay0 = Ay[o], az0 = Az[o];
+        const double rbt0 = 0.5 * ((bx + ax) + (nbx + nax));
+        const double rbt1 = 0.5 * ((y0 + ay0) + (y1 + Ay[o + dj]));
+        const double rbt2 = 0.5 * ((z0 + az0) + (z1 + Az[o + dk]));
+        const double rbi0 = 0.5 * (bx + nbx), rbi1 = 0.5 * (y0 + y1), rbi2 = 0.5 * (z0 + z1);
+        const double ru0 = U0[o], ru1 = U1[o], ru2 = U2[o];
+        // cell o - dj (left of the eta face): xi and zeta averages
+        const double nbxj = Bx[o - dj + di], naxj = Ax[o - dj + di];
+        const double jz0 = Bz[o - dj], jz1 = Bz[o - dj + dk];
+        const double jbt0 = 0.5 * ((bxj + axj) + (nbxj + naxj));
+        const double jbt2 = 0.5 * ((jz0 + Az[o - dj]) + (jz1 + Az[o - dj + dk]));
+        const double jbi0 = 0.5 * (bxj + nbxj), jbi2 = 0.5 * (jz0 + jz1);
+        const double ju0 = U0[o - dj], ju1 = U1[o - dj], ju2 = U2[o - dj];
+        // cell o - dk (left of the zeta face): xi and eta averages
+        const double nbxk = Bx[o - dk + di], naxk = Ax[o - dk + di];
+        const double ky0 = By[o - dk], ky1 = By[o - dk + dj];
+        const double kbt0 = 0.5 * ((bxk + axk) + (nbxk + naxk));
+        const double kbt1 = 0.5 * ((ky0 + Ay[o - dk]) + (ky1 + Ay[o - dk + dj]));
+        const double kbi0 = 0.5 * (bxk + nbxk), kbi1 = 0.5 * (ky0 + ky1);
+        const double ku0 = U0[o - dk], ku1 = U1[o - dk], ku2 = U2[o - dk];
+        // xi face: candidates along zeta (e1) and eta (e2)
+        if (i <= b.ni && jin && kin)
         {
-            const double Ge = emf_flux(Bf, Ba, UC, sl, step, 1, 0, oL, oR, bff);
-            const double Gz = emf_flux(Bf, Ba, UC, sl, step, 2, 0, oL, oR, bff);
-            e1 = Gz;
-            e2 = -Ge;
+            const double bff = bx + ax;
+            const double Ge = emf_pair(lbt1, rbt1, lbi1, rbi1, lu0, ru0, lu1, ru1, bff);
+            const double Gz = emf_pair(lbt2, rbt2, lbi2, rbi2, lu0, ru0, lu2, ru2, bff);
+            E0[sl + o] = Gz;
+            E0[2 * sl + o] = -Ge;
         }
-        else if (fax == 1)
+        // eta face: candidates along xi (e0) and zeta (e2)
+        if (i < b.ni && jon && kin)
         {
-            const double Gx = emf_flux(Bf, Ba, UC, sl, step, 0, 1, oL, oR, bff);
-            const double Gz = emf_flux(Bf, Ba, UC, sl, step, 2, 1, oL, oR, bff);
-            e0 = -Gz;
-            e2 = Gx;
+            const double bff = y0 + ay0;
+            const double Gx = emf_pair(jbt0, rbt0, jbi0, rbi0, ju1, ru1, ju0, ru0, bff);
+            const double Gz = emf_pair(jbt2, rbt2, jbi2, rbi2, ju1, ru1, ju2, ru2, bff);
+            E1[o] = -Gz;
+            E1[2 * sl + o] = Gx;
         }
-        else
+        // zeta face: candidates along xi (e0) and eta (e1)
+        if (i < b.ni && jin && kon)
         {
-            const double Gx = emf_flux(Bf, Ba, UC, sl, step, 0, 2, oL, oR, bff);
-            const double Ge = emf_flux(Bf, Ba, UC, sl, step, 1, 2, oL, oR, bff);
-            e0 = Ge;
-            e1 = -Gx;
+            const double bff = z0 + az0;
+            const double Gx = emf_pair(kbt0, rbt0, kbi0, rbi0, ku2, ru2, ku0, ru0, bff);
+            const double Ge = emf_pair(kbt1, rbt1, kbi1, rbi1, ku2, ru2, ku1, ru1, bff);
+            E2[o] = Ge;
+            E2[sl + o] = -Gx;
         }
-        // the component along the face normal is 0 and never read (the edge
-        // EMF and the exchange use the other two): not stored
-        double *Ef = S + (int64_t)(ST_EFACE + 3 * fax) * sl;
-        if (!in)
-            continue;
-        if (fax != 0)
-            Ef[o] = e0;
-        if (fax != 1)
-            Ef[sl + o] = e1;
-        if (fax != 2)
-            Ef[2 * sl + o] = e2;
+        lbt1 = rbt1;
+        lbt2 = rbt2;
+        lbi1 = rbi1;
+        lbi2 = rbi2;
+        lu0 = ru0;
+        lu1 = ru1;
+        lu2 = ru2;
+        bx = nbx;
+        ax = nax;
+        bxj = nbxj;
+        axj = naxj;
+        bxk = nbxk;
+        axk = naxk;
     }
 }
 
@@ -995,15 +1117,8 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
             const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
                                   By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
-            const WT *W = dweights<WT>(b) + (int64_t)(G_WB - G_NMETRIC) * sl;
-            double bc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int n = 0; n < 6; ++n)
-            {
-                bc[0] += W[(int64_t)n * sl + o] * pt[n];
-                bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
-                bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
-            }
+            double bc[3];
+            bcell_from_faces<WT>(b, o, q, pt, bc);
             if (inner)
             {
                 double *Bc = S + (int64_t)ST_BCELL * sl;
@@ -1059,57 +1174,96 @@ __global__ __launch_bounds__(256) void k_jedge(const MhdTileMap tm)
 // place: W[c][n] = Mi[c] . e_n for the active edges, e_n = node(end) -
 // node(start), with the host builder's arithmetic, so J_cell accumulates the
 // reference's W[c][n] * J_n in the reference's order n = 0..11.
+//
+// Over plane-chunk tiles: the cube's nodes are nd[0..3] = o + {0, dj, dk,
+// dj + dk} on plane i and nd[4..7] the same + di; edge n runs from node
+// kEs[n] to kEe[n] (xi edges 0-3, eta 4-7, zeta 8-11 as the J-edge
+// families).  The plane-(i + 1) nodes and the eta / zeta J edges at + di are
+// the next plane's own, so they are carried in registers.
 template <typename WT>
 __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
+    constexpr int kEs[12] = {0, 1, 2, 3, 0, 4, 2, 6, 0, 4, 1, 5};
+    constexpr int kEe[12] = {4, 5, 6, 7, 1, 5, 3, 7, 2, 6, 3, 7};
     MhdBlockDev b;
-    int i, j, k;
+    int i0, i1, j, k;
     bool wg;
-    if (!tile_cell(tm, b, i, j, k, wg))
+    if (!tile_planes(tm, b, i0, i1, j, k, wg))
         return;
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+    const int64_t sl = b.slot;
     const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
     double *S = dst_ptr(b);
     const WT *G = dweights<WT>(b);
-    const double *JE = S + (int64_t)ST_JE * sl;
+    const double *J0 = S + (int64_t)ST_JE * sl, *J1 = J0 + sl, *J2 = J0 + 2 * sl;
     const double *X = dmetric(b) + (int64_t)G_NODE * sl;
     const WT *Mw = G + (int64_t)(G_WJM - G_NMETRIC) * sl;
-    double mi[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q)
-        mi[q] = (double)Mw[(int64_t)q * sl + o];
-    const unsigned act = (unsigned)G[(int64_t)(G_WJA - G_NMETRIC) * sl + o];
-    // edge n: family n / 4, first node o + lo[n]
-    const int64_t lo[12] = {0, dj, dk, dj + dk, 0, di, dk, di + dk, 0, di, dj, di + dj};
-    double jc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int n = 0; n < 12; ++n)
-    {
-        const int fam = n >> 2;
-        const int64_t a = o + lo[n], e = a + (fam == 0 ? di : (fam == 1 ? dj : dk));
-        const double s = JE[(int64_t)fam * sl + a];
-        // the cube's nodes exist for every cell: load unconditionally (all 12
-        // edges' loads issue together) and select the inactive weights to 0
-        const double rx = X[e] - X[a], ry = X[sl + e] - X[sl + a], rz = X[2 * sl + e] - X[2 * sl + a];
-        const bool on = (act >> n) & 1u;
-        const double w0 = on ? mi[0] * rx + mi[1] * ry + mi[2] * rz : 0.0;
-        const double w1 = on ? mi[3] * rx + mi[4] * ry + mi[5] * rz : 0.0;
-        const double w2 = on ? mi[6] * rx + mi[7] * ry + mi[8] * rz : 0.0;
-        jc[0] += w0 * s;
-        jc[1] += w1 * s;
-        jc[2] += w2 * s;
-    }
+    const WT *Aw = G + (int64_t)(G_WJA - G_NMETRIC) * sl;
+    double *Jc = S + (int64_t)ST_JCELL * sl;
+    const int64_t q[4] = {0, dj, dk, dj + dk};
     // solid wall: J_cell = 0 on the two cell layers next to it
     // (Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:382-427)
-    const int idx[3] = {i, j, k}, nn[3] = {b.ni, b.nj, b.nk};
+    bool wall_jk = false;
+    if (b.bc[2] == BC_SOLID && j < 2)
+        wall_jk = true;
+    if (b.bc[3] == BC_SOLID && j >= b.nj - 2)
+        wall_jk = true;
+    if (b.bc[4] == BC_SOLID && k < 2)
+        wall_jk = true;
+    if (b.bc[5] == BC_SOLID && k >= b.nk - 2)
+        wall_jk = true;
+    int64_t o = blk_off(b, i0, j, k);
+    double nd[8][3];
 #pragma unroll
-    for (int f = 0; f < 6; ++f)
-        if (b.bc[f] == BC_SOLID && ((f & 1) ? idx[f >> 1] >= nn[f >> 1] - 2 : idx[f >> 1] < 2))
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            nd[m][c] = X[(int64_t)c * sl + o + q[m]];
+    double j1a = J1[o], j1b = J1[o + dk], j2a = J2[o], j2b = J2[o + dj];
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                nd[4 + m][c] = X[(int64_t)c * sl + o + di + q[m]];
+        // Mi is symmetric: rows (00 01 02) (01 11 12) (02 12 22)
+        const double m00 = Mw[o], m01 = Mw[sl + o], m02 = Mw[2 * sl + o], m11 = Mw[3 * sl + o], m12 = Mw[4 * sl + o],
+                     m22 = Mw[5 * sl + o];
+        const double mi[9] = {m00, m01, m02, m01, m11, m12, m02, m12, m22};
+        const unsigned act = (unsigned)Aw[o];
+        const double s[12] = {J0[o],      J0[o + dj], J0[o + dk], J0[o + dj + dk], j1a, J1[o + di],
+                              j1b,        J1[o + di + dk], j2a, J2[o + di], j2b, J2[o + di + dj]};
+        double jc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < 12; ++n)
+        {
+            const double rx = nd[kEe[n]][0] - nd[kEs[n]][0], ry = nd[kEe[n]][1] - nd[kEs[n]][1],
+                         rz = nd[kEe[n]][2] - nd[kEs[n]][2];
+            const bool on = (act >> n) & 1u;
+            const double w0 = on ? mi[0] * rx + mi[1] * ry + mi[2] * rz : 0.0;
+            const double w1 = on ? mi[3] * rx + mi[4] * ry + mi[5] * rz : 0.0;
+            const double w2 = on ? mi[6] * rx + mi[7] * ry + mi[8] * rz : 0.0;
+            jc[0] += w0 * s[n];
+            jc[1] += w1 * s[n];
+            jc[2] += w2 * s[n];
+        }
+        const bool wall = wall_jk || (b.bc[0] == BC_SOLID && i < 2) || (b.bc[1] == BC_SOLID && i >= b.ni - 2);
+        if (wall)
             jc[0] = jc[1] = jc[2] = 0.0;
-    double *Jc = S + (int64_t)ST_JCELL * sl;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        Jc[a * sl + o] = jc[a];
+        for (int a = 0; a < 3; ++a)
+            Jc[a * sl + o] = jc[a];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                nd[m][c] = nd[4 + m][c];
+        j1a = s[5];
+        j1b = s[7];
+        j2a = s[9];
+        j2b = s[11];
+    }
 }
 
 // ======================================================================
@@ -1198,15 +1352,8 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
             const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
             const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
                                   By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
-            const WT *W = dweights<WT>(b) + (int64_t)(G_WB - G_NMETRIC) * sl;
-            double bc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int n = 0; n < 6; ++n)
-            {
-                bc[0] += W[(int64_t)n * sl + o] * pt[n];
-                bc[1] += W[(int64_t)(6 + n) * sl + o] * pt[n];
-                bc[2] += W[(int64_t)(12 + n) * sl + o] * pt[n];
-            }
+            double bc[3];
+            bcell_from_faces<WT>(b, o, q, pt, bc);
             if (inner)
             {
                 double *Bc = S + (int64_t)ST_BCELL * sl;

@@ -126,8 +126,8 @@ bool field_slot(int f, int ucur, int &region, int &slot, int &loc, int &nc)
     case MHD_F_ALPHA_ZETA:
         slot = G_ALPHA + (f - MHD_F_ALPHA_XI), loc = LOC_EDGE_XI + (f - MHD_F_ALPHA_XI), nc = 1;
         return true;
-    case MHD_F_BCELL_W:
-        slot = G_WB, loc = LOC_CELL, nc = 18;
+    case MHD_F_BCELL_W: // region 3: the full table, held by host cases (device cases form it from Mi, mu, JD)
+        region = 3, slot = 0, loc = LOC_CELL, nc = 18;
         return true;
     case MHD_F_JCELL_W: // region 2: the full table, held by host cases only
         region = 2, slot = 0, loc = LOC_CELL, nc = 36;
@@ -268,7 +268,7 @@ extern "C" int mhd_host_download(mhd_host_case *h, int block, int field, double 
     BlockArrays &A = h->cs.blk[block];
     if (n != aos_count(A.d, loc, nc) || (region == 0 && slot + nc > st_nslot(A.d.model)))
         return MHD_ERR_ARG;
-    std::vector<double> &v = region == 0 ? A.st : (region == 1 ? A.geo : A.wj);
+    std::vector<double> &v = region == 0 ? A.st : (region == 1 ? A.geo : (region == 2 ? A.wj : A.wb));
     soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, aos, true);
     return MHD_OK;
 }
@@ -281,7 +281,7 @@ extern "C" int mhd_host_upload(mhd_host_case *h, int block, int field, const dou
     BlockArrays &A = h->cs.blk[block];
     if (n != aos_count(A.d, loc, nc) || (region == 0 && slot + nc > st_nslot(A.d.model)))
         return MHD_ERR_ARG;
-    std::vector<double> &v = region == 0 ? A.st : (region == 1 ? A.geo : A.wj);
+    std::vector<double> &v = region == 0 ? A.st : (region == 1 ? A.geo : (region == 2 ? A.wj : A.wb));
     soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, const_cast<double *>(aos), false);
     return MHD_OK;
 }
@@ -492,13 +492,14 @@ struct HostIO : FieldIO
     explicit HostIO(mhd_host_case *h_) : h(h_) {}
     int nblocks() override { return (int)h->cs.blk.size(); }
     const MhdBlockDev &block(int b) override { return h->cs.blk[b].d; }
-    // state slots, geometry slots (region 1) or the full J_cell table (region 2)
+    // state slots, geometry slots (region 1) or the full J_cell / B_cell tables (regions 2, 3)
     double *base(int b, int field, int &loc, int &nc)
     {
         int region, slot;
         field_slot(field, 0, region, slot, loc, nc);
         BlockArrays &A = h->cs.blk[b];
-        return (region == 0 ? A.st.data() : (region == 1 ? A.geo.data() : A.wj.data())) + (size_t)slot * A.d.slot;
+        return (region == 0 ? A.st.data() : (region == 1 ? A.geo.data() : (region == 2 ? A.wj.data() : A.wb.data()))) +
+               (size_t)slot * A.d.slot;
     }
     int get(int b, int field, std::vector<double> &aos) override
     {

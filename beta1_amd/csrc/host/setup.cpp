@@ -474,18 +474,23 @@ void sym_eig3(double A[3][3], double V[3][3], double d[3])
 
 // Pseudo-inverse least-squares weights: W_n = M^+ (mu_n s_n),
 // M = sum mu_n s_n s_n^T, with relative activity threshold tau_act and
-// eigenvalue truncation tau_eig (reference :489-622 and :794-912).  Mi_out
-// (9, row-major) and active_out, when given, receive M^+ and the mask of
-// active n (all zero where every weight is zero), from which
-// W[c n + q] = Mi[c][0] mu_q s_q[0] + Mi[c][1] mu_q s_q[1] + Mi[c][2] mu_q s_q[2].
+// eigenvalue truncation tau_eig (reference :489-622 and :794-912).  M^+ is
+// formed on and above the diagonal and mirrored (it is symmetric).  Mi_out
+// (6: 00, 01, 02, 11, 12, 22), mu_out (n: mu_q, 0 where q is inactive) and
+// active_out, when given, receive M^+, the weights and the mask of active q
+// (all zero where every weight is zero), from which
+// W[c n + q] = Mi[c][0] (mu_q s_q[0]) + Mi[c][1] (mu_q s_q[1]) + Mi[c][2] (mu_q s_q[2]).
 void pinv_weights(int n, const double (*s)[3], const double *mu_in, double tau_act, double *W /*3*n*/,
-                  double *Mi_out = nullptr, unsigned *active_out = nullptr)
+                  double *Mi_out = nullptr, unsigned *active_out = nullptr, double *mu_out = nullptr)
 {
     for (int q = 0; q < 3 * n; ++q)
         W[q] = 0.0;
     if (Mi_out)
-        for (int q = 0; q < 9; ++q)
+        for (int q = 0; q < 6; ++q)
             Mi_out[q] = 0.0;
+    if (mu_out)
+        for (int q = 0; q < n; ++q)
+            mu_out[q] = 0.0;
     if (active_out)
         *active_out = 0;
     double mag[36], ref = 0.0;
@@ -529,19 +534,29 @@ void pinv_weights(int n, const double (*s)[3], const double *mu_in, double tau_a
         il[q] = lam[q] > 1e-10 * lmax ? 1.0 / lam[q] : 0.0;
     double Mi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b)
+        for (int b = a; b < 3; ++b)
+        {
             for (int q = 0; q < 3; ++q)
                 Mi[a][b] += V[a][q] * il[q] * V[b][q];
+            Mi[b][a] = Mi[a][b];
+        }
     if (Mi_out)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                Mi_out[3 * a + b] = Mi[a][b];
+    {
+        Mi_out[0] = Mi[0][0];
+        Mi_out[1] = Mi[0][1];
+        Mi_out[2] = Mi[0][2];
+        Mi_out[3] = Mi[1][1];
+        Mi_out[4] = Mi[1][2];
+        Mi_out[5] = Mi[2][2];
+    }
     for (int q = 0; q < n; ++q)
     {
         if (!active[q])
             continue;
         if (active_out)
             *active_out |= 1u << q;
+        if (mu_out)
+            mu_out[q] = mu[q];
         const double rx = mu[q] * s[q][0], ry = mu[q] * s[q][1], rz = mu[q] * s[q][2];
         W[q] = Mi[0][0] * rx + Mi[0][1] * ry + Mi[0][2] * rz;
         W[n + q] = Mi[1][0] * rx + Mi[1][1] * ry + Mi[1][2] * rz;
@@ -590,6 +605,7 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
             d.bc[f] = B.face[f].kind == 0 ? B.face[f].bc : BC_NONE;
         A.geo.assign((size_t)G_NSLOT * d.slot, 0.0);
         A.wj.assign((size_t)36 * d.slot, 0.0);
+        A.wb.assign((size_t)18 * d.slot, 0.0);
         NodeView nv{B};
         auto G = [&](int s, int i, int j, int k) -> double & { return A.geo[(size_t)s * d.slot + blk_off(d, i, j, k)]; };
 #pragma omp parallel for schedule(dynamic)
@@ -659,10 +675,16 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
                             S[q][c] = sg * f[q][c];
                         mu[q] = 1.0 / std::max(vnorm(vsub(fc[q], xc)), 1e-14);
                     }
-                    double Wb[18];
-                    pinv_weights(6, S, mu, 1e-10, Wb);
+                    double Wb[18], Mb[6], mub[6];
+                    pinv_weights(6, S, mu, 1e-10, Wb, Mb, nullptr, mub);
+                    const int64_t ob = blk_off(d, i, j, k);
                     for (int q = 0; q < 18; ++q)
-                        G(G_WB + q, i, j, k) = Wb[q];
+                        A.wb[(size_t)q * d.slot + ob] = Wb[q];
+                    for (int q = 0; q < 6; ++q)
+                    {
+                        G(G_WBM + q, i, j, k) = Mb[q];
+                        G(G_WBU + q, i, j, k) = mub[q];
+                    }
                     // J_cell from twelve edges: xi (0..3), eta (4..7), zeta (8..11)
                     double Le[12][3], one[12];
                     const int e[12][6] = {{0, 0, 0, 1, 0, 0}, {0, 1, 0, 1, 1, 0}, {0, 0, 1, 1, 0, 1}, {0, 1, 1, 1, 1, 1},
@@ -676,13 +698,13 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
                         Le[q][2] = d0[2];
                         one[q] = 1.0;
                     }
-                    double Wj[36], Mi[9];
+                    double Wj[36], Mi[6];
                     unsigned act = 0;
                     pinv_weights(12, Le, one, 1e-10, Wj, Mi, &act);
                     const int64_t o = blk_off(d, i, j, k);
                     for (int q = 0; q < 36; ++q)
                         A.wj[(size_t)q * d.slot + o] = Wj[q];
-                    for (int q = 0; q < 9; ++q)
+                    for (int q = 0; q < 6; ++q)
                         G(G_WJM + q, i, j, k) = Mi[q];
                     G(G_WJA, i, j, k) = (double)act;
                 }

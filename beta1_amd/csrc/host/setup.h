@@ -75,6 +75,7 @@ struct BlockArrays
     std::vector<double> geo; // G_NSLOT * slot
     std::vector<double> st;  // st_nslot(d.model) * slot
     std::vector<double> wj;  // 36 * slot: Jcell_from_Jedge_w in full (host only)
+    std::vector<double> wb;  // 18 * slot: Bcell_from_Bface_w in full (host only)
 };
 
 // One element-wise operation of a boundary / halo phase.  Offsets are

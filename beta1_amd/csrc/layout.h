@@ -87,13 +87,16 @@ enum MhdStateSlot
 // State slots per block for a physics model (0 single fluid, 1 two-fluid).
 MHD_HD int st_nslot(int model) { return model == 1 ? (int)ST_NSLOT2 : (int)ST_NSLOT; }
 
-// Geometry slots (shared by every case with the same grid).  The J_cell
-// table Jcell_from_Jedge_w (36 weights per cell) is not stored: its weights
-// are W[c][q] = sum_b Mi[c][b] e_q[b] for the active edges q, with e_q the
-// edge vector (difference of two node coordinates) and Mi the inverse of the
-// least-squares normal matrix, so the kernel forms them from G_NODE, G_WJM
-// and G_WJA with the host builder's arithmetic (bit-identical weights from 13
-// stored values instead of 36).
+// Geometry slots (shared by every case with the same grid).  The two
+// least-squares weight tables of the reference are not stored: both are
+// W[c][q] = sum_b Mi[c][b] mu_q s_q[b] over the active terms q, with Mi the
+// (symmetric) truncated inverse of the normal matrix.  Bcell_from_Bface_w
+// (18 per cell): s_q = the oriented area vector of face q, which the B_cell
+// kernel loads anyway, mu_q its distance weight; Jcell_from_Jedge_w (36 per
+// cell): s_q = the edge vector (difference of two node coordinates), mu_q = 1.
+// The kernels form each weight with the host builder's arithmetic, so they
+// are bit-identical to the tables (which host cases still hold in full for
+// the C-ABI), from 12 and 7 stored values instead of 18 and 36.
 enum MhdGeoSlot
 {
     G_JAC = 0,     // cell volume
@@ -101,10 +104,11 @@ enum MhdGeoSlot
     G_NODE = 10,   // 3 slots: coordinates x, y, z of node (i, j, k)
     G_BETA = 13,   // 3 slots: face Hodge weight beta = |l*|/|S|
     G_ALPHA = 16,  // 3 slots: edge Hodge weight alpha = |e|/|S*|
-    G_WB = 19,     // 18 comps: Bcell_from_Bface_w
-    G_WJM = 37,    // 9 comps: Mi of the J_cell least squares, row-major
-    G_WJA = 46,    // 1 comp: active J edges of the cell, bit q for edge q (0..11)
-    G_NSLOT = 47,
+    G_WBM = 19,    // 6 comps: Mi of the B_cell least squares (00, 01, 02, 11, 12, 22)
+    G_WBU = 25,    // 6 comps: mu_q of the six faces (-xi, +xi, -eta, +eta, -zeta, +zeta), 0 if inactive
+    G_WJM = 31,    // 6 comps: Mi of the J_cell least squares (00, 01, 02, 11, 12, 22)
+    G_WJA = 37,    // 1 comp: active J edges of the cell, bit q for edge q (0..11)
+    G_NSLOT = 38,
     G_NMETRIC = 13 // slots [0, G_NMETRIC): metric and nodes, always double
 };
 
@@ -209,7 +213,7 @@ struct MhdTileMap
     int lo, hi; // range [lo, n + hi) on every axis
     int xcd_chunk;
     int i_inner; // 1: consecutive tiles step along i, 0: along the (j, k) plane
-    int ci;      // column tiles: planes per workgroup along i
+    int ci;      // column and plane-chunk tiles: planes per workgroup along i
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)

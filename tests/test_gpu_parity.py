@@ -154,3 +154,20 @@ def test_compute_dt_matches_oracle_and_leaves_step_untouched():
     assert ctx.compute_dt(cid) == orc.compute_dt()
     ctx.step(1)
     assert ctx.last_dt(cid) == orc.step(1)[0]
+
+
+def test_device_bcell_weights_equal_host_table():
+    """The device keeps Mi and mu of the B_cell least squares instead of the
+    18-weight table; the table it reports (formed from them and the face
+    area vectors, as the B_cell kernel forms each weight) equals the host
+    builder's table bit for bit."""
+    desc = WindDesc(nang=8, nrad=16)
+    ctx = MhdContext("fp64", recon=0, ghost=2, device=0)
+    cid = ctx.add(desc)
+    ctx.finalize()
+    hc = HostCase(desc, ghost=2)
+    for b in range(6):
+        dev = ctx.field(cid, "Bcell_from_Bface_w", b)
+        host = hc.field("Bcell_from_Bface_w", b)
+        assert np.abs(host).max() > 0.0
+        assert np.array_equal(dev, host), b
