@@ -84,7 +84,7 @@ __device__ __forceinline__ bool tile_block(const MhdTileMap &tm, MhdBlockDev &b,
 // Column tiles: a workgroup of 64 x COL_TJ threads owns the cells
 // (i0 .. i0 + ci - 1) x (j0 .. j0 + COL_TJ - 1) x (k0 .. k0 + 63) of a block
 // and marches along i; one thread per (j, k) column.
-constexpr int COL_TJ = 8, COL_TK = 64;
+constexpr int COL_TJ = 4, COL_TK = 64; // 4 x 64 cells: 187 us vs 196 at 8 x 64 (6 x 64^2 x 128 shell)
 __device__ __forceinline__ bool tile_col(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &j0, int &k0)
 {
     int lt;
@@ -1180,8 +1180,10 @@ __global__ __launch_bounds__(256) void k_jedge(const MhdTileMap tm)
 // kEs[n] to kEe[n] (xi edges 0-3, eta 4-7, zeta 8-11 as the J-edge
 // families).  The plane-(i + 1) nodes and the eta / zeta J edges at + di are
 // the next plane's own, so they are carried in registers.
+// (At 4 waves per SIMD -- 128 VGPRs, a few spilled -- it runs 102 us on the
+// 6 x 64^2 x 128 shell, 109 us with the compiler's own 165 VGPRs and 3 waves.)
 template <typename WT>
-__global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_jcell(const MhdTileMap tm)
 {
     constexpr int kEs[12] = {0, 1, 2, 3, 0, 4, 2, 6, 0, 4, 1, 5};
     constexpr int kEe[12] = {4, 5, 6, 7, 1, 5, 3, 7, 2, 6, 3, 7};
