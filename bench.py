@@ -234,7 +234,8 @@ def bench_ot(args):
                    step_hbm={"bytes_per_cell_update": STEP_BYTES,
                              "frac": round(res[0][0] * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4),
                              "note": "the box keeps stored metric and weight arrays like the shell (no closed-form "
-                                     "Cartesian specialisation), so it reads the shell's 712 B per cell-update"})
+                                     f"Cartesian specialisation), so it reads the shell's {STEP_BYTES} B per "
+                                     "cell-update"})
 
 
 def bench_precision(args):

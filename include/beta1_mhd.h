@@ -206,6 +206,10 @@ int mhd_write_tecplot(mhd_ctx *ctx, int case_id, const char *path, double time);
 int mhd_case_info(mhd_ctx *ctx, int case_id, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
 int mhd_field_count(mhd_ctx *ctx, int case_id, int field, size_t *n);
 int mhd_download(mhd_ctx *ctx, int case_id, int block, int field, double *aos, size_t n);
+/* A device case keeps the least-squares data of Bcell_from_Bface_w, not the
+   table: MHD_F_BCELL_W downloads the table formed from it (bit-identical to
+   the host builder's) and cannot be uploaded (MHD_ERR_ARG); MHD_F_JCELL_W is
+   held by host cases only. */
 int mhd_upload(mhd_ctx *ctx, int case_id, int block, int field, const double *aos, size_t n);
 /* Topology of a block: 6 faces (-i,+i,-j,+j,-k,+k) x 14 ints
    {kind (0 physical, 1 interface), bc, nb_block, nb_face, periodic,
