@@ -1565,6 +1565,32 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
     double *E = S + (int64_t)ST_E * sl;
     const int idx[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
     const double pe0 = amb ? pe_node(PE, o, st[0], st[1], st[2]) : 0.0;
+    // The edges of the three families at o meet the cells o + s0 di + s1 dj
+    // + s2 dk, s in {-1, 0}^3 with s_a = 0 for family a: 7 distinct cells
+    // (code (s0 + 1) 4 + (s1 + 1) 2 + (s2 + 1); code 0 is no family's).  Their
+    // densities and Hall terms are formed once and shared by the families,
+    // with the arithmetic of the per-family form.  The points are [0, N]^3 and
+    // every stencil stays in [-1, N + 1]: all loads are in bounds, and each
+    // family stores on its own range only.
+    double rh[8], rn[8], eh[8][3], bt[8];
+#pragma unroll
+    for (int cc = 1; cc < 8; ++cc)
+    {
+        const int64_t p = o + ((cc >> 2) - 1) * st[0] + (((cc >> 1) & 1) - 1) * st[1] + ((cc & 1) - 1) * st[2];
+        rh[cc] = RH[p];
+        rn[cc] = RN[p];
+        if (hall)
+        {
+            const NumInfo nm = num_limiter(c, rh[cc], rn[cc]);
+            const double alpha = c.hall_coef / nm.ne_eff;
+            const double Jx = Jc[p], Jy = Jc[sl + p], Jz = Jc[2 * sl + p];
+            const double Bx = Bc[p], By = Bc[sl + p], Bz = Bc[2 * sl + p];
+            eh[cc][0] = alpha * (Jy * Bz - Jz * By);
+            eh[cc][1] = alpha * (Jz * Bx - Jx * Bz);
+            eh[cc][2] = alpha * (Jx * By - Jy * Bx);
+            bt[cc] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
@@ -1572,17 +1598,21 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             in = in && (idx[q] < n[q] + (q == a ? 0 : 1));
-        if (!in)
-            continue;
         const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-        int64_t oc[4];
+        // the family's 4 cells in the order (s_t1, s_t2) = (-1,-1), (-1,0), (0,-1), (0,0)
+        int cq[4];
         {
             int q = 0;
 #pragma unroll
-            for (int s1 = -1; s1 <= 0; ++s1)
+            for (int s1 = 0; s1 <= 1; ++s1)
 #pragma unroll
-                for (int s2 = -1; s2 <= 0; ++s2)
-                    oc[q++] = o + s1 * st[t1] + s2 * st[t2];
+                for (int s2 = 0; s2 <= 1; ++s2)
+                {
+                    int sv[3] = {1, 1, 1};
+                    sv[t1] = s1;
+                    sv[t2] = s2;
+                    cq[q++] = sv[0] * 4 + sv[1] * 2 + sv[2];
+                }
         }
         double e = E[a * sl + o];
         if (amb)
@@ -1592,8 +1622,8 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                rhoH += RH[oc[q]];
-                rhoNa += RN[oc[q]];
+                rhoH += rh[cq[q]];
+                rhoNa += rn[cq[q]];
             }
             const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
             e -= c.pe_coef * (pe1 - pe0) / nm.ne_eff;
@@ -1614,25 +1644,11 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
         }
         if (hall)
         {
-            double eh[4][3], bt[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                const int64_t p = oc[q];
-                const NumInfo nm = num_limiter(c, RH[p], RN[p]);
-                const double alpha = c.hall_coef / nm.ne_eff;
-                const double Jx = Jc[p], Jy = Jc[sl + p], Jz = Jc[2 * sl + p];
-                const double Bx = Bc[p], By = Bc[sl + p], Bz = Bc[2 * sl + p];
-                eh[q][0] = alpha * (Jy * Bz - Jz * By);
-                eh[q][1] = alpha * (Jz * Bx - Jx * Bz);
-                eh[q][2] = alpha * (Jx * By - Jy * Bx);
-                bt[q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
-            }
             double Ec[3];
 #pragma unroll
             for (int m = 0; m < 3; ++m)
-                Ec[m] = 0.25 * (eh[0][m] + eh[1][m] + eh[2][m] + eh[3][m]);
-            const double bmax = dmax(dmax(bt[0], bt[1]), dmax(bt[2], bt[3]));
+                Ec[m] = 0.25 * (eh[cq[0]][m] + eh[cq[1]][m] + eh[cq[2]][m] + eh[cq[3]][m]);
+            const double bmax = dmax(dmax(bt[cq[0]], bt[cq[1]]), dmax(bt[cq[2]], bt[cq[3]]));
             // edge vector dr = node(o + st[a]) - node(o) (src/3_field/4_MPCNS_Geometry.cpp:1251-1320)
             const double *X = dmetric(b) + (int64_t)G_NODE * sl;
             const int64_t o2 = o + st[a];
@@ -1662,7 +1678,8 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
                                     : Ecen + 0.5 * mu[t1] * dB21 - 0.5 * mu[t2] * dB12;
             e += S[(int64_t)(ST_TAPER + a) * sl + o] * h;
         }
-        E[a * sl + o] = e;
+        if (in)
+            E[a * sl + o] = e;
     }
 }
 
