@@ -1382,9 +1382,11 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
 // both species (MercurySolver_Advance.cpp:73-112).  The pressure-gradient
 // metric averages faces i-1 and i (j-1 and j, k-1 and k) as the reference
 // does.
+// (At 3 waves per SIMD -- 168 VGPRs, a few spilled -- it runs 0.52 ms on the
+// 6 x 64^2 x 128 Mercury shell, 0.55 ms with the compiler's 182 VGPRs at 2.)
 template <typename R, int RECON>
-__global__ __launch_bounds__(256) void k_fluid2(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
-                                                const double *__restrict__ dts, int ucur)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_fluid2(
+    const MhdTileMap tm, const MhdCaseConst *__restrict__ cst, const double *__restrict__ dts, int ucur)
 {
     MhdBlockDev b;
     int i, j, k;
