@@ -5,7 +5,8 @@
  * reference's own AoS layout [i][j][k][m] (include/3_field/Field_Array.h:177)
  * with the extents of FieldBlock::compute_extent
  * (src/3_field/1_Field_Block.cpp:51-131).  It is used by the tests, by the
- * smoke check and by the bench's CPU-baseline leg as the checker; nothing in
+ * smoke check and by the bench's CPU-baseline leg, as the reference result to
+ * compare against (test infrastructure); nothing in
  * the product links or calls it.
  */
 #ifndef MHD_ORACLE_H

@@ -1,6 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/libmhd_oracle.so) for the tests.
 
-The oracle is the checker: it is loaded here, in ``__graft_entry__.smoke`` and
+The oracle is test infrastructure, the result the HIP path is compared with: it
+is loaded here, in ``__graft_entry__.smoke`` and
 in the bench's CPU-baseline leg only.
 """
 import ctypes as C
