@@ -82,6 +82,7 @@ SIGNATURES = {
     "mhd_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "mhd_autotune": (_I, [_P]),
     "mhd_diagnostics": (_I, [_P, _I, _PD]),
+    "mhd_diagnostics_noalloc": (_I, [_P, _PI, _I, _PD]),
     "mhd_host_case_ot": (_I, [C.POINTER(MhdOTDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_lunar": (_I, [C.POINTER(MhdWindDesc), _I, C.POINTER(_P)]),
     "mhd_host_case_mercury": (_I, [C.POINTER(MhdWindDesc), C.POINTER(MhdDipoleDesc), _I, C.POINTER(_P)]),

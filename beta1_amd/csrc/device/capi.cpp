@@ -165,6 +165,9 @@ struct mhd_ctx
     double *d_dtres = nullptr; // per-case dt of the current resistive subcycle
     unsigned long long *d_diag = nullptr; // 17 order keys per case (min/max diagnostics)
     size_t cap_diag = 0;
+    // this device's static scratch of mhd_diagnostics_noalloc (17 keys for
+    // each of up to mhdk::DIAG_SEL_MAX cases; not owned, never freed)
+    unsigned long long *d_scratch = nullptr;
     int max_nsub = 0;          // resistive subcycles of the resident cases (0: none)
     int *d_err = nullptr, *d_active = nullptr;
 
@@ -236,6 +239,8 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
     for (int e = 0; e < 2 && ok; ++e)
         ok = hipEventCreateWithFlags(&c->fork[e], hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&c->join[e], hipEventDisableTiming) == hipSuccess;
+    // the device address of the static diagnostics scratch (device current)
+    ok = ok && mhdk::diag_scratch(opt->device, &c->d_scratch) == hipSuccess && c->d_scratch;
     if (!ok)
     {
         mhd_destroy(c);
@@ -1332,6 +1337,43 @@ extern "C" int mhd_diagnostics(mhd_ctx *c, int case_id, double *out)
     HIPCHK(c, hipMemcpyAsync(keys, c->d_diag + 17 * (size_t)case_id, sizeof(keys), hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));
     for (int q = 0; q < 17; ++q)
+    {
+        const unsigned long long b = (keys[q] >> 63) ? (keys[q] & 0x7fffffffffffffffull) : ~keys[q];
+        std::memcpy(&out[q], &b, 8);
+    }
+    return MHD_OK;
+}
+
+// The reduction of mhd_diagnostics for a list of cases into the device's
+// static scratch: no device (or host heap) allocation, one fill kernel, one
+// reduction kernel, one copy of 17 * n keys to the caller's stack.
+extern "C" int mhd_diagnostics_noalloc(mhd_ctx *c, const int *case_ids, int n, double *out)
+{
+    if (!c || !case_ids || !out)
+        return MHD_ERR_ARG;
+    if (n < 1 || n > mhdk::DIAG_SEL_MAX)
+        return fail(c, MHD_ERR_ARG,
+                    "mhd_diagnostics_noalloc: " + std::to_string(n) + " cases requested; its static scratch holds 1 to " +
+                        std::to_string(mhdk::DIAG_SEL_MAX));
+    mhdk::DiagSel sel;
+    sel.n = n;
+    for (int q = 0; q < n; ++q)
+    {
+        if (!active_case(c, case_ids[q]))
+            return fail(c, MHD_ERR_ARG, "mhd_diagnostics_noalloc: case " + std::to_string(case_ids[q]) + " is not active");
+        for (int p = 0; p < q; ++p)
+            if (case_ids[p] == case_ids[q])
+                return fail(c, MHD_ERR_ARG, "mhd_diagnostics_noalloc: case " + std::to_string(case_ids[q]) +
+                                                " requested twice");
+        sel.ids[q] = case_ids[q];
+    }
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    mhdk::launch_diag_sel(launch_of(c, KP_DT), c->d_cst, c->d_scratch, c->ucur, sel, c->q);
+    unsigned long long keys[17 * mhdk::DIAG_SEL_MAX];
+    HIPCHK(c, hipMemcpyAsync(keys, c->d_scratch, sizeof(unsigned long long) * 17 * (size_t)n, hipMemcpyDeviceToHost,
+                             c->q));
+    HIPCHK(c, hipStreamSynchronize(c->q));
+    for (int q = 0; q < 17 * n; ++q)
     {
         const unsigned long long b = (keys[q] >> 63) ? (keys[q] & 0x7fffffffffffffffull) : ~keys[q];
         std::memcpy(&out[q], &b, 8);

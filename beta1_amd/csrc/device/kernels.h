@@ -49,6 +49,20 @@ void launch_ops2(const mpcns::MhdOp *opa, int na, int ushift_a, const mpcns::Mhd
                  const MhdCaseConst *cst, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);
-// 17 order keys per case (min/max diagnostics), set to the neutral keys first
+// 17 order keys per case (min/max diagnostics) at out + 17 * case id; the
+// caller sets them to the neutral keys first
 void launch_diag(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, hipStream_t q);
+// Diagnostics of the cases sel.ids[0 .. sel.n) into out + 17 * q (neutral
+// keys written first, by a kernel): the reduction of launch_diag.
+constexpr int DIAG_SEL_MAX = 64;
+struct DiagSel
+{
+    int n; // < 0: every case, at its case id
+    int ids[DIAG_SEL_MAX];
+};
+void launch_diag_sel(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, const DiagSel &sel,
+                     hipStream_t q);
+// Device address of the static 17 * DIAG_SEL_MAX-key scratch array on
+// `device`, which must be current; looked up once per device.
+hipError_t diag_scratch(int device, unsigned long long **ptr);
 } // namespace mhdk

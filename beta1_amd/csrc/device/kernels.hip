@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "../host/setup.h"
 #include "../layout.h"
@@ -1891,13 +1893,26 @@ __device__ __forceinline__ unsigned long long order_key(double x)
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
+// Keys go to out + 17 * slot: slot = case id when sel.n < 0, else the
+// position of the block's case in sel.ids (workgroups of other cases return
+// at once; a tile lies in one block, so the return is uniform).
 __global__ __launch_bounds__(256) void k_diag(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
-                                              unsigned long long *__restrict__ out, int ucur)
+                                              unsigned long long *__restrict__ out, int ucur, const mhdk::DiagSel sel)
 {
     MhdBlockDev b;
     int i, j, k;
     bool wg;
     const bool ok = tile_cell(tm, b, i, j, k, wg);
+    int slot = wg ? b.case_id : -1;
+    if (sel.n >= 0 && wg)
+    {
+        slot = -1;
+        for (int q = 0; q < sel.n; ++q)
+            if (sel.ids[q] == b.case_id)
+                slot = q;
+    }
+    if (slot < 0)
+        return;
     double v[17];
 #pragma unroll
     for (int q = 0; q < 8; ++q)
@@ -1956,9 +1971,9 @@ __global__ __launch_bounds__(256) void k_diag(const MhdTileMap tm, const MhdCase
         for (int q = 0; q < 17; ++q)
         {
             if (q < 8)
-                atomicMin(&out[17 * b.case_id + q], order_key(v[q]));
+                atomicMin(&out[17 * slot + q], order_key(v[q]));
             else
-                atomicMax(&out[17 * b.case_id + q], order_key(v[q]));
+                atomicMax(&out[17 * slot + q], order_key(v[q]));
         }
     }
 }
@@ -1969,6 +1984,22 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
     if (t < n)
         p[t] = v;
 }
+
+// Neutral keys of n cases' diagnostics: +inf for the 8 minima, -inf for the
+// 9 maxima of each case.
+__global__ void k_diag_reset(unsigned long long *p, int n17)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n17)
+        p[t] = order_key((t % 17) < 8 ? __longlong_as_double(0x7ff0000000000000ll)
+                                      : __longlong_as_double((long long)0xfff0000000000000ull));
+}
+
+// Scratch of mhd_diagnostics_noalloc: the order keys of up to
+// DIAG_SEL_MAX cases, one copy per GPU for the life of the process.  Host
+// code reaches it only through the address hipGetSymbolAddress gives for
+// the current device (diag_scratch below).
+static __device__ __align__(256) unsigned long long scratch[17 * mhdk::DIAG_SEL_MAX];
 
 // ======================================================================
 // launch helpers
@@ -2114,7 +2145,39 @@ void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_
 }
 void launch_diag(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, hipStream_t q)
 {
-    hipLaunchKernelGGL(k_diag, grid_of(L), block_of(L), 0, q, L.tm, cst, out, ucur);
+    DiagSel all;
+    all.n = -1;
+    hipLaunchKernelGGL(k_diag, grid_of(L), block_of(L), 0, q, L.tm, cst, out, ucur, all);
+}
+void launch_diag_sel(const Launch &L, const MhdCaseConst *cst, unsigned long long *out, int ucur, const DiagSel &sel,
+                     hipStream_t q)
+{
+    hipLaunchKernelGGL(k_diag_reset, dim3((17 * sel.n + 255) / 256), dim3(256), 0, q, out, 17 * sel.n);
+    hipLaunchKernelGGL(k_diag, grid_of(L), block_of(L), 0, q, L.tm, cst, out, ucur, sel);
+}
+hipError_t diag_scratch(int device, unsigned long long **ptr)
+{
+    // looked up once per device, with that device current
+    static std::mutex mu;
+    static std::map<int, unsigned long long *> by_device;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = by_device.find(device);
+    if (it == by_device.end())
+    {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess)
+            return e;
+        if (cur != device)
+            return hipErrorInvalidDevice;
+        unsigned long long *p = nullptr;
+        e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        it = by_device.emplace(device, p).first;
+    }
+    *ptr = it->second;
+    return hipSuccess;
 }
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q)
 {

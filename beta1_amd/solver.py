@@ -352,6 +352,20 @@ class MhdContext:
         d["divB_max"] = out[16]
         return d
 
+    def diagnostics_noalloc(self, cids):
+        """Raw 17 values per case (mhd_diagnostics order) of up to 64 cases,
+        through the entry point that allocates no device memory."""
+        ids = (C.c_int * len(cids))(*cids)
+        out = np.zeros(17 * max(len(cids), 1))
+        _check(lib().mhd_diagnostics_noalloc(self.ctx, ids, len(cids), _dptr(out)), self.ctx,
+               "diagnostics_noalloc")
+        return out.reshape(-1, 17)
+
+    def diagnostics_raw(self, cid):
+        out = np.zeros(17)
+        _check(lib().mhd_diagnostics(self.ctx, cid, _dptr(out)), self.ctx, "diagnostics")
+        return out
+
     def divb_max(self, cid):
         v = C.c_double()
         _check(lib().mhd_divb_max(self.ctx, cid, C.byref(v)), self.ctx, "divb")

@@ -167,6 +167,15 @@ int mhd_case_steps(mhd_ctx *ctx, int case_id, long long *steps);
    |B_cell|^2/2, out[16] max |div B| (Na entries +inf / -inf for a single
    fluid). */
 int mhd_diagnostics(mhd_ctx *ctx, int case_id, double *out17);
+/* No counterpart in the reference.  mhd_diagnostics of several cases in one
+   pass, for callers that cannot allocate device memory (a serving loop that
+   polls its resident cases between steps, a harness that forbids hipMalloc):
+   out + 17*q receives case_ids[q]'s 17 values, 1 <= n <= 64 distinct active
+   cases (MHD_ERR_ARG otherwise).  It allocates nothing: the order keys live in
+   one static device array of 17 * 64 keys (8.7 KB) per GPU, held until the
+   process ends, and each call resets what it uses.  Calls that overlap on one
+   GPU (other threads, other contexts on the same device) race on that array. */
+int mhd_diagnostics_noalloc(mhd_ctx *ctx, const int *case_ids, int n, double *out);
 
 /* Advance every case by nsteps explicit steps, each case with its own dt
    (LunarSolver::StepOnce, Z4_Lunar/src/LunarSolver_Driver.cpp:21-92).
