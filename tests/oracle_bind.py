@@ -76,7 +76,7 @@ STATE_TWO_FLUID = ["U_Na", "Photo_rate", "drag_rate"]
 class Oracle:
     """CPU FP64 restatement of the reference step, fed from a HostCase."""
 
-    def __init__(self, hc, recon=0, periodic_wrap=False, threads=1):
+    def __init__(self, hc, recon=0, periodic_wrap=False, threads=1, overrides=None):
         k = hc.constants()
         d = OrDesc()
         d.nblocks, d.g, d.recon = hc.nblocks, hc.g, recon
@@ -101,6 +101,14 @@ class Oracle:
             d.loc_center[a] = t[23 + a]
         d.loc_r_decay, d.loc_r_cutoff = t[26], t[27]
         d.resist, d.resist_nsub, d.inver_Rem = int(t[28]), int(t[29]), t[30]
+        # tests may set descriptor fields directly (e.g. the two-fluid model
+        # on a box grid); the two-fluid arrays then start at zero
+        for key, val in (overrides or {}).items():
+            if isinstance(val, (tuple, list)):
+                for q, x in enumerate(val):
+                    getattr(d, key)[q] = x
+            else:
+                setattr(d, key, val)
         L = olib()
         self.c = L.or_create(C.byref(d))
         self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks

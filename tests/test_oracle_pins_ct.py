@@ -387,3 +387,135 @@ def test_minmod_advection_is_tvd_without_new_extrema():
         assert cur.min() >= lo - 1e-12 and cur.max() <= hi + 1e-12
         prev = cur
     assert tv(prev) < tv(U1[:, 0])  # the profile did evolve (numerical diffusion)
+
+
+# ------------------------------------------- two-fluid terms on a box grid
+def aniso_box_case(tmp_path, n, L):
+    """The periodic n^3 box with lengths L (uniform spacing per axis)."""
+    d = str(tmp_path / "ABOX")
+    HostCase(OTDesc(n=n), ghost=G).write_dir(d)
+    path = os.path.join(d, "geometry", "grid0000.grd")
+    raw = bytearray(open(path, "rb").read())
+    off = 4 + 4 + struct.unpack_from("<i", raw, 4)[0]
+    dims = struct.unpack_from("<3i", raw, off)
+    off += 12
+    cnt = dims[0] * dims[1] * dims[2]
+    for a in range(3):
+        t = np.frombuffer(bytes(raw[off:off + 8 * cnt]), dtype="<f8")
+        raw[off:off + 8 * cnt] = (L[a] * t).astype("<f8").tobytes()
+        off += 8 * cnt
+    open(path, "wb").write(bytes(raw))
+    return HostCase(CaseDirDesc(d, WindDesc()), ghost=G)
+
+
+# H+ / Na+ on the box: masses relative to the reference mass, no sources
+# other than the term under test (no photoionization, drag, induction force)
+TWO_FLUID = dict(model=1, M_H=1.0, M_ref=1.0, M_Na=23.0, state_coeff=1.0, state_coeff_Na=1.0, T_ref=1.0,
+                 ne_floor=0.01, induce=0.0, a1_Na=0.0, a4=0.0, a5=0.0, a6=0.0, Tn0=0.0, local_arti=0,
+                 resist=0, ambipolar=0, hall=0, hall_coef=0.0, hall_rmin=0.0, hall_rmax=0.0)
+
+
+def set_two_fluid(orc, rhoH, rhoNa, pH, pNa):
+    """Species at rest with the given densities and pressures (arrays over
+    the cell box incl. ghosts, or scalars)."""
+    UH, UN = orc.view("U_H", 0), orc.view("U_Na", 0)
+    UH[...] = 0.0
+    UN[...] = 0.0
+    UH[..., 0], UH[..., 4] = rhoH, np.asarray(pH) / (GAMMA - 1)
+    UN[..., 0], UN[..., 4] = rhoNa, np.asarray(pNa) / (GAMMA - 1)
+    for name in FACES + ADD:
+        orc.view(name, 0)[...] = 0.0
+
+
+L_BOX = (1.0, 0.6, 1.7)
+
+
+@pytest.mark.parametrize("flow,field", PAIRS)
+def test_hall_whistler_term_is_llf_dissipation_at_the_whistler_speed(tmp_path, flow, field):
+    """The explicit Hall EMF with J = 0 is its whistler diffusion alone
+    (MercurySolver_MagExplicit_Hall.cpp:39-440): on the edge between two
+    faces of the field's family the LLF dissipation 1/2 v (jump of B across
+    the edge), v = Cwh alpha |B| / h the whistler speed at the grid scale h
+    across the jump (the mean width of the two cells the faces bound), with
+    alpha = hall_coef / n_e,eff and Cwh = 0.1.  In face-flux units the edge
+    EMF is 1/2 mu_i (phi_i - phi_{i-1}), mu_i = Cwh alpha |B| / hbar_i^2, and
+    one step changes phi_i by dt (E_{i+1} - E_i).  On the stretched box hbar
+    differs from edge to edge; the other face families do not change."""
+    n = 16
+    hc = stretched_box_case(tmp_path, n)
+    over = dict(TWO_FLUID, hall=1, hall_coef=0.8)
+    orc = Oracle(hc, overrides=over)
+    rhoH, rhoNa = 0.02, 0.23  # n_H = 0.02, n_Na = 0.01: n_e = 0.03
+    set_two_fluid(orc, rhoH, rhoNa, 0.05, 0.02)
+    t = np.arange(n) / n
+    phi = (0.2 + 0.15 * np.sin(2 * np.pi * t) + 0.05 * np.cos(6 * np.pi * t)) * 1e-3
+    B = orc.view(FACES[field], 0)
+    shape = [1, 1, 1]
+    shape[flow] = n
+    sl = [slice(G, G + n + (1 if a == field else 0)) for a in range(3)]
+    B[tuple(sl)] = phi.reshape(shape + [1])
+    orc.sync("Bface")
+    refresh(orc)
+    B0 = np.array([0.4, -0.7, 0.5])
+    orc.view("B_cell", 0)[...] = B0     # the step's Hall EMF reads these
+    orc.view("J_cell", 0)[...] = 0.0
+    dt = orc.step(1)[0]
+    ne = rhoH / 1.0 + rhoNa / 23.0
+    alpha = over["hall_coef"] / np.sqrt(ne * ne + 0.01 ** 2)
+    X = axis_nodes(hc)[flow]
+    gn = G + 1
+    width = np.diff(X)[gn - 1:gn + n + 1]          # cells -1 .. n
+    hbar = 0.5 * (width[:-1] + width[1:])          # at nodes 0 .. n
+    mu = 0.1 * alpha * np.linalg.norm(B0) / hbar ** 2
+    ext = np.concatenate([phi[-1:], phi, phi[:1]])  # cells -1 .. n (periodic)
+    E = 0.5 * mu * (ext[1:] - ext[:-1])             # nodes 0 .. n
+    expect = phi + dt * (E[1:] - E[:-1])
+    got = orc.view(FACES[field], 0)[..., 0]
+    idx = [G + 3] * 3
+    line = []
+    for i in range(n):
+        idx[flow] = G + i
+        line.append(got[tuple(idx)])
+    line = np.array(line)
+    # cells whose two edges see interior cells on both sides (the box's ghost
+    # cells have mirrored, not periodic, widths)
+    inner = slice(1, n - 1)
+    assert np.abs(line - phi).max() > 1e-4 * np.abs(phi).max()
+    assert np.abs(line[inner] - expect[inner]).max() <= 1e-13 * np.abs(phi).max(), (flow, field)
+    assert np.ptp(hbar[1:n]) > 0.2 * hbar.mean()
+    for a in range(3):
+        if a != field:
+            assert np.abs(orc.view(FACES[a], 0)).max() <= 1e-22
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_ambipolar_force_is_the_exact_gradient_of_a_linear_pressure(tmp_path, axis):
+    """The electron-pressure force on species s is -w_s grad p_e with
+    p_e = p_H + p_Na and w_s = n_s / n_e,eff, the species' share of the
+    electrons with the Hall_Num_Limiter floor (MercurySolver_FluidRHS.cpp:
+    206-527, MercurySolver_Tools.cpp:90-136).  On a box with uniform metric the discrete gradient is the
+    central difference, exact for a linear p_e: with H+ uniform (its own
+    fluxes cancel) and p_Na linear along one axis, one step changes the H+
+    momentum along that axis by -dt w_H dp_Na/dx and nothing else."""
+    n = 12
+    hc = aniso_box_case(tmp_path, n, L_BOX)
+    orc = Oracle(hc, overrides=dict(TWO_FLUID, ambipolar=1, pe_coef=0.0))
+    rhoH, rhoNa, slope = 0.5, 4.6, 0.03
+    # p_Na linear along `axis` in cell-centre coordinates, incl. ghosts
+    cnt = orc.view("U_Na", 0).shape[:3]
+    xc = (np.arange(cnt[axis]) - G + 0.5) * L_BOX[axis] / n
+    shp = [1, 1, 1]
+    shp[axis] = cnt[axis]
+    pNa = np.broadcast_to((0.4 + slope * xc).reshape(shp), cnt)
+    set_two_fluid(orc, rhoH, rhoNa, 0.3, pNa)
+    refresh(orc)
+    U0 = orc.view("U_H", 0).copy()
+    dt = orc.step(1)[0]
+    nH, nNa = rhoH / 1.0, rhoNa / 23.0
+    chi = nH / np.sqrt((nH + nNa) ** 2 + 0.01 ** 2)
+    inner = (slice(G + 1, G + n - 1),) * 3
+    dU = orc.view("U_H", 0)[inner] - U0[inner]
+    expect = -dt * chi * slope
+    for m in range(5):
+        want = expect if m == 1 + axis else 0.0
+        assert np.abs(dU[..., m] - want).max() <= 1e-12 * abs(expect), (axis, m)
