@@ -6,7 +6,11 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbeta1_mh
 
 
 class MhdError(RuntimeError):
-    pass
+    """A C-ABI call returned a negative status (``code``; MHD_ERR_DT = -3)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class MhdOptions(C.Structure):
@@ -78,6 +82,7 @@ SIGNATURES = {
     "mhd_memory": (_I, [_P, _PD]),
     "mhd_retire": (_I, [_P, _I]),
     "mhd_case_steps": (_I, [_P, _I, C.POINTER(C.c_longlong)]),
+    "mhd_case_error": (_I, [_P, _I, _PI]),
     "mhd_tuning": (_I, [_P, _I, _PI, _PI, _PI]),
     "mhd_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "mhd_autotune": (_I, [_P]),

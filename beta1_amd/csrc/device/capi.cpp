@@ -223,7 +223,10 @@ extern "C" int mhd_create(const mhd_options *opt, mhd_ctx **out)
 {
     if (!opt || !out)
         return MHD_ERR_ARG;
-    if (opt->ghost < 1 || (opt->recon == 1 && opt->ghost < 2) || opt->precision < 0 || opt->precision > 1)
+    // two ghost layers at least: the stencil kernels issue their loads ahead
+    // of the per-family range checks, and the slot padding of two layers is
+    // what keeps every such load inside the block's box
+    if (opt->ghost < 2 || opt->precision < 0 || opt->precision > 1 || opt->recon < 0 || opt->recon > 1)
         return MHD_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt->device)
@@ -322,6 +325,17 @@ template <class T> static int grow(mhd_ctx *c, T *&p, size_t &cap, size_t need)
     return MHD_OK;
 }
 
+// Forget the captured step (its kernel arguments name the block table, tile
+// maps, op groups and per-case arrays as they are now).
+static void drop_graph(mhd_ctx *c)
+{
+    if (c->gexec)
+    {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+}
+
 template <class T> static int regrow(mhd_ctx *c, T *&p, int old, int cap, bool keep)
 {
     T *np = nullptr;
@@ -342,6 +356,9 @@ static int ensure_case_capacity(mhd_ctx *c)
     const int old = c->case_cap;
     const int cap = std::max(8, std::max(need, 2 * old));
     HIPCHK(c, hipStreamSynchronize(c->q));
+    // the captured step holds the per-case arrays below as kernel arguments:
+    // it must not be replayed once they move
+    drop_graph(c);
     int rc;
     if ((rc = regrow(c, c->d_cst, old, cap, true)) || (rc = regrow(c, c->d_dt, old, cap, true)) ||
         (rc = regrow(c, c->d_zero, old, cap, false)) || (rc = regrow(c, c->d_dtbits, old, cap, true)) ||
@@ -558,11 +575,7 @@ static int rebuild(mhd_ctx *c)
     if (rc)
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->q));
-    if (c->gexec)
-    {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    drop_graph(c);
     c->h_blk.clear();
     c->max_nsub = 0;
     std::vector<MhdCaseConst> cst(c->case_cap);
@@ -641,10 +654,49 @@ static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt)
     return MHD_OK;
 }
 
+// Undo a partial placement: the case's op segments, its state and its
+// reference to the geometry set (freed with the last reference).
+static void unplace(mhd_ctx *c, CaseRec &r)
+{
+    for (int sy = 0; sy < SY_COUNT; ++sy)
+        if (r.op_n[sy])
+        {
+            (void)og_cut(c, sy, r.op_off[sy], r.op_n[sy]);
+            r.op_n[sy] = 0;
+        }
+    if (r.d_st)
+    {
+        free_t(r.d_st);
+        c->bytes_used -= r.st_bytes;
+        r.st_bytes = 0;
+    }
+    if (r.geo_set >= 0)
+    {
+        GeoSet &G = c->geos[r.geo_set];
+        if (--G.refs == 0)
+        {
+            free_t(G.d);
+            c->bytes_used -= G.bytes;
+        }
+        r.geo_set = -1;
+    }
+}
+
+static int place_case_(mhd_ctx *c, CaseRec &r);
 static int place_case(mhd_ctx *c, CaseRec &r)
+{
+    const int rc = place_case_(c, r);
+    if (rc && rc != MHD_ERR_MEM)
+        unplace(c, r); // a HIP failure part way: nothing of the case stays resident
+    return rc;
+}
+
+static int place_case_(mhd_ctx *c, CaseRec &r)
 {
     const int64_t slot = r.cs.blk[0].d.slot;
     r.slot = slot;
+    if (r.cs.shape_key.empty())
+        r.cs.shape_key = geometry_key(r.cs.blk);
     int gi = -1;
     for (int g = 0; g < (int)c->geos.size(); ++g)
         if (c->geos[g].refs > 0 && c->geos[g].key == r.cs.shape_key)
@@ -655,19 +707,34 @@ static int place_case(mhd_ctx *c, CaseRec &r)
     const double budget = c->opt.hbm_budget_gb * 1e9;
     if (budget > 0 && (double)(c->bytes_used + st_bytes + geo_bytes) > budget)
         return fail(c, MHD_ERR_MEM, "HBM budget exceeded");
-    if (gi < 0)
+    if (gi >= 0)
+    {
+        c->geos[gi].refs += 1;
+        r.geo_set = gi;
+    }
+    else
     {
         GeoSet G;
         G.key = r.cs.shape_key;
         G.block_bytes = geo_block_bytes(slot, weight_esize(c));
         G.bytes = geo_bytes;
         HIPCHK(c, hipMalloc((void **)&G.d, geo_bytes));
+        c->bytes_used += geo_bytes;
+        G.refs = 1;
+        gi = 0;
+        while (gi < (int)c->geos.size() && c->geos[gi].refs > 0)
+            ++gi;
+        if (gi == (int)c->geos.size())
+            c->geos.push_back(G);
+        else
+            c->geos[gi] = G;
+        r.geo_set = gi; // registered before the copies: a failed copy is rolled back with it
         std::vector<float> f32;
         const size_t metric = (size_t)G_NMETRIC * slot;
         for (int b = 0; b < r.nblocks; ++b)
         {
             const std::vector<double> &geo = r.cs.blk[b].geo;
-            char *dst = G.d + (int64_t)b * G.block_bytes;
+            char *dst = c->geos[gi].d + (int64_t)b * c->geos[gi].block_bytes;
             if (mixed_of(c))
             {
                 HIPCHK(c, hipMemcpy(dst, geo.data(), sizeof(double) * metric, hipMemcpyHostToDevice));
@@ -678,18 +745,10 @@ static int place_case(mhd_ctx *c, CaseRec &r)
             else
                 HIPCHK(c, hipMemcpy(dst, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice));
         }
-        c->bytes_used += geo_bytes;
-        gi = 0;
-        while (gi < (int)c->geos.size() && c->geos[gi].refs > 0)
-            ++gi;
-        if (gi == (int)c->geos.size())
-            c->geos.push_back(G);
-        else
-            c->geos[gi] = G;
     }
-    c->geos[gi].refs += 1;
-    r.geo_set = gi;
     HIPCHK(c, hipMalloc((void **)&r.d_st, st_bytes));
+    r.st_bytes = st_bytes;
+    c->bytes_used += st_bytes;
     for (int b = 0; b < r.nblocks; ++b)
     {
         double *dst = r.d_st + (int64_t)b * nslot * slot;
@@ -707,8 +766,6 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         std::vector<double>().swap(r.cs.blk[b].wb);
         std::vector<double>().swap(r.cs.blk[b].st);
     }
-    r.st_bytes = st_bytes;
-    c->bytes_used += st_bytes;
     // op lists: case-relative offsets -> absolute addresses, appended to the
     // device op groups
     for (int sy = 0; sy < SY_COUNT; ++sy)
@@ -727,6 +784,8 @@ static int place_case(mhd_ctx *c, CaseRec &r)
         if (rc)
             return rc;
         r.op_n[sy] = ops.size();
+        // the host copy is consumed: a failure after this point cannot be retried
+        r.cs.ops[sy].clear();
     }
     r.cs.ops.clear();
     r.state = 1;
@@ -1039,13 +1098,16 @@ static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
     HIPCHK(c, hipSetDevice(c->opt.device));
     int rc = ensure_case_capacity(c);
     if (rc)
+    {
+        c->cases.pop_back();
         return rc;
+    }
     if ((rc = place_case(c, *c->cases[id])))
     {
         if (rc == MHD_ERR_MEM) // nothing was placed: the id is free again
             c->cases.pop_back();
         else
-            c->cases[id]->state = 2;
+            c->cases[id]->state = 2; // rolled back by place_case
         return rc;
     }
     if ((rc = rebuild(c)))
@@ -1117,12 +1179,39 @@ static size_t case_bytes(const mhd_ctx *c, const CaseSetup &cs)
     return need;
 }
 
+// A host case whose full weight tables were replaced: the device forms the
+// weights from the least-squares data, so the tables must be the ones that
+// data gives (else the edit would be dropped silently).
+static int check_edited_tables(mhd_ctx *c, const CaseSetup &cs)
+{
+    if (!cs.tables_edited)
+        return MHD_OK;
+    for (size_t b = 0; b < cs.blk.size(); ++b)
+    {
+        const BlockArrays &A = cs.blk[b];
+        std::vector<double> wb(aos_count(A.d, LOC_CELL, 18)), wj(aos_count(A.d, LOC_CELL, 36));
+        soa_aos(A.d, LOC_CELL, 18, const_cast<double *>(A.wb.data()), wb.data(), true);
+        soa_aos(A.d, LOC_CELL, 36, const_cast<double *>(A.wj.data()), wj.data(), true);
+        std::string why;
+        if (!weight_tables_match(A, wb.data(), wj.data(), 1e-12, why))
+            return fail(c, MHD_ERR_ARG,
+                        "block " + std::to_string(b) + ": " + why +
+                            "; the device forms the weight tables from the grid's least-squares data, so edited "
+                            "tables cannot be adopted");
+    }
+    return MHD_OK;
+}
+
 extern "C" int mhd_adopt_host_case(mhd_ctx *c, mhd_host_case *h, int *case_id)
 {
     if (!c || !h || h->cs.blk.empty())
         return MHD_ERR_ARG;
     if (h->cs.blk[0].d.g != c->opt.ghost)
         return fail(c, MHD_ERR_ARG, "host case built with another ghost width");
+    if (h->cs.shape_key.empty())
+        h->cs.shape_key = geometry_key(h->cs.blk);
+    if (int rc = check_edited_tables(c, h->cs))
+        return rc;
     const double budget = c->opt.hbm_budget_gb * 1e9;
     if (c->finalized && budget > 0 && (double)(c->bytes_used + case_bytes(c, h->cs)) > budget)
         return fail(c, MHD_ERR_MEM, "HBM budget exceeded"); // the host case stays intact
@@ -1178,7 +1267,23 @@ extern "C" int mhd_retire(mhd_ctx *c, int case_id)
             return rc;
         r.op_n[sy] = 0;
     }
+    const int zero = 0;
+    HIPCHK(c, hipMemcpy(c->d_err + case_id, &zero, sizeof(int), hipMemcpyHostToDevice));
     return rebuild(c);
+}
+
+extern "C" int mhd_case_error(mhd_ctx *c, int case_id, int *dt_error)
+{
+    if (!c || !dt_error || case_id < 0 || case_id >= (int)c->cases.size() || !c->finalized)
+        return MHD_ERR_ARG;
+    int e = 0;
+    if (c->cases[case_id]->state == 1)
+    {
+        HIPCHK(c, hipMemcpyAsync(&e, c->d_err + case_id, sizeof(int), hipMemcpyDeviceToHost, c->q));
+        HIPCHK(c, hipStreamSynchronize(c->q));
+    }
+    *dt_error = e;
+    return MHD_OK;
 }
 
 // ------------------------------------------------------------- stepping
@@ -1443,6 +1548,40 @@ static int bcell_weights_down(mhd_ctx *c, const MhdBlockDev &d, bool down, doubl
     return MHD_OK;
 }
 
+// Copy-on-write of a case's geometry before an upload changes it: a set
+// other resident cases share is copied for this case alone (a key no later
+// case can match), so their steps are not touched.
+static int own_geometry(mhd_ctx *c, CaseRec &r)
+{
+    GeoSet &S = c->geos[r.geo_set];
+    if (S.refs <= 1)
+    {
+        S.key = "own:" + std::to_string(r.id);
+        return MHD_OK;
+    }
+    const double budget = c->opt.hbm_budget_gb * 1e9;
+    if (budget > 0 && (double)(c->bytes_used + S.bytes) > budget)
+        return fail(c, MHD_ERR_MEM, "HBM budget exceeded (copy of a shared geometry before an upload)");
+    GeoSet G;
+    G.key = "own:" + std::to_string(r.id);
+    G.block_bytes = S.block_bytes;
+    G.bytes = S.bytes;
+    G.refs = 1;
+    HIPCHK(c, hipMalloc((void **)&G.d, G.bytes));
+    HIPCHK(c, hipMemcpy(G.d, S.d, G.bytes, hipMemcpyDeviceToDevice));
+    c->bytes_used += G.bytes;
+    S.refs -= 1;
+    int gi = 0;
+    while (gi < (int)c->geos.size() && c->geos[gi].refs > 0)
+        ++gi;
+    if (gi == (int)c->geos.size())
+        c->geos.push_back(G);
+    else
+        c->geos[gi] = G;
+    r.geo_set = gi;
+    return rebuild(c); // block table: this case's new geometry addresses
+}
+
 static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, size_t n, bool down)
 {
     if (!aos || !c)
@@ -1455,14 +1594,26 @@ static int exchange(mhd_ctx *c, int case_id, int block, int field, double *aos, 
     int region, slot, loc, nc;
     if (!field_slot(field, c->ucur, region, slot, loc, nc) || region == 2)
         return fail(c, MHD_ERR_ARG, "field not stored on the device");
-    const MhdBlockDev &d = c->h_blk[r->first_block + block];
-    if (region == 0 && slot + nc > st_nslot(d.model))
-        return fail(c, MHD_ERR_ARG, "field not held by this case (two-fluid cases only)");
-    if (n != aos_count(d, loc, nc))
-        return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
+    {
+        const MhdBlockDev &d0 = c->h_blk[r->first_block + block];
+        if (region == 0 && slot + nc > st_nslot(d0.model))
+            return fail(c, MHD_ERR_ARG, "field not held by this case (two-fluid cases only)");
+        if (n != aos_count(d0, loc, nc))
+            return fail(c, MHD_ERR_ARG, "buffer size does not match the FieldBlock extent");
+    }
     HIPCHK(c, hipStreamSynchronize(c->q));
     if (region == 3)
-        return bcell_weights_down(c, d, down, aos);
+        return bcell_weights_down(c, c->h_blk[r->first_block + block], down, aos);
+    if (region == 1 && !down)
+    {
+        if (slot >= G_JD && slot < G_JD + 9)
+            return fail(c, MHD_ERR_ARG, "the B_cell weights of a device case are formed from its face area vectors: "
+                                        "JDxi/JDet/JDze cannot be replaced (build the case with the new metric)");
+        int rc = own_geometry(c, *r);
+        if (rc)
+            return rc;
+    }
+    const MhdBlockDev &d = c->h_blk[r->first_block + block]; // (re-read: own_geometry may move it)
     std::vector<double> soa((size_t)nc * d.slot);
     if (region == 1 && mixed_of(c) && slot >= G_NMETRIC)
     {
@@ -1685,8 +1836,10 @@ extern "C" int mhd_refresh(mhd_ctx *c)
     run_ops(c, SY_U_HALO, c->ucur);
     run_ops(c, SY_B_BC, 0);
     run_ops(c, SY_B_HALO, 0);
-    // the next step's dt is a min accumulated by the derived kernel: restart it
+    // the next step's dt is a min accumulated by the derived kernel: restart
+    // it, and the dt error flags with it (the state is a new one)
     mhdk::launch_fill_u64(c->d_dtbits, c->case_cap, dbits(1e100), c->q);
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int) * c->case_cap, c->q));
     derived(c);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->q));
@@ -1765,11 +1918,16 @@ extern "C" int mhd_read_restart(mhd_ctx *c, int case_id, const char *path, long 
         DeviceIO io(c, case_id, r);
         const int rc = restart_read(io, path, step, time);
         if (rc)
+        {
+            // a copy failed after the file was checked: bring the ghosts and
+            // derived fields in line with what was written
+            (void)mhd_refresh(c);
             return rc;
+        }
     }
     catch (const std::exception &e)
     {
-        return fail(c, MHD_ERR_ARG, e.what());
+        return fail(c, MHD_ERR_ARG, e.what()); // the file was rejected before any field was written
     }
     return mhd_refresh(c);
 }

@@ -146,6 +146,7 @@ static void finalize_grid(CaseSetup &cs, int model = 0)
 {
     fill_ghost_nodes(cs.grid);
     build_geometry(cs.grid, cs.blk);
+    cs.shape_key = geometry_key(cs.blk);
     for (auto &A : cs.blk)
     {
         A.d.case_id = 0;
@@ -159,9 +160,6 @@ CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t
     CaseSetup cs;
     cs.grid = make_periodic_box(n, n, n, 1.0, 1.0, 1.0, g);
     finalize_grid(cs);
-    char key[96];
-    std::snprintf(key, sizeof key, "box:%d:%d", n, g);
-    cs.shape_key = key;
 
     MhdCaseConst &c = cs.cst;
     c.gamma = gamma;
@@ -237,17 +235,15 @@ CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, 
                           const WindParams &w)
 {
     char key[128];
-    std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
-    return make_case_lunar_on(make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD), key, gamma,
-                              cfl, w);
+    return make_case_lunar_on(make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD), gamma, cfl,
+                              w);
 }
 
-CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gamma, double cfl, const WindParams &w)
+CaseSetup make_case_lunar_on(Grid grid, double gamma, double cfl, const WindParams &w)
 {
     CaseSetup cs;
     cs.grid = std::move(grid);
     finalize_grid(cs);
-    cs.shape_key = shape_key;
 
     const LunarRef R;
     MhdCaseConst &c = cs.cst;
@@ -387,9 +383,6 @@ CaseSetup make_case_mercury(int nang, int nrad, double r_in, double r_out, int g
     CaseSetup cs;
     cs.grid = make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD);
     finalize_grid(cs, dp.two_fluid ? 1 : 0);
-    char key[128];
-    std::snprintf(key, sizeof key, "shell:%d:%d:%.6g:%.6g:%d", nang, nrad, r_in, r_out, g);
-    cs.shape_key = key;
 
     struct
     {

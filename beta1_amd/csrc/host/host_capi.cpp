@@ -283,6 +283,10 @@ extern "C" int mhd_host_upload(mhd_host_case *h, int block, int field, const dou
         return MHD_ERR_ARG;
     std::vector<double> &v = region == 0 ? A.st : (region == 1 ? A.geo : (region == 2 ? A.wj : A.wb));
     soa_aos(A.d, loc, nc, v.data() + (size_t)slot * A.d.slot, const_cast<double *>(aos), false);
+    if (region != 0)
+        h->cs.shape_key.clear(); // edited geometry: no longer the same as any resident copy
+    if (region >= 2)
+        h->cs.tables_edited = true;
     return MHD_OK;
 }
 
@@ -444,10 +448,16 @@ int restart_read(FieldIO &io, const char *path, long long *step, double *time)
     long long st = 0;
     double t = 0.0;
     const std::vector<RestartField> fields = read_restart_file(path, st, t);
+    // every whitelisted field is checked before any is written
+    // (IOModule::ReadRestartBinFile, src/5_io/IOModule_Bin.cpp:234-370: the
+    // location, components and extents must agree, and a block the file holds
+    // no data for cannot restore a block the case holds); the ghost depth may
+    // differ (the overlap is copied)
+    const std::vector<RestartSpec> wl = restart_fields(io.block(0).model);
+    std::vector<std::pair<const RestartField *, const RestartSpec *>> use;
     for (const RestartField &F : fields)
     {
         const RestartSpec *spec = nullptr;
-        const std::vector<RestartSpec> wl = restart_fields(io.block(0).model);
         for (const RestartSpec &S : wl)
             if (F.name == S.name)
                 spec = &S;
@@ -467,6 +477,23 @@ int restart_read(FieldIO &io, const char *path, long long *step, double *time)
             for (int a = 0; a < 3; ++a)
                 if (B.hi[a] - B.lo[a] - 2 * F.nghost != hi[a] - lo[a] - 2 * io.block(b).g)
                     throw std::runtime_error("restart: block extent mismatch for " + F.name);
+            if (B.v.empty())
+                throw std::runtime_error("restart: block " + std::to_string(b) + " of " + F.name +
+                                         " is inactive in the file but held by the case");
+        }
+        use.emplace_back(&F, spec);
+    }
+    for (const auto &fs : use)
+    {
+        const RestartField &F = *fs.first;
+        const RestartSpec *spec = fs.second;
+        int region, slot, loc, nc;
+        field_slot(spec->field, 0, region, slot, loc, nc);
+        for (int b = 0; b < io.nblocks(); ++b)
+        {
+            const RestartBlock &B = F.blocks[b];
+            int lo[3], hi[3];
+            box_of(io.block(b), loc, lo, hi);
             std::vector<double> aos;
             int rc = io.get(b, spec->field, aos);
             if (rc)
@@ -543,8 +570,7 @@ extern "C" int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, i
     try
     {
         auto h = std::make_unique<mhd_host_case>();
-        h->cs = make_case_lunar_on(read_case_dir(dir, ghost), std::string("dir:") + dir + ":" + std::to_string(ghost),
-                                   d->gamma, d->cfl, wind_params(*d));
+        h->cs = make_case_lunar_on(read_case_dir(dir, ghost), d->gamma, d->cfl, wind_params(*d));
         *out = h.release();
         return MHD_OK;
     }

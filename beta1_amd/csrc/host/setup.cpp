@@ -711,6 +711,159 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
     }
 }
 
+// ------------------------------------------------- geometry key and tables
+namespace
+{
+// 64-bit hash of a run of words (four interleaved multiply-rotate lanes).
+uint64_t hash_words(const uint64_t *w, size_t n, uint64_t seed)
+{
+    const uint64_t P1 = 0x9E3779B185EBCA87ull, P2 = 0xC2B2AE3D27D4EB4Full;
+    uint64_t h[4] = {seed + P1 + P2, seed + P2, seed, seed - P1};
+    auto round = [&](uint64_t acc, uint64_t x) {
+        acc += x * P2;
+        acc = (acc << 31) | (acc >> 33);
+        return acc * P1;
+    };
+    size_t q = 0;
+    for (; q + 4 <= n; q += 4)
+        for (int l = 0; l < 4; ++l)
+            h[l] = round(h[l], w[q + l]);
+    uint64_t r = h[0] ^ (h[1] * 3) ^ (h[2] * 5) ^ (h[3] * 7) ^ n;
+    for (; q < n; ++q)
+        r = round(r, w[q]);
+    r ^= r >> 33;
+    r *= 0xff51afd7ed558ccdull;
+    r ^= r >> 33;
+    return r;
+}
+} // namespace
+
+std::string geometry_key(const std::vector<BlockArrays> &blk)
+{
+    const size_t chunk = (size_t)1 << 20;
+    std::vector<std::pair<size_t, size_t>> runs; // (block, first word)
+    for (size_t b = 0; b < blk.size(); ++b)
+        for (size_t o = 0; o < blk[b].geo.size(); o += chunk)
+            runs.emplace_back(b, o);
+    std::vector<uint64_t> part(runs.size());
+#pragma omp parallel for schedule(dynamic)
+    for (long r = 0; r < (long)runs.size(); ++r)
+    {
+        const std::vector<double> &g = blk[runs[r].first].geo;
+        const size_t o = runs[r].second, n = std::min(chunk, g.size() - o);
+        part[r] = hash_words(reinterpret_cast<const uint64_t *>(g.data() + o), n, (uint64_t)r);
+    }
+    const uint64_t h = hash_words(part.data(), part.size(), 0x243F6A8885A308D3ull);
+    std::string key = "geo:" + std::to_string(blk.size());
+    if (!blk.empty())
+    {
+        const MhdBlockDev &d = blk[0].d;
+        key += ":" + std::to_string(d.ni) + "x" + std::to_string(d.nj) + "x" + std::to_string(d.nk) + ":g" +
+               std::to_string(d.g);
+    }
+    char hex[24];
+    std::snprintf(hex, sizeof hex, ":%016llx", (unsigned long long)h);
+    return key + hex;
+}
+
+void formed_bcell_weights(const BlockArrays &A, int i, int j, int k, double W[18])
+{
+    const MhdBlockDev &d = A.d;
+    const int64_t sl = d.slot, o = blk_off(d, i, j, k), step[3] = {(int64_t)d.sj * d.sk, (int64_t)d.sk, 1};
+    const double *g = A.geo.data();
+    const double *mi = g + (size_t)G_WBM * sl, *mu = g + (size_t)G_WBU * sl, *jd = g + (size_t)G_JD * sl;
+    const double m00 = mi[o], m01 = mi[sl + o], m02 = mi[2 * sl + o], m11 = mi[3 * sl + o], m12 = mi[4 * sl + o],
+                 m22 = mi[5 * sl + o];
+    for (int n = 0; n < 6; ++n)
+    {
+        const int ax = n >> 1;
+        const int64_t oo = o + ((n & 1) ? step[ax] : 0);
+        double sv[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            const double v = jd[(size_t)(3 * ax + a) * sl + oo];
+            sv[a] = (n & 1) ? v : -v;
+        }
+        const double m = mu[(size_t)n * sl + o];
+        const double rx = m * sv[0], ry = m * sv[1], rz = m * sv[2];
+        W[n] = m00 * rx + m01 * ry + m02 * rz;
+        W[6 + n] = m01 * rx + m11 * ry + m12 * rz;
+        W[12 + n] = m02 * rx + m12 * ry + m22 * rz;
+    }
+}
+
+void formed_jcell_weights(const BlockArrays &A, int i, int j, int k, double W[36])
+{
+    // edge q runs from node kEs[q] to kEe[q] of the cell, node m = (i + (m >> 2),
+    // j + (m & 1), k + ((m >> 1) & 1)): xi edges 0..3, eta 4..7, zeta 8..11
+    static const int kEs[12] = {0, 1, 2, 3, 0, 4, 2, 6, 0, 4, 1, 5};
+    static const int kEe[12] = {4, 5, 6, 7, 1, 5, 3, 7, 2, 6, 3, 7};
+    const MhdBlockDev &d = A.d;
+    const int64_t sl = d.slot, o = blk_off(d, i, j, k);
+    const double *g = A.geo.data();
+    const double *X = g + (size_t)G_NODE * sl, *mw = g + (size_t)G_WJM * sl;
+    const int64_t q[4] = {0, (int64_t)d.sk, 1, (int64_t)d.sk + 1}, di = (int64_t)d.sj * d.sk;
+    double nd[8][3];
+    for (int m = 0; m < 8; ++m)
+        for (int c = 0; c < 3; ++c)
+            nd[m][c] = X[(size_t)c * sl + o + (m >= 4 ? di : 0) + q[m & 3]];
+    const double m00 = mw[o], m01 = mw[sl + o], m02 = mw[2 * sl + o], m11 = mw[3 * sl + o], m12 = mw[4 * sl + o],
+                 m22 = mw[5 * sl + o];
+    const double mi[9] = {m00, m01, m02, m01, m11, m12, m02, m12, m22};
+    const unsigned act = (unsigned)g[(size_t)G_WJA * sl + o];
+    for (int n = 0; n < 12; ++n)
+    {
+        const double rx = nd[kEe[n]][0] - nd[kEs[n]][0], ry = nd[kEe[n]][1] - nd[kEs[n]][1],
+                     rz = nd[kEe[n]][2] - nd[kEs[n]][2];
+        const bool on = (act >> n) & 1u;
+        W[n] = on ? mi[0] * rx + mi[1] * ry + mi[2] * rz : 0.0;
+        W[12 + n] = on ? mi[3] * rx + mi[4] * ry + mi[5] * rz : 0.0;
+        W[24 + n] = on ? mi[6] * rx + mi[7] * ry + mi[8] * rz : 0.0;
+    }
+}
+
+bool weight_tables_match(const BlockArrays &A, const double *wb_aos, const double *wj_aos, double rtol,
+                         std::string &why)
+{
+    const MhdBlockDev &d = A.d;
+    // AoS FieldBlock of a cell field with ghost g: [i][j][k][m] over (n + 2g)^3
+    const int ej = d.nj + 2 * d.g, ek = d.nk + 2 * d.g;
+    for (int i = 0; i < d.ni; ++i)
+        for (int j = 0; j < d.nj; ++j)
+            for (int k = 0; k < d.nk; ++k)
+            {
+                const size_t cell = ((size_t)(i + d.g) * ej + (size_t)(j + d.g)) * ek + (size_t)(k + d.g);
+                for (int t = 0; t < 2; ++t)
+                {
+                    const double *given = t == 0 ? wb_aos : wj_aos;
+                    if (!given)
+                        continue;
+                    const int nw = t == 0 ? 18 : 36;
+                    double W[36];
+                    if (t == 0)
+                        formed_bcell_weights(A, i, j, k, W);
+                    else
+                        formed_jcell_weights(A, i, j, k, W);
+                    double scale = 0.0;
+                    for (int q = 0; q < nw; ++q)
+                        scale = std::max(scale, std::fabs(W[q]));
+                    for (int q = 0; q < nw; ++q)
+                    {
+                        const double v = given[cell * nw + q];
+                        if (!(std::fabs(v - W[q]) <= rtol * scale))
+                        {
+                            char buf[192];
+                            std::snprintf(buf, sizeof buf, "%s weight %d of cell (%d, %d, %d): %.17g, the geometry gives %.17g",
+                                          t == 0 ? "Bcell_from_Bface_w" : "Jcell_from_Jedge_w", q, i, j, k, v, W[q]);
+                            why = buf;
+                            return false;
+                        }
+                    }
+                }
+            }
+    return true;
+}
+
 // ------------------------------------------------------------- halo rules
 static int map_loc(int loc, const int perm[3])
 {

@@ -134,7 +134,8 @@ struct CaseSetup
     // SY_COUNT op lists; offsets are case-relative state offsets
     // (block * ST_NSLOT * slot + state_slot * slot + blk_off)
     std::vector<std::vector<MhdOp>> ops;
-    std::string shape_key; // identical keys share geometry
+    std::string shape_key; // identical keys share geometry (geometry_key; empty: recompute)
+    bool tables_edited = false; // the full weight tables were replaced on the host
     double L_ref = 0.0;    // body radius [m] of a Mercury case (neutral exosphere for output)
 };
 
@@ -223,8 +224,27 @@ CaseSetup make_case_orszag_tang(int n, int g, double gamma, double cfl, uint64_t
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma,
                           double cfl, const WindParams &w);
 // Lunar physics and initial state on a given grid (ghost nodes are filled
-// here); shape_key groups cases that may share geometry.
-CaseSetup make_case_lunar_on(Grid grid, const std::string &shape_key, double gamma, double cfl, const WindParams &w);
+// here).
+CaseSetup make_case_lunar_on(Grid grid, double gamma, double cfl, const WindParams &w);
+
+// Content key of a case's geometry: block count, extents, ghost depth and a
+// 64-bit hash of every stored geometry word.  Cases share one device copy of
+// their geometry only when their keys are equal.
+std::string geometry_key(const std::vector<BlockArrays> &blk);
+
+// The weight tables the kernels form from the stored least-squares data (the
+// host builder's arithmetic, so bit-identical to its tables): 18
+// Bcell_from_Bface_w weights (x of the six faces, then y, then z) and 36
+// Jcell_from_Jedge_w weights (x of the twelve edges, then y, then z) of
+// interior cell (i, j, k).
+void formed_bcell_weights(const BlockArrays &A, int i, int j, int k, double W[18]);
+void formed_jcell_weights(const BlockArrays &A, int i, int j, int k, double W[36]);
+// Compare tables given in the AoS FieldBlock layout (18 / 36 components,
+// either may be null) with the formed ones over the interior cells: true
+// when every weight is within rtol of the cell's largest formed weight, else
+// false with the first offending block cell in `why`.
+bool weight_tables_match(const BlockArrays &A, const double *wb_aos, const double *wj_aos, double rtol,
+                         std::string &why);
 
 // Deterministic generator (SplitMix64).
 uint64_t splitmix64(uint64_t &s);

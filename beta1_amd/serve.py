@@ -33,6 +33,7 @@ class Request:
     cid: int = -1
     admitted: float = -1.0
     done: float = -1.0
+    failed: bool = False    # its dt met the abort condition: retired without a result
     result: dict = field(default_factory=dict)
 
     @property
@@ -84,7 +85,22 @@ class ServingEngine:
                 continue
             left = min(r.steps - self.ctx.case_steps(cid) for cid, r in active.items())
             n = max(1, min(self.chunk, left))
-            self.ctx.step(n, graph=self.graph and n >= 2)
+            try:
+                self.ctx.step(n, graph=self.graph and n >= 2)
+            except MhdError as e:
+                if e.code != -3:
+                    raise
+                # a case met the dt abort condition: it leaves the batch
+                # without a result; every other case stepped normally
+                for cid in [c for c in active if self.ctx.case_error(c)]:
+                    r = active.pop(cid)
+                    r.failed = True
+                    r.done = time.perf_counter() - t0
+                    self.ctx.retire(cid)
+                    finished.append(r)
+                    if log:
+                        log(f"[serve] request {r.rid} ({r.kind}) failed: dt abort after "
+                            f"{self.ctx.case_steps(cid)} steps")
             cell_steps += n * sum(r.cells for r in active.values())
             for cid in [c for c, r in active.items() if self.ctx.case_steps(c) >= r.steps]:
                 r = active.pop(cid)
@@ -99,9 +115,10 @@ class ServingEngine:
                     log(f"[serve] request {r.rid} ({r.kind}, {r.cells} cells) done: "
                         f"latency {r.done - r.arrival:.2f}s, {len(active)} active, {len(building)} queued")
         wall = time.perf_counter() - t0
-        lat = np.array([r.done - r.arrival for r in finished])
+        lat = np.array([r.done - r.arrival for r in finished if not r.failed])
         return {
             "requests": len(finished),
+            "failed": sum(r.failed for r in finished),
             "wall_s": wall,
             "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,

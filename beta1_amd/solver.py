@@ -116,7 +116,7 @@ def _check(rc, ctx=None, what=""):
         if ctx is not None:
             m = lib().mhd_last_error(ctx)
             msg = m.decode() if m else ""
-        raise MhdError(f"{what} failed ({rc}): {msg}")
+        raise MhdError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def _dptr(a):
@@ -273,6 +273,12 @@ class MhdContext:
         v = C.c_longlong()
         _check(lib().mhd_case_steps(self.ctx, cid, C.byref(v)), self.ctx, "case_steps")
         return v.value
+
+    def case_error(self, cid):
+        """True when a step of case ``cid`` met the dt abort condition."""
+        v = C.c_int()
+        _check(lib().mhd_case_error(self.ctx, cid, C.byref(v)), self.ctx, "case_error")
+        return bool(v.value)
 
     def tuning(self):
         """kernel -> (workgroup rows, tile order, "geo-shared" or "per-case")"""

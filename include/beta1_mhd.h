@@ -58,7 +58,7 @@ typedef struct
     int device;    /* HIP device ordinal */
     int precision; /* mhd_precision */
     int recon;     /* 0 first order (reference), 1 minmod MUSCL */
-    int ghost;     /* ghost layers (>= 1; >= 2 for minmod) */
+    int ghost;     /* ghost layers (>= 2; the first-order stencil reads one, minmod two) */
     double hbm_budget_gb; /* admission budget: bytes of state + geometry (0 = no limit) */
     int autotune;         /* 1: time workgroup heights per kernel at mhd_finalize */
     int xcd_order;        /* 1: deal tiles to the 8 XCDs in contiguous ranges */
@@ -160,6 +160,12 @@ int mhd_finalize(mhd_ctx *ctx);
 int mhd_retire(mhd_ctx *ctx, int case_id);
 /* Steps a case has been advanced by. */
 int mhd_case_steps(mhd_ctx *ctx, int case_id, long long *steps);
+/* 1 in *dt_error when a step of the case met the dt abort condition
+   (NaN or below the floor, LunarSolver_Timestep.cpp:128-139: where the
+   reference exits, mhd_step returns MHD_ERR_DT while such a case is
+   resident; the other cases keep their own dt and are advanced normally).
+   Cleared by mhd_retire and mhd_refresh.  0 for a retired case. */
+int mhd_case_error(mhd_ctx *ctx, int case_id, int *dt_error);
 /* Min/max diagnostics over the interior cells of a case
    (PrintMinMaxDiagnostics_, Z4_Lunar/src/tools/LunarSolver_PrintDiag.cpp:7-60,
    Z5_Mercury/src/tools/MercurySolver_PrintDiag.cpp:7-92): out[0..7] minima

@@ -208,3 +208,61 @@ def test_two_fluid_restart_holds_na(tmp_path):
     assert list(f) == ["U_H", "U_Na", "B_xi", "B_eta", "B_zeta"]  # the Mercury whitelist
     for b, (lo, hi, v) in enumerate(f["U_Na"][3]):
         assert np.array_equal(v, hc.field("U_Na", b))
+
+
+def restart_with_inactive_block(src, dst, field, block):
+    """Rewrite a restart file with `block` of `field` marked inactive (no data)."""
+    raw = bytearray(open(src, "rb").read())
+    nb, nf = struct.unpack_from("<ii", raw, 24)
+    off, out = 32, bytearray(raw[:32])
+    for _ in range(nf):
+        n = struct.unpack_from("<i", raw, off)[0]
+        name = raw[off + 4: off + 4 + n].decode()
+        head = raw[off: off + 4 + n + 12]
+        nc = struct.unpack_from("<i", raw, off + 4 + n + 4)[0]
+        off += 4 + n + 12
+        out += head
+        for b in range(nb):
+            lo = struct.unpack_from("<3i", raw, off)
+            hi = struct.unpack_from("<3i", raw, off + 12)
+            active = struct.unpack_from("<i", raw, off + 24)[0]
+            cnt = int(np.prod([h - l for l, h in zip(lo, hi)])) * nc if active else 0
+            data = raw[off + 28: off + 28 + 8 * cnt]
+            if name == field and b == block:
+                out += raw[off: off + 24] + struct.pack("<i", 0)
+            else:
+                out += raw[off: off + 28] + data
+            off += 28 + 8 * cnt
+    open(dst, "wb").write(bytes(out))
+
+
+def test_restart_rejects_bad_files_and_writes_nothing(tmp_path):
+    """IOModule::ReadRestartBinFile fails on a file block without data for a
+    block the case holds (src/5_io/IOModule_Bin.cpp:340-350) and on a cut
+    file; here every field is checked before any is written, so a rejected
+    file leaves the case as it was."""
+    hc = HostCase(WindDesc(nang=4, nrad=6), ghost=G)
+    for sy in (3, 4, 5, 6, 7):
+        hc.apply_sync(sy)
+    good = str(tmp_path / "good.bin")
+    hc.write_restart(good, step=3, time=0.25)
+    before = {(n, b): hc.field(n, b) for n in ("U_H", "B_xi", "B_eta", "B_zeta") for b in range(6)}
+    # a file from another state, then spoiled
+    other = HostCase(WindDesc(nang=4, nrad=6, v_kms=650.0), ghost=G)
+    for sy in (3, 4, 5, 6, 7):
+        other.apply_sync(sy)
+    src = str(tmp_path / "other.bin")
+    other.write_restart(src)
+    inactive = str(tmp_path / "inactive.bin")
+    restart_with_inactive_block(src, inactive, "B_zeta", 4)
+    cut = str(tmp_path / "cut.bin")
+    open(cut, "wb").write(open(src, "rb").read()[:-1000])
+    for bad in (inactive, cut):
+        with pytest.raises(MhdError):
+            hc.read_restart(bad)
+        for (n, b), a in before.items():
+            assert np.array_equal(hc.field(n, b), a), (bad, n, b)
+    assert hc.read_restart(good) == (3, 0.25)
+    # the spoiled file's other fields are fine: reading the unspoiled one works
+    hc.read_restart(src)
+    assert not np.array_equal(hc.field("U_H", 0), before[("U_H", 0)])
