@@ -7,4 +7,4 @@ missing, :func:`lib` raises.
 """
 from .capi import lib, LIB_PATH, MhdError  # noqa: F401
 from .fields import FIELDS, FIELD_ID, LOCATION, NCOMP  # noqa: F401
-from .solver import CaseDirDesc, HostCase, MhdContext, LunarSolver, MercuryDesc, OTDesc, WindDesc  # noqa: F401
+from .solver import CaseDirDesc, HostCase, MhdContext, LunarSolver, MercuryDesc, OTDesc, WindDesc, case_parts  # noqa: F401

@@ -35,7 +35,7 @@ def build(verbose=False):
     objs = []
     jobs = []
     for src in ("host/setup.cpp", "host/cases.cpp", "host/host_capi.cpp", "host/formats.cpp",
-                "host/tecplot.cpp"):
+                "host/tecplot.cpp", "host/dropin.cpp"):
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         jobs.append(["g++", *COMMON, "-fopenmp", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -102,6 +102,23 @@ SIGNATURES = {
     "mhd_host_constants_two_fluid": (_I, [_P, _PD]),
     "mhd_host_apply_sync": (_I, [_P, _I]),
     "mhd_host_op_count": (_I, [_P, _I, C.POINTER(_S)]),
+    "mhd_case_begin": (_I, [_P, _I, _PD, _PI]),
+    "mhd_add_block": (_I, [_P, _I, _I, _I, _I, _I, _PD, _PD, _PD, _S, C.POINTER(_PD)]),
+    "mhd_set_topology": (_I, [_P, _I, _I, _PI]),
+    "mhd_case_end": (_I, [_P, _I]),
+    "mhd_host_case_begin": (_I, [_I, _I, _PD, C.POINTER(_P)]),
+    "mhd_host_add_block": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _S, C.POINTER(_PD)]),
+    "mhd_host_set_topology": (_I, [_P, _I, _PI]),
+    "mhd_host_case_end": (_I, [_P]),
+    "mhd_host_last_error": (C.c_char_p, [_P]),
+    "mhd_step_dt": (_I, [_P]),
+    "mhd_fluid_rhs": (_I, [_P]),
+    "mhd_face_emf": (_I, [_P]),
+    "mhd_edge_emf": (_I, [_P]),
+    "mhd_curl_update": (_I, [_P]),
+    "mhd_apply_update": (_I, [_P]),
+    "mhd_sync": (_I, [_P, C.c_char_p]),
+    "mhd_derived": (_I, [_P]),
 }
 
 _lib = None

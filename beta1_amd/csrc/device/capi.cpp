@@ -31,11 +31,20 @@ struct CaseRec
     double *d_st = nullptr; // the case's state allocation
     int64_t slot = 0;
     int nblocks = 0;
-    int state = 0;        // 0 pending, 1 active, 2 retired
+    int state = 0;        // 0 pending, 1 active, 2 retired, 3 draft (mhd_case_begin .. mhd_case_end)
     int first_block = -1; // index into the block table while active
     size_t op_off[SY_COUNT] = {0}, op_n[SY_COUNT] = {0}; // segments in the op groups
     long long steps = 0;
     size_t st_bytes = 0;
+    mhd_host_case *draft = nullptr; // the blocks given so far (state 3)
+    CaseRec() = default;
+    CaseRec(const CaseRec &) = delete;
+    CaseRec &operator=(const CaseRec &) = delete;
+    ~CaseRec()
+    {
+        if (draft)
+            mhd_host_case_destroy(draft);
+    }
 };
 
 struct GeoSet
@@ -175,6 +184,9 @@ struct mhd_ctx
     OpGroup og[SY_COUNT];
 
     hipGraphExec_t gexec = nullptr;
+    // per-stage entry points: mhd_fluid_rhs has written the next U buffer and
+    // mhd_apply_update has not yet made it current
+    bool fluid_pending = false;
 
     bool prof = false;
     double prof_ms[KP_COUNT] = {0};
@@ -891,68 +903,91 @@ static void derived(mhd_ctx *c)
     run_ops2(c, SY_BCELL_HALO, 0, SY_JCELL_HALO, 0);
 }
 
-// One StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) for every active
-// case.  dt and the contravariant velocities were computed by the previous
-// step's derived-field kernel (or at placement).
-static void enqueue_step(mhd_ctx *c)
+// The stages of one StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92)
+// over every active case, in the reference's order.  enqueue_step runs them
+// back to back; the per-stage entry points of the C-ABI run them one at a time.
+
+// Compute_Timestep: the min the previous derived pass accumulated becomes
+// each case's dt (LunarSolver_Timestep.cpp:8-140, abort check :128-139).
+static void stage_dt(mhd_ctx *c)
 {
-    const int mixed = c->opt.precision == MHD_PREC_MIXED;
+    ProfScope p(c, KP_DTFIN);
+    mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, c->case_cap, c->d_active, c->q);
+}
+
+// Scheme_U_ + AddSourceToRHS_Fluid + the U part of ApplyUpdate_Euler_
+// (LunarSolver_FluidRHS.cpp:3-201,528-568, LunarSolver_Advance.cpp:61-122):
+// reads U[cur], B_cell, J_cell, p_e and dt, writes U + dt RHS into U[1-cur].
+static void stage_fluid(mhd_ctx *c, hipStream_t qf)
+{
+    if (runs(c, KP_FLUID))
     {
-        ProfScope p(c, KP_DTFIN);
-        mhdk::launch_dt_finalize(c->d_dtbits, c->d_dt, c->d_err, c->d_cst, c->case_cap, c->d_active, c->q);
+        ProfScope p(c, KP_FLUID);
+        launch_fluid_any(c, -1, c->d_dt, qf);
     }
-    // The fluid update reads U[cur], B_cell, J_cell, p_e and dt and writes
-    // U[1-cur]; the EMF / curl / resistive chain reads U[cur] and writes
-    // Eface, E, B faces, J_cell (subcycles) -- J_cell only after the fluid
-    // update has read it, so the subcycles wait for the fluid branch.
+    if (runs(c, KP_FLUID2))
     {
-        const hipStream_t qf = fork_branch(c, 0);
-        if (runs(c, KP_FLUID))
-        {
-            ProfScope p(c, KP_FLUID);
-            launch_fluid_any(c, -1, c->d_dt, qf);
-        }
-        if (runs(c, KP_FLUID2))
-        {
-            ProfScope p(c, KP_FLUID2);
-            mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed, c->opt.recon, qf);
-        }
+        ProfScope p(c, KP_FLUID2);
+        mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed_of(c), c->opt.recon, qf);
     }
+}
+
+// AssembleOneDirectionEMF_ for the three face families
+// (LunarSolver_MagExplicit_Ideal.cpp:348-496), after the cube-vertex cells
+// took the contravariant velocity of their source cells.
+static void stage_face_emf(mhd_ctx *c)
+{
     {
         ProfScope p(c, KP_OPS);
         run_ops(c, SY_UC_CORNER, 0);
     }
-    {
-        ProfScope p(c, KP_FACE);
-        mhdk::launch_face_emf(launch_of(c, KP_FACE), mixed, c->q);
-    }
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_EFACE_BC, 0);
-        run_ops(c, SY_EFACE_HALO, 0);
-    }
+    ProfScope p(c, KP_FACE);
+    mhdk::launch_face_emf(launch_of(c, KP_FACE), mixed_of(c), c->q);
+}
+
+// Sync("Eface"): the wall handler, then the ghost candidates the edge EMF reads.
+static void stage_sync_eface(mhd_ctx *c)
+{
+    ProfScope p(c, KP_OPS);
+    run_ops(c, SY_EFACE_BC, 0);
+    run_ops(c, SY_EFACE_HALO, 0);
+}
+
+// AssembleEdgeEMF_FromFaceE_Ideal_ (LunarSolver_MagExplicit_Ideal.cpp:53-287),
+// then the two-fluid EMF terms added before the Eedge sync
+// (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:30-33).
+static void stage_edge_emf(mhd_ctx *c)
+{
     {
         ProfScope p(c, KP_EDGE);
         mhdk::launch_edge_emf(launch_of(c, KP_EDGE), c->q);
     }
     if (runs(c, KP_AMBI))
     {
-        // two-fluid EMF terms added before the Eedge sync
-        // (Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:30-33)
         ProfScope p(c, KP_AMBI);
-        mhdk::launch_emf_extra(launch_of(c, KP_AMBI), c->d_cst, c->ucur, mixed, c->q);
+        mhdk::launch_emf_extra(launch_of(c, KP_AMBI), c->d_cst, c->ucur, mixed_of(c), c->q);
     }
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_EEDGE_BC, 0);
-    }
-    {
-        ProfScope p(c, KP_CURL);
-        mhdk::launch_curl_update(launch_of(c, KP_CURL), c->d_dt, c->q);
-    }
-    join_branch(c, 0);
-    // interior resistivity of the two-fluid cases: explicit subcycles on B
-    // (ResistiveDiffusionSubcycles_, Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141)
+}
+
+// Sync("Eedge"): the far-field edge handler.
+static void stage_sync_eedge(mhd_ctx *c)
+{
+    ProfScope p(c, KP_OPS);
+    run_ops(c, SY_EEDGE_BC, 0);
+}
+
+// CurlEdgeToFace(-1) + the B part of ApplyUpdate_Euler_ (CTOperators.cpp:6-56).
+static void stage_curl(mhd_ctx *c)
+{
+    ProfScope p(c, KP_CURL);
+    mhdk::launch_curl_update(launch_of(c, KP_CURL), c->d_dt, c->q);
+}
+
+// Interior resistivity of the two-fluid cases: explicit subcycles on B
+// (ResistiveDiffusionSubcycles_, Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141).
+static void stage_resistive(mhd_ctx *c)
+{
+    const int mixed = mixed_of(c);
     for (int isub = 0; isub < c->max_nsub; ++isub)
     {
         mhdk::launch_dt_res(c->d_dtres, c->d_dt, c->d_cst, c->case_cap, isub, c->q);
@@ -987,6 +1022,26 @@ static void enqueue_step(mhd_ctx *c)
         run_ops(c, SY_B_BC, 0);
         run_ops(c, SY_B_HALO, 0);
     }
+}
+
+// One StepOnce for every active case.  dt and the contravariant velocities
+// were computed by the previous step's derived-field kernel (or at
+// placement).
+static void enqueue_step(mhd_ctx *c)
+{
+    stage_dt(c);
+    // The fluid update reads U[cur], B_cell, J_cell, p_e and dt and writes
+    // U[1-cur]; the EMF / curl / resistive chain reads U[cur] and writes
+    // Eface, E, B faces, J_cell (subcycles) -- J_cell only after the fluid
+    // update has read it, so the subcycles wait for the fluid branch.
+    stage_fluid(c, fork_branch(c, 0));
+    stage_face_emf(c);
+    stage_sync_eface(c);
+    stage_edge_emf(c);
+    stage_sync_eedge(c);
+    stage_curl(c);
+    join_branch(c, 0);
+    stage_resistive(c);
     c->ucur ^= 1;
     {
         // U and B syncs: each field physical -> halo (the U wall ghosts after
@@ -1080,6 +1135,29 @@ static int autotune(mhd_ctx *c)
     return MHD_OK;
 }
 
+// Live admission of the pending case `id` into the running context: place,
+// rebuild the tables, compute the new case's derived fields, dt and
+// contravariant velocities (the derived kernel is idempotent for the cases
+// already running).  *placed tells whether anything of it reached the device.
+static int admit(mhd_ctx *c, int id, bool *placed)
+{
+    *placed = false;
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a step is in progress (mhd_fluid_rhs without mhd_apply_update)");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    int rc = ensure_case_capacity(c);
+    if (rc)
+        return rc;
+    if ((rc = place_case(c, *c->cases[id])))
+        return rc; // rolled back by place_case
+    *placed = true;
+    if ((rc = rebuild(c)))
+        return rc;
+    derived(c);
+    HIPCHK(c, hipGetLastError());
+    return MHD_OK;
+}
+
 static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
 {
     auto r = std::make_unique<CaseRec>();
@@ -1092,29 +1170,13 @@ static int add_case(mhd_ctx *c, CaseSetup &&cs, int *case_id)
         *case_id = id;
     if (!c->finalized)
         return MHD_OK;
-    // live admission: place, rebuild the tables, compute the new case's
-    // derived fields, dt and contravariant velocities (the derived kernel is
-    // idempotent for the cases already running)
-    HIPCHK(c, hipSetDevice(c->opt.device));
-    int rc = ensure_case_capacity(c);
-    if (rc)
-    {
-        c->cases.pop_back();
-        return rc;
-    }
-    if ((rc = place_case(c, *c->cases[id])))
-    {
-        if (rc == MHD_ERR_MEM) // nothing was placed: the id is free again
-            c->cases.pop_back();
-        else
-            c->cases[id]->state = 2; // rolled back by place_case
-        return rc;
-    }
-    if ((rc = rebuild(c)))
-        return rc;
-    derived(c);
-    HIPCHK(c, hipGetLastError());
-    return MHD_OK;
+    bool placed = false;
+    const int rc = admit(c, id, &placed);
+    if (rc && !placed)
+        c->cases.pop_back(); // nothing was placed: the id is free again
+    else if (rc)
+        c->cases[id]->state = 2;
+    return rc;
 }
 
 extern "C" int mhd_add_case_orszag_tang(mhd_ctx *c, const mhd_ot_desc *d, int *case_id)
@@ -1247,6 +1309,8 @@ extern "C" int mhd_retire(mhd_ctx *c, int case_id)
 {
     if (!c || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a staged step is open: mhd_apply_update first");
     CaseRec &r = *c->cases[case_id];
     if (r.state != 1)
         return fail(c, MHD_ERR_STATE, "case is not active");
@@ -1311,6 +1375,8 @@ extern "C" int mhd_step(mhd_ctx *c, int nsteps)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "mhd_finalize first");
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a staged step is open: mhd_apply_update first");
     if (c->h_blk.empty())
         return MHD_OK;
     HIPCHK(c, hipSetDevice(c->opt.device));
@@ -1327,6 +1393,8 @@ extern "C" int mhd_step_graph(mhd_ctx *c, int nsteps)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "mhd_finalize first");
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a staged step is open: mhd_apply_update first");
     if (c->h_blk.empty())
         return MHD_OK;
     if (c->prof)
@@ -1366,6 +1434,201 @@ extern "C" int mhd_synchronize(mhd_ctx *c)
         return MHD_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->q));
     return MHD_OK;
+}
+
+// ------------------------------------------------- per-stage entry points
+// The stages of mhd_step one call at a time, on the context stream, over
+// every active case (the fluid update then runs on the context stream too).
+static int stage_enter(mhd_ctx *c)
+{
+    if (!c)
+        return MHD_ERR_ARG;
+    if (!c->finalized)
+        return fail(c, MHD_ERR_STATE, "mhd_finalize first");
+    HIPCHK(c, hipSetDevice(c->opt.device));
+    return MHD_OK;
+}
+static int stage_leave(mhd_ctx *c)
+{
+    HIPCHK(c, hipGetLastError());
+    return MHD_OK;
+}
+
+extern "C" int mhd_step_dt(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "mhd_step_dt inside an open step");
+    if (c->h_blk.empty())
+        return MHD_OK;
+    stage_dt(c);
+    if (int rc = stage_leave(c))
+        return rc;
+    return check_err(c);
+}
+
+extern "C" int mhd_fluid_rhs(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "mhd_fluid_rhs twice in one step (mhd_apply_update ends a step)");
+    stage_fluid(c, c->q);
+    c->fluid_pending = true;
+    return stage_leave(c);
+}
+
+extern "C" int mhd_face_emf(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    stage_face_emf(c);
+    return stage_leave(c);
+}
+
+extern "C" int mhd_edge_emf(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    stage_edge_emf(c);
+    return stage_leave(c);
+}
+
+extern "C" int mhd_curl_update(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    stage_curl(c);
+    return stage_leave(c);
+}
+
+extern "C" int mhd_apply_update(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    if (!c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "mhd_apply_update without mhd_fluid_rhs in this step");
+    stage_resistive(c);
+    c->ucur ^= 1;
+    c->fluid_pending = false;
+    count_steps(c, 1);
+    return stage_leave(c);
+}
+
+extern "C" int mhd_sync(mhd_ctx *c, const char *group)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    if (!group)
+        return MHD_ERR_ARG;
+    const std::string g(group);
+    if (g == "Eface")
+        stage_sync_eface(c);
+    else if (g == "Eedge")
+        stage_sync_eedge(c);
+    else if (g == "Ucell")
+    {
+        run_ops(c, SY_U_BC0, c->ucur);
+        run_ops(c, SY_U_BC1, c->ucur);
+        run_ops(c, SY_U_HALO, c->ucur);
+    }
+    else if (g == "Bface")
+    {
+        run_ops(c, SY_B_BC, 0);
+        run_ops(c, SY_B_HALO, 0);
+    }
+    else if (g == "B_cell")
+    {
+        run_ops(c, SY_BCELL_BC, 0);
+        run_ops(c, SY_BCELL_HALO, 0);
+    }
+    else if (g == "J_cell")
+    {
+        run_ops(c, SY_JCELL_GH, 0);
+        run_ops(c, SY_JCELL_HALO, 0);
+    }
+    else if (g != "Jedge" && g != "Uplus" && g != "Ehall" && g != "Badd")
+        return fail(c, MHD_ERR_ARG, "unknown sync group " + g);
+    return stage_leave(c);
+}
+
+extern "C" int mhd_derived(mhd_ctx *c)
+{
+    if (int rc = stage_enter(c))
+        return rc;
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "mhd_derived inside an open step: mhd_apply_update first");
+    derived(c);
+    return stage_leave(c);
+}
+
+// --------------------------------------- a case from the caller's arrays
+static CaseRec *draft_of(mhd_ctx *c, int case_id)
+{
+    if (!c || case_id < 0 || case_id >= (int)c->cases.size())
+        return nullptr;
+    CaseRec *r = c->cases[case_id].get();
+    return r->state == 3 && r->draft ? r : nullptr;
+}
+
+extern "C" int mhd_case_begin(mhd_ctx *c, int nblocks, const double *consts22, int *case_id)
+{
+    if (!c || !case_id)
+        return MHD_ERR_ARG;
+    mhd_host_case *h = nullptr;
+    const int rc = mhd_host_case_begin(nblocks, c->opt.ghost, consts22, &h);
+    if (rc)
+        return fail(c, rc, "mhd_case_begin: bad block count or constants");
+    auto r = std::make_unique<CaseRec>();
+    r->id = (int)c->cases.size();
+    r->state = 3;
+    r->draft = h;
+    r->nblocks = nblocks;
+    *case_id = r->id;
+    c->cases.push_back(std::move(r));
+    return MHD_OK;
+}
+
+extern "C" int mhd_add_block(mhd_ctx *c, int case_id, int ib, int ni, int nj, int nk, const double *x, const double *y,
+                             const double *z, size_t nnode, const double *const *geometry)
+{
+    CaseRec *r = draft_of(c, case_id);
+    if (!r)
+        return c ? fail(c, MHD_ERR_STATE, "mhd_add_block: not a case begun with mhd_case_begin") : MHD_ERR_ARG;
+    const int rc = mhd_host_add_block(r->draft, ib, ni, nj, nk, x, y, z, nnode, geometry);
+    return rc ? fail(c, rc, r->draft->err) : MHD_OK;
+}
+
+extern "C" int mhd_set_topology(mhd_ctx *c, int case_id, int ib, const int *faces14x6)
+{
+    CaseRec *r = draft_of(c, case_id);
+    if (!r)
+        return c ? fail(c, MHD_ERR_STATE, "mhd_set_topology: not a case begun with mhd_case_begin") : MHD_ERR_ARG;
+    const int rc = mhd_host_set_topology(r->draft, ib, faces14x6);
+    return rc ? fail(c, rc, r->draft->err) : MHD_OK;
+}
+
+extern "C" int mhd_case_end(mhd_ctx *c, int case_id)
+{
+    CaseRec *r = draft_of(c, case_id);
+    if (!r)
+        return c ? fail(c, MHD_ERR_STATE, "mhd_case_end: not a case begun with mhd_case_begin") : MHD_ERR_ARG;
+    int rc = mhd_host_case_end(r->draft);
+    if (rc)
+        return fail(c, rc, r->draft->err); // the draft stays: blocks or topology may be given again
+    r->cs = std::move(r->draft->cs);
+    mhd_host_case_destroy(r->draft);
+    r->draft = nullptr;
+    r->nblocks = (int)r->cs.blk.size();
+    r->state = 0;
+    if (!c->finalized)
+        return MHD_OK;
+    bool placed = false;
+    rc = admit(c, case_id, &placed);
+    if (rc)
+        r->state = 2; // not admitted (e.g. MHD_ERR_MEM); the id is not reused
+    return rc;
 }
 
 static CaseRec *active_case(mhd_ctx *c, int case_id)
@@ -1653,6 +1916,8 @@ extern "C" int mhd_field_count(mhd_ctx *c, int case_id, int field, size_t *n)
 {
     if (!c || !n || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
+    if (c->cases[case_id]->state == 3)
+        return fail(c, MHD_ERR_STATE, "case not ended (mhd_case_end)");
     int region, slot, loc, nc;
     if (!field_slot(field, 0, region, slot, loc, nc))
         return MHD_ERR_ARG;
@@ -1664,6 +1929,8 @@ extern "C" int mhd_case_info(mhd_ctx *c, int case_id, int *nblocks, int *ni, int
 {
     if (!c || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
+    if (c->cases[case_id]->state == 3)
+        return fail(c, MHD_ERR_STATE, "case not ended (mhd_case_end)");
     const CaseRec &r = *c->cases[case_id];
     const MhdBlockDev &d = r.cs.blk[0].d;
     if (nblocks)
@@ -1683,6 +1950,8 @@ extern "C" int mhd_get_faces(mhd_ctx *c, int case_id, int block, int *out)
 {
     if (!c || !out || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
+    if (c->cases[case_id]->state == 3)
+        return fail(c, MHD_ERR_STATE, "case not ended (mhd_case_end)");
     const CaseRec &r = *c->cases[case_id];
     if (block < 0 || block >= r.nblocks)
         return MHD_ERR_ARG;
@@ -1710,6 +1979,8 @@ extern "C" int mhd_get_nodes(mhd_ctx *c, int case_id, int block, int comp, doubl
 {
     if (!c || !out || case_id < 0 || case_id >= (int)c->cases.size() || comp < 0 || comp > 2)
         return MHD_ERR_ARG;
+    if (c->cases[case_id]->state == 3)
+        return fail(c, MHD_ERR_STATE, "case not ended (mhd_case_end)");
     const CaseRec &r = *c->cases[case_id];
     if (block < 0 || block >= r.nblocks)
         return MHD_ERR_ARG;
@@ -1725,6 +1996,8 @@ extern "C" int mhd_get_constants(mhd_ctx *c, int case_id, double *o)
 {
     if (!c || !o || case_id < 0 || case_id >= (int)c->cases.size())
         return MHD_ERR_ARG;
+    if (c->cases[case_id]->state == 3)
+        return fail(c, MHD_ERR_STATE, "case not ended (mhd_case_end)");
     const MhdCaseConst &k = c->cases[case_id]->cs.cst;
     const double v[22] = {k.gamma,    k.cfl,         k.inver_MA2, k.momentum_hall_coeff, k.M_H,      k.M_ref,
                           k.ne_floor, k.state_coeff, k.qinf[0],   k.qinf[1],             k.qinf[2],  k.qinf[3],
@@ -1796,6 +2069,8 @@ extern "C" int mhd_autotune(mhd_ctx *c)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "context not finalized");
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a staged step is open: mhd_apply_update first");
     HIPCHK(c, hipSetDevice(c->opt.device));
     int rc = autotune(c);
     if (rc)
@@ -1830,6 +2105,8 @@ extern "C" int mhd_refresh(mhd_ctx *c)
         return MHD_ERR_ARG;
     if (!c->finalized)
         return fail(c, MHD_ERR_STATE, "context not finalized");
+    if (c->fluid_pending)
+        return fail(c, MHD_ERR_STATE, "a staged step is open: mhd_apply_update first");
     HIPCHK(c, hipSetDevice(c->opt.device));
     run_ops(c, SY_U_BC0, c->ucur);
     run_ops(c, SY_U_BC1, c->ucur);

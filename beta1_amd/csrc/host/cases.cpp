@@ -234,19 +234,19 @@ struct LunarRef
 CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, double gamma, double cfl,
                           const WindParams &w)
 {
-    char key[128];
     return make_case_lunar_on(make_cubed_sphere_shell(nang, nrad, r_in, r_out, g, BC_SOLID, BC_FARFIELD), gamma, cfl,
                               w);
 }
 
 CaseSetup make_case_lunar_on(Grid grid, double gamma, double cfl, const WindParams &w)
 {
-    CaseSetup cs;
-    cs.grid = std::move(grid);
-    finalize_grid(cs);
+    return make_case_with(std::move(grid), lunar_constants(gamma, cfl, w));
+}
 
+MhdCaseConst lunar_constants(double gamma, double cfl, const WindParams &w)
+{
     const LunarRef R;
-    MhdCaseConst &c = cs.cst;
+    MhdCaseConst c{};
     const double rho_ref = R.M * R.n / R.NA;
     const double M_H = 0.001 * w.m_amu;
     c.gamma = gamma;
@@ -277,7 +277,16 @@ CaseSetup make_case_lunar_on(Grid grid, double gamma, double cfl, const WindPara
     c.u_inf[2] = w0;
     for (int a = 0; a < 3; ++a)
         c.B_imf[a] = w.B_nT[a] * 1e-9 / R.B;
+    return c;
+}
 
+CaseSetup make_case_with(Grid grid, const MhdCaseConst &cst)
+{
+    CaseSetup cs;
+    cs.grid = std::move(grid);
+    finalize_grid(cs);
+    cs.cst = cst;
+    const MhdCaseConst &c = cs.cst;
     build_ops(cs);
 
     // U = q_inf everywhere, B_face = 0, Badd = B_imf . S_face

@@ -235,6 +235,8 @@ extern "C" int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *
 {
     if (!h)
         return MHD_ERR_ARG;
+    if (h->cs.blk.empty())
+        return MHD_ERR_STATE; // a case begun with mhd_host_case_begin and not yet ended
     const MhdBlockDev &d = h->cs.blk[0].d;
     if (nblocks)
         *nblocks = (int)h->cs.blk.size();
@@ -252,7 +254,7 @@ extern "C" int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *
 extern "C" int mhd_host_field_count(mhd_host_case *h, int field, size_t *n)
 {
     int region, slot, loc, nc;
-    if (!h || !n || !field_slot(field, 0, region, slot, loc, nc))
+    if (!h || !n || h->cs.blk.empty() || !field_slot(field, 0, region, slot, loc, nc))
         return MHD_ERR_ARG;
     if (region == 0 && slot + nc > st_nslot(h->cs.blk[0].d.model))
         return MHD_ERR_ARG;

@@ -1,16 +1,20 @@
 // A case built on the CPU and held behind the opaque mhd_host_case handle of
 // the C-ABI; the device context adopts its contents without rebuilding them.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../../include/beta1_mhd.h"
 #include "setup.h"
 
+struct mhd_case_draft; // blocks and topology given so far (mhd_host_case_begin .. _end)
+
 struct mhd_host_case
 {
     mpcns::CaseSetup cs;
     std::string err;
+    std::shared_ptr<mhd_case_draft> draft; // set between mhd_host_case_begin and mhd_host_case_end
 };
 
 namespace mpcns

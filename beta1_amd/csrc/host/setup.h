@@ -226,6 +226,12 @@ CaseSetup make_case_lunar(int nang, int nrad, double r_in, double r_out, int g, 
 // Lunar physics and initial state on a given grid (ghost nodes are filled
 // here).
 CaseSetup make_case_lunar_on(Grid grid, double gamma, double cfl, const WindParams &w);
+// The single-fluid (Lunar) constants of a solar wind (calc_physical_constant
+// and BuildLunarBackgroundState), and the Lunar set-up on a given grid with
+// given constants: geometry, weights, op lists and the initial state
+// (U = q_inf, B_face = 0, Badd = B_imf . S).
+MhdCaseConst lunar_constants(double gamma, double cfl, const WindParams &w);
+CaseSetup make_case_with(Grid grid, const MhdCaseConst &cst);
 
 // Content key of a case's geometry: block count, extents, ghost depth and a
 // 64-bit hash of every stored geometry word.  Cases share one device copy of

@@ -148,6 +148,36 @@ class HostCase:
             lib().mhd_host_case_destroy(self.h)
             self.h = None
 
+    @classmethod
+    def from_parts(cls, consts, blocks, ghost=2):
+        """A host case from a driver's own arrays (mhd_host_case_begin ..
+        mhd_host_case_end): `consts` as constants() reports them, `blocks` as
+        case_parts() returns them."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        k = np.ascontiguousarray(consts, dtype=np.float64)
+        _check(lib().mhd_host_case_begin(len(blocks), ghost, _dptr(k), C.byref(self.h)), what="mhd_host_case_begin")
+
+        def err(what):
+            m = lib().mhd_host_last_error(self.h)
+            return f"{what}: {m.decode() if m else ''}"
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks, ghost)):
+            rc = lib().mhd_host_add_block(self.h, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
+                                          xyz[0].size, ptrs)
+            if rc:
+                raise MhdError(err(f"mhd_host_add_block({ib})"), rc)
+            rc = lib().mhd_host_set_topology(self.h, ib, faces)
+            if rc:
+                raise MhdError(err(f"mhd_host_set_topology({ib})"), rc)
+        rc = lib().mhd_host_case_end(self.h)
+        if rc:
+            raise MhdError(err("mhd_host_case_end"), rc)
+        self.desc = None
+        nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
+        lib().mhd_host_case_info(self.h, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g))
+        self.nblocks, self.n, self.g = nb.value, (ni.value, nj.value, nk.value), g.value
+        return self
+
     def field(self, name, block):
         a = np.empty(shape(name, self.n, self.g), dtype=np.float64)
         _check(lib().mhd_host_download(self.h, block, FIELD_ID[name], _dptr(a), a.size), what=f"download {name}")
@@ -207,6 +237,46 @@ class HostCase:
         return n.value
 
 
+def _parts_args(blocks, ghost):
+    """ctypes arguments of mhd_(host_)add_block / set_topology for each block
+    of `blocks`: dicts with "n" (ni, nj, nk), "nodes" (x, y, z over the
+    interior nodes, shape (ni+1, nj+1, nk+1)), "faces" (6 x 14 ints) and an
+    optional "geometry" {field name: AoS array}."""
+    geo_names = ["Jac", "JDxi", "JDet", "JDze", "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta",
+                 "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
+    out = []
+    for blk in blocks:
+        ni, nj, nk = blk["n"]
+        xyz = [np.ascontiguousarray(a, dtype=np.float64) for a in blk["nodes"]]
+        for a in xyz:
+            if a.shape != (ni + 1, nj + 1, nk + 1):
+                raise ValueError("nodes must be the (ni+1, nj+1, nk+1) interior nodes")
+        geo = blk.get("geometry") or {}
+        keep = [np.ascontiguousarray(geo[n], dtype=np.float64) if n in geo else None for n in geo_names]
+        ptrs = (C.POINTER(C.c_double) * 12)(*[_dptr(a) if a is not None else None for a in keep])
+        faces = (C.c_int * 84)(*[int(v) for v in np.asarray(blk["faces"]).ravel()])
+        out.append((ni, nj, nk, xyz, keep, ptrs, faces))
+    return out
+
+
+def case_parts(hc, geometry=True):
+    """The arrays a driver would hand over for the grid of host case `hc`:
+    constants, and per block its extent, interior nodes, faces and
+    (optionally) the geometry fields."""
+    gn = hc.g + 1
+    names = ["Jac", "JDxi", "JDet", "JDze", "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta",
+             "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
+    blocks = []
+    for b in range(hc.nblocks):
+        n = hc.n
+        nodes = [a[gn:gn + n[0] + 1, gn:gn + n[1] + 1, gn:gn + n[2] + 1] for a in hc.nodes(b)]
+        blk = {"n": n, "nodes": nodes, "faces": hc.faces(b)}
+        if geometry:
+            blk["geometry"] = {nm: hc.field(nm, b) for nm in names}
+        blocks.append(blk)
+    return hc.constants(), blocks
+
+
 class MhdContext:
     """Device context holding one or more cases (all advanced together)."""
 
@@ -244,6 +314,31 @@ class MhdContext:
 
     def finalize(self):
         _check(lib().mhd_finalize(self.ctx), self.ctx, "mhd_finalize")
+
+    def add_parts(self, consts, blocks):
+        """A case from a driver's own arrays through mhd_case_begin,
+        mhd_add_block, mhd_set_topology and mhd_case_end."""
+        cid = C.c_int()
+        k = np.ascontiguousarray(consts, dtype=np.float64)
+        _check(lib().mhd_case_begin(self.ctx, len(blocks), _dptr(k), C.byref(cid)), self.ctx, "mhd_case_begin")
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks, self.ghost)):
+            _check(lib().mhd_add_block(self.ctx, cid, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
+                                       xyz[0].size, ptrs), self.ctx, f"mhd_add_block({ib})")
+            _check(lib().mhd_set_topology(self.ctx, cid, ib, faces), self.ctx, f"mhd_set_topology({ib})")
+        _check(lib().mhd_case_end(self.ctx, cid), self.ctx, "mhd_case_end")
+        self.cases.append(None)
+        return cid.value
+
+    def stage(self, name):
+        """One stage of the step: "dt", "fluid", "face_emf", "edge_emf",
+        "curl", "update", "derived", or "sync:<group>"."""
+        if name.startswith("sync:"):
+            _check(lib().mhd_sync(self.ctx, name[5:].encode()), self.ctx, name)
+            return
+        f = {"dt": lib().mhd_step_dt, "fluid": lib().mhd_fluid_rhs, "face_emf": lib().mhd_face_emf,
+             "edge_emf": lib().mhd_edge_emf, "curl": lib().mhd_curl_update, "update": lib().mhd_apply_update,
+             "derived": lib().mhd_derived}[name]
+        _check(f(self.ctx), self.ctx, name)
 
     def try_add(self, desc):
         """Admit a case into the running context; None when the HBM budget is full."""

@@ -151,6 +151,46 @@ int mhd_adopt_host_case(mhd_ctx *ctx, mhd_host_case *h, int *case_id);
    Whole-face patches, Farfield and Solid_Surface conditions.  The grid
    fields of d (nang, nrad, r_in, r_out) are ignored. */
 int mhd_add_case_from_dir(mhd_ctx *ctx, const char *dir, const mhd_wind_desc *d, int *case_id);
+/* A case from the caller's own grid, topology and geometry: what a reference
+   driver holds after Grid::Grid_Preprocess, TOPO::build_topology and the
+   Field constructor (Z4_Lunar/main.cpp:46-123), handed over block by block
+   instead of a generated grid.  Single-fluid (Lunar) physics.
+   mhd_case_begin replaces the Field / LunarSolver constructor's set-up for
+   nblocks blocks (src/3_field/2_MPCNS_Field.cpp, Z4_Lunar/src/LunarSolver.cpp:
+   12-235) with the case constants consts22 in the order mhd_get_constants
+   reports them (gamma, cfl, inver_MA2, momentum_hall_coeff, M_H, M_ref,
+   ne_floor, state_coeff, qinf[5], B_imf[3], u_inf[3], dt_floor, 2 unused;
+   calc_physical_constant, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:3-45).
+   mhd_add_block gives block ib (Block, include/1_grid/1_MPCNS_Grid.h:34-125):
+   ni x nj x nk cells, the (ni+1)(nj+1)(nk+1) interior node coordinates
+   x, y, z ([i][j][k], k fastest), and optionally the geometry the caller's
+   Field holds (Field::build_geometry, src/3_field/4_MPCNS_Geometry.cpp:4;
+   SetupHallFaceScratch_, Z4_Lunar/src/LunarSolver.cpp:243-930): geometry[q]
+   for q = 0..11 is the AoS FieldBlock of MHD_F_JAC + q (Jac, JDxi, JDet,
+   JDze, beta_xi/eta/zeta, alpha_xi/eta/zeta, Bcell_from_Bface_w,
+   Jcell_from_Jedge_w) with the context's ghost depth, or NULL.
+   mhd_set_topology gives its six faces (-i,+i,-j,+j,-k,+k) as 14 ints each,
+   the layout of mhd_get_faces: {kind (0 physical, 1 interface), bc (1
+   Farfield, 2 Solid_Surface), nb_block, nb_face, periodic, perm[3],
+   sign[3], offset[3]} with the reference's IndexTransform
+   nb[perm[a]] = sign[a]*local[a] + offset[a]
+   (include/2_topology/2_MPCNS_Topology.h:24-151).
+   mhd_case_end checks that every interface links back and maps each face
+   node onto a node at the same position (periodic links excepted), fills
+   the ghost nodes (mirror at physical faces, copy across interfaces), builds
+   the geometry, weights, halo / boundary ops and the Lunar initial state
+   (U = qinf, B_face = 0, Badd = B_imf . S; Z4_Lunar/include/2_Initial.h:
+   16-99), and compares every geometry array the caller gave with the one its
+   nodes give (rel 1e-9 of the array's largest value; MHD_ERR_ARG naming the
+   array and cell otherwise).  The library's own values are kept.  Upload the
+   driver's state with mhd_upload and call mhd_refresh.  After mhd_finalize,
+   mhd_case_end admits the case like mhd_add_case_*. */
+int mhd_case_begin(mhd_ctx *ctx, int nblocks, const double *consts22, int *case_id);
+int mhd_add_block(mhd_ctx *ctx, int case_id, int ib, int ni, int nj, int nk, const double *x, const double *y,
+                  const double *z, size_t nnode, const double *const *geometry);
+int mhd_set_topology(mhd_ctx *ctx, int case_id, int ib, const int *faces14x6);
+int mhd_case_end(mhd_ctx *ctx, int case_id);
+
 /* Upload everything added so far and compute the derived fields
    (UpdateDerivedFields_, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:143).
    After mhd_finalize, mhd_add_case_* admits a case into the running context
@@ -189,6 +229,44 @@ int mhd_diagnostics_noalloc(mhd_ctx *ctx, const int *case_ids, int n, double *ou
 int mhd_step(mhd_ctx *ctx, int nsteps);
 /* Same as mhd_step through a captured HIP graph of two steps. */
 int mhd_step_graph(mhd_ctx *ctx, int nsteps);
+/* The stages of mhd_step one at a time, over every active case, in the order
+   of LunarSolver::StepOnce (Z4_Lunar/src/LunarSolver_Driver.cpp:21-92):
+     mhd_step_dt        Compute_Timestep (LunarSolver_Timestep.cpp:8-140): each
+                        case's dt from the min the last derived pass took;
+                        MHD_ERR_DT as mhd_step
+     mhd_fluid_rhs      Scheme_U_ + AddSourceToRHS_Fluid
+                        (LunarSolver_FluidRHS.cpp:3-201,528-568) and the U part of
+                        ApplyUpdate_Euler_: U + dt RHS into the context's second U
+                        buffer (MHD_F_U_H still reads the current one)
+     mhd_face_emf       AssembleOneDirectionEMF_, three directions
+                        (LunarSolver_MagExplicit_Ideal.cpp:348-496)
+     mhd_sync "Eface"   LunarBoundary::Sync (Z4_Lunar/src/boundary/Boundary_Sync_.cpp:4-65)
+     mhd_edge_emf       AssembleEdgeEMF_FromFaceE_Ideal_ (:53-287) and the
+                        two-fluid EMF terms (MercurySolver_MagneticRHS.cpp:30-33)
+     mhd_sync "Eedge"
+     mhd_curl_update    CurlEdgeToFace(-1) (Z4_Lunar/src/operators/CTOperators.cpp:6-56)
+                        and the B part of ApplyUpdate_Euler_
+     mhd_apply_update   ResistiveDiffusionSubcycles_ (two-fluid cases), then the
+                        updated U becomes current; ends the step (step count)
+     mhd_sync "Ucell", "Bface"
+     mhd_derived        UpdateDerivedFields_ (LunarSolver_Tools.cpp:143-161) and
+                        the min of the next dt
+   The sequence gives mhd_step's result bit for bit.  mhd_sync takes the
+   reference's group names; "Eface", "Eedge", "Ucell", "Bface", "B_cell" and
+   "J_cell" run the device's op lists (the last two are empty unless a
+   two-fluid case with the Hall EMF is resident), "Jedge", "Uplus", "Ehall"
+   and "Badd" are accepted and do nothing (the device forms U_plus from U
+   where it is used and never reads those ghosts).  Between mhd_fluid_rhs and
+   mhd_apply_update a step is open: mhd_step, admission, retirement, refresh
+   and tuning return MHD_ERR_STATE. */
+int mhd_step_dt(mhd_ctx *ctx);
+int mhd_fluid_rhs(mhd_ctx *ctx);
+int mhd_face_emf(mhd_ctx *ctx);
+int mhd_edge_emf(mhd_ctx *ctx);
+int mhd_curl_update(mhd_ctx *ctx);
+int mhd_apply_update(mhd_ctx *ctx);
+int mhd_sync(mhd_ctx *ctx, const char *group);
+int mhd_derived(mhd_ctx *ctx);
 /* Compute_Timestep (Z4_Lunar/src/physics/LunarSolver_Timestep.cpp:8-140)
    on the current state, without stepping. */
 int mhd_compute_dt(mhd_ctx *ctx, int case_id, double *dt);
@@ -266,6 +344,15 @@ int mhd_host_case_mercury(const mhd_wind_desc *d, const mhd_dipole_desc *dp, int
    (Grid::Read_* above) and by mhd_host_case_from_dir / mhd_add_case_from_dir. */
 int mhd_host_case_write_dir(mhd_host_case *h, const char *dir);
 int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, int ghost, mhd_host_case **out);
+/* mhd_case_begin .. mhd_case_end on the host (no device): the host case can
+   be adopted by a context later (mhd_adopt_host_case) or kept on the host;
+   mhd_host_last_error gives the reason of a failed call. */
+int mhd_host_case_begin(int nblocks, int ghost, const double *consts22, mhd_host_case **out);
+int mhd_host_add_block(mhd_host_case *h, int ib, int ni, int nj, int nk, const double *x, const double *y,
+                       const double *z, size_t nnode, const double *const *geometry);
+int mhd_host_set_topology(mhd_host_case *h, int ib, const int *faces14x6);
+int mhd_host_case_end(mhd_host_case *h);
+const char *mhd_host_last_error(mhd_host_case *h);
 /* MPCNSRST restart of the host arrays (read also re-syncs U and B ghosts). */
 int mhd_host_write_restart(mhd_host_case *h, const char *path, long long step, double time);
 int mhd_host_read_restart(mhd_host_case *h, const char *path, long long *step, double *time);

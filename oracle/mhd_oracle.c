@@ -1832,6 +1832,65 @@ int or_step(or_ctx *c, double *dt_used)
     return 0;
 }
 
+/* One stage of or_step on the current arrays (tests), in or_step's order:
+ * "dt" (Compute_Timestep into the step's dt), "fluid" (RHS zeroed, Scheme_U_,
+ * AddSourceToRHS_Fluid), "face_emf" (E zeroed, the three face-EMF
+ * directions), "edge_emf" (UCT edge EMF and the two-fluid EMF terms), "curl"
+ * (CurlEdgeToFace(-1) into RHS_B), "update" (ApplyUpdate_Euler_ with the
+ * step's dt and the resistive subcycles), "derived" (UpdateDerivedFields_);
+ * syncs through or_sync.  Returns 0, -1 for an unknown stage, or the dt
+ * error of "dt". */
+int or_stage(or_ctx *c, const char *stage)
+{
+    if (!strcmp(stage, "dt"))
+    {
+        double dt;
+        const int rc = or_compute_dt(c, &dt);
+        c->dt = dt;
+        return rc;
+    }
+    if (!strcmp(stage, "fluid"))
+    {
+        for (int b = 0; b < c->d.nblocks; ++b)
+        {
+            zero_fblk(&c->b[b].f[OF_RHSH]);
+            zero_fblk(&c->b[b].f[OF_RHSNA]);
+            for (int a = 0; a < 3; ++a) zero_fblk(&c->b[b].f[OF_RHSBXI + a]);
+        }
+        scheme_u(c);
+        add_source(c);
+        return 0;
+    }
+    if (!strcmp(stage, "face_emf"))
+    {
+        for (int b = 0; b < c->d.nblocks; ++b)
+            for (int a = 0; a < 3; ++a) zero_fblk(&c->b[b].f[OF_EXI + a]);
+        face_emf(c);
+        return 0;
+    }
+    if (!strcmp(stage, "edge_emf"))
+    {
+        edge_emf(c);
+        if (c->d.model == 1 && c->d.ambipolar) add_ambipolar_emf(c);
+        if (c->d.model == 1 && c->d.local_arti) add_local_arti_emf(c);
+        if (c->d.model == 1 && c->d.hall) add_hall_emf(c);
+        return 0;
+    }
+    if (!strcmp(stage, "curl"))
+    {
+        curl_edge_to_face(c, -1.0);
+        return 0;
+    }
+    if (!strcmp(stage, "update"))
+    {
+        apply_update(c, c->dt);
+        if (c->d.model == 1 && c->d.resist) resistive_subcycles(c, c->dt);
+        return 0;
+    }
+    if (!strcmp(stage, "derived")) return or_update_derived(c);
+    return -1;
+}
+
 /* PrintMinMaxDiagnostics_ (Z4_Lunar/src/tools/LunarSolver_PrintDiag.cpp:7-60,
  * Z5_Mercury/src/tools/MercurySolver_PrintDiag.cpp:7-92) */
 void or_diagnostics(or_ctx *c, double *out)

@@ -93,6 +93,9 @@ void or_set_threads(or_ctx *c, int n);
 int or_sync(or_ctx *c, const char *group);
 /* the interior-resistivity EMF of the current J_cell into Eres (tests) */
 int or_resistive_emf(or_ctx *c);
+/* one stage of or_step ("dt", "fluid", "face_emf", "edge_emf", "curl",
+   "update", "derived"; syncs through or_sync) on the current arrays (tests) */
+int or_stage(or_ctx *c, const char *stage);
 
 #ifdef __cplusplus
 }

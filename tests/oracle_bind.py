@@ -62,6 +62,7 @@ def olib():
         L.or_sync.argtypes = [C.c_void_p, C.c_char_p]
         L.or_resistive_emf.argtypes = [C.c_void_p]
         L.or_diagnostics.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.or_stage.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -168,6 +169,15 @@ class Oracle:
         out = np.zeros(17)
         olib().or_diagnostics(self.c, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+    def stage(self, name):
+        """One stage of the oracle's step (or a sync group, "sync:<group>")."""
+        if name.startswith("sync:"):
+            return olib().or_sync(self.c, name[5:].encode())
+        rc = olib().or_stage(self.c, name.encode())
+        if rc == -1:
+            raise ValueError(name)
+        return rc
 
     def resistive_emf(self):
         return olib().or_resistive_emf(self.c)

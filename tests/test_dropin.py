@@ -1,0 +1,149 @@
+"""The drop-in set-up path (CPU only): a case assembled from a driver's own
+arrays -- interior nodes, faces in the IndexTransform convention, optional
+geometry fields -- through mhd_host_case_begin / add_block / set_topology /
+case_end.  Given the arrays of a generated case it reproduces that case bit
+for bit (nodes incl. the ghosts it fills, geometry, weight tables, op lists,
+initial state); inconsistent input is refused with a reason."""
+import numpy as np
+import pytest
+
+from beta1_amd import HostCase, OTDesc, WindDesc, case_parts
+from beta1_amd.capi import MhdError
+from oracle_bind import Oracle
+
+G = 2
+FIELDS = ["U_H", "B_xi", "B_eta", "B_zeta", "Badd_xi", "Badd_eta", "Badd_zeta", "Jac", "JDxi", "JDet", "JDze",
+          "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta", "alpha_zeta", "Bcell_from_Bface_w",
+          "Jcell_from_Jedge_w"]
+
+
+def assert_same_case(a, b, fields=FIELDS):
+    assert a.nblocks == b.nblocks and a.n == b.n and a.g == b.g
+    for blk in range(a.nblocks):
+        assert np.array_equal(a.faces(blk), b.faces(blk)), blk
+        for p, q in zip(a.nodes(blk), b.nodes(blk)):
+            assert np.array_equal(p, q), blk
+        for f in fields:
+            assert np.array_equal(a.field(f, blk), b.field(f, blk)), (blk, f)
+    for sy in range(14):
+        assert a.op_count(sy) == b.op_count(sy), sy
+
+
+@pytest.mark.parametrize("geometry", [True, False])
+def test_parts_of_a_shell_reproduce_it_bitwise(geometry):
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    k, blocks = case_parts(hc, geometry=geometry)
+    assert_same_case(hc, HostCase.from_parts(k, blocks, ghost=G))
+
+
+def test_parts_of_a_periodic_box_reproduce_its_grid():
+    """Self-links with a negative block id in the reference are periodic
+    links here; the ghost nodes stay mirrored.  The state is the Lunar
+    initial state of the constants; the box's own state uploads back."""
+    hc = HostCase(OTDesc(n=8), ghost=G)
+    k, blocks = case_parts(hc)
+    back = HostCase.from_parts(k, blocks, ghost=G)
+    assert_same_case(hc, back, fields=[f for f in FIELDS if f not in ("U_H", "B_xi", "B_eta", "B_zeta")])
+    for f in ("U_H", "B_xi", "B_eta", "B_zeta"):
+        back.set_field(f, 0, hc.field(f, 0))
+        assert np.array_equal(back.field(f, 0), hc.field(f, 0))
+
+
+def test_oracle_of_parts_steps_like_oracle_of_generated_case():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    back = HostCase.from_parts(*case_parts(hc), ghost=G)
+    a, b = Oracle(hc), Oracle(back)
+    assert a.step(3) == b.step(3)
+    for blk in range(6):
+        for f in ("U_H", "B_xi", "B_eta", "B_zeta", "J_cell"):
+            assert np.array_equal(a.view(f, blk), b.view(f, blk)), (blk, f)
+
+
+def spoiled(change):
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    k, blocks = case_parts(hc)
+    change(k, blocks)
+    return k, blocks
+
+
+def _link_not_back(k, blocks):
+    f = blocks[0]["faces"]
+    iface = [q for q in range(6) if f[q][0] == 1][0]
+    f[iface][3] = (f[iface][3] + 1) % 6
+
+
+def _offset(k, blocks):
+    f = blocks[1]["faces"]
+    iface = [q for q in range(6) if f[q][0] == 1][0]
+    f[iface][11] += 1
+
+
+def _metric(k, blocks):
+    blocks[2]["geometry"]["JDxi"] = blocks[2]["geometry"]["JDxi"] * (1 + 1e-6)
+
+
+def _table(k, blocks):
+    blocks[3]["geometry"]["Jcell_from_Jedge_w"] = blocks[3]["geometry"]["Jcell_from_Jedge_w"] * (1 + 1e-6)
+
+
+def _bc(k, blocks):
+    blocks[0]["faces"][4][1] = 7
+
+
+def _perm(k, blocks):
+    f = blocks[0]["faces"]
+    iface = [q for q in range(6) if f[q][0] == 1][0]
+    f[iface][5:8] = [0, 0, 1]
+
+
+def _node(k, blocks):
+    blocks[4]["nodes"][0][0, 0, 0] += 1e-3  # a corner node no longer matches its neighbours
+
+
+def _gamma(k, blocks):
+    k[0] = 1.0
+
+
+@pytest.mark.parametrize("change,reason", [
+    (_link_not_back, "does not link back"), (_offset, "another position|outside the neighbour"),
+    (_metric, "JDxi"), (_table, "Jcell_from_Jedge_w"), (_bc, "bc 1"), (_perm, "permutation"),
+    (_node, "another position"), (_gamma, "mhd_host_case_begin")])
+def test_inconsistent_parts_are_refused_with_a_reason(change, reason):
+    import re
+    k, blocks = spoiled(change)
+    with pytest.raises(MhdError) as e:
+        HostCase.from_parts(k, blocks, ghost=G)
+    assert re.search(reason, str(e.value)), str(e.value)
+
+
+def test_missing_block_or_wrong_node_count_is_refused():
+    hc = HostCase(WindDesc(nang=6, nrad=8), ghost=G)
+    k, blocks = case_parts(hc)
+    with pytest.raises(MhdError, match="nodes or topology missing"):
+        _five_of_six(k, blocks)
+    bad = dict(blocks[0])
+    bad["nodes"] = [a[:-1] for a in blocks[0]["nodes"]]
+    with pytest.raises(ValueError):
+        HostCase.from_parts(k, [bad] + blocks[1:], ghost=G)
+
+
+def _five_of_six(k, blocks):
+    """Begin six blocks, give five, end: the sixth is missing."""
+    import ctypes as C
+
+    from beta1_amd.capi import lib
+    from beta1_amd.solver import _dptr, _parts_args
+
+    h = C.c_void_p()
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    assert lib().mhd_host_case_begin(6, G, _dptr(kk), C.byref(h)) == 0
+    try:
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks[:5], G)):
+            assert lib().mhd_host_add_block(h, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
+                                            xyz[0].size, ptrs) == 0
+            assert lib().mhd_host_set_topology(h, ib, faces) == 0
+        rc = lib().mhd_host_case_end(h)
+        if rc:
+            raise MhdError(lib().mhd_host_last_error(h).decode(), rc)
+    finally:
+        lib().mhd_host_case_destroy(h)

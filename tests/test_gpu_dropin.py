@@ -1,0 +1,173 @@
+"""The drop-in boundary on the GPU.
+
+* A case assembled on the device from a driver's own arrays (mhd_case_begin,
+  mhd_add_block, mhd_set_topology, mhd_case_end) steps bit for bit like the
+  generated case (mhd_add_case_lunar), before and after mhd_finalize.
+* The per-stage entry points in StepOnce's order give mhd_step's result bit
+  for bit, and at every stage boundary the device arrays equal the oracle's
+  after the same stage (FP64: relative L-inf <= 1e-12; they are bitwise).
+"""
+import numpy as np
+import pytest
+
+from beta1_amd import HostCase, MercuryDesc, MhdContext, OTDesc, WindDesc, case_parts
+from beta1_amd.capi import MhdError
+from beta1_amd.fields import interior
+from oracle_bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U_H", "B_xi", "B_eta", "B_zeta")
+TOL = 1e-12
+
+
+def run(ctx, cid, steps):
+    ctx.step(steps)
+    nb = ctx.info(cid)[0]
+    return {(f, b): ctx.field(cid, f, b) for f in FIELDS for b in range(nb)}
+
+
+@pytest.mark.parametrize("live", [False, True])
+def test_device_case_from_driver_arrays_steps_like_generated(live):
+    desc = WindDesc(nang=8, nrad=16, v_kms=450.0)
+    ref = MhdContext()
+    rid = ref.add(desc)
+    ref.finalize()
+    want = run(ref, rid, 6)
+    k, blocks = case_parts(HostCase(desc))
+    ctx = MhdContext()
+    if live:
+        ctx.finalize()
+    cid = ctx.add_parts(k, blocks)
+    if not live:
+        ctx.finalize()
+    got = run(ctx, cid, 6)
+    for key, a in want.items():
+        assert np.array_equal(got[key], a), key
+    assert ctx.last_dt(cid) == ref.last_dt(rid)
+
+
+def test_device_drop_in_refuses_bad_arrays_and_unfinished_cases():
+    k, blocks = case_parts(HostCase(WindDesc(nang=8, nrad=16)))
+    blocks[1]["geometry"]["alpha_eta"] = blocks[1]["geometry"]["alpha_eta"] * 1.01
+    ctx = MhdContext()
+    with pytest.raises(MhdError, match="alpha_eta"):
+        ctx.add_parts(k, blocks)
+    ctx.finalize()
+    ctx.step(1)  # nothing resident: the failed draft is not a case
+
+
+STAGES = ["dt", "fluid", "face_emf", "sync:Eface", "edge_emf", "sync:Eedge", "curl", "update", "sync:Ucell",
+          "sync:Bface", "derived"]
+
+
+@pytest.mark.parametrize("desc", [WindDesc(nang=8, nrad=16), MercuryDesc(nang=8, nrad=16), OTDesc(n=12)])
+def test_staged_step_equals_mhd_step(desc):
+    a, b = MhdContext(), MhdContext()
+    ia, ib = a.add(desc), b.add(desc)
+    a.finalize()
+    b.finalize()
+    a.step(4)
+    for _ in range(3):
+        for s in STAGES:
+            b.stage(s)
+    b.step(1)  # and back to whole steps
+    assert a.case_steps(ia) == b.case_steps(ib) == 4
+    nb = a.info(ia)[0]
+    for f in FIELDS + ("B_cell", "J_cell"):
+        for blk in range(nb):
+            assert np.array_equal(a.field(ia, f, blk), b.field(ib, f, blk)), (f, blk)
+    assert a.last_dt(ia) == b.last_dt(ib)
+
+
+def test_open_step_blocks_whole_steps():
+    ctx = MhdContext()
+    ctx.add(WindDesc(nang=8, nrad=16))
+    ctx.finalize()
+    ctx.stage("dt")
+    ctx.stage("fluid")
+    for call in (lambda: ctx.step(1), lambda: ctx.refresh(), lambda: ctx.stage("fluid"), lambda: ctx.stage("derived")):
+        with pytest.raises(MhdError):
+            call()
+    for s in STAGES[2:]:
+        ctx.stage(s)
+    ctx.step(2)
+    with pytest.raises(MhdError):
+        ctx.stage("update")  # no open step
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def compare(ctx, cid, orc, names, n, g, sl_of=None, comps=None):
+    worst = 0.0
+    for name in names:
+        for blk in range(orc.nblocks):
+            sl = sl_of(name) if sl_of else interior(name, n, g)
+            a, r = ctx.field(cid, name, blk)[sl], orc.view(name, blk)[sl]
+            if comps is not None:
+                a, r = a[..., comps[name]], r[..., comps[name]]
+            assert np.all(np.isfinite(a)), (name, blk)
+            worst = max(worst, rel(a, r))
+    return worst
+
+
+def tangential(name):
+    ax = {"Eface_xi": 0, "Eface_eta": 1, "Eface_zeta": 2}[name]
+    return [c for c in range(3) if c != ax]
+
+
+@pytest.mark.parametrize("desc", [WindDesc(nang=8, nrad=16), MercuryDesc(nang=8, nrad=16)])
+def test_every_stage_matches_the_oracle(desc):
+    """After each stage the device's arrays equal the oracle's after the same
+    stage: face EMF candidates (the two tangential components the device
+    keeps), edge EMFs before and after the far-field handler, the updated
+    state, its synced ghosts (the single-hop ghosts both fill), B_cell and
+    J_cell, and each step's dt."""
+    hc = HostCase(desc, ghost=2)
+    orc = Oracle(hc)
+    ctx = MhdContext()
+    cid = ctx.add(desc)
+    ctx.finalize()
+    n, g = hc.n, hc.g
+    ef = ["Eface_xi", "Eface_eta", "Eface_zeta"]
+    ee = ["E_xi", "E_eta", "E_zeta"]
+    comps = {nm: tangential(nm) for nm in ef}
+    two = isinstance(desc, MercuryDesc) and desc.two_fluid
+    state = ["U_H", "B_xi", "B_eta", "B_zeta"] + (["U_Na"] if two else [])
+    for step in range(3):
+        dt_o = orc.compute_dt()
+        for s in STAGES:
+            ctx.stage(s)
+            orc.stage(s)
+            if s == "dt":
+                assert ctx.last_dt(cid) == dt_o, step
+            elif s == "face_emf":
+                assert compare(ctx, cid, orc, ef, n, g, comps=comps) <= TOL, (step, s)
+            elif s in ("edge_emf", "sync:Eedge"):
+                assert compare(ctx, cid, orc, ee, n, g) <= TOL, (step, s)
+            elif s == "update":
+                assert compare(ctx, cid, orc, state, n, g) <= TOL, (step, s)
+            elif s == "derived":
+                assert compare(ctx, cid, orc, ["B_cell", "J_cell"], n, g) <= TOL, (step, s)
+            elif s == "sync:Bface":
+                # the synced ghosts: one layer outside each block face
+                for name in state:
+                    assert face_ghosts_rel(ctx, cid, orc, name, n, g) <= TOL, (step, name)
+
+
+def face_ghosts_rel(ctx, cid, orc, name, n, g):
+    """Relative L-inf over the DOFs one layer outside exactly one face of
+    each block (interface and physical ghosts alike)."""
+    from beta1_amd.fields import LOCATION, delta
+    worst = 0.0
+    for blk in range(orc.nblocks):
+        a, r = ctx.field(cid, name, blk), orc.view(name, blk)
+        cnt = [n[q] + 1 - delta(LOCATION[name], q) for q in range(3)]
+        idx = np.meshgrid(*[np.arange(cnt[q] + 2 * g) - g for q in range(3)], indexing="ij")
+        out = sum(((idx[q] == -1) | (idx[q] == cnt[q])).astype(int) for q in range(3))
+        inside = np.all([(idx[q] >= -1) & (idx[q] <= cnt[q]) for q in range(3)], axis=0)
+        m = (out == 1) & inside
+        worst = max(worst, rel(a[m], r[m]))
+    return worst
