@@ -281,6 +281,83 @@ def bench_precision(args):
                    divb_max={k: float(v) for k, v in divb.items()})
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(desc, cells, args):
+    """The FP64 oracle on the same workload on the host cores this process may
+    use (its CPU affinity; OMP_NUM_THREADS when the machine sets a smaller
+    share), OpenMP over (block, i-plane), and on one thread: each timed over
+    whole steps after one untimed step, about 10 s of work each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from beta1_amd import HostCase
+    from oracle_bind import Oracle
+
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = max(1, min(share, int(env))) if env and env.isdigit() else max(1, share)
+    hc = HostCase(desc, ghost=2)
+    rates = {}
+    for nt in (1, threads):
+        orc = Oracle(hc, threads=nt)
+        t0 = time.perf_counter()
+        orc.step(1)                      # untimed: first touch of the arrays
+        per = time.perf_counter() - t0
+        nsteps = max(args.cpu_steps, int(10.0 / max(per, 1e-3)))
+        t0 = time.perf_counter()
+        orc.step(nsteps)
+        cs = time.perf_counter() - t0
+        rates[nt] = (cells * nsteps / cs / 1e6, nsteps, cs)
+        log(f"cpu oracle, {nt} thread(s): {cs / nsteps:.3f} s/step over {nsteps} steps")
+        del orc
+    del hc
+    v, nsteps, cs = rates[threads]
+    return {"value": round(v, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} steps (after 1 untimed) of the same {desc.cells}-cell shell, FP64 oracle with OpenMP "
+                      f"over (block, i-plane) on {threads} threads, {cs:.1f}s",
+            "s_per_MCell_step": round(1.0 / v, 5),
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "process_cpus": share,
+            "single_thread": {"value": round(rates[1][0], 4), "steps": rates[1][1], "seconds": round(rates[1][2], 2),
+                              "s_per_MCell_step": round(1.0 / rates[1][0], 4)}}
+
+
+def extra_configs(args):
+    """The other benchmark configurations, each at its own size and step
+    count, for the default JSON line: C1 (Orszag-Tang 64^3, first order and
+    minmod), C3 (6x128x128x256, 500 steps, FP64 and mixed with the error of
+    mixed against FP64), C4 (the 64-case mixed batch)."""
+    import copy
+    out = {}
+    a = copy.copy(args)
+    a.steps, a.warmup = 100, 5
+    r = bench_ot(a)
+    out["c1_orszag_tang_64"] = {"first_order": {"value": r["value"], "ms_per_step": r["ms_per_step"]},
+                                "minmod": r["minmod"], "divb_max": r["divb_max"], "steps": 100,
+                                "dtype": "f64", "step_hbm": r["step_hbm"]}
+    a = copy.copy(args)
+    a.nang, a.nrad, a.steps = 128, 256, 500
+    r = bench_precision(a)
+    out["c3_precision_6x128x128x256"] = {"mixed_value": r["value"], "fp64_value": r["fp64_value"], "steps": 500,
+                                         "errors_mixed_vs_fp64": r["errors_mixed_vs_fp64"],
+                                         "divb_max": r["divb_max"],
+                                         "note": "the FP64 step equals the FP64 oracle bit for bit at every size the "
+                                                 "tests run, so mixed vs FP64 is mixed vs the oracle"}
+    a = copy.copy(args)
+    a.cases, a.steps, a.warmup, a.precision = 64, 40, 4, "mixed"
+    r = bench_batch(a)
+    out["c4_batch_64_mixed"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "steps": 40,
+                                "cases": r["config"]["cases"], "cells": r["config"]["cells"],
+                                "step_hbm": r["step_hbm"], "dtype": r["dtype"]}
+    return out
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 counter
     summary of this workload (profiles/traffic_<workload>.md), or None."""
@@ -314,6 +391,8 @@ def main():
     ap.add_argument("--nrad", type=int, default=None, help="default 128 (precision: 256)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload only: skip the C1 / C3 / C4 figures added to the JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy five times (counter calibration)")
     ap.add_argument("--workload", default="lunar", choices=["lunar", "mercury", "batch", "serve", "precision", "ot"],
@@ -322,7 +401,7 @@ def main():
                          "context; serve: Poisson request stream with continuous batching; precision: FP64 vs "
                          "mixed on one shell; ot: Orszag-Tang 64^3 periodic box")
     ap.add_argument("--cases", type=int, default=64)
-    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--requests", type=int, default=100)
     ap.add_argument("--req-steps", type=int, default=1000)
     ap.add_argument("--load", type=float, default=0.8, help="offered load of the request stream")
     ap.add_argument("--budget-gb", type=float, default=200.0)
@@ -398,29 +477,7 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from beta1_amd import HostCase
-        from oracle_bind import Oracle
-
-        hc = HostCase(desc, ghost=2)
-        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)))
-        rates = {}
-        for nt, nsteps in ((1, args.cpu_steps), (threads, 4 * args.cpu_steps)):
-            orc = Oracle(hc, threads=nt)
-            orc.step(1)
-            t0 = time.perf_counter()
-            orc.step(nsteps)
-            cs = time.perf_counter() - t0
-            rates[nt] = (cells * nsteps / cs / 1e6, nsteps, cs)
-            print(f"[bench] cpu oracle, {nt} thread(s): {cs / nsteps:.3f} s/step", file=sys.stderr, flush=True)
-            del orc
-        del hc
-        v, nsteps, cs = rates[threads]
-        cpu = {"value": round(v, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{nsteps} steps (after 1 untimed) of the same 6x{args.nang}x{args.nang}x{args.nrad} "
-                         f"shell, FP64 oracle with OpenMP over (block, i-plane) on {threads} threads, {cs:.1f}s",
-               "single_thread": {"value": round(rates[1][0], 4), "steps": rates[1][1],
-                                 "seconds": round(rates[1][2], 2)}}
+        cpu = cpu_baseline(desc, cells, args)
 
     out = {
         "metric": "Mcell-updates/s + %HBM roofline (FP64 vs mixed/batched); p50/p99 solve latency",
@@ -449,6 +506,9 @@ def main():
     }
     if mercury:
         out["metric"] = "Mcell-updates/s (two-fluid Mercury shell)"
+    elif not args.no_extra:
+        del ctx
+        out["configs"] = extra_configs(args)
     print(json.dumps(out), flush=True)
 
 

@@ -167,3 +167,38 @@ def test_ghost_depth_below_two_is_rejected():
     opt = MhdOptions(0, 0, 0, 1, 0.0, 0, 1, 1)
     ctx = C.c_void_p()
     assert lib().mhd_create(C.byref(opt), C.byref(ctx)) == -1
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+def test_served_results_match_the_oracle(precision, tol):
+    """Requests served by continuous batching (Lunar and two-fluid Mercury,
+    mixed sizes, a budget that holds about two at a time) against the FP64
+    oracle run for the same number of steps."""
+    from beta1_amd import MercuryDesc
+    from beta1_amd.fields import interior
+    from beta1_amd.serve import Request, ServingEngine
+    from oracle_bind import Oracle
+
+    descs = [WindDesc(nang=8, nrad=16, v_kms=380.0), MercuryDesc(nang=8, nrad=16, n_cc=30.0),
+             WindDesc(nang=12, nrad=16, n_cc=9.0), MercuryDesc(nang=8, nrad=24, v_kms=450.0)]
+    probe = MhdContext(precision=precision)
+    probe.add(descs[3])
+    probe.finalize()
+    big = probe.memory_bytes()
+    probe.close()
+    reqs = [Request(i, d, steps=4 + 2 * i, arrival=0.01 * i, kind="mercury" if isinstance(d, MercuryDesc) else
+                    "lunar") for i, d in enumerate(descs)]
+    eng = ServingEngine(precision=precision, hbm_budget_gb=2.2 * big / 1e9, chunk=3, workers=2, keep_results=True)
+    out = eng.run(reqs)
+    eng.close()
+    assert out["requests"] == len(reqs) and out["failed"] == 0
+    for r in out["records"]:
+        hc = HostCase(r.desc, ghost=2)
+        orc = Oracle(hc, threads=8)
+        orc.step(r.steps)
+        for f in FIELDS:
+            for b in range(hc.nblocks):
+                sl = interior(f, hc.n, hc.g)
+                a, ref = r.result[(f, b)][sl], orc.view(f, b)[sl]
+                e = np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300)
+                assert e <= tol, (r.rid, f, b, e)
