@@ -71,6 +71,8 @@ SIGNATURES = {
     "mhd_divb_max": (_I, [_P, _I, _PD]),
     "mhd_synchronize": (_I, [_P]),
     "mhd_case_info": (_I, [_P, _I, _PI, _PI, _PI, _PI, _PI]),
+    "mhd_block_info": (_I, [_P, _I, _I, _PI, _PI, _PI]),
+    "mhd_host_block_info": (_I, [_P, _I, _PI, _PI, _PI]),
     "mhd_field_count": (_I, [_P, _I, _I, C.POINTER(_S)]),
     "mhd_download": (_I, [_P, _I, _I, _I, _PD, _S]),
     "mhd_upload": (_I, [_P, _I, _I, _I, _PD, _S]),

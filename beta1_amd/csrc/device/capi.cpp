@@ -1946,6 +1946,20 @@ extern "C" int mhd_case_info(mhd_ctx *c, int case_id, int *nblocks, int *ni, int
     return MHD_OK;
 }
 
+extern "C" int mhd_block_info(mhd_ctx *c, int case_id, int block, int *ni, int *nj, int *nk)
+{
+    if (!c || !ni || !nj || !nk || case_id < 0 || case_id >= (int)c->cases.size())
+        return MHD_ERR_ARG;
+    const CaseRec &r = *c->cases[case_id];
+    if (r.state == 3 || block < 0 || block >= (int)r.cs.blk.size())
+        return fail(c, MHD_ERR_ARG, "no such block (or case not ended)");
+    const MhdBlockDev &d = r.cs.blk[block].d;
+    *ni = d.ni;
+    *nj = d.nj;
+    *nk = d.nk;
+    return MHD_OK;
+}
+
 extern "C" int mhd_get_faces(mhd_ctx *c, int case_id, int block, int *out)
 {
     if (!c || !out || case_id < 0 || case_id >= (int)c->cases.size())

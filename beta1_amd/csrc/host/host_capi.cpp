@@ -251,6 +251,17 @@ extern "C" int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *
     return MHD_OK;
 }
 
+extern "C" int mhd_host_block_info(mhd_host_case *h, int block, int *ni, int *nj, int *nk)
+{
+    if (!h || !ni || !nj || !nk || block < 0 || block >= (int)h->cs.blk.size())
+        return MHD_ERR_ARG;
+    const MhdBlockDev &d = h->cs.blk[block].d;
+    *ni = d.ni;
+    *nj = d.nj;
+    *nk = d.nk;
+    return MHD_OK;
+}
+
 extern "C" int mhd_host_field_count(mhd_host_case *h, int field, size_t *n)
 {
     int region, slot, loc, nc;

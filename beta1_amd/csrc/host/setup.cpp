@@ -586,6 +586,15 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
     const int g = grd.g;
     const double eps = 1e-25;
     blk.resize(grd.blocks.size());
+    // one slot size for every block of the case (the largest box, 512-byte
+    // aligned): a case's blocks sit at multiples of it in one allocation and
+    // its op lists address them so
+    int64_t case_slot = 0;
+    for (const Block &B : grd.blocks)
+    {
+        const int64_t raw = (int64_t)(B.ni + 1 + 2 * g) * (B.nj + 1 + 2 * g) * (B.nk + 1 + 2 * g);
+        case_slot = std::max(case_slot, (raw + 63) / 64 * 64);
+    }
     for (size_t b = 0; b < grd.blocks.size(); ++b)
     {
         const Block &B = grd.blocks[b];
@@ -599,8 +608,7 @@ void build_geometry(const Grid &grd, std::vector<BlockArrays> &blk)
         d.g = g;
         d.sj = B.nj + 1 + 2 * g;
         d.sk = B.nk + 1 + 2 * g;
-        const int64_t raw = (int64_t)(B.ni + 1 + 2 * g) * d.sj * d.sk;
-        d.slot = (raw + 63) / 64 * 64; // 512-byte aligned slots
+        d.slot = case_slot;
         for (int f = 0; f < 6; ++f)
             d.bc[f] = B.face[f].kind == 0 ? B.face[f].bc : BC_NONE;
         A.geo.assign((size_t)G_NSLOT * d.slot, 0.0);

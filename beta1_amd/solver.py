@@ -139,9 +139,16 @@ class HostCase:
         else:
             _check(lib().mhd_host_case_lunar(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_lunar")
         self.desc = desc
+        self._info()
+
+    def _info(self):
         nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
         lib().mhd_host_case_info(self.h, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g))
         self.nblocks, self.n, self.g = nb.value, (ni.value, nj.value, nk.value), g.value
+        self.ns = []  # cells of each block (self.n: block 0)
+        for b in range(self.nblocks):
+            _check(lib().mhd_host_block_info(self.h, b, C.byref(ni), C.byref(nj), C.byref(nk)), what="block_info")
+            self.ns.append((ni.value, nj.value, nk.value))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -173,13 +180,11 @@ class HostCase:
         if rc:
             raise MhdError(err("mhd_host_case_end"), rc)
         self.desc = None
-        nb, ni, nj, nk, g = (C.c_int() for _ in range(5))
-        lib().mhd_host_case_info(self.h, C.byref(nb), C.byref(ni), C.byref(nj), C.byref(nk), C.byref(g))
-        self.nblocks, self.n, self.g = nb.value, (ni.value, nj.value, nk.value), g.value
+        self._info()
         return self
 
     def field(self, name, block):
-        a = np.empty(shape(name, self.n, self.g), dtype=np.float64)
+        a = np.empty(shape(name, self.ns[block], self.g), dtype=np.float64)
         _check(lib().mhd_host_download(self.h, block, FIELD_ID[name], _dptr(a), a.size), what=f"download {name}")
         return a
 
@@ -194,7 +199,7 @@ class HostCase:
 
     def nodes(self, block):
         gn = self.g + 1
-        s = tuple(self.n[a] + 1 + 2 * gn for a in range(3))
+        s = tuple(self.ns[block][a] + 1 + 2 * gn for a in range(3))
         out = []
         for comp in range(3):
             a = np.empty(s, dtype=np.float64)
@@ -268,7 +273,7 @@ def case_parts(hc, geometry=True):
              "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
     blocks = []
     for b in range(hc.nblocks):
-        n = hc.n
+        n = hc.ns[b]
         nodes = [a[gn:gn + n[0] + 1, gn:gn + n[1] + 1, gn:gn + n[2] + 1] for a in hc.nodes(b)]
         blk = {"n": n, "nodes": nodes, "faces": hc.faces(b)}
         if geometry:
@@ -423,9 +428,14 @@ class MhdContext:
                self.ctx, "info")
         return nb.value, (ni.value, nj.value, nk.value), g.value
 
+    def block_n(self, cid, block):
+        ni, nj, nk = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().mhd_block_info(self.ctx, cid, block, C.byref(ni), C.byref(nj), C.byref(nk)), self.ctx,
+               "block_info")
+        return ni.value, nj.value, nk.value
+
     def field(self, cid, name, block):
-        _, n, g = self.info(cid)
-        a = np.empty(shape(name, n, g), dtype=np.float64)
+        a = np.empty(shape(name, self.block_n(cid, block), self.ghost), dtype=np.float64)
         _check(lib().mhd_download(self.ctx, cid, block, FIELD_ID[name], _dptr(a), a.size), self.ctx,
                f"download {name}")
         return a

@@ -297,6 +297,8 @@ int mhd_write_tecplot(mhd_ctx *ctx, int case_id, const char *path, double time);
 /* Introspection and data exchange (FieldBlock::raw_data round trip,
    include/3_field/1_Field_Block.h:47). */
 int mhd_case_info(mhd_ctx *ctx, int case_id, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
+/* cells of one block (blocks of a case may differ; mhd_case_info gives block 0) */
+int mhd_block_info(mhd_ctx *ctx, int case_id, int block, int *ni, int *nj, int *nk);
 int mhd_field_count(mhd_ctx *ctx, int case_id, int field, size_t *n);
 int mhd_download(mhd_ctx *ctx, int case_id, int block, int field, double *aos, size_t n);
 /* A device case keeps the least-squares data of Bcell_from_Bface_w, not the
@@ -360,6 +362,7 @@ int mhd_host_read_restart(mhd_host_case *h, const char *path, long long *step, d
 int mhd_host_write_tecplot(mhd_host_case *h, const char *path, double time);
 int mhd_host_case_destroy(mhd_host_case *h);
 int mhd_host_case_info(mhd_host_case *h, int *nblocks, int *ni, int *nj, int *nk, int *ghost);
+int mhd_host_block_info(mhd_host_case *h, int block, int *ni, int *nj, int *nk);
 int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);
 int mhd_host_download(mhd_host_case *h, int block, int field, double *aos, size_t n);
 int mhd_host_upload(mhd_host_case *h, int block, int field, const double *aos, size_t n);

@@ -113,6 +113,7 @@ class Oracle:
         L = olib()
         self.c = L.or_create(C.byref(d))
         self.n, self.g, self.nblocks = hc.n, hc.g, hc.nblocks
+        self.ns = list(getattr(hc, "ns", [hc.n] * hc.nblocks))
         for b in range(hc.nblocks):
             fc = hc.faces(b)
             faces = (OrFace * 6)()
@@ -121,7 +122,8 @@ class Oracle:
                 faces[f].kind, faces[f].bc, faces[f].nb_block, faces[f].nb_face, faces[f].periodic = map(int, r[:5])
                 for a in range(3):
                     faces[f].perm[a], faces[f].sign[a], faces[f].offset[a] = int(r[5 + a]), int(r[8 + a]), int(r[11 + a])
-            L.or_set_block(self.c, b, hc.n[0], hc.n[1], hc.n[2], faces)
+            n = self.ns[b]
+            L.or_set_block(self.c, b, n[0], n[1], n[2], faces)
             for name in GEOMETRY + STATE + (STATE_TWO_FLUID if two else []):
                 self.view(name, b)[...] = hc.field(name, b)
             for comp, arr in enumerate(hc.nodes(b)):
@@ -142,7 +144,7 @@ class Oracle:
         cnt = C.c_long()
         p = olib().or_field(self.c, b, FIELD_ID[name], C.byref(cnt))
         g0 = 0 if name in ("F_xi", "F_eta", "F_zeta", "RHS_H", "RHS_Na", "RHS_B_xi", "RHS_B_eta", "RHS_B_zeta") else self.g
-        return np.ctypeslib.as_array(p, shape=(cnt.value,)).reshape(shape(name, self.n, g0))
+        return np.ctypeslib.as_array(p, shape=(cnt.value,)).reshape(shape(name, self.ns[b], g0))
 
     def step(self, n=1):
         dts = []

@@ -147,3 +147,49 @@ def _five_of_six(k, blocks):
             raise MhdError(lib().mhd_host_last_error(h).decode(), rc)
     finally:
         lib().mhd_host_case_destroy(h)
+
+
+def two_box_parts(perturb=1e-3, seed=4):
+    """A box [0,1] x [0,0.6] x [0,0.8] in two blocks of unequal extent
+    (4 and 6 cells along x, the interface at x = 0.4), periodic in y and z,
+    far field at both x ends; the Lunar constants of the default wind."""
+    k = HostCase(WindDesc(nang=4, nrad=6), ghost=G).constants()
+    nj, nk = 6, 8
+    blocks = []
+    for ni, x0, x1 in ((4, 0.0, 0.4), (6, 0.4, 1.0)):
+        xs = np.linspace(x0, x1, ni + 1)
+        ys, zs = np.linspace(0.0, 0.6, nj + 1), np.linspace(0.0, 0.8, nk + 1)
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        blocks.append({"n": (ni, nj, nk), "nodes": [X, Y, Z], "faces": np.zeros((6, 14), dtype=np.int64)})
+    ident = [0, 1, 2, 1, 1, 1]
+    for b, blk in enumerate(blocks):
+        ni = blk["n"][0]
+        f = blk["faces"]
+        for face, shift in ((2, (0, nj, 0)), (3, (0, -nj, 0)), (4, (0, 0, nk)), (5, (0, 0, -nk))):
+            f[face] = [1, 0, b, face ^ 1, 1] + ident + list(shift)
+        if b == 0:
+            f[0] = [0, 1, -1, -1, 0] + [0] * 9                      # far field at x = 0
+            f[1] = [1, 0, 1, 0, 0] + ident + [-ni, 0, 0]            # -> block 1, x = 0.4
+        else:
+            f[0] = [1, 0, 0, 1, 0] + ident + [blocks[0]["n"][0], 0, 0]
+            f[1] = [0, 1, -1, -1, 0] + [0] * 9                      # far field at x = 1
+    return k, blocks
+
+
+def test_blocks_of_unequal_extent():
+    """Blocks of different sizes share one slot size in a case; the
+    geometry of each is its own box's (closed form) and the halo carries the
+    neighbour's cells across the interface."""
+    k, blocks = two_box_parts()
+    hc = HostCase.from_parts(k, blocks, ghost=G)
+    assert hc.nblocks == 2
+    for b, (ni, h) in enumerate(((4, 0.1), (6, 0.1))):
+        jac = hc.field("Jac", b)[G:G + ni, G:G + 6, G:G + 8, 0]
+        assert np.allclose(jac, h * 0.1 * 0.1, rtol=1e-12)
+    # an oracle step on it keeps a uniform flow uniform (free stream)
+    orc = Oracle(hc)
+    U0 = [orc.view("U_H", b).copy() for b in range(2)]
+    orc.step(2)
+    for b, ni in enumerate((4, 6)):
+        a, r = orc.view("U_H", b)[G:G + ni, G:-G, G:-G], U0[b][G:G + ni, G:-G, G:-G]
+        assert np.abs(a - r).max() <= 1e-13 * np.abs(r).max()

@@ -171,3 +171,38 @@ def face_ghosts_rel(ctx, cid, orc, name, n, g):
         m = (out == 1) & inside
         worst = max(worst, rel(a[m], r[m]))
     return worst
+
+
+def test_blocks_of_unequal_extent_match_the_oracle():
+    """Two blocks of 4 and 6 cells along x (one slot size for the case) built
+    from a driver's arrays, a perturbed state, 6 steps: bitwise the oracle."""
+    from test_dropin import two_box_parts
+
+    k, blocks = two_box_parts()
+    rng = np.random.default_rng(9)
+    made = []
+    for _ in range(2):
+        hc = HostCase.from_parts(k, blocks, ghost=2)
+        r = np.random.default_rng(9)
+        for b in range(hc.nblocks):
+            U = hc.field("U_H", b)
+            n = hc.ns[b]
+            U[2:2 + n[0], 2:2 + n[1], 2:2 + n[2]] *= 1.0 + 1e-3 * r.uniform(-1, 1, (n[0], n[1], n[2], 1))
+            hc.set_field("U_H", b, U)
+        for sy in (3, 4, 5):
+            hc.apply_sync(sy)
+        made.append(hc)
+    del rng
+    orc = Oracle(made[0])
+    ctx = MhdContext()
+    cid = ctx.adopt(made[1])
+    ctx.finalize()
+    dts = orc.step(6)
+    ctx.step(6)
+    assert ctx.last_dt(cid) == dts[-1]
+    for f in FIELDS:
+        for b in range(2):
+            n = orc.ns[b]
+            sl = interior(f, n, 2)
+            a, r = ctx.field(cid, f, b)[sl], orc.view(f, b)[sl]
+            assert np.array_equal(a, r), (f, b)
