@@ -34,6 +34,7 @@ struct CaseRec
     int state = 0;        // 0 pending, 1 active, 2 retired, 3 draft (mhd_case_begin .. mhd_case_end)
     int first_block = -1; // index into the block table while active
     size_t op_off[SY_COUNT] = {0}, op_n[SY_COUNT] = {0}; // segments in the op groups
+    long long op_elems[SY_COUNT] = {0};
     long long steps = 0;
     size_t st_bytes = 0;
     mhd_host_case *draft = nullptr; // the blocks given so far (state 3)
@@ -70,6 +71,9 @@ struct OpGroup
 {
     MhdOp *d = nullptr;
     size_t cap = 0, n = 0;
+    long long elems = 0; // elements of all runs
+    // lanes per record: a wavefront for lists of long runs, else one thread
+    int lg() const { return n && elems >= 16 * (long long)n ? 6 : 0; }
 };
 
 enum Range
@@ -644,16 +648,19 @@ static int og_append(mhd_ctx *c, int sy, const std::vector<MhdOp> &ops, size_t &
     }
     HIPCHK(c, hipMemcpy(G.d + G.n, ops.data(), sizeof(MhdOp) * ops.size(), hipMemcpyHostToDevice));
     G.n += ops.size();
+    for (const MhdOp &o : ops)
+        G.elems += o.run > 1 ? o.run : 1;
     return MHD_OK;
 }
 
 // Remove [off, off + cnt) from a group: later ops move down in chunks of at
 // most cnt, so no chunk overlaps its destination.
-static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt)
+static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt, long long elems)
 {
     OpGroup &G = c->og[sy];
     if (cnt == 0)
         return MHD_OK;
+    G.elems -= elems;
     for (size_t s = off + cnt; s < G.n; s += cnt)
     {
         const size_t m = std::min(cnt, G.n - s);
@@ -673,7 +680,7 @@ static void unplace(mhd_ctx *c, CaseRec &r)
     for (int sy = 0; sy < SY_COUNT; ++sy)
         if (r.op_n[sy])
         {
-            (void)og_cut(c, sy, r.op_off[sy], r.op_n[sy]);
+            (void)og_cut(c, sy, r.op_off[sy], r.op_n[sy], r.op_elems[sy]);
             r.op_n[sy] = 0;
         }
     if (r.d_st)
@@ -796,6 +803,9 @@ static int place_case_(mhd_ctx *c, CaseRec &r)
         if (rc)
             return rc;
         r.op_n[sy] = ops.size();
+        r.op_elems[sy] = 0;
+        for (const MhdOp &o : ops)
+            r.op_elems[sy] += o.run > 1 ? o.run : 1;
         // the host copy is consumed: a failure after this point cannot be retried
         r.cs.ops[sy].clear();
     }
@@ -836,15 +846,15 @@ struct ProfScope
 
 static void run_ops(mhd_ctx *c, int sy, int ushift)
 {
-    mhdk::launch_ops(c->og[sy].d, (int)c->og[sy].n, c->d_cst, ushift, c->q);
+    mhdk::launch_ops(c->og[sy].d, (int)c->og[sy].n, c->og[sy].lg(), c->d_cst, ushift, c->q);
 }
 // Two op lists in one launch: only for lists with disjoint destinations of
 // which neither reads what the other writes (U-slot and B-slot lists, the
 // B_cell and J_cell ghost lists).
 static void run_ops2(mhd_ctx *c, int sya, int ushift_a, int syb, int ushift_b)
 {
-    mhdk::launch_ops2(c->og[sya].d, (int)c->og[sya].n, ushift_a, c->og[syb].d, (int)c->og[syb].n, ushift_b, c->d_cst,
-                      c->q);
+    mhdk::launch_ops2(c->og[sya].d, (int)c->og[sya].n, c->og[sya].lg(), ushift_a, c->og[syb].d, (int)c->og[syb].n,
+                      c->og[syb].lg(), ushift_b, c->d_cst, c->q);
 }
 
 // Derived fields (contravariant velocities, B_cell and the next dt, then
@@ -1326,7 +1336,7 @@ extern "C" int mhd_retire(mhd_ctx *c, int case_id)
     r.state = 2;
     for (int sy = 0; sy < SY_COUNT; ++sy)
     {
-        int rc = og_cut(c, sy, r.op_off[sy], r.op_n[sy]);
+        int rc = og_cut(c, sy, r.op_off[sy], r.op_n[sy], r.op_elems[sy]);
         if (rc)
             return rc;
         r.op_n[sy] = 0;

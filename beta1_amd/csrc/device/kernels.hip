@@ -1767,12 +1767,13 @@ __device__ __forceinline__ double solid_p(double gm, double rho, double mx, doub
 }
 
 // Op dst/src hold absolute device addresses; components are `stride`
-// elements apart.  `ushift` moves U ops to the second U buffer.
-__device__ __forceinline__ void apply_op(const MhdOp &op, const MhdCaseConst *__restrict__ cst, int ushift)
+// elements apart; element e of a run acts on dst + e dstep (src + e sstep).
+// `ushift` moves U ops to the second U buffer.
+__device__ __forceinline__ void apply_op(const MhdOp &op, int e, const MhdCaseConst *__restrict__ cst, int ushift)
 {
     const int64_t sh = ushift ? 5 * op.stride : 0;
-    double *d = gptr(op.dst) + sh;
-    const double *s = gptr(op.src) + sh;
+    double *d = gptr(op.dst) + sh + (int64_t)e * op.dstep;
+    const double *s = gptr(op.src) + sh + (int64_t)e * op.sstep;
     switch (op.kind)
     {
     case OP_COPY:
@@ -1827,25 +1828,36 @@ __device__ __forceinline__ void apply_op(const MhdOp &op, const MhdCaseConst *__
     }
 }
 
-__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n,
+// Records of a list on 2^lg lanes each (lg 0: a thread per record, lg 6: a
+// wavefront per record); the lanes of a record take its elements in turn.
+__device__ __forceinline__ void apply_rec(const MhdOp *__restrict__ ops, int t, int lg,
+                                          const MhdCaseConst *__restrict__ cst, int ushift)
+{
+    const MhdOp &op = ops[t >> lg];
+    const int lanes = 1 << lg, n = op.run > 1 ? op.run : 1;
+    for (int e = t & (lanes - 1); e < n; e += lanes)
+        apply_op(op, e, cst, ushift);
+}
+
+__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n, int lg,
                                              const MhdCaseConst *__restrict__ cst, int ushift)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n)
-        apply_op(ops[t], cst, ushift);
+    if (t < (n << lg))
+        apply_rec(ops, t, lg, cst, ushift);
 }
 
 // Two independent op lists (disjoint destinations, neither reads what the
-// other writes) in one launch: list a on threads [0, na), list b after it.
-__global__ __launch_bounds__(256) void k_ops2(const MhdOp *__restrict__ opa, int na, int ushift_a,
-                                              const MhdOp *__restrict__ opb, int nb, int ushift_b,
+// other writes) in one launch: list a on threads [0, na 2^lga), list b after it.
+__global__ __launch_bounds__(256) void k_ops2(const MhdOp *__restrict__ opa, int na, int lga, int ushift_a,
+                                              const MhdOp *__restrict__ opb, int nb, int lgb, int ushift_b,
                                               const MhdCaseConst *__restrict__ cst)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < na)
-        apply_op(opa[t], cst, ushift_a);
-    else if (t < na + nb)
-        apply_op(opb[t - na], cst, ushift_b);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, ta = na << lga;
+    if (t < ta)
+        apply_rec(opa, t, lga, cst, ushift_a);
+    else if (t - ta < (nb << lgb))
+        apply_rec(opb, t - ta, lgb, cst, ushift_b);
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
@@ -2122,21 +2134,23 @@ void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int
 {
     hipLaunchKernelGGL(k_dt_res, dim3((ncase + 63) / 64), dim3(64), 0, q, dtres, dt, cst, ncase, isub);
 }
-void launch_ops(const mpcns::MhdOp *ops, int n, const MhdCaseConst *cst, int ushift, hipStream_t q)
+void launch_ops(const mpcns::MhdOp *ops, int n, int lg, const MhdCaseConst *cst, int ushift, hipStream_t q)
 {
     if (n <= 0)
         return;
-    hipLaunchKernelGGL(k_ops, dim3((n + 255) / 256), dim3(256), 0, q, ops, n, cst, ushift);
+    const long long t = (long long)n << lg;
+    hipLaunchKernelGGL(k_ops, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, q, ops, n, lg, cst, ushift);
 }
-void launch_ops2(const mpcns::MhdOp *opa, int na, int ushift_a, const mpcns::MhdOp *opb, int nb, int ushift_b,
-                 const MhdCaseConst *cst, hipStream_t q)
+void launch_ops2(const mpcns::MhdOp *opa, int na, int lga, int ushift_a, const mpcns::MhdOp *opb, int nb, int lgb,
+                 int ushift_b, const MhdCaseConst *cst, hipStream_t q)
 {
     if (na <= 0)
-        return launch_ops(opb, nb, cst, ushift_b, q);
+        return launch_ops(opb, nb, lgb, cst, ushift_b, q);
     if (nb <= 0)
-        return launch_ops(opa, na, cst, ushift_a, q);
-    hipLaunchKernelGGL(k_ops2, dim3((na + nb + 255) / 256), dim3(256), 0, q, opa, na, ushift_a, opb, nb, ushift_b,
-                       cst);
+        return launch_ops(opa, na, lga, cst, ushift_a, q);
+    const long long t = ((long long)na << lga) + ((long long)nb << lgb);
+    hipLaunchKernelGGL(k_ops2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, q, opa, na, lga, ushift_a, opb, nb, lgb,
+                       ushift_b, cst);
 }
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q)
 {

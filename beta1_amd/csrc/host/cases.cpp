@@ -17,8 +17,12 @@ void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops
 {
     const int64_t bs = (int64_t)st_nslot(blk[0].d.model) * blk[0].d.slot;
     auto at = [&](int64_t rel) -> double & { return blk[rel / bs].st[rel % bs]; };
-    for (const MhdOp &o : ops)
+    for (const MhdOp &run : ops)
+        for (int e = 0; e < std::max(run.run, 1); ++e)
     {
+        MhdOp o = run;
+        o.dst += (int64_t)e * run.dstep;
+        o.src += (int64_t)e * run.sstep;
         if (o.kind == OP_SOLID_FIRST || o.kind == OP_SOLID_GHOST)
         {
             // BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184),

@@ -977,7 +977,9 @@ MhdOp mk(int64_t dst, int64_t src, int kind, int ncomp, int64_t stride, double v
     o.kind = kind;
     o.ncomp = ncomp;
     o.cas = 0;
-    o.pad = 0;
+    o.run = 1;
+    o.dstep = 0;
+    o.sstep = 0;
     return o;
 }
 
@@ -1066,6 +1068,94 @@ Slab inner_slab(const Block &B, int loc, int face)
     return s;
 }
 } // namespace
+
+void compact_ops(const std::vector<BlockArrays> &blk, std::vector<MhdOp> &ops)
+{
+    if (ops.size() < 2 || blk.empty())
+        return;
+    const int64_t bs = (int64_t)st_nslot(blk[0].d.model) * blk[0].d.slot, sl = blk[0].d.slot;
+    struct Item
+    {
+        MhdOp op;
+        int64_t grp;  // block * slots + slot of the destination
+        int c[3];     // (i, j, k) box coordinates of the destination
+    };
+    std::vector<Item> left;
+    left.reserve(ops.size());
+    for (const MhdOp &o : ops)
+    {
+        Item it;
+        it.op = o;
+        const int64_t b = o.dst / bs, r = o.dst % bs, s = r / sl, p = r % sl;
+        const MhdBlockDev &d = blk[b].d;
+        it.grp = b * (bs / sl + 1) + s;
+        it.c[0] = (int)(p / ((int64_t)d.sj * d.sk));
+        it.c[1] = (int)((p / d.sk) % d.sj);
+        it.c[2] = (int)(p % d.sk);
+        left.push_back(it);
+    }
+    auto same = [](const MhdOp &a, const MhdOp &b) {
+        return a.kind == b.kind && a.ncomp == b.ncomp && a.stride == b.stride && a.cas == b.cas && a.run == 1 &&
+               b.run == 1;
+    };
+    const bool uses_src[9] = {true, true, false, false, true, true, false, false, false};
+    std::vector<MhdOp> out;
+    out.reserve(ops.size() / 8 + 16);
+    const int order[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}}; // slow .. fast axes: runs along k, j, i
+    for (int pass = 0; pass < 3 && !left.empty(); ++pass)
+    {
+        const int *ax = order[pass];
+        std::sort(left.begin(), left.end(), [&](const Item &a, const Item &b) {
+            if (a.op.kind != b.op.kind) return a.op.kind < b.op.kind;
+            if (a.op.ncomp != b.op.ncomp) return a.op.ncomp < b.op.ncomp;
+            if (a.op.cas != b.op.cas) return a.op.cas < b.op.cas;
+            if (a.grp != b.grp) return a.grp < b.grp;
+            for (int q = 0; q < 3; ++q)
+                if (a.c[ax[q]] != b.c[ax[q]]) return a.c[ax[q]] < b.c[ax[q]];
+            return false;
+        });
+        std::vector<Item> rest;
+        const size_t min_run = pass < 2 ? 8 : 2; // short runs are retried along the next axis
+        size_t q = 0;
+        while (q < left.size())
+        {
+            const MhdOp &o = left[q].op;
+            size_t e = q + 1;
+            const bool src = o.kind >= 0 && o.kind < 9 && uses_src[o.kind];
+            int64_t ds = 0, ss = 0;
+            if (o.kind != OP_CONST && e < left.size() && same(o, left[e].op) && left[e].grp == left[q].grp &&
+                left[e].c[ax[0]] == left[q].c[ax[0]] && left[e].c[ax[1]] == left[q].c[ax[1]])
+            {
+                ds = left[e].op.dst - o.dst;
+                ss = src ? left[e].op.src - o.src : ds;
+                while (e < left.size() && same(o, left[e].op) && left[e].grp == left[q].grp &&
+                       left[e].c[ax[0]] == left[q].c[ax[0]] && left[e].c[ax[1]] == left[q].c[ax[1]] &&
+                       left[e].op.dst - left[e - 1].op.dst == ds &&
+                       (!src || left[e].op.src - left[e - 1].op.src == ss) && e - q < (1u << 30))
+                    ++e;
+            }
+            const size_t n = e - q;
+            if (n >= min_run && ds != 0 && ds == (int32_t)ds && ss == (int32_t)ss)
+            {
+                MhdOp r = o;
+                r.run = (int32_t)n;
+                r.dstep = (int32_t)ds;
+                r.sstep = (int32_t)(src ? ss : ds);
+                out.push_back(r);
+            }
+            else if (pass < 2)
+                rest.insert(rest.end(), left.begin() + q, left.begin() + e);
+            else
+                for (size_t t = q; t < e; ++t)
+                    out.push_back(left[t].op);
+            q = e;
+        }
+        left.swap(rest);
+    }
+    for (const Item &it : left)
+        out.push_back(it.op);
+    ops.swap(out);
+}
 
 void build_ops(CaseSetup &cs)
 {
@@ -1279,6 +1369,8 @@ void build_ops(CaseSetup &cs)
                      cs.ops[SY_EFACE_HALO]);
         }
     }
+    for (std::vector<MhdOp> &L : cs.ops)
+        compact_ops(cs.blk, L);
 }
 
 } // namespace mpcns

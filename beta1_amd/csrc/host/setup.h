@@ -78,8 +78,9 @@ struct BlockArrays
     std::vector<double> wb;  // 18 * slot: Bcell_from_Bface_w in full (host only)
 };
 
-// One element-wise operation of a boundary / halo phase.  Offsets are
-// element offsets relative to the arena the kernel receives.
+// One element-wise operation of a boundary / halo phase, or a run of them:
+// element e < run acts on dst + e * dstep (reading src + e * sstep).
+// Offsets are element offsets relative to the arena the kernel receives.
 struct MhdOp
 {
     int64_t dst;
@@ -89,7 +90,9 @@ struct MhdOp
     int32_t kind;  // MhdOpKind
     int32_t ncomp; // components handled by this op
     int32_t cas;   // case index (constants of OP_QINF / OP_SOLID_*)
-    int32_t pad;
+    int32_t run;   // elements in the run (1: a single op)
+    int32_t dstep; // destination and source step between the run's elements
+    int32_t sstep;
 };
 
 enum MhdOpKind
@@ -174,6 +177,12 @@ void build_ops(CaseSetup &cs);
 
 // Execute copy/zero/const op lists on host state arrays (start-up syncs).
 void apply_ops_host(std::vector<BlockArrays> &blk, const std::vector<MhdOp> &ops, const MhdCaseConst &cst);
+// Merge the ops of one list into runs: ops of one kind, component count and
+// case whose destinations (and sources) advance by constant steps within one
+// block slot become one record.  The ops of a list are independent (no op
+// reads what another writes, no two write one element), so their order is
+// free; runs are taken along k first, then along j and i for what is left.
+void compact_ops(const std::vector<BlockArrays> &blk, std::vector<MhdOp> &ops);
 
 // Field number (mhd_field) -> (region 0 state / 1 geometry / 2 the host-only
 // J_cell weight table, first slot, location, components); false when the
