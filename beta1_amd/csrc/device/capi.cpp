@@ -33,8 +33,8 @@ struct CaseRec
     int nblocks = 0;
     int state = 0;        // 0 pending, 1 active, 2 retired, 3 draft (mhd_case_begin .. mhd_case_end)
     int first_block = -1; // index into the block table while active
-    size_t op_off[SY_COUNT] = {0}, op_n[SY_COUNT] = {0}; // segments in the op groups
-    long long op_elems[SY_COUNT] = {0};
+    // segments in the op sub-lists (kOpCls per sync, see OpGroup)
+    size_t op_off[SY_COUNT * 2] = {0}, op_n[SY_COUNT * 2] = {0};
     long long steps = 0;
     size_t st_bytes = 0;
     mhd_host_case *draft = nullptr; // the blocks given so far (state 3)
@@ -67,13 +67,16 @@ struct TileArr
 
 // Op list of one sync group on the device: the cases' ops back to back,
 // each case's segment appended when it is placed and cut out when it retires.
+// The records of one sync, in two sub-lists by run length: runs shorter than
+// kLongRun on a thread each, longer ones on a wavefront each (its lanes take
+// the run's elements in turn), so a list mixing single points and long rows
+// has no thread walking a row alone.
+constexpr int kOpCls = 2, kLongRun = 16;
+static int op_cls(const MhdOp &o) { return o.run >= kLongRun ? 1 : 0; }
 struct OpGroup
 {
     MhdOp *d = nullptr;
     size_t cap = 0, n = 0;
-    long long elems = 0; // elements of all runs
-    // lanes per record: a wavefront for lists of long runs, else one thread
-    int lg() const { return n && elems >= 16 * (long long)n ? 6 : 0; }
 };
 
 enum Range
@@ -185,7 +188,7 @@ struct mhd_ctx
     int *d_err = nullptr, *d_active = nullptr;
 
     // op lists of the active cases, one device array per sync group
-    OpGroup og[SY_COUNT];
+    OpGroup og[SY_COUNT * kOpCls]; // sub-list cls of sync sy at sy * kOpCls + cls
 
     hipGraphExec_t gexec = nullptr;
     // per-stage entry points: mhd_fluid_rhs has written the next U buffer and
@@ -573,7 +576,7 @@ static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -
     return L;
 }
 
-// The fluid update: column tiles for the first-order scheme, cell tiles for minmod.
+// The fluid updates: column tiles for the first-order scheme, cell tiles for minmod.
 static void launch_fluid_any(mhd_ctx *c, int variant, const double *dts, hipStream_t q)
 {
     const int mixed = mixed_of(c);
@@ -581,6 +584,14 @@ static void launch_fluid_any(mhd_ctx *c, int variant, const double *dts, hipStre
         mhdk::launch_fluid_col(launch_of_col(c, KP_FLUID, variant), c->d_cst, dts, c->ucur, mixed, q);
     else
         mhdk::launch_fluid(launch_of(c, KP_FLUID, variant), c->d_cst, dts, c->ucur, mixed, c->opt.recon, q);
+}
+static void launch_fluid2_any(mhd_ctx *c, int variant, const double *dts, hipStream_t q)
+{
+    const int mixed = mixed_of(c);
+    if (c->opt.recon == 0)
+        mhdk::launch_fluid2_col(launch_of_col(c, KP_FLUID2, variant), c->d_cst, dts, c->ucur, mixed, q);
+    else
+        mhdk::launch_fluid2(launch_of(c, KP_FLUID2, variant), c->d_cst, dts, c->ucur, mixed, c->opt.recon, q);
 }
 
 // Rebuild the block table, tiles, op lists and per-case constants for the
@@ -629,9 +640,9 @@ static int rebuild(mhd_ctx *c)
 }
 
 // ------------------------------------------------------------- placement
-static int og_append(mhd_ctx *c, int sy, const std::vector<MhdOp> &ops, size_t &off)
+static int og_append(mhd_ctx *c, int q, const std::vector<MhdOp> &ops, size_t &off)
 {
-    OpGroup &G = c->og[sy];
+    OpGroup &G = c->og[q];
     off = G.n;
     if (ops.empty())
         return MHD_OK;
@@ -648,19 +659,16 @@ static int og_append(mhd_ctx *c, int sy, const std::vector<MhdOp> &ops, size_t &
     }
     HIPCHK(c, hipMemcpy(G.d + G.n, ops.data(), sizeof(MhdOp) * ops.size(), hipMemcpyHostToDevice));
     G.n += ops.size();
-    for (const MhdOp &o : ops)
-        G.elems += o.run > 1 ? o.run : 1;
     return MHD_OK;
 }
 
 // Remove [off, off + cnt) from a group: later ops move down in chunks of at
 // most cnt, so no chunk overlaps its destination.
-static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt, long long elems)
+static int og_cut(mhd_ctx *c, int q, size_t off, size_t cnt)
 {
-    OpGroup &G = c->og[sy];
+    OpGroup &G = c->og[q];
     if (cnt == 0)
         return MHD_OK;
-    G.elems -= elems;
     for (size_t s = off + cnt; s < G.n; s += cnt)
     {
         const size_t m = std::min(cnt, G.n - s);
@@ -668,8 +676,8 @@ static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt, long long elems)
     }
     G.n -= cnt;
     for (auto &r : c->cases)
-        if (r->state == 1 && r->op_off[sy] > off)
-            r->op_off[sy] -= cnt;
+        if (r->state == 1 && r->op_off[q] > off)
+            r->op_off[q] -= cnt;
     return MHD_OK;
 }
 
@@ -677,11 +685,11 @@ static int og_cut(mhd_ctx *c, int sy, size_t off, size_t cnt, long long elems)
 // reference to the geometry set (freed with the last reference).
 static void unplace(mhd_ctx *c, CaseRec &r)
 {
-    for (int sy = 0; sy < SY_COUNT; ++sy)
-        if (r.op_n[sy])
+    for (int q = 0; q < SY_COUNT * kOpCls; ++q)
+        if (r.op_n[q])
         {
-            (void)og_cut(c, sy, r.op_off[sy], r.op_n[sy], r.op_elems[sy]);
-            r.op_n[sy] = 0;
+            (void)og_cut(c, q, r.op_off[q], r.op_n[q]);
+            r.op_n[q] = 0;
         }
     if (r.d_st)
     {
@@ -799,13 +807,17 @@ static int place_case_(mhd_ctx *c, CaseRec &r)
                 o.src = o.dst;
             o.cas = r.id;
         }
-        int rc = og_append(c, sy, ops, r.op_off[sy]);
-        if (rc)
-            return rc;
-        r.op_n[sy] = ops.size();
-        r.op_elems[sy] = 0;
+        std::vector<MhdOp> by_cls[kOpCls];
         for (const MhdOp &o : ops)
-            r.op_elems[sy] += o.run > 1 ? o.run : 1;
+            by_cls[op_cls(o)].push_back(o);
+        for (int cl = 0; cl < kOpCls; ++cl)
+        {
+            const int q = sy * kOpCls + cl;
+            int rc = og_append(c, q, by_cls[cl], r.op_off[q]);
+            if (rc)
+                return rc;
+            r.op_n[q] = by_cls[cl].size();
+        }
         // the host copy is consumed: a failure after this point cannot be retried
         r.cs.ops[sy].clear();
     }
@@ -844,17 +856,30 @@ struct ProfScope
     }
 };
 
+static void add_sync(const mhd_ctx *c, mhdk::OpLaunch &L, int sy, int ushift)
+{
+    for (int cl = 0; cl < kOpCls; ++cl)
+    {
+        const OpGroup &G = c->og[sy * kOpCls + cl];
+        if (G.n)
+            L.seg[L.nseg++] = {G.d, (int)G.n, cl ? 6 : 0, ushift};
+    }
+}
 static void run_ops(mhd_ctx *c, int sy, int ushift)
 {
-    mhdk::launch_ops(c->og[sy].d, (int)c->og[sy].n, c->og[sy].lg(), c->d_cst, ushift, c->q);
+    mhdk::OpLaunch L{};
+    add_sync(c, L, sy, ushift);
+    mhdk::launch_ops(L, c->d_cst, c->q);
 }
 // Two op lists in one launch: only for lists with disjoint destinations of
 // which neither reads what the other writes (U-slot and B-slot lists, the
 // B_cell and J_cell ghost lists).
 static void run_ops2(mhd_ctx *c, int sya, int ushift_a, int syb, int ushift_b)
 {
-    mhdk::launch_ops2(c->og[sya].d, (int)c->og[sya].n, c->og[sya].lg(), ushift_a, c->og[syb].d, (int)c->og[syb].n,
-                      c->og[syb].lg(), ushift_b, c->d_cst, c->q);
+    mhdk::OpLaunch L{};
+    add_sync(c, L, sya, ushift_a);
+    add_sync(c, L, syb, ushift_b);
+    mhdk::launch_ops(L, c->d_cst, c->q);
 }
 
 // Derived fields (contravariant velocities, B_cell and the next dt, then
@@ -938,7 +963,7 @@ static void stage_fluid(mhd_ctx *c, hipStream_t qf)
     if (runs(c, KP_FLUID2))
     {
         ProfScope p(c, KP_FLUID2);
-        mhdk::launch_fluid2(launch_of(c, KP_FLUID2), c->d_cst, c->d_dt, c->ucur, mixed_of(c), c->opt.recon, qf);
+        launch_fluid2_any(c, -1, c->d_dt, qf);
     }
 }
 
@@ -1110,7 +1135,7 @@ static int autotune(mhd_ctx *c)
                     mhdk::launch_jedge(L, mixed, c->q);
                     break;
                 case KP_FLUID2:
-                    mhdk::launch_fluid2(L, c->d_cst, c->d_zero, c->ucur, mixed, c->opt.recon, c->q);
+                    launch_fluid2_any(c, ji, c->d_zero, c->q);
                     break;
                 case KP_DERIVED2:
                     mhdk::launch_derived2(L, c->d_cst, c->d_dtbits, c->ucur, mixed, c->q);
@@ -1334,12 +1359,12 @@ extern "C" int mhd_retire(mhd_ctx *c, int case_id)
         c->bytes_used -= G.bytes;
     }
     r.state = 2;
-    for (int sy = 0; sy < SY_COUNT; ++sy)
+    for (int q = 0; q < SY_COUNT * kOpCls; ++q)
     {
-        int rc = og_cut(c, sy, r.op_off[sy], r.op_n[sy], r.op_elems[sy]);
+        int rc = og_cut(c, q, r.op_off[q], r.op_n[q]);
         if (rc)
             return rc;
-        r.op_n[sy] = 0;
+        r.op_n[q] = 0;
     }
     const int zero = 0;
     HIPCHK(c, hipMemcpy(c->d_err + case_id, &zero, sizeof(int), hipMemcpyHostToDevice));

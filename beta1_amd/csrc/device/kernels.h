@@ -38,16 +38,29 @@ void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long
 // two-fluid (Mercury) step
 void launch_derived2(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
                      hipStream_t q);
+void launch_fluid2_col(const Launch &C, const MhdCaseConst *cst, const double *dts, int ucur, int mixed,
+                       hipStream_t q);
 void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                    hipStream_t q);
 void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q);
 void launch_eres(const Launch &L, const MhdCaseConst *cst, hipStream_t q);
 void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int ncase, int isub, hipStream_t q);
-// n records on 2^lg lanes each (lg 0 or 6)
-void launch_ops(const mpcns::MhdOp *ops, int n, int lg, const MhdCaseConst *cst, int ushift, hipStream_t q);
-// two independent op lists in one launch
-void launch_ops2(const mpcns::MhdOp *opa, int na, int lga, int ushift_a, const mpcns::MhdOp *opb, int nb, int lgb,
-                 int ushift_b, const MhdCaseConst *cst, hipStream_t q);
+// Up to OPSEG_MAX segments of op records in one launch: segment s holds n
+// records on 2^lg lanes each (lg 0 or 6), U-slot ops shifted to the other U
+// buffer when ushift is set.  The segments' destinations are disjoint and no
+// segment reads what another writes.
+struct OpSeg
+{
+    const mpcns::MhdOp *ops;
+    int n, lg, ushift;
+};
+constexpr int OPSEG_MAX = 4;
+struct OpLaunch
+{
+    OpSeg seg[OPSEG_MAX];
+    int nseg;
+};
+void launch_ops(const OpLaunch &L, const MhdCaseConst *cst, hipStream_t q);
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q);
 void launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t q);
 // 17 order keys per case (min/max diagnostics) at out + 17 * case id; the

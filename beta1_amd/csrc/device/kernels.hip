@@ -1376,6 +1376,129 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
         reduce_dt(local, dtbits, b.case_id);
 }
 
+// The sources and the Euler update of the two-fluid cell o, given the flux
+// divergence of a species in rhs (((0 - t_xi) - t_eta) - t_zeta) and both
+// primitive states ph, pn (calc_PV of U[cur]): the Mercury terms in the order
+// of MercurySolver_FluidRHS.cpp:206-527, then U[1-cur] = U[cur] + dt rhs
+// (MercurySolver_Advance.cpp:73-112).  two_fluid_cell forms what both
+// species read; two_fluid_species applies one species' terms.  Shared by the
+// cell and column forms of the update.
+struct TwoFluidCell
+{
+    NumInfo num;
+    double upx, upy, upz, Bx, By, Bz, sjbx, sjby, sjbz, dpex, dpey, dpez, sss, vst;
+};
+__device__ __forceinline__ TwoFluidCell two_fluid_cell(const MhdCaseConst &c, const double *__restrict__ S,
+                                                       const double *__restrict__ G, int64_t sl, int64_t o,
+                                                       const int64_t *step, int ucur, const Prim &ph, const Prim &pn)
+{
+    TwoFluidCell t;
+    const double *UH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
+    t.num = num_limiter(c, UH[o], UN[o]);
+    {
+        // U_plus of this cell (calc_Uplus), from the same U the step started from
+        const double rH = dmax(UH[o], 0.0), rN = dmax(UN[o], 0.0);
+        const NumInfo n = num_limiter(c, rH, rN);
+        const bool hH = rH > 1e-20, hN = rN > 1e-20;
+        t.upx = n.chiH * (hH ? ph.u : 0.0) + n.chiNa * (hN ? pn.u : 0.0);
+        t.upy = n.chiH * (hH ? ph.v : 0.0) + n.chiNa * (hN ? pn.v : 0.0);
+        t.upz = n.chiH * (hH ? ph.w : 0.0) + n.chiNa * (hN ? pn.w : 0.0);
+    }
+    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+    t.Bx = Bc[o];
+    t.By = Bc[sl + o];
+    t.Bz = Bc[2 * sl + o];
+    const double Jx = Jc[o], Jy = Jc[sl + o], Jz = Jc[2 * sl + o];
+    t.sjbx = Jy * t.Bz - Jz * t.By;
+    t.sjby = Jz * t.Bx - Jx * t.Bz;
+    t.sjbz = Jx * t.By - Jy * t.Bx;
+    t.dpex = t.dpey = t.dpez = 0.0;
+    if (c.ambipolar != 0.0)
+    {
+        double m9[9];
+        const double V = fabs(G[(int64_t)G_JAC * sl + o]);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+        {
+            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                m9[3 * ax + q] = V <= 0.0 ? 0.0 : 0.5 * (A[q * sl + o - step[ax]] + A[q * sl + o]) / V;
+        }
+        const double *PE = S + (int64_t)ST_PE * sl;
+        const double gi = PE[o + step[0]] - PE[o - step[0]];
+        const double gj = PE[o + step[1]] - PE[o - step[1]];
+        const double gk = PE[o + step[2]] - PE[o - step[2]];
+        t.dpex = 0.5 * (m9[0] * gi + m9[3] * gj + m9[6] * gk);
+        t.dpey = 0.5 * (m9[1] * gi + m9[4] * gj + m9[7] * gk);
+        t.dpez = 0.5 * (m9[2] * gi + m9[5] * gj + m9[8] * gk);
+    }
+    t.sss = S[(int64_t)ST_PHOTO * sl + o];
+    t.vst = S[(int64_t)ST_DRAG * sl + o];
+    return t;
+}
+
+__device__ __forceinline__ void two_fluid_species(const MhdCaseConst &c, double *__restrict__ S, int64_t sl,
+                                                  int64_t o, int ucur, int sp, const Prim &p, const TwoFluidCell &t,
+                                                  double *R_, double dt)
+{
+    const double *U = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * ucur) * sl;
+    const double n_s = sp ? t.num.nNa : t.num.nH;
+    const double chi = sp ? t.num.wNa : t.num.wH;
+    const double rho = dmax(U[o], 0.0);
+    const double u = p.u, v = p.v, w = p.w;
+    const double u2 = u * u + v * v + w * w;
+    const double subx = (v - t.upy) * t.Bz - (w - t.upz) * t.By;
+    const double suby = (w - t.upz) * t.Bx - (u - t.upx) * t.Bz;
+    const double subz = (u - t.upx) * t.By - (v - t.upy) * t.Bx;
+    const double subu = subx * u + suby * v + subz * w;
+    const double sjbu = t.sjbx * u + t.sjby * v + t.sjbz * w;
+    const double dpeu = t.dpex * u + t.dpey * v + t.dpez * w;
+    const double T = p.p / (dmax(p.rho_raw, 1e-12) * (sp ? c.state_coeff_Na : c.state_coeff));
+    const double Ts0 = T * c.T_ref;
+    const double vst = t.vst, sss = t.sss;
+    const double b1 = sp ? vst * (c.Tn0 - Ts0) * (c.M_Na / (c.M_Na + c.M_Na))
+                         : vst * (c.Tn0 - Ts0) * (c.M_H / (c.M_H + c.M_Na));
+    const double b2 = sp ? (c.M_Na / (c.M_Na + c.M_Na)) * vst : (c.M_Na / (c.M_H + c.M_Na)) * vst;
+    R_[1] += c.induce * n_s * subx + c.momentum_hall_coeff * chi * t.sjbx;
+    R_[2] += c.induce * n_s * suby + c.momentum_hall_coeff * chi * t.sjby;
+    R_[3] += c.induce * n_s * subz + c.momentum_hall_coeff * chi * t.sjbz;
+    R_[4] += c.induce * n_s * subu + c.momentum_hall_coeff * chi * sjbu;
+    if (c.ambipolar != 0.0)
+    {
+        R_[1] -= chi * t.dpex;
+        R_[2] -= chi * t.dpey;
+        R_[3] -= chi * t.dpez;
+        R_[4] -= chi * dpeu;
+    }
+    if (sp)
+    {
+        R_[0] += c.a1_Na * sss;
+        R_[4] += c.a6 * sss * c.Tn0;
+    }
+    R_[1] -= c.a4 * rho * u * vst;
+    R_[2] -= c.a4 * rho * v * vst;
+    R_[3] -= c.a4 * rho * w * vst;
+    R_[4] -= c.a4 * rho * u2 * vst;
+    R_[4] += c.a5 * n_s * b1;
+    R_[4] += c.a4 * rho * u2 * b2;
+    double *Un = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * (1 - ucur)) * sl;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        Un[m * sl + o] = U[m * sl + o] + dt * R_[m];
+}
+
+__device__ __forceinline__ void two_fluid_update(const MhdCaseConst &c, double *__restrict__ S,
+                                                 const double *__restrict__ G, int64_t sl, int64_t o,
+                                                 const int64_t *step, int ucur, const Prim &ph, const Prim &pn,
+                                                 double (&rhs)[2][5], double dt)
+{
+    const TwoFluidCell t = two_fluid_cell(c, S, G, sl, o, step, ucur, ph, pn);
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+        two_fluid_species(c, S, sl, o, ucur, sp, sp ? pn : ph, t, rhs[sp], dt);
+}
+
 // Fluid update of the two-fluid step: Rusanov fluxes of each species
 // (Z5_Mercury/src/physics/MercurySolver_FluidRHS.cpp:3-202), the Mercury
 // sources (:206-527: induction force n_s (u_s - u_+) x B, composition-
@@ -1426,95 +1549,146 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     const double *UH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
     const Prim ph = prim_of<sizeof(R) == 8>(UH, sl, o, gm), pn = prim_of<sizeof(R) == 8>(UN, sl, o, gm);
-    const NumInfo num = num_limiter(c, UH[o], UN[o]);
-    double upx, upy, upz;
-    {
-        // U_plus of this cell (calc_Uplus), from the same U the step started from
-        const double rH = dmax(UH[o], 0.0), rN = dmax(UN[o], 0.0);
-        const NumInfo n = num_limiter(c, rH, rN);
-        const bool hH = rH > 1e-20, hN = rN > 1e-20;
-        upx = n.chiH * (hH ? ph.u : 0.0) + n.chiNa * (hN ? pn.u : 0.0);
-        upy = n.chiH * (hH ? ph.v : 0.0) + n.chiNa * (hN ? pn.v : 0.0);
-        upz = n.chiH * (hH ? ph.w : 0.0) + n.chiNa * (hN ? pn.w : 0.0);
-    }
-    const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
-    const double Bx = Bc[o], By = Bc[sl + o], Bz = Bc[2 * sl + o];
-    const double Jx = Jc[o], Jy = Jc[sl + o], Jz = Jc[2 * sl + o];
-    const double sjbx = Jy * Bz - Jz * By;
-    const double sjby = Jz * Bx - Jx * Bz;
-    const double sjbz = Jx * By - Jy * Bx;
-    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
-    if (c.ambipolar != 0.0)
-    {
-        double m9[9];
-        const double V = fabs(G[(int64_t)G_JAC * sl + o]);
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-        {
-            const double *A = G + (int64_t)(G_JD + 3 * ax) * sl;
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                m9[3 * ax + q] = V <= 0.0 ? 0.0 : 0.5 * (A[q * sl + o - step[ax]] + A[q * sl + o]) / V;
-        }
-        const double *PE = S + (int64_t)ST_PE * sl;
-        const double gi = PE[o + step[0]] - PE[o - step[0]];
-        const double gj = PE[o + step[1]] - PE[o - step[1]];
-        const double gk = PE[o + step[2]] - PE[o - step[2]];
-        dpex = 0.5 * (m9[0] * gi + m9[3] * gj + m9[6] * gk);
-        dpey = 0.5 * (m9[1] * gi + m9[4] * gj + m9[7] * gk);
-        dpez = 0.5 * (m9[2] * gi + m9[5] * gj + m9[8] * gk);
-    }
-    const double sss = S[(int64_t)ST_PHOTO * sl + o];
-    const double vst = S[(int64_t)ST_DRAG * sl + o];
+    two_fluid_update(c, S, G, sl, o, step, ucur, ph, pn, rhs, dts[b.case_id]);
+}
+
+// The first-order two-fluid update in column tiles (k_fluid_col's form for
+// both species): per plane i the workgroup evaluates flux_cell of H+ and Na+
+// once for its 64 x COL_TJ cells and their one-cell halo in j and k (LDS); a
+// thread keeps its column's cell i-1 of both species in registers, so each
+// face flux is computed once (k_fluid2 evaluates every face twice, once from
+// each side).  Each flux is rusanov_pre of the same two cells as face_flux
+// in k_fluid2, the divergence adds the terms in its order and
+// two_fluid_update applies the same sources, so the result is bit for bit
+// k_fluid2's.
+template <typename R>
+__global__ __launch_bounds__(64 * COL_TJ) void k_fluid2_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                            const double *__restrict__ dts, int ucur)
+{
+    constexpr int PJ = COL_TJ + 2, PK = COL_TK + 2;
+    __shared__ double P[2][7][PJ][PK];
+    MhdBlockDev b;
+    int i0, j0, k0;
+    if (!tile_col(tm, b, i0, j0, k0)) // uniform over the workgroup
+        return;
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * 64 + tx;
+    const int j = j0 + ty, k = k0 + tx;
+    const bool act = j < b.nj && k < b.nk;
+    const int i1 = min(i0 + tm.ci, b.ni);
+    const MhdCaseConst &c = cst[b.case_id];
+    const double gm = c.gamma;
+    const int64_t sl = b.slot, di = (int64_t)b.sj * b.sk, dj = b.sk;
+    const int64_t step[3] = {di, dj, 1};
+    double *S = dst_ptr(b);
+    const double *Us[2] = {S + (int64_t)(ST_U0 + 5 * ucur) * sl, S + (int64_t)(ST_UNA0 + 5 * ucur) * sl};
+    const double *G = dmetric(b);
+    const double *Ax = G + (int64_t)G_JD * sl, *Ay = Ax + 3 * sl, *Az = Ax + 6 * sl;
     const double dt = dts[b.case_id];
-#pragma unroll
-    for (int sp = 0; sp < 2; ++sp)
+    int64_t o = blk_off(b, i0, act ? j : j0, act ? k : k0); // cell of plane `pl` in the loop
+    FluxCell prev[2] = {};
+    if (act)
     {
-        const Prim &p = sp ? pn : ph;
-        const double *U = sp ? UN : UH;
-        double *R_ = rhs[sp];
-        const double n_s = sp ? num.nNa : num.nH;
-        const double chi = sp ? num.wNa : num.wH;
-        const double rho = dmax(U[o], 0.0);
-        const double u = p.u, v = p.v, w = p.w;
-        const double u2 = u * u + v * v + w * w;
-        const double subx = (v - upy) * Bz - (w - upz) * By;
-        const double suby = (w - upz) * Bx - (u - upx) * Bz;
-        const double subz = (u - upx) * By - (v - upy) * Bx;
-        const double subu = subx * u + suby * v + subz * w;
-        const double sjbu = sjbx * u + sjby * v + sjbz * w;
-        const double dpeu = dpex * u + dpey * v + dpez * w;
-        const double T = p.p / (dmax(p.rho_raw, 1e-12) * (sp ? c.state_coeff_Na : c.state_coeff));
-        const double Ts0 = T * c.T_ref;
-        const double b1 = sp ? vst * (c.Tn0 - Ts0) * (c.M_Na / (c.M_Na + c.M_Na))
-                             : vst * (c.Tn0 - Ts0) * (c.M_H / (c.M_H + c.M_Na));
-        const double b2 = sp ? (c.M_Na / (c.M_Na + c.M_Na)) * vst : (c.M_Na / (c.M_H + c.M_Na)) * vst;
-        R_[1] += c.induce * n_s * subx + c.momentum_hall_coeff * chi * sjbx;
-        R_[2] += c.induce * n_s * suby + c.momentum_hall_coeff * chi * sjby;
-        R_[3] += c.induce * n_s * subz + c.momentum_hall_coeff * chi * sjbz;
-        R_[4] += c.induce * n_s * subu + c.momentum_hall_coeff * chi * sjbu;
-        if (c.ambipolar != 0.0)
-        {
-            R_[1] -= chi * dpex;
-            R_[2] -= chi * dpey;
-            R_[3] -= chi * dpez;
-            R_[4] -= chi * dpeu;
-        }
-        if (sp)
-        {
-            R_[0] += c.a1_Na * sss;
-            R_[4] += c.a6 * sss * c.Tn0;
-        }
-        R_[1] -= c.a4 * rho * u * vst;
-        R_[2] -= c.a4 * rho * v * vst;
-        R_[3] -= c.a4 * rho * w * vst;
-        R_[4] -= c.a4 * rho * u2 * vst;
-        R_[4] += c.a5 * n_s * b1;
-        R_[4] += c.a4 * rho * u2 * b2;
-        double *Un = S + (int64_t)((sp ? ST_UNA0 : ST_U0) + 5 * (1 - ucur)) * sl;
 #pragma unroll
-        for (int m = 0; m < 5; ++m)
-            Un[m * sl + o] = U[m * sl + o] + dt * R_[m];
+        for (int sp = 0; sp < 2; ++sp)
+            prev[sp] = flux_cell<R>(Us[sp], sl, o - di, gm);
+    }
+    // the open cell (plane pl - 1): lower i-face flux, j and k terms, 1/J
+    double Flo[2][5] = {}, tj[2][5] = {}, tk[2][5] = {};
+    double invJ = 0.0;
+    for (int pl = i0; pl <= i1; ++pl, o += di)
+    {
+        if (pl < i1)
+        {
+            for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
+            {
+                const int pj = p / PK, pk = p - pj * PK;
+                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+                if (jj <= b.nj && kk <= b.nk)
+                {
+#pragma unroll
+                    for (int sp = 0; sp < 2; ++sp)
+                    {
+                        const FluxCell f = flux_cell<R>(Us[sp], sl, blk_off(b, pl, jj, kk), gm);
+                        P[sp][0][pj][pk] = f.rho;
+                        P[sp][1][pj][pk] = f.u;
+                        P[sp][2][pj][pk] = f.v;
+                        P[sp][3][pj][pk] = f.w;
+                        P[sp][4][pj][pk] = f.p;
+                        P[sp][5][pj][pk] = f.e;
+                        P[sp][6][pj][pk] = f.c;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (act)
+        {
+            auto lds = [&](int sp, int pj, int pk) {
+                FluxCell f;
+                f.rho = P[sp][0][pj][pk];
+                f.u = P[sp][1][pj][pk];
+                f.v = P[sp][2][pj][pk];
+                f.w = P[sp][3][pj][pk];
+                f.p = P[sp][4][pj][pk];
+                f.e = P[sp][5][pj][pk];
+                f.c = P[sp][6][pj][pk];
+                return f;
+            };
+            FluxCell cur[2];
+            double F[2][5];
+#pragma unroll
+            for (int sp = 0; sp < 2; ++sp)
+            {
+                cur[sp] = pl < i1 ? lds(sp, ty + 1, tx + 1) : flux_cell<R>(Us[sp], sl, o, gm);
+                rusanov_pre<R>(prev[sp], cur[sp], Ax[o], Ax[sl + o], Ax[2 * sl + o], F[sp]);
+            }
+            if (pl > i0) // close cell pl - 1
+            {
+                double rhs[2][5];
+#pragma unroll
+                for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                    {
+                        double r = 0.0;
+                        r -= (F[sp][m] - Flo[sp][m]) * invJ;
+                        r -= tj[sp][m];
+                        r -= tk[sp][m];
+                        rhs[sp][m] = r;
+                    }
+                const Prim ph{prev[0].rho, prev[0].u, prev[0].v, prev[0].w, prev[0].p};
+                const Prim pn{prev[1].rho, prev[1].u, prev[1].v, prev[1].w, prev[1].p};
+                two_fluid_update(c, S, G, sl, o - di, step, ucur, ph, pn, rhs, dt);
+            }
+            if (pl < i1) // open cell pl
+            {
+                invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+#pragma unroll
+                for (int sp = 0; sp < 2; ++sp)
+                {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        Flo[sp][m] = F[sp][m];
+                    double Fl[5], Fh[5];
+                    rusanov_pre<R>(lds(sp, ty, tx + 1), cur[sp], Ay[o], Ay[sl + o], Ay[2 * sl + o], Fl);
+                    rusanov_pre<R>(cur[sp], lds(sp, ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj],
+                                   Fh);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        tj[sp][m] = (Fh[m] - Fl[m]) * invJ;
+                    rusanov_pre<R>(lds(sp, ty + 1, tx), cur[sp], Az[o], Az[sl + o], Az[2 * sl + o], Fl);
+                    rusanov_pre<R>(cur[sp], lds(sp, ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fh);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        tk[sp][m] = (Fh[m] - Fl[m]) * invJ;
+                }
+            }
+#pragma unroll
+            for (int sp = 0; sp < 2; ++sp)
+                prev[sp] = cur[sp];
+        }
+        if (pl < i1)
+            __syncthreads();
     }
 }
 
@@ -1839,25 +2013,22 @@ __device__ __forceinline__ void apply_rec(const MhdOp *__restrict__ ops, int t, 
         apply_op(op, e, cst, ushift);
 }
 
-__global__ __launch_bounds__(256) void k_ops(const MhdOp *__restrict__ ops, int n, int lg,
-                                             const MhdCaseConst *__restrict__ cst, int ushift)
+__global__ __launch_bounds__(256) void k_ops(const mhdk::OpLaunch L, const MhdCaseConst *__restrict__ cst)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < (n << lg))
-        apply_rec(ops, t, lg, cst, ushift);
-}
-
-// Two independent op lists (disjoint destinations, neither reads what the
-// other writes) in one launch: list a on threads [0, na 2^lga), list b after it.
-__global__ __launch_bounds__(256) void k_ops2(const MhdOp *__restrict__ opa, int na, int lga, int ushift_a,
-                                              const MhdOp *__restrict__ opb, int nb, int lgb, int ushift_b,
-                                              const MhdCaseConst *__restrict__ cst)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, ta = na << lga;
-    if (t < ta)
-        apply_rec(opa, t, lga, cst, ushift_a);
-    else if (t - ta < (nb << lgb))
-        apply_rec(opb, t - ta, lgb, cst, ushift_b);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int g = 0; g < mhdk::OPSEG_MAX; ++g)
+    {
+        if (g >= L.nseg)
+            return;
+        const int m = L.seg[g].n << L.seg[g].lg;
+        if (t < m)
+        {
+            apply_rec(L.seg[g].ops, t, L.seg[g].lg, cst, L.seg[g].ushift);
+            return;
+        }
+        t -= m;
+    }
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
@@ -2119,6 +2290,15 @@ void launch_fluid2(const Launch &L, const MhdCaseConst *cst, const double *dts, 
     else
         hipLaunchKernelGGL((k_fluid2<double, 0>), g, bl, 0, q, L.tm, cst, dts, ucur);
 }
+void launch_fluid2_col(const Launch &C, const MhdCaseConst *cst, const double *dts, int ucur, int mixed,
+                       hipStream_t q)
+{
+    const dim3 g = grid_of(C), bl(64, COL_TJ, 1);
+    if (mixed)
+        hipLaunchKernelGGL((k_fluid2_col<float>), g, bl, 0, q, C.tm, cst, dts, ucur);
+    else
+        hipLaunchKernelGGL((k_fluid2_col<double>), g, bl, 0, q, C.tm, cst, dts, ucur);
+}
 void launch_emf_extra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q)
 {
     if (mixed)
@@ -2134,23 +2314,14 @@ void launch_dt_res(double *dtres, const double *dt, const MhdCaseConst *cst, int
 {
     hipLaunchKernelGGL(k_dt_res, dim3((ncase + 63) / 64), dim3(64), 0, q, dtres, dt, cst, ncase, isub);
 }
-void launch_ops(const mpcns::MhdOp *ops, int n, int lg, const MhdCaseConst *cst, int ushift, hipStream_t q)
+void launch_ops(const OpLaunch &L, const MhdCaseConst *cst, hipStream_t q)
 {
-    if (n <= 0)
+    long long t = 0;
+    for (int g = 0; g < L.nseg; ++g)
+        t += (long long)L.seg[g].n << L.seg[g].lg;
+    if (t <= 0)
         return;
-    const long long t = (long long)n << lg;
-    hipLaunchKernelGGL(k_ops, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, q, ops, n, lg, cst, ushift);
-}
-void launch_ops2(const mpcns::MhdOp *opa, int na, int lga, int ushift_a, const mpcns::MhdOp *opb, int nb, int lgb,
-                 int ushift_b, const MhdCaseConst *cst, hipStream_t q)
-{
-    if (na <= 0)
-        return launch_ops(opb, nb, lgb, cst, ushift_b, q);
-    if (nb <= 0)
-        return launch_ops(opa, na, lga, cst, ushift_a, q);
-    const long long t = ((long long)na << lga) + ((long long)nb << lgb);
-    hipLaunchKernelGGL(k_ops2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, q, opa, na, lga, ushift_a, opb, nb, lgb,
-                       ushift_b, cst);
+    hipLaunchKernelGGL(k_ops, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, q, L, cst);
 }
 void launch_divb(const Launch &L, unsigned long long *out, int mixed, hipStream_t q)
 {
