@@ -90,8 +90,10 @@ const int kRangeLo[R_COUNT] = {0, 0, -1}, kRangeHi[R_COUNT] = {0, 1, 1};
 const int kJt[3] = {1, 2, 4};
 // launch variants: workgroup rows kJt[v % 3], tile order i-inner if
 // v % 6 >= 3, geometry-shared tile map (cases of one geometry set
-// interleaved per tile) if v >= 6
-const int kVariants = 12;
+// interleaved per tile) if v >= 6; v = 12 + rows index: geometry-shared map
+// with member-looping tiles (a tile covers every case of its geometry set,
+// which the kernel loads once), for the kernels in loops_members
+const int kVariants = 12, kLoopVariants = 15;
 
 // Launch tiling over a set of blocks.  An entry is one block or, in a
 // geometry-shared set, the same block of every case sharing a geometry set
@@ -106,6 +108,7 @@ struct TileSet
     TileArr tiles[R_COUNT][3];  // [range][rows]
     TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
     TileArr cols;              // column tiles of the interior cells (kColCi planes each)
+    TileArr kl[R_COUNT][3];    // geometry-shared sets: member-looping tiles [range][rows]
 };
 // planes along i per column-tile workgroup
 constexpr int kColCi = 8;
@@ -148,6 +151,8 @@ const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", 
                                      "k_curl_res", "k_jedge"};
 const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL,
                                     R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_FACE};
+// kernels with a member-looping variant (k_derived_k)
+bool loops_members(int kernel) { return kernel == KP_DERIVED; }
 } // namespace
 
 struct mhd_ctx
@@ -484,6 +489,28 @@ static int build_tiles(mhd_ctx *c)
                         return rc;
                     HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
                 }
+            if (sh == 1)
+                for (int r = 0; r < R_COUNT; ++r)
+                    for (int ji = 0; ji < 3; ++ji)
+                    {
+                        TileArr &A = T.kl[r][ji];
+                        const int lo = kRangeLo[r], hi = kRangeHi[r], jt = kJt[ji];
+                        std::vector<int> start(std::max(T.nent, 1), 0);
+                        long long total = 0;
+                        for (int e = 0; e < T.nent; ++e)
+                        {
+                            const MhdBlockDev &d = c->h_blk[T.members[T.lead_moff[e]]];
+                            const int ei = d.ni - lo + hi, ej = d.nj - lo + hi, ek = d.nk - lo + hi;
+                            start[e] = (int)total;
+                            total += (long long)ei * ((ej * ek + 64 * jt - 1) / (64 * jt));
+                        }
+                        if (total > (1LL << 31) - 64)
+                            return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                        A.n = (int)total;
+                        if ((rc = grow(c, A.d_start, A.cap_start, start.size())))
+                            return rc;
+                        HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+                    }
             {
                 TileArr &A = T.cols;
                 const int tj = mhdk::col_tile_rows();
@@ -539,9 +566,10 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     const int r = kKernelRange[kernel];
     const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
     const int ji = v % 3, sh = v >= 6 ? 1 : 0;
+    const bool kl = v >= kVariants;
     const TileSet &S = c->ts[set_of(c, kernel)][sh];
     const int ch = chunk_of(kernel);
-    const TileArr &T = ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
+    const TileArr &T = kl ? S.kl[r][ji] : ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
     const bool ident = S.members.empty();
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
@@ -554,8 +582,9 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     L.tm.lo = kRangeLo[r];
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
-    L.tm.i_inner = (v % 6) >= 3;
+    L.tm.i_inner = !kl && (v % 6) >= 3;
     L.tm.ci = ch >= 0 ? kChunkCi[ch] : 0;
+    L.tm.kloop = kl;
     L.jt = kJt[ji];
     return L;
 }
@@ -572,6 +601,7 @@ static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -
     L.tm.xcd_chunk = c->opt.xcd_order ? (S.cols.n + 7) / 8 : 0;
     L.tm.i_inner = 0;
     L.tm.ci = kColCi;
+    L.tm.kloop = 0;
     L.jt = mhdk::col_tile_rows();
     return L;
 }
@@ -1110,7 +1140,8 @@ static int autotune(mhd_ctx *c)
             continue;
         double best = 1e30;
         int best_j = c->jt_idx[k];
-        for (int ji = 0; ji < kVariants; ++ji)
+        const int nv = loops_members(k) ? kLoopVariants : kVariants;
+        for (int ji = 0; ji < nv; ++ji)
         {
             auto launch = [&]() {
                 const mhdk::Launch L = launch_of(c, k, ji);
@@ -2086,9 +2117,10 @@ extern "C" int mhd_tuning(mhd_ctx *c, int kernel, int *rows, int *i_inner, int *
 {
     if (!c || !rows || !i_inner || !geo_shared || kernel < 0 || kernel >= KP_COUNT)
         return MHD_ERR_ARG;
-    *rows = kJt[c->jt_idx[kernel] % 3];
-    *i_inner = (c->jt_idx[kernel] % 6) >= 3;
-    *geo_shared = c->jt_idx[kernel] >= 6;
+    const int v = c->jt_idx[kernel];
+    *rows = kJt[v % 3];
+    *i_inner = v < kVariants && (v % 6) >= 3;
+    *geo_shared = v >= kVariants ? 2 : v >= 6;
     return MHD_OK;
 }
 
@@ -2100,9 +2132,12 @@ extern "C" int mhd_set_tuning(mhd_ctx *c, int kernel, int rows, int i_inner, int
     for (int q = 0; q < 3; ++q)
         if (kJt[q] == rows)
             ji = q;
-    if (ji < 0 || (i_inner != 0 && i_inner != 1) || (geo_shared != 0 && geo_shared != 1))
-        return fail(c, MHD_ERR_ARG, "launch shape: rows 1, 2 or 4; i_inner and geo_shared 0 or 1");
-    c->jt_idx[kernel] = ji + 3 * i_inner + 6 * geo_shared;
+    if (ji < 0 || (i_inner != 0 && i_inner != 1) || geo_shared < 0 || geo_shared > 2)
+        return fail(c, MHD_ERR_ARG, "launch shape: rows 1, 2 or 4; i_inner 0 or 1; geo_shared 0, 1 or 2");
+    if (geo_shared == 2 && (i_inner || !loops_members(kernel)))
+        return fail(c, MHD_ERR_ARG, std::string("launch shape: member-looping tiles (geo_shared 2) are j-inner and only for ") +
+                                        kKernelName[KP_DERIVED]);
+    c->jt_idx[kernel] = geo_shared == 2 ? kVariants + ji : ji + 3 * i_inner + 6 * geo_shared;
     if (c->gexec)
     {
         (void)hipStreamSynchronize(c->q);

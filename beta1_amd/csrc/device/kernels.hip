@@ -100,15 +100,17 @@ __device__ __forceinline__ bool tile_col(const MhdTileMap &tm, MhdBlockDev &b, i
     return true;
 }
 
-__device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, int &i, int &j, int &k, bool &wg)
+// Workgroup -> (entry, tile index within the entry); false for padding
+// workgroups of the XCD-ordered grid.
+__device__ __forceinline__ bool tile_entry(const MhdTileMap &tm, int &bi, int &lt)
 {
     int t = (int)blockIdx.x;
     if (tm.xcd_chunk > 0)
         t = (t & 7) * tm.xcd_chunk + (t >> 3);
-    wg = t < tm.ntiles;
-    if (!wg)
+    if (t >= tm.ntiles)
         return false;
-    int bi = 0, hi = tm.nblk - 1;
+    bi = 0;
+    int hi = tm.nblk - 1;
     while (bi < hi)
     {
         const int mid = (bi + hi + 1) >> 1;
@@ -117,18 +119,16 @@ __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, 
         else
             hi = mid - 1;
     }
-    int lt = t - tm.block_start[bi];
-    if (tm.members)
-    {
-        const int kk = tm.lead_k[bi];
-        const int m = lt % kk;
-        lt /= kk;
-        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
-    }
-    else
-        b = tm.blks[bi];
-    // a tile is 64 * rows consecutive points of one (j, k) plane of the range,
-    // k fastest, so every wavefront is full however the extents divide by 64
+    lt = t - tm.block_start[bi];
+    return true;
+}
+
+// The point (i, j, k) of tile lt of block b's launch range: a tile is 64 *
+// rows consecutive points of one (j, k) plane of the range, k fastest, so
+// every wavefront is full however the extents divide by 64.  False for
+// padding threads.
+__device__ __forceinline__ bool tile_point(const MhdTileMap &tm, const MhdBlockDev &b, int lt, int &i, int &j, int &k)
+{
     const int per_tile = 64 * (int)blockDim.y;
     const int ext_i = b.ni - tm.lo + tm.hi, ext_j = b.nj - tm.lo + tm.hi, ext_k = b.nk - tm.lo + tm.hi;
     const int plane = ext_j * ext_k;
@@ -142,6 +142,40 @@ __device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, 
     j = tm.lo + f / ext_k;
     k = tm.lo + f % ext_k;
     return f < plane;
+}
+
+__device__ __forceinline__ bool tile_cell(const MhdTileMap &tm, MhdBlockDev &b, int &i, int &j, int &k, bool &wg)
+{
+    int bi, lt;
+    wg = tile_entry(tm, bi, lt);
+    if (!wg)
+        return false;
+    if (tm.members)
+    {
+        const int kk = tm.lead_k[bi];
+        const int m = lt % kk;
+        lt /= kk;
+        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
+    }
+    else
+        b = tm.blks[bi];
+    return tile_point(tm, b, lt, i, j, k);
+}
+
+// Member-looping tiles of a geometry-shared map (tm.kloop): a tile is a point
+// tile of the entry's first case; the K cases of the entry are the blocks
+// tm.blks[tm.members[m0 + m]], m < K, which share its geometry and extents.
+__device__ __forceinline__ bool tile_cell_k(const MhdTileMap &tm, MhdBlockDev &b, int &i, int &j, int &k, bool &wg,
+                                            int &m0, int &K)
+{
+    int bi, lt;
+    wg = tile_entry(tm, bi, lt);
+    if (!wg)
+        return false;
+    m0 = tm.lead_moff[bi];
+    K = tm.lead_k[bi];
+    b = tm.blks[tm.members[m0]];
+    return tile_point(tm, b, lt, i, j, k);
 }
 
 // Plane-chunk tiles: as tile_cell, but a tile spans the planes
@@ -399,14 +433,11 @@ struct CellGeo
     double a[6][3];
 };
 
-__device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__restrict__ U,
-                                          const double *__restrict__ G, int64_t o, CellGeo &q)
+__device__ __forceinline__ void load_cell_geo(const MhdBlockDev &b, const double *__restrict__ G, int64_t o,
+                                              CellGeo &q)
 {
     const int64_t sl = b.slot;
     const int64_t step[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
-#pragma unroll
-    for (int m = 0; m < 5; ++m)
-        q.u[m] = U[m * sl + o];
     q.jac = G[(int64_t)G_JAC * sl + o];
 #pragma unroll
     for (int f = 0; f < 6; ++f)
@@ -420,24 +451,45 @@ __device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__
     }
 }
 
+__device__ __forceinline__ void load_cell(const MhdBlockDev &b, const double *__restrict__ U,
+                                          const double *__restrict__ G, int64_t o, CellGeo &q)
+{
+    const int64_t sl = b.slot;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        q.u[m] = U[m * sl + o];
+    load_cell_geo(b, G, o, q);
+}
+
 // B_cell = sum_n W[c][n] pt[n] (calc_Bcell, Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
 // 194-236) with Bcell_from_Bface_w formed in place from the stored
 // least-squares data: W[c][n] = Mi[c] . (mu_n s_n), s_n = -+ the area vector
 // of face n (q.a), in the host builder's order of operations.
+// The least-squares data of a cell's B_cell weights: Mi (6: 00, 01, 02, 11,
+// 12, 22) and mu of the six faces.
 template <typename WT>
-__device__ __forceinline__ void bcell_from_faces(const MhdBlockDev &b, int64_t o, const CellGeo &q, const double *pt,
-                                                 double *bc)
+__device__ __forceinline__ void load_bcell_w(const MhdBlockDev &b, int64_t o, double *mi, double *mu)
 {
     const int64_t sl = b.slot;
     const WT *Wm = dweights<WT>(b) + (int64_t)(G_WBM - G_NMETRIC) * sl;
     const WT *Wu = dweights<WT>(b) + (int64_t)(G_WBU - G_NMETRIC) * sl;
-    const double m00 = Wm[o], m01 = Wm[sl + o], m02 = Wm[2 * sl + o], m11 = Wm[3 * sl + o], m12 = Wm[4 * sl + o],
-                 m22 = Wm[5 * sl + o];
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+    {
+        mi[n] = Wm[(int64_t)n * sl + o];
+        mu[n] = (double)Wu[(int64_t)n * sl + o];
+    }
+}
+
+__device__ __forceinline__ void bcell_from_w(const double *mi, const double *muq, const CellGeo &q, const double *pt,
+                                             double *bc)
+{
+    const double m00 = mi[0], m01 = mi[1], m02 = mi[2], m11 = mi[3], m12 = mi[4], m22 = mi[5];
     bc[0] = bc[1] = bc[2] = 0.0;
 #pragma unroll
     for (int n = 0; n < 6; ++n)
     {
-        const double mu = (double)Wu[(int64_t)n * sl + o];
+        const double mu = muq[n];
         const double s0 = (n & 1) ? q.a[n][0] : -q.a[n][0];
         const double s1 = (n & 1) ? q.a[n][1] : -q.a[n][1];
         const double s2 = (n & 1) ? q.a[n][2] : -q.a[n][2];
@@ -451,7 +503,51 @@ __device__ __forceinline__ void bcell_from_faces(const MhdBlockDev &b, int64_t o
     }
 }
 
-__device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &q, double Bx, double By, double Bz)
+template <typename WT>
+__device__ __forceinline__ void bcell_from_faces(const MhdBlockDev &b, int64_t o, const CellGeo &q, const double *pt,
+                                                 double *bc)
+{
+    double mi[6], mu[6];
+    load_bcell_w<WT>(b, o, mi, mu);
+    bcell_from_w(mi, mu, q, pt, bc);
+}
+
+// |A_f| and the outward unit normal of each face (zero where |A_f| = 0):
+// geometry only, so a kernel updating several cases of one grid forms them
+// once per cell.
+struct FaceNormals
+{
+    double An[6], n[6][3];
+};
+__device__ __forceinline__ void face_normals(const CellGeo &q, FaceNormals &fn)
+{
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        const double axv = q.a[f][0], ayv = q.a[f][1], azv = q.a[f][2];
+        const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
+        fn.An[f] = An;
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        if (An > 0.0)
+        {
+            nx = axv / An;
+            ny = ayv / An;
+            nz = azv / An;
+            if (!(f & 1))
+            {
+                nx = -nx;
+                ny = -ny;
+                nz = -nz;
+            }
+        }
+        fn.n[f][0] = nx;
+        fn.n[f][1] = ny;
+        fn.n[f][2] = nz;
+    }
+}
+
+__device__ __forceinline__ double cell_dt_n(const MhdCaseConst &c, const CellGeo &q, const FaceNormals &fn, double Bx,
+                                            double By, double Bz)
 {
     double local = 1e100;
     const double V = fabs(q.jac);
@@ -466,25 +562,18 @@ __device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &
         const double p = (gm - 1.0) * eint;
         const double cs2 = gm * p / rho;
         const double B2 = Bx * Bx + By * By + Bz * Bz;
+        const double vA2 = c.inver_MA2 * B2 / rho;
         double denom = 0.0;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
         {
-            const double axv = q.a[f][0], ayv = q.a[f][1], azv = q.a[f][2];
-            const double An = sqrt(axv * axv + ayv * ayv + azv * azv);
+            const double An = fn.An[f];
             double term = 0.0;
             if (An > 0.0)
             {
-                double nx = axv / An, ny = ayv / An, nz = azv / An;
-                if (!(f & 1))
-                {
-                    nx = -nx;
-                    ny = -ny;
-                    nz = -nz;
-                }
+                const double nx = fn.n[f][0], ny = fn.n[f][1], nz = fn.n[f][2];
                 const double un = ux * nx + uy * ny + uz * nz;
                 const double Bn = Bx * nx + By * ny + Bz * nz;
-                const double vA2 = c.inver_MA2 * B2 / rho;
                 const double vAn2 = c.inver_MA2 * (Bn * Bn) / rho;
                 const double tt = cs2 + vA2;
                 double disc = tt * tt - 4.0 * cs2 * vAn2;
@@ -502,6 +591,14 @@ __device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &
         }
     }
     return local;
+}
+
+// Compute_Timestep's per-cell limit (LunarSolver_Timestep.cpp:8-140)
+__device__ __forceinline__ double cell_dt(const MhdCaseConst &c, const CellGeo &q, double Bx, double By, double Bz)
+{
+    FaceNormals fn;
+    face_normals(q, fn);
+    return cell_dt_n(c, q, fn, Bx, By, Bz);
 }
 
 __device__ __forceinline__ void reduce_dt(double local, unsigned long long *__restrict__ dtbits, int case_id)
@@ -1135,6 +1232,72 @@ __global__ __launch_bounds__(256) void k_derived(const MhdTileMap tm, const MhdC
         reduce_dt(local, dtbits, b.case_id);
 }
 
+// k_derived over a geometry-shared map with member-looping tiles: the
+// cell's geometry (Jac, the six face area vectors, the B_cell least-squares
+// data) is loaded once and the K cases sharing it are updated in turn with
+// k_derived's arithmetic, each case's dt reduced over the workgroup.
+template <typename WT>
+__global__ __launch_bounds__(256) void k_derived_k(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                   unsigned long long *__restrict__ dtbits, int ucur)
+{
+    MhdBlockDev b;
+    int i, j, k, m0 = 0, K = 0;
+    bool wg;
+    const bool ok = tile_cell_k(tm, b, i, j, k, wg, m0, K);
+    if (!wg) // a whole workgroup
+        return;
+    const int64_t sl = b.slot, o = ok ? blk_off(b, i, j, k) : 0;
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    CellGeo q;
+    FaceNormals fn;
+    double mi[6], mu[6];
+    if (ok)
+    {
+        load_cell_geo(b, dmetric(b), o, q);
+        load_bcell_w<WT>(b, o, mi, mu);
+        face_normals(q, fn);
+    }
+    const bool inner = ok && i >= 0 && j >= 0 && k >= 0 && i < b.ni && j < b.nj && k < b.nk;
+#pragma unroll 1
+    for (int m = 0; m < K; ++m)
+    {
+        const MhdBlockDev bm = tm.blks[tm.members[m0 + m]];
+        double local = 1e100;
+        if (ok)
+        {
+            double *S = dst_ptr(bm);
+            const double *U = S + (int64_t)(ST_U0 + 5 * ucur) * sl;
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                q.u[c] = U[c * sl + o];
+            {
+                double uc[3];
+                cell_ucontra(cst[bm.case_id].gamma, q, uc);
+                double *UC = S + (int64_t)ST_UC * sl;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    UC[a * sl + o] = uc[a];
+            }
+            const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+            const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+            const double pt[6] = {-Bx[o] - Ax[o],          Bx[o + di] + Ax[o + di], -By[o] - Ay[o],
+                                  By[o + dj] + Ay[o + dj], -Bz[o] - Az[o],          Bz[o + dk] + Az[o + dk]};
+            double bc[3];
+            bcell_from_w(mi, mu, q, pt, bc);
+            if (inner)
+            {
+                double *Bc = S + (int64_t)ST_BCELL * sl;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    Bc[a * sl + o] = bc[a];
+                local = cell_dt_n(cst[bm.case_id], q, fn, bc[0], bc[1], bc[2]);
+            }
+        }
+        reduce_dt(local, dtbits, bm.case_id);
+        __syncthreads(); // the reduction's shared slots are reused for the next case
+    }
+}
+
 // Derived fields, part 2: Calc_J_Edge (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
 // 777-830; CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:
 // 59-110), J = alpha (CurlAdj of the beta-weighted face fluxes), each edge
@@ -1364,11 +1527,13 @@ __global__ __launch_bounds__(256) void k_derived2(const MhdTileMap tm, const Mhd
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
                     Bc[a * sl + o] = bc[a];
-                local = cell_dt(c, q, bc[0], bc[1], bc[2]);
+                FaceNormals fn;
+                face_normals(q, fn);
+                local = cell_dt_n(c, q, fn, bc[0], bc[1], bc[2]);
 #pragma unroll
                 for (int m = 0; m < 5; ++m)
                     q.u[m] = un[m];
-                local = dmin(local, cell_dt(c, q, bc[0], bc[1], bc[2]));
+                local = dmin(local, cell_dt_n(c, q, fn, bc[0], bc[1], bc[2]));
             }
         }
     }
@@ -1396,9 +1561,11 @@ __device__ __forceinline__ TwoFluidCell two_fluid_cell(const MhdCaseConst &c, co
     const double *UH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *UN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
     t.num = num_limiter(c, UH[o], UN[o]);
     {
-        // U_plus of this cell (calc_Uplus), from the same U the step started from
+        // U_plus of this cell (calc_Uplus), from the same U the step started
+        // from; its limiter of the clamped densities is t.num (num_limiter
+        // clamps at 0 itself, and dmax(dmax(x, 0), 0) == dmax(x, 0) for every x)
         const double rH = dmax(UH[o], 0.0), rN = dmax(UN[o], 0.0);
-        const NumInfo n = num_limiter(c, rH, rN);
+        const NumInfo &n = t.num;
         const bool hH = rH > 1e-20, hN = rN > 1e-20;
         t.upx = n.chiH * (hH ? ph.u : 0.0) + n.chiNa * (hN ? pn.u : 0.0);
         t.upy = n.chiH * (hH ? ph.v : 0.0) + n.chiNa * (hN ? pn.v : 0.0);
@@ -2264,6 +2431,14 @@ void launch_jcell(const Launch &L, int mixed, hipStream_t q)
 void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
                     hipStream_t q)
 {
+    if (L.tm.kloop)
+    {
+        if (mixed)
+            hipLaunchKernelGGL(k_derived_k<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+        else
+            hipLaunchKernelGGL(k_derived_k<double>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
+        return;
+    }
     if (mixed)
         hipLaunchKernelGGL(k_derived<float>, grid_of(L), block_of(L), 0, q, L.tm, cst, dtbits, ucur);
     else

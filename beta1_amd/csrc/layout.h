@@ -214,6 +214,8 @@ struct MhdTileMap
     int xcd_chunk;
     int i_inner; // 1: consecutive tiles step along i, 0: along the (j, k) plane
     int ci;      // column and plane-chunk tiles: planes per workgroup along i
+    int kloop;   // geometry-shared map: 1 = a tile covers every case of its entry (the kernel
+                 // loops over them with the geometry loaded once), 0 = one case per tile
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)

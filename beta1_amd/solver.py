@@ -381,16 +381,19 @@ class MhdContext:
         return bool(v.value)
 
     def tuning(self):
-        """kernel -> (workgroup rows, tile order, "geo-shared" or "per-case")"""
+        """kernel -> (workgroup rows, tile order, "per-case", "geo-shared" (cases of one
+        grid back to back per tile) or "geo-loop" (one tile updates every case of its grid))"""
         out = {}
         for k, name in enumerate(self.profile_read()):
             rows, inner, sh = C.c_int(), C.c_int(), C.c_int()
             _check(lib().mhd_tuning(self.ctx, k, C.byref(rows), C.byref(inner), C.byref(sh)), self.ctx, "tuning")
-            out[name] = (rows.value, "i-inner" if inner.value else "j-inner", "geo-shared" if sh.value else "per-case")
+            out[name] = (rows.value, "i-inner" if inner.value else "j-inner",
+                         ("per-case", "geo-shared", "geo-loop")[sh.value])
         return out
 
     def set_tuning(self, kernel, rows, i_inner=False, geo_shared=False):
-        """Pin a kernel's launch shape (kernel by name, as in tuning())."""
+        """Pin a kernel's launch shape (kernel by name, as in tuning());
+        geo_shared 0/False, 1/True or 2 (member-looping tiles, k_derived)."""
         k = list(self.profile_read()).index(kernel)
         _check(lib().mhd_set_tuning(self.ctx, k, rows, int(i_inner), int(geo_shared)), self.ctx, "set_tuning")
 

@@ -327,9 +327,12 @@ int mhd_memory(mhd_ctx *ctx, double *bytes);
    workgroup height (rows of 64 threads along j), tile order (i_inner 1:
    consecutive tiles step along i before the next row group of j) and
    geo_shared (1: the cases sharing one geometry set run each geometry tile
-   back to back, so it is fetched from HBM once for all of them) */
+   back to back, so it is fetched from HBM once for all of them while it is
+   in L2; 2: one workgroup loads the geometry tile into registers and updates
+   every case sharing it in turn -- k_derived only, i_inner 0) */
 int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner, int *geo_shared);
-/* pin a kernel's launch shape (rows 1, 2 or 4; i_inner, geo_shared 0/1) */
+/* pin a kernel's launch shape (rows 1, 2 or 4; i_inner 0/1; geo_shared 0, 1
+   or 2 as above) */
 int mhd_set_tuning(mhd_ctx *ctx, int kernel, int rows, int i_inner, int geo_shared);
 /* Re-run the launch-shape tuner on the cases resident now (the tuner of
    mhd_options.autotune runs once, at mhd_finalize).  No counterpart in the

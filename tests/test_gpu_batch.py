@@ -144,6 +144,46 @@ def test_geometry_shared_tiles_leave_each_case_as_alone(precision):
             same(ctx, cid, alone(d, 8, precision=precision))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("rows", [1, 4])
+def test_member_looping_derived_leaves_each_case_as_alone(precision, rows):
+    """k_derived with member-looping tiles: one workgroup loads a geometry
+    tile once and updates the three shells sharing it in turn (each case's dt
+    reduced on its own); every case evolves bit for bit as alone, also after a
+    member retires.  Only k_derived has the variant, j-inner."""
+    from beta1_amd.capi import MhdError
+
+    ctx = MhdContext(precision=precision)
+    ids = [ctx.add(d) for d in SHARED]
+    ctx.finalize()
+    ctx.set_tuning("k_derived", rows=rows, geo_shared=2)
+    assert ctx.tuning()["k_derived"] == (rows, "j-inner", "geo-loop")
+    for bad in (dict(kernel="k_jcell", rows=rows, geo_shared=2), dict(kernel="k_derived", rows=rows, i_inner=True,
+                                                                       geo_shared=2)):
+        with pytest.raises(MhdError):
+            ctx.set_tuning(**bad)
+    ctx.step(3, graph=True)
+    ctx.step(2)
+    for d, cid in zip(SHARED, ids):
+        same(ctx, cid, alone(d, 5, precision=precision))
+        assert ctx.last_dt(cid) == alone_dt(d, 5, precision)
+    ctx.retire(ids[0])
+    ctx.step(3, graph=True)
+    for d, cid in zip(SHARED, ids):
+        if cid != ids[0]:
+            same(ctx, cid, alone(d, 8, precision=precision))
+
+
+def alone_dt(desc, steps, precision):
+    ctx = MhdContext(precision=precision)
+    cid = ctx.add(desc)
+    ctx.finalize()
+    ctx.step(steps)
+    dt = ctx.last_dt(cid)
+    ctx.close()
+    return dt
+
+
 def test_autotune_after_admission_keeps_results():
     ctx = MhdContext(autotune=True)
     ids = [ctx.add(d) for d in SHARED[:3]]
