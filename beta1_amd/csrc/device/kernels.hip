@@ -2171,11 +2171,51 @@ __device__ __forceinline__ void apply_op(const MhdOp &op, int e, const MhdCaseCo
 
 // Records of a list on 2^lg lanes each (lg 0: a thread per record, lg 6: a
 // wavefront per record); the lanes of a record take its elements in turn.
+// Copies (and negated copies) of up to 5 components: a lane loads every
+// component of two elements before it stores any.  Written through one
+// pointer type the compiler must assume a store may feed a later load and
+// would order each component's load after the previous store; the
+// destinations of a list never overlap its sources, so loading first gives
+// the same values without the serialised round trips.
+constexpr int kCopyComp = 5;
+__device__ __forceinline__ void copy_run(const MhdOp &op, int e0, int lanes, int n, int ushift)
+{
+    const int64_t sh = ushift ? 5 * op.stride : 0;
+    const bool neg = op.kind == OP_NEG;
+    for (int e = e0; e < n; e += 2 * lanes)
+    {
+        const bool two = e + lanes < n;
+        const double *s0 = gptr(op.src) + sh + (int64_t)e * op.sstep, *s1 = s0 + (int64_t)lanes * op.sstep;
+        double *d0 = gptr(op.dst) + sh + (int64_t)e * op.dstep, *d1 = d0 + (int64_t)lanes * op.dstep;
+        double v0[kCopyComp], v1[kCopyComp];
+#pragma unroll
+        for (int c = 0; c < kCopyComp; ++c)
+            if (c < op.ncomp)
+            {
+                v0[c] = s0[c * op.stride];
+                v1[c] = two ? s1[c * op.stride] : 0.0;
+            }
+#pragma unroll
+        for (int c = 0; c < kCopyComp; ++c)
+            if (c < op.ncomp)
+            {
+                d0[c * op.stride] = neg ? -v0[c] : v0[c];
+                if (two)
+                    d1[c * op.stride] = neg ? -v1[c] : v1[c];
+            }
+    }
+}
+
 __device__ __forceinline__ void apply_rec(const MhdOp *__restrict__ ops, int t, int lg,
                                           const MhdCaseConst *__restrict__ cst, int ushift)
 {
     const MhdOp &op = ops[t >> lg];
     const int lanes = 1 << lg, n = op.run > 1 ? op.run : 1;
+    if ((op.kind == OP_COPY || op.kind == OP_NEG) && op.ncomp <= kCopyComp)
+    {
+        copy_run(op, t & (lanes - 1), lanes, n, ushift);
+        return;
+    }
     for (int e = t & (lanes - 1); e < n; e += lanes)
         apply_op(op, e, cst, ushift);
 }

@@ -7,6 +7,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
+#include <vector>
 
 #include "formats.h"
 #include "host_case.h"
@@ -390,6 +392,43 @@ extern "C" int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n)
     if (!h || !n || sync < 0 || sync >= SY_COUNT)
         return MHD_ERR_ARG;
     *n = h->cs.ops[sync].size();
+    return MHD_OK;
+}
+
+// The condition under which the device applies a list's records (and the
+// elements of a run) in any order, loading every source before its stores:
+// no element is written twice and no written element is read by the list.
+extern "C" int mhd_host_ops_independent(mhd_host_case *h, int sync, int *independent)
+{
+    if (!h || !independent || sync < 0 || sync >= SY_COUNT)
+        return MHD_ERR_ARG;
+    const std::vector<MhdOp> &ops = h->cs.ops[sync];
+    std::unordered_set<int64_t> dst;
+    bool ok = true;
+    for (const MhdOp &o : ops)
+    {
+        const int n = o.run > 1 ? o.run : 1;
+        const int nc = o.kind == OP_CONST ? 1 : o.kind == OP_NA_FLOOR || o.kind == OP_SOLID_FIRST ||
+                                                         o.kind == OP_SOLID_GHOST ? 5 : o.ncomp;
+        for (int e = 0; e < n && ok; ++e)
+            for (int c = 0; c < nc; ++c)
+                if (!dst.insert(o.dst + (int64_t)e * o.dstep + (int64_t)c * o.stride).second)
+                    ok = false;
+    }
+    for (const MhdOp &o : ops)
+    {
+        if (!ok)
+            break;
+        if (o.kind != OP_COPY && o.kind != OP_NEG && o.kind != OP_SOLID_FIRST && o.kind != OP_SOLID_GHOST)
+            continue;
+        const int n = o.run > 1 ? o.run : 1;
+        const int nc = o.kind == OP_COPY || o.kind == OP_NEG ? o.ncomp : 5;
+        for (int e = 0; e < n && ok; ++e)
+            for (int c = 0; c < nc; ++c)
+                if (dst.count(o.src + (int64_t)e * o.sstep + (int64_t)c * o.stride))
+                    ok = false;
+    }
+    *independent = ok ? 1 : 0;
     return MHD_OK;
 }
 

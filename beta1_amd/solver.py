@@ -241,6 +241,12 @@ class HostCase:
         lib().mhd_host_op_count(self.h, sync, C.byref(n))
         return n.value
 
+    def ops_independent(self, sync):
+        """True when op list `sync` writes no element twice and reads none it writes."""
+        v = C.c_int()
+        _check(lib().mhd_host_ops_independent(self.h, sync, C.byref(v)), what="ops_independent")
+        return bool(v.value)
+
 
 def _parts_args(blocks, ghost):
     """ctypes arguments of mhd_(host_)add_block / set_topology for each block

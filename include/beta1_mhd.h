@@ -385,6 +385,11 @@ int mhd_host_constants_two_fluid(mhd_host_case *h, double *out31);
    arrays */
 int mhd_host_apply_sync(mhd_host_case *h, int sync);
 int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n);
+/* 1 in *independent when no element of op list `sync` is written twice and
+   none it writes is read by the same list (the device applies a list's
+   records in any order and loads an element's sources before its stores;
+   no counterpart in the reference, whose Sync loops run in order) */
+int mhd_host_ops_independent(mhd_host_case *h, int sync, int *independent);
 
 #ifdef __cplusplus
 }

@@ -193,3 +193,28 @@ def test_blocks_of_unequal_extent():
     for b, ni in enumerate((4, 6)):
         a, r = orc.view("U_H", b)[G:G + ni, G:-G, G:-G], U0[b][G:G + ni, G:-G, G:-G]
         assert np.abs(a - r).max() <= 1e-13 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("make", [
+    lambda: HostCase(WindDesc(nang=6, nrad=8), ghost=G),
+    lambda: HostCase(OTDesc(n=8), ghost=G),
+    lambda: HostCase(WindDesc(nang=6, nrad=8), ghost=3),
+    lambda: HostCase.from_parts(*two_box_parts(), ghost=G),
+])
+def test_op_lists_are_order_independent(make):
+    """Every op list of a case writes each element once and reads none it
+    writes, so the device may apply its records in any order and load an
+    element's sources before its stores."""
+    hc = make()
+    for sy in range(14):
+        if hc.op_count(sy):
+            assert hc.ops_independent(sy), sy
+
+
+def test_mercury_op_lists_are_order_independent():
+    from beta1_amd import MercuryDesc
+    hc = HostCase(MercuryDesc(nang=6, nrad=8), ghost=G)
+    assert hc.op_count(10) and hc.op_count(12)
+    for sy in range(14):
+        if hc.op_count(sy):
+            assert hc.ops_independent(sy), sy
