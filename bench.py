@@ -178,14 +178,15 @@ def bench_serve(args):
 
     prec = args.precision
     # the request list at unit rate, rescaled below to the offered load
-    reqs = poisson_requests(args.requests, 1.0, seed=SEED, steps=args.req_steps)
+    reqs = poisson_requests(args.requests, 1.0, seed=SEED, steps=args.req_steps, mercury_r_in=args.mercury_r_in)
     mean_work = float(np.mean([r.cells for r in reqs])) * args.req_steps
     eng = ServingEngine(precision=prec, hbm_budget_gb=args.budget_gb, chunk=20, workers=2)
     from beta1_amd import HostCase, MercuryDesc, WindDesc
     # capacity of each model on a 6x64^2x128 shell alone: a request's service
     # time is its cell-updates at its model's rate
     cap = {}
-    for name, d in (("lunar", WindDesc(nang=64, nrad=128)), ("mercury", MercuryDesc(nang=64, nrad=128))):
+    for name, d in (("lunar", WindDesc(nang=64, nrad=128)),
+                    ("mercury", MercuryDesc(nang=64, nrad=128, r_in=args.mercury_r_in))):
         cal = eng.ctx.adopt(HostCase(d))
         eng.ctx.step(4, graph=True)
         eng.ctx.sync()
@@ -209,9 +210,12 @@ def bench_serve(args):
                    "Mcell-updates/s", args.req_steps, 0, None, "f32/f64" if prec == "mixed" else "f64",
                    {"workload": f"serve_poisson_{args.requests}req_{prec}", "requests": args.requests,
                     "rate_hz": round(rate, 4), "offered_load": round(offered, 3), "mix": "70% Lunar / 30% Mercury",
-                    "steps_per_request": args.req_steps, "hbm_budget_gb": args.budget_gb, "seed": SEED},
+                    "steps_per_request": args.req_steps, "hbm_budget_gb": args.budget_gb, "seed": SEED,
+                    "mercury_body": f"conducting: shell from r = {args.mercury_r_in} R_M, interior resistivity "
+                                    f"(0.84 < r < 1.04), 1 explicit resistive subcycle"
+                                    if args.mercury_r_in < 1.0 else "absorbing insulating wall at r = 1"},
                    latency={"p50_s": round(res["p50_s"], 3), "p99_s": round(res["p99_s"], 3),
-                            "wall_s": round(res["wall_s"], 2)})
+                            "wall_s": round(res["wall_s"], 2), "requests_failed": res["failed"]})
 
 
 def bench_ot(args):
@@ -414,6 +418,8 @@ def main():
     ap.add_argument("--req-steps", type=int, default=1000)
     ap.add_argument("--load", type=float, default=0.8, help="offered load of the request stream")
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--mercury-r-in", type=float, default=0.85,
+                    help="inner radius of the served Mercury shells (< 1: conducting interior)")
     args = ap.parse_args()
     big = args.workload == "precision"
     args.nang = args.nang or (128 if big else 64)

@@ -589,6 +589,22 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     return L;
 }
 
+// Plane-chunk launch (kChunkCi[ch] planes per tile) of a kernel's range and
+// block set, per-case entries, with its tuned workgroup rows.
+static mhdk::Launch launch_of_chunk(const mhd_ctx *c, int kernel, int ch)
+{
+    const int ji = c->jt_idx[kernel] % 3;
+    mhdk::Launch L = launch_of(c, kernel, ji);
+    const TileArr &T = c->ts[set_of(c, kernel)][0].chunks[ch][kKernelRange[kernel]][ji];
+    L.tm.block_start = T.d_start;
+    L.tm.ntiles = T.n;
+    L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
+    L.tm.i_inner = 0;
+    L.tm.ci = kChunkCi[ch];
+    L.tm.kloop = 0;
+    return L;
+}
+
 // Column-tile launch of a kernel's block set (per-case or geometry-shared
 // entries as the kernel's tuned variant says).
 static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -1)
@@ -1740,7 +1756,7 @@ extern "C" int mhd_divb_max(mhd_ctx *c, int case_id, double *out)
     if (!out || !active_case(c, case_id))
         return MHD_ERR_ARG;
     mhdk::launch_fill_u64(c->d_divb, c->case_cap, 0ull, c->q);
-    mhdk::launch_divb(launch_of(c, KP_DT), c->d_divb, mixed_of(c), c->q);
+    mhdk::launch_divb(launch_of_chunk(c, KP_DT, 1), c->d_divb, mixed_of(c), c->q);
     unsigned long long bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_divb + case_id, 8, hipMemcpyDeviceToHost, c->q));
     HIPCHK(c, hipStreamSynchronize(c->q));

@@ -127,9 +127,13 @@ class ServingEngine:
         }
 
 
-def poisson_requests(n, rate_hz, seed=20261203, steps=1000, sizes=None, mercury_frac=0.3, arrival_offset=0.0):
+def poisson_requests(n, rate_hz, seed=20261203, steps=1000, sizes=None, mercury_frac=0.3, arrival_offset=0.0,
+                     mercury_r_in=0.85):
     """Poisson arrivals at `rate_hz`; 70/30 Lunar/Mercury solar-wind cases with
-    grids from 6x32^2x64 to 6x128^2x256."""
+    grids from 6x32^2x64 to 6x128^2x256.  The Mercury shells reach inside the
+    planet to `mercury_r_in` (< 1: a conducting body, the reference's interior
+    resistivity inver_Rem yita0(r) acting on 0.84 < r < 1.04 with its explicit
+    subcycle, Z5_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:79-141)."""
     from .solver import WindDesc, MercuryDesc
 
     rng = np.random.default_rng(seed)
@@ -140,7 +144,7 @@ def poisson_requests(n, rate_hz, seed=20261203, steps=1000, sizes=None, mercury_
         t += rng.exponential(1.0 / rate_hz)
         nang, nrad = sizes[rng.integers(len(sizes))]
         if rng.random() < mercury_frac:
-            desc = MercuryDesc(nang=nang, nrad=nrad, v_kms=float(rng.uniform(300, 500)),
+            desc = MercuryDesc(nang=nang, nrad=nrad, r_in=mercury_r_in, v_kms=float(rng.uniform(300, 500)),
                                n_cc=float(rng.uniform(20, 60)))
             kind = "mercury"
         else:

@@ -172,15 +172,16 @@ def test_ghost_depth_below_two_is_rejected():
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
 def test_served_results_match_the_oracle(precision, tol):
     """Requests served by continuous batching (Lunar and two-fluid Mercury,
-    mixed sizes, a budget that holds about two at a time) against the FP64
-    oracle run for the same number of steps."""
+    one of them with the conducting interior r_in = 0.85, mixed sizes, a
+    budget that holds about two at a time) against the FP64 oracle run for
+    the same number of steps."""
     from beta1_amd import MercuryDesc
     from beta1_amd.fields import interior
     from beta1_amd.serve import Request, ServingEngine
     from oracle_bind import Oracle
 
     descs = [WindDesc(nang=8, nrad=16, v_kms=380.0), MercuryDesc(nang=8, nrad=16, n_cc=30.0),
-             WindDesc(nang=12, nrad=16, n_cc=9.0), MercuryDesc(nang=8, nrad=24, v_kms=450.0)]
+             WindDesc(nang=12, nrad=16, n_cc=9.0), MercuryDesc(nang=8, nrad=24, v_kms=450.0, r_in=0.85)]
     probe = MhdContext(precision=precision)
     probe.add(descs[3])
     probe.finalize()
