@@ -2239,22 +2239,30 @@ __global__ __launch_bounds__(256) void k_ops(const mhdk::OpLaunch L, const MhdCa
 }
 
 // div B: calc_divB (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:685-775), max |.| per case
+// Over plane-chunk tiles: a thread takes the max over its column of planes,
+// so a workgroup's one atomicMax covers 8 planes (the per-cell form spent
+// its time queueing 12 k atomics on one address per case).
 __global__ __launch_bounds__(256) void k_divb(const MhdTileMap tm, unsigned long long *__restrict__ out)
 {
     MhdBlockDev b;
-    int i, j, k;
+    int i0, i1, j, k;
     bool wg;
-    const bool ok = tile_cell(tm, b, i, j, k, wg);
+    const bool ok = tile_planes(tm, b, i0, i1, j, k, wg);
     double v = 0.0;
     if (ok)
     {
-        const int64_t sl = b.slot, o = blk_off(b, i, j, k);
+        const int64_t sl = b.slot, di = (int64_t)b.sj * b.sk;
         const double *Bx = dst_ptr(b) + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-        const double dv = (Bx[o + (int64_t)b.sj * b.sk] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) /
-                          dmetric(b)[(int64_t)G_JAC * sl + o];
-        v = fabs(dv);
-        if (!(v == v))
-            v = 1e300;
+        const double *Jac = dmetric(b) + (int64_t)G_JAC * sl;
+        int64_t o = blk_off(b, i0, j, k);
+        for (int i = i0; i < i1; ++i, o += di)
+        {
+            const double dv = (Bx[o + di] - Bx[o] + By[o + b.sk] - By[o] + Bz[o + 1] - Bz[o]) / Jac[o];
+            double a = fabs(dv);
+            if (!(a == a))
+                a = 1e300;
+            v = dmax(v, a);
+        }
     }
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1)
