@@ -34,7 +34,7 @@ struct CaseRec
     int state = 0;        // 0 pending, 1 active, 2 retired, 3 draft (mhd_case_begin .. mhd_case_end)
     int first_block = -1; // index into the block table while active
     // segments in the op sub-lists (kOpCls per sync, see OpGroup)
-    size_t op_off[SY_COUNT * 2] = {0}, op_n[SY_COUNT * 2] = {0};
+    size_t op_off[SY_COUNT * 4] = {0}, op_n[SY_COUNT * 4] = {0};
     long long steps = 0;
     size_t st_bytes = 0;
     mhd_host_case *draft = nullptr; // the blocks given so far (state 3)
@@ -67,12 +67,14 @@ struct TileArr
 
 // Op list of one sync group on the device: the cases' ops back to back,
 // each case's segment appended when it is placed and cut out when it retires.
-// The records of one sync, in two sub-lists by run length: runs shorter than
+// The records of one sync, in sub-lists by run length: runs shorter than
 // kLongRun on a thread each, longer ones on a wavefront each (its lanes take
 // the run's elements in turn), so a list mixing single points and long rows
-// has no thread walking a row alone.
-constexpr int kOpCls = 2, kLongRun = 16;
-static int op_cls(const MhdOp &o) { return o.run >= kLongRun ? 1 : 0; }
+// has no thread walking a row alone; and apart, the records the step's
+// kernels apply themselves (MhdOp::fold), which only refresh and the
+// per-group syncs run.
+constexpr int kOpCls = 4, kLongRun = 16;
+static int op_cls(const MhdOp &o) { return (o.fold ? 2 : 0) + (o.run >= kLongRun ? 1 : 0); }
 struct OpGroup
 {
     MhdOp *d = nullptr;
@@ -199,6 +201,9 @@ struct mhd_ctx
     // per-stage entry points: mhd_fluid_rhs has written the next U buffer and
     // mhd_apply_update has not yet made it current
     bool fluid_pending = false;
+    // enqueue_step is recording a whole step: its kernels fold the k-face
+    // boundary work (MhdTileMap::fold) and its syncs skip those records
+    bool in_step = false;
 
     bool prof = false;
     double prof_ms[KP_COUNT] = {0};
@@ -585,6 +590,7 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     L.tm.i_inner = !kl && (v % 6) >= 3;
     L.tm.ci = ch >= 0 ? kChunkCi[ch] : 0;
     L.tm.kloop = kl;
+    L.tm.fold = c->in_step ? 1 : 0;
     L.jt = kJt[ji];
     return L;
 }
@@ -902,30 +908,64 @@ struct ProfScope
     }
 };
 
-static void add_sync(const mhd_ctx *c, mhdk::OpLaunch &L, int sy, int ushift)
+// `folded`: also the records the step's kernels apply themselves.
+static void add_sync(const mhd_ctx *c, mhdk::OpLaunch &L, int sy, int ushift, bool folded)
 {
     for (int cl = 0; cl < kOpCls; ++cl)
     {
         const OpGroup &G = c->og[sy * kOpCls + cl];
-        if (G.n)
-            L.seg[L.nseg++] = {G.d, (int)G.n, cl ? 6 : 0, ushift};
+        if (G.n && (folded || cl < 2))
+            L.seg[L.nseg++] = {G.d, (int)G.n, (cl & 1) ? 6 : 0, ushift};
     }
 }
-static void run_ops(mhd_ctx *c, int sy, int ushift)
+static bool has_ops(const mhd_ctx *c, int sy, bool folded)
+{
+    for (int cl = 0; cl < (folded ? kOpCls : 2); ++cl)
+        if (c->og[sy * kOpCls + cl].n)
+            return true;
+    return false;
+}
+static void run_ops(mhd_ctx *c, int sy, int ushift, bool folded = true)
 {
     mhdk::OpLaunch L{};
-    add_sync(c, L, sy, ushift);
+    add_sync(c, L, sy, ushift, folded);
     mhdk::launch_ops(L, c->d_cst, c->q);
 }
 // Two op lists in one launch: only for lists with disjoint destinations of
 // which neither reads what the other writes (U-slot and B-slot lists, the
 // B_cell and J_cell ghost lists).
-static void run_ops2(mhd_ctx *c, int sya, int ushift_a, int syb, int ushift_b)
+static void run_ops2(mhd_ctx *c, int sya, int ushift_a, int syb, int ushift_b, bool folded = true)
 {
     mhdk::OpLaunch L{};
-    add_sync(c, L, sya, ushift_a);
-    add_sync(c, L, syb, ushift_b);
+    add_sync(c, L, sya, ushift_a, folded);
+    add_sync(c, L, syb, ushift_b, folded);
     mhdk::launch_ops(L, c->d_cst, c->q);
+}
+
+// The U and B syncs of a step: two chains of dependent lists (U wall first
+// layer, U wall ghosts, U halo; B wall, B halo), the k-face work the kernels
+// folded left out (U's only with the first-order fluid kernel); the lists
+// left in each chain go out in order, the two chains side by side, so a
+// shell whose only walls are k faces needs one launch.
+static void run_ub_chains(mhd_ctx *c, bool fold_u)
+{
+    const int uc[3] = {SY_U_BC0, SY_U_BC1, SY_U_HALO}, bc[2] = {SY_B_BC, SY_B_HALO};
+    int u[3], nu = 0, bb[2], nbb = 0;
+    for (int q : uc)
+        if (has_ops(c, q, !fold_u))
+            u[nu++] = q;
+    for (int q : bc)
+        if (has_ops(c, q, false))
+            bb[nbb++] = q;
+    for (int t = 0; t < std::max(nu, nbb); ++t)
+    {
+        mhdk::OpLaunch L{};
+        if (t < nu)
+            add_sync(c, L, u[t], c->ucur, !fold_u);
+        if (t < nbb)
+            add_sync(c, L, bb[t], 0, false);
+        mhdk::launch_ops(L, c->d_cst, c->q);
+    }
 }
 
 // Derived fields (contravariant velocities, B_cell and the next dt, then
@@ -1074,7 +1114,7 @@ static void stage_resistive(mhd_ctx *c)
         mhdk::launch_dt_res(c->d_dtres, c->d_dt, c->d_cst, c->case_cap, isub, c->q);
         {
             ProfScope p(c, KP_OPS);
-            run_ops(c, SY_B_BC, 0);
+            run_ops(c, SY_B_BC, 0, !c->in_step); // in a step the curl kernel has copied the k-face wall ghosts
             run_ops(c, SY_B_HALO, 0);
         }
         {
@@ -1100,7 +1140,7 @@ static void stage_resistive(mhd_ctx *c)
             mhdk::launch_curl_update(launch_of(c, KP_CURL_RES), c->d_dtres, c->q);
         }
         ProfScope p(c, KP_OPS);
-        run_ops(c, SY_B_BC, 0);
+        run_ops(c, SY_B_BC, 0, !c->in_step);
         run_ops(c, SY_B_HALO, 0);
     }
 }
@@ -1110,6 +1150,7 @@ static void stage_resistive(mhd_ctx *c)
 // placement).
 static void enqueue_step(mhd_ctx *c)
 {
+    c->in_step = true;
     stage_dt(c);
     // The fluid update reads U[cur], B_cell, J_cell, p_e and dt and writes
     // U[1-cur]; the EMF / curl / resistive chain reads U[cur] and writes
@@ -1128,11 +1169,10 @@ static void enqueue_step(mhd_ctx *c)
         // U and B syncs: each field physical -> halo (the U wall ghosts after
         // the wall's first layer); the U-slot and B-slot lists share launches
         ProfScope p(c, KP_OPS);
-        run_ops2(c, SY_U_BC0, c->ucur, SY_B_BC, 0);
-        run_ops2(c, SY_U_BC1, c->ucur, SY_B_HALO, 0);
-        run_ops(c, SY_U_HALO, c->ucur);
+        run_ub_chains(c, c->opt.recon == 0);
     }
     derived(c);
+    c->in_step = false;
 }
 
 // Autotuner: per kernel, time each workgroup height (64x1, 64x2, 64x4) on the

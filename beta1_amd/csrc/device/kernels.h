@@ -54,7 +54,7 @@ struct OpSeg
     const mpcns::MhdOp *ops;
     int n, lg, ushift;
 };
-constexpr int OPSEG_MAX = 4;
+constexpr int OPSEG_MAX = 8;
 struct OpLaunch
 {
     OpSeg seg[OPSEG_MAX];

@@ -373,6 +373,31 @@ __device__ __forceinline__ void rusanov_pre(const FluxCell &L, const FluxCell &R
     }
 }
 
+// Absorbing-wall state of BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184):
+// d = (rho_r, 0, 0, 0, p_r / (gamma - 1)) from the cell state s; the wall op
+// lists and the fluid kernel's folded k-face walls share it.
+__device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
+{
+    rho = dmax(rho, 2.5e-3);
+    const double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
+    double eint = E - ke;
+    if (eint < 0.0)
+        eint = 0.0;
+    const double p = (gm - 1.0) * eint;
+    return dmax(p, 1e-12);
+}
+__device__ __forceinline__ void solid_state(double gm, const double *s, double *d)
+{
+    const double rho_r = dmax(s[0], 2.5e-3);
+    const double p_r = solid_p(gm, rho_r, s[1], s[2], s[3], s[4]);
+    d[0] = rho_r;
+    d[1] = rho_r * 0.0;
+    d[2] = rho_r * 0.0;
+    d[3] = rho_r * 0.0;
+    const double ke_g = 0.5 * rho_r * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+    d[4] = p_r / (gm - 1.0) + ke_g;
+}
+
 template <typename T> __device__ __forceinline__ T minmod(T a, T b)
 {
     if (a * b <= (T)0)
@@ -926,12 +951,37 @@ __global__ __launch_bounds__(256) void k_curl_update(const MhdTileMap tm, const 
     const double ry = mult * ((Ex[o + dk] - Ex[o]) + (Ez[o] - Ez[o + di]));
     const double rz = mult * ((Ex[o] - Ex[o + dj]) + (Ey[o + di] - Ey[o]));
     const double bx = Bx[o], by = By[o], bz = Bz[o];
-    if (i <= b.ni && j < b.nj && k < b.nk)
-        Bx[o] = bx + dt * rx;
-    if (i < b.ni && j <= b.nj && k < b.nk)
-        By[o] = by + dt * ry;
-    if (i < b.ni && j < b.nj && k <= b.nk)
-        Bz[o] = bz + dt * rz;
+    const double nbx = bx + dt * rx, nby = by + dt * ry, nbz = bz + dt * rz;
+    const bool sx = i <= b.ni && j < b.nj, sy = i < b.ni && j <= b.nj, sz = i < b.ni && j < b.nj;
+    if (sx && k < b.nk)
+        Bx[o] = nbx;
+    if (sy && k < b.nk)
+        By[o] = nby;
+    if (sz && k <= b.nk)
+        Bz[o] = nbz;
+    // solid walls on the k faces (folds_kface_b): the ghost faces copy the
+    // inner layer -- the cell layer for B_xi, B_eta, the wall face for B_zeta
+    // (Boundary_Build.cpp:29-33 with the Solid_Surface handler)
+    if (tm.fold && folds_kface_b(b, 4) && k == 0)
+        for (int l = 1; l <= b.g; ++l)
+        {
+            if (sx)
+                Bx[o - l] = nbx;
+            if (sy)
+                By[o - l] = nby;
+            if (sz)
+                Bz[o - l] = nbz;
+        }
+    if (tm.fold && folds_kface_b(b, 5))
+        for (int l = 1; l <= b.g; ++l)
+        {
+            if (k == b.nk - 1 && sx)
+                Bx[o + l] = nbx;
+            if (k == b.nk - 1 && sy)
+                By[o + l] = nby;
+            if (k == b.nk && sz)
+                Bz[o + l] = nbz;
+        }
 }
 
 // ======================================================================
@@ -1086,9 +1136,13 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
     // the open cell (plane pl - 1): lower i-face flux, j and k terms, 1/J, source
     double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tj[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tk[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double src[4] = {0.0, 0.0, 0.0, 0.0}, invJ = 0.0;
+    // the k-face walls and far field this kernel applies itself (folds_kface_u)
+    const bool fu_lo = tm.fold && folds_kface_u(b, 4), fu_hi = tm.fold && folds_kface_u(b, 5);
+    const bool solid_lo = fu_lo && b.bc[4] == BC_SOLID, solid_hi = fu_hi && b.bc[5] == BC_SOLID;
     for (int pl = i0; pl <= i1; ++pl, o += di)
     {
         FluxCell cur{};
+        double unc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}; // the closed cell's new state
         if (pl < i1)
         {
             for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
@@ -1137,7 +1191,7 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
                     r -= tk[m];
                     if (m >= 1)
                         r += src[m - 1];
-                    Un[m * sl + oc] = U[m * sl + oc] + dt * r;
+                    unc[m] = U[m * sl + oc] + dt * r;
                 }
             }
             if (pl < i1) // open cell pl
@@ -1171,6 +1225,63 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
                 src[3] = fc * (jx * cur.u + jy * cur.v + jz * cur.w);
             }
             prev = cur;
+        }
+        if (pl > i0) // store the closed cell (the lanes of a wavefront together)
+        {
+            // a solid k wall's first layer takes the wall state of its
+            // neighbour's new value (OP_SOLID_FIRST), its ghosts that of the
+            // first layer (OP_SOLID_GHOST); far-field ghosts take q_inf (OP_QINF)
+            double nb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if (solid_lo)
+            {
+#pragma unroll
+                for (int m = 0; m < 5; ++m)
+                    nb[m] = __shfl_down(unc[m], 1, 64);
+            }
+            if (solid_hi)
+            {
+#pragma unroll
+                for (int m = 0; m < 5; ++m)
+                {
+                    const double v = __shfl_up(unc[m], 1, 64);
+                    if (k == b.nk - 1)
+                        nb[m] = v;
+                }
+            }
+            if (act)
+            {
+                const int64_t oc = o - di;
+                const bool wall_lo = solid_lo && k == 0, wall_hi = solid_hi && k == b.nk - 1;
+                if (wall_lo || wall_hi)
+                {
+                    double w[5], wg[5];
+                    solid_state(gm, nb, w);
+                    solid_state(gm, w, wg);
+                    const int64_t out = wall_lo ? -1 : 1;
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                    {
+                        Un[m * sl + oc] = w[m];
+                        for (int l = 1; l <= b.g; ++l)
+                            Un[m * sl + oc + out * l] = wg[m];
+                    }
+                }
+                else
+                {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        Un[m * sl + oc] = unc[m];
+                }
+                const bool far_lo = fu_lo && !solid_lo && k == 0, far_hi = fu_hi && !solid_hi && k == b.nk - 1;
+                if (far_lo || far_hi)
+                {
+                    const int64_t out = far_lo ? -1 : 1;
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        for (int l = 1; l <= b.g; ++l)
+                            Un[m * sl + oc + out * l] = c.qinf[m];
+                }
+            }
         }
         if (pl < i1)
             __syncthreads();
@@ -2096,16 +2207,6 @@ __global__ void k_dt_res(double *dtres, const double *dt, const MhdCaseConst *cs
 
 // ======================================================================
 // Boundary / halo op lists (one phase per launch).
-__device__ __forceinline__ double solid_p(double gm, double rho, double mx, double my, double mz, double E)
-{
-    rho = dmax(rho, 2.5e-3);
-    const double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
-    double eint = E - ke;
-    if (eint < 0.0)
-        eint = 0.0;
-    const double p = (gm - 1.0) * eint;
-    return dmax(p, 1e-12);
-}
 
 // Op dst/src hold absolute device addresses; components are `stride`
 // elements apart; element e of a run acts on dst + e dstep (src + e sstep).
@@ -2151,17 +2252,13 @@ __device__ __forceinline__ void apply_op(const MhdOp &op, int e, const MhdCaseCo
     case OP_SOLID_FIRST:
     case OP_SOLID_GHOST:
     {
-        // BC_Solid_Surface_ (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:19-184)
-        const double gm = cst[op.cas].gamma;
         const int64_t L = op.stride;
-        const double rho_r = dmax(s[0], 2.5e-3);
-        const double p_r = solid_p(gm, rho_r, s[L], s[2 * L], s[3 * L], s[4 * L]);
-        d[0] = rho_r;
-        d[L] = rho_r * 0.0;
-        d[2 * L] = rho_r * 0.0;
-        d[3 * L] = rho_r * 0.0;
-        const double ke_g = 0.5 * rho_r * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
-        d[4 * L] = p_r / (gm - 1.0) + ke_g;
+        const double sv[5] = {s[0], s[L], s[2 * L], s[3 * L], s[4 * L]};
+        double dv[5];
+        solid_state(cst[op.cas].gamma, sv, dv);
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            d[c * L] = dv[c];
         break;
     }
     default:

@@ -980,6 +980,7 @@ MhdOp mk(int64_t dst, int64_t src, int kind, int ncomp, int64_t stride, double v
     o.run = 1;
     o.dstep = 0;
     o.sstep = 0;
+    o.fold = 0;
     return o;
 }
 
@@ -1096,7 +1097,7 @@ void compact_ops(const std::vector<BlockArrays> &blk, std::vector<MhdOp> &ops)
     }
     auto same = [](const MhdOp &a, const MhdOp &b) {
         return a.kind == b.kind && a.ncomp == b.ncomp && a.stride == b.stride && a.cas == b.cas && a.run == 1 &&
-               b.run == 1;
+               b.run == 1 && a.fold == b.fold;
     };
     const bool uses_src[9] = {true, true, false, false, true, true, false, false, false};
     std::vector<MhdOp> out;
@@ -1109,6 +1110,7 @@ void compact_ops(const std::vector<BlockArrays> &blk, std::vector<MhdOp> &ops)
             if (a.op.kind != b.op.kind) return a.op.kind < b.op.kind;
             if (a.op.ncomp != b.op.ncomp) return a.op.ncomp < b.op.ncomp;
             if (a.op.cas != b.op.cas) return a.op.cas < b.op.cas;
+            if (a.op.fold != b.op.fold) return a.op.fold < b.op.fold;
             if (a.grp != b.grp) return a.grp < b.grp;
             for (int q = 0; q < 3; ++q)
                 if (a.c[ax[q]] != b.c[ax[q]]) return a.c[ax[q]] < b.c[ax[q]];
@@ -1165,15 +1167,22 @@ void build_ops(CaseSetup &cs)
     cs.ops.assign(SY_COUNT, {});
     const int nb = (int)grd.blocks.size();
 
+    // ops of one face, marked when the step's kernels fold them
+    auto push = [](std::vector<MhdOp> &L, MhdOp o, bool fold) {
+        o.fold = fold ? 1 : 0;
+        L.push_back(o);
+    };
     for (int b = 0; b < nb; ++b)
     {
         const Block &B = grd.blocks[b];
         const int64_t stride = cs.blk[b].d.slot;
+        const MhdBlockDev &D = cs.blk[b].d;
         for (int f = 0; f < 6; ++f)
         {
             if (B.face[f].kind != 0 || B.face[f].bc == BC_NONE)
                 continue;
             const int bc = B.face[f].bc;
+            const bool fold_b = folds_kface_b(D, f), fold_u = folds_kface_u(D, f);
             // ---- Eface: solid wall zeroes the inner slab and its ghosts
             //      (reference Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:787-803)
             if (bc == BC_SOLID)
@@ -1228,21 +1237,23 @@ void build_ops(CaseSetup &cs)
                                 {
                                     int q[3] = {i, j, k};
                                     q[s.ax] += s.sgn * l;
-                                    cs.ops[SY_U_BC0].push_back(mk(C.ref(b, ST_U0, q[0], q[1], q[2]), 0, OP_QINF, 5, stride));
+                                    push(cs.ops[SY_U_BC0], mk(C.ref(b, ST_U0, q[0], q[1], q[2]), 0, OP_QINF, 5, stride), fold_u);
                                 }
                             }
                             else
                             {
                                 int r[3] = {i, j, k};
                                 r[s.ax] -= s.sgn; // second interior layer
-                                cs.ops[SY_U_BC0].push_back(
-                                    mk(C.ref(b, ST_U0, i, j, k), C.ref(b, ST_U0, r[0], r[1], r[2]), OP_SOLID_FIRST, 5, stride));
+                                push(cs.ops[SY_U_BC0],
+                                     mk(C.ref(b, ST_U0, i, j, k), C.ref(b, ST_U0, r[0], r[1], r[2]), OP_SOLID_FIRST, 5, stride),
+                                     fold_u);
                                 for (int l = 1; l <= g; ++l)
                                 {
                                     int q[3] = {i, j, k};
                                     q[s.ax] += s.sgn * l;
-                                    cs.ops[SY_U_BC1].push_back(
-                                        mk(C.ref(b, ST_U0, q[0], q[1], q[2]), C.ref(b, ST_U0, i, j, k), OP_SOLID_GHOST, 5, stride));
+                                    push(cs.ops[SY_U_BC1],
+                                         mk(C.ref(b, ST_U0, q[0], q[1], q[2]), C.ref(b, ST_U0, i, j, k), OP_SOLID_GHOST, 5, stride),
+                                         fold_u);
                                 }
                             }
                         }
@@ -1261,8 +1272,9 @@ void build_ops(CaseSetup &cs)
                                 {
                                     int q[3] = {i, j, k};
                                     q[s.ax] += s.sgn * l;
-                                    cs.ops[SY_B_BC].push_back(
-                                        mk(C.ref(b, ST_B + a, q[0], q[1], q[2]), C.ref(b, ST_B + a, i, j, k), OP_COPY, 1, stride));
+                                    push(cs.ops[SY_B_BC],
+                                         mk(C.ref(b, ST_B + a, q[0], q[1], q[2]), C.ref(b, ST_B + a, i, j, k), OP_COPY, 1, stride),
+                                         fold_b);
                                 }
                 }
             // ---- J_cell: solid wall zeroes two interior layers

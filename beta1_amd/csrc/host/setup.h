@@ -93,6 +93,8 @@ struct MhdOp
     int32_t run;   // elements in the run (1: a single op)
     int32_t dstep; // destination and source step between the run's elements
     int32_t sstep;
+    int32_t fold;  // 1: boundary work the step's kernels do themselves (folds_kface_*);
+                   // the step skips it, refresh and the per-group syncs apply it too
 };
 
 enum MhdOpKind

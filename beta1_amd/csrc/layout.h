@@ -145,6 +145,26 @@ struct MhdBlockDev
     int pad_;
 };
 
+// Boundary work on the k faces (faces 4 and 5: k is the lanes' axis of
+// every kernel) that the step's kernels do themselves, so the step skips
+// those ops (refresh and the per-group syncs still apply them; re-applying
+// them is idempotent).  A solid wall's B_face ghost copies (k_curl_update,
+// every block); the single-fluid U wall layer, its ghosts and far-field
+// ghosts (k_fluid_col, first-order scheme), where the wall cell's neighbour
+// is the next lane of the same 64-lane wavefront -- not for a high wall whose
+// last cell opens a tile.  (Zeroing the wall's face EMFs inside k_face_emf
+// cost that kernel more than the op launch it saved.)
+MHD_HD bool folds_kface_b(const MhdBlockDev &b, int face)
+{
+    return (face == 4 || face == 5) && b.bc[face] == BC_SOLID;
+}
+MHD_HD bool folds_kface_u(const MhdBlockDev &b, int face)
+{
+    if (b.model != 0 || (face != 4 && face != 5) || (b.bc[face] != BC_SOLID && b.bc[face] != BC_FARFIELD))
+        return false;
+    return face == 4 || b.bc[face] != BC_SOLID || (b.nk - 1) % 64 != 0;
+}
+
 // Per-case constants (the reference's calc_physical_constant,
 // Z4_Lunar/src/physics/LunarSolver_Tools.cpp:3-45).
 struct MhdCaseConst
@@ -216,6 +236,8 @@ struct MhdTileMap
     int ci;      // column and plane-chunk tiles: planes per workgroup along i
     int kloop;   // geometry-shared map: 1 = a tile covers every case of its entry (the kernel
                  // loops over them with the geometry loaded once), 0 = one case per tile
+    int fold;    // 1: the kernel applies the k-face boundary work of folds_kface_* itself (a
+                 // whole step, which then skips those op records); 0: the syncs do it
 };
 
 MHD_HD int64_t blk_off(const MhdBlockDev &b, int i, int j, int k)
