@@ -197,7 +197,8 @@ struct mhd_ctx
     // op lists of the active cases, one device array per sync group
     OpGroup og[SY_COUNT * kOpCls]; // sub-list cls of sync sy at sy * kOpCls + cls
 
-    hipGraphExec_t gexec = nullptr;
+    // captured steps: kGraphShort and kGraphLong consecutive steps (even: the U double buffer alternates)
+    hipGraphExec_t gexec = nullptr, gexec_long = nullptr;
     // per-stage entry points: mhd_fluid_rhs has written the next U buffer and
     // mhd_apply_update has not yet made it current
     bool fluid_pending = false;
@@ -293,6 +294,8 @@ extern "C" int mhd_destroy(mhd_ctx *c)
         (void)hipStreamSynchronize(c->q2);
     if (c->gexec)
         (void)hipGraphExecDestroy(c->gexec);
+    if (c->gexec_long)
+        (void)hipGraphExecDestroy(c->gexec_long);
     for (auto &r : c->cases)
         free_t(r->d_st);
     for (auto &g : c->geos)
@@ -358,11 +361,12 @@ template <class T> static int grow(mhd_ctx *c, T *&p, size_t &cap, size_t need)
 // maps, op groups and per-case arrays as they are now).
 static void drop_graph(mhd_ctx *c)
 {
-    if (c->gexec)
-    {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    for (hipGraphExec_t *g : {&c->gexec, &c->gexec_long})
+        if (*g)
+        {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
 }
 
 template <class T> static int regrow(mhd_ctx *c, T *&p, int old, int cap, bool keep)
@@ -1056,12 +1060,15 @@ static void stage_fluid(mhd_ctx *c, hipStream_t qf)
 // AssembleOneDirectionEMF_ for the three face families
 // (LunarSolver_MagExplicit_Ideal.cpp:348-496), after the cube-vertex cells
 // took the contravariant velocity of their source cells.
-static void stage_face_emf(mhd_ctx *c)
+static void stage_uc_corner(mhd_ctx *c)
 {
-    {
-        ProfScope p(c, KP_OPS);
-        run_ops(c, SY_UC_CORNER, 0);
-    }
+    ProfScope p(c, KP_OPS);
+    run_ops(c, SY_UC_CORNER, 0);
+}
+static void stage_face_emf(mhd_ctx *c, bool corner = true)
+{
+    if (corner)
+        stage_uc_corner(c);
     ProfScope p(c, KP_FACE);
     mhdk::launch_face_emf(launch_of(c, KP_FACE), mixed_of(c), c->q);
 }
@@ -1152,12 +1159,16 @@ static void enqueue_step(mhd_ctx *c)
 {
     c->in_step = true;
     stage_dt(c);
+    // The corner velocities the face EMF reads go first: launched after the
+    // fluid kernel they wait for its workgroups to drain (77 us, on the EMF
+    // chain's critical path, instead of 6).
+    stage_uc_corner(c);
     // The fluid update reads U[cur], B_cell, J_cell, p_e and dt and writes
     // U[1-cur]; the EMF / curl / resistive chain reads U[cur] and writes
     // Eface, E, B faces, J_cell (subcycles) -- J_cell only after the fluid
     // update has read it, so the subcycles wait for the fluid branch.
     stage_fluid(c, fork_branch(c, 0));
-    stage_face_emf(c);
+    stage_face_emf(c, false);
     stage_sync_eface(c);
     stage_edge_emf(c);
     stage_sync_eedge(c);
@@ -1509,6 +1520,7 @@ extern "C" int mhd_step(mhd_ctx *c, int nsteps)
     return check_err(c);
 }
 
+constexpr int kGraphShort = 2, kGraphLong = 10;
 extern "C" int mhd_step_graph(mhd_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0)
@@ -1528,19 +1540,32 @@ extern "C" int mhd_step_graph(mhd_ctx *c, int nsteps)
         enqueue_step(c);
         ++s;
     }
-    if (nsteps - s >= 2)
+    // long runs replay kGraphLong steps per launch (each graph launch costs
+    // about the gap between two steps), the rest two at a time
+    auto capture = [&](hipGraphExec_t &ge, int n) -> int {
+        if (ge)
+            return MHD_OK;
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->q, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < n; ++q)
+            enqueue_step(c);
+        HIPCHK(c, hipStreamEndCapture(c->q, &g));
+        HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        return MHD_OK;
+    };
+    if (nsteps - s >= 2 * kGraphLong)
     {
-        if (!c->gexec)
-        {
-            hipGraph_t g;
-            HIPCHK(c, hipStreamBeginCapture(c->q, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c);
-            enqueue_step(c);
-            HIPCHK(c, hipStreamEndCapture(c->q, &g));
-            HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-        }
-        for (; s + 2 <= nsteps; s += 2)
+        if (int rc = capture(c->gexec_long, kGraphLong))
+            return rc;
+        for (; s + kGraphLong <= nsteps; s += kGraphLong)
+            HIPCHK(c, hipGraphLaunch(c->gexec_long, c->q));
+    }
+    if (nsteps - s >= kGraphShort)
+    {
+        if (int rc = capture(c->gexec, kGraphShort))
+            return rc;
+        for (; s + kGraphShort <= nsteps; s += kGraphShort)
             HIPCHK(c, hipGraphLaunch(c->gexec, c->q));
     }
     for (; s < nsteps; ++s)
@@ -2194,12 +2219,8 @@ extern "C" int mhd_set_tuning(mhd_ctx *c, int kernel, int rows, int i_inner, int
         return fail(c, MHD_ERR_ARG, std::string("launch shape: member-looping tiles (geo_shared 2) are j-inner and only for ") +
                                         kKernelName[KP_DERIVED]);
     c->jt_idx[kernel] = geo_shared == 2 ? kVariants + ji : ji + 3 * i_inner + 6 * geo_shared;
-    if (c->gexec)
-    {
-        (void)hipStreamSynchronize(c->q);
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    (void)hipStreamSynchronize(c->q);
+    drop_graph(c);
     return MHD_OK;
 }
 
@@ -2215,11 +2236,7 @@ extern "C" int mhd_autotune(mhd_ctx *c)
     int rc = autotune(c);
     if (rc)
         return rc;
-    if (c->gexec)
-    {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
+    drop_graph(c);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->q));
     return check_err(c);

@@ -206,3 +206,61 @@ def test_blocks_of_unequal_extent_match_the_oracle():
             sl = interior(f, n, 2)
             a, r = ctx.field(cid, f, b)[sl], orc.view(f, b)[sl]
             assert np.array_equal(a, r), (f, b)
+
+
+def kwall_box_parts(nk, hi_bc):
+    """One block of 4 x 4 x nk cells over [0,1]^2 x [0,2], periodic in x and
+    y, an absorbing wall at z = 0 and `hi_bc` (1 far field, 2 wall) at z = 2;
+    the Lunar constants of the default wind."""
+    from test_dropin import G
+    k = HostCase(WindDesc(nang=4, nrad=6), ghost=G).constants()
+    ni = nj = 4
+    xs, ys, zs = np.linspace(0, 1, ni + 1), np.linspace(0, 1, nj + 1), np.linspace(0, 2, nk + 1) ** 1.2
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    f = np.zeros((6, 14), dtype=np.int64)
+    ident = [0, 1, 2, 1, 1, 1]
+    f[0] = [1, 0, 0, 1, 1] + ident + [ni, 0, 0]
+    f[1] = [1, 0, 0, 0, 1] + ident + [-ni, 0, 0]
+    f[2] = [1, 0, 0, 3, 1] + ident + [0, nj, 0]
+    f[3] = [1, 0, 0, 2, 1] + ident + [0, -nj, 0]
+    f[4] = [0, 2, -1, -1, 0] + [0] * 9
+    f[5] = [0, hi_bc, -1, -1, 0] + [0] * 9
+    return k, [{"n": (ni, nj, nk), "nodes": [X, Y, Z], "faces": f}]
+
+
+@pytest.mark.parametrize("nk,hi_bc", [(8, 2), (8, 1), (65, 2)])
+def test_walls_on_k_faces_match_the_oracle(nk, hi_bc):
+    """The step's kernels apply the wall and far-field work of the k faces
+    themselves (the fluid kernel from the neighbour lane's new state); a box
+    with walls or far field on both k faces steps bit for bit like the
+    oracle, and like the per-stage entries (which leave that work to the
+    syncs).  nk = 65 puts the high wall's last cell at the start of a
+    64-lane tile, where the fluid kernel leaves the wall to the op list."""
+    k, blocks = kwall_box_parts(nk, hi_bc)
+    made = []
+    for _ in range(3):
+        hc = HostCase.from_parts(k, blocks, ghost=2)
+        r = np.random.default_rng(3)
+        U = hc.field("U_H", 0)
+        U[2:6, 2:6, 2:2 + nk] *= 1.0 + 2e-3 * r.uniform(-1, 1, (4, 4, nk, 1))
+        hc.set_field("U_H", 0, U)
+        for sy in (3, 4, 5):
+            hc.apply_sync(sy)
+        made.append(hc)
+    orc = Oracle(made[0])
+    ctx, staged = MhdContext(), MhdContext()
+    cid, sid = ctx.adopt(made[1]), staged.adopt(made[2])
+    ctx.finalize()
+    staged.finalize()
+    dts = orc.step(5)
+    ctx.step(3)
+    ctx.step(2, graph=True)
+    for _ in range(5):
+        for s in STAGES:
+            staged.stage(s)
+    assert ctx.last_dt(cid) == dts[-1] == staged.last_dt(sid)
+    for f in FIELDS:
+        sl = interior(f, (4, 4, nk), 2)
+        a, r, b = ctx.field(cid, f, 0), orc.view(f, 0), staged.field(sid, f, 0)
+        assert np.array_equal(a[sl], r[sl]), f
+        assert np.array_equal(a, b), f  # ghosts too

@@ -94,9 +94,11 @@ def test_graph_step_equals_eager():
     b = MhdContext()
     cb = b.add(desc)
     b.finalize()
-    a.step(6)
+    a.step(6 + 25 + 21)
     b.step(2, graph=True)
     b.step(4, graph=True)  # graph replays with eager syncs in between
+    b.step(25, graph=True)  # two ten-step graphs, two two-step graphs, one eager step
+    b.step(21, graph=True)  # from the odd buffer: one eager step first
     for name in CONSERVED:
         for blk in range(6):
             assert np.array_equal(a.field(ca, name, blk), b.field(cb, name, blk))
