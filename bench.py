@@ -29,6 +29,7 @@ KERNEL_PARTS = {
     "k_ucontra": (40 + 24, 72 + 8, 0),             # U -> UC | JD, Jac
     "k_face_emf": (24 + 24 + 24 + 72, 0, 0),       # B, Badd, UC -> Eface
     "k_edge_emf": (72 + 24 + 24 + 24, 0, 0),       # Eface, UC, B -> E
+    "k_emf": (24 + 24 + 24 + 24, 0, 0),            # B, Badd, UC -> E (face and edge EMF in one pass)
     "k_fluid": (40 + 24 + 24 + 40, 72 + 8, 0),     # U, B_cell, J_cell -> U' | JD, Jac
     "k_curl_update": (24 + 24 + 24, 0, 0),         # E, B -> B
     # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | B_cell least squares Mi (6), mu (6)

@@ -105,6 +105,7 @@ SIGNATURES = {
     "mhd_host_apply_sync": (_I, [_P, _I]),
     "mhd_host_op_count": (_I, [_P, _I, C.POINTER(_S)]),
     "mhd_host_ops_independent": (_I, [_P, _I, C.POINTER(_I)]),
+    "mhd_host_emf_shell_check": (_I, [_P, C.POINTER(_I)]),
     "mhd_case_begin": (_I, [_P, _I, _PD, _PI]),
     "mhd_add_block": (_I, [_P, _I, _I, _I, _I, _I, _PD, _PD, _PD, _S, C.POINTER(_PD)]),
     "mhd_set_topology": (_I, [_P, _I, _I, _PI]),

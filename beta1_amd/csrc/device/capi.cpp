@@ -110,6 +110,7 @@ struct TileSet
     TileArr tiles[R_COUNT][3];  // [range][rows]
     TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
     TileArr cols;              // column tiles of the interior cells (kColCi planes each)
+    TileArr shell;             // 256 shell points of the face range per tile
     TileArr kl[R_COUNT][3];    // geometry-shared sets: member-looping tiles [range][rows]
 };
 // planes along i per column-tile workgroup
@@ -145,14 +146,16 @@ enum KernelId
     KP_JCELL_RES,
     KP_CURL_RES,
     KP_JEDGE,
+    KP_EMF,   // face + edge EMF in one pass (whole steps)
+    KP_SHELL, // face EMF of the shell points before the Eface sync (whole steps)
     KP_COUNT
 };
 const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
                                      "k_fluid2",  "k_derived2",    "k_emf_extra", "k_eres",     "k_jcell_res",
-                                     "k_curl_res", "k_jedge"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL,
-                                    R_CELL, R_CELL, R_EXT, R_FACE, R_FACE, R_CELL, R_FACE, R_FACE};
+                                     "k_curl_res", "k_jedge",      "k_emf",     "k_face_shell"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL, R_CELL,
+                                    R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_FACE, R_FACE, R_FACE};
 // kernels with a member-looping variant (k_derived_k)
 bool loops_members(int kernel) { return kernel == KP_DERIVED; }
 } // namespace
@@ -179,6 +182,9 @@ struct mhd_ctx
     TileSet ts[BS_COUNT][2]; // [block set][geometry-shared]
     int jt_idx[KP_COUNT];
     int n_two = 0; // two-fluid blocks in the table
+    // whole steps form the face and edge EMF in one pass (k_face_shell, the
+    // Eface sync, k_emf); needs every block extent >= 2 (shell enumeration)
+    bool emf_fused = false;
 
     // case-indexed device arrays
     int case_cap = 0;
@@ -311,6 +317,7 @@ extern "C" int mhd_destroy(mhd_ctx *c)
                     for (auto &t : rr)
                         free_t(t.d_start);
             free_t(T.cols.d_start);
+            free_t(T.shell.d_start);
             free_t(T.d_lead_k);
             free_t(T.d_lead_moff);
             free_t(T.d_members);
@@ -412,8 +419,12 @@ static int build_tiles(mhd_ctx *c)
 {
     const int nblk = (int)c->h_blk.size();
     c->n_two = 0;
+    c->emf_fused = nblk > 0;
     for (const MhdBlockDev &d : c->h_blk)
+    {
         c->n_two += d.model == 1;
+        c->emf_fused = c->emf_fused && d.ni >= 2 && d.nj >= 2 && d.nk >= 2;
+    }
     for (int s = 0; s < BS_COUNT; ++s)
     {
         auto in_set = [&](int model) { return s == BS_ALL || (s == BS_ONE) == (model == 0); };
@@ -521,6 +532,24 @@ static int build_tiles(mhd_ctx *c)
                         HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
                     }
             {
+                TileArr &A = T.shell;
+                std::vector<int> start(std::max(T.nent, 1), 0);
+                long long total = 0;
+                for (int e = 0; e < T.nent; ++e)
+                {
+                    const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
+                    start[e] = (int)total;
+                    if (d.ni >= 2 && d.nj >= 2 && d.nk >= 2)
+                        total += (long long)(ident ? 1 : T.lead_k[e]) * ((shell_count(d) + 255) / 256);
+                }
+                if (total > (1LL << 31) - 64)
+                    return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                A.n = (int)total;
+                if ((rc = grow(c, A.d_start, A.cap_start, start.size())))
+                    return rc;
+                HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+            }
+            {
                 TileArr &A = T.cols;
                 const int tj = mhdk::col_tile_rows();
                 std::vector<int> start(std::max(T.nent, 1), 0);
@@ -563,7 +592,7 @@ static int set_of(const mhd_ctx *c, int kernel)
 // faster per point (its carried values halve the occupancy).
 static int chunk_of(int kernel)
 {
-    if (kernel == KP_FACE)
+    if (kernel == KP_FACE || kernel == KP_EMF)
         return 0;
     if (kernel == KP_JCELL || kernel == KP_JCELL_RES)
         return 1;
@@ -629,6 +658,20 @@ static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -
     L.tm.ci = kColCi;
     L.tm.kloop = 0;
     L.jt = mhdk::col_tile_rows();
+    return L;
+}
+
+// Shell-point launch of the face EMF (every block, per-case entries).
+static mhdk::Launch launch_of_shell(const mhd_ctx *c)
+{
+    mhdk::Launch L = launch_of(c, KP_SHELL, 0);
+    const TileSet &S = c->ts[BS_ALL][0];
+    L.tm.block_start = S.shell.d_start;
+    L.tm.ntiles = S.shell.n;
+    L.tm.xcd_chunk = c->opt.xcd_order ? (S.shell.n + 7) / 8 : 0;
+    L.tm.i_inner = 0;
+    L.tm.ci = 0;
+    L.tm.kloop = 0;
     return L;
 }
 
@@ -1097,6 +1140,32 @@ static void stage_edge_emf(mhd_ctx *c)
     }
 }
 
+// The face and edge EMF of a whole step in one pass: the shell's face
+// candidates (zero on a wall's inner slab), the halo copy of the Eface sync,
+// then k_emf, which forms every in-range candidate itself and takes the wall
+// handler's zeros inline; then the two-fluid EMF terms.  E equals the three
+// stages' bit for bit.
+static void stage_emf(mhd_ctx *c)
+{
+    {
+        ProfScope p(c, KP_SHELL);
+        mhdk::launch_face_shell(launch_of_shell(c), c->q);
+    }
+    {
+        ProfScope p(c, KP_OPS);
+        run_ops(c, SY_EFACE_HALO, 0);
+    }
+    {
+        ProfScope p(c, KP_EMF);
+        mhdk::launch_emf(launch_of(c, KP_EMF), c->q);
+    }
+    if (runs(c, KP_AMBI))
+    {
+        ProfScope p(c, KP_AMBI);
+        mhdk::launch_emf_extra(launch_of(c, KP_AMBI), c->d_cst, c->ucur, mixed_of(c), c->q);
+    }
+}
+
 // Sync("Eedge"): the far-field edge handler.
 static void stage_sync_eedge(mhd_ctx *c)
 {
@@ -1168,9 +1237,14 @@ static void enqueue_step(mhd_ctx *c)
     // Eface, E, B faces, J_cell (subcycles) -- J_cell only after the fluid
     // update has read it, so the subcycles wait for the fluid branch.
     stage_fluid(c, fork_branch(c, 0));
-    stage_face_emf(c, false);
-    stage_sync_eface(c);
-    stage_edge_emf(c);
+    if (c->emf_fused)
+        stage_emf(c);
+    else
+    {
+        stage_face_emf(c, false);
+        stage_sync_eface(c);
+        stage_edge_emf(c);
+    }
     stage_sync_eedge(c);
     stage_curl(c);
     join_branch(c, 0);
@@ -1196,14 +1270,14 @@ static int autotune(mhd_ctx *c)
     if (c->h_blk.empty())
         return MHD_OK;
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int kernels[] = {KP_FACE,   KP_EDGE,  KP_FLUID,    KP_CURL, KP_DERIVED,
-                           KP_JEDGE, KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI};
+    const int kernels[] = {KP_FACE,  KP_EDGE,  KP_FLUID,  KP_CURL,     KP_DERIVED, KP_JEDGE,
+                           KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI, KP_EMF};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     for (int k : kernels)
     {
-        if (!runs(c, k))
+        if (!runs(c, k) || (k == KP_EMF && !c->emf_fused))
             continue;
         double best = 1e30;
         int best_j = c->jt_idx[k];
@@ -1219,6 +1293,9 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_EDGE:
                     mhdk::launch_edge_emf(L, c->q);
+                    break;
+                case KP_EMF:
+                    mhdk::launch_emf(L, c->q); // writes E only, step scratch
                     break;
                 case KP_FLUID:
                     launch_fluid_any(c, ji, c->d_zero, c->q);

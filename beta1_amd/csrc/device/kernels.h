@@ -23,6 +23,12 @@ void launch_dt_finalize(unsigned long long *dtbits, double *dt, int *err, const 
 void launch_ucontra(const Launch &L, const MhdCaseConst *cst, int ucur, int mixed, hipStream_t q);
 void launch_face_emf(const Launch &L, int mixed, hipStream_t q);
 void launch_edge_emf(const Launch &L, hipStream_t q);
+// Face EMF candidates of each block's shell points (256 per workgroup; the
+// tile map's entries count ceil(shell_count / 256) tiles each), then the
+// face and edge EMF in one pass over plane-chunk tiles of [0, N]^3 (after
+// the halo copy of the shell's candidates); every extent >= 2.
+void launch_face_shell(const Launch &L, hipStream_t q);
+void launch_emf(const Launch &L, hipStream_t q);
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q);
 void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                   hipStream_t q);

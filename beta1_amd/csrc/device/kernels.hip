@@ -182,16 +182,24 @@ __device__ __forceinline__ bool tile_cell_k(const MhdTileMap &tm, MhdBlockDev &b
 // [i0, i1) (tm.ci planes, fewer in the block's last chunk) at one (j, k),
 // so a thread marches along i and keeps what plane i - 1 shares with plane
 // i in registers.  With tm.ci <= 1 a tile is one plane (i1 = i0 + 1).
+// With nsub > 1 the launch holds nsub workgroups per tile (sub = 0 ..
+// nsub - 1, consecutive after the XCD mapping, so the nsub workgroups of a
+// tile share an XCD's L2).
 __device__ __forceinline__ bool tile_planes(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &i1, int &j, int &k,
-                                            bool &wg)
+                                            bool &wg, int nsub = 1, int *sub = nullptr)
 {
     const int ci = tm.ci > 1 ? tm.ci : 1;
     int t = (int)blockIdx.x;
     if (tm.xcd_chunk > 0)
         t = (t & 7) * tm.xcd_chunk + (t >> 3);
-    wg = t < tm.ntiles;
+    wg = t < nsub * tm.ntiles;
     if (!wg)
         return false;
+    if (nsub > 1)
+    {
+        *sub = t % nsub;
+        t /= nsub;
+    }
     int bi = 0, hi = tm.nblk - 1;
     while (bi < hi)
     {
@@ -925,6 +933,383 @@ __global__ __launch_bounds__(256) void k_edge_emf(const MhdTileMap tm)
         if (i <= b.ni && j <= b.nj && k < b.nk)
             E[2 * sl + o] = v;
     }
+}
+
+// ======================================================================
+// Face and edge EMF in one pass (a whole step's form of k_face_emf, the
+// Eface sync and k_edge_emf).
+//
+// The pass forms every in-range face candidate from the same loads with the
+// same expressions as k_face_emf, takes zero where the wall handler zeroes
+// and reads the other ghost faces from Eface, which the shell pass below and
+// the halo copy fill first (face_source, shell_layers in layout.h); so E is
+// bit for bit what the three stages give, and only the shell's candidates
+// go through memory.
+
+// Face EMF candidates of the shell points (k_face_emf's expressions at one
+// point, zero on a wall's inner slab as the wall handler leaves them): 256
+// shell points per workgroup.
+__global__ __launch_bounds__(256) void k_face_shell(const MhdTileMap tm)
+{
+    MhdBlockDev b;
+    int lt;
+    if (!tile_block(tm, b, lt))
+        return;
+    const int t = lt * 256 + (int)threadIdx.x;
+    if (t >= shell_count(b))
+        return;
+    int i, j, k;
+    shell_point(b, t, i, j, k);
+    const int64_t sl = b.slot;
+    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    double *S = dst_ptr(b);
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
+    const double *Ax = S + (int64_t)ST_BADD * sl, *Ay = Ax + sl, *Az = Ax + 2 * sl;
+    const double *U0 = S + (int64_t)ST_UC * sl, *U1 = U0 + sl, *U2 = U0 + 2 * sl;
+    double *E0 = S + (int64_t)ST_EFACE * sl, *E1 = E0 + 3 * sl, *E2 = E0 + 6 * sl;
+    const int64_t o = blk_off(b, i, j, k);
+    // right cell o: averages along all three axes
+    const double bx = Bx[o], ax = Ax[o], nbx = Bx[o + di], nax = Ax[o + di];
+    const double y0 = By[o], y1 = By[o + dj], z0 = Bz[o], z1 = Bz[o + dk];
+    const double ay0 = Ay[o], az0 = Az[o];
+    const double rbt0 = 0.5 * ((bx + ax) + (nbx + nax));
+    const double rbt1 = 0.5 * ((y0 + ay0) + (y1 + Ay[o + dj]));
+    const double rbt2 = 0.5 * ((z0 + az0) + (z1 + Az[o + dk]));
+    const double rbi0 = 0.5 * (bx + nbx), rbi1 = 0.5 * (y0 + y1), rbi2 = 0.5 * (z0 + z1);
+    const double ru0 = U0[o], ru1 = U1[o], ru2 = U2[o];
+    if (i <= b.ni && j < b.nj && k < b.nk)
+    {
+        const int64_t x = o - di;
+        const double ly0 = By[x], ly1 = By[x + dj], lz0 = Bz[x], lz1 = Bz[x + dk];
+        const double lbt1 = 0.5 * ((ly0 + Ay[x]) + (ly1 + Ay[x + dj]));
+        const double lbt2 = 0.5 * ((lz0 + Az[x]) + (lz1 + Az[x + dk]));
+        const double lbi1 = 0.5 * (ly0 + ly1), lbi2 = 0.5 * (lz0 + lz1);
+        const double bff = bx + ax;
+        const double Ge = emf_pair(lbt1, rbt1, lbi1, rbi1, U0[x], ru0, U1[x], ru1, bff);
+        const double Gz = emf_pair(lbt2, rbt2, lbi2, rbi2, U0[x], ru0, U2[x], ru2, bff);
+        const bool z = face_source(b, 0, i, j, k) == 1;
+        E0[sl + o] = z ? 0.0 : Gz;
+        E0[2 * sl + o] = z ? 0.0 : -Ge;
+    }
+    if (i < b.ni && j <= b.nj && k < b.nk)
+    {
+        const int64_t x = o - dj;
+        const double bxj = Bx[x], nbxj = Bx[x + di], jz0 = Bz[x], jz1 = Bz[x + dk];
+        const double jbt0 = 0.5 * ((bxj + Ax[x]) + (nbxj + Ax[x + di]));
+        const double jbt2 = 0.5 * ((jz0 + Az[x]) + (jz1 + Az[x + dk]));
+        const double jbi0 = 0.5 * (bxj + nbxj), jbi2 = 0.5 * (jz0 + jz1);
+        const double bff = y0 + ay0;
+        const double Gx = emf_pair(jbt0, rbt0, jbi0, rbi0, U1[x], ru1, U0[x], ru0, bff);
+        const double Gz = emf_pair(jbt2, rbt2, jbi2, rbi2, U1[x], ru1, U2[x], ru2, bff);
+        const bool z = face_source(b, 1, i, j, k) == 1;
+        E1[o] = z ? 0.0 : -Gz;
+        E1[2 * sl + o] = z ? 0.0 : Gx;
+    }
+    if (i < b.ni && j < b.nj && k <= b.nk)
+    {
+        const int64_t x = o - dk;
+        const double bxk = Bx[x], nbxk = Bx[x + di], ky0 = By[x], ky1 = By[x + dj];
+        const double kbt0 = 0.5 * ((bxk + Ax[x]) + (nbxk + Ax[x + di]));
+        const double kbt1 = 0.5 * ((ky0 + Ay[x]) + (ky1 + Ay[x + dj]));
+        const double kbi0 = 0.5 * (bxk + nbxk), kbi1 = 0.5 * (ky0 + ky1);
+        const double bff = z0 + az0;
+        const double Gx = emf_pair(kbt0, rbt0, kbi0, rbi0, U2[x], ru2, U0[x], ru0, bff);
+        const double Ge = emf_pair(kbt1, rbt1, kbi1, rbi1, U2[x], ru2, U1[x], ru1, bff);
+        const bool z = face_source(b, 2, i, j, k) == 1;
+        E2[o] = z ? 0.0 : Ge;
+        E2[sl + o] = z ? 0.0 : -Gx;
+    }
+}
+
+// max(0, |a|, |b|, |c|, |d|) in max_abs4's order
+__device__ __forceinline__ double max_abs_of4(double a, double b, double c, double d)
+{
+    double m = 0.0;
+    m = dmax(m, fabs(a));
+    m = dmax(m, fabs(b));
+    m = dmax(m, fabs(c));
+    m = dmax(m, fabs(d));
+    return m;
+}
+
+// The candidate of a face from face_source: formed (0), zero (1) or Eface (2).
+// Element idx of a slot through a uniform base and a 32-bit byte offset
+// (idx < 2^29), which the compiler issues as one SGPR base plus one VGPR
+// offset instead of a 64-bit address per access.
+__device__ __forceinline__ double ld32(const double *base, uint32_t idx)
+{
+    return *(const double *)((const char *)base + (idx << 3));
+}
+__device__ __forceinline__ void st32(double *base, uint32_t idx, double v)
+{
+    *(double *)((char *)base + (idx << 3)) = v;
+}
+
+__device__ __forceinline__ double face_pick(int src, double formed, const double *F, uint32_t o)
+{
+    return src == 0 ? formed : (src == 1 ? 0.0 : ld32(F, o));
+}
+
+// The fused pass over plane-chunk tiles of the points [0, N]^3, three
+// workgroups per tile, one per edge family.  Each family takes its four
+// face candidates and upwind speeds from the cells around it in one
+// coordinate plane: the xi edge at o from the cells o, o - dj, o - dk,
+// o - dj - dk of plane i; the eta edge from o, o - dk and their plane-(i - 1)
+// neighbours, the zeta edge from o, o - dj and theirs, which a thread
+// marching along i carries in registers (with the candidate of its own zeta
+// or eta face).  A face candidate used by two families is a different
+// component for each, so splitting by family repeats only cell averages and
+// loads (served by the L2 the tile's three workgroups share).
+// Every load is in bounds: the stencil reaches -1 .. +1 from points of
+// [-1, N] and each slot spans [-g, N + g], g >= 2.
+struct EmfArrays
+{
+    const double *Bx, *By, *Bz, *Ax, *Ay, *Az, *U0, *U1, *U2, *F0, *F1, *F2;
+    double *E;
+    int64_t sl;
+    uint32_t di, dj, dk;
+};
+
+// xi edges of the planes [i0, i1) at (j, k)
+__device__ __forceinline__ void emf_xi(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+{
+    const int64_t sl = A.sl;
+    const uint32_t di = A.di, dj = A.dj, dk = A.dk;
+    const int sy0 = face_jk_bits(b, 1, j, k), sy1 = face_jk_bits(b, 1, j, k - 1);
+    const int sz0 = face_jk_bits(b, 2, j, k), sz1 = face_jk_bits(b, 2, j - 1, k);
+    const bool store = j <= b.nj && k <= b.nk;
+    uint32_t o = (uint32_t)blk_off(b, i0, j, k);
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        const int ib = face_axis_bits(b, 1, 0, i); // eta and zeta faces share the i range
+        const double yo = ld32(A.By, o), ayo = ld32(A.Ay, o), yoj = ld32(A.By, o + dj), ayoj = ld32(A.Ay, o + dj);
+        const double yj = ld32(A.By, o - dj), ayj = ld32(A.Ay, o - dj);
+        const double yk = ld32(A.By, o - dk), ayk = ld32(A.Ay, o - dk);
+        const double ykj = ld32(A.By, o - dk + dj), aykj = ld32(A.Ay, o - dk + dj);
+        const double yjk = ld32(A.By, o - dj - dk), ayjk = ld32(A.Ay, o - dj - dk);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
+        const double zj = ld32(A.Bz, o - dj), azj = ld32(A.Az, o - dj);
+        const double zjk1 = ld32(A.Bz, o - dj + dk), azjk1 = ld32(A.Az, o - dj + dk);
+        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
+        const double zjk = ld32(A.Bz, o - dj - dk), azjk = ld32(A.Az, o - dj - dk);
+        const double u1 = ld32(A.U1, o), u2 = ld32(A.U2, o), ju1 = ld32(A.U1, o - dj), ju2 = ld32(A.U2, o - dj);
+        const double ku1 = ld32(A.U1, o - dk), ku2 = ld32(A.U2, o - dk);
+        const double jku1 = ld32(A.U1, o - dj - dk), jku2 = ld32(A.U2, o - dj - dk);
+        // eta and zeta averages of the cells o, o - dj, o - dk, o - dj - dk
+        const double bt1 = 0.5 * ((yo + ayo) + (yoj + ayoj)), bi1 = 0.5 * (yo + yoj);
+        const double bt2 = 0.5 * ((zo + azo) + (zok + azok)), bi2 = 0.5 * (zo + zok);
+        const double jbt1 = 0.5 * ((yj + ayj) + (yo + ayo)), jbi1 = 0.5 * (yj + yo);
+        const double jbt2 = 0.5 * ((zj + azj) + (zjk1 + azjk1)), jbi2 = 0.5 * (zj + zjk1);
+        const double kbt1 = 0.5 * ((yk + ayk) + (ykj + aykj)), kbi1 = 0.5 * (yk + ykj);
+        const double kbt2 = 0.5 * ((zk + azk) + (zo + azo)), kbi2 = 0.5 * (zk + zo);
+        const double jkbt1 = 0.5 * ((yjk + ayjk) + (yk + ayk)), jkbi1 = 0.5 * (yjk + yk);
+        const double jkbt2 = 0.5 * ((zjk + azjk) + (zj + azj)), jkbi2 = 0.5 * (zjk + zj);
+        // eta faces at o (left cell o - dj) and o - dk; zeta faces at o (left o - dk) and o - dj
+        const double Fy0 = face_pick(face_source_of(ib, sy0),
+                                     -emf_pair(jbt2, bt2, jbi2, bi2, ju1, u1, ju2, u2, yo + ayo), A.F1, o);
+        const double nFy0 = face_pick(face_source_of(ib, sy1),
+                                      -emf_pair(jkbt2, kbt2, jkbi2, kbi2, jku1, ku1, jku2, ku2, yk + ayk), A.F1, o - dk);
+        const double Fz0 = face_pick(face_source_of(ib, sz0),
+                                     emf_pair(kbt1, bt1, kbi1, bi1, ku2, u2, ku1, u1, zo + azo), A.F2, o);
+        const double nFz0 = face_pick(face_source_of(ib, sz1),
+                                      emf_pair(jkbt1, jbt1, jkbi1, jbi1, jku2, ju2, jku1, ju1, zj + azj), A.F2, o - dj);
+        const double Ec0 = 0.25 * (Fy0 + nFy0 + Fz0 + nFz0);
+        const double ae = max_abs_of4(u1, ju1, ku1, jku1);
+        const double az = max_abs_of4(u2, ju2, ku2, jku2);
+        const double v = Ec0 + 0.5 * (0.5 * ae * (zo - zj) - 0.5 * az * (yo - yk));
+        if (i < b.ni && store)
+            st32(A.E, o, v);
+    }
+    (void)sl;
+}
+
+// eta edges of the planes [i0, i1) at (j, k): cells o, o - dk and, carried,
+// o - di, o - di - dk
+__device__ __forceinline__ void emf_eta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+{
+    const int64_t sl = A.sl;
+    const uint32_t di = A.di, dk = A.dk;
+    const int sx0 = face_jk_bits(b, 0, j, k), sx1 = face_jk_bits(b, 0, j, k - 1), sz0 = face_jk_bits(b, 2, j, k);
+    const bool store = j < b.nj && k <= b.nk;
+    uint32_t o = (uint32_t)blk_off(b, i0 - 1, j, k);
+    // plane i0 - 1: zeta averages and velocities of o and o - dk, the zeta
+    // face's candidate along eta, B_zeta at o; xi fluxes of plane i0
+    double cx = ld32(A.Bx, o + di), ca = ld32(A.Ax, o + di), cxk = ld32(A.Bx, o - dk + di), cak = ld32(A.Ax, o - dk + di);
+    double p_bt2, p_bi2, pk_bt2, pk_bi2, pu0, pu2, pku0, pku2, pFz1, pBz;
+    {
+        const double x0 = ld32(A.Bx, o), a0 = ld32(A.Ax, o), xk = ld32(A.Bx, o - dk), ak = ld32(A.Ax, o - dk);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
+        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
+        pu0 = ld32(A.U0, o);
+        pu2 = ld32(A.U2, o);
+        pku0 = ld32(A.U0, o - dk);
+        pku2 = ld32(A.U2, o - dk);
+        const double bt0 = 0.5 * ((x0 + a0) + (cx + ca)), bi0 = 0.5 * (x0 + cx);
+        const double kbt0 = 0.5 * ((xk + ak) + (cxk + cak)), kbi0 = 0.5 * (xk + cxk);
+        p_bt2 = 0.5 * ((zo + azo) + (zok + azok));
+        p_bi2 = 0.5 * (zo + zok);
+        pk_bt2 = 0.5 * ((zk + azk) + (zo + azo));
+        pk_bi2 = 0.5 * (zk + zo);
+        pFz1 = face_pick(face_source_of(face_axis_bits(b, 2, 0, i0 - 1), sz0),
+                         -emf_pair(kbt0, bt0, kbi0, bi0, pku2, pu2, pku0, pu0, zo + azo), A.F2 + sl, o);
+        pBz = zo;
+        o += di;
+    }
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        const int ibx = face_axis_bits(b, 0, 0, i), ib = face_axis_bits(b, 2, 0, i);
+        const double nx = ld32(A.Bx, o + di), na = ld32(A.Ax, o + di);
+        const double nxk = ld32(A.Bx, o - dk + di), nak = ld32(A.Ax, o - dk + di);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
+        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
+        const double u0 = ld32(A.U0, o), u2 = ld32(A.U2, o), ku0 = ld32(A.U0, o - dk), ku2 = ld32(A.U2, o - dk);
+        const double bt0 = 0.5 * ((cx + ca) + (nx + na)), bi0 = 0.5 * (cx + nx);
+        const double kbt0 = 0.5 * ((cxk + cak) + (nxk + nak)), kbi0 = 0.5 * (cxk + nxk);
+        const double bt2 = 0.5 * ((zo + azo) + (zok + azok)), bi2 = 0.5 * (zo + zok);
+        const double kbt2 = 0.5 * ((zk + azk) + (zo + azo)), kbi2 = 0.5 * (zk + zo);
+        // xi faces at o (left cell o - di) and o - dk (left o - dk - di); zeta face at o
+        const double Fx1 = face_pick(face_source_of(ibx, sx0),
+                                     emf_pair(p_bt2, bt2, p_bi2, bi2, pu0, u0, pu2, u2, cx + ca), A.F0 + sl, o);
+        const double nFx1 = face_pick(face_source_of(ibx, sx1),
+                                      emf_pair(pk_bt2, kbt2, pk_bi2, kbi2, pku0, ku0, pku2, ku2, cxk + cak), A.F0 + sl,
+                                      o - dk);
+        const double Fz1 = face_pick(face_source_of(ib, sz0),
+                                     -emf_pair(kbt0, bt0, kbi0, bi0, ku2, u2, ku0, u0, zo + azo), A.F2 + sl, o);
+        const double Ec1 = 0.25 * (Fx1 + nFx1 + Fz1 + pFz1);
+        const double az = max_abs_of4(u2, pu2, ku2, pku2);
+        const double ax = max_abs_of4(u0, pu0, ku0, pku0);
+        const double v = Ec1 + 0.5 * (0.5 * az * (cx - cxk) - 0.5 * ax * (zo - pBz));
+        if (i <= b.ni && store)
+            st32(A.E + sl, o, v);
+        p_bt2 = bt2;
+        p_bi2 = bi2;
+        pk_bt2 = kbt2;
+        pk_bi2 = kbi2;
+        pu0 = u0;
+        pu2 = u2;
+        pku0 = ku0;
+        pku2 = ku2;
+        pFz1 = Fz1;
+        pBz = zo;
+        cx = nx;
+        ca = na;
+        cxk = nxk;
+        cak = nak;
+    }
+}
+
+// zeta edges of the planes [i0, i1) at (j, k): cells o, o - dj and, carried,
+// o - di, o - di - dj
+__device__ __forceinline__ void emf_zeta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+{
+    const int64_t sl = A.sl;
+    const uint32_t di = A.di, dj = A.dj;
+    const int sx0 = face_jk_bits(b, 0, j, k), sx2 = face_jk_bits(b, 0, j - 1, k), sy0 = face_jk_bits(b, 1, j, k);
+    const bool store = j <= b.nj && k < b.nk;
+    uint32_t o = (uint32_t)blk_off(b, i0 - 1, j, k);
+    // plane i0 - 1: eta averages and velocities of o and o - dj, the eta
+    // face's candidate along zeta, B_eta at o; xi fluxes of plane i0
+    double cx = ld32(A.Bx, o + di), ca = ld32(A.Ax, o + di), cxj = ld32(A.Bx, o - dj + di), caj = ld32(A.Ax, o - dj + di);
+    double p_bt1, p_bi1, pj_bt1, pj_bi1, pu0, pu1, pju0, pju1, pFy2, pBy;
+    {
+        const double x0 = ld32(A.Bx, o), a0 = ld32(A.Ax, o), xj = ld32(A.Bx, o - dj), aj = ld32(A.Ax, o - dj);
+        const double yo = ld32(A.By, o), ayo = ld32(A.Ay, o), yoj = ld32(A.By, o + dj), ayoj = ld32(A.Ay, o + dj);
+        const double yj = ld32(A.By, o - dj), ayj = ld32(A.Ay, o - dj);
+        pu0 = ld32(A.U0, o);
+        pu1 = ld32(A.U1, o);
+        pju0 = ld32(A.U0, o - dj);
+        pju1 = ld32(A.U1, o - dj);
+        const double bt0 = 0.5 * ((x0 + a0) + (cx + ca)), bi0 = 0.5 * (x0 + cx);
+        const double jbt0 = 0.5 * ((xj + aj) + (cxj + caj)), jbi0 = 0.5 * (xj + cxj);
+        p_bt1 = 0.5 * ((yo + ayo) + (yoj + ayoj));
+        p_bi1 = 0.5 * (yo + yoj);
+        pj_bt1 = 0.5 * ((yj + ayj) + (yo + ayo));
+        pj_bi1 = 0.5 * (yj + yo);
+        pFy2 = face_pick(face_source_of(face_axis_bits(b, 1, 0, i0 - 1), sy0),
+                         emf_pair(jbt0, bt0, jbi0, bi0, pju1, pu1, pju0, pu0, yo + ayo), A.F1 + 2 * sl, o);
+        pBy = yo;
+        o += di;
+    }
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        const int ibx = face_axis_bits(b, 0, 0, i), ib = face_axis_bits(b, 1, 0, i);
+        const double nx = ld32(A.Bx, o + di), na = ld32(A.Ax, o + di);
+        const double nxj = ld32(A.Bx, o - dj + di), naj = ld32(A.Ax, o - dj + di);
+        const double yo = ld32(A.By, o), ayo = ld32(A.Ay, o), yoj = ld32(A.By, o + dj), ayoj = ld32(A.Ay, o + dj);
+        const double yj = ld32(A.By, o - dj), ayj = ld32(A.Ay, o - dj);
+        const double u0 = ld32(A.U0, o), u1 = ld32(A.U1, o), ju0 = ld32(A.U0, o - dj), ju1 = ld32(A.U1, o - dj);
+        const double bt0 = 0.5 * ((cx + ca) + (nx + na)), bi0 = 0.5 * (cx + nx);
+        const double jbt0 = 0.5 * ((cxj + caj) + (nxj + naj)), jbi0 = 0.5 * (cxj + nxj);
+        const double bt1 = 0.5 * ((yo + ayo) + (yoj + ayoj)), bi1 = 0.5 * (yo + yoj);
+        const double jbt1 = 0.5 * ((yj + ayj) + (yo + ayo)), jbi1 = 0.5 * (yj + yo);
+        // xi faces at o and o - dj (left o - dj - di); eta face at o
+        const double Fx2 = face_pick(face_source_of(ibx, sx0),
+                                     -emf_pair(p_bt1, bt1, p_bi1, bi1, pu0, u0, pu1, u1, cx + ca), A.F0 + 2 * sl, o);
+        const double nFx2 = face_pick(face_source_of(ibx, sx2),
+                                      -emf_pair(pj_bt1, jbt1, pj_bi1, jbi1, pju0, ju0, pju1, ju1, cxj + caj),
+                                      A.F0 + 2 * sl, o - dj);
+        const double Fy2 = face_pick(face_source_of(ib, sy0),
+                                     emf_pair(jbt0, bt0, jbi0, bi0, ju1, u1, ju0, u0, yo + ayo), A.F1 + 2 * sl, o);
+        const double Ec2 = 0.25 * (Fx2 + nFx2 + Fy2 + pFy2);
+        const double ax = max_abs_of4(u0, pu0, ju0, pju0);
+        const double ae = max_abs_of4(u1, pu1, ju1, pju1);
+        const double v = Ec2 + 0.5 * (0.5 * ax * (yo - pBy) - 0.5 * ae * (cx - cxj));
+        if (i <= b.ni && store)
+            st32(A.E + 2 * sl, o, v);
+        p_bt1 = bt1;
+        p_bi1 = bi1;
+        pj_bt1 = jbt1;
+        pj_bi1 = jbi1;
+        pu0 = u0;
+        pu1 = u1;
+        pju0 = ju0;
+        pju1 = ju1;
+        pFy2 = Fy2;
+        pBy = yo;
+        cx = nx;
+        ca = na;
+        cxj = nxj;
+        caj = naj;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_emf(const MhdTileMap tm)
+{
+    MhdBlockDev b;
+    int i0, i1, j, k, fam = 0;
+    bool wg;
+    if (!tile_planes(tm, b, i0, i1, j, k, wg, 3, &fam))
+        return;
+    // 32-bit element indices within a slot (a slot is < 2^29 elements): with
+    // the slot bases uniform, the loads take an SGPR base and one VGPR offset
+    EmfArrays A;
+    A.sl = b.slot;
+    A.di = (uint32_t)b.sj * (uint32_t)b.sk;
+    A.dj = (uint32_t)b.sk;
+    A.dk = 1;
+    double *S = dst_ptr(b);
+    A.Bx = S + (int64_t)ST_B * A.sl;
+    A.By = A.Bx + A.sl;
+    A.Bz = A.Bx + 2 * A.sl;
+    A.Ax = S + (int64_t)ST_BADD * A.sl;
+    A.Ay = A.Ax + A.sl;
+    A.Az = A.Ax + 2 * A.sl;
+    A.U0 = S + (int64_t)ST_UC * A.sl;
+    A.U1 = A.U0 + A.sl;
+    A.U2 = A.U0 + 2 * A.sl;
+    A.F0 = S + (int64_t)ST_EFACE * A.sl;
+    A.F1 = A.F0 + 3 * A.sl;
+    A.F2 = A.F0 + 6 * A.sl;
+    A.E = S + (int64_t)ST_E * A.sl;
+    if (fam == 0)
+        emf_xi(A, b, i0, i1, j, k);
+    else if (fam == 1)
+        emf_eta(A, b, i0, i1, j, k);
+    else
+        emf_zeta(A, b, i0, i1, j, k);
 }
 
 // ======================================================================
@@ -2531,6 +2916,19 @@ void launch_face_emf(const Launch &L, int mixed, hipStream_t q)
 void launch_edge_emf(const Launch &L, hipStream_t q)
 {
     hipLaunchKernelGGL(k_edge_emf, grid_of(L), block_of(L), 0, q, L.tm);
+}
+void launch_face_shell(const Launch &L, hipStream_t q)
+{
+    hipLaunchKernelGGL(k_face_shell, grid_of(L), dim3(256), 0, q, L.tm);
+}
+void launch_emf(const Launch &L, hipStream_t q)
+{
+    // three workgroups per tile (one per edge family)
+    MhdTileMap tm = L.tm;
+    const int n = 3 * tm.ntiles;
+    tm.xcd_chunk = tm.xcd_chunk > 0 ? (n + 7) / 8 : 0;
+    const int nwg = tm.xcd_chunk > 0 ? 8 * tm.xcd_chunk : n;
+    hipLaunchKernelGGL(k_emf, dim3((unsigned)std::max(nwg, 1)), block_of(L), 0, q, tm);
 }
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q)
 {

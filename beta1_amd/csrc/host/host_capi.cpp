@@ -432,6 +432,132 @@ extern "C" int mhd_host_ops_independent(mhd_host_case *h, int sync, int *indepen
     return MHD_OK;
 }
 
+namespace
+{
+// face_source evaluated in one piece (the kernels compose it from per-axis
+// parts): 0 in range and on no wall slab, 1 on a wall slab or out of range
+// along one axis only, beyond a wall, 2 otherwise
+int face_source_whole(const MhdBlockDev &b, int a, int i, int j, int k)
+{
+    const int p[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
+    int nout = 0, oax = 0;
+    for (int t = 0; t < 3; ++t)
+        if (p[t] < 0 || p[t] >= n[t] + (t == a ? 1 : 0))
+        {
+            ++nout;
+            oax = t;
+        }
+    if (nout == 0)
+    {
+        for (int t = 0; t < 3; ++t)
+            if ((b.bc[2 * t] == BC_SOLID && p[t] == 0) ||
+                (b.bc[2 * t + 1] == BC_SOLID && p[t] == n[t] + (t == a ? 1 : 0) - 1))
+                return 1;
+        return 0;
+    }
+    return nout == 1 && b.bc[2 * oax + (p[oax] < 0 ? 0 : 1)] == BC_SOLID ? 1 : 2;
+}
+} // namespace
+
+// What the one-pass EMF of a whole step assumes of the Eface sync, checked on
+// the op lists: face_source's per-axis composition is its whole evaluation;
+// the shell enumeration lists each shell point once and only those; every halo copy reads a face on the shell and writes a ghost face
+// that face_source sends to the Eface array; every face the wall handler
+// zeroes is one face_source makes zero, and every in-range face it makes zero
+// is zeroed by the handler.
+extern "C" int mhd_host_emf_shell_check(mhd_host_case *h, int *ok)
+{
+    if (!h || !ok)
+        return MHD_ERR_ARG;
+    const std::vector<BlockArrays> &blk = h->cs.blk;
+    bool good = !blk.empty();
+    for (const BlockArrays &B : blk)
+    {
+        const MhdBlockDev &d = B.d;
+        if (d.ni < 2 || d.nj < 2 || d.nk < 2)
+            continue;
+        std::unordered_set<int64_t> seen;
+        const int n = shell_count(d);
+        for (int t = 0; t < n && good; ++t)
+        {
+            int i, j, k;
+            shell_point(d, t, i, j, k);
+            good = i >= 0 && i <= d.ni && j >= 0 && j <= d.nj && k >= 0 && k <= d.nk && in_shell(d, i, j, k) &&
+                   seen.insert(blk_off(d, i, j, k)).second;
+        }
+        int brute = 0;
+        for (int i = 0; i <= d.ni; ++i)
+            for (int j = 0; j <= d.nj; ++j)
+                for (int k = 0; k <= d.nk; ++k)
+                    brute += in_shell(d, i, j, k) ? 1 : 0;
+        good = good && brute == n;
+        // the per-axis composition equals the whole evaluation on the stencil's reach
+        for (int a = 0; a < 3 && good; ++a)
+            for (int i = -2; i <= d.ni + 2 && good; ++i)
+                for (int j = -2; j <= d.nj + 2 && good; ++j)
+                    for (int k = -2; k <= d.nk + 2 && good; ++k)
+                        good = face_source(d, a, i, j, k) == face_source_whole(d, a, i, j, k);
+    }
+    const int64_t sl = blk[0].d.slot, bs = (int64_t)st_nslot(blk[0].d.model) * sl;
+    // element offset -> block, Eface family and component, point
+    auto decode = [&](int64_t off, int &b, int &fam, int &comp, int p[3]) {
+        b = (int)(off / bs);
+        const int64_t r = off % bs, s = r / sl, q = r % sl;
+        const MhdBlockDev &d = blk[b].d;
+        fam = (int)(s - ST_EFACE) / 3;
+        comp = (int)(s - ST_EFACE) % 3;
+        p[0] = (int)(q / ((int64_t)d.sj * d.sk)) - d.g;
+        p[1] = (int)((q / d.sk) % d.sj) - d.g;
+        p[2] = (int)(q % d.sk) - d.g;
+        return s >= ST_EFACE && s < ST_EFACE + 9;
+    };
+    for (const MhdOp &o : h->cs.ops[SY_EFACE_HALO])
+    {
+        const int n = o.run > 1 ? o.run : 1;
+        for (int e = 0; e < n && good; ++e)
+        {
+            int b, fam, comp, p[3];
+            good = decode(o.src + (int64_t)e * o.sstep, b, fam, comp, p) && in_shell(blk[b].d, p[0], p[1], p[2]) &&
+                   face_source(blk[b].d, fam, p[0], p[1], p[2]) != 2;
+            good = good && decode(o.dst + (int64_t)e * o.dstep, b, fam, comp, p) &&
+                   face_source(blk[b].d, fam, p[0], p[1], p[2]) == 2;
+        }
+    }
+    std::unordered_set<int64_t> zeroed;
+    for (const MhdOp &o : h->cs.ops[SY_EFACE_BC])
+    {
+        const int n = o.run > 1 ? o.run : 1;
+        for (int e = 0; e < n && good; ++e)
+            for (int c = 0; c < o.ncomp && good; ++c)
+            {
+                const int64_t a = o.dst + (int64_t)e * o.dstep + (int64_t)c * o.stride;
+                int b, fam, comp, p[3];
+                good = o.kind == OP_ZERO && decode(a, b, fam, comp, p) &&
+                       (comp == fam || face_source(blk[b].d, fam, p[0], p[1], p[2]) == 1);
+                zeroed.insert(a);
+            }
+    }
+    for (size_t b = 0; b < blk.size() && good; ++b)
+    {
+        const MhdBlockDev &d = blk[b].d;
+        for (int fam = 0; fam < 3 && good; ++fam)
+            for (int i = 0; i <= d.ni && good; ++i)
+                for (int j = 0; j <= d.nj && good; ++j)
+                    for (int k = 0; k <= d.nk && good; ++k)
+                    {
+                        if (i > d.ni - (fam == 0 ? 0 : 1) || j > d.nj - (fam == 1 ? 0 : 1) ||
+                            k > d.nk - (fam == 2 ? 0 : 1) || face_source(d, fam, i, j, k) != 1)
+                            continue;
+                        for (int comp = 0; comp < 3 && good; ++comp)
+                            if (comp != fam)
+                                good = zeroed.count((int64_t)b * bs + (int64_t)(ST_EFACE + 3 * fam + comp) * sl +
+                                                    blk_off(d, i, j, k)) > 0;
+                    }
+    }
+    *ok = good ? 1 : 0;
+    return MHD_OK;
+}
+
 // ------------------------------------------------- CASE directory, restart
 namespace
 {

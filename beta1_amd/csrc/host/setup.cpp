@@ -1163,6 +1163,29 @@ void build_ops(CaseSetup &cs)
 {
     const Grid &grd = cs.grid;
     const int g = grd.g;
+    // The reference's absorbing wall writes its first cell layer from the
+    // second and then its ghosts from the first, face after face
+    // (Z4_Lunar/src/boundary/Boundary_Condition_Fluid.cpp:20-184); the op
+    // lists apply every first layer, then every ghost, each in any order.
+    // The two agree unless walls on two axes of one block share an edge (the
+    // later wall rewrites a cell the earlier one's ghosts were taken from),
+    // so such a block is refused.
+    for (size_t b = 0; b < grd.blocks.size(); ++b)
+    {
+        int walls = 0, first = -1;
+        for (int ax = 0; ax < 3; ++ax)
+            if ((grd.blocks[b].face[2 * ax].kind == 0 && grd.blocks[b].face[2 * ax].bc == BC_SOLID) ||
+                (grd.blocks[b].face[2 * ax + 1].kind == 0 && grd.blocks[b].face[2 * ax + 1].bc == BC_SOLID))
+            {
+                if (walls++ == 0)
+                    first = ax;
+                else
+                    throw std::invalid_argument("block " + std::to_string(b) + ": solid walls on axes " +
+                                                std::to_string(first) + " and " + std::to_string(ax) +
+                                                " meet along an edge (the wall handler's face-by-face order "
+                                                "there is not supported)");
+            }
+    }
     OpCtx C{grd, cs.blk};
     cs.ops.assign(SY_COUNT, {});
     const int nb = (int)grd.blocks.size();

@@ -145,6 +145,135 @@ struct MhdBlockDev
     int pad_;
 };
 
+// Shell pass of the one-pass face/edge EMF of a whole step.  The halo copy
+// of the Eface sync fills a ghost face one layer outside an interface side
+// from the neighbour face it coincides with, which lies on the neighbour's
+// first layer inside its own interface side: index 0 along the axis of a low
+// side, n - 1 of a high one.  Those layers of every interface side (bc
+// BC_NONE) are the shell; the shell pass forms their face candidates before
+// the sync, the one-pass kernel forms every in-range candidate itself.
+// With every extent >= 2 the layers of an axis are distinct.
+struct ShellLayers
+{
+    int n;    // 0, 1 or 2 layers along the axis
+    int v[2]; // ascending
+};
+MHD_HD ShellLayers shell_layers(const MhdBlockDev &b, int ax)
+{
+    const int n = ax == 0 ? b.ni : (ax == 1 ? b.nj : b.nk);
+    ShellLayers L;
+    L.n = 0;
+    L.v[0] = L.v[1] = 0;
+    if (b.bc[2 * ax] == BC_NONE)
+        L.v[L.n++] = 0;
+    if (b.bc[2 * ax + 1] == BC_NONE)
+        L.v[L.n++] = n - 1;
+    return L;
+}
+// The m-th index of [0, n] that is not a layer of L.
+MHD_HD int shell_skip(const ShellLayers &L, int m)
+{
+    for (int q = 0; q < L.n; ++q)
+        if (m >= L.v[q])
+            ++m;
+    return m;
+}
+// Shell points of the face range [0, N]^3: the layer planes along i in full;
+// in each other plane the layer rows along j in full and the layer points
+// along k of the other rows.
+MHD_HD int shell_count(const MhdBlockDev &b)
+{
+    const ShellLayers Li = shell_layers(b, 0), Lj = shell_layers(b, 1), Lk = shell_layers(b, 2);
+    const int P = (b.nj + 1) * (b.nk + 1), Q = Lj.n * (b.nk + 1) + (b.nj + 1 - Lj.n) * Lk.n;
+    return Li.n * P + (b.ni + 1 - Li.n) * Q;
+}
+MHD_HD void shell_point(const MhdBlockDev &b, int t, int &i, int &j, int &k)
+{
+    const ShellLayers Li = shell_layers(b, 0), Lj = shell_layers(b, 1), Lk = shell_layers(b, 2);
+    const int nk1 = b.nk + 1, P = (b.nj + 1) * nk1, Q = Lj.n * nk1 + (b.nj + 1 - Lj.n) * Lk.n;
+    if (t < Li.n * P)
+    {
+        const int p = t / P, r = t - p * P;
+        i = Li.v[p];
+        j = r / nk1;
+        k = r - j * nk1;
+        return;
+    }
+    t -= Li.n * P;
+    const int m = t / Q;
+    i = shell_skip(Li, m);
+    int r = t - m * Q;
+    if (r < Lj.n * nk1)
+    {
+        const int row = r / nk1;
+        j = Lj.v[row];
+        k = r - row * nk1;
+        return;
+    }
+    r -= Lj.n * nk1;
+    const int mr = r / Lk.n;
+    j = shell_skip(Lj, mr);
+    k = Lk.v[r - mr * Lk.n];
+}
+MHD_HD bool in_shell(const MhdBlockDev &b, int i, int j, int k)
+{
+    const int p[3] = {i, j, k};
+    for (int ax = 0; ax < 3; ++ax)
+    {
+        const ShellLayers L = shell_layers(b, ax);
+        for (int q = 0; q < L.n; ++q)
+            if (p[ax] == L.v[q])
+                return true;
+    }
+    return false;
+}
+
+// Where the one-pass EMF kernel takes the candidates of a face of family a
+// (0 xi, 1 eta, 2 zeta) at point (i, j, k), as the Eface sync leaves them:
+// 0 formed from the face's cells (an in-range face), 1 zero (a solid wall's
+// inner slab and the ghost faces beyond it, which the wall handler zeroes),
+// 2 the Eface array (another ghost face: the halo copy, or a value no kept
+// edge depends on).
+//
+// face_source is composed of per-axis parts so that a kernel can form the
+// (j, k) part of a point once and the i part once per plane: along axis t a
+// face index p is out of range (bit 0), out beyond a solid wall (bit 1) or
+// on a solid wall's inner slab (bit 2).
+MHD_HD int face_axis_bits(const MhdBlockDev &b, int a, int t, int p)
+{
+    const int n = t == 0 ? b.ni : (t == 1 ? b.nj : b.nk);
+    const int cnt = n + (t == a ? 1 : 0);
+    const bool lo = p < 0, hi = p >= cnt;
+    const bool wlo = b.bc[2 * t] == BC_SOLID, whi = b.bc[2 * t + 1] == BC_SOLID;
+    const bool out = lo || hi;
+    const bool wall_out = out && (lo ? wlo : whi);
+    const bool slab = !out && ((wlo && p == 0) || (whi && p == cnt - 1));
+    return (out ? 1 : 0) | (wall_out ? 2 : 0) | (slab ? 4 : 0);
+}
+// (j, k) part: axes out of range (bits 0-1), the one out of range is beyond
+// a wall (bit 2), on a wall's slab (bit 3)
+MHD_HD int face_jk_bits(const MhdBlockDev &b, int a, int j, int k)
+{
+    const int bj = face_axis_bits(b, a, 1, j), bk = face_axis_bits(b, a, 2, k);
+    const int nout = (bj & 1) + (bk & 1);
+    const bool wall_out = ((bj & 3) == 3) || ((bk & 3) == 3);
+    const bool slab = ((bj | bk) & 4) != 0;
+    return nout | (wall_out ? 4 : 0) | (slab ? 8 : 0);
+}
+MHD_HD int face_source_of(int ibits, int jkbits)
+{
+    const int nout = (ibits & 1) + (jkbits & 3);
+    if (nout == 0)
+        return ((ibits & 4) || (jkbits & 8)) ? 1 : 0;
+    if (nout == 1)
+        return ((ibits & 1) ? (ibits & 2) : (jkbits & 4)) ? 1 : 2;
+    return 2;
+}
+MHD_HD int face_source(const MhdBlockDev &b, int a, int i, int j, int k)
+{
+    return face_source_of(face_axis_bits(b, a, 0, i), face_jk_bits(b, a, j, k));
+}
+
 // Boundary work on the k faces (faces 4 and 5: k is the lanes' axis of
 // every kernel) that the step's kernels do themselves, so the step skips
 // those ops (refresh and the per-group syncs still apply them; re-applying

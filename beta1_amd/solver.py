@@ -247,6 +247,14 @@ class HostCase:
         _check(lib().mhd_host_ops_independent(self.h, sync, C.byref(v)), what="ops_independent")
         return bool(v.value)
 
+    def emf_shell_ok(self):
+        """True when the Eface sync of this case is what the one-pass face/edge
+        EMF of a whole step assumes (shell enumeration, halo sources and
+        destinations, wall zeros)."""
+        v = C.c_int()
+        _check(lib().mhd_host_emf_shell_check(self.h, C.byref(v)), what="emf_shell_check")
+        return bool(v.value)
+
 
 def _parts_args(blocks, ghost):
     """ctypes arguments of mhd_(host_)add_block / set_topology for each block

@@ -390,6 +390,14 @@ int mhd_host_op_count(mhd_host_case *h, int sync, size_t *n);
    records in any order and loads an element's sources before its stores;
    no counterpart in the reference, whose Sync loops run in order) */
 int mhd_host_ops_independent(mhd_host_case *h, int sync, int *independent);
+/* 1 in *ok when the Eface sync of case h is what the one-pass face/edge EMF
+   of a whole step assumes: its shell enumeration lists each shell point
+   (the first layer inside every interface side) once, every Eface halo
+   copy reads a shell face and writes a ghost face the kernel loads, and the
+   wall handler zeroes exactly the faces the kernel takes as zero (no
+   counterpart in the reference: a check of this library's fused form of
+   Sync("Eface"), Z4_Lunar/src/LunarSolver_Driver.cpp:21-92) */
+int mhd_host_emf_shell_check(mhd_host_case *h, int *ok);
 
 #ifdef __cplusplus
 }

@@ -176,6 +176,30 @@ def two_box_parts(perturb=1e-3, seed=4):
     return k, blocks
 
 
+def walled_box_parts(n, sides, stretch=1.2):
+    """One block of n = (ni, nj, nk) cells over [0,1] x [0,1] x [0,2] (z
+    stretched), each axis either periodic ("p") or a pair of physical sides
+    (1 far field, 2 absorbing wall) for its low and high face; the Lunar
+    constants of the default wind."""
+    k = HostCase(WindDesc(nang=4, nrad=6), ghost=G).constants()
+    ni, nj, nk = n
+    xs, ys, zs = np.linspace(0, 1, ni + 1), np.linspace(0, 1, nj + 1), np.linspace(0, 2, nk + 1) ** stretch
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    f = np.zeros((6, 14), dtype=np.int64)
+    ident = [0, 1, 2, 1, 1, 1]
+    for ax, kind in enumerate(sides):
+        lo, hi = 2 * ax, 2 * ax + 1
+        if kind == "p":
+            shift = [0, 0, 0]
+            shift[ax] = n[ax]
+            f[lo] = [1, 0, 0, hi, 1] + ident + shift
+            f[hi] = [1, 0, 0, lo, 1] + ident + [-v for v in shift]
+        else:
+            f[lo] = [0, kind[0], -1, -1, 0] + [0] * 9
+            f[hi] = [0, kind[1], -1, -1, 0] + [0] * 9
+    return k, [{"n": tuple(n), "nodes": [X, Y, Z], "faces": f}]
+
+
 def test_blocks_of_unequal_extent():
     """Blocks of different sizes share one slot size in a case; the
     geometry of each is its own box's (closed form) and the halo carries the
@@ -211,6 +235,24 @@ def test_op_lists_are_order_independent(make):
             assert hc.ops_independent(sy), sy
 
 
+@pytest.mark.parametrize("make", [
+    lambda: HostCase(WindDesc(nang=6, nrad=8), ghost=G),
+    lambda: HostCase(OTDesc(n=8), ghost=G),
+    lambda: HostCase(WindDesc(nang=6, nrad=8), ghost=3),
+    lambda: HostCase.from_parts(*two_box_parts(), ghost=G),
+    lambda: HostCase.from_parts(*walled_box_parts((5, 6, 7), ((2, 1), (1, 1), "p")), ghost=G),
+    lambda: HostCase.from_parts(*walled_box_parts((4, 5, 6), ("p", (2, 2), (1, 1))), ghost=3),
+])
+def test_eface_sync_is_what_the_one_pass_emf_assumes(make):
+    """The whole step forms the face EMF candidates inside the edge EMF
+    kernel and only the shell's through memory: every Eface halo copy must
+    read a face on the shell (the first layer inside an interface side) and
+    write a ghost face the kernel loads, the shell enumeration must list each
+    shell point once, and the wall handler must zero exactly the faces the
+    kernel takes as zero."""
+    assert make().emf_shell_ok()
+
+
 def test_mercury_op_lists_are_order_independent():
     from beta1_amd import MercuryDesc
     hc = HostCase(MercuryDesc(nang=6, nrad=8), ghost=G)
@@ -218,3 +260,13 @@ def test_mercury_op_lists_are_order_independent():
     for sy in range(14):
         if hc.op_count(sy):
             assert hc.ops_independent(sy), sy
+
+
+def test_walls_meeting_along_an_edge_are_refused():
+    """Walls on two axes of one block share an edge, where the reference's
+    face-by-face wall order is not what the library's order-free wall phases
+    give: such a case is refused with the reason, walls on one axis are not."""
+    from beta1_amd.capi import MhdError
+    with pytest.raises(MhdError, match="meet along an edge"):
+        HostCase.from_parts(*walled_box_parts((5, 6, 7), ((2, 1), (1, 2), "p")), ghost=G)
+    HostCase.from_parts(*walled_box_parts((5, 6, 7), ((2, 2), (1, 1), "p")), ghost=G)

@@ -264,3 +264,42 @@ def test_walls_on_k_faces_match_the_oracle(nk, hi_bc):
         a, r, b = ctx.field(cid, f, 0), orc.view(f, 0), staged.field(sid, f, 0)
         assert np.array_equal(a[sl], r[sl]), f
         assert np.array_equal(a, b), f  # ghosts too
+
+
+@pytest.mark.parametrize("n,sides", [((6, 5, 7), ((2, 1), (1, 1), "p")), ((5, 6, 66), ("p", (2, 2), (1, 1)))])
+def test_walls_on_i_and_j_faces_match_the_oracle(n, sides):
+    """Walls and far field on the i and j faces (the shells have them on k
+    only), meeting far-field faces along edges: the whole step, whose
+    one-pass EMF kernel takes the wall's zero face candidates inline and the
+    far-field ghosts from the Eface array, steps bit for bit like the oracle
+    and like the per-stage entries."""
+    from test_dropin import walled_box_parts
+    k, blocks = walled_box_parts(n, sides)
+    made = []
+    for _ in range(3):
+        hc = HostCase.from_parts(k, blocks, ghost=2)
+        r = np.random.default_rng(5)
+        U = hc.field("U_H", 0)
+        g = 2
+        U[g:g + n[0], g:g + n[1], g:g + n[2]] *= 1.0 + 2e-3 * r.uniform(-1, 1, (*n, 1))
+        hc.set_field("U_H", 0, U)
+        for sy in (3, 4, 5):
+            hc.apply_sync(sy)
+        made.append(hc)
+    orc = Oracle(made[0])
+    ctx, staged = MhdContext(), MhdContext()
+    cid, sid = ctx.adopt(made[1]), staged.adopt(made[2])
+    ctx.finalize()
+    staged.finalize()
+    dts = orc.step(5)
+    ctx.step(3)
+    ctx.step(2, graph=True)
+    for _ in range(5):
+        for s in STAGES:
+            staged.stage(s)
+    assert ctx.last_dt(cid) == dts[-1] == staged.last_dt(sid)
+    for f in FIELDS:
+        sl = interior(f, n, 2)
+        a, r, b = ctx.field(cid, f, 0), orc.view(f, 0), staged.field(sid, f, 0)
+        assert np.array_equal(a[sl], r[sl]), f
+        assert np.array_equal(a, b), f  # ghosts too
