@@ -111,6 +111,7 @@ struct TileSet
     TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
     TileArr cols;              // column tiles of the interior cells (kColCi planes each)
     TileArr shell;             // 256 shell points of the face range per tile
+    TileArr pad[3];            // padded-row plane chunks of the face range (kChunkCi[0] planes) [rows]
     TileArr kl[R_COUNT][3];    // geometry-shared sets: member-looping tiles [range][rows]
 };
 // planes along i per column-tile workgroup
@@ -318,6 +319,8 @@ extern "C" int mhd_destroy(mhd_ctx *c)
                         free_t(t.d_start);
             free_t(T.cols.d_start);
             free_t(T.shell.d_start);
+            for (auto &t : T.pad)
+                free_t(t.d_start);
             free_t(T.d_lead_k);
             free_t(T.d_lead_moff);
             free_t(T.d_members);
@@ -531,6 +534,27 @@ static int build_tiles(mhd_ctx *c)
                             return rc;
                         HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
                     }
+            // padded-row tiles of the face range (mhdk::pad_tiles_per_plane waves
+            // of 62 owned points per plane, kChunkCi[0] planes per tile)
+            for (int ji = 0; ji < 3; ++ji)
+            {
+                TileArr &A = T.pad[ji];
+                std::vector<int> start(std::max(T.nent, 1), 0);
+                long long total = 0;
+                for (int e = 0; e < T.nent; ++e)
+                {
+                    const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
+                    start[e] = (int)total;
+                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni + 1 + kChunkCi[0] - 1) / kChunkCi[0]) *
+                             mhdk::pad_tiles_per_plane(d.nj, d.nk, kJt[ji]);
+                }
+                if (total > (1LL << 31) - 64)
+                    return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
+                A.n = (int)total;
+                if ((rc = grow(c, A.d_start, A.cap_start, start.size())))
+                    return rc;
+                HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+            }
             {
                 TileArr &A = T.shell;
                 std::vector<int> start(std::max(T.nent, 1), 0);
@@ -607,7 +631,7 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     const bool kl = v >= kVariants;
     const TileSet &S = c->ts[set_of(c, kernel)][sh];
     const int ch = chunk_of(kernel);
-    const TileArr &T = kl ? S.kl[r][ji] : ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
+    const TileArr &T = kernel == KP_EMF ? S.pad[ji] : kl ? S.kl[r][ji] : ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
     const bool ident = S.members.empty();
     mhdk::Launch L;
     L.tm.blks = c->d_blk;

@@ -234,6 +234,78 @@ __device__ __forceinline__ bool tile_planes(const MhdTileMap &tm, MhdBlockDev &b
     return f < plane;
 }
 
+// Padded-row plane-chunk tiles of the face range: the (j, k) plane of
+// [0, N] with each row padded by one point on either side (k = -1 and
+// k = N + 1) and flattened, k fastest.  Wavefront w of a plane takes the 64
+// consecutive padded points from 62 w - 1: lanes 1 .. 62 own 62 points, and
+// every owned point's neighbours along k are the lanes beside it, so a
+// kernel can take what they formed by a lane shift instead of loading it.
+// Lanes past the plane are clamped onto its last point (own = false); with
+// nsub > 1, nsub workgroups per tile as in tile_planes.
+constexpr int PAD_OWN = 62;
+__device__ __forceinline__ bool tile_padded(const MhdTileMap &tm, MhdBlockDev &b, int &i0, int &i1, int &j, int &k,
+                                            bool &own, int nsub, int &sub)
+{
+    const int ci = tm.ci > 1 ? tm.ci : 1;
+    int t = (int)blockIdx.x;
+    if (tm.xcd_chunk > 0)
+        t = (t & 7) * tm.xcd_chunk + (t >> 3);
+    if (t >= nsub * tm.ntiles)
+        return false;
+    sub = t % nsub;
+    t /= nsub;
+    int bi = 0, hi = tm.nblk - 1;
+    while (bi < hi)
+    {
+        const int mid = (bi + hi + 1) >> 1;
+        if (tm.block_start[mid] <= t)
+            bi = mid;
+        else
+            hi = mid - 1;
+    }
+    int lt = t - tm.block_start[bi];
+    if (tm.members)
+    {
+        const int kk = tm.lead_k[bi];
+        const int m = lt % kk;
+        lt /= kk;
+        b = tm.blks[tm.members[tm.lead_moff[bi] + m]];
+    }
+    else
+        b = tm.blks[bi];
+    const int jt = (int)blockDim.y, ext_i = b.ni + 1, Kp = b.nk + 3, P = (b.nj + 1) * Kp;
+    const int nch = (ext_i + ci - 1) / ci, nw = (P + PAD_OWN - 1) / PAD_OWN, ntp = (nw + jt - 1) / jt;
+    const int pt = tm.i_inner ? lt / nch : lt % ntp;
+    const int iy = tm.i_inner ? lt % nch : lt / ntp;
+    const int w = pt * jt + (int)threadIdx.y, lane = (int)threadIdx.x;
+    int F = PAD_OWN * w + lane - 1;
+    own = lane >= 1 && lane <= PAD_OWN && F < P;
+    F = min(max(F, 0), P - 1);
+    j = F / Kp;
+    k = F - j * Kp - 1;
+    own = own && k >= 0 && k <= b.nk;
+    i0 = iy * ci;
+    i1 = min(i0 + ci, ext_i);
+    return true;
+}
+
+// The value lane - 1 (lane_prev) or lane + 1 (lane_next) of the wavefront
+// holds (DPP wave shifts by one lane; lane 0, resp. 63, keeps its own).
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)x, (int)x, 0x138, 0xf, 0xf, false); // wave_shr:1
+    const int hi = __builtin_amdgcn_update_dpp((int)(x >> 32), (int)(x >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)x, (int)x, 0x130, 0xf, 0xf, false); // wave_shl:1
+    const int hi = __builtin_amdgcn_update_dpp((int)(x >> 32), (int)(x >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // ------------------------------------------------------------------ state
 struct Prim
 {
@@ -1050,34 +1122,36 @@ __device__ __forceinline__ double face_pick(int src, double formed, const double
     return src == 0 ? formed : (src == 1 ? 0.0 : ld32(F, o));
 }
 
-// The fused pass over plane-chunk tiles of the points [0, N]^3, three
-// workgroups per tile, one per edge family.  Each family takes its four
-// face candidates and upwind speeds from the cells around it in one
-// coordinate plane: the xi edge at o from the cells o, o - dj, o - dk,
-// o - dj - dk of plane i; the eta edge from o, o - dk and their plane-(i - 1)
-// neighbours, the zeta edge from o, o - dj and theirs, which a thread
-// marching along i carries in registers (with the candidate of its own zeta
-// or eta face).  A face candidate used by two families is a different
-// component for each, so splitting by family repeats only cell averages and
-// loads (served by the L2 the tile's three workgroups share).
+// The fused pass over padded-row plane-chunk tiles of the points [0, N]^3
+// (tile_padded), three workgroups per tile, one per edge family.  Each
+// family takes its four face candidates and upwind speeds from the cells
+// around it in one coordinate plane: the xi edge at o from the cells o,
+// o - dj, o - dk, o - dj - dk of plane i; the eta edge from o, o - dk and
+// their plane-(i - 1) neighbours, the zeta edge from o, o - dj and theirs,
+// which a thread marching along i carries in registers (with the candidate
+// of its own zeta or eta face).  What a point needs at o - dk (cell averages,
+// velocities, the candidate of the face there) is what the lane before it
+// formed as its own o, and the averages along zeta need B at o + dk, which
+// the lane after it loaded: lane shifts instead of loads, the same values.
+// A face candidate used by two families is a different component for each,
+// so splitting by family repeats only cell averages and loads (served by
+// the L2 the tile's three workgroups share).
 // Every load is in bounds: the stencil reaches -1 .. +1 from points of
-// [-1, N] and each slot spans [-g, N + g], g >= 2.
+// [-1, N] x [0, N] x [-1, N + 1] and each slot spans [-g, N + g], g >= 2.
 struct EmfArrays
 {
     const double *Bx, *By, *Bz, *Ax, *Ay, *Az, *U0, *U1, *U2, *F0, *F1, *F2;
     double *E;
     int64_t sl;
-    uint32_t di, dj, dk;
+    uint32_t di, dj;
 };
 
 // xi edges of the planes [i0, i1) at (j, k)
-__device__ __forceinline__ void emf_xi(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+__device__ __forceinline__ void emf_xi(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k, bool own)
 {
-    const int64_t sl = A.sl;
-    const uint32_t di = A.di, dj = A.dj, dk = A.dk;
-    const int sy0 = face_jk_bits(b, 1, j, k), sy1 = face_jk_bits(b, 1, j, k - 1);
-    const int sz0 = face_jk_bits(b, 2, j, k), sz1 = face_jk_bits(b, 2, j - 1, k);
-    const bool store = j <= b.nj && k <= b.nk;
+    const uint32_t di = A.di, dj = A.dj;
+    const int sy0 = face_jk_bits(b, 1, j, k), sz0 = face_jk_bits(b, 2, j, k), sz1 = face_jk_bits(b, 2, j - 1, k);
+    const bool store = own && j <= b.nj;
     uint32_t o = (uint32_t)blk_off(b, i0, j, k);
 #pragma unroll 1
     for (int i = i0; i < i1; ++i, o += di)
@@ -1085,31 +1159,23 @@ __device__ __forceinline__ void emf_xi(const EmfArrays &A, const MhdBlockDev &b,
         const int ib = face_axis_bits(b, 1, 0, i); // eta and zeta faces share the i range
         const double yo = ld32(A.By, o), ayo = ld32(A.Ay, o), yoj = ld32(A.By, o + dj), ayoj = ld32(A.Ay, o + dj);
         const double yj = ld32(A.By, o - dj), ayj = ld32(A.Ay, o - dj);
-        const double yk = ld32(A.By, o - dk), ayk = ld32(A.Ay, o - dk);
-        const double ykj = ld32(A.By, o - dk + dj), aykj = ld32(A.Ay, o - dk + dj);
-        const double yjk = ld32(A.By, o - dj - dk), ayjk = ld32(A.Ay, o - dj - dk);
-        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
-        const double zj = ld32(A.Bz, o - dj), azj = ld32(A.Az, o - dj);
-        const double zjk1 = ld32(A.Bz, o - dj + dk), azjk1 = ld32(A.Az, o - dj + dk);
-        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
-        const double zjk = ld32(A.Bz, o - dj - dk), azjk = ld32(A.Az, o - dj - dk);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zj = ld32(A.Bz, o - dj), azj = ld32(A.Az, o - dj);
         const double u1 = ld32(A.U1, o), u2 = ld32(A.U2, o), ju1 = ld32(A.U1, o - dj), ju2 = ld32(A.U2, o - dj);
-        const double ku1 = ld32(A.U1, o - dk), ku2 = ld32(A.U2, o - dk);
-        const double jku1 = ld32(A.U1, o - dj - dk), jku2 = ld32(A.U2, o - dj - dk);
-        // eta and zeta averages of the cells o, o - dj, o - dk, o - dj - dk
+        // eta and zeta averages of the cells o and o - dj (zeta: B at + dk from the next lane)
+        const double zok = lane_next(zo), azok = lane_next(azo), zjk1 = lane_next(zj), azjk1 = lane_next(azj);
         const double bt1 = 0.5 * ((yo + ayo) + (yoj + ayoj)), bi1 = 0.5 * (yo + yoj);
         const double bt2 = 0.5 * ((zo + azo) + (zok + azok)), bi2 = 0.5 * (zo + zok);
         const double jbt1 = 0.5 * ((yj + ayj) + (yo + ayo)), jbi1 = 0.5 * (yj + yo);
         const double jbt2 = 0.5 * ((zj + azj) + (zjk1 + azjk1)), jbi2 = 0.5 * (zj + zjk1);
-        const double kbt1 = 0.5 * ((yk + ayk) + (ykj + aykj)), kbi1 = 0.5 * (yk + ykj);
-        const double kbt2 = 0.5 * ((zk + azk) + (zo + azo)), kbi2 = 0.5 * (zk + zo);
-        const double jkbt1 = 0.5 * ((yjk + ayjk) + (yk + ayk)), jkbi1 = 0.5 * (yjk + yk);
-        const double jkbt2 = 0.5 * ((zjk + azjk) + (zj + azj)), jkbi2 = 0.5 * (zjk + zj);
-        // eta faces at o (left cell o - dj) and o - dk; zeta faces at o (left o - dk) and o - dj
+        // the cells o - dk and o - dj - dk: the previous lane's o and o - dj
+        const double kbt1 = lane_prev(bt1), kbi1 = lane_prev(bi1), ku1 = lane_prev(u1), ku2 = lane_prev(u2);
+        const double jkbt1 = lane_prev(jbt1), jkbi1 = lane_prev(jbi1), jku1 = lane_prev(ju1), jku2 = lane_prev(ju2);
+        const double yk = lane_prev(yo);
+        // eta face at o (left cell o - dj), the previous lane's at o - dk;
+        // zeta faces at o (left o - dk) and o - dj (left o - dj - dk)
         const double Fy0 = face_pick(face_source_of(ib, sy0),
                                      -emf_pair(jbt2, bt2, jbi2, bi2, ju1, u1, ju2, u2, yo + ayo), A.F1, o);
-        const double nFy0 = face_pick(face_source_of(ib, sy1),
-                                      -emf_pair(jkbt2, kbt2, jkbi2, kbi2, jku1, ku1, jku2, ku2, yk + ayk), A.F1, o - dk);
+        const double nFy0 = lane_prev(Fy0);
         const double Fz0 = face_pick(face_source_of(ib, sz0),
                                      emf_pair(kbt1, bt1, kbi1, bi1, ku2, u2, ku1, u1, zo + azo), A.F2, o);
         const double nFz0 = face_pick(face_source_of(ib, sz1),
@@ -1121,38 +1187,33 @@ __device__ __forceinline__ void emf_xi(const EmfArrays &A, const MhdBlockDev &b,
         if (i < b.ni && store)
             st32(A.E, o, v);
     }
-    (void)sl;
 }
 
 // eta edges of the planes [i0, i1) at (j, k): cells o, o - dk and, carried,
 // o - di, o - di - dk
-__device__ __forceinline__ void emf_eta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+__device__ __forceinline__ void emf_eta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k, bool own)
 {
     const int64_t sl = A.sl;
-    const uint32_t di = A.di, dk = A.dk;
-    const int sx0 = face_jk_bits(b, 0, j, k), sx1 = face_jk_bits(b, 0, j, k - 1), sz0 = face_jk_bits(b, 2, j, k);
-    const bool store = j < b.nj && k <= b.nk;
+    const uint32_t di = A.di;
+    const int sx0 = face_jk_bits(b, 0, j, k), sz0 = face_jk_bits(b, 2, j, k);
+    const bool store = own && j < b.nj;
     uint32_t o = (uint32_t)blk_off(b, i0 - 1, j, k);
-    // plane i0 - 1: zeta averages and velocities of o and o - dk, the zeta
-    // face's candidate along eta, B_zeta at o; xi fluxes of plane i0
-    double cx = ld32(A.Bx, o + di), ca = ld32(A.Ax, o + di), cxk = ld32(A.Bx, o - dk + di), cak = ld32(A.Ax, o - dk + di);
-    double p_bt2, p_bi2, pk_bt2, pk_bi2, pu0, pu2, pku0, pku2, pFz1, pBz;
+    // plane i0 - 1: zeta average and velocity of o, the zeta face's
+    // candidate along eta, B_zeta at o; the xi flux of plane i0
+    double cx = ld32(A.Bx, o + di), ca = ld32(A.Ax, o + di);
+    double p_bt2, p_bi2, pu0, pu2, pFz1, pBz;
     {
-        const double x0 = ld32(A.Bx, o), a0 = ld32(A.Ax, o), xk = ld32(A.Bx, o - dk), ak = ld32(A.Ax, o - dk);
-        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
-        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
+        const double x0 = ld32(A.Bx, o), a0 = ld32(A.Ax, o);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o);
         pu0 = ld32(A.U0, o);
         pu2 = ld32(A.U2, o);
-        pku0 = ld32(A.U0, o - dk);
-        pku2 = ld32(A.U2, o - dk);
+        const double zok = lane_next(zo), azok = lane_next(azo);
         const double bt0 = 0.5 * ((x0 + a0) + (cx + ca)), bi0 = 0.5 * (x0 + cx);
-        const double kbt0 = 0.5 * ((xk + ak) + (cxk + cak)), kbi0 = 0.5 * (xk + cxk);
+        const double kbt0 = lane_prev(bt0), kbi0 = lane_prev(bi0), ku0 = lane_prev(pu0), ku2 = lane_prev(pu2);
         p_bt2 = 0.5 * ((zo + azo) + (zok + azok));
         p_bi2 = 0.5 * (zo + zok);
-        pk_bt2 = 0.5 * ((zk + azk) + (zo + azo));
-        pk_bi2 = 0.5 * (zk + zo);
         pFz1 = face_pick(face_source_of(face_axis_bits(b, 2, 0, i0 - 1), sz0),
-                         -emf_pair(kbt0, bt0, kbi0, bi0, pku2, pu2, pku0, pu0, zo + azo), A.F2 + sl, o);
+                         -emf_pair(kbt0, bt0, kbi0, bi0, ku2, pu2, ku0, pu0, zo + azo), A.F2 + sl, o);
         pBz = zo;
         o += di;
     }
@@ -1161,20 +1222,18 @@ __device__ __forceinline__ void emf_eta(const EmfArrays &A, const MhdBlockDev &b
     {
         const int ibx = face_axis_bits(b, 0, 0, i), ib = face_axis_bits(b, 2, 0, i);
         const double nx = ld32(A.Bx, o + di), na = ld32(A.Ax, o + di);
-        const double nxk = ld32(A.Bx, o - dk + di), nak = ld32(A.Ax, o - dk + di);
-        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o), zok = ld32(A.Bz, o + dk), azok = ld32(A.Az, o + dk);
-        const double zk = ld32(A.Bz, o - dk), azk = ld32(A.Az, o - dk);
-        const double u0 = ld32(A.U0, o), u2 = ld32(A.U2, o), ku0 = ld32(A.U0, o - dk), ku2 = ld32(A.U2, o - dk);
+        const double zo = ld32(A.Bz, o), azo = ld32(A.Az, o);
+        const double u0 = ld32(A.U0, o), u2 = ld32(A.U2, o);
+        const double zok = lane_next(zo), azok = lane_next(azo);
         const double bt0 = 0.5 * ((cx + ca) + (nx + na)), bi0 = 0.5 * (cx + nx);
-        const double kbt0 = 0.5 * ((cxk + cak) + (nxk + nak)), kbi0 = 0.5 * (cxk + nxk);
         const double bt2 = 0.5 * ((zo + azo) + (zok + azok)), bi2 = 0.5 * (zo + zok);
-        const double kbt2 = 0.5 * ((zk + azk) + (zo + azo)), kbi2 = 0.5 * (zk + zo);
-        // xi faces at o (left cell o - di) and o - dk (left o - dk - di); zeta face at o
+        // the cell o - dk (and o - di - dk): the previous lane's o
+        const double kbt0 = lane_prev(bt0), kbi0 = lane_prev(bi0), ku0 = lane_prev(u0), ku2 = lane_prev(u2);
+        const double pku0 = lane_prev(pu0), pku2 = lane_prev(pu2), cxk = lane_prev(cx);
+        // xi face at o (left cell o - di), the previous lane's at o - dk; zeta face at o
         const double Fx1 = face_pick(face_source_of(ibx, sx0),
                                      emf_pair(p_bt2, bt2, p_bi2, bi2, pu0, u0, pu2, u2, cx + ca), A.F0 + sl, o);
-        const double nFx1 = face_pick(face_source_of(ibx, sx1),
-                                      emf_pair(pk_bt2, kbt2, pk_bi2, kbi2, pku0, ku0, pku2, ku2, cxk + cak), A.F0 + sl,
-                                      o - dk);
+        const double nFx1 = lane_prev(Fx1);
         const double Fz1 = face_pick(face_source_of(ib, sz0),
                                      -emf_pair(kbt0, bt0, kbi0, bi0, ku2, u2, ku0, u0, zo + azo), A.F2 + sl, o);
         const double Ec1 = 0.25 * (Fx1 + nFx1 + Fz1 + pFz1);
@@ -1185,29 +1244,23 @@ __device__ __forceinline__ void emf_eta(const EmfArrays &A, const MhdBlockDev &b
             st32(A.E + sl, o, v);
         p_bt2 = bt2;
         p_bi2 = bi2;
-        pk_bt2 = kbt2;
-        pk_bi2 = kbi2;
         pu0 = u0;
         pu2 = u2;
-        pku0 = ku0;
-        pku2 = ku2;
         pFz1 = Fz1;
         pBz = zo;
         cx = nx;
         ca = na;
-        cxk = nxk;
-        cak = nak;
     }
 }
 
 // zeta edges of the planes [i0, i1) at (j, k): cells o, o - dj and, carried,
 // o - di, o - di - dj
-__device__ __forceinline__ void emf_zeta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k)
+__device__ __forceinline__ void emf_zeta(const EmfArrays &A, const MhdBlockDev &b, int i0, int i1, int j, int k, bool own)
 {
     const int64_t sl = A.sl;
     const uint32_t di = A.di, dj = A.dj;
     const int sx0 = face_jk_bits(b, 0, j, k), sx2 = face_jk_bits(b, 0, j - 1, k), sy0 = face_jk_bits(b, 1, j, k);
-    const bool store = j <= b.nj && k < b.nk;
+    const bool store = own && j <= b.nj && k < b.nk;
     uint32_t o = (uint32_t)blk_off(b, i0 - 1, j, k);
     // plane i0 - 1: eta averages and velocities of o and o - dj, the eta
     // face's candidate along zeta, B_eta at o; xi fluxes of plane i0
@@ -1280,8 +1333,8 @@ __global__ __launch_bounds__(256) void k_emf(const MhdTileMap tm)
 {
     MhdBlockDev b;
     int i0, i1, j, k, fam = 0;
-    bool wg;
-    if (!tile_planes(tm, b, i0, i1, j, k, wg, 3, &fam))
+    bool own;
+    if (!tile_padded(tm, b, i0, i1, j, k, own, 3, fam)) // uniform over the workgroup
         return;
     // 32-bit element indices within a slot (a slot is < 2^29 elements): with
     // the slot bases uniform, the loads take an SGPR base and one VGPR offset
@@ -1289,7 +1342,6 @@ __global__ __launch_bounds__(256) void k_emf(const MhdTileMap tm)
     A.sl = b.slot;
     A.di = (uint32_t)b.sj * (uint32_t)b.sk;
     A.dj = (uint32_t)b.sk;
-    A.dk = 1;
     double *S = dst_ptr(b);
     A.Bx = S + (int64_t)ST_B * A.sl;
     A.By = A.Bx + A.sl;
@@ -1305,11 +1357,11 @@ __global__ __launch_bounds__(256) void k_emf(const MhdTileMap tm)
     A.F2 = A.F0 + 6 * A.sl;
     A.E = S + (int64_t)ST_E * A.sl;
     if (fam == 0)
-        emf_xi(A, b, i0, i1, j, k);
+        emf_xi(A, b, i0, i1, j, k, own);
     else if (fam == 1)
-        emf_eta(A, b, i0, i1, j, k);
+        emf_eta(A, b, i0, i1, j, k, own);
     else
-        emf_zeta(A, b, i0, i1, j, k);
+        emf_zeta(A, b, i0, i1, j, k, own);
 }
 
 // ======================================================================
@@ -2956,6 +3008,11 @@ void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dt
         hipLaunchKernelGGL((k_fluid_col<double>), g, bl, 0, q, C.tm, cst, dts, ucur);
 }
 int col_tile_rows() { return COL_TJ; }
+int pad_tiles_per_plane(int nj, int nk, int rows)
+{
+    const int P = (nj + 1) * (nk + 3), nw = (P + PAD_OWN - 1) / PAD_OWN;
+    return (nw + rows - 1) / rows;
+}
 void launch_jedge(const Launch &L, int mixed, hipStream_t q)
 {
     if (mixed)
