@@ -111,7 +111,7 @@ struct TileSet
     TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
     TileArr cols;              // column tiles of the interior cells (kColCi planes each)
     TileArr shell;             // 256 shell points of the face range per tile
-    TileArr pad[3];            // padded-row plane chunks of the face range (kChunkCi[0] planes) [rows]
+    TileArr pad[R_COUNT][3];   // padded-row plane chunks (kChunkCi[0] planes) [range][rows]
     TileArr kl[R_COUNT][3];    // geometry-shared sets: member-looping tiles [range][rows]
 };
 // planes along i per column-tile workgroup
@@ -159,6 +159,9 @@ const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CE
                                     R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_FACE, R_FACE, R_FACE};
 // kernels with a member-looping variant (k_derived_k)
 bool loops_members(int kernel) { return kernel == KP_DERIVED; }
+// kernels over padded-row plane chunks (their lanes take neighbours along k
+// from the lanes beside them), kChunkCi[0] planes a tile
+bool uses_padded_tiles(int kernel) { return kernel == KP_EMF; }
 } // namespace
 
 struct mhd_ctx
@@ -319,8 +322,9 @@ extern "C" int mhd_destroy(mhd_ctx *c)
                         free_t(t.d_start);
             free_t(T.cols.d_start);
             free_t(T.shell.d_start);
-            for (auto &t : T.pad)
-                free_t(t.d_start);
+            for (auto &rr : T.pad)
+                for (auto &t : rr)
+                    free_t(t.d_start);
             free_t(T.d_lead_k);
             free_t(T.d_lead_moff);
             free_t(T.d_members);
@@ -534,19 +538,21 @@ static int build_tiles(mhd_ctx *c)
                             return rc;
                         HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
                     }
-            // padded-row tiles of the face range (mhdk::pad_tiles_per_plane waves
-            // of 62 owned points per plane, kChunkCi[0] planes per tile)
+            // padded-row tiles of each range (mhdk::pad_tiles_per_plane tiles of
+            // 62 owned points a wavefront per plane, kChunkCi[0] planes per tile)
+            for (int r = 0; r < R_COUNT; ++r)
             for (int ji = 0; ji < 3; ++ji)
             {
-                TileArr &A = T.pad[ji];
+                TileArr &A = T.pad[r][ji];
+                const int lo = kRangeLo[r], hi = kRangeHi[r];
                 std::vector<int> start(std::max(T.nent, 1), 0);
                 long long total = 0;
                 for (int e = 0; e < T.nent; ++e)
                 {
                     const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
                     start[e] = (int)total;
-                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni + 1 + kChunkCi[0] - 1) / kChunkCi[0]) *
-                             mhdk::pad_tiles_per_plane(d.nj, d.nk, kJt[ji]);
+                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni - lo + hi + kChunkCi[0] - 1) / kChunkCi[0]) *
+                             mhdk::pad_tiles_per_plane(d.nj - lo + hi, d.nk - lo + hi, kJt[ji]);
                 }
                 if (total > (1LL << 31) - 64)
                     return fail(c, MHD_ERR_MEM, "too many tiles for one launch");
@@ -631,7 +637,8 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     const bool kl = v >= kVariants;
     const TileSet &S = c->ts[set_of(c, kernel)][sh];
     const int ch = chunk_of(kernel);
-    const TileArr &T = kernel == KP_EMF ? S.pad[ji] : kl ? S.kl[r][ji] : ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
+    const bool padded = uses_padded_tiles(kernel) && !kl;
+    const TileArr &T = padded ? S.pad[r][ji] : kl ? S.kl[r][ji] : ch >= 0 ? S.chunks[ch][r][ji] : S.tiles[r][ji];
     const bool ident = S.members.empty();
     mhdk::Launch L;
     L.tm.blks = c->d_blk;
@@ -645,7 +652,7 @@ static mhdk::Launch launch_of(const mhd_ctx *c, int kernel, int jt_index = -1)
     L.tm.hi = kRangeHi[r];
     L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
     L.tm.i_inner = !kl && (v % 6) >= 3;
-    L.tm.ci = ch >= 0 ? kChunkCi[ch] : 0;
+    L.tm.ci = padded ? kChunkCi[0] : ch >= 0 ? kChunkCi[ch] : 0;
     L.tm.kloop = kl;
     L.tm.fold = c->in_step ? 1 : 0;
     L.jt = kJt[ji];

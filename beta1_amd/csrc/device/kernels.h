@@ -29,9 +29,9 @@ void launch_edge_emf(const Launch &L, hipStream_t q);
 // the halo copy of the shell's candidates); every extent >= 2.
 void launch_face_shell(const Launch &L, hipStream_t q);
 void launch_emf(const Launch &L, hipStream_t q);
-// tiles per plane of k_emf's padded-row tiles (62 owned points per wavefront,
-// rows wavefronts per tile) over the face range of an nj x nk plane
-int pad_tiles_per_plane(int nj, int nk, int rows);
+// tiles per plane of the padded-row tiles (62 owned points per wavefront,
+// rows wavefronts per tile) of a launch range ext_j x ext_k points a plane
+int pad_tiles_per_plane(int ext_j, int ext_k, int rows);
 void launch_curl_update(const Launch &L, const double *dts, hipStream_t q);
 void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, int recon,
                   hipStream_t q);

@@ -234,9 +234,9 @@ __device__ __forceinline__ bool tile_planes(const MhdTileMap &tm, MhdBlockDev &b
     return f < plane;
 }
 
-// Padded-row plane-chunk tiles of the face range: the (j, k) plane of
-// [0, N] with each row padded by one point on either side (k = -1 and
-// k = N + 1) and flattened, k fastest.  Wavefront w of a plane takes the 64
+// Padded-row plane-chunk tiles of a launch range: its (j, k) plane with
+// each row padded by one point on either side (k = lo - 1 and the point past
+// the range's last) and flattened, k fastest.  Wavefront w of a plane takes the 64
 // consecutive padded points from 62 w - 1: lanes 1 .. 62 own 62 points, and
 // every owned point's neighbours along k are the lanes beside it, so a
 // kernel can take what they formed by a lane shift instead of loading it.
@@ -273,7 +273,9 @@ __device__ __forceinline__ bool tile_padded(const MhdTileMap &tm, MhdBlockDev &b
     }
     else
         b = tm.blks[bi];
-    const int jt = (int)blockDim.y, ext_i = b.ni + 1, Kp = b.nk + 3, P = (b.nj + 1) * Kp;
+    const int jt = (int)blockDim.y, lo = tm.lo;
+    const int ext_i = b.ni - lo + tm.hi, ext_j = b.nj - lo + tm.hi, ext_k = b.nk - lo + tm.hi;
+    const int Kp = ext_k + 2, P = ext_j * Kp;
     const int nch = (ext_i + ci - 1) / ci, nw = (P + PAD_OWN - 1) / PAD_OWN, ntp = (nw + jt - 1) / jt;
     const int pt = tm.i_inner ? lt / nch : lt % ntp;
     const int iy = tm.i_inner ? lt % nch : lt / ntp;
@@ -281,11 +283,12 @@ __device__ __forceinline__ bool tile_padded(const MhdTileMap &tm, MhdBlockDev &b
     int F = PAD_OWN * w + lane - 1;
     own = lane >= 1 && lane <= PAD_OWN && F < P;
     F = min(max(F, 0), P - 1);
-    j = F / Kp;
-    k = F - j * Kp - 1;
-    own = own && k >= 0 && k <= b.nk;
-    i0 = iy * ci;
-    i1 = min(i0 + ci, ext_i);
+    const int r = F / Kp;
+    j = lo + r;
+    k = lo - 1 + (F - r * Kp);
+    own = own && k >= lo && k < lo + ext_k;
+    i0 = lo + iy * ci;
+    i1 = min(i0 + ci, lo + ext_i);
     return true;
 }
 
@@ -1108,9 +1111,9 @@ __device__ __forceinline__ double max_abs_of4(double a, double b, double c, doub
 // Element idx of a slot through a uniform base and a 32-bit byte offset
 // (idx < 2^29), which the compiler issues as one SGPR base plus one VGPR
 // offset instead of a 64-bit address per access.
-__device__ __forceinline__ double ld32(const double *base, uint32_t idx)
+template <typename T> __device__ __forceinline__ T ld32(const T *base, uint32_t idx)
 {
-    return *(const double *)((const char *)base + (idx << 3));
+    return *(const T *)((const char *)base + idx * (uint32_t)sizeof(T));
 }
 __device__ __forceinline__ void st32(double *base, uint32_t idx, double v)
 {
@@ -3008,9 +3011,9 @@ void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dt
         hipLaunchKernelGGL((k_fluid_col<double>), g, bl, 0, q, C.tm, cst, dts, ucur);
 }
 int col_tile_rows() { return COL_TJ; }
-int pad_tiles_per_plane(int nj, int nk, int rows)
+int pad_tiles_per_plane(int ext_j, int ext_k, int rows)
 {
-    const int P = (nj + 1) * (nk + 3), nw = (P + PAD_OWN - 1) / PAD_OWN;
+    const int P = ext_j * (ext_k + 2), nw = (P + PAD_OWN - 1) / PAD_OWN;
     return (nw + rows - 1) / rows;
 }
 void launch_jedge(const Launch &L, int mixed, hipStream_t q)
