@@ -240,6 +240,12 @@ def bench_ot(args):
         res[recon] = (cells * args.steps / el / 1e6, 1e3 * el / args.steps, ctx.divb_max(cid))
         log(f"orszag-tang recon={recon}: {res[recon][0]:.1f} Mcell-updates/s, {res[recon][1]:.3f} ms/step, "
             f"div B {res[recon][2]:.2e}")
+        ctx.profile(True)
+        ctx.step(4)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        log(f"orszag-tang recon={recon} per-kernel avg ms: "
+            f"{ {k: round(ms / n, 4) for k, (ms, n) in prof.items() if n} }")
         ctx.close()
     return _record("Mcell-updates/s (Orszag-Tang 64^3)", res[0][0], "Mcell-updates/s", args.steps, args.warmup,
                    res[0][1], "f64", {"workload": "orszag_tang_fp64_64^3", "cells": cells, "recon": "first-order"},
