@@ -161,7 +161,7 @@ const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CE
 bool loops_members(int kernel) { return kernel == KP_DERIVED; }
 // kernels over padded-row plane chunks (their lanes take neighbours along k
 // from the lanes beside them), kChunkCi[0] planes a tile
-bool uses_padded_tiles(int kernel) { return kernel == KP_EMF; }
+bool uses_padded_tiles(int kernel) { return kernel == KP_EMF || kernel == KP_AMBI; }
 } // namespace
 
 struct mhd_ctx

@@ -2422,17 +2422,122 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid2_col(const MhdTileMap tm,
 //    taper (<alpha J x B>_4 cells . dr + whistler LLF diffusion of the
 //    transverse face-flux jumps), alpha = hall_coef / ne_eff.  The Ehall sync
 //    of the reference only fills ghosts, which E never reads.
-__device__ __forceinline__ double pe_node(const double *__restrict__ PE, int64_t o, int64_t di, int64_t dj, int64_t dk)
+// The kernel runs over padded-row plane chunks of the points [0, N]^3
+// (tile_padded) and marches along i.  The edges of the three families at o
+// meet the cells o + s0 di + s1 dj + s2 dk, s in {-1, 0}^3 with s_a = 0 for
+// family a: 7 distinct cells (code (s0 + 1) 4 + (s1 + 1) 2 + (s2 + 1)).  A
+// point forms the densities and Hall terms of the cells o (code 7) and o - dj
+// (5); those at o - dk (6, 4) are what the lane before it formed, and those of
+// plane i - 1 (3, 1, and 2 from the lane before) it carries from its previous
+// plane.  Node pressures (the mean of the 8 cells around a node), beta B
+// products, node coordinates and face-area norms are shared the same way:
+// each value is the one the per-point form computes, by the same operations,
+// so E is bit for bit as before.  Every load stays in [-1, N + 1] (the padded
+// lanes sit at k = -1 and N + 1, the warm-up plane at i0 - 1 >= -1).
+struct XCell
+{
+    double rh, rn, e0, e1, e2, bt;
+};
+
+__device__ __forceinline__ XCell xcell_form(const MhdCaseConst &c, bool hall, const double *RH, const double *RN,
+                                            const double *Jc, const double *Bc, int64_t sl, uint32_t p)
+{
+    XCell r;
+    r.rh = ld32(RH, p);
+    r.rn = ld32(RN, p);
+    r.e0 = r.e1 = r.e2 = r.bt = 0.0;
+    if (hall)
+    {
+        const NumInfo nm = num_limiter(c, r.rh, r.rn);
+        const double alpha = c.hall_coef / nm.ne_eff;
+        const double Jx = ld32(Jc, p), Jy = ld32(Jc + sl, p), Jz = ld32(Jc + 2 * sl, p);
+        const double Bx = ld32(Bc, p), By = ld32(Bc + sl, p), Bz = ld32(Bc + 2 * sl, p);
+        r.e0 = alpha * (Jy * Bz - Jz * By);
+        r.e1 = alpha * (Jz * Bx - Jx * Bz);
+        r.e2 = alpha * (Jx * By - Jy * Bx);
+        r.bt = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+    }
+    return r;
+}
+
+__device__ __forceinline__ XCell xcell_prev(const XCell &a, bool hall)
+{
+    XCell r;
+    r.rh = lane_prev(a.rh);
+    r.rn = lane_prev(a.rn);
+    r.e0 = r.e1 = r.e2 = r.bt = 0.0;
+    if (hall)
+    {
+        r.e0 = lane_prev(a.e0);
+        r.e1 = lane_prev(a.e1);
+        r.e2 = lane_prev(a.e2);
+        r.bt = lane_prev(a.bt);
+    }
+    return r;
+}
+
+// mean of the 8 cells around a node, summed in the order (i, j, k) offsets
+// (-1,-1,-1), (-1,-1,0), (-1,0,-1), ... (0,0,0)
+__device__ __forceinline__ double node_mean8(double v0, double v1, double v2, double v3, double v4, double v5,
+                                             double v6, double v7)
 {
     double sum = 0.0;
-#pragma unroll
-    for (int a = -1; a <= 0; ++a)
-#pragma unroll
-        for (int bb = -1; bb <= 0; ++bb)
-#pragma unroll
-            for (int cc = -1; cc <= 0; ++cc)
-                sum += PE[o + a * di + bb * dj + cc * dk];
+    sum += v0;
+    sum += v1;
+    sum += v2;
+    sum += v3;
+    sum += v4;
+    sum += v5;
+    sum += v6;
+    sum += v7;
     return 0.125 * sum;
+}
+
+__device__ __forceinline__ double area_norm(const double *A, int64_t sl, uint32_t p)
+{
+    const double a0 = ld32(A, p), a1 = ld32(A + sl, p), a2 = ld32(A + 2 * sl, p);
+    return sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+}
+
+// ambipolar term of one edge: the 4 cells q0..q3 in the order (s_t1, s_t2) =
+// (-1,-1), (-1,0), (0,-1), (0,0), node pressures at the edge's ends
+__device__ __forceinline__ double ambi_term(const MhdCaseConst &c, const XCell &q0, const XCell &q1, const XCell &q2,
+                                            const XCell &q3, double pe0, double pe1)
+{
+    double rhoH = 0.0, rhoNa = 0.0;
+    rhoH += q0.rh;
+    rhoNa += q0.rn;
+    rhoH += q1.rh;
+    rhoNa += q1.rn;
+    rhoH += q2.rh;
+    rhoNa += q2.rn;
+    rhoH += q3.rh;
+    rhoNa += q3.rn;
+    const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
+    return c.pe_coef * (pe1 - pe0) / nm.ne_eff;
+}
+
+// Hall term of one edge before the taper: <alpha J x B> . dr over the 4 cells
+// plus the whistler diffusion of the transverse face-flux jumps.  mu1 goes
+// with d21 and mu2 with d12 as (Ecen + 0.5 mu1 d21) - 0.5 mu2 d12; the
+// whistler speeds come from the face-area norms (A0, A1) of the two face
+// families along the edge.
+__device__ __forceinline__ void hall_avg(const XCell &q0, const XCell &q1, const XCell &q2, const XCell &q3,
+                                         double d0, double d1, double d2, double &Ecen, double &bmax, double &L)
+{
+    const double Ec0 = 0.25 * (q0.e0 + q1.e0 + q2.e0 + q3.e0);
+    const double Ec1 = 0.25 * (q0.e1 + q1.e1 + q2.e1 + q3.e1);
+    const double Ec2 = 0.25 * (q0.e2 + q1.e2 + q2.e2 + q3.e2);
+    bmax = dmax(dmax(q0.bt, q1.bt), dmax(q2.bt, q3.bt));
+    Ecen = Ec0 * d0 + Ec1 * d1 + Ec2 * d2;
+    L = dmax(sqrt(d0 * d0 + d1 * d1 + d2 * d2), 1e-14);
+}
+__device__ __forceinline__ double whistler_mu(double bmax, double L, double A0, double A1)
+{
+    const double eps = 1e-14, Cwh = 0.1;
+    const double Abar = 0.5 * (A0 + A1);
+    const double inv_h = L / dmax(Abar, eps);
+    return Cwh * bmax * inv_h * inv_h;
 }
 
 template <typename WT>
@@ -2440,142 +2545,177 @@ __global__ __launch_bounds__(256) void k_emf_extra(const MhdTileMap tm, const Mh
                                                    int ucur)
 {
     MhdBlockDev b;
-    int i, j, k;
-    bool wg;
-    if (!tile_cell(tm, b, i, j, k, wg))
+    int i0, i1, j, k, sub;
+    bool own;
+    if (!tile_padded(tm, b, i0, i1, j, k, own, 1, sub)) // uniform over the workgroup
         return;
     const MhdCaseConst &c = cst[b.case_id];
     const bool amb = c.ambipolar != 0.0, loc = c.local_arti != 0.0, hall = c.hall != 0.0;
-    if (!amb && !loc && !hall)
+    if (!amb && !loc && !hall) // one case per block: uniform
         return;
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const int64_t st[3] = {(int64_t)b.sj * b.sk, (int64_t)b.sk, 1};
+    const bool cells = amb || hall, faces = loc || hall;
+    // 32-bit element indices within a slot, slot bases uniform
+    const int64_t sl = b.slot;
+    const uint32_t di = (uint32_t)b.sj * (uint32_t)b.sk, dj = (uint32_t)b.sk;
     double *S = dst_ptr(b);
     const double *PE = S + (int64_t)ST_PE * sl;
     const double *RH = S + (int64_t)(ST_U0 + 5 * ucur) * sl, *RN = S + (int64_t)(ST_UNA0 + 5 * ucur) * sl;
-    const double *Bf = S + (int64_t)ST_B * sl;
+    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
     const double *Bc = S + (int64_t)ST_BCELL * sl, *Jc = S + (int64_t)ST_JCELL * sl;
+    const double *TP = S + (int64_t)ST_TAPER * sl, *EL = S + (int64_t)ST_ETALOC * sl;
+    double *E = S + (int64_t)ST_E * sl;
     const double *G = dmetric(b);
+    const double *X = G + (int64_t)G_NODE * sl, *JD = G + (int64_t)G_JD * sl;
     const WT *Wt = dweights<WT>(b);
     const WT *beta = Wt + (int64_t)(G_BETA - G_NMETRIC) * sl, *alph = Wt + (int64_t)(G_ALPHA - G_NMETRIC) * sl;
-    double *E = S + (int64_t)ST_E * sl;
-    const int idx[3] = {i, j, k}, n[3] = {b.ni, b.nj, b.nk};
-    const double pe0 = amb ? pe_node(PE, o, st[0], st[1], st[2]) : 0.0;
-    // The edges of the three families at o meet the cells o + s0 di + s1 dj
-    // + s2 dk, s in {-1, 0}^3 with s_a = 0 for family a: 7 distinct cells
-    // (code (s0 + 1) 4 + (s1 + 1) 2 + (s2 + 1); code 0 is no family's).  Their
-    // densities and Hall terms are formed once and shared by the families,
-    // with the arithmetic of the per-family form.  The points are [0, N]^3 and
-    // every stencil stays in [-1, N + 1]: all loads are in bounds, and each
-    // family stores on its own range only.
-    double rh[8], rn[8], eh[8][3], bt[8];
-#pragma unroll
-    for (int cc = 1; cc < 8; ++cc)
+
+    // warm-up, plane i0 - 1: cells (i - 1, j, k) and (i - 1, j - 1, k); the
+    // pressures of planes i - 1 and i around node (i, j, k); node (i, j, k);
+    // what the families take from the faces at o - di
+    uint32_t o = (uint32_t)blk_off(b, i0 - 1, j, k);
+    XCell c3 = {}, c1 = {};
+    if (cells)
     {
-        const int64_t p = o + ((cc >> 2) - 1) * st[0] + (((cc >> 1) & 1) - 1) * st[1] + ((cc & 1) - 1) * st[2];
-        rh[cc] = RH[p];
-        rn[cc] = RN[p];
-        if (hall)
-        {
-            const NumInfo nm = num_limiter(c, rh[cc], rn[cc]);
-            const double alpha = c.hall_coef / nm.ne_eff;
-            const double Jx = Jc[p], Jy = Jc[sl + p], Jz = Jc[2 * sl + p];
-            const double Bx = Bc[p], By = Bc[sl + p], Bz = Bc[2 * sl + p];
-            eh[cc][0] = alpha * (Jy * Bz - Jz * By);
-            eh[cc][1] = alpha * (Jz * Bx - Jx * Bz);
-            eh[cc][2] = alpha * (Jx * By - Jy * Bx);
-            bt[cc] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
-        }
+        c3 = xcell_form(c, hall, RH, RN, Jc, Bc, sl, o);
+        c1 = xcell_form(c, hall, RH, RN, Jc, Bc, sl, o - dj);
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
+    double pe0 = 0.0, pm = 0.0, pj = 0.0, pjj = 0.0, qj = 0.0; // P[i][j-1], P[i][j], P[i-1][j+1], P[i-1][j]
+    if (amb)
     {
-        bool in = true;
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            in = in && (idx[q] < n[q] + (q == a ? 0 : 1));
-        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-        // the family's 4 cells in the order (s_t1, s_t2) = (-1,-1), (-1,0), (0,-1), (0,0)
-        int cq[4];
+        const double a0 = ld32(PE, o - dj), a1 = ld32(PE, o), b0 = ld32(PE, o + di - dj), b1 = ld32(PE, o + di);
+        pe0 = node_mean8(lane_prev(a0), a0, lane_prev(a1), a1, lane_prev(b0), b0, lane_prev(b1), b1);
+        pm = b0;
+        pj = b1;
+        pjj = ld32(PE, o + dj);
+        qj = a1;
+    }
+    double xo0 = 0.0, xo1 = 0.0, xo2 = 0.0, pn1 = 0.0, pn2 = 0.0, pBy = 0.0, pBz = 0.0, ppy = 0.0, ppz = 0.0;
+    if (hall)
+    {
+        xo0 = ld32(X, o + di);
+        xo1 = ld32(X + sl, o + di);
+        xo2 = ld32(X + 2 * sl, o + di);
+        pn1 = area_norm(JD + 3 * sl, sl, o);
+        pn2 = area_norm(JD + 6 * sl, sl, o);
+    }
+    if (faces)
+    {
+        pBy = ld32(By, o);
+        pBz = ld32(Bz, o);
+    }
+    if (loc)
+    {
+        ppy = ld32(beta + sl, o) * pBy;
+        ppz = ld32(beta + 2 * sl, o) * pBz;
+    }
+    o += di;
+    const bool st0 = own && j <= b.nj, st1 = own && j < b.nj, st2 = own && j <= b.nj && k < b.nk;
+#pragma unroll 1
+    for (int i = i0; i < i1; ++i, o += di)
+    {
+        double e0 = ld32(E, o), e1 = ld32(E + sl, o), e2 = ld32(E + 2 * sl, o);
+        XCell c7 = {}, c5 = {};
+        if (cells)
         {
-            int q = 0;
-#pragma unroll
-            for (int s1 = 0; s1 <= 1; ++s1)
-#pragma unroll
-                for (int s2 = 0; s2 <= 1; ++s2)
-                {
-                    int sv[3] = {1, 1, 1};
-                    sv[t1] = s1;
-                    sv[t2] = s2;
-                    cq[q++] = sv[0] * 4 + sv[1] * 2 + sv[2];
-                }
+            c7 = xcell_form(c, hall, RH, RN, Jc, Bc, sl, o);
+            c5 = xcell_form(c, hall, RH, RN, Jc, Bc, sl, o - dj);
         }
-        double e = E[a * sl + o];
+        const XCell c6 = xcell_prev(c7, hall), c4 = xcell_prev(c5, hall), c2 = xcell_prev(c3, hall);
+        double pex = 0.0;
         if (amb)
         {
-            const double pe1 = pe_node(PE, o + st[a], st[0], st[1], st[2]);
-            double rhoH = 0.0, rhoNa = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                rhoH += rh[cq[q]];
-                rhoNa += rn[cq[q]];
-            }
-            const NumInfo nm = num_limiter(c, 0.25 * rhoH, 0.25 * rhoNa);
-            e -= c.pe_coef * (pe1 - pe0) / nm.ne_eff;
+            // nodes (i + 1, j, k), (i, j + 1, k), (i, j, k + 1)
+            const double a0 = ld32(PE, o + di - dj), a1 = ld32(PE, o + di), cj = ld32(PE, o + dj);
+            pex = node_mean8(lane_prev(pm), pm, lane_prev(pj), pj, lane_prev(a0), a0, lane_prev(a1), a1);
+            const double pey = node_mean8(lane_prev(qj), qj, lane_prev(pjj), pjj, lane_prev(pj), pj, lane_prev(cj), cj);
+            const double pez = lane_next(pe0);
+            e0 -= ambi_term(c, c4, c5, c6, c7, pe0, pex);
+            e1 -= ambi_term(c, c2, c3, c6, c7, pe0, pey);
+            e2 -= ambi_term(c, c1, c3, c5, c7, pe0, pez);
+            qj = pj;
+            pj = a1;
+            pm = a0;
+            pjj = cj;
         }
+        double bxo = 0.0, byo = 0.0, bzo = 0.0, bxj = 0.0, bzj = 0.0;
+        if (faces)
+        {
+            bxo = ld32(Bx, o);
+            byo = ld32(By, o);
+            bzo = ld32(Bz, o);
+            bxj = ld32(Bx, o - dj);
+            bzj = ld32(Bz, o - dj);
+        }
+        double py = 0.0, pz = 0.0;
         if (loc)
         {
-            // J edge of family a (CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110)
-            const int64_t u = st[t1], v = st[t2];
-            const double *Bu = Bf + (int64_t)t1 * sl, *Bv = Bf + (int64_t)t2 * sl;
-            const WT *bu = beta + (int64_t)t1 * sl, *bv = beta + (int64_t)t2 * sl;
-            double val;
-            if (a == 1) // eta edge: (bx Bx(o) - bx Bx(o-dk)) + (bz Bz(o-di) - bz Bz(o))
-                val = (bu[o] * Bu[o] - bu[o - v] * Bu[o - v]) + (bv[o - u] * Bv[o - u] - bv[o] * Bv[o]);
-            else // xi: (by By(o-dk) - by By(o)) + (bz Bz(o) - bz Bz(o-dj)); zeta: (bx Bx(o-dj) - bx Bx(o)) + (by By(o) - by By(o-di))
-                val = (bu[o - v] * Bu[o - v] - bu[o] * Bu[o]) + (bv[o] * Bv[o] - bv[o - u] * Bv[o - u]);
-            const double Je = (1.0 * val) * alph[(int64_t)a * sl + o];
-            e += S[(int64_t)(ST_ETALOC + a) * sl + o] * Je;
+            // J edges alpha CurlAdj(beta B) (CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:59-110)
+            const double px = ld32(beta, o) * bxo;
+            py = ld32(beta + sl, o) * byo;
+            pz = ld32(beta + 2 * sl, o) * bzo;
+            const double pxj = ld32(beta, o - dj) * bxj, pzj = ld32(beta + 2 * sl, o - dj) * bzj;
+            const double pxk = lane_prev(px), pyk = lane_prev(py);
+            const double J0 = (1.0 * ((pyk - py) + (pz - pzj))) * ld32(alph, o);
+            const double J1 = (1.0 * ((px - pxk) + (ppz - pz))) * ld32(alph + sl, o);
+            const double J2 = (1.0 * ((pxj - px) + (py - ppy))) * ld32(alph + 2 * sl, o);
+            e0 += ld32(EL, o) * J0;
+            e1 += ld32(EL + sl, o) * J1;
+            e2 += ld32(EL + 2 * sl, o) * J2;
         }
+        double n1 = 0.0, n2 = 0.0, xn0 = 0.0, xn1 = 0.0, xn2 = 0.0;
         if (hall)
         {
-            double Ec[3];
-#pragma unroll
-            for (int m = 0; m < 3; ++m)
-                Ec[m] = 0.25 * (eh[cq[0]][m] + eh[cq[1]][m] + eh[cq[2]][m] + eh[cq[3]][m]);
-            const double bmax = dmax(dmax(bt[cq[0]], bt[cq[1]]), dmax(bt[cq[2]], bt[cq[3]]));
-            // edge vector dr = node(o + st[a]) - node(o) (src/3_field/4_MPCNS_Geometry.cpp:1251-1320)
-            const double *X = dmetric(b) + (int64_t)G_NODE * sl;
-            const int64_t o2 = o + st[a];
-            const double d0 = X[o2] - X[o], d1 = X[sl + o2] - X[sl + o], d2 = X[2 * sl + o2] - X[2 * sl + o];
-            const double Ecen = Ec[0] * d0 + Ec[1] * d1 + Ec[2] * d2;
-            const double eps = 1e-14, Cwh = 0.1;
-            const double L = dmax(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
-            double mu[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int f = 0; f < 3; ++f)
+            // edge vectors dr = node(o + st[a]) - node(o) (src/3_field/4_MPCNS_Geometry.cpp:1251-1320)
+            xn0 = ld32(X, o + di);
+            xn1 = ld32(X + sl, o + di);
+            xn2 = ld32(X + 2 * sl, o + di);
+            const double xj0 = ld32(X, o + dj), xj1 = ld32(X + sl, o + dj), xj2 = ld32(X + 2 * sl, o + dj);
+            const double xk0 = lane_next(xo0), xk1 = lane_next(xo1), xk2 = lane_next(xo2);
+            const double n0 = area_norm(JD, sl, o);
+            n1 = area_norm(JD + 3 * sl, sl, o);
+            n2 = area_norm(JD + 6 * sl, sl, o);
+            const double n0j = area_norm(JD, sl, o - dj), n2j = area_norm(JD + 6 * sl, sl, o - dj);
+            const double n0k = lane_prev(n0), n1k = lane_prev(n1);
+            const double byk = lane_prev(byo), bxk = lane_prev(bxo);
+            double Ecen, bmax, L;
             {
-                if (f == a)
-                    continue;
-                const int oax = 3 - a - f; // face family f averaged along axis oax
-                const double *A = G + (int64_t)(G_JD + 3 * f) * sl;
-                const int64_t o1 = o - st[oax];
-                const double A0 = sqrt(A[o] * A[o] + A[sl + o] * A[sl + o] + A[2 * sl + o] * A[2 * sl + o]);
-                const double A1 = sqrt(A[o1] * A[o1] + A[sl + o1] * A[sl + o1] + A[2 * sl + o1] * A[2 * sl + o1]);
-                const double Abar = 0.5 * (A0 + A1);
-                const double inv_h = L / dmax(Abar, eps);
-                mu[oax] = Cwh * bmax * inv_h * inv_h;
+                hall_avg(c4, c5, c6, c7, xn0 - xo0, xn1 - xo1, xn2 - xo2, Ecen, bmax, L);
+                const double mu2 = whistler_mu(bmax, L, n1, n1k), mu1 = whistler_mu(bmax, L, n2, n2j);
+                const double h = Ecen + 0.5 * mu1 * (bzo - bzj) - 0.5 * mu2 * (byo - byk);
+                e0 += ld32(TP, o) * h;
             }
-            const double *F1 = Bf + (int64_t)t1 * sl, *F2 = Bf + (int64_t)t2 * sl;
-            const double dB21 = F2[o] - F2[o - st[t1]]; // family t2 across t1
-            const double dB12 = F1[o] - F1[o - st[t2]]; // family t1 across t2
-            const double h = a == 1 ? Ecen + 0.5 * mu[t2] * dB12 - 0.5 * mu[t1] * dB21
-                                    : Ecen + 0.5 * mu[t1] * dB21 - 0.5 * mu[t2] * dB12;
-            e += S[(int64_t)(ST_TAPER + a) * sl + o] * h;
+            {
+                hall_avg(c2, c3, c6, c7, xj0 - xo0, xj1 - xo1, xj2 - xo2, Ecen, bmax, L);
+                const double mu2 = whistler_mu(bmax, L, n0, n0k), mu0 = whistler_mu(bmax, L, n2, pn2);
+                const double h = Ecen + 0.5 * mu2 * (bxo - bxk) - 0.5 * mu0 * (bzo - pBz);
+                e1 += ld32(TP + sl, o) * h;
+            }
+            {
+                hall_avg(c1, c3, c5, c7, xk0 - xo0, xk1 - xo1, xk2 - xo2, Ecen, bmax, L);
+                const double mu1 = whistler_mu(bmax, L, n0, n0j), mu0 = whistler_mu(bmax, L, n1, pn1);
+                const double h = Ecen + 0.5 * mu0 * (byo - pBy) - 0.5 * mu1 * (bxo - bxj);
+                e2 += ld32(TP + 2 * sl, o) * h;
+            }
         }
-        if (in)
-            E[a * sl + o] = e;
+        if (i < b.ni && st0)
+            st32(E, o, e0);
+        if (i <= b.ni && st1)
+            st32(E + sl, o, e1);
+        if (i <= b.ni && st2)
+            st32(E + 2 * sl, o, e2);
+        // carry plane i to plane i + 1
+        c3 = c7;
+        c1 = c5;
+        pe0 = pex;
+        xo0 = xn0;
+        xo1 = xn1;
+        xo2 = xn2;
+        pn1 = n1;
+        pn2 = n2;
+        pBy = byo;
+        pBz = bzo;
+        ppy = py;
+        ppz = pz;
     }
 }
 
