@@ -34,10 +34,8 @@ KERNEL_PARTS = {
     "k_curl_update": (24 + 24 + 24, 0, 0),         # E, B -> B
     # U, B, Badd -> UC, B_cell (+ next step's dt) | JD, Jac | B_cell least squares Mi (6), mu (6)
     "k_derived": (40 + 24 + 24 + 24 + 24, 72 + 8, 96),
-    # B -> J edges | beta, alpha
-    "k_jedge": (24 + 24, 0, 24 + 24),
-    # J edges -> J_cell | nodes | Mi (6), active-edge mask
-    "k_jcell": (24 + 24, 24, 48 + 8),
+    # B -> J_cell (J edges formed in the pass) | nodes | beta, alpha, Mi (6), active-edge mask
+    "k_jcell": (24 + 24, 24, 24 + 24 + 48 + 8),
     # two-fluid step: U_H, U_Na -> U_H', U_Na', B_cell, J_cell, p_e, photoionization, drag | JD, Jac
     "k_fluid2": (80 + 80 + 24 + 24 + 8 + 8 + 8, 72 + 8, 0),
     # U_H, U_Na, B, Badd -> UC, p_e, B_cell (+ dt) | JD, Jac | Mi, mu
@@ -166,7 +164,7 @@ def bench_batch(args):
                    step_hbm={"bytes_per_cell_update": round(step_bytes, 1), "frac": round(hbm, 4),
                              "geometry_sharing_K": round(K, 2) if loop else 1,
                              "geometry_sharing": {k: v[2] for k, v in ctx.tuning().items()
-                                                  if k in ("k_derived", "k_fluid", "k_jcell", "k_jedge")}},
+                                                  if k in ("k_derived", "k_fluid", "k_jcell")}},
                    per_kernel=per_kernel)
 
 

@@ -146,7 +146,6 @@ enum KernelId
     KP_ERES,
     KP_JCELL_RES,
     KP_CURL_RES,
-    KP_JEDGE,
     KP_EMF,   // face + edge EMF in one pass (whole steps)
     KP_SHELL, // face EMF of the shell points before the Eface sync (whole steps)
     KP_COUNT
@@ -154,14 +153,17 @@ enum KernelId
 const char *kKernelName[KP_COUNT] = {"k_dt",      "k_dt_finalize", "k_ucontra", "k_face_emf", "k_edge_emf",
                                      "k_fluid",   "k_curl_update", "k_derived", "k_ops",      "k_jcell",
                                      "k_fluid2",  "k_derived2",    "k_emf_extra", "k_eres",     "k_jcell_res",
-                                     "k_curl_res", "k_jedge",      "k_emf",     "k_face_shell"};
-const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL, R_CELL,
-                                    R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_FACE, R_FACE, R_FACE};
+                                     "k_curl_res", "k_emf",       "k_face_shell"};
+const int kKernelRange[KP_COUNT] = {R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_EXT, R_CELL,
+                                    R_CELL, R_CELL, R_EXT,  R_FACE, R_FACE, R_CELL, R_FACE, R_FACE, R_FACE};
 // kernels with a member-looping variant (k_derived_k)
 bool loops_members(int kernel) { return kernel == KP_DERIVED; }
 // kernels over padded-row plane chunks (their lanes take neighbours along k
 // from the lanes beside them), kChunkCi[0] planes a tile
-bool uses_padded_tiles(int kernel) { return kernel == KP_EMF || kernel == KP_AMBI; }
+bool uses_padded_tiles(int kernel)
+{
+    return kernel == KP_EMF || kernel == KP_AMBI || kernel == KP_JCELL || kernel == KP_JCELL_RES;
+}
 } // namespace
 
 struct mhd_ctx
@@ -618,14 +620,12 @@ static int set_of(const mhd_ctx *c, int kernel)
 
 // Plane-chunk size of a kernel (index into kChunkCi), -1 for one plane per
 // tile; measured per kernel on the 6 x 64 x 64 x 128 shell: the face EMF
-// 152 -> 102 us at 4 planes, J_cell 153 -> 128 us at 8; the edge EMF is
-// faster per point (its carried values halve the occupancy).
+// 152 -> 102 us at 4 planes; the edge EMF is faster per point (its carried
+// values halve the occupancy).  The padded-row kernels take kChunkCi[0].
 static int chunk_of(int kernel)
 {
     if (kernel == KP_FACE || kernel == KP_EMF)
         return 0;
-    if (kernel == KP_JCELL || kernel == KP_JCELL_RES)
-        return 1;
     return -1;
 }
 
@@ -1075,12 +1075,8 @@ static void join_branch(mhd_ctx *c, int b)
 static void derived(mhd_ctx *c)
 {
     {
-        // J edges and J_cell read only B (synced) and the geometry: beside B_cell / U_plus / dt
+        // J_cell (with its J edges) reads only B (synced) and the geometry: beside B_cell / U_plus / dt
         const hipStream_t qb = fork_branch(c, 1);
-        {
-            ProfScope p(c, KP_JEDGE);
-            mhdk::launch_jedge(launch_of(c, KP_JEDGE), mixed_of(c), qb);
-        }
         ProfScope p(c, KP_JCELL);
         mhdk::launch_jcell(launch_of(c, KP_JCELL), mixed_of(c), qb);
     }
@@ -1226,7 +1222,6 @@ static void stage_resistive(mhd_ctx *c)
         }
         {
             ProfScope p(c, KP_JCELL_RES);
-            mhdk::launch_jedge(launch_of(c, KP_CURL_RES), mixed, c->q); // two-fluid blocks, face range
             mhdk::launch_jcell(launch_of(c, KP_JCELL_RES), mixed, c->q);
         }
         {
@@ -1301,7 +1296,7 @@ static int autotune(mhd_ctx *c)
     if (c->h_blk.empty())
         return MHD_OK;
     const int mixed = c->opt.precision == MHD_PREC_MIXED;
-    const int kernels[] = {KP_FACE,  KP_EDGE,  KP_FLUID,  KP_CURL,     KP_DERIVED, KP_JEDGE,
+    const int kernels[] = {KP_FACE,  KP_EDGE,     KP_FLUID, KP_CURL, KP_DERIVED,
                            KP_JCELL, KP_FLUID2, KP_DERIVED2, KP_AMBI, KP_EMF};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
@@ -1336,9 +1331,6 @@ static int autotune(mhd_ctx *c)
                     break;
                 case KP_JCELL:
                     mhdk::launch_jcell(L, mixed, c->q);
-                    break;
-                case KP_JEDGE:
-                    mhdk::launch_jedge(L, mixed, c->q);
                     break;
                 case KP_FLUID2:
                     launch_fluid2_any(c, ji, c->d_zero, c->q);

@@ -40,7 +40,6 @@ void launch_fluid(const Launch &L, const MhdCaseConst *cst, const double *dts, i
 void launch_fluid_col(const Launch &C, const MhdCaseConst *cst, const double *dts, int ucur, int mixed, hipStream_t q);
 int col_tile_rows();
 // J edges (face range), then J_cell from them (interior cells).
-void launch_jedge(const Launch &L, int mixed, hipStream_t q);
 void launch_jcell(const Launch &L, int mixed, hipStream_t q);
 void launch_derived(const Launch &L, const MhdCaseConst *cst, unsigned long long *dtbits, int ucur, int mixed,
                     hipStream_t q);

@@ -1849,75 +1849,89 @@ __global__ __launch_bounds__(256) void k_derived_k(const MhdTileMap tm, const Mh
     }
 }
 
-// Derived fields, part 2: Calc_J_Edge (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:
-// 777-830; CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:
-// 59-110), J = alpha (CurlAdj of the beta-weighted face fluxes), each edge
-// of [0, N] once (its family's own range, as the reference's EdgeXi/Et/Ze
-// fields), into the J-edge scratch slots.
-template <typename WT>
-__global__ __launch_bounds__(256) void k_jedge(const MhdTileMap tm)
+// Derived fields, part 2, interior cells: Calc_J_Edge then calc_Jcell
+// (Z4_Lunar/src/physics/LunarSolver_Tools.cpp:777-891) in one pass.  A J
+// edge is alpha CurlAdj of the beta-weighted face fluxes
+// (CTOperators::CurlAdjFaceToEdge, Z4_Lunar/src/operators/CTOperators.cpp:
+// 59-110); J_cell is sum_n W[c][n] J_n over the cell's 12 edges, with
+// Jcell_from_Jedge_w formed in place: W[c][n] = Mi[c] . e_n for the active
+// edges, e_n = node(end) - node(start), with the host builder's arithmetic,
+// accumulated in the reference's order n = 0..11.
+//
+// The cube's nodes are nd[0..3] = o + {0, dj, dk, dj + dk} on plane i and
+// nd[4..7] the same + di; edge n runs from node kEs[n] to kEe[n] (xi edges
+// 0-3, eta 4-7, zeta 8-11).  Over padded-row plane chunks (tile_padded) of
+// the cells [0, N)^3, a thread marching along i forms the J edges of plane
+// i + 1 at (j, k) and (j + 1, k) -- the xi and zeta edges there, the eta
+// edge at (j, k) -- and carries those of plane i; the edges and nodes at
+// + dk are what the lane after it formed, the beta B products at - dk what
+// the lane before it formed.  Each J edge is the value the per-edge form
+// stores, by the same operations, so J_cell is bit for bit as from stored
+// edges.  Loads reach planes i0 - 1 .. N, rows -1 .. N and k = -1 .. N (the
+// padded lanes).
+struct JEdges
 {
-    MhdBlockDev b;
-    int i, j, k;
-    bool wg;
-    if (!tile_cell(tm, b, i, j, k, wg))
-        return;
-    const int64_t sl = b.slot, o = blk_off(b, i, j, k);
-    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
-    double *S = dst_ptr(b);
-    const WT *G = dweights<WT>(b);
-    const double *Bx = S + (int64_t)ST_B * sl, *By = Bx + sl, *Bz = Bx + 2 * sl;
-    const WT *bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl, *by = bx + sl, *bz = bx + 2 * sl;
-    const WT *ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl, *ay = ax + sl, *az = ax + 2 * sl;
-    double *JE = S + (int64_t)ST_JE * sl;
-    // points [0, N]^3, stencil o - 1 .. o per axis: in bounds everywhere, so
-    // the three families' loads issue together; each stores on its own range
-    // (xi edges: i in [0, N), j, k in [0, N]; and so on)
-    const double vx = (by[o - dk] * By[o - dk] - by[o] * By[o]) + (bz[o] * Bz[o] - bz[o - dj] * Bz[o - dj]);
-    const double vy = (bx[o] * Bx[o] - bx[o - dk] * Bx[o - dk]) + (bz[o - di] * Bz[o - di] - bz[o] * Bz[o]);
-    const double vz = (bx[o - dj] * Bx[o - dj] - bx[o] * Bx[o]) + (by[o] * By[o] - by[o - di] * By[o - di]);
-    const double jx = (1.0 * vx) * ax[o], jy = (1.0 * vy) * ay[o], jz = (1.0 * vz) * az[o];
-    if (i < b.ni)
-        JE[o] = jx;
-    if (j < b.nj)
-        JE[sl + o] = jy;
-    if (k < b.nk)
-        JE[2 * sl + o] = jz;
+    double jx, jxj, jy, jz, jzj; // xi at q, q + dj; eta at q; zeta at q, q + dj
+    double py, pz, pyj;          // beta B products at q (eta, zeta faces) and q + dj (eta face)
+};
+
+template <typename WT> struct JArrays
+{
+    const double *Bx, *By, *Bz;
+    const WT *bx, *by, *bz, *ax, *ay, *az;
+};
+
+// the J edges of plane q given the beta B products of plane q - di
+template <typename WT>
+__device__ __forceinline__ JEdges jedges_of(const JArrays<WT> &A, uint32_t q, uint32_t dj, double ppy, double ppz,
+                                            double ppyj)
+{
+    JEdges r;
+    const double px = ld32(A.bx, q) * ld32(A.Bx, q);
+    r.py = ld32(A.by, q) * ld32(A.By, q);
+    r.pz = ld32(A.bz, q) * ld32(A.Bz, q);
+    const double pxj = ld32(A.bx, q + dj) * ld32(A.Bx, q + dj);
+    r.pyj = ld32(A.by, q + dj) * ld32(A.By, q + dj);
+    const double pzj = ld32(A.bz, q + dj) * ld32(A.Bz, q + dj);
+    const double pxm = ld32(A.bx, q - dj) * ld32(A.Bx, q - dj), pzm = ld32(A.bz, q - dj) * ld32(A.Bz, q - dj);
+    const double pxk = lane_prev(px), pyk = lane_prev(r.py), pyjk = lane_prev(r.pyj);
+    // xi: (by By(-dk) - by By) + (bz Bz - bz Bz(-dj)); eta: (bx Bx - bx Bx(-dk)) +
+    // (bz Bz(-di) - bz Bz); zeta: (bx Bx(-dj) - bx Bx) + (by By - by By(-di))
+    r.jx = (1.0 * ((pyk - r.py) + (r.pz - pzm))) * ld32(A.ax, q);
+    r.jxj = (1.0 * ((pyjk - r.pyj) + (pzj - r.pz))) * ld32(A.ax, q + dj);
+    r.jy = (1.0 * ((px - pxk) + (ppz - r.pz))) * ld32(A.ay, q);
+    r.jz = (1.0 * ((pxm - px) + (r.py - ppy))) * ld32(A.az, q);
+    r.jzj = (1.0 * ((px - pxj) + (r.pyj - ppyj))) * ld32(A.az, q + dj);
+    return r;
 }
 
-// Derived fields, part 3, interior cells: calc_Jcell (LunarSolver_Tools.cpp:
-// 833-891) from the cell's 12 J edges.  Jcell_from_Jedge_w is formed in
-// place: W[c][n] = Mi[c] . e_n for the active edges, e_n = node(end) -
-// node(start), with the host builder's arithmetic, so J_cell accumulates the
-// reference's W[c][n] * J_n in the reference's order n = 0..11.
-//
-// Over plane-chunk tiles: the cube's nodes are nd[0..3] = o + {0, dj, dk,
-// dj + dk} on plane i and nd[4..7] the same + di; edge n runs from node
-// kEs[n] to kEe[n] (xi edges 0-3, eta 4-7, zeta 8-11 as the J-edge
-// families).  The plane-(i + 1) nodes and the eta / zeta J edges at + di are
-// the next plane's own, so they are carried in registers.
-// (At 4 waves per SIMD -- 128 VGPRs, a few spilled -- it runs 102 us on the
-// 6 x 64^2 x 128 shell, 109 us with the compiler's own 165 VGPRs and 3 waves.)
-template <typename WT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_jcell(const MhdTileMap tm)
+template <typename WT> __global__ __launch_bounds__(256) void k_jcell(const MhdTileMap tm)
 {
     constexpr int kEs[12] = {0, 1, 2, 3, 0, 4, 2, 6, 0, 4, 1, 5};
     constexpr int kEe[12] = {4, 5, 6, 7, 1, 5, 3, 7, 2, 6, 3, 7};
     MhdBlockDev b;
-    int i0, i1, j, k;
-    bool wg;
-    if (!tile_planes(tm, b, i0, i1, j, k, wg))
+    int i0, i1, j, k, sub;
+    bool own;
+    if (!tile_padded(tm, b, i0, i1, j, k, own, 1, sub)) // uniform over the workgroup
         return;
     const int64_t sl = b.slot;
-    const int64_t di = (int64_t)b.sj * b.sk, dj = b.sk, dk = 1;
+    const uint32_t di = (uint32_t)b.sj * (uint32_t)b.sk, dj = (uint32_t)b.sk;
     double *S = dst_ptr(b);
     const WT *G = dweights<WT>(b);
-    const double *J0 = S + (int64_t)ST_JE * sl, *J1 = J0 + sl, *J2 = J0 + 2 * sl;
-    const double *X = dmetric(b) + (int64_t)G_NODE * sl;
+    JArrays<WT> A;
+    A.Bx = S + (int64_t)ST_B * sl;
+    A.By = A.Bx + sl;
+    A.Bz = A.Bx + 2 * sl;
+    A.bx = G + (int64_t)(G_BETA - G_NMETRIC) * sl;
+    A.by = A.bx + sl;
+    A.bz = A.bx + 2 * sl;
+    A.ax = G + (int64_t)(G_ALPHA - G_NMETRIC) * sl;
+    A.ay = A.ax + sl;
+    A.az = A.ax + 2 * sl;
+    const double *X = dmetric(b) + (int64_t)G_NODE * sl, *Y = X + sl, *Z = X + 2 * sl;
     const WT *Mw = G + (int64_t)(G_WJM - G_NMETRIC) * sl;
     const WT *Aw = G + (int64_t)(G_WJA - G_NMETRIC) * sl;
     double *Jc = S + (int64_t)ST_JCELL * sl;
-    const int64_t q[4] = {0, dj, dk, dj + dk};
     // solid wall: J_cell = 0 on the two cell layers next to it
     // (Z4_Lunar/src/boundary/Boundary_Condition_EM.cpp:382-427)
     bool wall_jk = false;
@@ -1929,29 +1943,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         wall_jk = true;
     if (b.bc[5] == BC_SOLID && k >= b.nk - 2)
         wall_jk = true;
-    int64_t o = blk_off(b, i0, j, k);
+    uint32_t o = (uint32_t)blk_off(b, i0, j, k);
+    // plane i0: its J edges from the products of plane i0 - 1, its nodes
+    JEdges cur;
+    {
+        const uint32_t m = o - di;
+        const double ppy = ld32(A.by, m) * ld32(A.By, m), ppz = ld32(A.bz, m) * ld32(A.Bz, m);
+        const double ppyj = ld32(A.by, m + dj) * ld32(A.By, m + dj);
+        cur = jedges_of(A, o, dj, ppy, ppz, ppyj);
+    }
     double nd[8][3];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            nd[m][c] = X[(int64_t)c * sl + o + q[m]];
-    double j1a = J1[o], j1b = J1[o + dk], j2a = J2[o], j2b = J2[o + dj];
+    nd[0][0] = ld32(X, o);
+    nd[0][1] = ld32(Y, o);
+    nd[0][2] = ld32(Z, o);
+    nd[1][0] = ld32(X, o + dj);
+    nd[1][1] = ld32(Y, o + dj);
+    nd[1][2] = ld32(Z, o + dj);
 #pragma unroll 1
     for (int i = i0; i < i1; ++i, o += di)
     {
+        const JEdges nx = jedges_of(A, o + di, dj, cur.py, cur.pz, cur.pyj);
+        nd[4][0] = ld32(X, o + di);
+        nd[4][1] = ld32(Y, o + di);
+        nd[4][2] = ld32(Z, o + di);
+        nd[5][0] = ld32(X, o + di + dj);
+        nd[5][1] = ld32(Y, o + di + dj);
+        nd[5][2] = ld32(Z, o + di + dj);
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                nd[4 + m][c] = X[(int64_t)c * sl + o + di + q[m]];
+        for (int c = 0; c < 3; ++c)
+        {
+            nd[2][c] = lane_next(nd[0][c]);
+            nd[3][c] = lane_next(nd[1][c]);
+            nd[6][c] = lane_next(nd[4][c]);
+            nd[7][c] = lane_next(nd[5][c]);
+        }
         // Mi is symmetric: rows (00 01 02) (01 11 12) (02 12 22)
-        const double m00 = Mw[o], m01 = Mw[sl + o], m02 = Mw[2 * sl + o], m11 = Mw[3 * sl + o], m12 = Mw[4 * sl + o],
-                     m22 = Mw[5 * sl + o];
+        const double m00 = ld32(Mw, o), m01 = ld32(Mw + sl, o), m02 = ld32(Mw + 2 * sl, o),
+                     m11 = ld32(Mw + 3 * sl, o), m12 = ld32(Mw + 4 * sl, o), m22 = ld32(Mw + 5 * sl, o);
         const double mi[9] = {m00, m01, m02, m01, m11, m12, m02, m12, m22};
-        const unsigned act = (unsigned)Aw[o];
-        const double s[12] = {J0[o],      J0[o + dj], J0[o + dk], J0[o + dj + dk], j1a, J1[o + di],
-                              j1b,        J1[o + di + dk], j2a, J2[o + di], j2b, J2[o + di + dj]};
+        const unsigned act = (unsigned)ld32(Aw, o);
+        const double s[12] = {cur.jx, cur.jxj, lane_next(cur.jx), lane_next(cur.jxj),
+                              cur.jy, nx.jy,   lane_next(cur.jy), lane_next(nx.jy),
+                              cur.jz, nx.jz,   cur.jzj,           nx.jzj};
         double jc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int n = 0; n < 12; ++n)
@@ -1969,18 +2002,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const bool wall = wall_jk || (b.bc[0] == BC_SOLID && i < 2) || (b.bc[1] == BC_SOLID && i >= b.ni - 2);
         if (wall)
             jc[0] = jc[1] = jc[2] = 0.0;
+        if (own)
+        {
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            Jc[a * sl + o] = jc[a];
+            for (int a = 0; a < 3; ++a)
+                st32(Jc + a * sl, o, jc[a]);
+        }
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                nd[m][c] = nd[4 + m][c];
-        j1a = s[5];
-        j1b = s[7];
-        j2a = s[9];
-        j2b = s[11];
+        for (int c = 0; c < 3; ++c)
+        {
+            nd[0][c] = nd[4][c];
+            nd[1][c] = nd[5][c];
+        }
+        cur = nx;
     }
 }
 
@@ -3155,13 +3189,6 @@ int pad_tiles_per_plane(int ext_j, int ext_k, int rows)
 {
     const int P = ext_j * (ext_k + 2), nw = (P + PAD_OWN - 1) / PAD_OWN;
     return (nw + rows - 1) / rows;
-}
-void launch_jedge(const Launch &L, int mixed, hipStream_t q)
-{
-    if (mixed)
-        hipLaunchKernelGGL(k_jedge<float>, grid_of(L), block_of(L), 0, q, L.tm);
-    else
-        hipLaunchKernelGGL(k_jedge<double>, grid_of(L), block_of(L), 0, q, L.tm);
 }
 void launch_jcell(const Launch &L, int mixed, hipStream_t q)
 {
