@@ -1535,20 +1535,62 @@ __global__ __launch_bounds__(256) void k_fluid(const MhdTileMap tm, const MhdCas
 }
 
 // The first-order fluid update of k_fluid in column tiles.  Per plane i the
-// workgroup evaluates flux_cell once for its 64 x COL_TJ cells and their one-
-// cell halo in j and k (LDS); a thread keeps its own column's cell i-1 in
-// registers.  Each i-face flux is computed once: the face between planes i-1
-// and i closes cell i-1 (whose j and k terms, 1/J and source were kept) and
-// opens cell i.  Every face flux is rusanov_pre of the same two cells as in
-// k_fluid and the update adds the same terms in the same order
-// (((0 - t_i) - t_j) - t_k, then the J x B source), so the result is bit for
-// bit k_fluid's, with about a fifth of its divisions.
+// workgroup holds flux_cell of its 64 x COL_TJ cells and their one-cell halo
+// in j and k (LDS); a thread forms flux_cell of its own column's cell i + 1
+// as it closes cell i, which is that cell's entry for the next plane, so only
+// the halo is formed by the fill.  Each i-face flux is computed once (the
+// upper face of cell i is carried as the lower face of cell i + 1), and the
+// update adds the terms in k_fluid's order, ((0 - t_i) - t_j) - t_k, then the
+// J x B source, as each term is formed: every face flux is rusanov_pre of the
+// same two cells as in k_fluid, so the result is bit for bit k_fluid's, with
+// about a fifth of its divisions and only the cell and its lower i-face
+// carried from plane to plane.
 template <typename R>
-__global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
-                                                           const double *__restrict__ dts, int ucur)
+__device__ __forceinline__ void col_put(double (&P)[7][COL_TJ + 2][COL_TK + 2], int pj, int pk, const FluxCell &f)
+{
+    P[0][pj][pk] = f.rho;
+    P[1][pj][pk] = f.u;
+    P[2][pj][pk] = f.v;
+    P[3][pj][pk] = f.w;
+    P[4][pj][pk] = f.p;
+    P[5][pj][pk] = f.e;
+    P[6][pj][pk] = f.c;
+}
+__device__ __forceinline__ FluxCell col_get(const double (&P)[7][COL_TJ + 2][COL_TK + 2], int pj, int pk)
+{
+    FluxCell f;
+    f.rho = P[0][pj][pk];
+    f.u = P[1][pj][pk];
+    f.v = P[2][pj][pk];
+    f.w = P[3][pj][pk];
+    f.p = P[4][pj][pk];
+    f.e = P[5][pj][pk];
+    f.c = P[6][pj][pk];
+    return f;
+}
+// halo entry h of a column tile's LDS plane: the rows j0 - 1 and j0 + COL_TJ
+// (every k of the tile and its halo), then the k halo of the tile's rows
+__device__ __forceinline__ void col_halo(int h, int &pj, int &pk)
 {
     constexpr int PJ = COL_TJ + 2, PK = COL_TK + 2;
-    __shared__ double P[7][PJ][PK];
+    if (h < 2 * PK)
+    {
+        pj = h < PK ? 0 : PJ - 1;
+        pk = h < PK ? h : h - PK;
+    }
+    else
+    {
+        pj = 1 + (h - 2 * PK) / 2;
+        pk = (h & 1) ? PK - 1 : 0;
+    }
+}
+constexpr int COL_NHALO = 2 * (COL_TK + 2) + 2 * COL_TJ;
+
+template <typename R>
+__global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
+                                                           const double *__restrict__ dts, int ucur)
+{
+    __shared__ double P[7][COL_TJ + 2][COL_TK + 2];
     MhdBlockDev b;
     int i0, j0, k0;
     if (!tile_col(tm, b, i0, j0, k0)) // uniform over the workgroup
@@ -1556,6 +1598,8 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * 64 + tx;
     const int j = j0 + ty, k = k0 + tx;
     const bool act = j < b.nj && k < b.nk;
+    // the thread's LDS entry is a cell (interior, or the ghost layer at N)
+    const bool mine = j <= b.nj && k <= b.nk;
     const int i1 = min(i0 + tm.ci, b.ni);
     const MhdCaseConst &c = cst[b.case_id];
     const double gm = c.gamma;
@@ -1569,104 +1613,74 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
     const double dt = dts[b.case_id];
     // the J x B weight of k_fluid (Hall_Num_Limiter)
     const double fcoef = c.momentum_hall_coeff;
-    int64_t o = blk_off(b, i0, act ? j : j0, act ? k : k0); // cell of plane `pl` in the loop
-    FluxCell prev{};
+    int64_t o = blk_off(b, i0, mine ? j : j0, mine ? k : k0); // cell of plane `pl` in the loop
+    // plane i0: the thread's own cell, its lower i-face flux, the halo
+    FluxCell cur{};
+    double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (mine)
+        cur = flux_cell<R>(U, sl, o, gm);
     if (act)
-        prev = flux_cell<R>(U, sl, o - di, gm);
-    // the open cell (plane pl - 1): lower i-face flux, j and k terms, 1/J, source
-    double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tj[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, tk[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double src[4] = {0.0, 0.0, 0.0, 0.0}, invJ = 0.0;
+        rusanov_pre<R>(flux_cell<R>(U, sl, o - di, gm), cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], Flo);
+    if (mine)
+        col_put<R>(P, ty + 1, tx + 1, cur);
+    if (tid < COL_NHALO)
+    {
+        int pj, pk;
+        col_halo(tid, pj, pk);
+        const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+        if (jj <= b.nj && kk <= b.nk)
+            col_put<R>(P, pj, pk, flux_cell<R>(U, sl, blk_off(b, i0, jj, kk), gm));
+    }
+    __syncthreads();
     // the k-face walls and far field this kernel applies itself (folds_kface_u)
     const bool fu_lo = tm.fold && folds_kface_u(b, 4), fu_hi = tm.fold && folds_kface_u(b, 5);
     const bool solid_lo = fu_lo && b.bc[4] == BC_SOLID, solid_hi = fu_hi && b.bc[5] == BC_SOLID;
-    for (int pl = i0; pl <= i1; ++pl, o += di)
+    for (int pl = i0; pl < i1; ++pl, o += di)
     {
-        FluxCell cur{};
+        // the next plane's entry of this thread: cell pl + 1 (the ghost layer at N at the end)
+        FluxCell nxt{};
+        if (mine)
+            nxt = flux_cell<R>(U, sl, o + di, gm);
         double unc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}; // the closed cell's new state
-        if (pl < i1)
-        {
-            for (int p = tid; p < PJ * PK; p += 64 * COL_TJ)
-            {
-                const int pj = p / PK, pk = p - pj * PK;
-                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
-                if (jj <= b.nj && kk <= b.nk)
-                {
-                    const FluxCell f = flux_cell<R>(U, sl, blk_off(b, pl, jj, kk), gm);
-                    P[0][pj][pk] = f.rho;
-                    P[1][pj][pk] = f.u;
-                    P[2][pj][pk] = f.v;
-                    P[3][pj][pk] = f.w;
-                    P[4][pj][pk] = f.p;
-                    P[5][pj][pk] = f.e;
-                    P[6][pj][pk] = f.c;
-                }
-            }
-            __syncthreads();
-        }
         if (act)
         {
-            auto lds = [&](int pj, int pk) {
-                FluxCell f;
-                f.rho = P[0][pj][pk];
-                f.u = P[1][pj][pk];
-                f.v = P[2][pj][pk];
-                f.w = P[3][pj][pk];
-                f.p = P[4][pj][pk];
-                f.e = P[5][pj][pk];
-                f.c = P[6][pj][pk];
-                return f;
-            };
-            cur = pl < i1 ? lds(ty + 1, tx + 1) : flux_cell<R>(U, sl, o, gm);
-            double F[5];
-            rusanov_pre<R>(prev, cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], F);
-            if (pl > i0) // close cell pl - 1
+            double r[5], F[5], Fl[5], Fh[5];
+            const double invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
+            rusanov_pre<R>(cur, nxt, Ax[o + di], Ax[sl + o + di], Ax[2 * sl + o + di], F);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
             {
-                const int64_t oc = o - di;
-#pragma unroll
-                for (int m = 0; m < 5; ++m)
-                {
-                    double r = 0.0;
-                    r -= (F[m] - Flo[m]) * invJ;
-                    r -= tj[m];
-                    r -= tk[m];
-                    if (m >= 1)
-                        r += src[m - 1];
-                    unc[m] = U[m * sl + oc] + dt * r;
-                }
+                r[m] = 0.0;
+                r[m] -= (F[m] - Flo[m]) * invJ;
+                Flo[m] = F[m];
             }
-            if (pl < i1) // open cell pl
-            {
+            rusanov_pre<R>(col_get(P, ty, tx + 1), cur, Ay[o], Ay[sl + o], Ay[2 * sl + o], Fl);
+            rusanov_pre<R>(cur, col_get(P, ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj], Fh);
 #pragma unroll
-                for (int m = 0; m < 5; ++m)
-                    Flo[m] = F[m];
-                invJ = 1.0 / G[(int64_t)G_JAC * sl + o];
-                double Fl[5], Fh[5];
-                rusanov_pre<R>(lds(ty, tx + 1), cur, Ay[o], Ay[sl + o], Ay[2 * sl + o], Fl);
-                rusanov_pre<R>(cur, lds(ty + 2, tx + 1), Ay[o + dj], Ay[sl + o + dj], Ay[2 * sl + o + dj], Fh);
+            for (int m = 0; m < 5; ++m)
+                r[m] -= (Fh[m] - Fl[m]) * invJ;
+            rusanov_pre<R>(col_get(P, ty + 1, tx), cur, Az[o], Az[sl + o], Az[2 * sl + o], Fl);
+            rusanov_pre<R>(cur, col_get(P, ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fh);
 #pragma unroll
-                for (int m = 0; m < 5; ++m)
-                    tj[m] = (Fh[m] - Fl[m]) * invJ;
-                rusanov_pre<R>(lds(ty + 1, tx), cur, Az[o], Az[sl + o], Az[2 * sl + o], Fl);
-                rusanov_pre<R>(cur, lds(ty + 1, tx + 2), Az[o + 1], Az[sl + o + 1], Az[2 * sl + o + 1], Fh);
+            for (int m = 0; m < 5; ++m)
+                r[m] -= (Fh[m] - Fl[m]) * invJ;
+            const double n_true = dmax(cur.rho, 0.0) / c.M_H * c.M_ref;
+            const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
+            const double fc = fcoef * (n_true / ne_eff);
+            const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
+            const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
+            const double jx = J1 * B2 - J2 * B1;
+            const double jy = J2 * B0 - J0 * B2;
+            const double jz = J0 * B1 - J1 * B0;
+            r[1] += fc * jx;
+            r[2] += fc * jy;
+            r[3] += fc * jz;
+            r[4] += fc * (jx * cur.u + jy * cur.v + jz * cur.w);
 #pragma unroll
-                for (int m = 0; m < 5; ++m)
-                    tk[m] = (Fh[m] - Fl[m]) * invJ;
-                const double n_true = dmax(cur.rho, 0.0) / c.M_H * c.M_ref;
-                const double ne_eff = sqrt(n_true * n_true + c.ne_floor * c.ne_floor);
-                const double fc = fcoef * (n_true / ne_eff);
-                const double B0 = Bc[o], B1 = Bc[sl + o], B2 = Bc[2 * sl + o];
-                const double J0 = Jc[o], J1 = Jc[sl + o], J2 = Jc[2 * sl + o];
-                const double jx = J1 * B2 - J2 * B1;
-                const double jy = J2 * B0 - J0 * B2;
-                const double jz = J0 * B1 - J1 * B0;
-                src[0] = fc * jx;
-                src[1] = fc * jy;
-                src[2] = fc * jz;
-                src[3] = fc * (jx * cur.u + jy * cur.v + jz * cur.w);
-            }
-            prev = cur;
+            for (int m = 0; m < 5; ++m)
+                unc[m] = U[m * sl + o] + dt * r[m];
         }
-        if (pl > i0) // store the closed cell (the lanes of a wavefront together)
+        // store the closed cell (the lanes of a wavefront together)
         {
             // a solid k wall's first layer takes the wall state of its
             // neighbour's new value (OP_SOLID_FIRST), its ghosts that of the
@@ -1690,7 +1704,6 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
             }
             if (act)
             {
-                const int64_t oc = o - di;
                 const bool wall_lo = solid_lo && k == 0, wall_hi = solid_hi && k == b.nk - 1;
                 if (wall_lo || wall_hi)
                 {
@@ -1701,16 +1714,16 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
 #pragma unroll
                     for (int m = 0; m < 5; ++m)
                     {
-                        Un[m * sl + oc] = w[m];
+                        Un[m * sl + o] = w[m];
                         for (int l = 1; l <= b.g; ++l)
-                            Un[m * sl + oc + out * l] = wg[m];
+                            Un[m * sl + o + out * l] = wg[m];
                     }
                 }
                 else
                 {
 #pragma unroll
                     for (int m = 0; m < 5; ++m)
-                        Un[m * sl + oc] = unc[m];
+                        Un[m * sl + o] = unc[m];
                 }
                 const bool far_lo = fu_lo && !solid_lo && k == 0, far_hi = fu_hi && !solid_hi && k == b.nk - 1;
                 if (far_lo || far_hi)
@@ -1719,12 +1732,26 @@ __global__ __launch_bounds__(64 * COL_TJ) void k_fluid_col(const MhdTileMap tm, 
 #pragma unroll
                     for (int m = 0; m < 5; ++m)
                         for (int l = 1; l <= b.g; ++l)
-                            Un[m * sl + oc + out * l] = c.qinf[m];
+                            Un[m * sl + o + out * l] = c.qinf[m];
                 }
             }
         }
-        if (pl < i1)
+        if (pl + 1 < i1) // the LDS plane pl + 1: own entries, then the halo
+        {
             __syncthreads();
+            if (mine)
+                col_put<R>(P, ty + 1, tx + 1, nxt);
+            if (tid < COL_NHALO)
+            {
+                int pj, pk;
+                col_halo(tid, pj, pk);
+                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+                if (jj <= b.nj && kk <= b.nk)
+                    col_put<R>(P, pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
+            }
+            __syncthreads();
+        }
+        cur = nxt;
     }
 }
 
