@@ -1590,7 +1590,7 @@ template <typename R>
 __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                            const double *__restrict__ dts, int ucur)
 {
-    __shared__ double P[7][COL_TJ + 2][COL_TK + 2];
+    __shared__ double P2[2][7][COL_TJ + 2][COL_TK + 2]; // planes pl and pl + 1, alternating
     MhdBlockDev b;
     int i0, j0, k0;
     if (!tile_col(tm, b, i0, j0, k0)) // uniform over the workgroup
@@ -1622,14 +1622,14 @@ __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))
     if (act)
         rusanov_pre<R>(flux_cell<R>(U, sl, o - di, gm), cur, Ax[o], Ax[sl + o], Ax[2 * sl + o], Flo);
     if (mine)
-        col_put<R>(P, ty + 1, tx + 1, cur);
+        col_put<R>(P2[0], ty + 1, tx + 1, cur);
     if (tid < COL_NHALO)
     {
         int pj, pk;
         col_halo(tid, pj, pk);
         const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
         if (jj <= b.nj && kk <= b.nk)
-            col_put<R>(P, pj, pk, flux_cell<R>(U, sl, blk_off(b, i0, jj, kk), gm));
+            col_put<R>(P2[0], pj, pk, flux_cell<R>(U, sl, blk_off(b, i0, jj, kk), gm));
     }
     __syncthreads();
     // the k-face walls and far field this kernel applies itself (folds_kface_u)
@@ -1641,6 +1641,21 @@ __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))
         FluxCell nxt{};
         if (mine)
             nxt = flux_cell<R>(U, sl, o + di, gm);
+        const int cb = (pl - i0) & 1;
+        auto &P = P2[cb];
+        if (pl + 1 < i1) // the LDS plane pl + 1 (its buffer's last reads were before the previous barrier)
+        {
+            if (mine)
+                col_put<R>(P2[cb ^ 1], ty + 1, tx + 1, nxt);
+            if (tid < COL_NHALO)
+            {
+                int pj, pk;
+                col_halo(tid, pj, pk);
+                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+                if (jj <= b.nj && kk <= b.nk)
+                    col_put<R>(P2[cb ^ 1], pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
+            }
+        }
         double unc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}; // the closed cell's new state
         if (act)
         {
@@ -1736,21 +1751,8 @@ __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))
                 }
             }
         }
-        if (pl + 1 < i1) // the LDS plane pl + 1: own entries, then the halo
-        {
+        if (pl + 1 < i1)
             __syncthreads();
-            if (mine)
-                col_put<R>(P, ty + 1, tx + 1, nxt);
-            if (tid < COL_NHALO)
-            {
-                int pj, pk;
-                col_halo(tid, pj, pk);
-                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
-                if (jj <= b.nj && kk <= b.nk)
-                    col_put<R>(P, pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
-            }
-            __syncthreads();
-        }
         cur = nxt;
     }
 }
