@@ -1590,7 +1590,11 @@ template <typename R>
 __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))) void k_fluid_col(const MhdTileMap tm, const MhdCaseConst *__restrict__ cst,
                                                            const double *__restrict__ dts, int ucur)
 {
-    __shared__ double P2[2][7][COL_TJ + 2][COL_TK + 2]; // planes pl and pl + 1, alternating
+    // FP64: two LDS planes used alternately (the next plane is written while
+    // the current one is read, one barrier per plane); the mixed step's
+    // lighter threads keep one plane and 4 workgroups per CU
+    constexpr int NB = sizeof(R) == 8 ? 2 : 1;
+    __shared__ double P2[NB][7][COL_TJ + 2][COL_TK + 2];
     MhdBlockDev b;
     int i0, j0, k0;
     if (!tile_col(tm, b, i0, j0, k0)) // uniform over the workgroup
@@ -1641,19 +1645,19 @@ __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))
         FluxCell nxt{};
         if (mine)
             nxt = flux_cell<R>(U, sl, o + di, gm);
-        const int cb = (pl - i0) & 1;
+        const int cb = (pl - i0) & (NB - 1);
         auto &P = P2[cb];
-        if (pl + 1 < i1) // the LDS plane pl + 1 (its buffer's last reads were before the previous barrier)
+        if (NB == 2 && pl + 1 < i1) // the LDS plane pl + 1 (its buffer's last reads were before the previous barrier)
         {
             if (mine)
-                col_put<R>(P2[cb ^ 1], ty + 1, tx + 1, nxt);
+                col_put<R>(P2[cb ^ (NB - 1)], ty + 1, tx + 1, nxt);
             if (tid < COL_NHALO)
             {
                 int pj, pk;
                 col_halo(tid, pj, pk);
                 const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
                 if (jj <= b.nj && kk <= b.nk)
-                    col_put<R>(P2[cb ^ 1], pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
+                    col_put<R>(P2[cb ^ (NB - 1)], pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
             }
         }
         double unc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}; // the closed cell's new state
@@ -1749,6 +1753,20 @@ __global__ __launch_bounds__(64 * COL_TJ) __attribute__((amdgpu_waves_per_eu(3))
                         for (int l = 1; l <= b.g; ++l)
                             Un[m * sl + o + out * l] = c.qinf[m];
                 }
+            }
+        }
+        if (NB == 1 && pl + 1 < i1) // one plane: every read of plane pl, then plane pl + 1
+        {
+            __syncthreads();
+            if (mine)
+                col_put<R>(P, ty + 1, tx + 1, nxt);
+            if (tid < COL_NHALO)
+            {
+                int pj, pk;
+                col_halo(tid, pj, pk);
+                const int jj = j0 - 1 + pj, kk = k0 - 1 + pk;
+                if (jj <= b.nj && kk <= b.nk)
+                    col_put<R>(P, pj, pk, flux_cell<R>(U, sl, blk_off(b, pl + 1, jj, kk), gm));
             }
         }
         if (pl + 1 < i1)
