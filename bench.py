@@ -249,6 +249,7 @@ def bench_ot(args):
                    res[0][1], "f64", {"workload": "orszag_tang_fp64_64^3", "cells": cells, "recon": "first-order"},
                    minmod={"value": round(res[1][0], 3), "ms_per_step": round(res[1][1], 4)},
                    divb_max=float(max(res[0][2], res[1][2])),
+                   fp64=fp64_record("orszag_tang_fp64_64cubed", res[0][0]),
                    step_hbm={"bytes_per_cell_update": STEP_BYTES,
                              "frac": round(res[0][0] * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4),
                              "note": "the box keeps stored metric and weight arrays like the shell (no closed-form "
@@ -358,7 +359,7 @@ def extra_configs(args):
     r = bench_ot(a)
     out["c1_orszag_tang_64"] = {"first_order": {"value": r["value"], "ms_per_step": r["ms_per_step"]},
                                 "minmod": r["minmod"], "divb_max": r["divb_max"], "steps": 100,
-                                "dtype": "f64", "step_hbm": r["step_hbm"]}
+                                "dtype": "f64", "step_hbm": r["step_hbm"], "fp64": r.get("fp64")}
     a = copy.copy(args)
     a.nang, a.nrad, a.steps = 128, 256, 500
     r = bench_precision(a)
@@ -396,6 +397,32 @@ def measured_traffic(workload, kernel):
             except ValueError:
                 return None
     return None
+
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def measured_flops(workload):
+    """FP64 flops per cell-update of a step from the committed VALU counter
+    summary of this workload (profiles/valu_<workload>.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("FP64 flops per cell-update:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def fp64_record(workload, mcells):
+    """The step's FP64 rate against MI355X's 78.6 TFLOP/s vector FP64 peak,
+    from the measured flops per cell-update."""
+    f = measured_flops(workload)
+    if f is None:
+        return None
+    tf = mcells * 1e6 * f / 1e12
+    return {"flops_per_cell_update": f, "tflops": round(tf, 2), "peak_tflops": FP64_PEAK_TFLOPS,
+            "frac": round(tf / FP64_PEAK_TFLOPS, 4), "source": f"profiles/valu_{workload}.md"}
 
 
 def main():
@@ -523,6 +550,8 @@ def main():
         "cpu_baseline": cpu,
         "step_hbm": {"bytes_per_cell_update": step_bytes,
                      "frac": round(value * 1e6 * step_bytes / (HBM_PEAK_GBS * 1e9), 4)},
+        "fp64": fp64_record(f"{args.workload}_{args.precision}_6x{args.nang}x{args.nang}x{args.nrad}", value)
+        if args.precision == "fp64" else None,
     }
     if mercury:
         out["metric"] = "Mcell-updates/s (two-fluid Mercury shell)"
