@@ -109,13 +109,16 @@ struct TileSet
     int nent = 0;
     TileArr tiles[R_COUNT][3];  // [range][rows]
     TileArr chunks[2][R_COUNT][3]; // [kChunkCi index][range][rows]
-    TileArr cols;              // column tiles of the interior cells (kColCi planes each)
+    TileArr cols[3];           // column tiles of the interior cells (kColCi[q] planes each)
     TileArr shell;             // 256 shell points of the face range per tile
     TileArr pad[R_COUNT][3];   // padded-row plane chunks (kChunkCi[0] planes) [range][rows]
     TileArr kl[R_COUNT][3];    // geometry-shared sets: member-looping tiles [range][rows]
 };
 // planes along i per column-tile workgroup
-constexpr int kColCi = 8;
+// planes per column tile of the first-order fluid kernels, by the launch
+// variant's row index: a small grid (the 64^3 box: 128 tiles of 8 planes
+// for 256 CUs) needs the shorter columns, large ones the longer
+constexpr int kColCi[3] = {8, 4, 2};
 // planes along i per plane-chunk tile (kernels that march along i and carry
 // the values plane i - 1 shares with plane i in registers): two sizes
 constexpr int kChunkCi[2] = {4, 8};
@@ -322,7 +325,8 @@ extern "C" int mhd_destroy(mhd_ctx *c)
                 for (auto &rr : cc)
                     for (auto &t : rr)
                         free_t(t.d_start);
-            free_t(T.cols.d_start);
+            for (TileArr &A : T.cols)
+                free_t(A.d_start);
             free_t(T.shell.d_start);
             for (auto &rr : T.pad)
                 for (auto &t : rr)
@@ -581,8 +585,9 @@ static int build_tiles(mhd_ctx *c)
                     return rc;
                 HIPCHK(c, hipMemcpy(A.d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
             }
+            for (int q = 0; q < 3; ++q)
             {
-                TileArr &A = T.cols;
+                TileArr &A = T.cols[q];
                 const int tj = mhdk::col_tile_rows();
                 std::vector<int> start(std::max(T.nent, 1), 0);
                 long long total = 0;
@@ -590,7 +595,7 @@ static int build_tiles(mhd_ctx *c)
                 {
                     const MhdBlockDev &d = c->h_blk[ident ? e : T.members[T.lead_moff[e]]];
                     start[e] = (int)total;
-                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni + kColCi - 1) / kColCi) *
+                    total += (long long)(ident ? 1 : T.lead_k[e]) * ((d.ni + kColCi[q] - 1) / kColCi[q]) *
                              ((d.nj + tj - 1) / tj) * ((d.nk + 63) / 64);
                 }
                 if (total > (1LL << 31) - 64)
@@ -676,17 +681,19 @@ static mhdk::Launch launch_of_chunk(const mhd_ctx *c, int kernel, int ch)
 }
 
 // Column-tile launch of a kernel's block set (per-case or geometry-shared
-// entries as the kernel's tuned variant says).
+// entries as the kernel's tuned variant says; the variant's row index picks
+// the column length kColCi, the column tile's rows are fixed).
 static mhdk::Launch launch_of_col(const mhd_ctx *c, int kernel, int jt_index = -1)
 {
     const int v = jt_index >= 0 ? jt_index : c->jt_idx[kernel];
     mhdk::Launch L = launch_of(c, kernel, v);
     const TileSet &S = c->ts[set_of(c, kernel)][v >= 6 ? 1 : 0];
-    L.tm.block_start = S.cols.d_start;
-    L.tm.ntiles = S.cols.n;
-    L.tm.xcd_chunk = c->opt.xcd_order ? (S.cols.n + 7) / 8 : 0;
+    const TileArr &T = S.cols[v % 3];
+    L.tm.block_start = T.d_start;
+    L.tm.ntiles = T.n;
+    L.tm.xcd_chunk = c->opt.xcd_order ? (T.n + 7) / 8 : 0;
     L.tm.i_inner = 0;
-    L.tm.ci = kColCi;
+    L.tm.ci = kColCi[v % 3];
     L.tm.kloop = 0;
     L.jt = mhdk::col_tile_rows();
     return L;

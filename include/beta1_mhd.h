@@ -329,7 +329,10 @@ int mhd_memory(mhd_ctx *ctx, double *bytes);
    geo_shared (1: the cases sharing one geometry set run each geometry tile
    back to back, so it is fetched from HBM once for all of them while it is
    in L2; 2: one workgroup loads the geometry tile into registers and updates
-   every case sharing it in turn -- k_derived only, i_inner 0) */
+   every case sharing it in turn -- k_derived only, i_inner 0).  The
+   first-order fluid kernels run column tiles of fixed height; for them
+   rows 1, 2 and 4 select columns of 8, 4 and 2 planes and i_inner has no
+   effect. */
 int mhd_tuning(mhd_ctx *ctx, int kernel, int *rows, int *i_inner, int *geo_shared);
 /* pin a kernel's launch shape (rows 1, 2 or 4; i_inner 0/1; geo_shared 0, 1
    or 2 as above) */
