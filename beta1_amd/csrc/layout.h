@@ -69,19 +69,18 @@ enum MhdStateSlot
     ST_EFACE = 22, // 9 slots: Eface_xi(3 comps), Eface_eta(3), Eface_zeta(3)
     ST_E = 31,     // 3 slots: E_xi (EdgeXi), E_eta, E_zeta
     ST_UC = 34,    // 3 slots: cell contravariant velocity per axis (step scratch)
-    ST_JE = 37,    // 3 slots: J_xi, J_eta, J_zeta edges (EdgeXi/Et/Ze) of the J_cell pass (step scratch)
-    ST_NSLOT = 40,
+    ST_NSLOT = 37, // (J edges are formed inside the J_cell pass, never stored)
     // two-fluid (Mercury, H+ and Na+) cases add:
-    ST_UNA0 = 40,  // 5 comps: Na+ conservative state (U_Na)
-    ST_UNA1 = 45,  // 5 comps (second buffer)
-    ST_PHOTO = 50, // Na+ photoionization production rate [cm^-3 s^-1] (static)
-    ST_DRAG = 51,  // ion-neutral drag frequency [1/s] (static)
-    ST_PE = 52,    // electron pressure p_H + p_Na of cells [-1, N] (step scratch)
+    ST_UNA0 = 37,  // 5 comps: Na+ conservative state (U_Na)
+    ST_UNA1 = 42,  // 5 comps (second buffer)
+    ST_PHOTO = 47, // Na+ photoionization production rate [cm^-3 s^-1] (static)
+    ST_DRAG = 48,  // ion-neutral drag frequency [1/s] (static)
+    ST_PE = 49,    // electron pressure p_H + p_Na of cells [-1, N] (step scratch)
     // (edge vectors dr are differences of the geometry's node coordinates)
-    ST_TAPER = 53, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
-    ST_ETALOC = 56, // 3 slots: local artificial resistivity of the edges (static)
-    ST_ETARES = 59, // 3 slots: interior resistivity inver_Rem yita0(r) of the edges (static)
-    ST_NSLOT2 = 62
+    ST_TAPER = 50, // 3 slots: radial Hall taper at the xi / eta / zeta edge midpoints (static)
+    ST_ETALOC = 53, // 3 slots: local artificial resistivity of the edges (static)
+    ST_ETARES = 56, // 3 slots: interior resistivity inver_Rem yita0(r) of the edges (static)
+    ST_NSLOT2 = 59
 };
 
 // State slots per block for a physics model (0 single fluid, 1 two-fluid).

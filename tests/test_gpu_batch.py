@@ -64,8 +64,8 @@ def test_admission_respects_hbm_budget():
     probe.finalize()
     one = probe.memory_bytes()
     probe.close()
-    # one case is state (40 slots) + geometry (38 slots) of its shape: a
-    # second case of the same shape adds its state only (~0.51 of `one`), a
+    # one case is state (37 slots) + geometry (38 slots) of its shape: a
+    # second case of the same shape adds its state only (~0.49 of `one`), a
     # case of the 1.5x larger shape adds ~1.4 of `one`
     ctx = MhdContext(hbm_budget_gb=1.6 * one / 1e9)
     ctx.finalize()
