@@ -99,6 +99,7 @@ SIGNATURES = {
     "mhd_host_download": (_I, [_P, _I, _I, _PD, _S]),
     "mhd_host_upload": (_I, [_P, _I, _I, _PD, _S]),
     "mhd_host_faces": (_I, [_P, _I, _PI]),
+    "mhd_host_face_parts": (_I, [_P, _I, _I, _PI, _I, _PI]),
     "mhd_host_nodes": (_I, [_P, _I, _I, _PD, _S]),
     "mhd_host_constants": (_I, [_P, _PD]),
     "mhd_host_constants_two_fluid": (_I, [_P, _PD]),

@@ -2228,7 +2228,7 @@ extern "C" int mhd_get_faces(mhd_ctx *c, int case_id, int block, int *out)
     {
         const FaceLink &L = B.face[f];
         int *o = out + 14 * f;
-        o[0] = L.kind;
+        o[0] = L.parts.empty() ? L.kind : 2; // 2: several interface patches, the first described
         o[1] = L.bc;
         o[2] = L.nb_block;
         o[3] = L.nb_face;

@@ -366,4 +366,11 @@ extern "C" int mhd_host_case_end(mhd_host_case *h)
     }
 }
 
-extern "C" const char *mhd_host_last_error(mhd_host_case *h) { return h ? h->err.c_str() : "null host case"; }
+// the reason the calling thread's last host-case constructor failed
+static thread_local std::string g_host_ctor_err;
+void mpcns::host_ctor_error(const std::string &msg) { g_host_ctor_err = msg; }
+
+extern "C" const char *mhd_host_last_error(mhd_host_case *h)
+{
+    return h ? h->err.c_str() : (g_host_ctor_err.empty() ? "null host case" : g_host_ctor_err.c_str());
+}

@@ -167,7 +167,10 @@ void write_case_dir(const Grid &grd, const std::string &dir, const std::string &
         for (int b = 0; b < nb; ++b)
             for (int f = 0; f < 6; ++f)
                 if (grd.blocks[b].face[f].kind == 1)
-                    ++cnt[b], ++total;
+                {
+                    const int np = std::max(1, (int)grd.blocks[b].face[f].parts.size());
+                    cnt[b] += np, total += np;
+                }
         o << total << "\n";
         for (int b = 0; b < nb; ++b)
             o << cnt[b] << (b + 1 < nb ? " " : "\n");
@@ -177,28 +180,40 @@ void write_case_dir(const Grid &grd, const std::string &dir, const std::string &
             const Block &B = grd.blocks[b];
             for (int f = 0; f < 6; ++f)
             {
-                const FaceLink &L = B.face[f];
-                if (L.kind != 1)
+                const FaceLink &F = B.face[f];
+                if (F.kind != 1)
                     continue;
-                const IndexTransform &T = L.trans;
-                const int ax = f / 2;
-                int sub[3] = {0, 0, 0}, sup[3] = {B.ni, B.nj, B.nk};
-                sub[ax] = sup[ax] = (f & 1) ? B.n(ax) : 0;
-                int tsub[3], tsup[3];
-                for (int a = 0; a < 3; ++a)
+                const int ax = f / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+                std::vector<FacePart> parts = F.parts;
+                if (parts.empty())
                 {
-                    tsub[T.perm[a]] = T.sign[a] * sub[a] + T.offset[a];
-                    tsup[T.perm[a]] = T.sign[a] * sup[a] + T.offset[a];
+                    FacePart W;
+                    W.hi[0] = B.n(t1), W.hi[1] = B.n(t2);
+                    W.nb_block = F.nb_block, W.nb_face = F.nb_face, W.periodic = F.periodic, W.trans = F.trans;
+                    parts.push_back(W);
                 }
-                const int mark = ax == 0 ? 1 : 0; // first in-plane axis
-                auto code = [](int v, bool neg) { return neg ? -(v + 1) : v + 1; };
-                const int src = L.periodic ? -(b + 1) : b + 1, tar = L.periodic ? -(L.nb_block + 1) : L.nb_block + 1;
-                for (int a = 0; a < 3; ++a)
-                    o << code(sub[a], a == mark) << " " << code(sup[a], a == mark) << " ";
-                o << src << " " << block_name << "   ";
-                for (int a = 0; a < 3; ++a)
-                    o << code(tsub[a], a == T.perm[mark]) << " " << code(tsup[a], a == T.perm[mark]) << " ";
-                o << tar << " " << block_name << "\n";
+                for (const FacePart &L : parts)
+                {
+                    const IndexTransform &T = L.trans;
+                    int sub[3] = {0, 0, 0}, sup[3] = {B.ni, B.nj, B.nk};
+                    sub[ax] = sup[ax] = (f & 1) ? B.n(ax) : 0;
+                    sub[t1] = L.lo[0], sup[t1] = L.hi[0], sub[t2] = L.lo[1], sup[t2] = L.hi[1];
+                    int tsub[3], tsup[3];
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        tsub[T.perm[a]] = T.sign[a] * sub[a] + T.offset[a];
+                        tsup[T.perm[a]] = T.sign[a] * sup[a] + T.offset[a];
+                    }
+                    const int mark = ax == 0 ? 1 : 0; // first in-plane axis
+                    auto code = [](int v, bool neg) { return neg ? -(v + 1) : v + 1; };
+                    const int src = L.periodic ? -(b + 1) : b + 1, tar = L.periodic ? -(L.nb_block + 1) : L.nb_block + 1;
+                    for (int a = 0; a < 3; ++a)
+                        o << code(sub[a], a == mark) << " " << code(sup[a], a == mark) << " ";
+                    o << src << " " << block_name << "   ";
+                    for (int a = 0; a < 3; ++a)
+                        o << code(tsub[a], a == T.perm[mark]) << " " << code(tsup[a], a == T.perm[mark]) << " ";
+                    o << tar << " " << block_name << "\n";
+                }
             }
         }
     }
@@ -272,9 +287,18 @@ Grid read_case_dir(const std::string &dir, int g)
     for (auto &B : grd.blocks)
         for (auto &f : B.face)
             f = FaceLink{};
-    std::vector<std::array<bool, 6>> seen(nb);
-    for (auto &s : seen)
-        s.fill(false);
+    // the patches of each block face, physical (kind 0) or interface (1),
+    // with their node ranges along the face's tangential axes
+    struct Patch
+    {
+        int kind, bc;
+        FacePart part;
+    };
+    std::vector<std::array<std::vector<Patch>, 6>> patches(nb);
+    auto tangential = [](int f, const int lo[3], const int hi[3], FacePart &P) {
+        const int ax = f / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+        P.lo[0] = lo[t1], P.hi[0] = hi[t1], P.lo[1] = lo[t2], P.hi[1] = hi[t2];
+    };
     {
         std::ifstream in(bcd + "/boundary0000.txt");
         if (!in)
@@ -294,14 +318,9 @@ Grid read_case_dir(const std::string &dir, int g)
                 const int f = face_of_range(lo, hi, grd.blocks[b]);
                 if (f < 0)
                     throw std::runtime_error("boundary: range is not a block face");
-                const int ax = f / 2;
-                for (int a = 0; a < 3; ++a)
-                    if (a != ax && (lo[a] != 0 || hi[a] != grd.blocks[b].n(a)))
-                        throw std::runtime_error("boundary: only whole-face patches are supported");
-                FaceLink &L = grd.blocks[b].face[f];
-                L.kind = 0;
-                L.bc = bc_of(type);
-                seen[b][f] = true;
+                Patch P{0, bc_of(type), {}};
+                tangential(f, lo, hi, P.part);
+                patches[b][f].push_back(P);
             }
         }
     }
@@ -376,24 +395,69 @@ Grid read_case_dir(const std::string &dir, int g)
                 const int tf = face_of_range(blo, bhi, grd.blocks[nbk]);
                 if (f < 0 || tf < 0)
                     throw std::runtime_error("inner: range is not a block face");
+                int lo[3], hi[3];
                 for (int a = 0; a < 3; ++a)
-                    if (a != f / 2 && (std::min(alo[a], ahi[a]) != 0 || std::max(alo[a], ahi[a]) != B.n(a)))
-                        throw std::runtime_error("inner: only whole-face patches are supported");
-                FaceLink &L = grd.blocks[b].face[f];
-                L.kind = 1;
-                L.bc = BC_NONE;
-                L.nb_block = nbk;
-                L.nb_face = tf;
-                L.periodic = sb < 0;
-                L.trans = T;
-                seen[b][f] = true;
+                    lo[a] = std::min(alo[a], ahi[a]), hi[a] = std::max(alo[a], ahi[a]);
+                Patch P{1, BC_NONE, {}};
+                tangential(f, lo, hi, P.part);
+                P.part.nb_block = nbk;
+                P.part.nb_face = tf;
+                P.part.periodic = sb < 0;
+                P.part.trans = T;
+                patches[b][f].push_back(P);
             }
     }
+    // Each face: one condition, or several patches of one kind that tile it.
+    // Interface patches may split a face among neighbours (FaceLink::parts);
+    // physical patches of one type merge; a face mixing a physical condition
+    // with interfaces, or two conditions, is refused (the boundary phases and
+    // the kernels' folded boundary work hold one condition per face).
     for (int b = 0; b < nb; ++b)
         for (int f = 0; f < 6; ++f)
-            if (!seen[b][f])
-                throw std::runtime_error("block " + std::to_string(b) + " face " + std::to_string(f) +
-                                         " has neither a boundary condition nor an inner patch");
+        {
+            Block &B = grd.blocks[b];
+            const std::vector<Patch> &ps = patches[b][f];
+            const std::string where = "block " + std::to_string(b) + " face " + std::to_string(f);
+            if (ps.empty())
+                throw std::runtime_error(where + " has neither a boundary condition nor an inner patch");
+            const int ax = f / 2, n1 = B.n(ax == 0 ? 1 : 0), n2 = B.n(ax == 2 ? 1 : 2);
+            std::vector<int> cover((size_t)n1 * n2, 0);
+            for (const Patch &P : ps)
+            {
+                if (P.part.lo[0] < 0 || P.part.hi[0] > n1 || P.part.lo[1] < 0 || P.part.hi[1] > n2 ||
+                    P.part.lo[0] >= P.part.hi[0] || P.part.lo[1] >= P.part.hi[1])
+                    throw std::runtime_error(where + ": patch range outside the face");
+                for (int u = P.part.lo[0]; u < P.part.hi[0]; ++u)
+                    for (int v = P.part.lo[1]; v < P.part.hi[1]; ++v)
+                        cover[(size_t)u * n2 + v] += 1;
+            }
+            for (int c : cover)
+                if (c != 1)
+                    throw std::runtime_error(where + ": patches do not cover the face exactly once");
+            FaceLink &L = B.face[f];
+            if (ps[0].kind == 0)
+            {
+                for (const Patch &P : ps)
+                    if (P.kind != 0 || P.bc != ps[0].bc)
+                        throw std::runtime_error(where + " mixes boundary conditions or interfaces");
+                L.kind = 0;
+                L.bc = ps[0].bc;
+                continue;
+            }
+            for (const Patch &P : ps)
+                if (P.kind != 1 || P.part.periodic != ps[0].part.periodic)
+                    throw std::runtime_error(where + " mixes interfaces with boundary conditions or periodic links "
+                                                     "with plain ones");
+            L.kind = 1;
+            L.bc = BC_NONE;
+            L.nb_block = ps[0].part.nb_block;
+            L.nb_face = ps[0].part.nb_face;
+            L.periodic = ps[0].part.periodic;
+            L.trans = ps[0].part.trans;
+            if (ps.size() > 1)
+                for (const Patch &P : ps)
+                    L.parts.push_back(P.part);
+        }
     return grd;
 }
 

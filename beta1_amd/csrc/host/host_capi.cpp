@@ -185,8 +185,9 @@ extern "C" int mhd_host_case_ot(const mhd_ot_desc *d, int ghost, mhd_host_case *
         *out = h.release();
         return MHD_OK;
     }
-    catch (const std::exception &)
+    catch (const std::exception &e)
     {
+        host_ctor_error(e.what());
         return MHD_ERR_ARG;
     }
 }
@@ -202,8 +203,9 @@ extern "C" int mhd_host_case_lunar(const mhd_wind_desc *d, int ghost, mhd_host_c
         *out = h.release();
         return MHD_OK;
     }
-    catch (const std::exception &)
+    catch (const std::exception &e)
     {
+        host_ctor_error(e.what());
         return MHD_ERR_ARG;
     }
 }
@@ -221,8 +223,9 @@ extern "C" int mhd_host_case_mercury(const mhd_wind_desc *d, const mhd_dipole_de
         *out = h.release();
         return MHD_OK;
     }
-    catch (const std::exception &)
+    catch (const std::exception &e)
     {
+        host_ctor_error(e.what());
         return MHD_ERR_ARG;
     }
 }
@@ -314,7 +317,7 @@ extern "C" int mhd_host_faces(mhd_host_case *h, int block, int *out)
     {
         const FaceLink &L = B.face[f];
         int *o = out + 14 * f;
-        o[0] = L.kind;
+        o[0] = L.parts.empty() ? L.kind : 2; // 2: several interface patches, the first described
         o[1] = L.bc;
         o[2] = L.nb_block;
         o[3] = L.nb_face;
@@ -324,6 +327,30 @@ extern "C" int mhd_host_faces(mhd_host_case *h, int block, int *out)
             o[5 + a] = L.trans.perm[a];
             o[8 + a] = L.trans.sign[a];
             o[11 + a] = L.trans.offset[a];
+        }
+    }
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_face_parts(mhd_host_case *h, int block, int face, int *out, int cap, int *nparts)
+{
+    if (!h || !nparts || block < 0 || block >= (int)h->cs.grid.blocks.size() || face < 0 || face > 5)
+        return MHD_ERR_ARG;
+    const FaceLink &L = h->cs.grid.blocks[block].face[face];
+    *nparts = (int)L.parts.size();
+    if (*nparts > cap || (*nparts > 0 && !out))
+        return MHD_ERR_ARG;
+    for (int q = 0; q < *nparts; ++q)
+    {
+        const FacePart &P = L.parts[q];
+        int *o = out + 16 * q;
+        o[0] = P.lo[0], o[1] = P.lo[1], o[2] = P.hi[0], o[3] = P.hi[1];
+        o[4] = P.nb_block, o[5] = P.nb_face, o[6] = P.periodic;
+        for (int a = 0; a < 3; ++a)
+        {
+            o[7 + a] = P.trans.perm[a];
+            o[10 + a] = P.trans.sign[a];
+            o[13 + a] = P.trans.offset[a];
         }
     }
     return MHD_OK;
@@ -752,8 +779,9 @@ extern "C" int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, i
         *out = h.release();
         return MHD_OK;
     }
-    catch (const std::exception &)
+    catch (const std::exception &e)
     {
+        host_ctor_error(e.what());
         return MHD_ERR_ARG;
     }
 }

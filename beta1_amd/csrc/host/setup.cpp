@@ -137,6 +137,30 @@ static void face_tangents(int face, int &ax, int &t1, int &t2)
     t2 = ax == 2 ? 1 : 2;
 }
 
+// The link a point crosses through interface face f: the face's own or, on
+// a face several interface patches share, the part covering the point.  dc
+// are doubled node coordinates (2 idx + the location's delta per axis),
+// clamped onto the face; a point on the line between two parts takes the
+// first part that covers it (both map it onto the same physical point).
+struct LinkView
+{
+    int nb_block, nb_face, periodic;
+    IndexTransform trans;
+};
+static LinkView link_at(const Block &B, int f, const int dc[3])
+{
+    const FaceLink &L = B.face[f];
+    if (L.parts.empty())
+        return {L.nb_block, L.nb_face, L.periodic, L.trans};
+    int ax, t1, t2;
+    face_tangents(f, ax, t1, t2);
+    const int c1 = std::max(0, std::min(dc[t1], 2 * B.n(t1))), c2 = std::max(0, std::min(dc[t2], 2 * B.n(t2)));
+    for (const FacePart &P : L.parts)
+        if (2 * P.lo[0] <= c1 && c1 <= 2 * P.hi[0] && 2 * P.lo[1] <= c2 && c2 <= 2 * P.hi[1])
+            return {P.nb_block, P.nb_face, P.periodic, P.trans};
+    throw std::runtime_error("face parts do not cover the face");
+}
+
 static void face_point(const Block &b, int face, int a1, int a2, int idx[3])
 {
     int ax, t1, t2;
@@ -283,7 +307,8 @@ static void node_get(const Grid &grd, int b, const int n[3], double p[3], int de
     }
     if (out_ax >= 0)
     {
-        const FaceLink &L = B.face[2 * out_ax + out_side];
+        const int dc[3] = {2 * n[0], 2 * n[1], 2 * n[2]};
+        const LinkView L = link_at(B, 2 * out_ax + out_side, dc);
         int m[3];
         for (int a = 0; a < 3; ++a)
             m[L.trans.perm[a]] = L.trans.sign[a] * n[a] + L.trans.offset[a];
@@ -915,11 +940,11 @@ DofSource resolve_dof(const Grid &grd, int b, int loc, const int idx_in[3])
         {
             if (++hops > 8)
                 throw std::runtime_error("resolve_dof: too many hops");
-            const FaceLink &L = B.face[2 * hop_ax + hop_side];
-            const IndexTransform &T = L.trans;
             int dc[3], ndc[3];
             for (int a = 0; a < 3; ++a)
                 dc[a] = 2 * r.idx[a] + loc_delta(r.loc, a);
+            const LinkView L = link_at(B, 2 * hop_ax + hop_side, dc);
+            const IndexTransform &T = L.trans;
             for (int a = 0; a < 3; ++a)
                 ndc[T.perm[a]] = T.sign[a] * dc[a] + 2 * T.offset[a];
             const int nloc = map_loc(r.loc, T.perm);

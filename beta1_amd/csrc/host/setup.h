@@ -33,6 +33,19 @@ struct IndexTransform
 };
 
 // One face of a block: -i,+i,-j,+j,-k,+k  (face = 2*axis + side).
+// One of several interface patches that share a block face (the reference's
+// inner patches may cover part of a face, src/1_grid/1_MPCNS_Grid.cpp:392-505):
+// the node range it covers along the face's two tangential axes (in axis
+// order, inclusive) and its link.
+struct FacePart
+{
+    int lo[2] = {0, 0}, hi[2] = {0, 0};
+    int nb_block = -1;
+    int nb_face = -1;
+    int periodic = 0;
+    IndexTransform trans{};
+};
+
 struct FaceLink
 {
     int kind = 0;     // 0 = physical, 1 = interface
@@ -41,6 +54,10 @@ struct FaceLink
     int nb_face = -1;
     int periodic = 0; // periodic interface: ghost nodes are mirrored (reference src/1_grid/1_MPCNS_Grid.cpp:1268)
     IndexTransform trans{};
+    // several interface patches (kind 1, one periodic flag): a point crosses
+    // the face through the part that covers it (nb_block .. trans above are
+    // parts[0]'s); empty: one link for the whole face
+    std::vector<FacePart> parts;
 };
 
 struct Block
@@ -262,6 +279,10 @@ void formed_jcell_weights(const BlockArrays &A, int i, int j, int k, double W[36
 // false with the first offending block cell in `why`.
 bool weight_tables_match(const BlockArrays &A, const double *wb_aos, const double *wj_aos, double rtol,
                          std::string &why);
+
+// Records why a host-case constructor failed (mhd_host_last_error(NULL)
+// reports it on the calling thread).
+void host_ctor_error(const std::string &msg);
 
 // Deterministic generator (SplitMix64).
 uint64_t splitmix64(uint64_t &s);

@@ -134,8 +134,10 @@ class HostCase:
             _check(lib().mhd_host_case_mercury(C.byref(desc.c()), C.byref(desc.dipole()), ghost, C.byref(self.h)),
                    what="mhd_host_case_mercury")
         elif isinstance(desc, CaseDirDesc):
-            _check(lib().mhd_host_case_from_dir(desc.path.encode(), C.byref(desc.wind.c()), ghost, C.byref(self.h)),
-                   what=f"mhd_host_case_from_dir({desc.path})")
+            rc = lib().mhd_host_case_from_dir(desc.path.encode(), C.byref(desc.wind.c()), ghost, C.byref(self.h))
+            if rc:
+                m = lib().mhd_host_last_error(None)
+                raise MhdError(f"mhd_host_case_from_dir({desc.path}) failed ({rc}): {m.decode() if m else ''}", rc)
         else:
             _check(lib().mhd_host_case_lunar(C.byref(desc.c()), ghost, C.byref(self.h)), what="mhd_host_case_lunar")
         self.desc = desc
@@ -196,6 +198,16 @@ class HostCase:
         out = (C.c_int * 84)()
         _check(lib().mhd_host_faces(self.h, block, out), what="faces")
         return np.array(out, dtype=np.int32).reshape(6, 14)
+
+    def face_parts(self, block, face):
+        """Patches of a face several interface patches share: rows of
+        (lo[2], hi[2], nb_block, nb_face, periodic, perm[3], sign[3],
+        offset[3]); empty for other faces."""
+        n = C.c_int()
+        cap = 64
+        out = (C.c_int * (16 * cap))()
+        _check(lib().mhd_host_face_parts(self.h, block, face, out, cap, C.byref(n)), what="face parts")
+        return np.array(out[:16 * n.value], dtype=np.int32).reshape(n.value, 16)
 
     def nodes(self, block):
         gn = self.g + 1

@@ -310,7 +310,9 @@ int mhd_upload(mhd_ctx *ctx, int case_id, int block, int field, const double *ao
    {kind (0 physical, 1 interface), bc, nb_block, nb_face, periodic,
     perm[3], sign[3], offset[3]} with the reference's IndexTransform
    convention nb[perm[a]] = sign[a]*local[a] + offset[a]
-   (include/2_topology/2_MPCNS_Topology.h:22-29). */
+   (include/2_topology/2_MPCNS_Topology.h:22-29).  kind 2 marks a face
+   several interface patches share (read from a CASE directory's inner file,
+   each patch over part of the face); the other fields describe the first. */
 int mhd_get_faces(mhd_ctx *ctx, int case_id, int block, int *faces14x6);
 /* node coordinates incl. ghost+1 layers, comp 0..2 */
 int mhd_get_nodes(mhd_ctx *ctx, int case_id, int block, int comp, double *out, size_t n);
@@ -354,7 +356,9 @@ int mhd_host_case_write_dir(mhd_host_case *h, const char *dir);
 int mhd_host_case_from_dir(const char *dir, const mhd_wind_desc *d, int ghost, mhd_host_case **out);
 /* mhd_case_begin .. mhd_case_end on the host (no device): the host case can
    be adopted by a context later (mhd_adopt_host_case) or kept on the host;
-   mhd_host_last_error gives the reason of a failed call. */
+   mhd_host_last_error gives the reason of a failed call; with a NULL case,
+   the reason the calling thread's last host-case constructor
+   (mhd_host_case_lunar / _mercury / _ot / _from_dir) failed. */
 int mhd_host_case_begin(int nblocks, int ghost, const double *consts22, mhd_host_case **out);
 int mhd_host_add_block(mhd_host_case *h, int ib, int ni, int nj, int nk, const double *x, const double *y,
                        const double *z, size_t nnode, const double *const *geometry);
@@ -373,6 +377,14 @@ int mhd_host_field_count(mhd_host_case *h, int field, size_t *n);
 int mhd_host_download(mhd_host_case *h, int block, int field, double *aos, size_t n);
 int mhd_host_upload(mhd_host_case *h, int block, int field, const double *aos, size_t n);
 int mhd_host_faces(mhd_host_case *h, int block, int *faces14x6);
+/* The patches of a face several interface patches share (mhd_host_faces
+   kind 2; a CASE directory's inner file), 16 ints each: {lo[2], hi[2] (node
+   range along the face's two tangential axes in axis order, inclusive),
+   nb_block, nb_face, periodic, perm[3], sign[3], offset[3]}; *nparts = 0 for
+   other faces.  No counterpart in the reference, which keeps its inner
+   patches in TOPO::Topology (include/2_topology/2_MPCNS_Topology.h:154);
+   the tests hand them to the oracle. */
+int mhd_host_face_parts(mhd_host_case *h, int block, int face, int *out16xn, int cap, int *nparts);
 int mhd_host_nodes(mhd_host_case *h, int block, int comp, double *out, size_t n);
 int mhd_host_constants(mhd_host_case *h, double *out22);
 /* two-fluid constants: model, ambipolar, M_Na, state_coeff_Na, qinfs[5],

@@ -45,6 +45,8 @@ typedef struct
 {
     int n[3];
     or_face face[6];
+    or_face_part *parts[6]; /* faces several interface patches share */
+    int nparts[6];
     fblk f[OF_COUNT];
     double *node[3];
     int gn;
@@ -112,6 +114,7 @@ void or_destroy(or_ctx *c)
     {
         for (int f = 0; f < OF_COUNT; ++f) free(c->b[b].f[f].d);
         for (int q = 0; q < 3; ++q) free(c->b[b].node[q]);
+        for (int f = 0; f < 6; ++f) free(c->b[b].parts[f]);
     }
     for (int l = 0; l < 8; ++l) free(c->hm[l].e);
     free(c->b);
@@ -147,6 +150,21 @@ int or_set_block(or_ctx *c, int b, int ni, int nj, int nk, const or_face *faces)
         for (int a = 0; a < 3; ++a) tot *= (size_t)(B->n[a] + 1 + 2 * B->gn);
         free(B->node[q]);
         B->node[q] = (double *)calloc(tot, sizeof(double));
+    }
+    return 0;
+}
+
+int or_set_face_parts(or_ctx *c, int b, int f, int n, const or_face_part *parts)
+{
+    if (b < 0 || b >= c->d.nblocks || f < 0 || f > 5 || n < 0) return -1;
+    oblock *B = &c->b[b];
+    free(B->parts[f]);
+    B->parts[f] = NULL;
+    B->nparts[f] = n;
+    if (n > 0)
+    {
+        B->parts[f] = (or_face_part *)malloc((size_t)n * sizeof(or_face_part));
+        memcpy(B->parts[f], parts, (size_t)n * sizeof(or_face_part));
     }
     return 0;
 }
@@ -191,6 +209,28 @@ static int map_loc(int loc, const int *perm)
     return loc;
 }
 
+/* the link a ghost crosses through interface face f: the face's own, or
+   the patch covering its doubled node coordinates dc (clamped onto the
+   face) on a face several patches share -- the reference copies each inner
+   patch's range (src/4_halo/1_MPCNS_Halo_Inner.cpp:60-69) */
+static const or_face *face_link(const oblock *B, int f, const int dc[3])
+{
+    if (B->nparts[f] == 0) return &B->face[f];
+    const int ax = f / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    int c1 = dc[t1], c2 = dc[t2];
+    if (c1 < 0) c1 = 0;
+    if (c1 > 2 * B->n[t1]) c1 = 2 * B->n[t1];
+    if (c2 < 0) c2 = 0;
+    if (c2 > 2 * B->n[t2]) c2 = 2 * B->n[t2];
+    for (int q = 0; q < B->nparts[f]; ++q)
+    {
+        const or_face_part *P = &B->parts[f][q];
+        if (2 * P->lo[0] <= c1 && c1 <= 2 * P->hi[0] && 2 * P->lo[1] <= c2 && c2 <= 2 * P->hi[1])
+            return &P->link;
+    }
+    return &B->face[f];
+}
+
 static osrc resolve(const or_ctx *c, int b, int loc, const int idx[3])
 {
     osrc r;
@@ -213,9 +253,9 @@ static osrc resolve(const or_ctx *c, int b, int loc, const int idx[3])
         }
         if (hax >= 0 && hops < 8)
         {
-            const or_face *F = &B->face[2 * hax + hside];
             int dc[3], nd[3];
             for (int a = 0; a < 3; ++a) dc[a] = 2 * r.idx[a] + delta_of(r.loc, a);
+            const or_face *F = face_link(B, 2 * hax + hside, dc);
             for (int a = 0; a < 3; ++a) nd[F->perm[a]] = F->sign[a] * dc[a] + 2 * F->offset[a];
             const int nl = map_loc(r.loc, F->perm);
             for (int a = 0; a < 3; ++a) r.idx[a] = (nd[a] - delta_of(nl, a)) / 2;

@@ -25,6 +25,16 @@ typedef struct
     int perm[3], sign[3], offset[3]; /* nb[perm[a]] = sign[a]*local[a] + offset[a] (node space) */
 } or_face;
 
+/* one of several interface patches sharing a block face (the reference's
+   inner patches may cover part of a face, src/1_grid/1_MPCNS_Grid.cpp:392-505):
+   its node range along the face's two tangential axes (axis order,
+   inclusive) and its link (kind 1) */
+typedef struct
+{
+    int lo[2], hi[2];
+    or_face link;
+} or_face_part;
+
 typedef struct
 {
     int nblocks;
@@ -68,6 +78,10 @@ or_ctx *or_create(const or_desc *d);
 void or_destroy(or_ctx *c);
 /* faces: 6 entries (-i,+i,-j,+j,-k,+k) */
 int or_set_block(or_ctx *c, int b, int ni, int nj, int nk, const or_face *faces);
+/* a face several interface patches share (after or_set_block): a ghost
+   crosses it through the patch covering its position in the face (clamped
+   onto the face; on the line between two patches, the first listed) */
+int or_set_face_parts(or_ctx *c, int b, int f, int n, const or_face_part *parts);
 /* pointer to the AoS storage of a field (allocated by or_set_block) */
 double *or_field(or_ctx *c, int b, int fid, long *count);
 /* node coordinates, box [-(g+1), n+g+1]^3, k fastest */

@@ -20,6 +20,10 @@ class OrFace(C.Structure):
                 ("periodic", C.c_int), ("perm", C.c_int * 3), ("sign", C.c_int * 3), ("offset", C.c_int * 3)]
 
 
+class OrFacePart(C.Structure):
+    _fields_ = [("lo", C.c_int * 2), ("hi", C.c_int * 2), ("link", OrFace)]
+
+
 class OrDesc(C.Structure):
     _fields_ = [("nblocks", C.c_int), ("g", C.c_int), ("recon", C.c_int),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("inver_MA2", C.c_double),
@@ -48,6 +52,7 @@ def olib():
         L.or_create.argtypes = [C.POINTER(OrDesc)]
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_set_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(OrFace)]
+        L.or_set_face_parts.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(OrFacePart)]
         L.or_field.restype = C.POINTER(C.c_double)
         L.or_field.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_long)]
         L.or_nodes.restype = C.POINTER(C.c_double)
@@ -123,7 +128,21 @@ class Oracle:
                 for a in range(3):
                     faces[f].perm[a], faces[f].sign[a], faces[f].offset[a] = int(r[5 + a]), int(r[8 + a]), int(r[11 + a])
             n = self.ns[b]
+            parts = {}
+            for f in range(6):
+                if faces[f].kind == 2:  # several interface patches share the face
+                    faces[f].kind = 1
+                    parts[f] = hc.face_parts(b, f)
             L.or_set_block(self.c, b, n[0], n[1], n[2], faces)
+            for f, rows in parts.items():
+                arr = (OrFacePart * len(rows))()
+                for q, r in enumerate(rows):
+                    arr[q].lo[0], arr[q].lo[1], arr[q].hi[0], arr[q].hi[1] = map(int, r[:4])
+                    lk = arr[q].link
+                    lk.kind, lk.bc, lk.nb_block, lk.nb_face, lk.periodic = 1, 0, int(r[4]), int(r[5]), int(r[6])
+                    for a in range(3):
+                        lk.perm[a], lk.sign[a], lk.offset[a] = int(r[7 + a]), int(r[10 + a]), int(r[13 + a])
+                L.or_set_face_parts(self.c, b, f, len(rows), arr)
             for name in GEOMETRY + STATE + (STATE_TWO_FLUID if two else []):
                 self.view(name, b)[...] = hc.field(name, b)
             for comp, arr in enumerate(hc.nodes(b)):

@@ -74,3 +74,34 @@ def test_tecplot_of_the_device_state_equals_host_writer(tmp_path, desc):
     host = str(tmp_path / "host.plt")
     hc.write_tecplot(host, time=0.75)
     assert open(dev, "rb").read() == open(host, "rb").read()
+
+
+@pytest.mark.parametrize("layout", ["aligned", "turned"])
+def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout):
+    """A CASE directory whose block faces are split among neighbours (the
+    periodic box as three blocks, A's two i faces each shared by B and C, C
+    aligned or turned half a turn about k) steps on the device bit for bit
+    as the oracle steps it: the op lists the host builds from the patches
+    and the oracle's own halo resolution agree, through the one-pass EMF."""
+    import test_face_patches as T
+    from beta1_amd.fields import interior
+    from oracle_bind import Oracle
+
+    T.use(layout)
+    three = T.three_block_dir(tmp_path, T.one_block_dir(tmp_path))
+    orc = Oracle(HostCase(CaseDirDesc(three, WindDesc()), ghost=2))
+    T.fill(orc, [0, 1, 2])
+    ctx = MhdContext()
+    cid = ctx.add(CaseDirDesc(three, WindDesc()))
+    ctx.finalize()
+    for b in range(3):
+        for name in ("U_H",) + T.FACES:
+            ctx.set_field(cid, name, b, orc.view(name, b))
+    ctx.refresh()
+    dts = orc.step(4)
+    ctx.step(4)
+    assert ctx.last_dt(cid) == dts[-1]
+    for b, n in enumerate(T.DIMS):
+        for name in ("U_H",) + T.FACES:
+            sl = interior(name, n, 2)
+            assert np.array_equal(ctx.field(cid, name, b)[sl], orc.view(name, b)[sl]), (name, b)
