@@ -110,10 +110,12 @@ SIGNATURES = {
     "mhd_case_begin": (_I, [_P, _I, _PD, _PI]),
     "mhd_add_block": (_I, [_P, _I, _I, _I, _I, _I, _PD, _PD, _PD, _S, C.POINTER(_PD)]),
     "mhd_set_topology": (_I, [_P, _I, _I, _PI]),
+    "mhd_set_face_parts": (_I, [_P, _I, _I, _I, _I, _PI]),
     "mhd_case_end": (_I, [_P, _I]),
     "mhd_host_case_begin": (_I, [_I, _I, _PD, C.POINTER(_P)]),
     "mhd_host_add_block": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _S, C.POINTER(_PD)]),
     "mhd_host_set_topology": (_I, [_P, _I, _PI]),
+    "mhd_host_set_face_parts": (_I, [_P, _I, _I, _I, _PI]),
     "mhd_host_case_end": (_I, [_P]),
     "mhd_host_last_error": (C.c_char_p, [_P]),
     "mhd_step_dt": (_I, [_P]),

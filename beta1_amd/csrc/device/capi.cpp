@@ -1863,6 +1863,15 @@ extern "C" int mhd_set_topology(mhd_ctx *c, int case_id, int ib, const int *face
     return rc ? fail(c, rc, r->draft->err) : MHD_OK;
 }
 
+extern "C" int mhd_set_face_parts(mhd_ctx *c, int case_id, int ib, int face, int nparts, const int *parts16)
+{
+    CaseRec *r = draft_of(c, case_id);
+    if (!r)
+        return c ? fail(c, MHD_ERR_STATE, "mhd_set_face_parts: not a case begun with mhd_case_begin") : MHD_ERR_ARG;
+    const int rc = mhd_host_set_face_parts(r->draft, ib, face, nparts, parts16);
+    return rc ? fail(c, rc, r->draft->err) : MHD_OK;
+}
+
 extern "C" int mhd_case_end(mhd_ctx *c, int case_id)
 {
     CaseRec *r = draft_of(c, case_id);

@@ -32,6 +32,7 @@ struct DraftBlock
     int ni = 0, nj = 0, nk = 0;
     std::vector<double> x, y, z;             // interior nodes, [i][j][k], k fastest
     FaceLink face[6];
+    bool split[6] = {false, false, false, false, false, false}; // kind 2: patches to come (mhd_host_set_face_parts)
     std::vector<std::vector<double>> given;  // kGeoFields AoS arrays (empty: not given)
 };
 } // namespace
@@ -64,16 +65,56 @@ bool perm_ok(const int *p)
     return true;
 }
 
+// Patches of a split face: each a non-empty cell range [lo, hi) of the
+// face's two tangential axes (n1 cells along the first, n2 along the
+// second), together covering every cell face once, all periodic or all not.
+// Empty string when they do.
+std::string face_parts_problem(const std::vector<FacePart> &parts, int n1, int n2)
+{
+    std::vector<int> cover((size_t)n1 * n2, 0);
+    for (const FacePart &P : parts)
+    {
+        if (P.lo[0] < 0 || P.hi[0] > n1 || P.lo[1] < 0 || P.hi[1] > n2 || P.lo[0] >= P.hi[0] || P.lo[1] >= P.hi[1])
+            return "patch range outside the face";
+        if (P.periodic != parts[0].periodic)
+            return "periodic and plain patches on one face";
+        for (int u = P.lo[0]; u < P.hi[0]; ++u)
+            for (int v = P.lo[1]; v < P.hi[1]; ++v)
+                cover[(size_t)u * n2 + v] += 1;
+    }
+    for (int c : cover)
+        if (c != 1)
+            return "patches do not cover the face exactly once";
+    return "";
+}
+
+// The links of an interface face with the cell range each covers: the
+// parts of a split face, else the whole face as one.
+std::vector<FacePart> face_links(const DraftBlock &A, int f)
+{
+    const FaceLink &L = A.face[f];
+    if (!L.parts.empty())
+        return L.parts;
+    const int n[3] = {A.ni, A.nj, A.nk}, ax = f / 2;
+    FacePart P;
+    P.lo[0] = P.lo[1] = 0;
+    P.hi[0] = n[ax == 0 ? 1 : 0];
+    P.hi[1] = n[ax == 2 ? 1 : 2];
+    P.nb_block = L.nb_block, P.nb_face = L.nb_face, P.periodic = L.periodic;
+    P.trans = L.trans;
+    return {P};
+}
+
 // Node (i, j, k) of face f's node plane, the transform image in the
 // neighbour, both as coordinates; false when the image is outside it.
-bool face_node_image(const std::vector<DraftBlock> &B, int b, int f, const int loc[3], double p[3], double q[3])
+bool face_node_image(const std::vector<DraftBlock> &B, int b, int nb, const IndexTransform &T, const int loc[3],
+                     double p[3], double q[3])
 {
     const DraftBlock &A = B[b];
-    const FaceLink &L = A.face[f];
-    const DraftBlock &N = B[L.nb_block];
+    const DraftBlock &N = B[nb];
     int t[3];
     for (int a = 0; a < 3; ++a)
-        t[L.trans.perm[a]] = L.trans.sign[a] * loc[a] + L.trans.offset[a];
+        t[T.perm[a]] = T.sign[a] * loc[a] + T.offset[a];
     const int nn[3] = {N.ni, N.nj, N.nk};
     for (int a = 0; a < 3; ++a)
         if (t[a] < 0 || t[a] > nn[a])
@@ -261,11 +302,66 @@ extern "C" int mhd_host_set_topology(mhd_host_case *h, int ib, const int *faces)
                     return fail(h, MHD_ERR_ARG, where + "transform signs must be +1 or -1");
             L.bc = BC_NONE;
         }
+        else if (L.kind == 2) // several interface patches, given by mhd_host_set_face_parts
+        {
+            L = FaceLink{};
+            L.kind = 1;
+        }
         else
-            return fail(h, MHD_ERR_ARG, where + "kind must be 0 (physical) or 1 (interface)");
+            return fail(h, MHD_ERR_ARG, where + "kind must be 0 (physical), 1 (interface) or 2 (several interface "
+                                                "patches)");
         B.face[f] = L;
+        B.split[f] = o[0] == 2;
     }
     B.topo = true;
+    return MHD_OK;
+}
+
+extern "C" int mhd_host_set_face_parts(mhd_host_case *h, int ib, int face, int nparts, const int *parts)
+{
+    if (!h || !h->draft)
+        return fail(h, MHD_ERR_STATE, "mhd_host_set_face_parts: not a case begun with mhd_host_case_begin");
+    mhd_case_draft &D = *h->draft;
+    if (ib < 0 || ib >= (int)D.blk.size() || face < 0 || face > 5)
+        return fail(h, MHD_ERR_ARG, "mhd_host_set_face_parts: bad block or face index");
+    DraftBlock &B = D.blk[ib];
+    const std::string where = "block " + std::to_string(ib) + " face " + std::to_string(face) + ": ";
+    if (!B.topo || !B.split[face])
+        return fail(h, MHD_ERR_STATE, where + "declare the face kind 2 with mhd_host_set_topology first");
+    if (nparts < 2 || !parts)
+        return fail(h, MHD_ERR_ARG, where + "a split face takes two or more patches");
+    FaceLink L;
+    L.kind = 1;
+    for (int q = 0; q < nparts; ++q)
+    {
+        const int *o = parts + 16 * q;
+        FacePart P;
+        P.lo[0] = o[0], P.lo[1] = o[1], P.hi[0] = o[2], P.hi[1] = o[3];
+        P.nb_block = o[4], P.nb_face = o[5], P.periodic = o[6];
+        for (int a = 0; a < 3; ++a)
+        {
+            P.trans.perm[a] = o[7 + a];
+            P.trans.sign[a] = o[10 + a];
+            P.trans.offset[a] = o[13 + a];
+        }
+        if (P.nb_block < 0 || P.nb_block >= (int)D.blk.size() || P.nb_face < 0 || P.nb_face > 5 ||
+            !perm_ok(P.trans.perm))
+            return fail(h, MHD_ERR_ARG, where + "patch " + std::to_string(q) + ": bad neighbour or permutation");
+        for (int a = 0; a < 3; ++a)
+            if (P.trans.sign[a] != 1 && P.trans.sign[a] != -1)
+                return fail(h, MHD_ERR_ARG, where + "transform signs must be +1 or -1");
+        L.parts.push_back(P);
+    }
+    const int ax = face / 2, n[3] = {B.ni, B.nj, B.nk};
+    const std::string bad = face_parts_problem(L.parts, n[ax == 0 ? 1 : 0], n[ax == 2 ? 1 : 2]);
+    if (!bad.empty())
+        return fail(h, MHD_ERR_ARG, where + bad);
+    L.nb_block = L.parts[0].nb_block;
+    L.nb_face = L.parts[0].nb_face;
+    L.periodic = L.parts[0].periodic;
+    L.trans = L.parts[0].trans;
+    B.face[face] = L;
+    B.split[face] = false;
     return MHD_OK;
 }
 
@@ -276,42 +372,78 @@ extern "C" int mhd_host_case_end(mhd_host_case *h)
     const mhd_case_draft &D = *h->draft;
     const int nb = (int)D.blk.size();
     for (int b = 0; b < nb; ++b)
+    {
         if (!D.blk[b].have || !D.blk[b].topo)
             return fail(h, MHD_ERR_STATE, "block " + std::to_string(b) + ": nodes or topology missing");
-    // interfaces: reciprocal, and (unless periodic) every node of the face
-    // maps onto a coincident node of the neighbour
+        for (int f = 0; f < 6; ++f)
+            if (D.blk[b].split[f])
+                return fail(h, MHD_ERR_STATE, "block " + std::to_string(b) + " face " + std::to_string(f) +
+                                                  ": declared split, patches missing (mhd_host_set_face_parts)");
+    }
+    // interfaces: each link (a whole face, or each patch of a split face)
+    // is met by a neighbour link that maps every one of its cell faces back
+    // (same periodic flag, inverse transform), and (unless periodic) maps
+    // every node of its range onto a coincident node of the neighbour
     for (int b = 0; b < nb; ++b)
         for (int f = 0; f < 6; ++f)
         {
-            const FaceLink &L = D.blk[b].face[f];
-            if (L.kind != 1)
-                continue;
-            const FaceLink &R = D.blk[L.nb_block].face[L.nb_face];
-            const std::string where = "block " + std::to_string(b) + " face " + std::to_string(f) + ": ";
-            if (R.kind != 1 || R.nb_block != b || R.nb_face != f || R.periodic != L.periodic)
-                return fail(h, MHD_ERR_ARG, where + "the neighbour face does not link back");
-            if (L.periodic)
-                continue;
             const DraftBlock &A = D.blk[b];
-            const int n[3] = {A.ni, A.nj, A.nk}, ax = f / 2;
+            if (A.face[f].kind != 1)
+                continue;
+            const std::string where = "block " + std::to_string(b) + " face " + std::to_string(f) + ": ";
+            const int n[3] = {A.ni, A.nj, A.nk}, ax = f / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
             double scale = 0.0;
             for (size_t q = 0; q < A.x.size(); ++q)
                 scale = std::max(scale, std::max(std::fabs(A.x[q]), std::max(std::fabs(A.y[q]), std::fabs(A.z[q]))));
-            int loc[3];
-            for (loc[0] = 0; loc[0] <= n[0]; ++loc[0])
-                for (loc[1] = 0; loc[1] <= n[1]; ++loc[1])
-                    for (loc[2] = 0; loc[2] <= n[2]; ++loc[2])
+            for (const FacePart &P : face_links(A, f))
+            {
+                if (D.blk[P.nb_block].face[P.nb_face].kind != 1)
+                    return fail(h, MHD_ERR_ARG, where + "the neighbour face does not link back");
+                const std::vector<FacePart> back = face_links(D.blk[P.nb_block], P.nb_face);
+                const int bax = P.nb_face / 2, b1 = bax == 0 ? 1 : 0, b2 = bax == 2 ? 1 : 2;
+                // cell-face centres in doubled node coordinates
+                int dc[3];
+                dc[ax] = (f & 1) ? 2 * n[ax] : 0;
+                for (int u = P.lo[0]; u < P.hi[0]; ++u)
+                    for (int v = P.lo[1]; v < P.hi[1]; ++v)
                     {
-                        if (loc[ax] != ((f & 1) ? n[ax] : 0))
-                            continue;
+                        dc[t1] = 2 * u + 1;
+                        dc[t2] = 2 * v + 1;
+                        int tc[3];
+                        for (int a = 0; a < 3; ++a)
+                            tc[P.trans.perm[a]] = P.trans.sign[a] * dc[a] + 2 * P.trans.offset[a];
+                        const FacePart *R = nullptr;
+                        for (const FacePart &Q : back)
+                            if (2 * Q.lo[0] < tc[b1] && tc[b1] < 2 * Q.hi[0] && 2 * Q.lo[1] < tc[b2] &&
+                                tc[b2] < 2 * Q.hi[1])
+                                R = &Q;
+                        bool ok = R && R->nb_block == b && R->nb_face == f && R->periodic == P.periodic;
+                        if (ok)
+                        {
+                            int rc[3];
+                            for (int a = 0; a < 3; ++a)
+                                rc[R->trans.perm[a]] = R->trans.sign[a] * tc[a] + 2 * R->trans.offset[a];
+                            ok = rc[0] == dc[0] && rc[1] == dc[1] && rc[2] == dc[2];
+                        }
+                        if (!ok)
+                            return fail(h, MHD_ERR_ARG, where + "the neighbour face does not link back");
+                    }
+                if (P.periodic)
+                    continue;
+                int loc[3];
+                loc[ax] = (f & 1) ? n[ax] : 0;
+                for (loc[t1] = P.lo[0]; loc[t1] <= P.hi[0]; ++loc[t1])
+                    for (loc[t2] = P.lo[1]; loc[t2] <= P.hi[1]; ++loc[t2])
+                    {
                         double p[3], q[3];
-                        if (!face_node_image(D.blk, b, f, loc, p, q))
+                        if (!face_node_image(D.blk, b, P.nb_block, P.trans, loc, p, q))
                             return fail(h, MHD_ERR_ARG, where + "the transform maps a face node outside the neighbour");
                         for (int a = 0; a < 3; ++a)
                             if (!(std::fabs(p[a] - q[a]) <= 1e-9 * std::max(scale, 1.0)))
                                 return fail(h, MHD_ERR_ARG, where + "the transform maps a face node onto a node "
                                                                     "at another position");
                     }
+            }
         }
     try
     {

@@ -170,7 +170,7 @@ class HostCase:
         def err(what):
             m = lib().mhd_host_last_error(self.h)
             return f"{what}: {m.decode() if m else ''}"
-        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks, ghost)):
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces, parts) in enumerate(_parts_args(blocks, ghost)):
             rc = lib().mhd_host_add_block(self.h, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
                                           xyz[0].size, ptrs)
             if rc:
@@ -178,6 +178,10 @@ class HostCase:
             rc = lib().mhd_host_set_topology(self.h, ib, faces)
             if rc:
                 raise MhdError(err(f"mhd_host_set_topology({ib})"), rc)
+            for f, npart, rows in parts:
+                rc = lib().mhd_host_set_face_parts(self.h, ib, f, npart, rows)
+                if rc:
+                    raise MhdError(err(f"mhd_host_set_face_parts({ib}, {f})"), rc)
         rc = lib().mhd_host_case_end(self.h)
         if rc:
             raise MhdError(err("mhd_host_case_end"), rc)
@@ -271,8 +275,9 @@ class HostCase:
 def _parts_args(blocks, ghost):
     """ctypes arguments of mhd_(host_)add_block / set_topology for each block
     of `blocks`: dicts with "n" (ni, nj, nk), "nodes" (x, y, z over the
-    interior nodes, shape (ni+1, nj+1, nk+1)), "faces" (6 x 14 ints) and an
-    optional "geometry" {field name: AoS array}."""
+    interior nodes, shape (ni+1, nj+1, nk+1)), "faces" (6 x 14 ints), an
+    optional "parts" {face: rows of 16 ints} for faces of kind 2 (several
+    interface patches) and an optional "geometry" {field name: AoS array}."""
     geo_names = ["Jac", "JDxi", "JDet", "JDze", "beta_xi", "beta_eta", "beta_zeta", "alpha_xi", "alpha_eta",
                  "alpha_zeta", "Bcell_from_Bface_w", "Jcell_from_Jedge_w"]
     out = []
@@ -286,7 +291,13 @@ def _parts_args(blocks, ghost):
         keep = [np.ascontiguousarray(geo[n], dtype=np.float64) if n in geo else None for n in geo_names]
         ptrs = (C.POINTER(C.c_double) * 12)(*[_dptr(a) if a is not None else None for a in keep])
         faces = (C.c_int * 84)(*[int(v) for v in np.asarray(blk["faces"]).ravel()])
-        out.append((ni, nj, nk, xyz, keep, ptrs, faces))
+        parts = []
+        for f, rows in sorted((blk.get("parts") or {}).items()):
+            flat = [int(v) for v in np.asarray(rows).ravel()]
+            if len(flat) % 16:
+                raise ValueError("face patches are rows of 16 ints")
+            parts.append((int(f), len(flat) // 16, (C.c_int * max(len(flat), 1))(*flat)))
+        out.append((ni, nj, nk, xyz, keep, ptrs, faces, parts))
     return out
 
 
@@ -302,6 +313,9 @@ def case_parts(hc, geometry=True):
         n = hc.ns[b]
         nodes = [a[gn:gn + n[0] + 1, gn:gn + n[1] + 1, gn:gn + n[2] + 1] for a in hc.nodes(b)]
         blk = {"n": n, "nodes": nodes, "faces": hc.faces(b)}
+        parts = {f: hc.face_parts(b, f) for f in range(6) if blk["faces"][f][0] == 2}
+        if parts:
+            blk["parts"] = parts
         if geometry:
             blk["geometry"] = {nm: hc.field(nm, b) for nm in names}
         blocks.append(blk)
@@ -348,14 +362,17 @@ class MhdContext:
 
     def add_parts(self, consts, blocks):
         """A case from a driver's own arrays through mhd_case_begin,
-        mhd_add_block, mhd_set_topology and mhd_case_end."""
+        mhd_add_block, mhd_set_topology, mhd_set_face_parts and mhd_case_end."""
         cid = C.c_int()
         k = np.ascontiguousarray(consts, dtype=np.float64)
         _check(lib().mhd_case_begin(self.ctx, len(blocks), _dptr(k), C.byref(cid)), self.ctx, "mhd_case_begin")
-        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks, self.ghost)):
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces, parts) in enumerate(_parts_args(blocks, self.ghost)):
             _check(lib().mhd_add_block(self.ctx, cid, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
                                        xyz[0].size, ptrs), self.ctx, f"mhd_add_block({ib})")
             _check(lib().mhd_set_topology(self.ctx, cid, ib, faces), self.ctx, f"mhd_set_topology({ib})")
+            for f, npart, rows in parts:
+                _check(lib().mhd_set_face_parts(self.ctx, cid, ib, f, npart, rows), self.ctx,
+                       f"mhd_set_face_parts({ib}, {f})")
         _check(lib().mhd_case_end(self.ctx, cid), self.ctx, "mhd_case_end")
         self.cases.append(None)
         return cid.value

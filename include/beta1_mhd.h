@@ -170,13 +170,23 @@ int mhd_add_case_from_dir(mhd_ctx *ctx, const char *dir, const mhd_wind_desc *d,
    JDze, beta_xi/eta/zeta, alpha_xi/eta/zeta, Bcell_from_Bface_w,
    Jcell_from_Jedge_w) with the context's ghost depth, or NULL.
    mhd_set_topology gives its six faces (-i,+i,-j,+j,-k,+k) as 14 ints each,
-   the layout of mhd_get_faces: {kind (0 physical, 1 interface), bc (1
-   Farfield, 2 Solid_Surface), nb_block, nb_face, periodic, perm[3],
-   sign[3], offset[3]} with the reference's IndexTransform
+   the layout of mhd_get_faces: {kind (0 physical, 1 interface, 2 several
+   interface patches), bc (1 Farfield, 2 Solid_Surface), nb_block, nb_face,
+   periodic, perm[3], sign[3], offset[3]} with the reference's IndexTransform
    nb[perm[a]] = sign[a]*local[a] + offset[a]
-   (include/2_topology/2_MPCNS_Topology.h:24-151).
-   mhd_case_end checks that every interface links back and maps each face
-   node onto a node at the same position (periodic links excepted), fills
+   (include/2_topology/2_MPCNS_Topology.h:24-151); the other fields of a
+   kind 2 face are ignored.
+   mhd_set_face_parts gives the nparts >= 2 interface patches of a face set
+   to kind 2, 16 ints each in the layout of mhd_host_face_parts: the cell
+   range [lo0, hi0) x [lo1, hi1) of the face's two tangential axes (j, k on
+   an i face; i, k on a j face; i, j on a k face), then nb_block, nb_face,
+   periodic, perm[3], sign[3], offset[3] (the inner-file patches of one face,
+   Grid::Read_Inner_File, src/1_grid/2_MPCNS_Grid_Read.cpp).  The patches
+   cover every cell face of the face once and are all periodic or all not.
+   mhd_case_end checks that every interface (each patch of a split face)
+   links back, each of its cell faces mapped back onto itself by the
+   neighbour's patch, and maps each of its nodes onto a node at the same
+   position (periodic links excepted), fills
    the ghost nodes (mirror at physical faces, copy across interfaces), builds
    the geometry, weights, halo / boundary ops and the Lunar initial state
    (U = qinf, B_face = 0, Badd = B_imf . S; Z4_Lunar/include/2_Initial.h:
@@ -189,6 +199,7 @@ int mhd_case_begin(mhd_ctx *ctx, int nblocks, const double *consts22, int *case_
 int mhd_add_block(mhd_ctx *ctx, int case_id, int ib, int ni, int nj, int nk, const double *x, const double *y,
                   const double *z, size_t nnode, const double *const *geometry);
 int mhd_set_topology(mhd_ctx *ctx, int case_id, int ib, const int *faces14x6);
+int mhd_set_face_parts(mhd_ctx *ctx, int case_id, int ib, int face, int nparts, const int *parts16);
 int mhd_case_end(mhd_ctx *ctx, int case_id);
 
 /* Upload everything added so far and compute the derived fields
@@ -363,6 +374,7 @@ int mhd_host_case_begin(int nblocks, int ghost, const double *consts22, mhd_host
 int mhd_host_add_block(mhd_host_case *h, int ib, int ni, int nj, int nk, const double *x, const double *y,
                        const double *z, size_t nnode, const double *const *geometry);
 int mhd_host_set_topology(mhd_host_case *h, int ib, const int *faces14x6);
+int mhd_host_set_face_parts(mhd_host_case *h, int ib, int face, int nparts, const int *parts16);
 int mhd_host_case_end(mhd_host_case *h);
 const char *mhd_host_last_error(mhd_host_case *h);
 /* MPCNSRST restart of the host arrays (read also re-syncs U and B ghosts). */

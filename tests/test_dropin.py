@@ -105,7 +105,7 @@ def _gamma(k, blocks):
 
 
 @pytest.mark.parametrize("change,reason", [
-    (_link_not_back, "does not link back"), (_offset, "another position|outside the neighbour"),
+    (_link_not_back, "does not link back"), (_offset, "does not link back|another position|outside the neighbour"),
     (_metric, "JDxi"), (_table, "Jcell_from_Jedge_w"), (_bc, "bc 1"), (_perm, "permutation"),
     (_node, "another position"), (_gamma, "mhd_host_case_begin")])
 def test_inconsistent_parts_are_refused_with_a_reason(change, reason):
@@ -138,7 +138,7 @@ def _five_of_six(k, blocks):
     kk = np.ascontiguousarray(k, dtype=np.float64)
     assert lib().mhd_host_case_begin(6, G, _dptr(kk), C.byref(h)) == 0
     try:
-        for ib, (ni, nj, nk, xyz, keep, ptrs, faces) in enumerate(_parts_args(blocks[:5], G)):
+        for ib, (ni, nj, nk, xyz, keep, ptrs, faces, _parts) in enumerate(_parts_args(blocks[:5], G)):
             assert lib().mhd_host_add_block(h, ib, ni, nj, nk, _dptr(xyz[0]), _dptr(xyz[1]), _dptr(xyz[2]),
                                             xyz[0].size, ptrs) == 0
             assert lib().mhd_host_set_topology(h, ib, faces) == 0

@@ -10,7 +10,9 @@ each shared by B and C, one through plain and one through periodic patches --
 must step to the same values bit for bit, every interior cell and face, with
 the same dt.  A wrong part chosen for a ghost, a wrong offset or a wrong
 split line moves a value and fails it.  The case directory round trips with
-its patches, and faces that mix patch kinds or leave a gap are refused.
+its patches, the same case handed over as a driver's arrays (split faces
+through mhd_host_set_face_parts) is that case bit for bit, and faces that
+mix patch kinds, leave a gap or link back elsewhere are refused.
 """
 import os
 import shutil
@@ -19,7 +21,7 @@ import struct
 import numpy as np
 import pytest
 
-from beta1_amd import CaseDirDesc, HostCase, OTDesc, WindDesc
+from beta1_amd import CaseDirDesc, HostCase, OTDesc, WindDesc, case_parts
 from beta1_amd.capi import MhdError
 from oracle_bind import Oracle
 
@@ -310,3 +312,69 @@ def test_faces_mixing_patch_kinds_or_leaving_gaps_are_refused(tmp_path):
     ff = ["1", f"{CI} {CI} {CJ} {N} 0 {N} 2", "0", "0"]
     with pytest.raises(MhdError, match="mixes"):
         HostCase(CaseDirDesc(three_block_dir(tmp_path / "mix", one, extra=ff, drop=a_to_c), WindDesc()), ghost=G)
+
+
+def test_split_faces_through_the_drop_in_calls(tmp_path):
+    """The turned three-block case as a driver hands it over: faces of kind 2
+    with their patches.  Nodes, faces, patches, geometry, op lists and the
+    oracle's steps equal those of the case its CASE directory gives."""
+    use("turned")
+    h3 = HostCase(CaseDirDesc(three_block_dir(tmp_path, one_block_dir(tmp_path)), WindDesc()), ghost=G)
+    k, blocks = case_parts(h3, geometry=False)
+    assert sorted(blocks[0]["parts"]) == [0, 1]
+    back = HostCase.from_parts(k, blocks, ghost=G)
+    for b in range(3):
+        assert np.array_equal(h3.faces(b), back.faces(b))
+        for f in range(6):
+            assert np.array_equal(h3.face_parts(b, f), back.face_parts(b, f))
+        for p, q in zip(h3.nodes(b), back.nodes(b)):
+            assert np.array_equal(p, q)
+        for name in ("Jac", "JDxi", "JDet", "JDze", "Bcell_from_Bface_w", "Jcell_from_Jedge_w", "U_H"):
+            assert np.array_equal(h3.field(name, b), back.field(name, b)), (name, b)
+    for sy in range(14):
+        assert h3.op_count(sy) == back.op_count(sy), sy
+    o3, ob = Oracle(h3), Oracle(back)
+    fill(o3, [0, 1, 2])
+    fill(ob, [0, 1, 2])
+    assert o3.step(2) == ob.step(2)
+    for b in range(3):
+        for name in ("U_H",) + FACES:
+            assert np.array_equal(o3.view(name, b), ob.view(name, b)), (name, b)
+
+
+def test_inconsistent_split_faces_are_refused(tmp_path):
+    use("aligned")
+    h3 = HostCase(CaseDirDesc(three_block_dir(tmp_path, one_block_dir(tmp_path)), WindDesc()), ghost=G)
+
+    def refused(change, reason):
+        k, blocks = case_parts(h3, geometry=False)
+        change(blocks)
+        with pytest.raises(MhdError, match=reason):
+            HostCase.from_parts(k, blocks, ghost=G)
+
+    def drop_part(blocks):
+        blocks[0]["parts"][1] = blocks[0]["parts"][1][:1]
+    refused(drop_part, "two or more patches")
+
+    def shrink(blocks):
+        p = blocks[0]["parts"][1].copy()
+        p[0, 3] -= 1  # the first patch stops one cell short along k
+        blocks[0]["parts"][1] = p
+    refused(shrink, "cover the face")
+
+    def shift(blocks):
+        p = blocks[0]["parts"][1].copy()
+        p[0, 15] += 1  # the first patch lands one cell further along k
+        blocks[0]["parts"][1] = p
+    refused(shift, "link back")
+
+    def no_parts(blocks):
+        del blocks[0]["parts"]
+    refused(no_parts, "patches missing")
+
+    def kind_one(blocks):
+        f = blocks[0]["faces"].copy()
+        f[1][0] = 1  # A's +i face a single interface to B: B's part of it links back, C's does not
+        blocks[0]["faces"] = f
+        del blocks[0]["parts"][1]
+    refused(kind_one, "link back|another position|outside")

@@ -4,7 +4,7 @@ read by another continues bit for bit."""
 import numpy as np
 import pytest
 
-from beta1_amd import CaseDirDesc, HostCase, MercuryDesc, MhdContext, WindDesc
+from beta1_amd import CaseDirDesc, HostCase, MercuryDesc, MhdContext, WindDesc, case_parts
 
 pytestmark = pytest.mark.gpu
 
@@ -76,13 +76,16 @@ def test_tecplot_of_the_device_state_equals_host_writer(tmp_path, desc):
     assert open(dev, "rb").read() == open(host, "rb").read()
 
 
+@pytest.mark.parametrize("source", ["dir", "parts"])
 @pytest.mark.parametrize("layout", ["aligned", "turned"])
-def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout):
+def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout, source):
     """A CASE directory whose block faces are split among neighbours (the
     periodic box as three blocks, A's two i faces each shared by B and C, C
     aligned or turned half a turn about k) steps on the device bit for bit
     as the oracle steps it: the op lists the host builds from the patches
-    and the oracle's own halo resolution agree, through the one-pass EMF."""
+    and the oracle's own halo resolution agree, through the one-pass EMF.
+    The case comes either from the directory or as a driver's arrays
+    (mhd_case_begin .. mhd_set_face_parts .. mhd_case_end)."""
     import test_face_patches as T
     from beta1_amd.fields import interior
     from oracle_bind import Oracle
@@ -92,7 +95,10 @@ def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout):
     orc = Oracle(HostCase(CaseDirDesc(three, WindDesc()), ghost=2))
     T.fill(orc, [0, 1, 2])
     ctx = MhdContext()
-    cid = ctx.add(CaseDirDesc(three, WindDesc()))
+    if source == "dir":
+        cid = ctx.add(CaseDirDesc(three, WindDesc()))
+    else:
+        cid = ctx.add_parts(*case_parts(HostCase(CaseDirDesc(three, WindDesc()), ghost=2)))
     ctx.finalize()
     for b in range(3):
         for name in ("U_H",) + T.FACES:
