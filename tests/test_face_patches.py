@@ -69,16 +69,22 @@ def stretched(t, amp):
     return t + amp * np.sin(2 * np.pi * t) / (2 * np.pi)
 
 
-def one_block_dir(tmp_path):
-    """The periodic N^3 box (self-linked faces) with stretched coordinates."""
-    d = str(tmp_path / "ONE")
+def one_block_dir(tmp_path, curved=False):
+    """The periodic N^3 box (self-linked faces) with stretched coordinates;
+    `curved` adds to each coordinate a term periodic in the other two, so
+    the grid lines bend and cross at varying angles (a non-orthogonal,
+    curvilinear grid whose periodic faces stay translates of each other)."""
+    d = str(tmp_path / ("CURVED" if curved else "ONE"))
     HostCase(OTDesc(n=N), ghost=G).write_dir(d)
     path = os.path.join(d, "geometry", "grid0000.grd")
     (x, y, z), = read_grd(path)
-    xyz = [x, y, z]
-    for a in range(3):
-        L, amp = STRETCH[a]
-        xyz[a] = L * stretched(xyz[a], amp)
+    t = [x, y, z]
+    xyz = [STRETCH[a][0] * stretched(t[a], STRETCH[a][1]) for a in range(3)]
+    if curved:
+        w = 2 * np.pi
+        xyz[0] = xyz[0] + 0.03 * np.sin(w * t[1]) * np.cos(w * t[2])
+        xyz[1] = xyz[1] + 0.025 * np.sin(w * t[2] + 0.3) * np.cos(w * t[0])
+        xyz[2] = xyz[2] + 0.03 * np.sin(w * t[0]) * np.sin(w * t[1])
     write_grd(path, [xyz])
     return d
 
@@ -244,14 +250,18 @@ def olib_update(orc):
     olib().or_update_derived(orc.c)
 
 
+@pytest.mark.parametrize("curved", [False, True])
 @pytest.mark.parametrize("layout", ["aligned", "turned"])
-def test_split_faces_step_like_the_single_block(tmp_path, layout):
+def test_split_faces_step_like_the_single_block(tmp_path, layout, curved):
     """Aligned, every block computes its cells in the single block's order,
     so the values are equal bit for bit; with C turned, C sums its face and
     edge terms in the reversed order along i and j, so they agree to
-    round-off (relative 1e-12 of each field's largest value)."""
+    round-off (relative 1e-12 of each field's largest value).  On the curved
+    grid the turned block's metric, B_cell / J_cell weights and UCT edge
+    sums run with its i and j axes reversed on skewed cells, which a sign
+    or index slip in any of them breaks."""
     use(layout)
-    one = one_block_dir(tmp_path)
+    one = one_block_dir(tmp_path, curved)
     three = three_block_dir(tmp_path, one)
     h1 = HostCase(CaseDirDesc(one, WindDesc()), ghost=G)
     h3 = HostCase(CaseDirDesc(three, WindDesc()), ghost=G)

@@ -76,22 +76,24 @@ def test_tecplot_of_the_device_state_equals_host_writer(tmp_path, desc):
     assert open(dev, "rb").read() == open(host, "rb").read()
 
 
-@pytest.mark.parametrize("source", ["dir", "parts"])
-@pytest.mark.parametrize("layout", ["aligned", "turned"])
-def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout, source):
+@pytest.mark.parametrize("layout,source,curved", [
+    ("aligned", "dir", False), ("turned", "dir", False), ("aligned", "parts", False), ("turned", "parts", False),
+    ("turned", "dir", True)])
+def test_faces_shared_by_several_patches_match_the_oracle(tmp_path, layout, source, curved):
     """A CASE directory whose block faces are split among neighbours (the
     periodic box as three blocks, A's two i faces each shared by B and C, C
     aligned or turned half a turn about k) steps on the device bit for bit
     as the oracle steps it: the op lists the host builds from the patches
     and the oracle's own halo resolution agree, through the one-pass EMF.
     The case comes either from the directory or as a driver's arrays
-    (mhd_case_begin .. mhd_set_face_parts .. mhd_case_end)."""
+    (mhd_case_begin .. mhd_set_face_parts .. mhd_case_end); the curved grid
+    is the skewed, non-orthogonal variant of the box."""
     import test_face_patches as T
     from beta1_amd.fields import interior
     from oracle_bind import Oracle
 
     T.use(layout)
-    three = T.three_block_dir(tmp_path, T.one_block_dir(tmp_path))
+    three = T.three_block_dir(tmp_path, T.one_block_dir(tmp_path, curved))
     orc = Oracle(HostCase(CaseDirDesc(three, WindDesc()), ghost=2))
     T.fill(orc, [0, 1, 2])
     ctx = MhdContext()
