@@ -148,8 +148,11 @@ int mhd_adopt_host_case(mhd_ctx *ctx, mhd_host_case *h, int *case_id);
    inner0000.txt} (Grid::Read_Phy_Boundary / Read_Phy_Name /
    Read_Inner_Boundary, :224-505), interface transforms derived as
    TOPO::build_topology does (src/2_topology/2_MPCNS_Topology.cpp:77-165).
-   Whole-face patches, Farfield and Solid_Surface conditions.  The grid
-   fields of d (nang, nrad, r_in, r_out) are ignored. */
+   Farfield and Solid_Surface conditions, one per face (patches of one
+   type may tile a face); a face may be split among several interface
+   patches (mhd_get_faces kind 2, mhd_host_face_parts), not mix a condition
+   with interfaces.  The grid fields of d (nang, nrad, r_in, r_out) are
+   ignored. */
 int mhd_add_case_from_dir(mhd_ctx *ctx, const char *dir, const mhd_wind_desc *d, int *case_id);
 /* A case from the caller's own grid, topology and geometry: what a reference
    driver holds after Grid::Grid_Preprocess, TOPO::build_topology and the
